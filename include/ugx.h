@@ -1,0 +1,184 @@
+/*
+ * ugx.h -- C ABI of the MI355X (gfx950) closed-shell CCSD / CCSD(T) library.
+ *
+ * This is the boundary a Psi4 plugin (or any other host) binds to replace
+ * ugacc's CPU hot path.  Signatures use plain pointers and sizes only; no
+ * exception crosses the ABI; every call returns a status code, and the text
+ * of the last error is available from ugx_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the ugacc source tree):
+ *   ugx_default_options   <- read_options, ugacc.cc:54-73 (WFN, MAXITER,
+ *                            DIIS, R_CONVERGENCE, OOC with their defaults)
+ *   ugx_load_hamiltonian  <- Hamiltonian::Hamiltonian, hamiltonian.cc:19-103
+ *                            (fock_, ints_, L_; here six Dirac blocks that
+ *                            stay resident on the device)
+ *   ugx_ccsd              <- CCWfn::CCWfn (denominators and MP2 guess,
+ *                            ccwfn.cc:56-94) + CCWfn::compute_energy's
+ *                            iterations, ccwfn.cc:188-231
+ *   ugx_triples           <- CCWfn::tcorr_ooc_TJL, ccwfn.cc:795-852 with
+ *                            W3_ijk :1388-1418 (the same number tcorr and
+ *                            tcorr_ooc return, :647-793)
+ *   ugx_energy            <- CCWfn::energy, ccwfn.cc:260-282
+ *   ugx_residual          <- build_F/build_W/build_t1/build_t2,
+ *                            ccwfn.cc:311-539
+ *   ugx_intermediates     <- the members Fvv_, Foo_, Fov_, Woooo_, Wovov_,
+ *                            Wovvo_, Wovoo_ (ccwfn.h) after build_F/build_W
+ *   ugx_w3                <- CCWfn::W3_ijk, ccwfn.cc:1388-1418
+ *   UGX_NOT_CONVERGED     <- PSIEXCEPTION("Computation has not converged."),
+ *                            ccwfn.cc:230-231
+ *
+ * Orbital order is the reference's QT order restricted to active orbitals:
+ * occupied (no) first, then virtual (nv).  Integrals are real Dirac
+ * <pq|rs> = (pr|qs); each block is row-major in the index order its name
+ * gives:
+ *   oooo[m][n][i][j] = <mn|ij>      no^4
+ *   ooov[m][n][i][e] = <mn|ie>      no^3 nv
+ *   oovv[m][n][e][f] = <mn|ef>      no^2 nv^2
+ *   ovov[m][b][j][e] = <mb|je>      no^2 nv^2
+ *   ovvv[m][b][e][f] = <mb|ef>      no nv^3
+ *   vvvv[a][b][e][f] = <ab|ef>      nv^4
+ * The Fock matrix is nact x nact (nact = no + nv), row-major.
+ * Amplitudes: t1[i][a] (no*nv), t2[i][j][a][b] (no^2 nv^2), b fastest.
+ *
+ * Threading: one context = one HIP stream on one device; calls on a context
+ * must come from one host thread at a time.  Host buffers are read during
+ * the call only and never retained.
+ */
+#ifndef UGX_H
+#define UGX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  UGX_OK = 0,
+  UGX_NOT_CONVERGED = 1,
+  UGX_BAD_INPUT = 2,
+  UGX_OOM = 3,
+  UGX_HIP_ERROR = 4,
+  UGX_NO_DEVICE = 5,
+  UGX_NOT_READY = 6,
+  UGX_NONFINITE = 7
+};
+
+/* Arithmetic of the dense contractions. */
+enum {
+  UGX_PREC_FP64 = 0,   /* FP64 MFMA, FP64 accumulate: the parity form */
+  UGX_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulate */
+  UGX_PREC_BF16X3 = 2  /* split-BF16x3 MFMA (hi*hi + hi*lo + lo*hi), FP32 accumulate */
+};
+
+enum { UGX_GUESS_MP2 = 0, UGX_GUESS_USER = 1 };
+
+/* Energy expression: the reference's (ccwfn.cc:272 assigns instead of
+ * accumulating, so only the last (i,a) term of 2 f_ia t1_ia survives; zero
+ * for canonical RHF orbitals) or the full sum. */
+enum { UGX_ENERGY_REFERENCE = 0, UGX_ENERGY_FULL = 1 };
+
+typedef struct ugx_ctx ugx_ctx;
+
+typedef struct {
+  int wfn_ccsd_t;          /* WFN: 0 = "CCSD", 1 = "CCSD_T" (ugacc.cc:60) */
+  int maxiter;             /* MAXITER, default 100 (ugacc.cc:63) */
+  int diis;                /* DIIS, default 1 (ugacc.cc:64) */
+  double r_convergence;    /* R_CONVERGENCE, default 1e-7 (ugacc.cc:65) */
+  int ooc;                 /* OOC, default 0 (ugacc.cc:67); reported only, the
+                              (T) value is the same for both settings */
+  int diis_nvec;           /* subspace size, 8 (ccwfn.cc:205) */
+  int diis_min_vecs;       /* extrapolate once this many vectors are stored;
+                              3 = "subspace_size() > 2" (ccwfn.cc:220) */
+  int diis_policy;         /* 0: replace the oldest vector once full;
+                              1: replace the one with the largest error */
+  int precision;           /* UGX_PREC_* of the low-precision phase */
+  double polish_threshold; /* mixed forms: switch to FP64 once ||R|| drops
+                              below this (e.g. 1e-5); ignored for FP64 */
+  int guess;               /* UGX_GUESS_MP2 (ccwfn.cc:81-87) or UGX_GUESS_USER */
+  int energy_form;         /* UGX_ENERGY_REFERENCE or UGX_ENERGY_FULL */
+  int print;               /* PRINT: 0 silent, >=1 iteration table on stdout */
+} ugx_options;
+
+typedef struct {
+  double e_mp2;            /* correlation energy of the initial amplitudes (ccwfn.cc:196) */
+  double e_ccsd;           /* CCSD correlation energy as the reference reports it (ccwfn.cc:233) */
+  double e_ccsd_final;     /* energy of the final amplitudes */
+  double e_t;              /* (T) correction, if computed by ugx_ccsd (WFN=CCSD_T) */
+  double rms;              /* final ||R||_2 (ccwfn.cc:568-581) */
+  int niter;               /* iterations */
+  int niter_low;           /* iterations in the low-precision phase */
+  double seconds_ccsd;     /* wall time of the iterations */
+  double seconds_triples;  /* wall time of (T) */
+  double flops_ccsd;       /* algorithmic flops executed by the contractions */
+} ugx_result;
+
+/* Fills *opt with the reference's defaults (ugacc.cc:58-68). */
+void ugx_default_options(ugx_options *opt);
+
+int ugx_create(ugx_ctx **ctx, int device);
+void ugx_destroy(ugx_ctx *ctx);
+const char *ugx_last_error(const ugx_ctx *ctx);
+
+/* Copies the Fock matrix and the six Dirac blocks to the device, where they
+ * stay resident (hamiltonian.cc:19-103). */
+int ugx_load_hamiltonian(ugx_ctx *ctx, int no, int nv, const double *fock, const double *oooo,
+                         const double *ooov, const double *oovv, const double *ovov, const double *ovvv,
+                         const double *vvvv);
+
+/* Initial amplitudes for UGX_GUESS_USER (for example a learned guess). */
+int ugx_set_amplitudes(ugx_ctx *ctx, const double *t1, const double *t2);
+
+/* Solves the CCSD equations (ccwfn.cc:188-231), then (T) when
+ * opt->wfn_ccsd_t.  Returns UGX_OK or UGX_NOT_CONVERGED; res is filled in
+ * either case. */
+int ugx_ccsd(ugx_ctx *ctx, const ugx_options *opt, ugx_result *res);
+
+/* (T) correction from the current amplitudes (ccwfn.cc:795-852). */
+int ugx_triples(ugx_ctx *ctx, int precision, double *e_t);
+
+/* Per-triplet (T) energies in the i>=j>=k loop order of ccwfn.cc:810-812
+ * (no(no+1)(no+2)/6 values). */
+int ugx_triples_per_triplet(ugx_ctx *ctx, int precision, double *e);
+
+/* W3 of one triplet (ccwfn.cc:1388-1418), nv^3 values in [a][b][c] order. */
+int ugx_w3(ugx_ctx *ctx, int i, int j, int k, int precision, double *W3);
+
+/* Copies the current amplitudes to the host (either pointer may be NULL). */
+int ugx_get_amplitudes(ugx_ctx *ctx, double *t1, double *t2);
+
+/* Correlation energy of the current amplitudes (ccwfn.cc:260-282). */
+int ugx_energy(ugx_ctx *ctx, int energy_form, double *e);
+
+/* One residual evaluation from amplitudes t1/t2 (device copies are set from
+ * them): raw residuals R1 (build_t1, ccwfn.cc:432-469) and R2 (build_t2,
+ * :471-539).  Either output may be NULL. */
+int ugx_residual(ugx_ctx *ctx, const double *t1, const double *t2, int precision, double *R1, double *R2);
+
+/* The F and W intermediates of the last ugx_residual call in the
+ * reference's layouts: Fvv[a][e], Foo[m][i], Fov[m][e], Woooo[m][n][i][j],
+ * Wovov[m][b][j][e], Wovvo[m][b][e][j], Wovoo[m][b][i][j] (ccwfn.cc:311-430).
+ * Any pointer may be NULL. */
+int ugx_intermediates(ugx_ctx *ctx, double *Fvv, double *Foo, double *Fov, double *Woooo, double *Wovov,
+                      double *Wovvo, double *Wovoo);
+
+/* Device time per contraction label of the last ugx_ccsd / ugx_triples
+ * call when profiling is on, as "label launches ms gflop\n" lines.  Returns
+ * the number of bytes needed (including the terminating zero). */
+void ugx_set_profiling(ugx_ctx *ctx, int on);
+size_t ugx_kernel_stats(ugx_ctx *ctx, char *buf, size_t buflen);
+
+/* General GEMM on device buffers with the library's MFMA kernel:
+ * C = alpha op(A) op(B) + beta C (row-major, op 'N' or 'T'); pointers are
+ * device addresses; A/B are double for UGX_PREC_FP64 and float otherwise.
+ * tile = -1 picks the tile by size.  For kernel tests and benchmarks. */
+int ugx_gemm(ugx_ctx *ctx, char opa, char opb, int M, int N, int K, double alpha, const void *A, long lda,
+             const void *B, long ldb, double beta, double *C, long ldc, int precision, int tile);
+
+/* Synchronises the context's stream. */
+int ugx_synchronize(ugx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UGX_H */

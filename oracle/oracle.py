@@ -1,0 +1,180 @@
+"""ctypes binding of ugacc_oracle.c (test infrastructure only).
+
+Each method names the reference routine it restates; the C source carries
+the file:line citations.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def libpath(omp=False):
+    return os.path.join(_HERE, "liboracle_omp.so" if omp else "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_libs = {}
+
+
+def _lib(omp=False):
+    if omp not in _libs:
+        path = libpath(omp)
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
+        L.orc_destroy.argtypes = [vp]
+        for fn in ("orc_energy", "orc_energy_full", "orc_t1norm", "orc_tcorr", "orc_tcorr_ooc", "orc_tcorr_tjl"):
+            getattr(L, fn).restype = ctypes.c_double
+            getattr(L, fn).argtypes = [vp]
+        L.orc_tcorr_tjl_range.restype = ctypes.c_double
+        L.orc_tcorr_tjl_range.argtypes = [vp, ctypes.c_long, ctypes.c_long]
+        L.orc_ntriplets.restype = ctypes.c_long
+        L.orc_ntriplets.argtypes = [ctypes.c_int]
+        L.orc_set_amps.argtypes = [vp, dp, dp]
+        L.orc_get_amps.argtypes = [vp, dp, dp]
+        L.orc_get_tau.argtypes = [vp, dp, dp]
+        L.orc_get_L.argtypes = [vp, dp]
+        L.orc_build_tstar.argtypes = [vp]
+        L.orc_iterate.restype = ctypes.c_double
+        L.orc_iterate.argtypes = [vp, dp, dp]
+        L.orc_get_intermediates.argtypes = [vp] + [dp] * 7
+        L.orc_ccsd.restype = ctypes.c_int
+        L.orc_ccsd.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, dp, dp]
+        L.orc_tjl_per_triplet.argtypes = [vp, dp]
+        L.orc_w3.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_iteration_sample.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.orc_num_threads.restype = ctypes.c_int
+        _libs[omp] = L
+    return _libs[omp]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """State of one CCWfn (ccwfn.cc:12-120) over a Hamiltonian (fock, ints)."""
+
+    def __init__(self, fock, ints, no, omp=False):
+        self.L = _lib(omp)
+        self.no = no
+        self.nv = fock.shape[0] - no
+        self._fock = np.ascontiguousarray(fock, dtype=np.float64)
+        self._ints = np.ascontiguousarray(ints, dtype=np.float64)
+        self.h = self.L.orc_create(no, self.nv, _p(self._fock), _p(self._ints))
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def threads(self):
+        return self.L.orc_num_threads()
+
+    def energy(self):
+        return self.L.orc_energy(self.h)
+
+    def energy_full(self):
+        return self.L.orc_energy_full(self.h)
+
+    def t1norm(self):
+        return self.L.orc_t1norm(self.h)
+
+    def set_amps(self, t1, t2):
+        t1 = np.ascontiguousarray(t1, dtype=np.float64)
+        t2 = np.ascontiguousarray(t2, dtype=np.float64)
+        self.L.orc_set_amps(self.h, _p(t1), _p(t2))
+
+    def amps(self):
+        t1 = np.zeros((self.no, self.nv))
+        t2 = np.zeros((self.no, self.no, self.nv, self.nv))
+        self.L.orc_get_amps(self.h, _p(t1), _p(t2))
+        return t1, t2
+
+    def tau(self):
+        tau = np.zeros((self.no, self.no, self.nv, self.nv))
+        ttau = np.zeros_like(tau)
+        self.L.orc_get_tau(self.h, _p(tau), _p(ttau))
+        return tau, ttau
+
+    def Lints(self):
+        n = self.no + self.nv
+        L = np.zeros((n, n, n, n))
+        self.L.orc_get_L(self.h, _p(L))
+        return L
+
+    def iterate(self):
+        """One Jacobi step; returns (rms, R1, R2) with the raw residuals."""
+        R1 = np.zeros((self.no, self.nv))
+        R2 = np.zeros((self.no, self.no, self.nv, self.nv))
+        rms = self.L.orc_iterate(self.h, _p(R1), _p(R2))
+        return rms, R1, R2
+
+    def intermediates(self):
+        o, v = self.no, self.nv
+        out = dict(Fvv=np.zeros((v, v)), Foo=np.zeros((o, o)), Fov=np.zeros((o, v)),
+                   Woooo=np.zeros((o, o, o, o)), Wovov=np.zeros((o, v, o, v)),
+                   Wovvo=np.zeros((o, v, v, o)), Wovoo=np.zeros((o, v, o, o)))
+        self.L.orc_get_intermediates(self.h, *[_p(out[k]) for k in
+                                               ("Fvv", "Foo", "Fov", "Woooo", "Wovov", "Wovvo", "Wovoo")])
+        return out
+
+    def ccsd(self, maxiter=100, conv=1e-7, diis=True, mode=0, diis_policy=0, min_vecs=3):
+        """compute_energy's iteration (ccwfn.cc:188-231) for mode 0, the
+        attic driver (attic/ugacc.cc:100-114) for mode 1."""
+        out = np.zeros(4)
+        trace = np.zeros(maxiter + 1)
+        rc = self.L.orc_ccsd(self.h, maxiter, conv, int(diis), mode, diis_policy, min_vecs, _p(out), _p(trace))
+        n = int(out[3])
+        return dict(converged=(rc == 0), emp2=out[0], eccsd=out[1], rms=out[2], niter=n,
+                    trace=trace[: n + 1].copy())
+
+    def build_tstar(self):
+        self.L.orc_build_tstar(self.h)
+
+    def tcorr(self):
+        return self.L.orc_tcorr(self.h)
+
+    def tcorr_ooc(self):
+        return self.L.orc_tcorr_ooc(self.h)
+
+    def tcorr_tjl(self):
+        return self.L.orc_tcorr_tjl(self.h)
+
+    def tcorr_tjl_range(self, t0, t1):
+        return self.L.orc_tcorr_tjl_range(self.h, t0, t1)
+
+    def ntriplets(self):
+        return self.L.orc_ntriplets(self.no)
+
+    def tjl_per_triplet(self):
+        e = np.zeros(self.ntriplets())
+        self.L.orc_tjl_per_triplet(self.h, _p(e))
+        return e
+
+    def w3(self, i, j, k):
+        W = np.zeros((self.nv,) * 3)
+        self.L.orc_w3(self.h, i, j, k, _p(W))
+        return W
+
+    def iteration_sample(self, irows, part):
+        self.L.orc_iteration_sample(self.h, irows, part)

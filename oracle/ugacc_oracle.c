@@ -1,0 +1,847 @@
+/*
+ * ugacc_oracle.c -- FP64 CPU restatement of ugacc's closed-shell CCSD and (T)
+ * hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: only the test
+ * suite, the smoke check and the CPU-baseline leg of the benchmark may load
+ * it, and only as the checker (or as the timed CPU baseline).  No product
+ * code path calls it; the HIP library fails loudly instead of falling back.
+ *
+ * It keeps the reference's data layout (full nact^4 Dirac integral array,
+ * <pq|rs> = ints[p][q][r][s], L = 2<pq|rs> - <pq|sr>), its loop structure and
+ * its FP64 arithmetic, one function per reference routine, each citing the
+ * reference file:line it follows (paths relative to the ugacc source tree).
+ *
+ * Pins (see tests/test_oracle_*.py): spin-orbital textbook CCSD/(T) on tiny
+ * systems, brute-force FCI for two electrons, orbital-rotation invariance,
+ * size extensivity, agreement of the three (T) variants, closed-form MP2,
+ * and the known-answer energies in psi3/output.c1_fzc and
+ * psi4/hof/output.{c1_nofzc,cs_fzc} through the chemistry front end.
+ * The DIIS trajectory of Psi4's DIISManager is parity unpinned (third party,
+ * absent); the attic/diis.cc Pulay DIIS restated here is pinned by the
+ * iteration trace of psi3/output.c1_fzc:394-414.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stddef.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+  int no, nv, nact;
+  double *fock;            /* nact x nact */
+  double *ints;            /* nact^4, <pq|rs> */
+  double *L;               /* nact^4, 2<pq|rs> - <pq|sr> */
+  double *D1, *D2;
+  double *t1, *t1old, *t2, *t2old, *tau, *ttau;
+  double *Fvv, *Foo, *Fov;
+  double *Woooo, *Wovov, *Wovvo, *Wovoo;
+  double *t1s, *t2s;
+  /* DIIS storage (ring of up to 8 vectors of length no*nv + no^2 nv^2) */
+  int diis_nvec;
+  double *diis_err, *diis_amp, *diis_errnorm;
+  int diis_count, diis_next;
+} orc_state;
+
+#define NA (s->nact)
+#define I4(p, q, r, t) ((((size_t)(p) * NA + (q)) * NA + (r)) * NA + (t))
+#define INT(p, q, r, t) (s->ints[I4(p, q, r, t)])
+#define LL(p, q, r, t) (s->L[I4(p, q, r, t)])
+#define F(p, q) (s->fock[(size_t)(p) * NA + (q)])
+#define O2(i, a) ((size_t)(i) * nv + (a))
+#define T4(i, j, a, b) ((((size_t)(i) * no + (j)) * nv + (a)) * nv + (b))
+#define V3(a, b, c) ((((size_t)(a)) * nv + (b)) * nv + (c))
+
+static double *zalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+
+/* hamiltonian.cc:96-102 -- L(pqrs) = 2<pq|rs> - <pq|sr> */
+static void build_L(orc_state *s) {
+  int n = s->nact;
+  for (int p = 0; p < n; p++)
+    for (int q = 0; q < n; q++)
+      for (int r = 0; r < n; r++)
+        for (int t = 0; t < n; t++) LL(p, q, r, t) = 2 * INT(p, q, r, t) - INT(p, q, t, r);
+}
+
+/* ccwfn.cc:284-298 */
+void orc_build_tau(orc_state *s) {
+  int no = s->no, nv = s->nv;
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) {
+          double x = s->t1[O2(i, a)] * s->t1[O2(j, b)];
+          s->tau[T4(i, j, a, b)] = s->t2[T4(i, j, a, b)] + x;
+          s->ttau[T4(i, j, a, b)] = s->t2[T4(i, j, a, b)] + 0.5 * x;
+        }
+}
+
+/* ccwfn.cc:12-120 (denominators :62-72, guess :75-87, tau :91-94) */
+void *orc_create(int no, int nv, const double *fock, const double *ints) {
+  orc_state *s = (orc_state *)calloc(1, sizeof(orc_state));
+  s->no = no; s->nv = nv; s->nact = no + nv;
+  size_t n = s->nact, n4 = n * n * n * n, ov = (size_t)no * nv, oovv = ov * ov;
+  s->fock = zalloc(n * n); memcpy(s->fock, fock, n * n * sizeof(double));
+  s->ints = zalloc(n4); memcpy(s->ints, ints, n4 * sizeof(double));
+  s->L = zalloc(n4); build_L(s);
+  s->D1 = zalloc(ov); s->D2 = zalloc(oovv);
+  s->t1 = zalloc(ov); s->t1old = zalloc(ov);
+  s->t2 = zalloc(oovv); s->t2old = zalloc(oovv);
+  s->tau = zalloc(oovv); s->ttau = zalloc(oovv);
+  s->Fvv = zalloc((size_t)nv * nv); s->Foo = zalloc((size_t)no * no); s->Fov = zalloc(ov);
+  s->Woooo = zalloc((size_t)no * no * no * no);
+  s->Wovov = zalloc(oovv); s->Wovvo = zalloc(oovv);
+  s->Wovoo = zalloc(ov * no * no);
+  s->t1s = zalloc(ov); s->t2s = zalloc(oovv);
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) s->D1[O2(i, a)] = F(i, i) - F(a + no, a + no);
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++)
+          s->D2[T4(i, j, a, b)] = F(i, i) + F(j, j) - F(a + no, a + no) - F(b + no, b + no);
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) s->t1[O2(i, a)] = F(i, a + no) / s->D1[O2(i, a)];
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++)
+          s->t2[T4(i, j, a, b)] = INT(i, j, a + no, b + no) / s->D2[T4(i, j, a, b)];
+  orc_build_tau(s);
+  return s;
+}
+
+static void diis_free(orc_state *s) {
+  free(s->diis_err); free(s->diis_amp); free(s->diis_errnorm);
+  s->diis_err = s->diis_amp = s->diis_errnorm = NULL;
+  s->diis_count = s->diis_next = 0;
+}
+
+void orc_destroy(void *p) {
+  orc_state *s = (orc_state *)p;
+  if (!s) return;
+  free(s->fock); free(s->ints); free(s->L); free(s->D1); free(s->D2);
+  free(s->t1); free(s->t1old); free(s->t2); free(s->t2old); free(s->tau); free(s->ttau);
+  free(s->Fvv); free(s->Foo); free(s->Fov);
+  free(s->Woooo); free(s->Wovov); free(s->Wovvo); free(s->Wovoo);
+  free(s->t1s); free(s->t2s);
+  diis_free(s);
+  free(s);
+}
+
+void orc_set_amps(void *p, const double *t1, const double *t2) {
+  orc_state *s = (orc_state *)p;
+  size_t ov = (size_t)s->no * s->nv;
+  memcpy(s->t1, t1, ov * sizeof(double));
+  memcpy(s->t2, t2, ov * ov * sizeof(double));
+  orc_build_tau(s);
+}
+
+void orc_get_amps(void *p, double *t1, double *t2) {
+  orc_state *s = (orc_state *)p;
+  size_t ov = (size_t)s->no * s->nv;
+  if (t1) memcpy(t1, s->t1, ov * sizeof(double));
+  if (t2) memcpy(t2, s->t2, ov * ov * sizeof(double));
+}
+
+void orc_get_tau(void *p, double *tau, double *ttau) {
+  orc_state *s = (orc_state *)p;
+  size_t ov = (size_t)s->no * s->nv;
+  if (tau) memcpy(tau, s->tau, ov * ov * sizeof(double));
+  if (ttau) memcpy(ttau, s->ttau, ov * ov * sizeof(double));
+}
+
+void orc_get_L(void *p, double *L) {
+  orc_state *s = (orc_state *)p;
+  size_t n = s->nact;
+  memcpy(L, s->L, n * n * n * n * sizeof(double));
+}
+
+/* ccwfn.cc:260-282, reproducing the '=' at :272 (only the last (i,a) term of
+ * the one-electron part survives).  fock_ov = 0 for a canonical RHF
+ * reference, where this equals orc_energy_full. */
+double orc_energy(void *p) {
+  orc_state *s = (orc_state *)p;
+  int no = s->no, nv = s->nv;
+  double one = 0;
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) one = 2 * F(i, a + no) * s->t1[O2(i, a)];
+  double two = 0;
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) two += s->tau[T4(i, j, a, b)] * LL(i, j, a + no, b + no);
+  return one + two;
+}
+
+/* ccwfn.cc:260-282 with the one-electron sum taken over all (i,a). */
+double orc_energy_full(void *p) {
+  orc_state *s = (orc_state *)p;
+  int no = s->no, nv = s->nv;
+  double one = 0;
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) one += 2 * F(i, a + no) * s->t1[O2(i, a)];
+  double two = 0;
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) two += s->tau[T4(i, j, a, b)] * LL(i, j, a + no, b + no);
+  return one + two;
+}
+
+/* ccwfn.cc:300-309 */
+static void amp_save(orc_state *s) {
+  double *x = s->t2; s->t2 = s->t2old; s->t2old = x;
+  double *y = s->t1; s->t1 = s->t1old; s->t1old = y;
+}
+
+/* ccwfn.cc:311-356 */
+static void build_F(orc_state *s) {
+  int no = s->no, nv = s->nv;
+  const double *t1 = s->t1old, *ttau = s->ttau;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int a = 0; a < nv; a++)
+    for (int e = 0; e < nv; e++) {
+      double value = F(a + no, e + no);
+      for (int m = 0; m < no; m++) {
+        value -= 0.5 * F(m, e + no) * t1[O2(m, a)];
+        for (int f = 0; f < nv; f++) {
+          value += t1[O2(m, f)] * LL(m, a + no, f + no, e + no);
+          for (int n = 0; n < no; n++) value -= ttau[T4(m, n, a, f)] * LL(m, n, e + no, f + no);
+        }
+      }
+      s->Fvv[(size_t)a * nv + e] = value;
+    }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int m = 0; m < no; m++)
+    for (int i = 0; i < no; i++) {
+      double value = F(m, i);
+      for (int e = 0; e < nv; e++) {
+        value += 0.5 * t1[O2(i, e)] * F(m, e + no);
+        for (int n = 0; n < no; n++) {
+          value += t1[O2(n, e)] * LL(m, n, i, e + no);
+          for (int f = 0; f < nv; f++) value += ttau[T4(i, n, e, f)] * LL(m, n, e + no, f + no);
+        }
+      }
+      s->Foo[(size_t)m * no + i] = value;
+    }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int m = 0; m < no; m++)
+    for (int e = 0; e < nv; e++) {
+      double value = F(m, e + no);
+      for (int n = 0; n < no; n++)
+        for (int f = 0; f < nv; f++) value += t1[O2(n, f)] * LL(m, n, e + no, f + no);
+      s->Fov[O2(m, e)] = value;
+    }
+}
+
+/* ccwfn.cc:358-430 */
+static void build_W(orc_state *s) {
+  int no = s->no, nv = s->nv;
+  const double *tau = s->tau, *t1 = s->t1old, *t2 = s->t2old;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int m = 0; m < no; m++)
+    for (int n = 0; n < no; n++)
+      for (int i = 0; i < no; i++)
+        for (int j = 0; j < no; j++) {
+          double value = INT(m, n, i, j);
+          for (int e = 0; e < nv; e++) {
+            value += t1[O2(j, e)] * INT(m, n, i, e + no) + t1[O2(i, e)] * INT(m, n, e + no, j);
+            for (int f = 0; f < nv; f++) value += tau[T4(i, j, e, f)] * INT(m, n, e + no, f + no);
+          }
+          s->Woooo[(((size_t)m * no + n) * no + i) * no + j] = value;
+        }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int m = 0; m < no; m++)
+    for (int b = 0; b < nv; b++)
+      for (int j = 0; j < no; j++)
+        for (int e = 0; e < nv; e++) {
+          double value = -INT(m, b + no, j, e + no);
+          for (int f = 0; f < nv; f++) value -= t1[O2(j, f)] * INT(m, b + no, f + no, e + no);
+          for (int n = 0; n < no; n++) value += t1[O2(n, b)] * INT(m, n, j, e + no);
+          for (int n = 0; n < no; n++)
+            for (int f = 0; f < nv; f++)
+              value += INT(m, n, f + no, e + no) * (0.5 * t2[T4(j, n, f, b)] + t1[O2(j, f)] * t1[O2(n, b)]);
+          s->Wovov[(((size_t)m * nv + b) * no + j) * nv + e] = value;
+        }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int m = 0; m < no; m++)
+    for (int b = 0; b < nv; b++)
+      for (int e = 0; e < nv; e++)
+        for (int j = 0; j < no; j++) {
+          double value = INT(m, b + no, e + no, j);
+          for (int f = 0; f < nv; f++) value += t1[O2(j, f)] * INT(m, b + no, e + no, f + no);
+          for (int n = 0; n < no; n++) value -= t1[O2(n, b)] * INT(m, n, e + no, j);
+          for (int n = 0; n < no; n++)
+            for (int f = 0; f < nv; f++)
+              value -= INT(m, n, e + no, f + no) * (0.5 * t2[T4(j, n, f, b)] + t1[O2(j, f)] * t1[O2(n, b)]);
+          for (int n = 0; n < no; n++)
+            for (int f = 0; f < nv; f++) value += 0.5 * LL(m, n, e + no, f + no) * t2[T4(n, j, f, b)];
+          s->Wovvo[(((size_t)m * nv + b) * nv + e) * no + j] = value;
+        }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int m = 0; m < no; m++)
+    for (int b = 0; b < nv; b++)
+      for (int i = 0; i < no; i++)
+        for (int j = 0; j < no; j++) {
+          double value = 0.0;
+          for (int e = 0; e < nv; e++)
+            for (int f = 0; f < nv; f++) value += INT(m, b + no, e + no, f + no) * tau[T4(i, j, e, f)];
+          s->Wovoo[(((size_t)m * nv + b) * no + i) * no + j] = value;
+        }
+}
+
+/* ccwfn.cc:432-469 */
+static void build_t1(orc_state *s) {
+  int no = s->no, nv = s->nv;
+  double *t1new = s->t1;
+  const double *t1 = s->t1old, *t2 = s->t2old;
+  const double *Fae = s->Fvv, *Fmi = s->Foo, *Fme = s->Fov;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      double value = F(a + no, i);
+      for (int e = 0; e < nv; e++) value += t1[O2(i, e)] * Fae[(size_t)a * nv + e];
+      for (int m = 0; m < no; m++) value -= t1[O2(m, a)] * Fmi[(size_t)m * no + i];
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++) value += (2 * t2[T4(i, m, a, e)] - t2[T4(i, m, e, a)]) * Fme[O2(m, e)];
+      for (int n = 0; n < no; n++)
+        for (int f = 0; f < nv; f++) value += t1[O2(n, f)] * LL(n, a + no, f + no, i);
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++)
+          for (int f = 0; f < nv; f++)
+            value += (2 * t2[T4(m, i, e, f)] - t2[T4(m, i, f, e)]) * INT(m, a + no, e + no, f + no);
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++)
+          for (int n = 0; n < no; n++) value -= t2[T4(m, n, a, e)] * LL(n, m, e + no, i);
+      t1new[O2(i, a)] = value;
+    }
+}
+
+/* ccwfn.cc:471-539; rows i in [i0, i1) (the whole range for a normal
+ * iteration; a sub-range is used only by the bounded CPU-baseline sample). */
+static void build_t2_rows(orc_state *s, int i0, int i1) {
+  int no = s->no, nv = s->nv;
+  double *t2new = s->t2;
+  const double *t2 = s->t2old, *tau = s->tau, *t1 = s->t1old;
+  const double *Wmnij = s->Woooo, *Wmbej = s->Wovvo, *Wmbje = s->Wovov, *Zmbij = s->Wovoo;
+  const double *Fme = s->Fov, *Fmi = s->Foo, *Fae = s->Fvv;
+#define WOOOO(m, n, i, j) Wmnij[(((size_t)(m) * no + (n)) * no + (i)) * no + (j)]
+#define WOVVO(m, b, e, j) Wmbej[(((size_t)(m) * nv + (b)) * nv + (e)) * no + (j)]
+#define WOVOV(m, b, j, e) Wmbje[(((size_t)(m) * nv + (b)) * no + (j)) * nv + (e)]
+#define ZOVOO(m, b, i, j) Zmbij[(((size_t)(m) * nv + (b)) * no + (i)) * no + (j)]
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = i0; i < i1; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) {
+          double value = INT(i, j, a + no, b + no);
+          for (int e = 0; e < nv; e++)
+            value += t2[T4(i, j, a, e)] * Fae[(size_t)b * nv + e] + t2[T4(j, i, b, e)] * Fae[(size_t)a * nv + e];
+          for (int e = 0; e < nv; e++)
+            for (int m = 0; m < no; m++)
+              value -= 0.5 * (t2[T4(i, j, a, e)] * t1[O2(m, b)] * Fme[O2(m, e)] +
+                              t2[T4(i, j, e, b)] * t1[O2(m, a)] * Fme[O2(m, e)]);
+          for (int m = 0; m < no; m++)
+            value -= t2[T4(i, m, a, b)] * Fmi[(size_t)m * no + j] + t2[T4(m, j, a, b)] * Fmi[(size_t)m * no + i];
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++)
+              value -= 0.5 * (t2[T4(i, m, a, b)] * t1[O2(j, e)] * Fme[O2(m, e)] +
+                              t2[T4(m, j, a, b)] * t1[O2(i, e)] * Fme[O2(m, e)]);
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++) value += tau[T4(m, n, a, b)] * WOOOO(m, n, i, j);
+          for (int e = 0; e < nv; e++)
+            for (int f = 0; f < nv; f++) value += tau[T4(i, j, e, f)] * INT(a + no, b + no, e + no, f + no);
+          for (int e = 0; e < nv; e++)
+            value += t1[O2(i, e)] * INT(a + no, b + no, e + no, j) + t1[O2(j, e)] * INT(b + no, a + no, e + no, i);
+          for (int m = 0; m < no; m++)
+            value -= t1[O2(m, a)] * INT(m, b + no, i, j) + t1[O2(m, b)] * INT(m, a + no, j, i);
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++) {
+              value += (t2[T4(i, m, a, e)] - t2[T4(i, m, e, a)]) * WOVVO(m, b, e, j);
+              value += (t2[T4(j, m, b, e)] - t2[T4(j, m, e, b)]) * WOVVO(m, a, e, i);
+              value += t2[T4(i, m, a, e)] * (WOVVO(m, b, e, j) + WOVOV(m, b, j, e));
+              value += t2[T4(j, m, b, e)] * (WOVVO(m, a, e, i) + WOVOV(m, a, i, e));
+              value += t2[T4(m, j, a, e)] * WOVOV(m, b, i, e);
+              value += t2[T4(i, m, e, b)] * WOVOV(m, a, j, e);
+            }
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++) {
+              value -= t1[O2(i, e)] * t1[O2(m, a)] * INT(m, b + no, e + no, j);
+              value -= t1[O2(i, e)] * t1[O2(m, b)] * INT(m, a + no, j, e + no);
+              value -= t1[O2(j, e)] * t1[O2(m, a)] * INT(m, b + no, i, e + no);
+              value -= t1[O2(j, e)] * t1[O2(m, b)] * INT(m, a + no, e + no, i);
+            }
+          for (int m = 0; m < no; m++) value -= t1[O2(m, a)] * ZOVOO(m, b, i, j) + t1[O2(m, b)] * ZOVOO(m, a, j, i);
+          t2new[T4(i, j, a, b)] = value;
+        }
+#undef WOOOO
+#undef WOVVO
+#undef WOVOV
+#undef ZOVOO
+}
+
+/* ccwfn.cc:541-553 */
+double orc_t1norm(void *p) {
+  orc_state *s = (orc_state *)p;
+  int no = s->no, nv = s->nv;
+  double diag = 0;
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) diag += s->t1[O2(i, a)] * s->t1[O2(i, a)];
+  return sqrt(diag / (2 * no));
+}
+
+/* ccwfn.cc:555-582: rms = ||R||_2 (raw residual), t = t_old + R/D */
+static double increment_amps(orc_state *s) {
+  int no = s->no, nv = s->nv;
+  double r1 = 0, r2 = 0;
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      r1 += s->t1[O2(i, a)] * s->t1[O2(i, a)];
+      s->t1[O2(i, a)] = s->t1old[O2(i, a)] + s->t1[O2(i, a)] / s->D1[O2(i, a)];
+      for (int j = 0; j < no; j++)
+        for (int b = 0; b < nv; b++) {
+          size_t x = T4(i, j, a, b);
+          r2 += s->t2[x] * s->t2[x];
+          s->t2[x] = s->t2old[x] + s->t2[x] / s->D2[x];
+        }
+    }
+  return sqrt(r1 + r2);
+}
+
+/* One Jacobi step of ccwfn.cc:210-215 (amp_save, build_F, build_W,
+ * build_t1, build_t2, increment_amps).  If R1/R2 are non-NULL the raw
+ * residuals (before the denominator update) are copied out. */
+double orc_iterate(void *p, double *R1, double *R2) {
+  orc_state *s = (orc_state *)p;
+  size_t ov = (size_t)s->no * s->nv;
+  amp_save(s);
+  build_F(s);
+  build_W(s);
+  build_t1(s);
+  build_t2_rows(s, 0, s->no);
+  if (R1) memcpy(R1, s->t1, ov * sizeof(double));
+  if (R2) memcpy(R2, s->t2, ov * ov * sizeof(double));
+  return increment_amps(s);
+}
+
+void orc_get_intermediates(void *p, double *Fvv, double *Foo, double *Fov, double *Woooo, double *Wovov,
+                           double *Wovvo, double *Wovoo) {
+  orc_state *s = (orc_state *)p;
+  size_t no = s->no, nv = s->nv;
+  if (Fvv) memcpy(Fvv, s->Fvv, nv * nv * sizeof(double));
+  if (Foo) memcpy(Foo, s->Foo, no * no * sizeof(double));
+  if (Fov) memcpy(Fov, s->Fov, no * nv * sizeof(double));
+  if (Woooo) memcpy(Woooo, s->Woooo, no * no * no * no * sizeof(double));
+  if (Wovov) memcpy(Wovov, s->Wovov, no * nv * no * nv * sizeof(double));
+  if (Wovvo) memcpy(Wovvo, s->Wovvo, no * nv * nv * no * sizeof(double));
+  if (Wovoo) memcpy(Wovoo, s->Wovoo, no * nv * no * no * sizeof(double));
+}
+
+/* ---------------------------------------------------------------------
+ * Pulay DIIS, attic/diis.cc:13-161.  Error vector = t - t_old over
+ * (t1 | t2) flattened in [i][a] and [i][j][a][b] order (attic/diis.cc:40-53);
+ * B = error dot products with a -1 border, scaled by max |B_pq|
+ * (:95-119); C solves B C = (0,..,0,-1) (:122-126); the new amplitudes are
+ * the C-weighted sum of the stored amplitude vectors (:128-153).
+ * policy 0: ring replacement (slot = count mod nvec), as attic/diis.cc:37-38.
+ * policy 1: replace the vector with the largest error norm once full.
+ * ------------------------------------------------------------------- */
+static int lin_solve(double *A, double *b, int n) {
+  /* Gaussian elimination with partial pivoting, A row-major n x n. */
+  for (int c = 0; c < n; c++) {
+    int piv = c;
+    for (int r = c + 1; r < n; r++)
+      if (fabs(A[r * n + c]) > fabs(A[piv * n + c])) piv = r;
+    if (A[piv * n + c] == 0.0) return -1;
+    if (piv != c) {
+      for (int k = 0; k < n; k++) { double t = A[c * n + k]; A[c * n + k] = A[piv * n + k]; A[piv * n + k] = t; }
+      double t = b[c]; b[c] = b[piv]; b[piv] = t;
+    }
+    for (int r = c + 1; r < n; r++) {
+      double f = A[r * n + c] / A[c * n + c];
+      for (int k = c; k < n; k++) A[r * n + k] -= f * A[c * n + k];
+      b[r] -= f * b[c];
+    }
+  }
+  for (int r = n - 1; r >= 0; r--) {
+    double x = b[r];
+    for (int k = r + 1; k < n; k++) x -= A[r * n + k] * b[k];
+    b[r] = x / A[r * n + r];
+  }
+  return 0;
+}
+
+/* Adds (t - t_old) and t to the subspace; if the subspace holds at least
+ * min_vecs vectors, overwrites t1/t2 with the extrapolated amplitudes.
+ * Returns the subspace size used (0 when no extrapolation happened). */
+static int diis_step(orc_state *s, int nvec, int policy, int min_vecs) {
+  size_t ov = (size_t)s->no * s->nv, len = ov + ov * ov;
+  if (!s->diis_err) {
+    s->diis_nvec = nvec;
+    s->diis_err = zalloc(len * nvec);
+    s->diis_amp = zalloc(len * nvec);
+    s->diis_errnorm = zalloc(nvec);
+    s->diis_count = 0; s->diis_next = 0;
+  }
+  int slot;
+  if (s->diis_count < nvec) slot = s->diis_count;
+  else if (policy == 0) slot = s->diis_next;
+  else {
+    slot = 0;
+    for (int p = 1; p < nvec; p++)
+      if (s->diis_errnorm[p] > s->diis_errnorm[slot]) slot = p;
+  }
+  double *e = s->diis_err + slot * len, *x = s->diis_amp + slot * len;
+  double nrm = 0;
+  for (size_t k = 0; k < ov; k++) { e[k] = s->t1[k] - s->t1old[k]; x[k] = s->t1[k]; nrm += e[k] * e[k]; }
+  for (size_t k = 0; k < ov * ov; k++) {
+    e[ov + k] = s->t2[k] - s->t2old[k]; x[ov + k] = s->t2[k]; nrm += e[ov + k] * e[ov + k];
+  }
+  s->diis_errnorm[slot] = nrm;
+  if (s->diis_count < nvec) s->diis_count++;
+  s->diis_next = (slot + 1) % nvec;
+  int n = s->diis_count;
+  if (n < min_vecs) return 0;
+  double *B = zalloc((size_t)(n + 1) * (n + 1)), *C = zalloc(n + 1);
+  for (int p = 0; p < n; p++)
+    for (int q = 0; q <= p; q++) {
+      double d = 0;
+      const double *ep = s->diis_err + p * len, *eq = s->diis_err + q * len;
+      for (size_t k = 0; k < len; k++) d += ep[k] * eq[k];
+      B[p * (n + 1) + q] = B[q * (n + 1) + p] = d;
+    }
+  for (int p = 0; p < n; p++) { B[p * (n + 1) + n] = -1; B[n * (n + 1) + p] = -1; }
+  B[n * (n + 1) + n] = 0;
+  double mx = fabs(B[0]);
+  for (int p = 0; p < n; p++)
+    for (int q = 0; q < n; q++)
+      if (fabs(B[p * (n + 1) + q]) > mx) mx = fabs(B[p * (n + 1) + q]);
+  for (int p = 0; p < n; p++)
+    for (int q = 0; q < n; q++) B[p * (n + 1) + q] /= mx;
+  C[n] = -1;
+  if (lin_solve(B, C, n + 1) != 0) { free(B); free(C); return 0; }
+  for (size_t k = 0; k < ov; k++) {
+    double v = 0;
+    for (int p = 0; p < n; p++) v += C[p] * s->diis_amp[p * len + k];
+    s->t1[k] = v;
+  }
+  for (size_t k = 0; k < ov * ov; k++) {
+    double v = 0;
+    for (int p = 0; p < n; p++) v += C[p] * s->diis_amp[p * len + ov + k];
+    s->t2[k] = v;
+  }
+  free(B); free(C);
+  return n;
+}
+
+/* The CCSD driver.
+ * mode 0: ccwfn.cc:188-231 -- energy evaluated after DIIS and build_tau, so
+ *         the reported E_CCSD lags the final Jacobi update by one step; the
+ *         loop breaks on ||R|| < conv before DIIS; the final t1/t2 are the
+ *         un-extrapolated update; DIIS extrapolates once min_vecs vectors are
+ *         stored (Psi4's "subspace_size() > 2" is min_vecs = 3).
+ * mode 1: attic/ugacc.cc:100-114 -- energy of the fresh Jacobi update printed
+ *         before the convergence test, DIIS after it (attic DIIS starts with
+ *         two vectors, min_vecs = 2).
+ * trace[0..maxiter] receives E_MP2 followed by the per-iteration energies.
+ * Returns 0 on convergence, 1 if not converged (ccwfn.cc:230-231). */
+int orc_ccsd(void *p, int maxiter, double conv, int do_diis, int mode, int diis_policy, int min_vecs,
+             double *out /* emp2, eccsd, rms, niter */, double *trace) {
+  orc_state *s = (orc_state *)p;
+  diis_free(s);
+  double emp2 = orc_energy(s), eccsd = emp2, rms = 0;
+  if (trace) trace[0] = emp2;
+  int iter, niter = 0;
+  for (iter = 1; iter <= maxiter; iter++) {
+    /* tau/ttau already hold the amplitudes being saved (the attic's
+     * tau_build(iter, t1old, t2old) at attic/ugacc.cc:102 gives the same). */
+    amp_save(s);
+    build_F(s);
+    build_W(s);
+    build_t1(s);
+    build_t2_rows(s, 0, s->no);
+    rms = increment_amps(s);
+    niter = iter;
+    if (mode == 1) {
+      orc_build_tau(s);
+      eccsd = orc_energy(s);
+      if (trace) trace[iter] = eccsd;
+      if (rms < conv) break;
+      if (do_diis) { diis_step(s, 8, diis_policy, min_vecs); }
+      orc_build_tau(s);
+    } else {
+      if (rms < conv) break;
+      if (do_diis) diis_step(s, 8, diis_policy, min_vecs);
+      orc_build_tau(s);
+      eccsd = orc_energy(s);
+      if (trace) trace[iter] = eccsd;
+    }
+  }
+  orc_build_tau(s);
+  /* build_tstar, ccwfn.cc:628-645 */
+  {
+    int no = s->no, nv = s->nv;
+    for (int i = 0; i < no; i++)
+      for (int a = 0; a < nv; a++) {
+        s->t1s[O2(i, a)] = 2.0 * s->t1[O2(i, a)];
+        for (int j = 0; j < no; j++)
+          for (int b = 0; b < nv; b++) s->t2s[T4(i, j, a, b)] = 4.0 * s->t2[T4(i, j, a, b)] - 2.0 * s->t2[T4(i, j, b, a)];
+      }
+  }
+  if (out) { out[0] = emp2; out[1] = eccsd; out[2] = rms; out[3] = niter; }
+  return rms < conv ? 0 : 1;
+}
+
+/* build_tstar, ccwfn.cc:628-645 (for callers that set amplitudes directly) */
+void orc_build_tstar(void *p) {
+  orc_state *s = (orc_state *)p;
+  int no = s->no, nv = s->nv;
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      s->t1s[O2(i, a)] = 2.0 * s->t1[O2(i, a)];
+      for (int j = 0; j < no; j++)
+        for (int b = 0; b < nv; b++) s->t2s[T4(i, j, a, b)] = 4.0 * s->t2[T4(i, j, a, b)] - 2.0 * s->t2[T4(i, j, b, a)];
+    }
+}
+
+/* ------------------------------- (T) ------------------------------------ */
+
+/* The connected W3 sum of ccwfn.cc:1388-1418 (W3_ijk), also the numerator of
+ * t3_ijk (:1318-1352) and of tcorr (:664-689). */
+static inline double w3_elem(const orc_state *s, const double *t2, int i, int j, int k, int a, int b, int c) {
+  int no = s->no, nv = s->nv;
+  double value = 0.0;
+  for (int e = 0; e < nv; e++) {
+    value += +INT(i, e + no, a + no, b + no) * t2[T4(k, j, c, e)] + INT(i, e + no, a + no, c + no) * t2[T4(j, k, b, e)] +
+             INT(k, e + no, c + no, a + no) * t2[T4(j, i, b, e)] + INT(k, e + no, c + no, b + no) * t2[T4(i, j, a, e)] +
+             INT(j, e + no, b + no, c + no) * t2[T4(i, k, a, e)] + INT(j, e + no, b + no, a + no) * t2[T4(k, i, c, e)];
+  }
+  for (int m = 0; m < no; m++) {
+    value -= +INT(j, k, m, c + no) * t2[T4(i, m, a, b)] + INT(k, j, m, b + no) * t2[T4(i, m, a, c)] +
+             INT(i, j, m, b + no) * t2[T4(k, m, c, a)] + INT(j, i, m, a + no) * t2[T4(k, m, c, b)] +
+             INT(k, i, m, a + no) * t2[T4(j, m, b, c)] + INT(i, k, m, c + no) * t2[T4(j, m, b, a)];
+  }
+  return value;
+}
+
+/* ccwfn.cc:647-743, in-core (T) with the full o^3 v^3 t3. */
+double orc_tcorr(void *p) {
+  orc_state *s = (orc_state *)p;
+  int no = s->no, nv = s->nv;
+  const double *t1 = s->t1, *t2 = s->t2;
+  size_t vvv = (size_t)nv * nv * nv;
+  double *t3 = zalloc((size_t)no * no * no * vvv);
+#define T3(i, j, k, a, b, c) t3[((((size_t)(i) * no + (j)) * no + (k)) * vvv) + V3(a, b, c)]
+#pragma omp parallel for collapse(3) schedule(static)
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int k = 0; k < no; k++)
+        for (int a = 0; a < nv; a++)
+          for (int b = 0; b < nv; b++)
+            for (int c = 0; c < nv; c++) {
+              double denom = F(i, i) + F(j, j) + F(k, k);
+              denom -= F(a + no, a + no) + F(b + no, b + no) + F(c + no, c + no);
+              T3(i, j, k, a, b, c) = w3_elem(s, t2, i, j, k, a, b, c) / denom;
+            }
+  double *X1 = zalloc((size_t)no * nv);
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++)
+      for (int k = 0; k < no; k++)
+        for (int l = 0; l < no; l++)
+          for (int c = 0; c < nv; c++)
+            for (int d = 0; d < nv; d++)
+              X1[O2(i, a)] += (T3(i, k, l, a, c, d) - T3(l, k, i, a, c, d)) * LL(k, l, c + no, d + no);
+  double *X2 = zalloc((size_t)no * no * nv * nv);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) {
+          double x = 0;
+          for (int k = 0; k < no; k++)
+            for (int c = 0; c < nv; c++) x += (T3(i, j, k, a, b, c) - T3(k, j, i, a, b, c)) * F(k, c + no);
+          for (int k = 0; k < no; k++)
+            for (int c = 0; c < nv; c++)
+              for (int l = 0; l < no; l++) {
+                x -= T3(i, k, l, a, b, c) * LL(k, l, j, c + no);
+                x += T3(l, k, i, a, b, c) * INT(k, l, j, c + no);
+              }
+          for (int k = 0; k < no; k++)
+            for (int c = 0; c < nv; c++)
+              for (int d = 0; d < nv; d++) {
+                x += T3(i, j, k, a, c, d) * LL(b + no, k, c + no, d + no);
+                x -= T3(k, j, i, a, c, d) * INT(b + no, k, c + no, d + no);
+              }
+          X2[T4(i, j, a, b)] = x;
+        }
+  double et = 0.0;
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      et += t1[O2(i, a)] * X1[O2(i, a)];
+      for (int j = 0; j < no; j++)
+        for (int b = 0; b < nv; b++) et += (2.0 * t2[T4(i, j, a, b)] - t2[T4(i, j, b, a)]) * X2[T4(i, j, a, b)];
+    }
+  et *= 2.0;
+#undef T3
+  free(t3); free(X1); free(X2);
+  return et;
+}
+
+/* ccwfn.cc:745-793 with t3_ijk (:1318-1352): loop over all (i,j,k) with a
+ * v^3 buffer; uses t1s/t2s from build_tstar. */
+double orc_tcorr_ooc(void *p) {
+  orc_state *s = (orc_state *)p;
+  int no = s->no, nv = s->nv;
+  const double *t2 = s->t2;
+  size_t vvv = (size_t)nv * nv * nv;
+  double *t3 = zalloc(vvv);
+  double *X1 = zalloc((size_t)no * nv), *X2 = zalloc((size_t)no * no * nv * nv);
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int k = 0; k < no; k++) {
+        for (int a = 0; a < nv; a++)
+          for (int b = 0; b < nv; b++)
+            for (int c = 0; c < nv; c++) {
+              double denom = F(i, i) + F(j, j) + F(k, k);
+              denom -= F(a + no, a + no) + F(b + no, b + no) + F(c + no, c + no);
+              t3[V3(a, b, c)] = w3_elem(s, t2, i, j, k, a, b, c) / denom;
+            }
+        for (int a = 0; a < nv; a++)
+          for (int b = 0; b < nv; b++)
+            for (int c = 0; c < nv; c++) {
+              X1[O2(i, a)] += (t3[V3(a, b, c)] - t3[V3(c, b, a)]) * LL(j, k, b + no, c + no);
+              X2[T4(i, j, a, b)] += (t3[V3(a, b, c)] - t3[V3(c, b, a)]) * F(k, c + no);
+              double y = 2.0 * t3[V3(a, b, c)] - t3[V3(a, c, b)] - t3[V3(c, b, a)];
+              for (int l = 0; l < no; l++) X2[T4(i, l, a, b)] -= y * INT(j, k, l, c + no);
+              for (int d = 0; d < nv; d++) X2[T4(i, j, a, d)] += y * INT(d + no, k, b + no, c + no);
+            }
+      }
+  double et = 0.0;
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      et += s->t1s[O2(i, a)] * X1[O2(i, a)];
+      for (int j = 0; j < no; j++)
+        for (int b = 0; b < nv; b++) et += s->t2s[T4(i, j, a, b)] * X2[T4(i, j, a, b)];
+    }
+  free(t3); free(X1); free(X2);
+  return et;
+}
+
+/* Energy of one triplet (i >= j >= k) in ccwfn.cc:795-852, W3 from
+ * W3_ijk (:1388-1418).  Buffers W3, V3 are nv^3 each. */
+static double tjl_triplet(const orc_state *s, int i, int j, int k, double *W3, double *V3) {
+  int no = s->no, nv = s->nv;
+  const double *t1 = s->t1, *t2 = s->t2;
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++)
+      for (int c = 0; c < nv; c++) W3[V3(a, b, c)] = w3_elem(s, t2, i, j, k, a, b, c);
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++)
+      for (int c = 0; c < nv; c++)
+        V3[V3(a, b, c)] = (W3[V3(a, b, c)] + INT(i, j, a + no, b + no) * t1[O2(k, c)] +
+                           INT(i, k, a + no, c + no) * t1[O2(j, b)] + INT(j, k, b + no, c + no) * t1[O2(i, a)]) /
+                          (1.0 + (a == b) + (a == c) + (b == c));
+  double et = 0.0;
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b <= a; b++)
+      for (int c = 0; c <= b; c++) {
+        double X = W3[V3(a, b, c)] * V3[V3(a, b, c)] + W3[V3(a, c, b)] * V3[V3(a, c, b)] +
+                   W3[V3(b, a, c)] * V3[V3(b, a, c)] + W3[V3(b, c, a)] * V3[V3(b, c, a)] +
+                   W3[V3(c, a, b)] * V3[V3(c, a, b)] + W3[V3(c, b, a)] * V3[V3(c, b, a)];
+        double Y = V3[V3(a, b, c)] + V3[V3(b, c, a)] + V3[V3(c, a, b)];
+        double Z = V3[V3(a, c, b)] + V3[V3(b, a, c)] + V3[V3(c, b, a)];
+        double denom = F(i, i) + F(j, j) + F(k, k);
+        denom -= F(a + no, a + no) + F(b + no, b + no) + F(c + no, c + no);
+        et += ((Y - 2.0 * Z) * (W3[V3(a, b, c)] + W3[V3(b, c, a)] + W3[V3(c, a, b)]) +
+               (Z - 2.0 * Y) * (W3[V3(a, c, b)] + W3[V3(b, a, c)] + W3[V3(c, b, a)]) + 3.0 * X) *
+              (2.0 - ((i == j) + (i == k) + (j == k))) / denom;
+      }
+  return et;
+}
+
+/* Number of i >= j >= k triplets, o(o+1)(o+2)/6. */
+long orc_ntriplets(int no) { return (long)no * (no + 1) * (no + 2) / 6; }
+
+/* Triplet index -> (i,j,k) in the loop order of ccwfn.cc:810-812. */
+static void triplet_of(long t, int *pi, int *pj, int *pk) {
+  int i = 0;
+  while ((long)(i + 1) * (i + 2) * (i + 3) / 6 <= t) i++;
+  t -= (long)i * (i + 1) * (i + 2) / 6;
+  int j = 0;
+  while ((long)(j + 1) * (j + 2) / 2 <= t) j++;
+  t -= (long)j * (j + 1) / 2;
+  *pi = i; *pj = j; *pk = (int)t;
+}
+
+/* ccwfn.cc:795-852 over triplets [t0, t1) of the i>=j>=k order (the full
+ * range gives tcorr_ooc_TJL; sub-ranges serve the bounded CPU sample). */
+double orc_tcorr_tjl_range(void *p, long t0, long t1) {
+  orc_state *s = (orc_state *)p;
+  size_t vvv = (size_t)s->nv * s->nv * s->nv;
+  double et = 0.0;
+#pragma omp parallel reduction(+ : et)
+  {
+    double *W3 = zalloc(vvv), *V3b = zalloc(vvv);
+#pragma omp for schedule(dynamic, 1)
+    for (long t = t0; t < t1; t++) {
+      int i, j, k;
+      triplet_of(t, &i, &j, &k);
+      et += tjl_triplet(s, i, j, k, W3, V3b);
+    }
+    free(W3); free(V3b);
+  }
+  return et;
+}
+
+double orc_tcorr_tjl(void *p) {
+  orc_state *s = (orc_state *)p;
+  return orc_tcorr_tjl_range(p, 0, orc_ntriplets(s->no));
+}
+
+/* Per-triplet energies in i>=j>=k order (for parity of the GPU triplet
+ * kernel on small systems). */
+void orc_tjl_per_triplet(void *p, double *e_out) {
+  orc_state *s = (orc_state *)p;
+  size_t vvv = (size_t)s->nv * s->nv * s->nv;
+  long nt = orc_ntriplets(s->no);
+  double *W3 = zalloc(vvv), *V3b = zalloc(vvv);
+  for (long t = 0; t < nt; t++) {
+    int i, j, k;
+    triplet_of(t, &i, &j, &k);
+    e_out[t] = tjl_triplet(s, i, j, k, W3, V3b);
+  }
+  free(W3); free(V3b);
+}
+
+/* W3_ijk for one triplet (ccwfn.cc:1388-1418), for kernel parity. */
+void orc_w3(void *p, int i, int j, int k, double *W3) {
+  orc_state *s = (orc_state *)p;
+  int nv = s->nv;
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++)
+      for (int c = 0; c < nv; c++) W3[V3(a, b, c)] = w3_elem(s, s->t2, i, j, k, a, b, c);
+}
+
+/* Bounded CPU sample of one CCSD iteration: the full build_F / build_W /
+ * build_t1 and the t2 residual for rows i in [0, irows); returns nothing,
+ * the caller times it and scales the t2 part by no/irows. */
+void orc_iteration_sample(void *p, int irows, int part) {
+  orc_state *s = (orc_state *)p;
+  if (part == 0) { amp_save(s); build_F(s); build_W(s); build_t1(s); amp_save(s); }
+  else {
+    amp_save(s);
+    build_t2_rows(s, 0, irows < s->no ? irows : s->no);
+    amp_save(s);
+  }
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,80 @@
+"""Parity of the gfx950 CCSD / (T) path with the FP64 oracle on seeded
+synthetic Hamiltonians: intermediates, residuals, converged energies and
+amplitudes, W3 and the per-triplet (T) energies.
+
+Bars (FP64 form): energies 1e-10 Eh, amplitudes 1e-9 max-abs; per-function
+outputs 1e-11 relative to their scale."""
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from ugacc_amd import ugx
+from tests.gpu_util import loaded, announce
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    dict(no=2, nv=5, seed=1),
+    dict(no=3, nv=7, seed=2, offdiag=0.02),
+    dict(no=4, nv=9, seed=3, offdiag=0.01, fov=0.01),
+    dict(no=5, nv=19, seed=4, scale=0.05),
+]
+
+
+def _case_id(c):
+    return "o%d_v%d_s%d" % (c["no"], c["nv"], c["seed"])
+
+
+def _load(case):
+    kw = {k: v for k, v in case.items() if k not in ("no", "nv", "seed")}
+    return loaded(case["no"], case["nv"], case["seed"], **kw)
+
+
+@pytest.mark.parametrize("case", CASES, ids=_case_id)
+def test_residual_and_intermediates(case):
+    announce("residual " + _case_id(case))
+    c, fock, ints = _load(case)
+    no = case["no"]
+    orc = Oracle(fock, ints, no)
+    rng = np.random.default_rng(case["seed"] + 100)
+    t1 = 0.02 * rng.standard_normal((no, c.nv))
+    t2 = 0.02 * rng.standard_normal((no, no, c.nv, c.nv))
+    t2 = 0.5 * (t2 + t2.transpose(1, 0, 3, 2))
+    orc.set_amps(t1, t2)
+    _, R1o, R2o = orc.iterate()
+    mo = orc.intermediates()
+    R1, R2 = c.residual(t1, t2)
+    mg = c.intermediates()
+    for k in mo:
+        s = max(1.0, np.max(np.abs(mo[k])))
+        assert np.max(np.abs(mg[k] - mo[k])) < 1e-11 * s, k
+    assert np.max(np.abs(R1 - R1o)) < 1e-11 * max(1.0, np.max(np.abs(R1o)))
+    assert np.max(np.abs(R2 - R2o)) < 1e-11 * max(1.0, np.max(np.abs(R2o)))
+
+
+@pytest.mark.parametrize("case", CASES, ids=_case_id)
+def test_ccsd_and_triples_parity(case):
+    announce("ccsd+triples " + _case_id(case))
+    c, fock, ints = _load(case)
+    no = case["no"]
+    orc = Oracle(fock, ints, no)
+    ro = orc.ccsd(maxiter=100, conv=1e-10, diis=True, mode=0, diis_policy=0, min_vecs=3)
+    rg = c.ccsd(maxiter=100, r_convergence=1e-10, diis=1, diis_policy=0, diis_min_vecs=3)
+    assert ro["converged"] and rg["converged"]
+    assert abs(rg["e_mp2"] - ro["emp2"]) < 1e-12
+    assert abs(rg["e_ccsd"] - ro["eccsd"]) < 1e-10
+    t1o, t2o = orc.amps()
+    t1, t2 = c.amplitudes()
+    assert np.max(np.abs(t1 - t1o)) < 1e-9
+    assert np.max(np.abs(t2 - t2o)) < 1e-9
+    # (T) from the same (oracle) amplitudes: per-triplet and total
+    c.set_amplitudes(t1o, t2o)
+    et_o = orc.tcorr_tjl()
+    per_o = orc.tjl_per_triplet()
+    per_g = c.triples_per_triplet()
+    assert np.max(np.abs(per_g - per_o)) < 1e-13
+    assert abs(c.triples() - et_o) < 1e-12
+    assert abs(et_o - orc.tcorr_ooc()) < 1e-12
+    W_o = orc.w3(no - 1, no // 2, 0)
+    W_g = c.w3(no - 1, no // 2, 0)
+    assert np.max(np.abs(W_g - W_o)) < 1e-12 * max(1.0, np.max(np.abs(W_o)))

@@ -1,0 +1,58 @@
+"""MFMA GEMM kernel against a float64 reference of the same product, for the
+three arithmetic forms, both operand layouts and every tile shape."""
+import numpy as np
+import pytest
+
+from ugacc_amd import ugx
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev(a, dtype):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dtype).cuda()
+
+
+@pytest.mark.parametrize("prec", [ugx.PREC_FP64, ugx.PREC_FP32, ugx.PREC_BF16X3])
+@pytest.mark.parametrize("opa,opb", [("N", "N"), ("N", "T"), ("T", "N"), ("T", "T")])
+@pytest.mark.parametrize("M,N,K", [(1, 37, 50), (45, 70, 33), (130, 129, 67), (200, 40, 300)])
+def test_gemm_forms(prec, opa, opb, M, N, K):
+    from tests.gpu_util import context, announce
+    announce("gemm prec=%d %s%s M=%d N=%d K=%d" % (prec, opa, opb, M, N, K))
+    c = context()
+    rng = np.random.default_rng(M * 1000 + N + K)
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N))
+    C0 = rng.standard_normal((M, N))
+    As = A if opa == "N" else A.T.copy()
+    Bs = B if opb == "N" else B.T.copy()
+    tdt = torch.float64 if prec == ugx.PREC_FP64 else torch.float32
+    dA, dB, dC = _dev(As, tdt), _dev(Bs, tdt), _dev(C0, torch.float64)
+    torch.cuda.synchronize()
+    c.gemm(opa, opb, M, N, K, 0.7, dA.data_ptr(), As.shape[1], dB.data_ptr(), Bs.shape[1], -0.3, dC.data_ptr(), N,
+           precision=prec)
+    c.synchronize()
+    ref = 0.7 * A @ B - 0.3 * C0
+    got = dC.cpu().numpy()
+    scale = np.abs(A) @ np.abs(B)
+    tol = {ugx.PREC_FP64: 1e-13, ugx.PREC_FP32: 2e-6, ugx.PREC_BF16X3: 2e-5}[prec]
+    err = np.max(np.abs(got - ref) / (scale + 1.0))
+    assert err < tol, err
+
+
+@pytest.mark.parametrize("tile", range(9))
+def test_gemm_tiles_fp64(tile):
+    from tests.gpu_util import context, announce
+    announce("gemm tile=%d" % tile)
+    c = context()
+    rng = np.random.default_rng(tile)
+    M, N, K = 300, 170, 90
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((N, K))
+    dA, dB = _dev(A, torch.float64), _dev(B, torch.float64)
+    dC = torch.zeros((M, N), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    c.gemm("N", "T", M, N, K, 1.0, dA.data_ptr(), K, dB.data_ptr(), K, 0.0, dC.data_ptr(), N, tile=tile)
+    c.synchronize()
+    np.testing.assert_allclose(dC.cpu().numpy(), A @ B.T, rtol=0, atol=1e-12)

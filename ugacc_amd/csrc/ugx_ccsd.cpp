@@ -1,0 +1,930 @@
+// C-ABI layer and host driver of the gfx950 CCSD / CCSD(T) library.
+//
+// The host owns the iteration (ccwfn.cc:188-258): each iteration is a fixed
+// sequence of device work on the context's stream (tau, the F and W
+// intermediates and the T1/T2 residuals as MFMA contractions, the fused
+// update/denominator/norm kernel, the DIIS vectors), and only scalars come
+// back to the host: the residual norm, the energy partials and the DIIS dot
+// products.  The DIIS linear solve is done on the host in FP64.
+//
+// The T2 residual is assembled from its half X: every term of build_t2
+// (ccwfn.cc:471-539) comes in a pair related by (i,a) <-> (j,b), and
+// R2_ijab = X_ijab + X_jiba, so each pair is computed once.  Terms that are
+// symmetric on their own (<ij|ab>, tau.Wmnij, the ladder tau.<ab|ef>) enter
+// X with a factor 1/2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ugx.h"
+#include "ugx_engine.h"
+#include "ugx_internal.h"
+
+using namespace ugx;
+
+namespace {
+
+struct UgxError : std::runtime_error {
+  int code;
+  UgxError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t b) {
+    if (bytes >= b && p) return;
+    release();
+    if (b == 0) b = 8;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      throw UgxError(UGX_OOM, "device allocation of " + std::to_string(b) + " bytes failed");
+    }
+    bytes = b;
+  }
+  double *d() const { return (double *)p; }
+  float *f() const { return (float *)p; }
+};
+
+void check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw UgxError(UGX_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+struct ugx_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::string err;
+  std::unique_ptr<Engine> eng;
+  int no = 0, nv = 0;
+  bool loaded = false;
+  bool have_amps = false;
+
+  // Hamiltonian (device, constant)
+  DBuf fock, eps, fov, fvo, foo, fvv;
+  DBuf oooo, ooov, oovv, ovov, ovvv, vvvv;
+  DBuf Loovv, Looov, Lovvv;
+  // amplitudes and intermediates
+  DBuf t1, t1o, t2, t2o, tau, ttau, U, Y;
+  DBuf Fvv, Foo, Fov, FaeP, FmiP;
+  DBuf Woooo, WJ, WB, Z, Zx, P2;
+  DBuf R1, X;
+  DBuf part;
+  // DIIS
+  std::vector<std::unique_ptr<DBuf>> diis_err, diis_amp;
+  std::vector<double> diis_B;  // nvec x nvec, slot order
+  std::vector<double> diis_norm;
+  int diis_count = 0, diis_next = 0, diis_nvec = 0;
+  // (T)
+  DBuf tP, tWbuf, ttrip, tptrs, tE, tPart;
+  DBuf f_tP, f_t2, f_ooov;  // FP32 operand copies for the mixed forms
+  bool tables_ready = false;
+  std::vector<int> trip_host;
+  // residual of the last ugx_residual, for ugx_intermediates
+  bool have_intermediates = false;
+  double last_flops = 0.0;
+
+  long ov() const { return (long)no * nv; }
+  long oovv_n() const { return ov() * ov(); }
+};
+
+namespace {
+
+void require_loaded(ugx_ctx *c) {
+  if (!c->loaded) throw UgxError(UGX_NOT_READY, "no Hamiltonian loaded (call ugx_load_hamiltonian)");
+}
+
+void upload(DBuf &b, const double *h, size_t n, hipStream_t st) {
+  b.alloc(n * 8);
+  check(hipMemcpyAsync(b.p, h, n * 8, hipMemcpyHostToDevice, st), "host-to-device copy");
+}
+
+double host_sum(ugx_ctx *c, int n) {
+  std::vector<double> h(n);
+  check(hipMemcpyAsync(h.data(), c->part.p, n * 8, hipMemcpyDeviceToHost, c->st), "partials copy");
+  check(hipStreamSynchronize(c->st), "stream sync");
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s += h[i];
+  return s;
+}
+
+int npart_max(ugx_ctx *c) {
+  return std::max(update_nparts(c->no, c->nv), energy_nparts(c->no, c->nv)) * 8 + 16;
+}
+
+// -------------------------------------------------------------- energy
+// ccwfn.cc:260-282 for the amplitudes in t1 and tau.
+double energy(ugx_ctx *c, int form) {
+  const int no = c->no, nv = c->nv;
+  int np = energy_nparts(no, nv);
+  launch_energy(no, nv, c->fov.d(), c->t1.d(), c->tau.d(), c->Loovv.d(), c->part.d(), np, c->st);
+  double two = host_sum(c, np);
+  std::vector<double> t1(c->ov()), fov(c->ov());
+  check(hipMemcpyAsync(t1.data(), c->t1.p, c->ov() * 8, hipMemcpyDeviceToHost, c->st), "t1 copy");
+  check(hipMemcpyAsync(fov.data(), c->fov.p, c->ov() * 8, hipMemcpyDeviceToHost, c->st), "fov copy");
+  check(hipStreamSynchronize(c->st), "stream sync");
+  double one = 0.0;
+  if (form == UGX_ENERGY_REFERENCE) {
+    // the '=' at ccwfn.cc:272 keeps only the last (i,a) term
+    if (c->ov() > 0) one = 2 * fov[c->ov() - 1] * t1[c->ov() - 1];
+  } else {
+    for (long k = 0; k < c->ov(); k++) one += 2 * fov[k] * t1[k];
+  }
+  return one + two;
+}
+
+double t1norm(ugx_ctx *c) {
+  std::vector<double> t1(c->ov());
+  check(hipMemcpy(t1.data(), c->t1.p, c->ov() * 8, hipMemcpyDeviceToHost), "t1 copy");
+  double s = 0.0;
+  for (double x : t1) s += x * x;
+  return std::sqrt(s / (2 * c->no));
+}
+
+void build_tau(ugx_ctx *c) { launch_tau(c->no, c->nv, c->t1.d(), c->t2.d(), c->tau.d(), c->ttau.d(), c->st); }
+
+// ------------------------------------------------------------ residual
+// Residuals of the amplitudes (t1o, t2o) with tau/ttau built from them:
+// R1 -> c->R1, the T2 half X -> c->X.  prec is the arithmetic of the
+// o^2v^4 / o^3v^3 contractions; the small terms stay FP64.
+void residual(ugx_ctx *c, int prec) {
+  Engine &E = *c->eng;
+  const int o = c->no, v = c->nv;
+  const hipStream_t st = c->st;
+  const int P64 = PREC_FP64;
+  const double *t1 = c->t1o.d(), *t2 = c->t2o.d(), *tau = c->tau.d(), *ttau = c->ttau.d();
+  auto V = [](const double *p, std::vector<int> d) { return vref(p, std::move(d)); };
+  auto Cn = [](const double *p, std::vector<int> d) { return cref(p, std::move(d)); };
+  const auto T1 = V(t1, {o, v}), T2 = V(t2, {o, o, v, v}), TAU = V(tau, {o, o, v, v}), TTAU = V(ttau, {o, o, v, v});
+  const auto OOOO = Cn(c->oooo.d(), {o, o, o, o}), OOOV = Cn(c->ooov.d(), {o, o, o, v});
+  const auto OOVV = Cn(c->oovv.d(), {o, o, v, v}), OVOV = Cn(c->ovov.d(), {o, v, o, v});
+  const auto OVVV = Cn(c->ovvv.d(), {o, v, v, v}), VVVV = Cn(c->vvvv.d(), {v, v, v, v});
+  const auto LOOVV = Cn(c->Loovv.d(), {o, o, v, v}), LOOOV = Cn(c->Looov.d(), {o, o, o, v});
+  const auto LOVVV = Cn(c->Lovvv.d(), {o, v, v, v});
+  const auto FOV = Cn(c->fov.d(), {o, v});
+
+  // U = 2 t2 - t2(ab swapped), the spin-adapted combination used by build_t1
+  // (ccwfn.cc:455,461) and the Wmbej ring term (ccwfn.cc:518-519).
+  E.permute("ijab->ijab", 2.0, T2, 0.0, V(c->U.d(), {o, o, v, v}));
+  E.permute("ijba->ijab", -1.0, T2, 1.0, V(c->U.d(), {o, o, v, v}));
+  const auto UU = V(c->U.d(), {o, o, v, v});
+
+  // ---- build_F, ccwfn.cc:311-356
+  const auto FVV = V(c->Fvv.d(), {v, v}), FOO = V(c->Foo.d(), {o, o}), FOVI = V(c->Fov.d(), {o, v});
+  check(hipMemcpyAsync(c->Fvv.p, c->fvv.p, (size_t)v * v * 8, hipMemcpyDeviceToDevice, st), "copy");
+  E.contract("me,ma->ae", -0.5, FOV, T1, 1.0, FVV, P64, "F");
+  E.contract("mf,mafe->ae", 1.0, T1, LOVVV, 1.0, FVV, P64, "F");
+  E.contract("mnaf,mnef->ae", -1.0, TTAU, LOOVV, 1.0, FVV, prec, "F");
+  check(hipMemcpyAsync(c->Foo.p, c->foo.p, (size_t)o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
+  E.contract("ie,me->mi", 0.5, T1, FOV, 1.0, FOO, P64, "F");
+  E.contract("ne,mnie->mi", 1.0, T1, LOOOV, 1.0, FOO, P64, "F");
+  E.contract("inef,mnef->mi", 1.0, TTAU, LOOVV, 1.0, FOO, prec, "F");
+  check(hipMemcpyAsync(c->Fov.p, c->fov.p, (size_t)o * v * 8, hipMemcpyDeviceToDevice, st), "copy");
+  E.contract("nf,mnef->me", 1.0, T1, LOOVV, 1.0, FOVI, P64, "F");
+
+  // ---- build_W, ccwfn.cc:358-430
+  const auto WO = V(c->Woooo.d(), {o, o, o, o});
+  check(hipMemcpyAsync(c->Woooo.p, c->oooo.p, (size_t)o * o * o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
+  E.contract("je,mnie->mnij", 1.0, T1, OOOV, 1.0, WO, P64, "Woooo");
+  E.contract("ie,nmje->mnij", 1.0, T1, OOOV, 1.0, WO, P64, "Woooo");  // <mn|ej> = <nm|je>
+  E.contract("ijef,mnef->mnij", 1.0, TAU, OOVV, 1.0, WO, prec, "Woooo");
+
+  // Y[n][f][j][b] = 0.5 t2[j][n][f][b] + t1[j][f] t1[n][b]
+  launch_build_Y(o, v, t1, t2, c->Y.d(), st);
+  const auto YY = V(c->Y.d(), {o, v, o, v});
+  // Wmbje stored as WJ[m][e][j][b]
+  const auto WJ = V(c->WJ.d(), {o, v, o, v});
+  E.permute("mbje->mejb", -1.0, OVOV, 0.0, WJ);
+  E.contract("jf,mbfe->mejb", -1.0, T1, OVVV, 1.0, WJ, P64, "Wmbje");
+  E.contract("nb,mnje->mejb", 1.0, T1, OOOV, 1.0, WJ, P64, "Wmbje");
+  E.contract("mnfe,nfjb->mejb", 1.0, OOVV, YY, 1.0, WJ, prec, "Wmbje");
+  // Wmbej stored as WB[m][e][j][b];  <mb|ej> = <mj|eb>
+  const auto WB = V(c->WB.d(), {o, v, o, v});
+  E.permute("mjeb->mejb", 1.0, OOVV, 0.0, WB);
+  E.contract("jf,mbef->mejb", 1.0, T1, OVVV, 1.0, WB, P64, "Wmbej");
+  E.contract("nb,nmje->mejb", -1.0, T1, OOOV, 1.0, WB, P64, "Wmbej");
+  E.contract("mnef,nfjb->mejb", -1.0, OOVV, YY, 1.0, WB, prec, "Wmbej");
+  E.contract("mnef,njfb->mejb", 0.5, LOOVV, T2, 1.0, WB, prec, "Wmbej");
+  // Zmbij = sum_ef <mb|ef> tau_ijef
+  const auto ZZ = V(c->Z.d(), {o, v, o, o});
+  E.contract("mbef,ijef->mbij", 1.0, OVVV, TAU, 0.0, ZZ, prec, "Zmbij");
+
+  // ---- build_t1, ccwfn.cc:432-469
+  const auto RR1 = V(c->R1.d(), {o, v});
+  E.permute("ai->ia", 1.0, Cn(c->fvo.d(), {v, o}), 0.0, RR1);
+  E.contract("ie,ae->ia", 1.0, T1, FVV, 1.0, RR1, P64, "t1");
+  E.contract("ma,mi->ia", -1.0, T1, FOO, 1.0, RR1, P64, "t1");
+  E.contract("imae,me->ia", 1.0, UU, FOVI, 1.0, RR1, P64, "t1");
+  E.contract("nf,nifa->ia", 2.0, T1, OOVV, 1.0, RR1, P64, "t1");   // 2<na|fi> = 2<ni|fa>
+  E.contract("nf,naif->ia", -1.0, T1, OVOV, 1.0, RR1, P64, "t1");  // -<na|if>
+  E.contract("mief,maef->ia", 1.0, UU, OVVV, 1.0, RR1, prec, "t1");
+  E.contract("mnae,mnie->ia", -2.0, T2, OOOV, 1.0, RR1, P64, "t1");  // L<nm|ei> = 2<mn|ie> - <nm|ie>
+  E.contract("mnae,nmie->ia", 1.0, T2, OOOV, 1.0, RR1, P64, "t1");
+
+  // ---- build_t2 (half X), ccwfn.cc:471-539
+  const auto XX = V(c->X.d(), {o, o, v, v});
+  E.permute("ijab->ijab", 0.5, OOVV, 0.0, XX);
+  // Fae - 0.5 t1.Fme and Fmi + 0.5 t1.Fme fold the t1*Fme terms (:489-491, :496-498)
+  const auto FAEP = V(c->FaeP.d(), {v, v}), FMIP = V(c->FmiP.d(), {o, o});
+  check(hipMemcpyAsync(c->FaeP.p, c->Fvv.p, (size_t)v * v * 8, hipMemcpyDeviceToDevice, st), "copy");
+  E.contract("mb,me->be", -0.5, T1, FOVI, 1.0, FAEP, P64, "t2");
+  check(hipMemcpyAsync(c->FmiP.p, c->Foo.p, (size_t)o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
+  E.contract("je,me->mj", 0.5, T1, FOVI, 1.0, FMIP, P64, "t2");
+  E.contract("ijae,be->ijab", 1.0, T2, FAEP, 1.0, XX, prec, "t2_F");
+  E.contract("imab,mj->ijab", -1.0, T2, FMIP, 1.0, XX, prec, "t2_F");
+  E.contract("mnab,mnij->ijab", 0.5, TAU, WO, 1.0, XX, prec, "t2_Wmnij");
+  E.contract("ijef,abef->ijab", 0.5, TAU, VVVV, 1.0, XX, prec, "ladder");
+  E.contract("ie,jeba->ijab", 1.0, T1, OVVV, 1.0, XX, P64, "t2_t1");  // <ab|ej> = <je|ba>
+  // Z' = Zmbij + sum_e t1_ie <mb|ej> + <mb|ij>  (the t1 t1 <mb|ej> and
+  // t1 <mb|ij> terms of :512-515, :528-535 share Zmbij's t1_ma contraction)
+  const auto ZX = V(c->Zx.d(), {o, v, o, o});
+  E.permute("mbij->mbij", 1.0, ZZ, 0.0, ZX);
+  E.contract("ie,mjeb->mbij", 1.0, T1, OOVV, 1.0, ZX, P64, "t2_t1");
+  E.permute("ijmb->mbij", 1.0, OOOV, 1.0, ZX);
+  E.contract("ma,mbij->ijab", -1.0, T1, ZX, 1.0, XX, P64, "t2_t1");
+  // P2[m][a][j][i] = sum_e t1_ie <ma|je>
+  const auto PP2 = V(c->P2.d(), {o, v, o, o});
+  E.contract("ie,maje->maji", 1.0, T1, OVOV, 0.0, PP2, P64, "t2_t1");
+  E.contract("mb,maji->ijab", -1.0, T1, PP2, 1.0, XX, P64, "t2_t1");
+  // ring terms, ccwfn.cc:516-526
+  E.contract("imae,mejb->ijab", 1.0, UU, WB, 1.0, XX, prec, "ring");
+  E.contract("imae,mejb->ijab", 1.0, T2, WJ, 1.0, XX, prec, "ring");
+  E.contract("mjae,meib->ijab", 1.0, T2, WJ, 1.0, XX, prec, "ring");
+  c->have_intermediates = true;
+}
+
+// t = t_old + R/D, returns ||R||_2 (ccwfn.cc:555-582)
+double increment(ugx_ctx *c, double *R2out) {
+  int np = update_nparts(c->no, c->nv);
+  launch_update(c->no, c->nv, c->eps.d(), c->R1.d(), c->X.d(), c->t1o.d(), c->t2o.d(), c->t1.d(), c->t2.d(), R2out,
+                c->part.d(), np, c->st);
+  return std::sqrt(host_sum(c, np));
+}
+
+void amp_save(ugx_ctx *c) {
+  std::swap(c->t1.p, c->t1o.p);
+  std::swap(c->t1.bytes, c->t1o.bytes);
+  std::swap(c->t2.p, c->t2o.p);
+  std::swap(c->t2.bytes, c->t2o.bytes);
+}
+
+// ---------------------------------------------------------------- DIIS
+// Pulay DIIS: error = t - t_old over (t1 | t2), B_pq = e_p . e_q with a -1
+// border, scaled by max |B_pq|, B c = (0,..,0,-1); t = sum_p c_p t_p.
+// Subspace handling follows the oracle (attic/diis.cc:13-161 semantics).
+void diis_reset(ugx_ctx *c) {
+  c->diis_count = 0;
+  c->diis_next = 0;
+}
+
+int lin_solve(std::vector<double> &A, std::vector<double> &b, int n) {
+  for (int col = 0; col < n; col++) {
+    int piv = col;
+    for (int r = col + 1; r < n; r++)
+      if (std::fabs(A[r * n + col]) > std::fabs(A[piv * n + col])) piv = r;
+    if (A[piv * n + col] == 0.0) return -1;
+    if (piv != col) {
+      for (int k = 0; k < n; k++) std::swap(A[col * n + k], A[piv * n + k]);
+      std::swap(b[col], b[piv]);
+    }
+    for (int r = col + 1; r < n; r++) {
+      double f = A[r * n + col] / A[col * n + col];
+      for (int k = col; k < n; k++) A[r * n + k] -= f * A[col * n + k];
+      b[r] -= f * b[col];
+    }
+  }
+  for (int r = n - 1; r >= 0; r--) {
+    double x = b[r];
+    for (int k = r + 1; k < n; k++) x -= A[r * n + k] * b[k];
+    b[r] = x / A[r * n + r];
+  }
+  return 0;
+}
+
+int diis_step(ugx_ctx *c, int nvec, int policy, int min_vecs) {
+  const long n1 = c->ov(), n2 = c->oovv_n(), len = n1 + n2;
+  if (c->diis_nvec != nvec || (int)c->diis_err.size() != nvec) {
+    c->diis_err.clear();
+    c->diis_amp.clear();
+    for (int p = 0; p < nvec; p++) {
+      c->diis_err.emplace_back(new DBuf());
+      c->diis_amp.emplace_back(new DBuf());
+    }
+    c->diis_nvec = nvec;
+    c->diis_B.assign((size_t)nvec * nvec, 0.0);
+    c->diis_norm.assign(nvec, 0.0);
+    diis_reset(c);
+  }
+  int slot;
+  if (c->diis_count < nvec) slot = c->diis_count;
+  else if (policy == 0) slot = c->diis_next;
+  else {
+    slot = 0;
+    for (int p = 1; p < nvec; p++)
+      if (c->diis_norm[p] > c->diis_norm[slot]) slot = p;
+  }
+  DBuf &e = *c->diis_err[slot];
+  DBuf &x = *c->diis_amp[slot];
+  e.alloc(len * 8);
+  x.alloc(len * 8);
+  launch_diis_store(n1, n2, c->t1.d(), c->t1o.d(), c->t2.d(), c->t2o.d(), e.d(), x.d(), c->st);
+  if (c->diis_count < nvec) c->diis_count++;
+  c->diis_next = (slot + 1) % nvec;
+  const int n = c->diis_count;
+  // dots of the new error with every stored error (slot order)
+  std::vector<const double *> vecs(n);
+  for (int p = 0; p < n; p++) vecs[p] = c->diis_err[p]->d();
+  int np = 512;
+  launch_diis_dots(len, e.d(), vecs.data(), n, c->part.d(), np, c->st);
+  std::vector<double> h((size_t)n * np);
+  check(hipMemcpyAsync(h.data(), c->part.p, h.size() * 8, hipMemcpyDeviceToHost, c->st), "diis dots copy");
+  check(hipStreamSynchronize(c->st), "stream sync");
+  for (int p = 0; p < n; p++) {
+    double s = 0.0;
+    for (int b = 0; b < np; b++) s += h[(size_t)p * np + b];
+    c->diis_B[(size_t)slot * nvec + p] = c->diis_B[(size_t)p * nvec + slot] = s;
+  }
+  c->diis_norm[slot] = c->diis_B[(size_t)slot * nvec + slot];
+  if (n < min_vecs) return 0;
+  const int m = n + 1;
+  std::vector<double> B((size_t)m * m, 0.0), C(m, 0.0);
+  for (int p = 0; p < n; p++)
+    for (int q = 0; q < n; q++) B[p * m + q] = c->diis_B[(size_t)p * nvec + q];
+  for (int p = 0; p < n; p++) { B[p * m + n] = -1; B[n * m + p] = -1; }
+  double mx = std::fabs(B[0]);
+  for (int p = 0; p < n; p++)
+    for (int q = 0; q < n; q++) mx = std::max(mx, std::fabs(B[p * m + q]));
+  if (mx > 0)
+    for (int p = 0; p < n; p++)
+      for (int q = 0; q < n; q++) B[p * m + q] /= mx;
+  C[n] = -1;
+  if (lin_solve(B, C, m) != 0) return 0;
+  std::vector<const double *> amps(n);
+  for (int p = 0; p < n; p++) amps[p] = c->diis_amp[p]->d();
+  launch_diis_combine(n1, n2, amps.data(), C.data(), n, c->t1.d(), c->t2.d(), c->st);
+  return n;
+}
+
+// -------------------------------------------------------------- setup
+void alloc_work(ugx_ctx *c) {
+  const long o = c->no, v = c->nv, ov = o * v, oovv = ov * ov;
+  for (DBuf *b : {&c->t1, &c->t1o, &c->R1}) b->alloc(ov * 8);
+  for (DBuf *b : {&c->t2, &c->t2o, &c->tau, &c->ttau, &c->U, &c->Y, &c->WJ, &c->WB, &c->X}) b->alloc(oovv * 8);
+  c->Fvv.alloc(v * v * 8); c->FaeP.alloc(v * v * 8);
+  c->Foo.alloc(o * o * 8); c->FmiP.alloc(o * o * 8);
+  c->Fov.alloc(ov * 8);
+  c->Woooo.alloc(o * o * o * o * 8);
+  c->Z.alloc(o * v * o * o * 8); c->Zx.alloc(o * v * o * o * 8); c->P2.alloc(o * v * o * o * 8);
+  c->part.alloc((size_t)std::max(npart_max(c), 512 * 8) * 8);
+}
+
+void guess_mp2(ugx_ctx *c) {
+  launch_guess(c->no, c->nv, c->fock.d(), c->oovv.d(), c->t1.d(), c->t2.d(), c->st);
+}
+
+// ------------------------------------------------------------------ (T)
+// W3 of a batch of triplets: six grouped MFMA contractions, one per
+// permutation of the (ia, jb, kc) pairs (ccwfn.cc:1396-1413):
+//   X(p,q,r)[x][y][z] = sum_e <pe|xy> t2_rq^ze - sum_m t2_pm^xy <qr|mz>
+//   W = X(ijk)[abc] + X(ikj)[acb] + X(kij)[cab] + X(kji)[cba] + X(jki)[bca]
+//       + X(jik)[bac]
+// as GEMMs with rows (x,y) (v^2), columns z (v) and K = v (particle) + o
+// (hole), written to W[a][b][c] through the output row split.
+struct Perm3 {
+  int p, q, r;  // which of (i,j,k) plays p, q, r: 0 = i, 1 = j, 2 = k
+  int x, y, z;  // which of (a,b,c) plays x, y, z: 0 = a, 1 = b, 2 = c
+};
+const Perm3 kPerms[6] = {{0, 1, 2, 0, 1, 2}, {0, 2, 1, 0, 2, 1}, {2, 0, 1, 2, 0, 1},
+                         {2, 1, 0, 2, 1, 0}, {1, 2, 0, 1, 2, 0}, {1, 0, 2, 1, 0, 2}};
+
+void triples_prepare(ugx_ctx *c, int prec) {
+  const int o = c->no, v = c->nv;
+  const long v3 = (long)v * v * v;
+  if (!c->tables_ready) {
+    // P[p][x][y][e] = <pe|xy> = ovvv[p][e][x][y]
+    c->tP.alloc((size_t)o * v3 * 8);
+    c->eng->permute("pexy->pxye", 1.0, cref(c->ovvv.d(), {o, v, v, v}), 0.0, vref(c->tP.d(), {o, v, v, v}));
+    c->trip_host.clear();
+    for (int i = 0; i < o; i++)
+      for (int j = 0; j <= i; j++)
+        for (int k = 0; k <= j; k++) {
+          if (i == j && j == k) continue;  // the bracket vanishes for i = j = k
+          c->trip_host.push_back(i);
+          c->trip_host.push_back(j);
+          c->trip_host.push_back(k);
+        }
+    const long nt = (long)c->trip_host.size() / 3;
+    c->ttrip.alloc(std::max<long>(nt, 1) * 3 * 4);
+    if (nt) check(hipMemcpy(c->ttrip.p, c->trip_host.data(), nt * 3 * 4, hipMemcpyHostToDevice), "triplet table");
+    c->tables_ready = true;
+  }
+  if (prec != PREC_FP64 && !c->f_tP.p) {
+    auto conv = [&](DBuf &dst, const double *src, long n) {
+      dst.alloc(n * 4);
+      PermDesc pd{};
+      pd.rank = 1; pd.od[0] = (int)n; pd.istr[0] = 1; pd.in = src; pd.out = dst.p; pd.in_t = 0; pd.out_t = 1;
+      pd.alpha = 1.0; pd.beta = 0.0;
+      launch_permute(pd, c->st);
+    };
+    conv(c->f_tP, c->tP.d(), (long)o * v3);
+    conv(c->f_ooov, c->ooov.d(), (long)o * o * o * v);
+  }
+}
+
+// Builds W for triplets [t0, t0 + nb) of the table into c->tWbuf.
+void triples_build_w(ugx_ctx *c, int prec, long t0, int nb, const std::vector<int> &trip) {
+  const int o = c->no, v = c->nv;
+  const long v2 = (long)v * v, v3 = v2 * v;
+  const bool f32 = (prec != PREC_FP64);
+  const size_t es = f32 ? 4 : 8;
+  const char *P = (const char *)(f32 ? c->f_tP.p : c->tP.p);
+  const char *T2 = (const char *)(f32 ? c->f_t2.p : c->t2.p);
+  const char *OV = (const char *)(f32 ? c->f_ooov.p : c->ooov.p);
+  std::vector<GemmPtrs> tab((size_t)6 * nb);
+  for (int s = 0; s < 6; s++)
+    for (int b = 0; b < nb; b++) {
+      const int *ijk = &trip[3 * (t0 + b)];
+      const Perm3 &pm = kPerms[s];
+      int p = ijk[pm.p], q = ijk[pm.q], r = ijk[pm.r];
+      GemmPtrs &g = tab[(size_t)s * nb + b];
+      g.A[0] = P + (size_t)p * v3 * es;                             // P[p][x][y][e]
+      g.B[0] = T2 + ((size_t)r * o + q) * v2 * es;                  // t2[r][q][z][e]
+      g.A[1] = T2 + (size_t)p * o * v2 * es;                        // t2[p][m][x][y]
+      g.B[1] = OV + ((size_t)q * o + r) * o * v * es;               // <qr|mz>
+      g.C = c->tWbuf.d() + (size_t)b * v3;
+    }
+  c->tptrs.alloc(tab.size() * sizeof(GemmPtrs));
+  check(hipMemcpyAsync(c->tptrs.p, tab.data(), tab.size() * sizeof(GemmPtrs), hipMemcpyHostToDevice, c->st),
+        "pointer table copy");
+  const long stride[3] = {v2, v, 1};  // of a, b, c in W[a][b][c]
+  for (int s = 0; s < 6; s++) {
+    const Perm3 &pm = kPerms[s];
+    GemmDesc d{};
+    d.M = (int)v2; d.N = v; d.nseg = 2; d.batch = nb;
+    d.seg[0].sam = v; d.seg[0].sak = 1; d.seg[0].sbk = 1; d.seg[0].sbn = v; d.seg[0].K = v; d.seg[0].sign = 1.0;
+    d.seg[1].sam = 1; d.seg[1].sak = v2; d.seg[1].sbk = v; d.seg[1].sbn = 1; d.seg[1].K = o; d.seg[1].sign = -1.0;
+    d.cdiv = v; d.scm1 = stride[pm.x]; d.scm = stride[pm.y]; d.scn = stride[pm.z];
+    d.alpha = 1.0; d.beta = (s == 0) ? 0.0 : 1.0;
+    int tile = v > 128 ? 5 : (v > 64 ? 8 : (v > 32 ? 1 : 2));
+    launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)s * nb, prec, tile, c->st);
+  }
+}
+
+// (T) over all triplets; per-triplet energies (table order, i = j = k
+// entries reported as 0) into etrip_out when non-null.
+double triples(ugx_ctx *c, int prec, std::vector<double> *etrip_out) {
+  const int o = c->no, v = c->nv;
+  const long v3 = (long)v * v * v;
+  triples_prepare(c, prec);
+  if (prec != PREC_FP64) {
+    const long n = c->oovv_n();
+    c->f_t2.alloc(n * 4);
+    PermDesc pd{};
+    pd.rank = 1; pd.od[0] = (int)n; pd.istr[0] = 1; pd.in = c->t2.p; pd.out = c->f_t2.p; pd.in_t = 0; pd.out_t = 1;
+    pd.alpha = 1.0; pd.beta = 0.0;
+    launch_permute(pd, c->st);
+  }
+  const long nt = (long)c->trip_host.size() / 3;
+  // batch: W of up to ~2 GB
+  long nb = std::max<long>(1, std::min<long>(nt, (2L << 30) / (v3 * 8)));
+  nb = std::min<long>(nb, 4096);
+  c->tWbuf.alloc((size_t)nb * v3 * 8);
+  c->tE.alloc(std::max<long>(nt, 1) * 8);
+  c->tPart.alloc((size_t)nb * v * 8);
+  for (long t0 = 0; t0 < nt; t0 += nb) {
+    int b = (int)std::min<long>(nb, nt - t0);
+    triples_build_w(c, prec, t0, b, c->trip_host);
+    launch_triples_energy(o, v, b, (const int *)c->ttrip.p + 3 * t0, c->tWbuf.d(), v3, c->eps.d(), c->oovv.d(),
+                          c->t1.d(), c->tE.d() + t0, c->tPart.d(), c->st);
+  }
+  std::vector<double> e(nt);
+  if (nt) check(hipMemcpyAsync(e.data(), c->tE.p, nt * 8, hipMemcpyDeviceToHost, c->st), "triples copy");
+  check(hipStreamSynchronize(c->st), "stream sync");
+  double et = 0.0;
+  for (double x : e) et += x;
+  if (etrip_out) {
+    etrip_out->clear();
+    long k = 0;
+    for (int i = 0; i < o; i++)
+      for (int j = 0; j <= i; j++)
+        for (int kk = 0; kk <= j; kk++) {
+          if (i == j && j == kk) etrip_out->push_back(0.0);
+          else etrip_out->push_back(e[k++]);
+        }
+  }
+  return et;
+}
+
+double triples_flops(int o, int v) {
+  double ntrip = (double)o * (o + 1) * (o + 2) / 6.0 - o;
+  return 12.0 * ntrip * ((double)v * v * v * v + (double)o * v * v * v);
+}
+
+template <typename F>
+int guarded(ugx_ctx *c, F &&f) {
+  if (!c) return UGX_BAD_INPUT;
+  try {
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) throw UgxError(UGX_HIP_ERROR, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return f();
+  } catch (const UgxError &e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    c->err = "host allocation failed";
+    return UGX_OOM;
+  } catch (const std::exception &e) {
+    c->err = e.what();
+    return UGX_HIP_ERROR;
+  }
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+void ugx_default_options(ugx_options *opt) {
+  if (!opt) return;
+  std::memset(opt, 0, sizeof(*opt));
+  opt->wfn_ccsd_t = 0;
+  opt->maxiter = 100;
+  opt->diis = 1;
+  opt->r_convergence = 1e-7;
+  opt->ooc = 0;
+  opt->diis_nvec = 8;
+  opt->diis_min_vecs = 3;
+  opt->diis_policy = 0;
+  opt->precision = UGX_PREC_FP64;
+  opt->polish_threshold = 1e-5;
+  opt->guess = UGX_GUESS_MP2;
+  opt->energy_form = UGX_ENERGY_REFERENCE;
+  opt->print = 0;
+}
+
+int ugx_create(ugx_ctx **out, int device) {
+  if (!out) return UGX_BAD_INPUT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return UGX_NO_DEVICE;
+  }
+  if (device < 0 || device >= n) return UGX_NO_DEVICE;
+  ugx_ctx *c = new (std::nothrow) ugx_ctx();
+  if (!c) return UGX_OOM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return UGX_HIP_ERROR;
+  }
+  c->eng.reset(new Engine(c->st));
+  *out = c;
+  return UGX_OK;
+}
+
+void ugx_destroy(ugx_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  c->eng.reset();
+  hipStream_t st = c->st;
+  delete c;
+  if (st) (void)hipStreamDestroy(st);
+}
+
+const char *ugx_last_error(const ugx_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const double *oooo, const double *ooov,
+                         const double *oovv, const double *ovov, const double *ovvv, const double *vvvv) {
+  return guarded(c, [&]() {
+    if (no <= 0 || nv <= 0 || !fock || !oooo || !ooov || !oovv || !ovov || !ovvv || !vvvv)
+      throw UgxError(UGX_BAD_INPUT, "ugx_load_hamiltonian: sizes must be positive and all blocks given");
+    c->eng->drop_cache();
+    c->loaded = false;
+    c->tables_ready = false;
+    c->have_amps = false;
+    c->have_intermediates = false;
+    c->f_tP.release(); c->f_ooov.release(); c->f_t2.release();
+    c->diis_err.clear(); c->diis_amp.clear(); c->diis_nvec = 0;
+    c->no = no; c->nv = nv;
+    const long n = no + nv, o = no, v = nv;
+    for (long k = 0; k < n * n; k++)
+      if (!std::isfinite(fock[k])) throw UgxError(UGX_NONFINITE, "non-finite Fock matrix element");
+    std::vector<double> h(n * n);
+    upload(c->fock, fock, n * n, c->st);
+    std::vector<double> eps(n), fov(o * v), fvo(v * o), foo(o * o), fvv(v * v);
+    for (long p = 0; p < n; p++) eps[p] = fock[p * n + p];
+    for (long i = 0; i < o; i++)
+      for (long a = 0; a < v; a++) { fov[i * v + a] = fock[i * n + o + a]; fvo[a * o + i] = fock[(o + a) * n + i]; }
+    for (long i = 0; i < o; i++)
+      for (long j = 0; j < o; j++) foo[i * o + j] = fock[i * n + j];
+    for (long a = 0; a < v; a++)
+      for (long b = 0; b < v; b++) fvv[a * v + b] = fock[(o + a) * n + o + b];
+    upload(c->eps, eps.data(), n, c->st);
+    upload(c->fov, fov.data(), o * v, c->st);
+    upload(c->fvo, fvo.data(), o * v, c->st);
+    upload(c->foo, foo.data(), o * o, c->st);
+    upload(c->fvv, fvv.data(), v * v, c->st);
+    upload(c->oooo, oooo, o * o * o * o, c->st);
+    upload(c->ooov, ooov, o * o * o * v, c->st);
+    upload(c->oovv, oovv, o * o * v * v, c->st);
+    upload(c->ovov, ovov, o * v * o * v, c->st);
+    upload(c->ovvv, ovvv, o * v * v * v, c->st);
+    upload(c->vvvv, vvvv, v * v * v * v, c->st);
+    Engine &E = *c->eng;
+    // L = 2<pq|rs> - <pq|sr> for the blocks the equations read (hamiltonian.cc:96-102)
+    c->Loovv.alloc(o * o * v * v * 8);
+    E.permute("mnef->mnef", 2.0, cref(c->oovv.d(), {no, no, nv, nv}), 0.0, vref(c->Loovv.d(), {no, no, nv, nv}));
+    E.permute("mnfe->mnef", -1.0, cref(c->oovv.d(), {no, no, nv, nv}), 1.0, vref(c->Loovv.d(), {no, no, nv, nv}));
+    // L<mn|ie> = 2<mn|ie> - <mn|ei> = 2 ooov[m][n][i][e] - ooov[n][m][i][e]
+    c->Looov.alloc(o * o * o * v * 8);
+    E.permute("mnie->mnie", 2.0, cref(c->ooov.d(), {no, no, no, nv}), 0.0, vref(c->Looov.d(), {no, no, no, nv}));
+    E.permute("nmie->mnie", -1.0, cref(c->ooov.d(), {no, no, no, nv}), 1.0, vref(c->Looov.d(), {no, no, no, nv}));
+    // L<ma|fe> = 2<ma|fe> - <ma|ef>
+    c->Lovvv.alloc(o * v * v * v * 8);
+    E.permute("mafe->mafe", 2.0, cref(c->ovvv.d(), {no, nv, nv, nv}), 0.0, vref(c->Lovvv.d(), {no, nv, nv, nv}));
+    E.permute("maef->mafe", -1.0, cref(c->ovvv.d(), {no, nv, nv, nv}), 1.0, vref(c->Lovvv.d(), {no, nv, nv, nv}));
+    alloc_work(c);
+    check(hipStreamSynchronize(c->st), "stream sync");
+    c->loaded = true;
+    return UGX_OK;
+  });
+}
+
+int ugx_set_amplitudes(ugx_ctx *c, const double *t1, const double *t2) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!t1 || !t2) throw UgxError(UGX_BAD_INPUT, "ugx_set_amplitudes: null amplitudes");
+    check(hipMemcpyAsync(c->t1.p, t1, c->ov() * 8, hipMemcpyHostToDevice, c->st), "t1 copy");
+    check(hipMemcpyAsync(c->t2.p, t2, c->oovv_n() * 8, hipMemcpyHostToDevice, c->st), "t2 copy");
+    check(hipStreamSynchronize(c->st), "stream sync");
+    c->have_amps = true;
+    return UGX_OK;
+  });
+}
+
+int ugx_get_amplitudes(ugx_ctx *c, double *t1, double *t2) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (t1) check(hipMemcpyAsync(t1, c->t1.p, c->ov() * 8, hipMemcpyDeviceToHost, c->st), "t1 copy");
+    if (t2) check(hipMemcpyAsync(t2, c->t2.p, c->oovv_n() * 8, hipMemcpyDeviceToHost, c->st), "t2 copy");
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+int ugx_energy(ugx_ctx *c, int form, double *e) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!e) throw UgxError(UGX_BAD_INPUT, "null output");
+    build_tau(c);
+    *e = energy(c, form);
+    return UGX_OK;
+  });
+}
+
+int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    ugx_options opt;
+    if (opt_in) opt = *opt_in; else ugx_default_options(&opt);
+    if (opt.maxiter < 0 || opt.diis_nvec < 1 || opt.diis_nvec > 8 || opt.diis_min_vecs < 1 ||
+        opt.precision < 0 || opt.precision > 2 || !(opt.r_convergence > 0))
+      throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: invalid options");
+    if (opt.guess == UGX_GUESS_USER && !c->have_amps)
+      throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: UGX_GUESS_USER without ugx_set_amplitudes");
+    ugx_result r;
+    std::memset(&r, 0, sizeof(r));
+    Engine &E = *c->eng;
+    E.reset_counters();
+    auto t0 = std::chrono::steady_clock::now();
+    if (opt.guess == UGX_GUESS_MP2) guess_mp2(c);
+    build_tau(c);
+    double emp2 = energy(c, opt.energy_form);
+    r.e_mp2 = emp2;
+    if (opt.print)
+      std::printf("\t Iter   Correlation Energy   T1 Norm      RMS   \n\t  %3d  %20.15f\n", 0, emp2);
+    double eccsd = emp2, rms = 0.0;
+    int prec = opt.precision;
+    c->diis_err.clear(); c->diis_amp.clear(); c->diis_nvec = 0;
+    int iter, niter = 0, nlow = 0;
+    for (iter = 1; iter <= opt.maxiter; iter++) {
+      amp_save(c);
+      residual(c, prec);
+      rms = increment(c, nullptr);
+      niter = iter;
+      if (prec != PREC_FP64) nlow++;
+      if (!std::isfinite(rms)) throw UgxError(UGX_NONFINITE, "non-finite residual norm");
+      if (rms < opt.r_convergence && prec == PREC_FP64) break;
+      if (prec != PREC_FP64 && rms < opt.polish_threshold) {
+        // polish: the remaining iterations run in FP64; the subspace built
+        // from low-precision residuals is dropped
+        prec = PREC_FP64;
+        diis_reset(c);
+      }
+      if (opt.diis) diis_step(c, opt.diis_nvec, opt.diis_policy, opt.diis_min_vecs);
+      build_tau(c);
+      eccsd = energy(c, opt.energy_form);
+      if (opt.print) std::printf("\t  %3d  %20.15f  %8.6f  %8.6e\n", iter, eccsd, t1norm(c), rms);
+    }
+    build_tau(c);
+    check(hipStreamSynchronize(c->st), "stream sync");
+    auto t1c = std::chrono::steady_clock::now();
+    r.e_ccsd = eccsd;
+    r.e_ccsd_final = energy(c, opt.energy_form);
+    r.rms = rms;
+    r.niter = niter;
+    r.niter_low = nlow;
+    r.seconds_ccsd = std::chrono::duration<double>(t1c - t0).count();
+    r.flops_ccsd = E.flops_total();
+    E.collect_stats();
+    const bool conv = (rms < opt.r_convergence);
+    if (conv && opt.wfn_ccsd_t) {
+      auto t2c = std::chrono::steady_clock::now();
+      r.e_t = triples(c, opt.precision, nullptr);
+      r.seconds_triples = std::chrono::duration<double>(std::chrono::steady_clock::now() - t2c).count();
+    }
+    if (res) *res = r;
+    c->have_amps = true;
+    if (!conv) {
+      c->err = "Computation has not converged.";
+      return UGX_NOT_CONVERGED;
+    }
+    return UGX_OK;
+  });
+}
+
+int ugx_triples(ugx_ctx *c, int precision, double *e_t) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!e_t || precision < 0 || precision > 2) throw UgxError(UGX_BAD_INPUT, "ugx_triples: bad arguments");
+    *e_t = triples(c, precision, nullptr);
+    return UGX_OK;
+  });
+}
+
+int ugx_triples_per_triplet(ugx_ctx *c, int precision, double *e) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!e || precision < 0 || precision > 2) throw UgxError(UGX_BAD_INPUT, "bad arguments");
+    std::vector<double> per;
+    triples(c, precision, &per);
+    std::memcpy(e, per.data(), per.size() * 8);
+    return UGX_OK;
+  });
+}
+
+int ugx_w3(ugx_ctx *c, int i, int j, int k, int precision, double *W3) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!W3 || i < 0 || j < 0 || k < 0 || i >= c->no || j >= c->no || k >= c->no || precision < 0 ||
+        precision > 2)
+      throw UgxError(UGX_BAD_INPUT, "ugx_w3: bad arguments");
+    triples_prepare(c, precision);
+    if (precision != PREC_FP64) {
+      const long n = c->oovv_n();
+      c->f_t2.alloc(n * 4);
+      PermDesc pd{};
+      pd.rank = 1; pd.od[0] = (int)n; pd.istr[0] = 1; pd.in = c->t2.p; pd.out = c->f_t2.p; pd.in_t = 0;
+      pd.out_t = 1; pd.alpha = 1.0; pd.beta = 0.0;
+      launch_permute(pd, c->st);
+    }
+    const long v3 = (long)c->nv * c->nv * c->nv;
+    if (c->tWbuf.bytes < (size_t)v3 * 8) c->tWbuf.alloc(v3 * 8);
+    std::vector<int> one = {i, j, k};
+    triples_build_w(c, precision, 0, 1, one);
+    check(hipMemcpyAsync(W3, c->tWbuf.p, v3 * 8, hipMemcpyDeviceToHost, c->st), "W3 copy");
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+int ugx_residual(ugx_ctx *c, const double *t1, const double *t2, int precision, double *R1, double *R2) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!t1 || !t2 || precision < 0 || precision > 2) throw UgxError(UGX_BAD_INPUT, "ugx_residual: bad arguments");
+    check(hipMemcpyAsync(c->t1o.p, t1, c->ov() * 8, hipMemcpyHostToDevice, c->st), "t1 copy");
+    check(hipMemcpyAsync(c->t2o.p, t2, c->oovv_n() * 8, hipMemcpyHostToDevice, c->st), "t2 copy");
+    launch_tau(c->no, c->nv, c->t1o.d(), c->t2o.d(), c->tau.d(), c->ttau.d(), c->st);
+    residual(c, precision);
+    // R2 = X + X^T via the update kernel (its t outputs go to scratch: U and R1 copy)
+    DBuf r2;
+    r2.alloc(c->oovv_n() * 8);
+    DBuf t1s, t2s;
+    t1s.alloc(c->ov() * 8);
+    t2s.alloc(c->oovv_n() * 8);
+    int np = update_nparts(c->no, c->nv);
+    launch_update(c->no, c->nv, c->eps.d(), c->R1.d(), c->X.d(), c->t1o.d(), c->t2o.d(), t1s.d(), t2s.d(), r2.d(),
+                  c->part.d(), np, c->st);
+    if (R1) check(hipMemcpyAsync(R1, c->R1.p, c->ov() * 8, hipMemcpyDeviceToHost, c->st), "R1 copy");
+    if (R2) check(hipMemcpyAsync(R2, r2.p, c->oovv_n() * 8, hipMemcpyDeviceToHost, c->st), "R2 copy");
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+int ugx_intermediates(ugx_ctx *c, double *Fvv, double *Foo, double *Fov, double *Woooo, double *Wovov,
+                      double *Wovvo, double *Wovoo) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!c->have_intermediates) throw UgxError(UGX_NOT_READY, "no residual evaluated yet");
+    const int o = c->no, v = c->nv;
+    const long ovov = (long)o * v * o * v;
+    auto get = [&](double *dst, const void *src, long n) {
+      if (dst) check(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, c->st), "copy");
+    };
+    get(Fvv, c->Fvv.p, (long)v * v);
+    get(Foo, c->Foo.p, (long)o * o);
+    get(Fov, c->Fov.p, (long)o * v);
+    get(Woooo, c->Woooo.p, (long)o * o * o * o);
+    get(Wovoo, c->Z.p, (long)o * v * o * o);
+    DBuf tmp;
+    tmp.alloc(ovov * 8);
+    if (Wovov) {  // WJ[m][e][j][b] -> Wovov[m][b][j][e]
+      c->eng->permute("mejb->mbje", 1.0, vref(c->WJ.d(), {o, v, o, v}), 0.0, vref(tmp.d(), {o, v, o, v}));
+      get(Wovov, tmp.p, ovov);
+      check(hipStreamSynchronize(c->st), "stream sync");
+    }
+    if (Wovvo) {  // WB[m][e][j][b] -> Wovvo[m][b][e][j]
+      c->eng->permute("mejb->mbej", 1.0, vref(c->WB.d(), {o, v, o, v}), 0.0, vref(tmp.d(), {o, v, v, o}));
+      get(Wovvo, tmp.p, ovov);
+    }
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+void ugx_set_profiling(ugx_ctx *c, int on) {
+  if (c && c->eng) c->eng->set_profiling(on != 0);
+}
+
+size_t ugx_kernel_stats(ugx_ctx *c, char *buf, size_t buflen) {
+  if (!c || !c->eng) return 0;
+  std::string s;
+  try {
+    c->eng->collect_stats();
+    char line[256];
+    for (auto &kv : c->eng->stats()) {
+      std::snprintf(line, sizeof(line), "%s %ld %.6f %.6f\n", kv.first.c_str(), kv.second.launches, kv.second.ms,
+                    kv.second.flops * 1e-9);
+      s += line;
+    }
+  } catch (...) {
+    return 0;
+  }
+  if (buf && buflen) {
+    size_t n = std::min(buflen - 1, s.size());
+    std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
+  }
+  return s.size() + 1;
+}
+
+int ugx_gemm(ugx_ctx *c, char opa, char opb, int M, int N, int K, double alpha, const void *A, long lda,
+             const void *B, long ldb, double beta, double *C, long ldc, int precision, int tile) {
+  return guarded(c, [&]() {
+    if (M < 0 || N < 0 || K < 0 || !A || !B || !C || precision < 0 || precision > 2 || tile >= gemm_num_tiles())
+      throw UgxError(UGX_BAD_INPUT, "ugx_gemm: bad arguments");
+    GemmDesc d{};
+    d.M = M; d.N = N; d.nseg = 1; d.batch = 1; d.cdiv = 0x7fffffff;
+    GemmSeg &g = d.seg[0];
+    g.A = A; g.B = B; g.K = K; g.sign = 1.0;
+    if (opa == 'N' || opa == 'n') { g.sam = lda; g.sak = 1; } else { g.sam = 1; g.sak = lda; }
+    if (opb == 'N' || opb == 'n') { g.sbk = ldb; g.sbn = 1; } else { g.sbk = 1; g.sbn = ldb; }
+    d.C = C; d.scm = ldc; d.scn = 1; d.alpha = alpha; d.beta = beta;
+    launch_gemm(d, nullptr, precision, tile, c->st);
+    check(hipGetLastError(), "gemm launch");
+    return UGX_OK;
+  });
+}
+
+int ugx_synchronize(ugx_ctx *c) {
+  return guarded(c, [&]() {
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,297 @@
+// Contraction engine (see ugx_engine.h).
+#include "ugx_engine.h"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+
+namespace ugx {
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+struct Spec {
+  std::string a, b, c;
+};
+
+Spec parse(const std::string &s) {
+  Spec r;
+  size_t comma = s.find(','), arrow = s.find("->");
+  if (arrow == std::string::npos) throw std::runtime_error("ugx: contraction spec needs '->': " + s);
+  if (comma == std::string::npos || comma > arrow) {
+    r.a = s.substr(0, arrow);
+    r.c = s.substr(arrow + 2);
+  } else {
+    r.a = s.substr(0, comma);
+    r.b = s.substr(comma + 1, arrow - comma - 1);
+    r.c = s.substr(arrow + 2);
+  }
+  return r;
+}
+
+long extent_of(const std::string &letters, const std::map<char, int> &ext) {
+  long n = 1;
+  for (char ch : letters) n *= ext.at(ch);
+  return n;
+}
+
+// Input strides (elements) of a row-major tensor with the given letters.
+std::map<char, long> strides_of(const std::string &letters, const std::map<char, int> &ext) {
+  std::map<char, long> s;
+  long st = 1;
+  for (int d = (int)letters.size() - 1; d >= 0; d--) {
+    s[letters[d]] = st;
+    st *= ext.at(letters[d]);
+  }
+  return s;
+}
+
+bool is_perm_of(const std::string &x, const std::string &y) {
+  std::string a = x, b = y;
+  std::sort(a.begin(), a.end());
+  std::sort(b.begin(), b.end());
+  return a == b;
+}
+
+size_t esize(int dtype) { return dtype == 0 ? 8 : 4; }
+
+}  // namespace
+
+Engine::~Engine() {
+  for (auto &s : slots_)
+    if (s.first) (void)hipFree(s.first);
+  for (auto &kv : cache_) (void)hipFree(kv.second.first);
+  for (auto &p : pending_) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  for (auto e : free_events_) (void)hipEventDestroy(e);
+}
+
+void Engine::drop_cache() {
+  hip_check(hipStreamSynchronize(st_), "drop_cache sync");
+  for (auto &kv : cache_) (void)hipFree(kv.second.first);
+  cache_.clear();
+}
+
+size_t Engine::cached_bytes() const {
+  size_t n = 0;
+  for (auto &kv : cache_) n += kv.second.second;
+  return n;
+}
+
+void *Engine::slot(int i, size_t bytes) {
+  auto &s = slots_.at(i);
+  if (s.second < bytes) {
+    hip_check(hipStreamSynchronize(st_), "workspace sync");
+    if (s.first) hip_check(hipFree(s.first), "workspace free");
+    s.first = nullptr;
+    hip_check(hipMalloc(&s.first, bytes), "workspace alloc");
+    s.second = bytes;
+  }
+  return s.first;
+}
+
+hipEvent_t Engine::event() {
+  if (!free_events_.empty()) {
+    hipEvent_t e = free_events_.back();
+    free_events_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hip_check(hipEventCreate(&e), "event create");
+  return e;
+}
+
+void Engine::collect_stats() {
+  if (pending_.empty()) return;
+  hip_check(hipStreamSynchronize(st_), "stats sync");
+  for (auto &p : pending_) {
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, p.a, p.b), "event elapsed");
+    auto &k = stats_[p.label];
+    k.launches++;
+    k.ms += ms;
+    k.flops += p.flops;
+    free_events_.push_back(p.a);
+    free_events_.push_back(p.b);
+  }
+  pending_.clear();
+}
+
+// Returns a device pointer holding tensor T (letters) in the order `want`
+// with element type dtype.  No copy when T already is that; constants are
+// permuted/converted once and cached; variables go to workspace slot_id.
+const void *Engine::prepared(const TRef &T, const std::string &letters, const std::string &want, int dtype,
+                             int slot_id) {
+  if (letters == want && T.dtype == dtype) return T.p;
+  std::map<char, int> ext;
+  for (size_t d = 0; d < letters.size(); d++) ext[letters[d]] = T.dims[d];
+  long n = extent_of(letters, ext);
+  void *dst;
+  if (T.constant) {
+    auto key = std::make_tuple(T.p, want, dtype);
+    auto it = cache_.find(key);
+    if (it != cache_.end()) return it->second.first;
+    hip_check(hipMalloc(&dst, n * esize(dtype)), "cache alloc");
+    cache_[key] = {dst, n * esize(dtype)};
+  } else {
+    dst = slot(slot_id, n * esize(dtype));
+  }
+  auto is = strides_of(letters, ext);
+  PermDesc pd{};
+  pd.rank = (int)want.size();
+  if (pd.rank == 0) throw std::runtime_error("ugx: empty operand");
+  for (int d = 0; d < pd.rank; d++) { pd.od[d] = ext.at(want[d]); pd.istr[d] = is.at(want[d]); }
+  pd.in = T.p; pd.out = dst; pd.in_t = T.dtype; pd.out_t = dtype; pd.alpha = 1.0; pd.beta = 0.0;
+  launch_permute(pd, st_);
+  return dst;
+}
+
+void Engine::permute(const std::string &s, double alpha, const TRef &A, double beta, const TRef &C) {
+  Spec sp = parse(s);
+  if (!sp.b.empty()) throw std::runtime_error("ugx: permute takes one operand");
+  if (!is_perm_of(sp.a, sp.c)) throw std::runtime_error("ugx: permute letters differ: " + s);
+  if (A.dims.size() != sp.a.size() || C.dims.size() != sp.c.size())
+    throw std::runtime_error("ugx: permute rank mismatch: " + s);
+  std::map<char, int> ext;
+  for (size_t d = 0; d < sp.a.size(); d++) ext[sp.a[d]] = A.dims[d];
+  for (size_t d = 0; d < sp.c.size(); d++)
+    if (ext.at(sp.c[d]) != C.dims[d]) throw std::runtime_error("ugx: permute extent mismatch: " + s);
+  auto is = strides_of(sp.a, ext);
+  PermDesc pd{};
+  pd.rank = (int)sp.c.size();
+  for (int d = 0; d < pd.rank; d++) { pd.od[d] = ext.at(sp.c[d]); pd.istr[d] = is.at(sp.c[d]); }
+  pd.in = A.p; pd.out = const_cast<void *>(C.p); pd.in_t = A.dtype; pd.out_t = C.dtype;
+  pd.alpha = alpha; pd.beta = beta;
+  launch_permute(pd, st_);
+}
+
+struct Engine::Plan {
+  double cost;
+  bool swap;          // GEMM computes with (B, A) in the (A, B) roles
+  std::string m, n, k;
+};
+
+void Engine::contract(const std::string &s, double alpha, const TRef &A, const TRef &B, double beta, const TRef &C,
+                      int prec, const char *label) {
+  Spec sp = parse(s);
+  if (sp.b.empty()) throw std::runtime_error("ugx: contraction needs two operands: " + s);
+  if (A.dims.size() != sp.a.size() || B.dims.size() != sp.b.size() || C.dims.size() != sp.c.size())
+    throw std::runtime_error("ugx: rank mismatch in " + s);
+  if (C.dtype != 0) throw std::runtime_error("ugx: contraction output must be FP64");
+  std::map<char, int> ext;
+  auto add = [&](const std::string &l, const TRef &T) {
+    for (size_t d = 0; d < l.size(); d++) {
+      auto it = ext.find(l[d]);
+      if (it != ext.end() && it->second != T.dims[d]) throw std::runtime_error("ugx: extent clash in " + s);
+      ext[l[d]] = T.dims[d];
+    }
+  };
+  add(sp.a, A); add(sp.b, B); add(sp.c, C);
+
+  auto in = [](const std::string &x, char ch) { return x.find(ch) != std::string::npos; };
+  std::string K, Ma, Nb;  // K in A's order, M in A's order, N in B's order
+  for (char ch : sp.a) {
+    if (in(sp.b, ch) && !in(sp.c, ch)) K += ch;
+    else if (in(sp.c, ch) && !in(sp.b, ch)) Ma += ch;
+    else throw std::runtime_error(std::string("ugx: unsupported index '") + ch + "' in " + s);
+  }
+  for (char ch : sp.b) {
+    if (in(sp.a, ch)) continue;
+    if (in(sp.c, ch)) Nb += ch;
+    else throw std::runtime_error(std::string("ugx: unsupported index '") + ch + "' in " + s);
+  }
+  for (char ch : sp.c)
+    if (!in(sp.a, ch) && !in(sp.b, ch)) throw std::runtime_error("ugx: output index not in operands: " + s);
+  auto order_in = [&](const std::string &group, const std::string &ref) {
+    std::string r;
+    for (char ch : ref)
+      if (in(group, ch)) r += ch;
+    return r;
+  };
+  std::string Mc = order_in(Ma, sp.c), Nc = order_in(Nb, sp.c), Kb = order_in(K, sp.b);
+
+  const double sA = (double)extent_of(sp.a, ext), sB = (double)extent_of(sp.b, ext), sC = (double)extent_of(sp.c, ext);
+  const bool fp64 = (prec == PREC_FP64);
+  auto opcost = [&](const TRef &T, const std::string &letters, const std::string &g1, const std::string &g2,
+                    double size) {
+    bool ok = (letters == g1 + g2) || (letters == g2 + g1);
+    bool needs = !ok || (!fp64 && T.dtype == 0);
+    if (!needs) return 0.0;
+    return T.constant ? 0.01 * size : 2.0 * size;
+  };
+  Plan best{1e300, false, "", "", ""};
+  for (int sw = 0; sw < 2; sw++)
+    for (int mi = 0; mi < 2; mi++)
+      for (int ni = 0; ni < 2; ni++)
+        for (int ki = 0; ki < 2; ki++) {
+          std::string m = mi ? Ma : Mc, n = ni ? Nb : Nc, k = ki ? Kb : K;
+          double c = opcost(A, sp.a, m, k, sA) + opcost(B, sp.b, k, n, sB);
+          // C = A B with rows m, cols n (sw = 0) or rows n, cols m (sw = 1)
+          std::string rows = sw ? n : m, cols = sw ? m : n;
+          if (sp.c == rows + cols) c += 0.0;
+          else c += 3.0 * sC;
+          if (c < best.cost) best = Plan{c, sw == 1, m, n, k};
+        }
+
+  const int dt = fp64 ? 0 : 1;
+  const long Me = extent_of(best.m, ext), Ne = extent_of(best.n, ext), Ke = extent_of(best.k, ext);
+  // Operand A as matrix (m, k); B as (k, n).
+  auto mat = [&](const TRef &T, const std::string &letters, const std::string &g1, const std::string &g2,
+                 long e1, long e2, int slot_id, long &s1, long &s2) -> const void * {
+    // returns pointer; strides s1 (along g1 index), s2 (along g2 index)
+    if (letters == g1 + g2 && (fp64 || T.dtype == 1) && T.dtype == dt) { s1 = e2; s2 = 1; return T.p; }
+    if (letters == g2 + g1 && T.dtype == dt) { s1 = 1; s2 = e1; return T.p; }
+    // permute/convert into g1+g2 unless the g2+g1 layout only needs a conversion
+    if (letters == g2 + g1) {
+      s1 = 1; s2 = e1;
+      return prepared(T, letters, g2 + g1, dt, slot_id);
+    }
+    s1 = e2; s2 = 1;
+    return prepared(T, letters, g1 + g2, dt, slot_id);
+  };
+  GemmDesc d{};
+  long sam, sak, sbk, sbn;
+  const void *Ap = mat(A, sp.a, best.m, best.k, Me, Ke, 0, sam, sak);
+  long skb, snb;
+  const void *Bp = mat(B, sp.b, best.n, best.k, Ne, Ke, 1, snb, skb);
+  sbk = skb; sbn = snb;
+
+  std::string rows = best.swap ? best.n : best.m, cols = best.swap ? best.m : best.n;
+  const long Re = best.swap ? Ne : Me, Ce = best.swap ? Me : Ne;
+  GemmSeg g{};
+  if (!best.swap) {
+    g.A = Ap; g.sam = sam; g.sak = sak; g.B = Bp; g.sbk = sbk; g.sbn = sbn;
+  } else {
+    // rows = n: A' (n,k) = B(k,n);  B' (k,m) = A(m,k)
+    g.A = Bp; g.sam = sbn; g.sak = sbk; g.B = Ap; g.sbk = sak; g.sbn = sam;
+  }
+  g.K = (int)Ke; g.sign = 1.0;
+  d.M = (int)Re; d.N = (int)Ce; d.nseg = 1; d.seg[0] = g; d.batch = 1; d.cdiv = 0x7fffffff;
+  if (Re > 0x7fffffffL || Ce > 0x7fffffffL || Ke > 0x7fffffffL) throw std::runtime_error("ugx: GEMM too large");
+
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (profiling_ && label) { e0 = event(); e1 = event(); hip_check(hipEventRecord(e0, st_), "event record"); }
+
+  const bool direct = (sp.c == rows + cols);
+  if (direct) {
+    d.C = (double *)C.p; d.scm = Ce; d.scn = 1; d.scm1 = 0; d.alpha = alpha; d.beta = beta;
+    launch_gemm(d, nullptr, prec, -1, st_);
+  } else {
+    double *tmp = (double *)slot(2, (size_t)Re * Ce * 8);
+    d.C = tmp; d.scm = Ce; d.scn = 1; d.scm1 = 0; d.alpha = 1.0; d.beta = 0.0;
+    launch_gemm(d, nullptr, prec, -1, st_);
+    TRef T = vref(tmp, {});
+    for (char ch : rows + cols) T.dims.push_back(ext.at(ch));
+    permute(rows + cols + "->" + sp.c, alpha, T, beta, C);
+  }
+  const double fl = 2.0 * (double)Me * (double)Ne * (double)Ke;
+  flops_ += fl;
+  if (profiling_ && label) {
+    hip_check(hipEventRecord(e1, st_), "event record");
+    pending_.push_back(Pending{label, e0, e1, fl});
+  }
+}
+
+}  // namespace ugx

@@ -1,0 +1,87 @@
+// Contraction engine: einsum-style tensor contractions executed as
+// (permute) -> MFMA GEMM -> (permute-accumulate) on one HIP stream.
+//
+// A contraction "mnfe,nfjb->mejb" names the index letters of A, B and C (all
+// row-major).  Letters in A and B but not C are summed (K); letters of A in C
+// form the GEMM rows (M), letters of B in C the columns (N).  The planner
+// tries the index orders that can make each operand a plain strided matrix
+// (M and N in C's or the operand's order, K in A's or B's order, and the
+// A/B role swap) and picks the plan with the least permutation traffic.
+// Permuted copies of constant operands (the integral blocks) are cached, so
+// after the first iteration they cost nothing; FP32 copies for the mixed
+// forms are cached the same way.  C is always FP64.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "ugx_internal.h"
+
+namespace ugx {
+
+struct TRef {
+  const void *p = nullptr;
+  std::vector<int> dims;
+  bool constant = false;  // contents never change while cached copies live
+  int dtype = 0;          // 0 double, 1 float
+};
+
+inline TRef cref(const double *p, std::vector<int> d) { TRef t; t.p = p; t.dims = std::move(d); t.constant = true; return t; }
+inline TRef vref(const double *p, std::vector<int> d) { TRef t; t.p = p; t.dims = std::move(d); return t; }
+
+struct KernelStat {
+  long launches = 0;
+  double ms = 0.0;
+  double flops = 0.0;
+};
+
+class Engine {
+ public:
+  explicit Engine(hipStream_t st) : st_(st) {}
+  ~Engine();
+  Engine(const Engine &) = delete;
+  Engine &operator=(const Engine &) = delete;
+
+  // C = alpha * contract(A, B) + beta * C at the given precision.
+  void contract(const std::string &spec, double alpha, const TRef &A, const TRef &B, double beta, const TRef &C,
+                int prec, const char *label = nullptr);
+  // C = alpha * permute(A) + beta * C, spec "mbje->mejb".
+  void permute(const std::string &spec, double alpha, const TRef &A, double beta, const TRef &C);
+
+  // Per-label device time (HIP events around each contraction) when on.
+  void set_profiling(bool on) { profiling_ = on; }
+  void collect_stats();  // waits for the stream, folds event pairs into stats_
+  std::map<std::string, KernelStat> &stats() { return stats_; }
+  double flops_total() const { return flops_; }
+  void reset_counters() { flops_ = 0.0; stats_.clear(); }
+  void drop_cache();
+  size_t cached_bytes() const;
+  hipStream_t stream() const { return st_; }
+
+ private:
+  struct Plan;
+  void *slot(int i, size_t bytes);
+  const void *prepared(const TRef &T, const std::string &letters, const std::string &want, int dtype,
+                       int slot_id);
+  hipStream_t st_;
+  std::vector<std::pair<void *, size_t>> slots_{4, {nullptr, 0}};
+  std::map<std::tuple<const void *, std::string, int>, std::pair<void *, size_t>> cache_;
+  bool profiling_ = false;
+  struct Pending {
+    std::string label;
+    hipEvent_t a, b;
+    double flops;
+  };
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> free_events_;
+  std::map<std::string, KernelStat> stats_;
+  double flops_ = 0.0;
+  hipEvent_t event();
+};
+
+void hip_check(hipError_t e, const char *what);
+
+}  // namespace ugx

@@ -1,0 +1,116 @@
+// Internal declarations shared by the gfx950 kernels, the contraction engine
+// and the C-ABI layer.  Nothing here crosses the library boundary.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace ugx {
+
+// Arithmetic of a dense contraction.
+enum Prec : int {
+  PREC_FP64 = 0,    // FP64 operands, v_mfma_f64_16x16x4_f64, FP64 accumulate
+  PREC_FP32 = 1,    // FP32 operands, v_mfma_f32_16x16x4_f32, FP32 accumulate
+  PREC_BF16X3 = 2,  // FP32 operands split hi+lo into BF16, three
+                    // v_mfma_f32_16x16x32_bf16 products (hi*hi + hi*lo +
+                    // lo*hi), FP32 accumulate
+};
+
+// One K segment of a contraction C(m,n) += sign * sum_k A(m,k) B(k,n).
+//   A(m,k) = A[m*sam + k*sak]       B(k,n) = B[k*sbk + n*sbn]
+// Element type is double for PREC_FP64 and float otherwise.
+struct GemmSeg {
+  const void *A;
+  const void *B;
+  long sam, sak, sbk, sbn;
+  long sAz, sBz;  // per-batch offsets (elements) when no pointer table
+  int K;
+  double sign;
+};
+
+// C(m,n) lives at C[(m / cdiv) * scm1 + (m % cdiv) * scm + n * scn]; with
+// cdiv >= M this is the plain C[m*scm + n*scn].  C is always double: the
+// FP32/BF16 forms widen their FP32 accumulators in the epilogue, so every
+// result tensor stays FP64.
+struct GemmDesc {
+  int M, N;
+  int nseg;
+  GemmSeg seg[2];
+  double *C;
+  long scm, scn, scm1, sCz;
+  int cdiv;
+  double alpha, beta;
+  int batch;
+};
+
+// Per-batch pointers for grouped launches (used by the (T) build where each
+// batch entry is one triplet): A/B of both segments and C.
+struct GemmPtrs {
+  const void *A[2];
+  const void *B[2];
+  double *C;
+};
+
+// Launches C = alpha * sum_seg sign * A B + beta * C over desc.batch
+// entries.  ptrs (device memory, desc.batch entries) overrides the strided
+// batch addressing when non-null.  tile = -1 picks a tile shape by size.
+void launch_gemm(const GemmDesc &d, const GemmPtrs *ptrs, int prec, int tile, hipStream_t st);
+int gemm_num_tiles();
+const char *gemm_tile_name(int tile);
+
+// ------------------------------------------------------------ streaming
+// out[idx] = alpha * in[perm(idx)] + beta * out[idx] over a row-major output
+// of rank <= 6 with dims od[]; istr[] gives, per output dim, the stride of
+// the input element.  in/out element types: 0 = double, 1 = float.
+struct PermDesc {
+  int rank;
+  int od[6];
+  long istr[6];
+  const void *in;
+  void *out;
+  int in_t, out_t;
+  double alpha, beta;
+};
+void launch_permute(const PermDesc &p, hipStream_t st);
+
+void launch_fill(long n, double v, double *y, hipStream_t st);
+void launch_axpby(long n, double a, const double *x, double b, double *y, hipStream_t st);  // y = a x + b y
+// tau = t2 + t1 (x) t1, ttau = t2 + 0.5 t1 (x) t1 (ttau may be null)
+void launch_tau(int no, int nv, const double *t1, const double *t2, double *tau, double *ttau, hipStream_t st);
+// Y[n][f][j][b] = 0.5 t2[j][n][f][b] + t1[j][f] t1[n][b]
+void launch_build_Y(int no, int nv, const double *t1, const double *t2, double *Y, hipStream_t st);
+// R2 = X + X^T(ij,ab);  rsq partials of |R1|^2 + |R2|^2;  t = t_old + R/D
+// with D from the Fock diagonal eps (nact).  R1/R2 are written out when the
+// pointers are non-null.
+int update_nparts(int no, int nv);
+void launch_update(int no, int nv, const double *eps, const double *R1, const double *X, const double *t1old,
+                   const double *t2old, double *t1, double *t2, double *R2out, double *partials, int npart,
+                   hipStream_t st);
+// MP2 guess t1 = f_ia/D1, t2 = <ij|ab>/D2 from the full Fock matrix
+void launch_guess(int no, int nv, const double *fock, const double *oovv, double *t1, double *t2, hipStream_t st);
+// partial sums of x.y
+void launch_dot(long n, const double *x, const double *y, double *partials, int npart, hipStream_t st);
+void launch_reduce(const double *partials, int n, double *out, hipStream_t st);
+// DIIS: err = t - t_old, amp = t (over concatenated t1|t2)
+void launch_diis_store(long n1, long n2, const double *t1, const double *t1old, const double *t2,
+                       const double *t2old, double *err, double *amp, hipStream_t st);
+// partials[p*npart + blk] = sum err_new . err_p over nvec stored vectors
+void launch_diis_dots(long len, const double *e, const double *const *vecs, int nvec, double *partials,
+                      int npart, hipStream_t st);
+void launch_diis_combine(long n1, long n2, const double *const *amps, const double *coef, int nvec, double *t1,
+                         double *t2, hipStream_t st);
+// E = 2 sum f_ia t1_ia + sum tau_ijab L_ijab into partials
+void launch_energy(int no, int nv, const double *fov, const double *t1, const double *tau, const double *Loovv,
+                   double *partials, int npart, hipStream_t st);
+int energy_nparts(int no, int nv);
+
+// ------------------------------------------------------------ (T)
+// Energy epilogue over a batch of triplets: for each triplet (i,j,k) with W
+// (v^3, [a][b][c]) already built, sums the (T) bracket over a>=b>=c.
+// etrip receives ntrip values; part is scratch of ntrip*nv doubles.
+void launch_triples_energy(int no, int nv, int ntrip, const int *trip_ijk, const double *W, long wstride,
+                           const double *eps, const double *oovv, const double *t1, double *etrip, double *part,
+                           hipStream_t st);
+
+}  // namespace ugx

@@ -1,0 +1,392 @@
+// Streaming kernels: tensor permutation (with type conversion and
+// accumulation), tau, the Jacobi update with the residual norm, dot products,
+// the DIIS vectors and the correlation energy.  All are HBM-bound.  Each
+// writes its output coalesced; the permutation kernel stages a 64x64 tile
+// through LDS whenever the output's fastest index is not the input's, so
+// both sides stay coalesced.  Reductions are deterministic: per-block
+// partial sums (wave64 butterflies, then across the block's waves) followed
+// by one single-block pass, all in FP64.
+#include "ugx_internal.h"
+
+#include <stdexcept>
+
+namespace ugx {
+namespace {
+
+constexpr int NT = 256;
+
+inline int nblocks(long n, int per = NT, int cap = 1 << 16) {
+  long b = (n + per - 1) / per;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+__device__ inline double block_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __shared__ double red[NT / 64];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NT / 64; k++) s += red[k];
+  return s;  // valid in thread 0
+}
+
+template <typename T>
+__device__ inline double ld(const void *p, long i) { return (double)((const T *)p)[i]; }
+
+// ---------------------------------------------------------------- permute
+struct PermArgs {
+  int rank;
+  int od[6];
+  long istr[6];
+  const void *in;
+  void *out;
+  double alpha, beta;
+};
+
+// Linear form: used when the output's fastest index is also the input's
+// unit-stride index (or the tensor is small).
+template <typename TI, typename TO>
+__global__ void k_perm_linear(PermArgs p, long n) {
+  for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n; idx += (long)gridDim.x * NT) {
+    long r = idx, off = 0;
+    for (int d = p.rank - 1; d >= 0; d--) {
+      int c = (int)(r % p.od[d]);
+      r /= p.od[d];
+      off += c * p.istr[d];
+    }
+    double v = p.alpha * (double)((const TI *)p.in)[off];
+    if (p.beta != 0.0) v += p.beta * (double)((TO *)p.out)[idx];
+    ((TO *)p.out)[idx] = (TO)v;
+  }
+}
+
+// Tiled form: q = the output dim with input stride 1, l = the last output
+// dim.  A 64x64 (q,l) tile is read along q and written along l.
+template <typename TI, typename TO>
+__global__ void k_perm_tiled(PermArgs p, int q, long nrest, int tq, int tl) {
+  __shared__ TO tile[64][65];
+  const int L = p.rank - 1;
+  const int Dq = p.od[q], Dl = p.od[L];
+  long ostr[6];
+  {
+    long s = 1;
+    for (int d = p.rank - 1; d >= 0; d--) { ostr[d] = s; s *= p.od[d]; }
+  }
+  long nb = (long)tq * tl * nrest;
+  for (long b = blockIdx.x; b < nb; b += gridDim.x) {
+    long r = b;
+    int bl = (int)(r % tl); r /= tl;
+    int bq = (int)(r % tq); r /= tq;
+    // remaining dims (all except q and L), row-major over output order
+    long ibase = 0, obase = 0;
+    for (int d = p.rank - 1; d >= 0; d--) {
+      if (d == q || d == L) continue;
+      int c = (int)(r % p.od[d]);
+      r /= p.od[d];
+      ibase += c * p.istr[d];
+      obase += c * ostr[d];
+    }
+    const int q0 = bq * 64, l0 = bl * 64;
+    // read: threads along q (unit input stride)
+    for (int e = threadIdx.x; e < 64 * 64; e += NT) {
+      int lq = e & 63, ll = e >> 6;
+      int gq = q0 + lq, gl = l0 + ll;
+      TO v = 0;
+      if (gq < Dq && gl < Dl) v = (TO)(p.alpha * (double)((const TI *)p.in)[ibase + gq + gl * p.istr[L]]);
+      tile[lq][ll] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 64; e += NT) {
+      int ll = e & 63, lq = e >> 6;
+      int gq = q0 + lq, gl = l0 + ll;
+      if (gq < Dq && gl < Dl) {
+        long o = obase + gq * ostr[q] + gl;
+        double v = (double)tile[lq][ll];
+        if (p.beta != 0.0) v += p.beta * (double)((TO *)p.out)[o];
+        ((TO *)p.out)[o] = (TO)v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename TI, typename TO>
+void perm_launch(const PermArgs &a, hipStream_t st) {
+  long n = 1;
+  for (int d = 0; d < a.rank; d++) n *= a.od[d];
+  if (n <= 0) return;
+  const int L = a.rank - 1;
+  int q = -1;
+  if (a.istr[L] != 1)
+    for (int d = 0; d < L; d++)
+      if (a.istr[d] == 1 && a.od[d] > 1) q = d;
+  if (q < 0 || n < 4096) {
+    hipLaunchKernelGGL((k_perm_linear<TI, TO>), dim3(nblocks(n)), dim3(NT), 0, st, a, n);
+  } else {
+    int tq = (a.od[q] + 63) / 64, tl = (a.od[L] + 63) / 64;
+    long nrest = n / ((long)a.od[q] * a.od[L]);
+    long nb = (long)tq * tl * nrest;
+    hipLaunchKernelGGL((k_perm_tiled<TI, TO>), dim3((unsigned)(nb < 65536 ? nb : 65536)), dim3(NT), 0, st, a, q,
+                       nrest, tq, tl);
+  }
+}
+
+// ---------------------------------------------------------------- misc
+__global__ void k_axpby(long n, double a, const double *x, double b, double *y) {
+  for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT)
+    y[i] = a * x[i] + (b != 0.0 ? b * y[i] : 0.0);
+}
+
+__global__ void k_fill(long n, double v, double *y) {
+  for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) y[i] = v;
+}
+
+// ccwfn.cc:284-298
+__global__ void k_tau(int no, int nv, const double *t1, const double *t2, double *tau, double *ttau) {
+  long n = (long)no * no * nv * nv;
+  for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n; idx += (long)gridDim.x * NT) {
+    long r = idx;
+    int b = r % nv; r /= nv;
+    int a = r % nv; r /= nv;
+    int j = r % no; r /= no;
+    int i = (int)r;
+    double x = t1[i * nv + a] * t1[j * nv + b];
+    double t = t2[idx];
+    tau[idx] = t + x;
+    if (ttau) ttau[idx] = t + 0.5 * x;
+  }
+}
+
+// Y[n][f][j][b] = 0.5 t2[j][n][f][b] + t1[j][f] t1[n][b] (ccwfn.cc:393,410),
+// written in the (n f)(j b) order the Wmbej/Wmbje GEMMs read.
+__global__ void k_build_Y(int no, int nv, const double *t1, const double *t2, double *Y) {
+  long tot = (long)no * no * nv * nv;
+  for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < tot; idx += (long)gridDim.x * NT) {
+    long r = idx;
+    int b = r % nv; r /= nv;
+    int j = r % no; r /= no;
+    int f = r % nv; r /= nv;
+    int n = (int)r;
+    Y[idx] = 0.5 * t2[(((long)j * no + n) * nv + f) * nv + b] + t1[j * nv + f] * t1[n * nv + b];
+  }
+}
+
+// Jacobi update, ccwfn.cc:555-582, with the t2 residual assembled from its
+// half X (R2_ijab = X_ijab + X_jiba): ||R||^2 partials, t = t_old + R/D,
+// D_ijab = (f_ii + f_jj) - (f_aa + f_bb) so that D_ijab == D_jiba exactly.
+__global__ void k_update(int no, int nv, const double *eps, const double *R1, const double *X, const double *t1old,
+                         const double *t2old, double *t1, double *t2, double *R2out, double *partials) {
+  long n1 = (long)no * nv, n2 = n1 * n1;
+  double s = 0.0;
+  for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n2 + n1; idx += (long)gridDim.x * NT) {
+    if (idx < n2) {
+      long r = idx;
+      int b = r % nv; r /= nv;
+      int a = r % nv; r /= nv;
+      int j = r % no; r /= no;
+      int i = (int)r;
+      double R = X[idx] + X[(((long)j * no + i) * nv + b) * nv + a];
+      double D = (eps[i] + eps[j]) - (eps[no + a] + eps[no + b]);
+      s += R * R;
+      if (R2out) R2out[idx] = R;
+      t2[idx] = t2old[idx] + R / D;
+    } else {
+      long q = idx - n2;
+      int i = q / nv, a = q % nv;
+      double R = R1[q];
+      s += R * R;
+      t1[q] = t1old[q] + R / (eps[i] - eps[no + a]);
+    }
+  }
+  double tot = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// MP2 guess, ccwfn.cc:75-87: t1 = f_ia / D1, t2 = <ij|ab> / D2.
+__global__ void k_guess(int no, int nv, const double *fock, const double *oovv, double *t1, double *t2) {
+  const int n = no + nv;
+  long n1 = (long)no * nv, n2 = n1 * n1;
+  for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n2 + n1; idx += (long)gridDim.x * NT) {
+    if (idx < n2) {
+      long r = idx;
+      int b = r % nv; r /= nv;
+      int a = r % nv; r /= nv;
+      int j = r % no; r /= no;
+      int i = (int)r;
+      double D = (fock[(long)i * n + i] + fock[(long)j * n + j]) -
+                 (fock[(long)(no + a) * n + no + a] + fock[(long)(no + b) * n + no + b]);
+      t2[idx] = oovv[idx] / D;
+    } else {
+      long q = idx - n2;
+      int i = q / nv, a = q % nv;
+      t1[q] = fock[(long)i * n + no + a] / (fock[(long)i * n + i] - fock[(long)(no + a) * n + no + a]);
+    }
+  }
+}
+
+__global__ void k_dot(long n, const double *x, const double *y, double *partials) {
+  double s = 0.0;
+  for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) s += x[i] * y[i];
+  double tot = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+__global__ void k_reduce(const double *partials, int n, double *out) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT) s += partials[i];
+  double tot = block_sum(s);
+  if (threadIdx.x == 0) out[0] = tot;
+}
+
+// ccwfn.cc:260-282.  The one-electron part is written into partials[npart]
+// separately so the host can apply either the reference's form (only the
+// last (i,a) term survives the '=' at ccwfn.cc:272) or the full sum.
+__global__ void k_energy(int no, int nv, const double *fov, const double *t1, const double *tau,
+                         const double *Loovv, double *partials) {
+  long n1 = (long)no * nv, n2 = n1 * n1;
+  double s = 0.0;
+  for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n2; idx += (long)gridDim.x * NT)
+    s += tau[idx] * Loovv[idx];
+  double tot = block_sum(s);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+__global__ void k_diis_store(long n1, long n2, const double *t1, const double *t1old, const double *t2,
+                             const double *t2old, double *err, double *amp) {
+  for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n1 + n2; i += (long)gridDim.x * NT) {
+    double x, xo;
+    if (i < n1) { x = t1[i]; xo = t1old[i]; } else { x = t2[i - n1]; xo = t2old[i - n1]; }
+    err[i] = x - xo;
+    amp[i] = x;
+  }
+}
+
+struct VecList {
+  const double *v[8];
+  double c[8];
+};
+
+__global__ void k_diis_dots(long len, const double *e, VecList vl, int nvec, double *partials, int npart) {
+  double s[8];
+#pragma unroll
+  for (int p = 0; p < 8; p++) s[p] = 0.0;
+  for (long i = blockIdx.x * (long)NT + threadIdx.x; i < len; i += (long)gridDim.x * NT) {
+    double x = e[i];
+#pragma unroll
+    for (int p = 0; p < 8; p++)
+      if (p < nvec) s[p] += x * vl.v[p][i];
+  }
+  for (int p = 0; p < nvec; p++) {
+    double tot = block_sum(s[p]);
+    if (threadIdx.x == 0) partials[p * npart + blockIdx.x] = tot;
+  }
+}
+
+__global__ void k_diis_combine(long n1, long n2, VecList vl, int nvec, double *t1, double *t2) {
+  for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n1 + n2; i += (long)gridDim.x * NT) {
+    double s = 0.0;
+#pragma unroll
+    for (int p = 0; p < 8; p++)
+      if (p < nvec) s += vl.c[p] * vl.v[p][i];
+    if (i < n1) t1[i] = s; else t2[i - n1] = s;
+  }
+}
+
+}  // namespace
+
+void launch_permute(const PermDesc &p, hipStream_t st) {
+  if (p.rank < 1 || p.rank > 6) throw std::runtime_error("ugx: permute rank out of range");
+  PermArgs a;
+  a.rank = p.rank;
+  for (int d = 0; d < 6; d++) { a.od[d] = d < p.rank ? p.od[d] : 1; a.istr[d] = d < p.rank ? p.istr[d] : 0; }
+  a.in = p.in; a.out = p.out; a.alpha = p.alpha; a.beta = p.beta;
+  if (p.in_t == 0 && p.out_t == 0) perm_launch<double, double>(a, st);
+  else if (p.in_t == 0 && p.out_t == 1) perm_launch<double, float>(a, st);
+  else if (p.in_t == 1 && p.out_t == 0) perm_launch<float, double>(a, st);
+  else perm_launch<float, float>(a, st);
+}
+
+void launch_fill(long n, double v, double *y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(NT), 0, st, n, v, y);
+}
+
+void launch_axpby(long n, double a, const double *x, double b, double *y, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(NT), 0, st, n, a, x, b, y);
+}
+
+void launch_tau(int no, int nv, const double *t1, const double *t2, double *tau, double *ttau, hipStream_t st) {
+  long n = (long)no * no * nv * nv;
+  hipLaunchKernelGGL(k_tau, dim3(nblocks(n)), dim3(NT), 0, st, no, nv, t1, t2, tau, ttau);
+}
+
+void launch_build_Y(int no, int nv, const double *t1, const double *t2, double *Y, hipStream_t st) {
+  long n = (long)no * no * nv * nv;
+  hipLaunchKernelGGL(k_build_Y, dim3(nblocks(n)), dim3(NT), 0, st, no, nv, t1, t2, Y);
+}
+
+int update_nparts(int no, int nv) {
+  long n = (long)no * nv * (1 + (long)no * nv);
+  return nblocks(n, NT, 2048);
+}
+
+void launch_update(int no, int nv, const double *eps, const double *R1, const double *X, const double *t1old,
+                   const double *t2old, double *t1, double *t2, double *R2out, double *partials, int npart,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_update, dim3(npart), dim3(NT), 0, st, no, nv, eps, R1, X, t1old, t2old, t1, t2, R2out,
+                     partials);
+}
+
+int energy_nparts(int no, int nv) {
+  long n = (long)no * no * nv * nv;
+  return nblocks(n, NT, 2048);
+}
+
+void launch_energy(int no, int nv, const double *fov, const double *t1, const double *tau, const double *Loovv,
+                   double *partials, int npart, hipStream_t st) {
+  hipLaunchKernelGGL(k_energy, dim3(npart), dim3(NT), 0, st, no, nv, fov, t1, tau, Loovv, partials);
+}
+
+void launch_guess(int no, int nv, const double *fock, const double *oovv, double *t1, double *t2,
+                  hipStream_t st) {
+  long n = (long)no * nv * (1 + (long)no * nv);
+  hipLaunchKernelGGL(k_guess, dim3(nblocks(n)), dim3(NT), 0, st, no, nv, fock, oovv, t1, t2);
+}
+
+void launch_dot(long n, const double *x, const double *y, double *partials, int npart, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot, dim3(npart), dim3(NT), 0, st, n, x, y, partials);
+}
+
+void launch_reduce(const double *partials, int n, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(NT), 0, st, partials, n, out);
+}
+
+void launch_diis_store(long n1, long n2, const double *t1, const double *t1old, const double *t2,
+                       const double *t2old, double *err, double *amp, hipStream_t st) {
+  hipLaunchKernelGGL(k_diis_store, dim3(nblocks(n1 + n2)), dim3(NT), 0, st, n1, n2, t1, t1old, t2, t2old, err,
+                     amp);
+}
+
+void launch_diis_dots(long len, const double *e, const double *const *vecs, int nvec, double *partials,
+                      int npart, hipStream_t st) {
+  if (nvec > 8) throw std::runtime_error("ugx: DIIS subspace above 8");
+  VecList vl{};
+  for (int p = 0; p < nvec; p++) vl.v[p] = vecs[p];
+  hipLaunchKernelGGL(k_diis_dots, dim3(npart), dim3(NT), 0, st, len, e, vl, nvec, partials, npart);
+}
+
+void launch_diis_combine(long n1, long n2, const double *const *amps, const double *coef, int nvec, double *t1,
+                         double *t2, hipStream_t st) {
+  if (nvec > 8) throw std::runtime_error("ugx: DIIS subspace above 8");
+  VecList vl{};
+  for (int p = 0; p < nvec; p++) { vl.v[p] = amps[p]; vl.c[p] = coef[p]; }
+  hipLaunchKernelGGL(k_diis_combine, dim3(nblocks(n1 + n2)), dim3(NT), 0, st, n1, n2, vl, nvec, t1, t2);
+}
+
+}  // namespace ugx

@@ -1,0 +1,243 @@
+"""ctypes binding of the C ABI in include/ugx.h (libugx.so, built in-tree).
+
+The library is the product path: there is no CPU fallback.  Loading fails
+with an ImportError-style message when libugx.so has not been built, and
+every call that returns an error status raises ``UgxError`` carrying the
+library's own message.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libugx.so")
+
+OK, NOT_CONVERGED, BAD_INPUT, OOM, HIP_ERROR, NO_DEVICE, NOT_READY, NONFINITE = range(8)
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+GUESS_MP2, GUESS_USER = 0, 1
+ENERGY_REFERENCE, ENERGY_FULL = 0, 1
+PREC_NAMES = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+
+# Every function include/ugx.h declares, in declaration order.
+EXPORTS = [
+    "ugx_default_options", "ugx_create", "ugx_destroy", "ugx_last_error", "ugx_load_hamiltonian",
+    "ugx_set_amplitudes", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
+    "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_set_profiling",
+    "ugx_kernel_stats", "ugx_gemm", "ugx_synchronize",
+]
+
+
+class UgxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("wfn_ccsd_t", ctypes.c_int), ("maxiter", ctypes.c_int), ("diis", ctypes.c_int),
+        ("r_convergence", ctypes.c_double), ("ooc", ctypes.c_int), ("diis_nvec", ctypes.c_int),
+        ("diis_min_vecs", ctypes.c_int), ("diis_policy", ctypes.c_int), ("precision", ctypes.c_int),
+        ("polish_threshold", ctypes.c_double), ("guess", ctypes.c_int), ("energy_form", ctypes.c_int),
+        ("print", ctypes.c_int),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("e_mp2", ctypes.c_double), ("e_ccsd", ctypes.c_double), ("e_ccsd_final", ctypes.c_double),
+        ("e_t", ctypes.c_double), ("rms", ctypes.c_double), ("niter", ctypes.c_int), ("niter_low", ctypes.c_int),
+        ("seconds_ccsd", ctypes.c_double), ("seconds_triples", ctypes.c_double), ("flops_ccsd", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def load_library(path=LIBPATH):
+    """Loads libugx.so and declares the ABI.  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError("libugx.so is not built (%s); run `python -m ugacc_amd.build`" % path)
+    L = ctypes.CDLL(path)
+    vp, ip, dp, cd = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_double
+    L.ugx_default_options.argtypes = [ctypes.POINTER(Options)]
+    L.ugx_default_options.restype = None
+    L.ugx_create.argtypes = [ctypes.POINTER(vp), ip]
+    L.ugx_destroy.argtypes = [vp]
+    L.ugx_destroy.restype = None
+    L.ugx_last_error.argtypes = [vp]
+    L.ugx_last_error.restype = ctypes.c_char_p
+    L.ugx_load_hamiltonian.argtypes = [vp, ip, ip] + [dp] * 7
+    L.ugx_set_amplitudes.argtypes = [vp, dp, dp]
+    L.ugx_ccsd.argtypes = [vp, ctypes.POINTER(Options), ctypes.POINTER(Result)]
+    L.ugx_triples.argtypes = [vp, ip, dp]
+    L.ugx_triples_per_triplet.argtypes = [vp, ip, dp]
+    L.ugx_w3.argtypes = [vp, ip, ip, ip, ip, dp]
+    L.ugx_get_amplitudes.argtypes = [vp, dp, dp]
+    L.ugx_energy.argtypes = [vp, ip, dp]
+    L.ugx_residual.argtypes = [vp, dp, dp, ip, dp, dp]
+    L.ugx_intermediates.argtypes = [vp] + [dp] * 7
+    L.ugx_set_profiling.argtypes = [vp, ip]
+    L.ugx_set_profiling.restype = None
+    L.ugx_kernel_stats.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+    L.ugx_kernel_stats.restype = ctypes.c_size_t
+    L.ugx_gemm.argtypes = [vp, ctypes.c_char, ctypes.c_char, ip, ip, ip, cd, vp, ctypes.c_long, vp, ctypes.c_long,
+                           cd, vp, ctypes.c_long, ip, ip]
+    L.ugx_synchronize.argtypes = [vp]
+    non_int = ("ugx_default_options", "ugx_destroy", "ugx_last_error", "ugx_set_profiling", "ugx_kernel_stats")
+    for name in EXPORTS:
+        if name not in non_int:
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def default_options():
+    L = load_library()
+    o = Options()
+    L.ugx_default_options(ctypes.byref(o))
+    return o
+
+
+def split_blocks(ints, no):
+    """Six Dirac blocks from a full nact^4 <pq|rs> array (occupied first)."""
+    o, v = slice(0, no), slice(no, ints.shape[0])
+    return dict(oooo=_c64(ints[o, o, o, o]), ooov=_c64(ints[o, o, o, v]), oovv=_c64(ints[o, o, v, v]),
+                ovov=_c64(ints[o, v, o, v]), ovvv=_c64(ints[o, v, v, v]), vvvv=_c64(ints[v, v, v, v]))
+
+
+class Context:
+    """One ugx_ctx: a HIP stream on one device with a resident Hamiltonian."""
+
+    def __init__(self, device=0):
+        self.L = load_library()
+        h = ctypes.c_void_p()
+        rc = self.L.ugx_create(ctypes.byref(h), device)
+        if rc != OK:
+            raise UgxError(rc, "ugx_create failed with status %d (no usable GPU?)" % rc)
+        self.h = h
+        self.no = self.nv = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ugx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, allow=(OK,)):
+        if rc not in allow:
+            raise UgxError(rc, self.L.ugx_last_error(self.h).decode())
+        return rc
+
+    def last_error(self):
+        return self.L.ugx_last_error(self.h).decode()
+
+    def load_hamiltonian(self, no, nv, fock, blocks):
+        self._keep = [_c64(fock)] + [blocks[k] for k in ("oooo", "ooov", "oovv", "ovov", "ovvv", "vvvv")]
+        self._check(self.L.ugx_load_hamiltonian(self.h, no, nv, *[_p(_c64(x)) for x in self._keep]))
+        self._keep = None
+        self.no, self.nv = no, nv
+
+    def set_amplitudes(self, t1, t2):
+        t1, t2 = _c64(t1), _c64(t2)
+        self._check(self.L.ugx_set_amplitudes(self.h, _p(t1), _p(t2)))
+
+    def amplitudes(self):
+        t1 = np.zeros((self.no, self.nv))
+        t2 = np.zeros((self.no, self.no, self.nv, self.nv))
+        self._check(self.L.ugx_get_amplitudes(self.h, _p(t1), _p(t2)))
+        return t1, t2
+
+    def ccsd(self, options=None, **kw):
+        o = options if options is not None else default_options()
+        for k, v in kw.items():
+            setattr(o, k, v)
+        r = Result()
+        rc = self._check(self.L.ugx_ccsd(self.h, ctypes.byref(o), ctypes.byref(r)), allow=(OK, NOT_CONVERGED))
+        d = r.as_dict()
+        d["converged"] = rc == OK
+        return d
+
+    def triples(self, precision=PREC_FP64):
+        e = ctypes.c_double()
+        self._check(self.L.ugx_triples(self.h, precision, ctypes.byref(e)))
+        return e.value
+
+    def triples_per_triplet(self, precision=PREC_FP64):
+        n = self.no * (self.no + 1) * (self.no + 2) // 6
+        e = np.zeros(n)
+        self._check(self.L.ugx_triples_per_triplet(self.h, precision, _p(e)))
+        return e
+
+    def w3(self, i, j, k, precision=PREC_FP64):
+        W = np.zeros((self.nv,) * 3)
+        self._check(self.L.ugx_w3(self.h, i, j, k, precision, _p(W)))
+        return W
+
+    def energy(self, form=ENERGY_REFERENCE):
+        e = ctypes.c_double()
+        self._check(self.L.ugx_energy(self.h, form, ctypes.byref(e)))
+        return e.value
+
+    def residual(self, t1, t2, precision=PREC_FP64):
+        t1, t2 = _c64(t1), _c64(t2)
+        R1 = np.zeros_like(t1)
+        R2 = np.zeros_like(t2)
+        self._check(self.L.ugx_residual(self.h, _p(t1), _p(t2), precision, _p(R1), _p(R2)))
+        return R1, R2
+
+    def intermediates(self):
+        o, v = self.no, self.nv
+        out = dict(Fvv=np.zeros((v, v)), Foo=np.zeros((o, o)), Fov=np.zeros((o, v)),
+                   Woooo=np.zeros((o, o, o, o)), Wovov=np.zeros((o, v, o, v)),
+                   Wovvo=np.zeros((o, v, v, o)), Wovoo=np.zeros((o, v, o, o)))
+        self._check(self.L.ugx_intermediates(self.h, *[_p(out[k]) for k in
+                                                       ("Fvv", "Foo", "Fov", "Woooo", "Wovov", "Wovvo", "Wovoo")]))
+        return out
+
+    def set_profiling(self, on=True):
+        self.L.ugx_set_profiling(self.h, 1 if on else 0)
+
+    def kernel_stats(self):
+        n = self.L.ugx_kernel_stats(self.h, None, 0)
+        buf = ctypes.create_string_buffer(max(n, 1))
+        self.L.ugx_kernel_stats(self.h, buf, n)
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, launches, ms, gflop = line.rsplit(" ", 3)
+            out[name] = dict(launches=int(launches), ms=float(ms), gflop=float(gflop))
+        return out
+
+    def gemm(self, opa, opb, M, N, K, alpha, A_ptr, lda, B_ptr, ldb, beta, C_ptr, ldc, precision=PREC_FP64,
+             tile=-1):
+        self._check(self.L.ugx_gemm(self.h, opa.encode(), opb.encode(), M, N, K, alpha, A_ptr, lda, B_ptr, ldb,
+                                    beta, C_ptr, ldc, precision, tile))
+
+    def synchronize(self):
+        self._check(self.L.ugx_synchronize(self.h))
