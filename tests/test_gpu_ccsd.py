@@ -74,7 +74,9 @@ def test_ccsd_and_triples_parity(case):
     per_g = c.triples_per_triplet()
     assert np.max(np.abs(per_g - per_o)) < 1e-13
     assert abs(c.triples() - et_o) < 1e-12
-    assert abs(et_o - orc.tcorr_ooc()) < 1e-12
+    if not case.get("offdiag") and not case.get("fov"):
+        # the three reference (T) algorithms agree for canonical orbitals
+        assert abs(et_o - orc.tcorr_ooc()) < 1e-12
     W_o = orc.w3(no - 1, no // 2, 0)
     W_g = c.w3(no - 1, no // 2, 0)
     assert np.max(np.abs(W_g - W_o)) < 1e-12 * max(1.0, np.max(np.abs(W_o)))

@@ -130,7 +130,10 @@ const void *Engine::prepared(const TRef &T, const std::string &letters, const st
   long n = extent_of(letters, ext);
   void *dst;
   if (T.constant) {
-    auto key = std::make_tuple(T.p, want, dtype);
+    // the key names the permutation (source letters -> wanted order), not
+    // just the wanted order: one stored block is read under several letter
+    // assignments (e.g. <mn|ef> as "mnef" and as "mnfe")
+    auto key = std::make_tuple(T.p, letters + "->" + want, dtype);
     auto it = cache_.find(key);
     if (it != cache_.end()) return it->second.first;
     hip_check(hipMalloc(&dst, n * esize(dtype)), "cache alloc");

@@ -26,8 +26,10 @@
 // MFMA forms:
 //   FP64: v_mfma_f64_16x16x4_f64   (A: lane l -> A[l%16][l/16],
 //                                     B: lane l -> B[l/16][l%16],
+//                                     D: lane l, item r -> D[l/16 + 4r][l%16],
+//                                     measured on gfx950)
+//   FP32: v_mfma_f32_16x16x4_f32   (A, B as FP64;
 //                                     D: lane l, item r -> D[4(l/16)+r][l%16])
-//   FP32: v_mfma_f32_16x16x4_f32   (same lane layout)
 //   BF16X3: v_mfma_f32_16x16x32_bf16 on hi/lo BF16 splits of FP32 operands
 //           (A: lane l -> A[l%16][8(l/16) .. 8(l/16)+7], B likewise along k)
 #include "ugx_internal.h"
@@ -52,6 +54,8 @@ struct Mfma<double> {
   __device__ static inline acc_t op(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // output row of accumulator item r held by lane group kq = lane / 16
+  __device__ static inline int row(int kq, int r) { return kq + 4 * r; }
 };
 template <>
 struct Mfma<float> {
@@ -60,6 +64,7 @@ struct Mfma<float> {
   __device__ static inline acc_t op(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int row(int kq, int r) { return 4 * kq + r; }
 };
 
 struct SegView {
@@ -256,7 +261,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
     for (int j = 0; j < TN; j++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int m = m0 + (wm * TM + i) * 16 + 4 * kq + r;
+        int m = m0 + (wm * TM + i) * 16 + MF::row(kq, r);
         int n = n0 + (wn * TN + j) * 16 + r16;
         if (m < M && n < N) {
           long addr = (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm + (long)n * d.scn;
