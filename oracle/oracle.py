@@ -54,7 +54,7 @@ def _lib(omp=False):
                                ctypes.c_int, dp, dp]
         L.orc_tjl_per_triplet.argtypes = [vp, dp]
         L.orc_w3.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
-        L.orc_iteration_sample.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.orc_iteration_sample.argtypes = [vp, ctypes.c_int]
         L.orc_num_threads.restype = ctypes.c_int
         _libs[omp] = L
     return _libs[omp]
@@ -176,5 +176,6 @@ class Oracle:
         self.L.orc_w3(self.h, i, j, k, _p(W))
         return W
 
-    def iteration_sample(self, irows, part):
-        self.L.orc_iteration_sample(self.h, irows, part)
+    def iteration_sample(self, rows):
+        """Runs rows/no of one CCSD iteration (for the bounded CPU baseline)."""
+        self.L.orc_iteration_sample(self.h, rows)

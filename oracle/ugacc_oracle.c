@@ -42,6 +42,9 @@ typedef struct {
   /* DIIS storage (ring of up to 8 vectors of length no*nv + no^2 nv^2) */
   int diis_nvec;
   double *diis_err, *diis_amp, *diis_errnorm;
+  /* upper bounds of the outermost occupied / virtual loops of build_F,
+   * build_W and build_t1: no / nv except inside orc_iteration_sample */
+  int lim_o, lim_v;
   int diis_count, diis_next;
 } orc_state;
 
@@ -82,6 +85,7 @@ void orc_build_tau(orc_state *s) {
 void *orc_create(int no, int nv, const double *fock, const double *ints) {
   orc_state *s = (orc_state *)calloc(1, sizeof(orc_state));
   s->no = no; s->nv = nv; s->nact = no + nv;
+  s->lim_o = no; s->lim_v = nv;
   size_t n = s->nact, n4 = n * n * n * n, ov = (size_t)no * nv, oovv = ov * ov;
   s->fock = zalloc(n * n); memcpy(s->fock, fock, n * n * sizeof(double));
   s->ints = zalloc(n4); memcpy(s->ints, ints, n4 * sizeof(double));
@@ -202,7 +206,7 @@ static void build_F(orc_state *s) {
   int no = s->no, nv = s->nv;
   const double *t1 = s->t1old, *ttau = s->ttau;
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int a = 0; a < nv; a++)
+  for (int a = 0; a < s->lim_v; a++)
     for (int e = 0; e < nv; e++) {
       double value = F(a + no, e + no);
       for (int m = 0; m < no; m++) {
@@ -215,7 +219,7 @@ static void build_F(orc_state *s) {
       s->Fvv[(size_t)a * nv + e] = value;
     }
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int m = 0; m < no; m++)
+  for (int m = 0; m < s->lim_o; m++)
     for (int i = 0; i < no; i++) {
       double value = F(m, i);
       for (int e = 0; e < nv; e++) {
@@ -228,7 +232,7 @@ static void build_F(orc_state *s) {
       s->Foo[(size_t)m * no + i] = value;
     }
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int m = 0; m < no; m++)
+  for (int m = 0; m < s->lim_o; m++)
     for (int e = 0; e < nv; e++) {
       double value = F(m, e + no);
       for (int n = 0; n < no; n++)
@@ -242,7 +246,7 @@ static void build_W(orc_state *s) {
   int no = s->no, nv = s->nv;
   const double *tau = s->tau, *t1 = s->t1old, *t2 = s->t2old;
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int m = 0; m < no; m++)
+  for (int m = 0; m < s->lim_o; m++)
     for (int n = 0; n < no; n++)
       for (int i = 0; i < no; i++)
         for (int j = 0; j < no; j++) {
@@ -254,7 +258,7 @@ static void build_W(orc_state *s) {
           s->Woooo[(((size_t)m * no + n) * no + i) * no + j] = value;
         }
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int m = 0; m < no; m++)
+  for (int m = 0; m < s->lim_o; m++)
     for (int b = 0; b < nv; b++)
       for (int j = 0; j < no; j++)
         for (int e = 0; e < nv; e++) {
@@ -267,7 +271,7 @@ static void build_W(orc_state *s) {
           s->Wovov[(((size_t)m * nv + b) * no + j) * nv + e] = value;
         }
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int m = 0; m < no; m++)
+  for (int m = 0; m < s->lim_o; m++)
     for (int b = 0; b < nv; b++)
       for (int e = 0; e < nv; e++)
         for (int j = 0; j < no; j++) {
@@ -282,7 +286,7 @@ static void build_W(orc_state *s) {
           s->Wovvo[(((size_t)m * nv + b) * nv + e) * no + j] = value;
         }
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int m = 0; m < no; m++)
+  for (int m = 0; m < s->lim_o; m++)
     for (int b = 0; b < nv; b++)
       for (int i = 0; i < no; i++)
         for (int j = 0; j < no; j++) {
@@ -300,7 +304,7 @@ static void build_t1(orc_state *s) {
   const double *t1 = s->t1old, *t2 = s->t2old;
   const double *Fae = s->Fvv, *Fmi = s->Foo, *Fme = s->Fov;
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int i = 0; i < no; i++)
+  for (int i = 0; i < s->lim_o; i++)
     for (int a = 0; a < nv; a++) {
       double value = F(a + no, i);
       for (int e = 0; e < nv; e++) value += t1[O2(i, e)] * Fae[(size_t)a * nv + e];
@@ -825,17 +829,24 @@ void orc_w3(void *p, int i, int j, int k, double *W3) {
       for (int c = 0; c < nv; c++) W3[V3(a, b, c)] = w3_elem(s, s->t2, i, j, k, a, b, c);
 }
 
-/* Bounded CPU sample of one CCSD iteration: the full build_F / build_W /
- * build_t1 and the t2 residual for rows i in [0, irows); returns nothing,
- * the caller times it and scales the t2 part by no/irows. */
-void orc_iteration_sample(void *p, int irows, int part) {
+/* Bounded CPU sample of one CCSD iteration: build_F, build_W, build_t1 and
+ * build_t2 with their outermost occupied loops cut to [0, rows) (the Fvv
+ * virtual loop to the same fraction), so the time is rows/no of a full
+ * iteration's.  The amplitudes are left as they were. */
+void orc_iteration_sample(void *p, int rows) {
   orc_state *s = (orc_state *)p;
-  if (part == 0) { amp_save(s); build_F(s); build_W(s); build_t1(s); amp_save(s); }
-  else {
-    amp_save(s);
-    build_t2_rows(s, 0, irows < s->no ? irows : s->no);
-    amp_save(s);
-  }
+  if (rows < 1) rows = 1;
+  if (rows > s->no) rows = s->no;
+  s->lim_o = rows;
+  s->lim_v = (int)(((long)s->nv * rows + s->no - 1) / s->no);
+  amp_save(s);
+  build_F(s);
+  build_W(s);
+  build_t1(s);
+  build_t2_rows(s, 0, rows);
+  amp_save(s);
+  s->lim_o = s->no;
+  s->lim_v = s->nv;
 }
 
 int orc_num_threads(void) {

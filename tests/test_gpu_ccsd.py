@@ -18,6 +18,7 @@ CASES = [
     dict(no=3, nv=7, seed=2, offdiag=0.02),
     dict(no=4, nv=9, seed=3, offdiag=0.01, fov=0.01),
     dict(no=5, nv=19, seed=4, scale=0.05),
+    dict(no=6, nv=22, seed=5, offdiag=0.01),  # long-K terms take the split-K path
 ]
 
 

@@ -97,8 +97,8 @@ struct ugx_ctx {
   std::vector<double> diis_norm;
   int diis_count = 0, diis_next = 0, diis_nvec = 0;
   // (T)
-  DBuf tP, tWbuf, ttrip, tptrs, tE, tPart;
-  DBuf f_tP, f_t2, f_ooov;  // FP32 operand copies for the mixed forms
+  DBuf tP, tQ, tWbuf, ttrip, tptrs, tE, tPart;
+  DBuf f_tP, f_tQ, f_t2, f_ooov;  // FP32 operand copies for the mixed forms
   bool tables_ready = false;
   std::vector<int> trip_host;
   // residual of the last ugx_residual, for ugx_intermediates
@@ -425,6 +425,9 @@ void triples_prepare(ugx_ctx *c, int prec) {
     // P[p][x][y][e] = <pe|xy> = ovvv[p][e][x][y]
     c->tP.alloc((size_t)o * v3 * 8);
     c->eng->permute("pexy->pxye", 1.0, cref(c->ovvv.d(), {o, v, v, v}), 0.0, vref(c->tP.d(), {o, v, v, v}));
+    // Q[p][y][x][e] = <pe|xy>: P with the row pair swapped
+    c->tQ.alloc((size_t)o * v3 * 8);
+    c->eng->permute("pexy->pyxe", 1.0, cref(c->ovvv.d(), {o, v, v, v}), 0.0, vref(c->tQ.d(), {o, v, v, v}));
     c->trip_host.clear();
     for (int i = 0; i < o; i++)
       for (int j = 0; j <= i; j++)
@@ -448,46 +451,81 @@ void triples_prepare(ugx_ctx *c, int prec) {
       launch_permute(pd, c->st);
     };
     conv(c->f_tP, c->tP.d(), (long)o * v3);
+    conv(c->f_tQ, c->tQ.d(), (long)o * v3);
     conv(c->f_ooov, c->ooov.d(), (long)o * o * o * v);
   }
 }
 
 // Builds W for triplets [t0, t0 + nb) of the table into c->tWbuf.
+//
+// Each permutation is one grouped GEMM over the batch.  Its output index
+// with unit stride in W (c) decides the orientation so that the 16 lanes of
+// an MFMA output row write consecutive addresses:
+//   z = c: rows (x,y) from P[p][x][y][e], columns z;
+//   y = c: rows z, columns (x,y) from P (transposed operands);
+//   x = c: rows z, columns (y,x) from Q[p][y][x][e] and t2[m][p][y][x]
+//          (t2_pm^xy = t2_mp^yx), so the fast column index is x = c.
 void triples_build_w(ugx_ctx *c, int prec, long t0, int nb, const std::vector<int> &trip) {
   const int o = c->no, v = c->nv;
   const long v2 = (long)v * v, v3 = v2 * v;
   const bool f32 = (prec != PREC_FP64);
   const size_t es = f32 ? 4 : 8;
   const char *P = (const char *)(f32 ? c->f_tP.p : c->tP.p);
+  const char *Q = (const char *)(f32 ? c->f_tQ.p : c->tQ.p);
   const char *T2 = (const char *)(f32 ? c->f_t2.p : c->t2.p);
   const char *OV = (const char *)(f32 ? c->f_ooov.p : c->ooov.p);
+  const long stride[3] = {v2, v, 1};  // of a, b, c in W[a][b][c]
   std::vector<GemmPtrs> tab((size_t)6 * nb);
-  for (int s = 0; s < 6; s++)
+  for (int s = 0; s < 6; s++) {
+    const Perm3 &pm = kPerms[s];
+    const int mode = (pm.z == 2) ? 0 : (pm.y == 2 ? 1 : 2);
     for (int b = 0; b < nb; b++) {
       const int *ijk = &trip[3 * (t0 + b)];
-      const Perm3 &pm = kPerms[s];
       int p = ijk[pm.p], q = ijk[pm.q], r = ijk[pm.r];
       GemmPtrs &g = tab[(size_t)s * nb + b];
-      g.A[0] = P + (size_t)p * v3 * es;                             // P[p][x][y][e]
-      g.B[0] = T2 + ((size_t)r * o + q) * v2 * es;                  // t2[r][q][z][e]
-      g.A[1] = T2 + (size_t)p * o * v2 * es;                        // t2[p][m][x][y]
-      g.B[1] = OV + ((size_t)q * o + r) * o * v * es;               // <qr|mz>
+      const char *Pp = (mode == 2 ? Q : P) + (size_t)p * v3 * es;  // P[p][x][y][e] or Q[p][y][x][e]
+      const char *T2rq = T2 + ((size_t)r * o + q) * v2 * es;       // t2[r][q][z][e]
+      const char *T2p = (mode == 2) ? T2 + (size_t)p * v2 * es     // t2[m][p][y][x]
+                                    : T2 + (size_t)p * o * v2 * es;  // t2[p][m][x][y]
+      const char *OVqr = OV + ((size_t)q * o + r) * o * v * es;     // <qr|mz>
+      if (mode == 0) { g.A[0] = Pp; g.B[0] = T2rq; g.A[1] = T2p; g.B[1] = OVqr; }
+      else { g.A[0] = T2rq; g.B[0] = Pp; g.A[1] = OVqr; g.B[1] = T2p; }
       g.C = c->tWbuf.d() + (size_t)b * v3;
     }
+  }
   c->tptrs.alloc(tab.size() * sizeof(GemmPtrs));
   check(hipMemcpyAsync(c->tptrs.p, tab.data(), tab.size() * sizeof(GemmPtrs), hipMemcpyHostToDevice, c->st),
         "pointer table copy");
-  const long stride[3] = {v2, v, 1};  // of a, b, c in W[a][b][c]
   for (int s = 0; s < 6; s++) {
     const Perm3 &pm = kPerms[s];
+    const int mode = (pm.z == 2) ? 0 : (pm.y == 2 ? 1 : 2);
     GemmDesc d{};
-    d.M = (int)v2; d.N = v; d.nseg = 2; d.batch = nb;
-    d.seg[0].sam = v; d.seg[0].sak = 1; d.seg[0].sbk = 1; d.seg[0].sbn = v; d.seg[0].K = v; d.seg[0].sign = 1.0;
-    d.seg[1].sam = 1; d.seg[1].sak = v2; d.seg[1].sbk = v; d.seg[1].sbn = 1; d.seg[1].K = o; d.seg[1].sign = -1.0;
-    d.cdiv = v; d.scm1 = stride[pm.x]; d.scm = stride[pm.y]; d.scn = stride[pm.z];
+    d.nseg = 2; d.batch = nb;
+    d.seg[0].K = v; d.seg[0].sign = 1.0;
+    d.seg[1].K = o; d.seg[1].sign = -1.0;
+    int tile;
+    if (mode == 0) {
+      d.M = (int)v2; d.N = v;
+      d.seg[0].sam = v; d.seg[0].sak = 1; d.seg[0].sbk = 1; d.seg[0].sbn = v;
+      d.seg[1].sam = 1; d.seg[1].sak = v2; d.seg[1].sbk = v; d.seg[1].sbn = 1;
+      d.cdiv = v; d.scm1 = stride[pm.x]; d.scm = stride[pm.y]; d.scn = stride[pm.z];
+      tile = v > 128 ? 5 : (v > 64 ? 8 : (v > 32 ? 1 : 2));
+    } else {
+      d.M = v; d.N = (int)v2;
+      d.seg[0].sam = v; d.seg[0].sak = 1; d.seg[0].sbk = 1; d.seg[0].sbn = v;
+      d.seg[1].sam = 1; d.seg[1].sak = v;
+      d.seg[1].sbk = (mode == 1) ? v2 : (long)o * v2; d.seg[1].sbn = 1;
+      d.scm = stride[pm.z];
+      d.ndiv = v;
+      if (mode == 1) { d.scn1 = stride[pm.x]; d.scn = stride[pm.y]; }
+      else { d.scn1 = stride[pm.y]; d.scn = stride[pm.x]; }
+      tile = v > 128 ? 9 : (v > 64 ? 10 : (v > 32 ? 1 : 2));
+    }
     d.alpha = 1.0; d.beta = (s == 0) ? 0.0 : 1.0;
-    int tile = v > 128 ? 5 : (v > 64 ? 8 : (v > 32 ? 1 : 2));
-    launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)s * nb, prec, tile, c->st);
+    const double fl = 2.0 * (double)nb * (double)v2 * v * (v + o);
+    c->eng->timed("triples_w", fl, [&]() {
+      launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)s * nb, prec, tile, c->st);
+    });
   }
 }
 
@@ -515,8 +553,10 @@ double triples(ugx_ctx *c, int prec, std::vector<double> *etrip_out) {
   for (long t0 = 0; t0 < nt; t0 += nb) {
     int b = (int)std::min<long>(nb, nt - t0);
     triples_build_w(c, prec, t0, b, c->trip_host);
-    launch_triples_energy(o, v, b, (const int *)c->ttrip.p + 3 * t0, c->tWbuf.d(), v3, c->eps.d(), c->oovv.d(),
-                          c->t1.d(), c->tE.d() + t0, c->tPart.d(), c->st);
+    c->eng->timed("triples_e", 0.0, [&]() {
+      launch_triples_energy(o, v, b, (const int *)c->ttrip.p + 3 * t0, c->tWbuf.d(), v3, c->eps.d(), c->oovv.d(),
+                            c->t1.d(), c->tE.d() + t0, c->tPart.d(), c->st);
+    });
   }
   std::vector<double> e(nt);
   if (nt) check(hipMemcpyAsync(e.data(), c->tE.p, nt * 8, hipMemcpyDeviceToHost, c->st), "triples copy");
@@ -626,7 +666,7 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
     c->tables_ready = false;
     c->have_amps = false;
     c->have_intermediates = false;
-    c->f_tP.release(); c->f_ooov.release(); c->f_t2.release();
+    c->f_tP.release(); c->f_tQ.release(); c->f_ooov.release(); c->f_t2.release();
     c->diis_err.clear(); c->diis_amp.clear(); c->diis_nvec = 0;
     c->no = no; c->nv = nv;
     const long n = no + nv, o = no, v = nv;
@@ -765,6 +805,7 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
       auto t2c = std::chrono::steady_clock::now();
       r.e_t = triples(c, opt.precision, nullptr);
       r.seconds_triples = std::chrono::duration<double>(std::chrono::steady_clock::now() - t2c).count();
+      E.collect_stats();
     }
     if (res) *res = r;
     c->have_amps = true;
@@ -908,7 +949,7 @@ int ugx_gemm(ugx_ctx *c, char opa, char opb, int M, int N, int K, double alpha, 
     if (M < 0 || N < 0 || K < 0 || !A || !B || !C || precision < 0 || precision > 2 || tile >= gemm_num_tiles())
       throw UgxError(UGX_BAD_INPUT, "ugx_gemm: bad arguments");
     GemmDesc d{};
-    d.M = M; d.N = N; d.nseg = 1; d.batch = 1; d.cdiv = 0x7fffffff;
+    d.M = M; d.N = N; d.nseg = 1; d.batch = 1;
     GemmSeg &g = d.seg[0];
     g.A = A; g.B = B; g.K = K; g.sign = 1.0;
     if (opa == 'N' || opa == 'n') { g.sam = lda; g.sak = 1; } else { g.sam = 1; g.sak = lda; }

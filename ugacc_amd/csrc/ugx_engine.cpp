@@ -170,6 +170,17 @@ void Engine::permute(const std::string &s, double alpha, const TRef &A, double b
   launch_permute(pd, st_);
 }
 
+void Engine::timed(const char *label, double flops, const std::function<void()> &fn) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (profiling_) { e0 = event(); e1 = event(); hip_check(hipEventRecord(e0, st_), "event record"); }
+  fn();
+  flops_ += flops;
+  if (profiling_) {
+    hip_check(hipEventRecord(e1, st_), "event record");
+    pending_.push_back(Pending{label, e0, e1, flops});
+  }
+}
+
 struct Engine::Plan {
   double cost;
   bool swap;          // GEMM computes with (B, A) in the (A, B) roles
@@ -271,13 +282,54 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
     g.A = Bp; g.sam = sbn; g.sak = sbk; g.B = Ap; g.sbk = sak; g.sbn = sam;
   }
   g.K = (int)Ke; g.sign = 1.0;
-  d.M = (int)Re; d.N = (int)Ce; d.nseg = 1; d.seg[0] = g; d.batch = 1; d.cdiv = 0x7fffffff;
+  d.M = (int)Re; d.N = (int)Ce; d.nseg = 1; d.seg[0] = g; d.batch = 1;
   if (Re > 0x7fffffffL || Ce > 0x7fffffffL || Ke > 0x7fffffffL) throw std::runtime_error("ugx: GEMM too large");
 
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (profiling_ && label) { e0 = event(); e1 = event(); hip_check(hipEventRecord(e0, st_), "event record"); }
 
   const bool direct = (sp.c == rows + cols);
+  // Split-K: a small output with a long contraction (e.g. Foo = ttau.L over
+  // K = o v^2) would occupy only a few workgroups; cut K into S chunks that
+  // run as a batch into a workspace, then reduce.
+  {
+    int tile = gemm_pick_tile(d), bm, bn;
+    gemm_tile_dims(tile, bm, bn);
+    long tiles = ((Re + bm - 1) / bm) * ((Ce + bn - 1) / bn);
+    if (tiles < 256 && Ke >= 2048) {
+      long S = std::min<long>((512 + tiles - 1) / tiles, Ke / 512);
+      S = std::min<long>(S, 4096);
+      if (S > 1) {
+        long chunk = (Ke + S - 1) / S;
+        S = (Ke + chunk - 1) / chunk;
+        double *W = (double *)slot(3, (size_t)S * Re * Ce * 8);
+        GemmDesc d2 = d;
+        d2.batch = (int)S;
+        d2.seg[0].K = (int)chunk;
+        d2.seg[0].sAz = chunk * d.seg[0].sak;
+        d2.seg[0].sBz = chunk * d.seg[0].sbk;
+        d2.ktotal = Ke;
+        d2.C = W; d2.scm = Ce; d2.scn = 1; d2.scm1 = 0; d2.sCz = Re * Ce; d2.alpha = 1.0; d2.beta = 0.0;
+        launch_gemm(d2, nullptr, prec, tile, st_);
+        if (direct) {
+          launch_splitk_reduce((int)S, (int)Re, (int)Ce, W, alpha, beta, (double *)C.p, Ce, 1, st_);
+        } else {
+          double *tmp = (double *)slot(2, (size_t)Re * Ce * 8);
+          launch_splitk_reduce((int)S, (int)Re, (int)Ce, W, 1.0, 0.0, tmp, Ce, 1, st_);
+          TRef T = vref(tmp, {});
+          for (char ch : rows + cols) T.dims.push_back(ext.at(ch));
+          permute(rows + cols + "->" + sp.c, alpha, T, beta, C);
+        }
+        const double fl = 2.0 * (double)Me * (double)Ne * (double)Ke;
+        flops_ += fl;
+        if (profiling_ && label) {
+          hip_check(hipEventRecord(e1, st_), "event record");
+          pending_.push_back(Pending{label, e0, e1, fl});
+        }
+        return;
+      }
+    }
+  }
   if (direct) {
     d.C = (double *)C.p; d.scm = Ce; d.scn = 1; d.scm1 = 0; d.alpha = alpha; d.beta = beta;
     launch_gemm(d, nullptr, prec, -1, st_);

@@ -13,6 +13,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <map>
 #include <string>
 #include <tuple>
@@ -51,6 +52,10 @@ class Engine {
   // C = alpha * permute(A) + beta * C, spec "mbje->mejb".
   void permute(const std::string &spec, double alpha, const TRef &A, double beta, const TRef &C);
 
+  // Runs fn (which enqueues device work on the stream) between a pair of
+  // events recorded under label when profiling is on; flops are credited to
+  // the label either way.
+  void timed(const char *label, double flops, const std::function<void()> &fn);
   // Per-label device time (HIP events around each contraction) when on.
   void set_profiling(bool on) { profiling_ = on; }
   void collect_stats();  // waits for the stream, folds event pairs into stats_

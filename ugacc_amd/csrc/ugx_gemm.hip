@@ -88,60 +88,58 @@ __device__ inline SegView seg_view(const GemmDesc &d, const GemmPtrs *ptrs, int 
   return v;
 }
 
-// Loads one BK-slice of A (BM rows) into registers, scaled by the segment
-// sign.  Follows the unit stride: k-contiguous (sak == 1) reads BK
-// consecutive k per row; otherwise m-contiguous.
-template <typename T, int BM>
-__device__ inline void load_a(T (&r)[BM * BK / NT], const T *A, const SegView &S, int m0, int k0, int M, int tid) {
-  const bool kc = (S.sak == 1);
-#pragma unroll
-  for (int q = 0; q < BM * BK / NT; q++) {
-    int e = tid + q * NT;
-    int row, kk;
-    if (kc) { row = e / BK; kk = e % BK; } else { kk = e / BM; row = e % BM; }
-    int m = m0 + row, k = k0 + kk;
-    T v = T(0);
-    if (m < M && k < S.K) v = A[(long)m * S.sam + (long)k * S.sak];
-    r[q] = v * T(S.sign);
-  }
-}
+// Per-thread loader of one operand tile (R rows x BK k-values per slice).
+// The unit-stride side decides the thread mapping: k-contiguous operands are
+// read as BK consecutive k per row (16 lanes cover one 128-byte row
+// segment), row-contiguous ones as R consecutive rows per k.  The pointer
+// of the first element is set up once per segment and advanced by BK k per
+// slice, so the inner loop is an add per element.
+template <typename T, int R>
+struct OpLoader {
+  static constexpr int N_ = R * BK / NT;  // elements per thread per slice
+  const T *ptr;                           // element of q = 0 at the current slice
+  long qoff;                              // pointer step between q
+  long kadv;                              // pointer step per slice
+  int kk0, kkq;                           // k offset within the slice: kk0 + q*kkq
+  unsigned rowmask;                       // bit q: row of q is inside the operand
+  bool kc;
+  int K;
 
-template <typename T, int BM, int LD>
-__device__ inline void store_a(const T (&r)[BM * BK / NT], T (*As)[LD], bool kc, int tid) {
+  __device__ inline void setup(const T *base, long srow, long sk, int row0, int rows, int K_, int tid) {
+    K = K_;
+    kc = (sk == 1);
+    int r, rq;
+    if (kc) { r = tid / BK; kk0 = tid % BK; rq = NT / BK; kkq = 0; }
+    else { r = tid % R; kk0 = tid / R; rq = 0; kkq = NT / R; }
+    ptr = base + (long)(row0 + r) * srow + (long)kk0 * sk;
+    qoff = (long)rq * srow + (long)kkq * sk;
+    kadv = (long)BK * sk;
+    rowmask = 0;
 #pragma unroll
-  for (int q = 0; q < BM * BK / NT; q++) {
-    int e = tid + q * NT;
-    int row, kk;
-    if (kc) { row = e / BK; kk = e % BK; } else { kk = e / BM; row = e % BM; }
-    As[kk][row] = r[q];
+    for (int q = 0; q < N_; q++)
+      if (row0 + r + q * rq < rows) rowmask |= 1u << q;
   }
-}
-
-template <typename T, int BN>
-__device__ inline void load_b(T (&r)[BN * BK / NT], const T *B, const SegView &S, int n0, int k0, int N, int tid) {
-  const bool kc = (S.sbk == 1);
+  __device__ inline void load(T (&v)[N_], int k0, T scale) const {
 #pragma unroll
-  for (int q = 0; q < BN * BK / NT; q++) {
-    int e = tid + q * NT;
-    int col, kk;
-    if (kc) { col = e / BK; kk = e % BK; } else { kk = e / BN; col = e % BN; }
-    int n = n0 + col, k = k0 + kk;
-    T v = T(0);
-    if (n < N && k < S.K) v = B[(long)k * S.sbk + (long)n * S.sbn];
-    r[q] = v;
+    for (int q = 0; q < N_; q++) {
+      int k = k0 + kk0 + q * kkq;
+      T x = T(0);
+      if (((rowmask >> q) & 1u) && k < K) x = ptr[q * qoff];
+      v[q] = x * scale;
+    }
   }
-}
-
-template <typename T, int BN, int LD>
-__device__ inline void store_b(const T (&r)[BN * BK / NT], T (*Bs)[LD], bool kc, int tid) {
+  __device__ inline void advance() { ptr += kadv; }
+  template <int LD>
+  __device__ inline void store(const T (&v)[N_], T (*S)[LD], int tid) const {
 #pragma unroll
-  for (int q = 0; q < BN * BK / NT; q++) {
-    int e = tid + q * NT;
-    int col, kk;
-    if (kc) { col = e / BK; kk = e % BK; } else { kk = e / BN; col = e % BN; }
-    Bs[kk][col] = r[q];
+    for (int q = 0; q < N_; q++) {
+      int r, kk;
+      if (kc) { r = tid / BK + q * (NT / BK); kk = tid % BK; }
+      else { r = tid % R; kk = tid / R + q * (NT / R); }
+      S[kk][r] = v[q];
+    }
   }
-}
+};
 
 // Maps the linear block id to an output tile (tm, tn).  Block ids are dealt
 // round-robin to the 8 XCDs; the remap gives XCD x the contiguous id range
@@ -164,16 +162,20 @@ __device__ inline void tile_of(int bid, int nblk, int tiles_m, int tiles_n, int 
   tn = in / rows;
 }
 
+// FP64 / FP32 main kernel.  Two LDS stages: while the MFMAs consume slice t
+// from one stage, slice t+1 (loaded into registers one iteration earlier) is
+// written to the other stage and the global loads of slice t+2 are issued,
+// so each global load has a full slice of MFMA work to land and there is one
+// barrier per slice.
 template <typename T, int WGM, int WGN, int TM, int TN>
 __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restrict__ ptrs) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
   constexpr int LDA = BM + (sizeof(T) == 8 ? 1 : 2), LDB = BN + (sizeof(T) == 8 ? 1 : 2);
-  constexpr int RA = BM * BK / NT, RB = BN * BK / NT;
-  static_assert(RA >= 1 && RB >= 1, "tile too small for 256 threads");
-  __shared__ T As[BK][LDA];
-  __shared__ T Bs[BK][LDB];
+  static_assert(BM * BK / NT >= 1 && BN * BK / NT >= 1, "tile too small for 256 threads");
+  __shared__ T As[2][BK][LDA];
+  __shared__ T Bs[2][BK][LDB];
 
   const int M = d.M, N = d.N;
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
@@ -182,27 +184,38 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
   if (tm >= tiles_m || tn >= tiles_n) return;
   const int m0 = tm * BM, n0 = tn * BN;
   const long z = blockIdx.y;
+  const int tid = threadIdx.x;
 
-  SegView S[2];
-  const T *Ap[2];
-  const T *Bp[2];
-  int nt[2] = {0, 0};
-  for (int s = 0; s < 2; s++) {
-    if (s < d.nseg) {
-      S[s] = seg_view(d, ptrs, s, z);
-      if (ptrs) { Ap[s] = (const T *)S[s].A; Bp[s] = (const T *)S[s].B; }
-      else {
-        Ap[s] = (const T *)d.seg[s].A + z * d.seg[s].sAz;
-        Bp[s] = (const T *)d.seg[s].B + z * d.seg[s].sBz;
-      }
-      nt[s] = (S[s].K + BK - 1) / BK;
-    } else {
-      S[s] = S[0]; Ap[s] = nullptr; Bp[s] = nullptr; nt[s] = 0;
+  // segment operands for this batch entry
+  const T *Ab[2] = {nullptr, nullptr};
+  const T *Bb[2] = {nullptr, nullptr};
+  int Ks[2] = {0, 0}, nt[2] = {0, 0};
+  T sg[2] = {T(1), T(1)};
+  for (int s = 0; s < d.nseg; s++) {
+    const GemmSeg &g = d.seg[s];
+    if (ptrs) { Ab[s] = (const T *)ptrs[z].A[s]; Bb[s] = (const T *)ptrs[z].B[s]; }
+    else { Ab[s] = (const T *)g.A + z * g.sAz; Bb[s] = (const T *)g.B + z * g.sBz; }
+    int K = g.K;
+    if (s == 0 && d.ktotal > 0) {
+      long rem = d.ktotal - z * (long)g.K;
+      if (rem < K) K = rem > 0 ? (int)rem : 0;
     }
+    Ks[s] = K;
+    nt[s] = (K + BK - 1) / BK;
+    sg[s] = T(g.sign);
   }
   const int ntiles = nt[0] + nt[1];
 
-  const int tid = threadIdx.x;
+  OpLoader<T, BM> la;
+  OpLoader<T, BN> lb;
+  int cur_seg = -1;
+  auto seg_setup = [&](int s) {
+    const GemmSeg &g = d.seg[s];
+    la.setup(Ab[s], g.sam, g.sak, m0, M, Ks[s], tid);
+    lb.setup(Bb[s], g.sbn, g.sbk, n0, N, Ks[s], tid);
+    cur_seg = s;
+  };
+
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WGN, wn = wave % WGN;
   const int r16 = lane & 15, kq = lane >> 4;
@@ -213,48 +226,53 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
 #pragma unroll
     for (int j = 0; j < TN; j++) acc[i][j] = MF::zero();
 
-  T ra[RA], rb[RB];
+  T ra[OpLoader<T, BM>::N_], rb[OpLoader<T, BN>::N_];
+  // slice t -> (segment, k0); the loaders walk the slices in order
   auto fetch = [&](int t) {
     int s = (t < nt[0]) ? 0 : 1;
     int k0 = (s == 0 ? t : t - nt[0]) * BK;
-    load_a<T, BM>(ra, Ap[s], S[s], m0, k0, M, tid);
-    load_b<T, BN>(rb, Bp[s], S[s], n0, k0, N, tid);
-    return s;
+    if (s != cur_seg) seg_setup(s);
+    la.load(ra, k0, sg[s]);
+    lb.load(rb, k0, T(1));
+    la.advance();
+    lb.advance();
   };
+
   if (ntiles > 0) {
-    int s = fetch(0);
-    store_a<T, BM, LDA>(ra, As, S[s].sak == 1, tid);
-    store_b<T, BN, LDB>(rb, Bs, S[s].sbk == 1, tid);
+    fetch(0);
+    la.template store<LDA>(ra, As[0], tid);
+    lb.template store<LDB>(rb, Bs[0], tid);
+    if (ntiles > 1) fetch(1);
   }
   __syncthreads();
 
   for (int t = 0; t < ntiles; t++) {
-    const bool more = (t + 1 < ntiles);
-    int sn = 0;
-    if (more) sn = fetch(t + 1);
+    const int st = t & 1;
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T a[TM], b[TN];
 #pragma unroll
-      for (int i = 0; i < TM; i++) a[i] = As[kk + kq][(wm * TM + i) * 16 + r16];
+      for (int i = 0; i < TM; i++) a[i] = As[st][kk + kq][(wm * TM + i) * 16 + r16];
 #pragma unroll
-      for (int j = 0; j < TN; j++) b[j] = Bs[kk + kq][(wn * TN + j) * 16 + r16];
+      for (int j = 0; j < TN; j++) b[j] = Bs[st][kk + kq][(wn * TN + j) * 16 + r16];
 #pragma unroll
       for (int i = 0; i < TM; i++)
 #pragma unroll
         for (int j = 0; j < TN; j++) acc[i][j] = MF::op(a[i], b[j], acc[i][j]);
     }
-    __syncthreads();
-    if (more) {
-      store_a<T, BM, LDA>(ra, As, S[sn].sak == 1, tid);
-      store_b<T, BN, LDB>(rb, Bs, S[sn].sbk == 1, tid);
+    if (t + 1 < ntiles) {
+      // the other stage was last read in iteration t-1, which every wave
+      // finished before the barrier that ended it
+      la.template store<LDA>(ra, As[st ^ 1], tid);
+      lb.template store<LDB>(rb, Bs[st ^ 1], tid);
+      if (t + 2 < ntiles) fetch(t + 2);
     }
     __syncthreads();
   }
 
   double *C = ptrs ? ptrs[z].C : d.C + z * d.sCz;
   const double alpha = d.alpha, beta = d.beta;
-  const int cdiv = d.cdiv;
+  const int cdiv = d.cdiv, ndiv = d.ndiv;
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
@@ -264,7 +282,8 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
         int m = m0 + (wm * TM + i) * 16 + MF::row(kq, r);
         int n = n0 + (wn * TN + j) * 16 + r16;
         if (m < M && n < N) {
-          long addr = (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm + (long)n * d.scn;
+          long addr = (cdiv > 0 ? (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm : (long)m * d.scm) +
+                      (ndiv > 0 ? (long)(n / ndiv) * d.scn1 + (long)(n % ndiv) * d.scn : (long)n * d.scn);
           double v = alpha * (double)acc[i][j][r];
           if (beta != 0.0) v += beta * C[addr];
           C[addr] = v;
@@ -368,6 +387,10 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(GemmDesc d, const GemmPtrs *
         Ap[s] = (const float *)d.seg[s].A + z * d.seg[s].sAz;
         Bp[s] = (const float *)d.seg[s].B + z * d.seg[s].sBz;
       }
+      if (s == 0 && d.ktotal > 0) {
+        long rem = d.ktotal - z * (long)S[0].K;
+        if (rem < S[0].K) S[0].K = rem > 0 ? (int)rem : 0;
+      }
       nt[s] = (S[s].K + BK3 - 1) / BK3;
     } else {
       S[s] = S[0]; Ap[s] = nullptr; Bp[s] = nullptr; nt[s] = 0;
@@ -435,7 +458,7 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(GemmDesc d, const GemmPtrs *
 
   double *C = ptrs ? ptrs[z].C : d.C + z * d.sCz;
   const double alpha = d.alpha, beta = d.beta;
-  const int cdiv = d.cdiv;
+  const int cdiv = d.cdiv, ndiv = d.ndiv;
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
@@ -445,7 +468,8 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(GemmDesc d, const GemmPtrs *
         int m = m0 + (wm * TM + i) * 16 + 4 * kq + r;
         int n = n0 + (wn * TN + j) * 16 + r16;
         if (m < M && n < N) {
-          long addr = (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm + (long)n * d.scn;
+          long addr = (cdiv > 0 ? (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm : (long)m * d.scm) +
+                      (ndiv > 0 ? (long)(n / ndiv) * d.scn1 + (long)(n % ndiv) * d.scn : (long)n * d.scn);
           double v = alpha * (double)acc[i][j][r];
           if (beta != 0.0) v += beta * C[addr];
           C[addr] = v;
@@ -462,7 +486,7 @@ struct TileShape {
 const TileShape kTiles[] = {
     {2, 2, 4, 4, "128x128"}, {2, 2, 2, 2, "64x64"},  {2, 2, 1, 1, "32x32"},  {4, 1, 2, 4, "128x64"},
     {1, 4, 4, 2, "64x128"},  {2, 2, 4, 5, "128x160"}, {4, 1, 1, 4, "64x64w"}, {2, 2, 2, 5, "64x160"},
-    {4, 1, 2, 5, "128x80"},
+    {4, 1, 2, 5, "128x80"},  {2, 2, 5, 4, "160x128"}, {1, 4, 5, 2, "80x128"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
@@ -492,6 +516,8 @@ void launch_tile(int tile, const GemmDesc &d, const GemmPtrs *ptrs, int prec, hi
     case 6: launch_t<T, 4, 1, 1, 4>(d, ptrs, prec, st); break;
     case 7: launch_t<T, 2, 2, 2, 5>(d, ptrs, prec, st); break;
     case 8: launch_t<T, 4, 1, 2, 5>(d, ptrs, prec, st); break;
+    case 9: launch_t<T, 2, 2, 5, 4>(d, ptrs, prec, st); break;
+    case 10: launch_t<T, 1, 4, 5, 2>(d, ptrs, prec, st); break;
     default: throw std::runtime_error("ugx: bad GEMM tile id");
   }
 }
@@ -499,7 +525,7 @@ void launch_tile(int tile, const GemmDesc &d, const GemmPtrs *ptrs, int prec, hi
 // Default tile choice: the largest tile whose grid still puts at least two
 // workgroups on each of the 256 CUs; skinny shapes get the matching
 // rectangular tiles.
-int pick_tile(const GemmDesc &d) {
+int pick_tile_impl(const GemmDesc &d) {
   auto tiles = [&](int bm, int bn) { return (long)((d.M + bm - 1) / bm) * ((d.N + bn - 1) / bn) * d.batch; };
   const long want = 512;
   if (d.M >= 128 && d.N >= 128 && tiles(128, 128) >= want) return 0;
@@ -513,13 +539,18 @@ int pick_tile(const GemmDesc &d) {
 }  // namespace
 
 int gemm_num_tiles() { return kNumTiles; }
+int gemm_pick_tile(const GemmDesc &d) { return pick_tile_impl(d); }
+void gemm_tile_dims(int tile, int &bm, int &bn) {
+  const TileShape &t = kTiles[tile];
+  bm = 16 * t.wgm * t.tm;
+  bn = 16 * t.wgn * t.tn;
+}
 const char *gemm_tile_name(int tile) { return (tile >= 0 && tile < kNumTiles) ? kTiles[tile].name : "?"; }
 
 void launch_gemm(const GemmDesc &d, const GemmPtrs *ptrs, int prec, int tile, hipStream_t st) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
   if (d.nseg < 1 || d.nseg > 2) throw std::runtime_error("ugx: GEMM needs one or two K segments");
-  if (d.cdiv <= 0) throw std::runtime_error("ugx: GEMM row split must be positive");
-  if (tile < 0) tile = pick_tile(d);
+  if (tile < 0) tile = pick_tile_impl(d);
   if (prec == PREC_FP64) launch_tile<double>(tile, d, ptrs, prec, st);
   else if (prec == PREC_FP32 || prec == PREC_BF16X3) launch_tile<float>(tile, d, ptrs, prec, st);
   else throw std::runtime_error("ugx: unknown contraction precision");

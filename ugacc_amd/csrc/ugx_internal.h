@@ -29,8 +29,11 @@ struct GemmSeg {
   double sign;
 };
 
-// C(m,n) lives at C[(m / cdiv) * scm1 + (m % cdiv) * scm + n * scn]; with
-// cdiv >= M this is the plain C[m*scm + n*scn].  C is always double: the
+// C(m,n) lives at C[(m / cdiv) * scm1 + (m % cdiv) * scm
+//                   + (n / ndiv) * scn1 + (n % ndiv) * scn];
+// cdiv / ndiv <= 0 (or >= M / N) mean no split: C[m*scm + n*scn].  The
+// splits let a (T) permutation write W[a][b][c] in any index order.  C is
+// always double: the
 // FP32/BF16 forms widen their FP32 accumulators in the epilogue, so every
 // result tensor stays FP64.
 struct GemmDesc {
@@ -38,10 +41,13 @@ struct GemmDesc {
   int nseg;
   GemmSeg seg[2];
   double *C;
-  long scm, scn, scm1, sCz;
-  int cdiv;
+  long scm, scn, scm1, scn1, sCz;
+  int cdiv, ndiv;
   double alpha, beta;
   int batch;
+  // split-K: when > 0, batch entry z covers k in [z*K, min((z+1)*K, ktotal))
+  // of segment 0 (its sAz/sBz advance by K*sak / K*sbk)
+  long ktotal;
 };
 
 // Per-batch pointers for grouped launches (used by the (T) build where each
@@ -58,6 +64,11 @@ struct GemmPtrs {
 void launch_gemm(const GemmDesc &d, const GemmPtrs *ptrs, int prec, int tile, hipStream_t st);
 int gemm_num_tiles();
 const char *gemm_tile_name(int tile);
+int gemm_pick_tile(const GemmDesc &d);
+void gemm_tile_dims(int tile, int &bm, int &bn);
+// C[m*scm + n*scn] = alpha * sum_z W[z][m][n] + beta * C  (split-K reduce)
+void launch_splitk_reduce(int S, int M, int N, const double *W, double alpha, double beta, double *C, long scm,
+                          long scn, hipStream_t st);
 
 // ------------------------------------------------------------ streaming
 // out[idx] = alpha * in[perm(idx)] + beta * out[idx] over a row-major output

@@ -229,6 +229,20 @@ __global__ void k_guess(int no, int nv, const double *fock, const double *oovv, 
   }
 }
 
+__global__ void k_splitk_reduce(int S, int M, int N, const double *W, double alpha, double beta, double *C,
+                                long scm, long scn) {
+  long mn = (long)M * N;
+  for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < mn; idx += (long)gridDim.x * NT) {
+    double s = 0.0;
+    for (int z = 0; z < S; z++) s += W[(long)z * mn + idx];
+    long m = idx / N, n = idx % N;
+    long o = m * scm + n * scn;
+    double v = alpha * s;
+    if (beta != 0.0) v += beta * C[o];
+    C[o] = v;
+  }
+}
+
 __global__ void k_dot(long n, const double *x, const double *y, double *partials) {
   double s = 0.0;
   for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) s += x[i] * y[i];
@@ -357,6 +371,12 @@ void launch_guess(int no, int nv, const double *fock, const double *oovv, double
                   hipStream_t st) {
   long n = (long)no * nv * (1 + (long)no * nv);
   hipLaunchKernelGGL(k_guess, dim3(nblocks(n)), dim3(NT), 0, st, no, nv, fock, oovv, t1, t2);
+}
+
+void launch_splitk_reduce(int S, int M, int N, const double *W, double alpha, double beta, double *C, long scm,
+                          long scn, hipStream_t st) {
+  long n = (long)M * N;
+  hipLaunchKernelGGL(k_splitk_reduce, dim3(nblocks(n)), dim3(NT), 0, st, S, M, N, W, alpha, beta, C, scm, scn);
 }
 
 void launch_dot(long n, const double *x, const double *y, double *partials, int npart, hipStream_t st) {

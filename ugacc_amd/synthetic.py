@@ -94,3 +94,34 @@ def rotate(fock, ints, no, seed=0, angle=0.3):
     f = U.T @ fock @ U
     g = np.einsum("pqrs,pa,qb,rc,sd->abcd", ints, U, U, U, U, optimize=True)
     return f, g
+
+
+def random_blocks(no, nv, seed=0, scale=0.08, rank=None, gap=0.5, dtype=np.float64):
+    """(fock, blocks) for large sizes without forming the full nact^4 array.
+
+    Same construction as random_hamiltonian -- chemist integrals
+    (pr|qs) = sum_K B[p,r,K] B[q,s,K] with B symmetric in its orbital pair --
+    but each of the six Dirac blocks <pq|rs> = (pr|qs) is formed directly
+    from the factor blocks, so memory stays at the size of the blocks."""
+    rng = np.random.default_rng(seed)
+    n = no + nv
+    rank = rank or max(4, n)
+    B = rng.standard_normal((n, n, rank)) * scale / np.sqrt(rank) ** 0.5
+    B = 0.5 * (B + B.transpose(1, 0, 2))
+    o, v = slice(0, no), slice(no, n)
+
+    def block(P, Q, R, S):
+        # <pq|rs> = sum_K B[p,r,K] B[q,s,K] -> [p][q][r][s]
+        X = B[P, R]  # (p, r, K)
+        Y = B[Q, S]  # (q, s, K)
+        p_, r_ = X.shape[:2]
+        q_, s_ = Y.shape[:2]
+        G = X.reshape(p_ * r_, rank) @ Y.reshape(q_ * s_, rank).T  # [(p r), (q s)]
+        return np.ascontiguousarray(G.reshape(p_, r_, q_, s_).transpose(0, 2, 1, 3))
+
+    blocks = dict(oooo=block(o, o, o, o), ooov=block(o, o, o, v), oovv=block(o, o, v, v),
+                  ovov=block(o, v, o, v), ovvv=block(o, v, v, v), vvvv=block(v, v, v, v))
+    eo = -np.sort(rng.uniform(0.4, 2.0, no))[::-1]
+    ev = np.sort(rng.uniform(gap, gap + 2.5, nv))
+    fock = np.diag(np.concatenate([eo, ev]))
+    return np.ascontiguousarray(fock), blocks
