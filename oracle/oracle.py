@@ -56,6 +56,7 @@ def _lib(omp=False):
         L.orc_w3.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.orc_iteration_sample.argtypes = [vp, ctypes.c_int]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_residual_rows.argtypes = [vp, ctypes.c_int, dp, dp]
         _libs[omp] = L
     return _libs[omp]
 
@@ -175,6 +176,13 @@ class Oracle:
         W = np.zeros((self.nv,) * 3)
         self.L.orc_w3(self.h, i, j, k, _p(W))
         return W
+
+    def residual_rows(self, rows):
+        """(R1, R2[:rows]) of the current amplitudes."""
+        R1 = np.zeros((self.no, self.nv))
+        R2 = np.zeros((rows, self.no, self.nv, self.nv))
+        self.L.orc_residual_rows(self.h, rows, _p(R1), _p(R2))
+        return R1, R2
 
     def iteration_sample(self, rows):
         """Runs rows/no of one CCSD iteration (for the bounded CPU baseline)."""

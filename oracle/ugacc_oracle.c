@@ -849,6 +849,26 @@ void orc_iteration_sample(void *p, int rows) {
   s->lim_v = s->nv;
 }
 
+/* Residuals of the current amplitudes with the t2 residual restricted to
+ * rows i in [0, rows): R1 (no*nv, full) and R2 rows (rows*no*nv*nv).  The
+ * amplitudes are left as they were.  For parity checks at sizes where a full
+ * oracle iteration is too slow. */
+void orc_residual_rows(void *p, int rows, double *R1, double *R2) {
+  orc_state *s = (orc_state *)p;
+  int no = s->no, nv = s->nv;
+  if (rows < 1) rows = 1;
+  if (rows > no) rows = no;
+  amp_save(s);
+  build_F(s);
+  build_W(s);
+  build_t1(s);
+  build_t2_rows(s, 0, rows);
+  memcpy(R1, s->t1, (size_t)no * nv * sizeof(double));
+  memcpy(R2, s->t2, (size_t)rows * no * nv * nv * sizeof(double));
+  amp_save(s);
+  orc_build_tau(s);
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -1,0 +1,100 @@
+"""The front end and the FP64 oracle against the reference's own saved
+outputs (tests/golden/known_answers.json, extracted from
+psi3/output.c1_fzc and psi4/hof/output.{c1_nofzc,cs_fzc}).
+
+The psi3 run is the PSI3 version of ugacc itself (H2O/cc-pVDZ at 0.9 A,
+1 frozen core): it pins the SCF, the MO integrals, E_MP2, E_CCSD, E_(T),
+the largest amplitudes and, through the attic driver and its Pulay DIIS
+(attic/ugacc.cc:100-114, attic/diis.cc), the whole CCSD iteration trace.
+The HOF runs (Psi4 ccenergy/cctriples) pin STO-3G all-electron and
+frozen-core energies and the i>=j>=k triplet counts."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from ugacc_amd.chem import molecules
+from ugacc_amd.chem.scf import Reference
+
+KA = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "known_answers.json")))
+
+
+@pytest.fixture(scope="module")
+def h2o_fc():
+    return Reference(molecules.PSI3_H2O, "cc-pvdz", nfzc=1)
+
+
+def test_h2o_scf_and_front_end(h2o_fc):
+    ka = KA["h2o_ccpvdz_fc1"]
+    ref = h2o_fc
+    assert abs(ref.enuc - ka["e_nuc"]) < 1e-11
+    w = np.linalg.eigvalsh(ref.scf.S)
+    assert abs(w.min() - ka["s_min_eig"]) < 1e-8 and abs(w.max() - ka["s_max_eig"]) < 1e-6
+    assert abs(ref.escf - ka["e_scf"]) < 1e-10
+    assert abs(ref.efzc - ka["e_fzc"]) < 1e-10
+    nocc = ref.scf.nocc
+    np.testing.assert_allclose(ref.eps[:nocc], ka["eps_occ"], atol=1e-6)
+    np.testing.assert_allclose(ref.eps[nocc:], ka["eps_vir"], atol=1e-6)
+
+
+def test_h2o_ccsd_t_energies_and_amplitudes(h2o_fc):
+    ka = KA["h2o_ccpvdz_fc1"]
+    ref = h2o_fc
+    o = Oracle(ref.fock, ref.full_ints(), ref.no)
+    assert abs(o.energy() - ka["ccsd_trace"][0]) < 1e-11  # E_MP2 (iteration 0)
+    r = o.ccsd(maxiter=100, conv=1e-13, mode=0)
+    assert r["converged"]
+    assert abs(ref.escf + r["eccsd"] - ka["e_ccsd_total"]) < 1e-10
+    et = o.tcorr_tjl()
+    assert abs(et - (ka["e_ccsd_t_total"] - ka["e_ccsd_total"])) < 1e-11
+    assert abs(o.tcorr_ooc() - et) < 1e-12
+    t1, t2 = o.amps()
+    np.testing.assert_allclose(np.sort(np.abs(t1.ravel()))[::-1][:20], ka["t1_largest"], atol=1e-9)
+    np.testing.assert_allclose(np.sort(np.abs(t2.ravel()))[::-1][:20], ka["t2_largest"], atol=1e-9)
+
+
+def test_h2o_attic_driver_reproduces_iteration_trace(h2o_fc):
+    ka = KA["h2o_ccpvdz_fc1"]
+    ref = h2o_fc
+    o = Oracle(ref.fock, ref.full_ints(), ref.no)
+    # attic driver: energy of each fresh Jacobi update, DIIS from two vectors
+    r = o.ccsd(maxiter=100, conv=1e-13, mode=1, diis_policy=0, min_vecs=2)
+    trace = np.array(ka["ccsd_trace"])
+    assert r["niter"] == len(trace) - 1
+    np.testing.assert_allclose(r["trace"], trace, atol=1e-12, rtol=0)
+
+
+@pytest.mark.parametrize("key,nfzc", [("hof_sto3g_c1_allelec", 0), ("hof_sto3g_fc2", 2)])
+def test_hof_sto3g(key, nfzc):
+    ka = KA[key]
+    ref = Reference(molecules.HOF, "sto-3g", nfzc=nfzc)
+    assert abs(ref.enuc - ka["e_nuc"]) < 1e-11
+    assert abs(ref.escf - ka["e_scf"]) < 1e-10
+    o = Oracle(ref.fock, ref.full_ints(), ref.no)
+    assert abs(o.energy() - ka["e_mp2"]) < 1e-11
+    r = o.ccsd(maxiter=200, conv=1e-13)
+    assert abs(r["eccsd"] - ka["e_ccsd"]) < 1e-10
+    assert o.ntriplets() == ka["n_triplets"]
+    assert abs(o.tcorr_tjl() - ka["e_t"]) < 1e-12
+
+
+def test_spherical_d_shell_is_orthonormal():
+    # a lone d shell: the five pure functions are orthonormal
+    from ugacc_amd.chem import integrals
+    S, _, _ = integrals.one_electron([((0.1, -0.2, 0.3), 2, [(1.185, 1.0)])], [], np.zeros((0, 3)))
+    np.testing.assert_allclose(S, np.eye(5), atol=1e-14)
+
+
+def test_water_cluster_generator_constraints():
+    for n, seed in [(4, 1), (8, 3)]:
+        atoms = molecules.water_cluster(n, seed)
+        assert len(atoms) == 3 * n
+        pos = np.array([x for _, x in atoms]) / molecules.ANGSTROM
+        O = pos[0::3]
+        d = [np.linalg.norm(O[k] - O[(k + 1) % n]) for k in range(n)]
+        assert min(d) >= 2.70 - 1e-9 and max(d) <= 2.90 + 1e-9
+        for k in range(n):  # monomer geometry kept
+            assert abs(np.linalg.norm(pos[3 * k + 1] - pos[3 * k]) - 0.9572) < 1e-9
+        assert molecules.water_cluster(n, seed)[5][1] == atoms[5][1]  # seeded
