@@ -81,3 +81,25 @@ def test_ccsd_and_triples_parity(case):
     W_o = orc.w3(no - 1, no // 2, 0)
     W_g = c.w3(no - 1, no // 2, 0)
     assert np.max(np.abs(W_g - W_o)) < 1e-12 * max(1.0, np.max(np.abs(W_o)))
+
+
+@pytest.mark.parametrize("no,nv", [(3, 70), (2, 130)])
+def test_triples_large_virtual_space(no, nv):
+    """(T) with v above 64 and 128, where the W3 GEMMs use the 80- and
+    160-wide tiles: per-triplet energies and one W3 against the oracle, from
+    seeded amplitudes."""
+    announce("triples o=%d v=%d" % (no, nv))
+    c, fock, ints = loaded(no, nv, seed=nv)
+    orc = Oracle(fock, ints, no, omp=True)
+    rng = np.random.default_rng(nv)
+    t1 = 0.01 * rng.standard_normal((no, nv))
+    t2 = 0.01 * rng.standard_normal((no, no, nv, nv))
+    t2 = 0.5 * (t2 + t2.transpose(1, 0, 3, 2))
+    orc.set_amps(t1, t2)
+    c.set_amplitudes(t1, t2)
+    per_o = orc.tjl_per_triplet()
+    per_g = c.triples_per_triplet()
+    assert np.max(np.abs(per_g - per_o)) < 1e-12 * max(1e-6, np.max(np.abs(per_o))) / 1e-6
+    W_o = orc.w3(no - 1, 0, 0)
+    W_g = c.w3(no - 1, 0, 0)
+    assert np.max(np.abs(W_g - W_o)) < 1e-13

@@ -41,18 +41,31 @@ def test_gemm_forms(prec, opa, opb, M, N, K):
     assert err < tol, err
 
 
-@pytest.mark.parametrize("tile", range(9))
-def test_gemm_tiles_fp64(tile):
+@pytest.mark.parametrize("prec", [ugx.PREC_FP64, ugx.PREC_FP32])
+@pytest.mark.parametrize("layout", ["NT", "TN"])
+@pytest.mark.parametrize("tile", range(11))
+def test_gemm_tiles(tile, layout, prec):
+    """Every tile shape (including the 80- and 160-row ones) with both the
+    k-contiguous (NT) and the row-contiguous (TN) operand loaders."""
     from tests.gpu_util import context, announce
-    announce("gemm tile=%d" % tile)
+    announce("gemm tile=%d %s prec=%d" % (tile, layout, prec))
     c = context()
     rng = np.random.default_rng(tile)
     M, N, K = 300, 170, 90
     A = rng.standard_normal((M, K))
     B = rng.standard_normal((N, K))
-    dA, dB = _dev(A, torch.float64), _dev(B, torch.float64)
+    tdt = torch.float64 if prec == ugx.PREC_FP64 else torch.float32
     dC = torch.zeros((M, N), dtype=torch.float64, device="cuda")
-    torch.cuda.synchronize()
-    c.gemm("N", "T", M, N, K, 1.0, dA.data_ptr(), K, dB.data_ptr(), K, 0.0, dC.data_ptr(), N, tile=tile)
+    if layout == "NT":
+        dA, dB = _dev(A, tdt), _dev(B, tdt)
+        torch.cuda.synchronize()
+        c.gemm("N", "T", M, N, K, 1.0, dA.data_ptr(), K, dB.data_ptr(), K, 0.0, dC.data_ptr(), N, precision=prec,
+               tile=tile)
+    else:
+        dA, dB = _dev(A.T.copy(), tdt), _dev(B.T.copy(), tdt)
+        torch.cuda.synchronize()
+        c.gemm("T", "N", M, N, K, 1.0, dA.data_ptr(), M, dB.data_ptr(), N, 0.0, dC.data_ptr(), N, precision=prec,
+               tile=tile)
     c.synchronize()
-    np.testing.assert_allclose(dC.cpu().numpy(), A @ B.T, rtol=0, atol=1e-12)
+    tol = 1e-12 if prec == ugx.PREC_FP64 else 1e-4
+    np.testing.assert_allclose(dC.cpu().numpy(), A @ B.T, rtol=0, atol=tol)

@@ -89,53 +89,48 @@ __device__ inline SegView seg_view(const GemmDesc &d, const GemmPtrs *ptrs, int 
 }
 
 // Per-thread loader of one operand tile (R rows x BK k-values per slice).
-// The unit-stride side decides the thread mapping: k-contiguous operands are
-// read as BK consecutive k per row (16 lanes cover one 128-byte row
-// segment), row-contiguous ones as R consecutive rows per k.  The pointer
-// of the first element is set up once per segment and advanced by BK k per
-// slice, so the inner loop is an add per element.
+// Element e = tid + q*NT (q < R*BK/NT) of the slice maps to (row, k) along
+// the operand's unit-stride side: k-contiguous operands are read as BK
+// consecutive k per row (16 lanes cover one 128-byte row segment),
+// row-contiguous ones as R consecutive rows per k.  The mapping is a
+// bijection for any R that is a multiple of 16.  The slice pointer is set up
+// once per segment and advanced by BK k per slice.
 template <typename T, int R>
 struct OpLoader {
   static constexpr int N_ = R * BK / NT;  // elements per thread per slice
-  const T *ptr;                           // element of q = 0 at the current slice
-  long qoff;                              // pointer step between q
-  long kadv;                              // pointer step per slice
-  int kk0, kkq;                           // k offset within the slice: kk0 + q*kkq
-  unsigned rowmask;                       // bit q: row of q is inside the operand
-  bool kc;
+  static_assert(R % 16 == 0 && (R * BK) % NT == 0, "tile rows must be a multiple of 16");
+  const T *ptr;   // element (row0, k0) of the current slice
+  long srow, sk;
+  int rows_left;  // rows of the operand from row0 on
   int K;
+  bool kc;
 
-  __device__ inline void setup(const T *base, long srow, long sk, int row0, int rows, int K_, int tid) {
-    K = K_;
-    kc = (sk == 1);
-    int r, rq;
-    if (kc) { r = tid / BK; kk0 = tid % BK; rq = NT / BK; kkq = 0; }
-    else { r = tid % R; kk0 = tid / R; rq = 0; kkq = NT / R; }
-    ptr = base + (long)(row0 + r) * srow + (long)kk0 * sk;
-    qoff = (long)rq * srow + (long)kkq * sk;
-    kadv = (long)BK * sk;
-    rowmask = 0;
-#pragma unroll
-    for (int q = 0; q < N_; q++)
-      if (row0 + r + q * rq < rows) rowmask |= 1u << q;
+  __device__ static inline void rk(bool kc, int e, int &r, int &kk) {
+    if (kc) { r = e / BK; kk = e % BK; } else { r = e % R; kk = e / R; }
   }
-  __device__ inline void load(T (&v)[N_], int k0, T scale) const {
+  __device__ inline void setup(const T *base, long srow_, long sk_, int row0, int rows, int K_) {
+    srow = srow_; sk = sk_; K = K_;
+    kc = (sk == 1);
+    ptr = base + (long)row0 * srow;
+    rows_left = rows - row0;
+  }
+  __device__ inline void load(T (&v)[N_], int k0, T scale, int tid) const {
 #pragma unroll
     for (int q = 0; q < N_; q++) {
-      int k = k0 + kk0 + q * kkq;
+      int r, kk;
+      rk(kc, tid + q * NT, r, kk);
       T x = T(0);
-      if (((rowmask >> q) & 1u) && k < K) x = ptr[q * qoff];
+      if (r < rows_left && k0 + kk < K) x = ptr[(long)r * srow + (long)kk * sk];
       v[q] = x * scale;
     }
   }
-  __device__ inline void advance() { ptr += kadv; }
+  __device__ inline void advance() { ptr += (long)BK * sk; }
   template <int LD>
   __device__ inline void store(const T (&v)[N_], T (*S)[LD], int tid) const {
 #pragma unroll
     for (int q = 0; q < N_; q++) {
       int r, kk;
-      if (kc) { r = tid / BK + q * (NT / BK); kk = tid % BK; }
-      else { r = tid % R; kk = tid / R + q * (NT / R); }
+      rk(kc, tid + q * NT, r, kk);
       S[kk][r] = v[q];
     }
   }
@@ -211,8 +206,8 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
   int cur_seg = -1;
   auto seg_setup = [&](int s) {
     const GemmSeg &g = d.seg[s];
-    la.setup(Ab[s], g.sam, g.sak, m0, M, Ks[s], tid);
-    lb.setup(Bb[s], g.sbn, g.sbk, n0, N, Ks[s], tid);
+    la.setup(Ab[s], g.sam, g.sak, m0, M, Ks[s]);
+    lb.setup(Bb[s], g.sbn, g.sbk, n0, N, Ks[s]);
     cur_seg = s;
   };
 
@@ -232,8 +227,8 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
     int s = (t < nt[0]) ? 0 : 1;
     int k0 = (s == 0 ? t : t - nt[0]) * BK;
     if (s != cur_seg) seg_setup(s);
-    la.load(ra, k0, sg[s]);
-    lb.load(rb, k0, T(1));
+    la.load(ra, k0, sg[s], tid);
+    lb.load(rb, k0, T(1), tid);
     la.advance();
     lb.advance();
   };
