@@ -1,0 +1,86 @@
+"""Host logic that runs without a GPU: the C-ABI library loads and exports
+every function include/ugx.h declares, the options mirror the reference's
+read_options, the plugin entry rejects the inputs the reference rejects, and
+the layout helpers round-trip."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ugacc_amd import plugin, ugx
+from ugacc_amd.hamiltonian import blocks_from_full, full_from_blocks
+from ugacc_amd.synthetic import random_blocks, random_hamiltonian
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    txt = open(os.path.join(ROOT, "include", "ugx.h")).read()
+    return re.findall(r"^\w[\w\s\*]*?\b(ugx_\w+)\s*\(", txt, flags=re.M)
+
+
+def test_library_exports_every_declared_function():
+    L = ugx.load_library()
+    names = declared_functions()
+    assert len(names) >= 15
+    assert names == ugx.EXPORTS
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_default_options_match_read_options():
+    o = ugx.default_options()
+    ro = plugin.read_options()
+    assert o.maxiter == ro["MAXITER"] == 100
+    assert o.diis == 1 and ro["DIIS"] is True
+    assert o.r_convergence == ro["R_CONVERGENCE"] == 1e-7
+    assert o.ooc == 0 and ro["OOC"] is False
+    assert ro["WFN"] == "CCSD" and o.wfn_ccsd_t == 0
+    assert o.diis_nvec == 8 and o.diis_min_vecs == 3
+
+
+class _Ref:
+    no, nv, escf = 2, 3, -1.0
+    fock = np.eye(5)
+    blocks = None
+
+
+def test_plugin_rejects_what_the_reference_rejects():
+    opts = plugin.read_options()
+    with pytest.raises(plugin.PsiException, match="SCF has not been run yet!"):
+        plugin.ugacc(None, opts)
+    bad = dict(opts, REFERENCE="UHF")
+    with pytest.raises(plugin.PsiException, match="Only for use with RHF references."):
+        plugin.ugacc(_Ref(), bad)
+    r = _Ref()
+    r.socc = 2
+    with pytest.raises(plugin.PsiException, match="closed-shell systems only"):
+        plugin.ugacc(r, opts)
+
+
+def test_context_without_gpu_reports_no_device():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(ugx.UgxError) as e:
+        ugx.Context(0)
+    assert e.value.code == ugx.NO_DEVICE
+
+
+def test_blocks_full_round_trip():
+    f, g = random_hamiltonian(3, 4, seed=5)
+    b = blocks_from_full(g, 3)
+    np.testing.assert_allclose(full_from_blocks(3, 4, b), g, atol=1e-15, rtol=0)
+
+
+def test_random_blocks_match_full_construction():
+    f, b = random_blocks(2, 5, seed=9)
+    g = full_from_blocks(2, 5, b)
+    # 8-fold symmetry of real integrals: <pq|rs> = <qp|sr> = <rs|pq> = <rq|ps>
+    np.testing.assert_allclose(g, g.transpose(1, 0, 3, 2), atol=1e-15)
+    np.testing.assert_allclose(g, g.transpose(2, 3, 0, 1), atol=1e-15)
+    np.testing.assert_allclose(g, g.transpose(2, 1, 0, 3), atol=1e-15)
