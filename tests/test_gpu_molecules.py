@@ -91,3 +91,24 @@ def test_h2o4_full_size_sampled_parity():
     nt = len(per)
     for t in range(0, nt, nt // 7):
         assert abs(per[t] - o.tcorr_tjl_range(t, t + 1)) < 1e-13, t
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16x3"])
+def test_mixed_precision_forms_reach_the_fp64_fixed_point(form):
+    """Low-precision iterations to ||R|| < 1e-5, then FP64 polish: the same
+    CCSD fixed point as the FP64 form (|dE| <= 1e-9).  (T) evaluated in the
+    low precision (FP64 energy sums and denominators) stays within 1e-7
+    (FP32) / 5e-7 (BF16x3, whose dropped lo*lo term is ~2^-16 per product)
+    relative, far inside the 1 microhartree budget of the mixed form."""
+    announce("mixed form " + form + " (h2o)2")
+    ref = Reference(molecules.water_cluster(2, seed=3), "cc-pvdz")
+    c = context()
+    c.load_hamiltonian(ref.no, ref.nv, ref.fock, ref.blocks)
+    r64 = c.ccsd(maxiter=100, r_convergence=1e-10, wfn_ccsd_t=1)
+    rm = c.ccsd(maxiter=100, r_convergence=1e-10, wfn_ccsd_t=1, precision=ugx.PREC_NAMES[form],
+                polish_threshold=1e-5)
+    assert r64["converged"] and rm["converged"]
+    assert rm["niter_low"] > 0
+    assert abs(rm["e_ccsd"] - r64["e_ccsd"]) < 1e-9
+    rel = 1e-7 if form == "fp32" else 5e-7
+    assert abs(rm["e_t"] - r64["e_t"]) < rel * abs(r64["e_t"])

@@ -404,20 +404,12 @@ void guess_mp2(ugx_ctx *c) {
 }
 
 // ------------------------------------------------------------------ (T)
-// W3 of a batch of triplets: six grouped MFMA contractions, one per
-// permutation of the (ia, jb, kc) pairs (ccwfn.cc:1396-1413):
+// W3 of a triplet (ccwfn.cc:1396-1413) is the sum over the six permutations
+// of the (ia, jb, kc) pairs of
 //   X(p,q,r)[x][y][z] = sum_e <pe|xy> t2_rq^ze - sum_m t2_pm^xy <qr|mz>
 //   W = X(ijk)[abc] + X(ikj)[acb] + X(kij)[cab] + X(kji)[cba] + X(jki)[bca]
-//       + X(jik)[bac]
-// as GEMMs with rows (x,y) (v^2), columns z (v) and K = v (particle) + o
-// (hole), written to W[a][b][c] through the output row split.
-struct Perm3 {
-  int p, q, r;  // which of (i,j,k) plays p, q, r: 0 = i, 1 = j, 2 = k
-  int x, y, z;  // which of (a,b,c) plays x, y, z: 0 = a, 1 = b, 2 = c
-};
-const Perm3 kPerms[6] = {{0, 1, 2, 0, 1, 2}, {0, 2, 1, 0, 2, 1}, {2, 0, 1, 2, 0, 1},
-                         {2, 1, 0, 2, 1, 0}, {1, 2, 0, 1, 2, 0}, {1, 0, 2, 1, 0, 2}};
-
+//       + X(jik)[bac],
+// each an MFMA GEMM with K = v (particle) + o (hole).
 void triples_prepare(ugx_ctx *c, int prec) {
   const int o = c->no, v = c->nv;
   const long v3 = (long)v * v * v;
@@ -458,73 +450,88 @@ void triples_prepare(ugx_ctx *c, int prec) {
 
 // Builds W for triplets [t0, t0 + nb) of the table into c->tWbuf.
 //
-// Each permutation is one grouped GEMM over the batch.  Its output index
-// with unit stride in W (c) decides the orientation so that the 16 lanes of
-// an MFMA output row write consecutive addresses:
-//   z = c: rows (x,y) from P[p][x][y][e], columns z;
-//   y = c: rows z, columns (x,y) from P (transposed operands);
-//   x = c: rows z, columns (y,x) from Q[p][y][x][e] and t2[m][p][y][x]
-//          (t2_pm^xy = t2_mp^yx), so the fast column index is x = c.
+// The six permutations pair up by their third output index z, and the two
+// members of a pair share one accumulator (one grouped GEMM with four K
+// segments: two particle terms, K = v, and two hole terms, K = o):
+//   z = c: X(ijk)[abc] + X(jik)[bac]   rows (a,b), columns c
+//   z = b: X(ikj)[acb] + X(kij)[cab]   rows b, columns (a,c)
+//   z = a: X(jki)[bca] + X(kji)[cba]   rows a, columns (b,c)
+// The column index of each launch is the one with unit stride in W, so the
+// 16 lanes of an MFMA output row write consecutive addresses.  The second
+// member of a pair reads its integrals from Q[p][y][x][e] = <pe|xy> and its
+// hole amplitudes as t2_pm^xy = t2_mp^yx, which puts its rows/columns in
+// the first member's order.  W is written once and accumulated twice.
 void triples_build_w(ugx_ctx *c, int prec, long t0, int nb, const std::vector<int> &trip) {
   const int o = c->no, v = c->nv;
-  const long v2 = (long)v * v, v3 = v2 * v;
+  const long v2 = (long)v * v, v3 = v2 * v, ov = (long)o * v, ov2 = (long)o * v2;
   const bool f32 = (prec != PREC_FP64);
   const size_t es = f32 ? 4 : 8;
   const char *P = (const char *)(f32 ? c->f_tP.p : c->tP.p);
   const char *Q = (const char *)(f32 ? c->f_tQ.p : c->tQ.p);
   const char *T2 = (const char *)(f32 ? c->f_t2.p : c->t2.p);
   const char *OV = (const char *)(f32 ? c->f_ooov.p : c->ooov.p);
-  const long stride[3] = {v2, v, 1};  // of a, b, c in W[a][b][c]
-  std::vector<GemmPtrs> tab((size_t)6 * nb);
-  for (int s = 0; s < 6; s++) {
-    const Perm3 &pm = kPerms[s];
-    const int mode = (pm.z == 2) ? 0 : (pm.y == 2 ? 1 : 2);
-    for (int b = 0; b < nb; b++) {
-      const int *ijk = &trip[3 * (t0 + b)];
-      int p = ijk[pm.p], q = ijk[pm.q], r = ijk[pm.r];
-      GemmPtrs &g = tab[(size_t)s * nb + b];
-      const char *Pp = (mode == 2 ? Q : P) + (size_t)p * v3 * es;  // P[p][x][y][e] or Q[p][y][x][e]
-      const char *T2rq = T2 + ((size_t)r * o + q) * v2 * es;       // t2[r][q][z][e]
-      const char *T2p = (mode == 2) ? T2 + (size_t)p * v2 * es     // t2[m][p][y][x]
-                                    : T2 + (size_t)p * o * v2 * es;  // t2[p][m][x][y]
-      const char *OVqr = OV + ((size_t)q * o + r) * o * v * es;     // <qr|mz>
-      if (mode == 0) { g.A[0] = Pp; g.B[0] = T2rq; g.A[1] = T2p; g.B[1] = OVqr; }
-      else { g.A[0] = T2rq; g.B[0] = Pp; g.A[1] = OVqr; g.B[1] = T2p; }
-      g.C = c->tWbuf.d() + (size_t)b * v3;
-    }
+  auto at = [&](const char *base, long off) { return (const void *)(base + (size_t)off * es); };
+  std::vector<GemmPtrs> tab((size_t)3 * nb);
+  for (int b = 0; b < nb; b++) {
+    const long i = trip[3 * (t0 + b)], j = trip[3 * (t0 + b) + 1], k = trip[3 * (t0 + b) + 2];
+    double *W = c->tWbuf.d() + (size_t)b * v3;
+    GemmPtrs &A = tab[b], &B = tab[(size_t)nb + b], &C = tab[(size_t)2 * nb + b];
+    // z = c
+    A.A[0] = at(P, i * v3);              A.B[0] = at(T2, (k * o + j) * v2);
+    A.A[1] = at(T2, i * ov2);            A.B[1] = at(OV, (j * o + k) * ov);
+    A.A[2] = at(Q, j * v3);              A.B[2] = at(T2, (k * o + i) * v2);
+    A.A[3] = at(T2, j * v2);             A.B[3] = at(OV, (i * o + k) * ov);
+    A.C = W;
+    // z = b
+    B.A[0] = at(T2, (j * o + k) * v2);   B.B[0] = at(P, i * v3);
+    B.A[1] = at(OV, (k * o + j) * ov);   B.B[1] = at(T2, i * ov2);
+    B.A[2] = at(T2, (j * o + i) * v2);   B.B[2] = at(Q, k * v3);
+    B.A[3] = at(OV, (i * o + j) * ov);   B.B[3] = at(T2, k * v2);
+    B.C = W;
+    // z = a
+    C.A[0] = at(T2, (i * o + k) * v2);   C.B[0] = at(P, j * v3);
+    C.A[1] = at(OV, (k * o + i) * ov);   C.B[1] = at(T2, j * ov2);
+    C.A[2] = at(T2, (i * o + j) * v2);   C.B[2] = at(Q, k * v3);
+    C.A[3] = at(OV, (j * o + i) * ov);   C.B[3] = at(T2, k * v2);
+    C.C = W;
   }
   c->tptrs.alloc(tab.size() * sizeof(GemmPtrs));
   check(hipMemcpyAsync(c->tptrs.p, tab.data(), tab.size() * sizeof(GemmPtrs), hipMemcpyHostToDevice, c->st),
         "pointer table copy");
-  for (int s = 0; s < 6; s++) {
-    const Perm3 &pm = kPerms[s];
-    const int mode = (pm.z == 2) ? 0 : (pm.y == 2 ? 1 : 2);
+  auto tile_rows_v = [&](int n) { return n <= 32 ? 2 : n <= 64 ? 3 : n <= 80 ? 8 : n <= 128 ? 0 : n <= 160 ? 5 : 0; };
+  auto tile_cols_v = [&](int n) { return n <= 32 ? 2 : n <= 64 ? 4 : n <= 80 ? 10 : n <= 128 ? 0 : n <= 160 ? 9 : 0; };
+  const double fl = 4.0 * (double)nb * (double)v3 * (double)(v + o);
+  for (int g = 0; g < 3; g++) {
     GemmDesc d{};
-    d.nseg = 2; d.batch = nb;
-    d.seg[0].K = v; d.seg[0].sign = 1.0;
-    d.seg[1].K = o; d.seg[1].sign = -1.0;
+    d.nseg = 4;
+    d.batch = nb;
+    for (int s = 0; s < 4; s++) {
+      d.seg[s].K = (s % 2 == 0) ? v : o;
+      d.seg[s].sign = (s % 2 == 0) ? 1.0 : -1.0;
+    }
     int tile;
-    if (mode == 0) {
+    if (g == 0) {
       d.M = (int)v2; d.N = v;
       d.seg[0].sam = v; d.seg[0].sak = 1; d.seg[0].sbk = 1; d.seg[0].sbn = v;
       d.seg[1].sam = 1; d.seg[1].sak = v2; d.seg[1].sbk = v; d.seg[1].sbn = 1;
-      d.cdiv = v; d.scm1 = stride[pm.x]; d.scm = stride[pm.y]; d.scn = stride[pm.z];
-      tile = v > 128 ? 5 : (v > 64 ? 8 : (v > 32 ? 1 : 2));
+      d.seg[2].sam = v; d.seg[2].sak = 1; d.seg[2].sbk = 1; d.seg[2].sbn = v;
+      d.seg[3].sam = 1; d.seg[3].sak = ov2; d.seg[3].sbk = v; d.seg[3].sbn = 1;
+      d.scm = v; d.scn = 1;
+      tile = tile_rows_v(v);
     } else {
       d.M = v; d.N = (int)v2;
       d.seg[0].sam = v; d.seg[0].sak = 1; d.seg[0].sbk = 1; d.seg[0].sbn = v;
-      d.seg[1].sam = 1; d.seg[1].sak = v;
-      d.seg[1].sbk = (mode == 1) ? v2 : (long)o * v2; d.seg[1].sbn = 1;
-      d.scm = stride[pm.z];
-      d.ndiv = v;
-      if (mode == 1) { d.scn1 = stride[pm.x]; d.scn = stride[pm.y]; }
-      else { d.scn1 = stride[pm.y]; d.scn = stride[pm.x]; }
-      tile = v > 128 ? 9 : (v > 64 ? 10 : (v > 32 ? 1 : 2));
+      d.seg[1].sam = 1; d.seg[1].sak = v; d.seg[1].sbk = v2; d.seg[1].sbn = 1;
+      d.seg[2].sam = v; d.seg[2].sak = 1; d.seg[2].sbk = 1; d.seg[2].sbn = v;
+      d.seg[3].sam = 1; d.seg[3].sak = v; d.seg[3].sbk = ov2; d.seg[3].sbn = 1;
+      if (g == 1) { d.scm = v; d.ndiv = v; d.scn1 = v2; d.scn = 1; }
+      else { d.scm = v2; d.scn = 1; }
+      tile = tile_cols_v(v);
     }
-    d.alpha = 1.0; d.beta = (s == 0) ? 0.0 : 1.0;
-    const double fl = 2.0 * (double)nb * (double)v2 * v * (v + o);
+    d.alpha = 1.0;
+    d.beta = (g == 0) ? 0.0 : 1.0;
     c->eng->timed("triples_w", fl, [&]() {
-      launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)s * nb, prec, tile, c->st);
+      launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)g * nb, prec, tile, c->st);
     });
   }
 }

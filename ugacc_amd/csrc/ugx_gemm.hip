@@ -3,11 +3,11 @@
 // One kernel family serves every contraction of the CCSD residual and the
 // (T) W3 build:
 //   C(m,n) = alpha * sum_seg sign_s * sum_k A_s(m,k) B_s(k,n) + beta * C(m,n)
-// with up to two K segments that have their own operands and strides (the
-// (T) particle term, K = v, and hole term, K = o, of one output permutation
-// share an accumulator that way), a batch dimension (strided or a device
-// table of pointers), and an output row split so that a (T) permutation can
-// write W[a][b][c] with any of the six index orders.
+// with up to four K segments that have their own operands and strides (the
+// (T) particle terms, K = v, and hole terms, K = o, of two output
+// permutations share one accumulator that way), a batch dimension (strided
+// or a device table of pointers), and row/column splits of the output
+// address so that a (T) permutation can write W[a][b][c] in any index order.
 //
 // Work decomposition: 256 threads = 4 wave64s arranged WGM x WGN; each wave
 // owns TM x TN blocks of 16x16 MFMA outputs, so a workgroup tile is
@@ -41,6 +41,7 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int BK = 16;
+constexpr int MAXSEG = 4;
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
@@ -67,38 +68,17 @@ struct Mfma<float> {
   __device__ static inline int row(int kq, int r) { return 4 * kq + r; }
 };
 
-struct SegView {
-  const void *A;
-  const void *B;
-  long sam, sak, sbk, sbn;
-  int K;
-  double sign;
-};
-
-__device__ inline SegView seg_view(const GemmDesc &d, const GemmPtrs *ptrs, int s, long z) {
-  const GemmSeg &g = d.seg[s];
-  SegView v;
-  v.sam = g.sam; v.sak = g.sak; v.sbk = g.sbk; v.sbn = g.sbn; v.K = g.K; v.sign = g.sign;
-  if (ptrs) {
-    v.A = ptrs[z].A[s];
-    v.B = ptrs[z].B[s];
-  } else {
-    v.A = nullptr; v.B = nullptr;  // resolved by the caller with the element size
-  }
-  return v;
-}
-
-// Per-thread loader of one operand tile (R rows x BK k-values per slice).
-// Element e = tid + q*NT (q < R*BK/NT) of the slice maps to (row, k) along
-// the operand's unit-stride side: k-contiguous operands are read as BK
-// consecutive k per row (16 lanes cover one 128-byte row segment),
-// row-contiguous ones as R consecutive rows per k.  The mapping is a
-// bijection for any R that is a multiple of 16.  The slice pointer is set up
-// once per segment and advanced by BK k per slice.
-template <typename T, int R>
+// Per-thread loader of one operand tile (R rows x KB k-values per slice).
+// Element e = tid + q*NT (q < R*KB/NT) of the slice maps to (row, k) along
+// the operand's unit-stride side: k-contiguous operands are read as KB
+// consecutive k per row (16 lanes cover one row segment), row-contiguous ones
+// as R consecutive rows per k.  The mapping is a bijection for any R that is
+// a multiple of 16.  The slice pointer is set up once per segment and
+// advanced by KB k per slice.
+template <typename T, int R, int KB>
 struct OpLoader {
-  static constexpr int N_ = R * BK / NT;  // elements per thread per slice
-  static_assert(R % 16 == 0 && (R * BK) % NT == 0, "tile rows must be a multiple of 16");
+  static constexpr int N_ = R * KB / NT;  // elements per thread per slice
+  static_assert(R % 16 == 0 && (R * KB) % NT == 0, "tile rows must be a multiple of 16");
   const T *ptr;   // element (row0, k0) of the current slice
   long srow, sk;
   int rows_left;  // rows of the operand from row0 on
@@ -106,7 +86,7 @@ struct OpLoader {
   bool kc;
 
   __device__ static inline void rk(bool kc, int e, int &r, int &kk) {
-    if (kc) { r = e / BK; kk = e % BK; } else { r = e % R; kk = e / R; }
+    if (kc) { r = e / KB; kk = e % KB; } else { r = e % R; kk = e / R; }
   }
   __device__ inline void setup(const T *base, long srow_, long sk_, int row0, int rows, int K_) {
     srow = srow_; sk = sk_; K = K_;
@@ -124,15 +104,56 @@ struct OpLoader {
       v[q] = x * scale;
     }
   }
-  __device__ inline void advance() { ptr += (long)BK * sk; }
+  __device__ inline void advance() { ptr += (long)KB * sk; }
+  // k-major LDS tile S[k][row]
   template <int LD>
-  __device__ inline void store(const T (&v)[N_], T (*S)[LD], int tid) const {
+  __device__ inline void store_kmajor(const T (&v)[N_], T (*S)[LD], int tid) const {
 #pragma unroll
     for (int q = 0; q < N_; q++) {
       int r, kk;
       rk(kc, tid + q * NT, r, kk);
       S[kk][r] = v[q];
     }
+  }
+};
+
+// Segment bookkeeping shared by the kernels: operand bases of batch entry z,
+// K per segment (the split-K tail applied to segment 0) and slice counts.
+template <typename T, int KB>
+struct Segs {
+  const T *A[MAXSEG];
+  const T *B[MAXSEG];
+  int K[MAXSEG], nt[MAXSEG];
+  T sign[MAXSEG];
+  int n, ntiles;
+  __device__ inline void init(const GemmDesc &d, const GemmPtrs *ptrs, long z) {
+    n = d.nseg;
+    ntiles = 0;
+#pragma unroll
+    for (int s = 0; s < MAXSEG; s++) {
+      A[s] = nullptr; B[s] = nullptr; K[s] = 0; nt[s] = 0; sign[s] = T(1);
+      if (s < n) {
+        const GemmSeg &g = d.seg[s];
+        if (ptrs) { A[s] = (const T *)ptrs[z].A[s]; B[s] = (const T *)ptrs[z].B[s]; }
+        else { A[s] = (const T *)g.A + z * g.sAz; B[s] = (const T *)g.B + z * g.sBz; }
+        int k = g.K;
+        if (s == 0 && d.ktotal > 0) {
+          long rem = d.ktotal - z * (long)g.K;
+          if (rem < k) k = rem > 0 ? (int)rem : 0;
+        }
+        K[s] = k;
+        nt[s] = (k + KB - 1) / KB;
+        sign[s] = T(g.sign);
+        ntiles += nt[s];
+      }
+    }
+  }
+  // slice t -> segment and its k0
+  __device__ inline int locate(int t, int &k0) const {
+    int s = 0;
+    while (s < n - 1 && t >= nt[s]) { t -= nt[s]; s++; }
+    k0 = t * KB;
+    return s;
   }
 };
 
@@ -181,33 +202,17 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
   const long z = blockIdx.y;
   const int tid = threadIdx.x;
 
-  // segment operands for this batch entry
-  const T *Ab[2] = {nullptr, nullptr};
-  const T *Bb[2] = {nullptr, nullptr};
-  int Ks[2] = {0, 0}, nt[2] = {0, 0};
-  T sg[2] = {T(1), T(1)};
-  for (int s = 0; s < d.nseg; s++) {
-    const GemmSeg &g = d.seg[s];
-    if (ptrs) { Ab[s] = (const T *)ptrs[z].A[s]; Bb[s] = (const T *)ptrs[z].B[s]; }
-    else { Ab[s] = (const T *)g.A + z * g.sAz; Bb[s] = (const T *)g.B + z * g.sBz; }
-    int K = g.K;
-    if (s == 0 && d.ktotal > 0) {
-      long rem = d.ktotal - z * (long)g.K;
-      if (rem < K) K = rem > 0 ? (int)rem : 0;
-    }
-    Ks[s] = K;
-    nt[s] = (K + BK - 1) / BK;
-    sg[s] = T(g.sign);
-  }
-  const int ntiles = nt[0] + nt[1];
+  Segs<T, BK> S;
+  S.init(d, ptrs, z);
+  const int ntiles = S.ntiles;
 
-  OpLoader<T, BM> la;
-  OpLoader<T, BN> lb;
+  OpLoader<T, BM, BK> la;
+  OpLoader<T, BN, BK> lb;
   int cur_seg = -1;
   auto seg_setup = [&](int s) {
     const GemmSeg &g = d.seg[s];
-    la.setup(Ab[s], g.sam, g.sak, m0, M, Ks[s]);
-    lb.setup(Bb[s], g.sbn, g.sbk, n0, N, Ks[s]);
+    la.setup(S.A[s], g.sam, g.sak, m0, M, S.K[s]);
+    lb.setup(S.B[s], g.sbn, g.sbk, n0, N, S.K[s]);
     cur_seg = s;
   };
 
@@ -221,13 +226,13 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
 #pragma unroll
     for (int j = 0; j < TN; j++) acc[i][j] = MF::zero();
 
-  T ra[OpLoader<T, BM>::N_], rb[OpLoader<T, BN>::N_];
+  T ra[OpLoader<T, BM, BK>::N_], rb[OpLoader<T, BN, BK>::N_];
   // slice t -> (segment, k0); the loaders walk the slices in order
   auto fetch = [&](int t) {
-    int s = (t < nt[0]) ? 0 : 1;
-    int k0 = (s == 0 ? t : t - nt[0]) * BK;
+    int k0;
+    int s = S.locate(t, k0);
     if (s != cur_seg) seg_setup(s);
-    la.load(ra, k0, sg[s], tid);
+    la.load(ra, k0, S.sign[s], tid);
     lb.load(rb, k0, T(1), tid);
     la.advance();
     lb.advance();
@@ -235,8 +240,8 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
 
   if (ntiles > 0) {
     fetch(0);
-    la.template store<LDA>(ra, As[0], tid);
-    lb.template store<LDB>(rb, Bs[0], tid);
+    la.template store_kmajor<LDA>(ra, As[0], tid);
+    lb.template store_kmajor<LDB>(rb, Bs[0], tid);
     if (ntiles > 1) fetch(1);
   }
   __syncthreads();
@@ -258,8 +263,8 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
     if (t + 1 < ntiles) {
       // the other stage was last read in iteration t-1, which every wave
       // finished before the barrier that ended it
-      la.template store<LDA>(ra, As[st ^ 1], tid);
-      lb.template store<LDB>(rb, Bs[st ^ 1], tid);
+      la.template store_kmajor<LDA>(ra, As[st ^ 1], tid);
+      lb.template store_kmajor<LDB>(rb, Bs[st ^ 1], tid);
       if (t + 2 < ntiles) fetch(t + 2);
     }
     __syncthreads();
@@ -288,11 +293,13 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
 
 // ------------------------------------------------------------ BF16x3
 // FP32 operands are split on the LDS store into hi = bf16(x) and
-// lo = bf16(x - hi); the product is hi*hi + hi*lo + lo*hi with FP32
-// accumulation (the lo*lo term, ~2^-16 relative, is dropped).  LDS holds the
-// splits m-major ([m][k], k fastest) so each lane's eight consecutive k
-// values are one 16-byte read.
+// lo = bf16(x - hi) (round to nearest even); the product is
+// hi*hi + hi*lo + lo*hi with FP32 accumulation (the dropped lo*lo term is
+// ~2^-16 relative).  LDS holds the splits row-major ([row][k], k fastest) so
+// each lane's eight consecutive k values are one 16-byte read.  Same
+// two-stage pipeline as the FP64/FP32 kernel, with 32-deep slices.
 constexpr int BK3 = 32;
+constexpr int LD3 = BK3 + 8;  // halves per LDS row (80 bytes: 16-byte aligned rows)
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -303,64 +310,27 @@ __device__ inline unsigned short bf16_bits_rne(float x) {
 }
 __device__ inline float bf16_to_float(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
-template <int BM>
-__device__ inline void load3_a(float (&r)[BM * BK3 / NT], const float *A, const SegView &S, int m0, int k0, int M,
-                               int tid) {
-  const bool kc = (S.sak == 1);
+template <int R>
+__device__ inline void store_split(const float (&v)[R * BK3 / NT], const OpLoader<float, R, BK3> &L,
+                                   unsigned short (*H)[LD3], unsigned short (*Lo)[LD3], int tid) {
 #pragma unroll
-  for (int q = 0; q < BM * BK3 / NT; q++) {
-    int e = tid + q * NT;
-    int row, kk;
-    if (kc) { row = e / BK3; kk = e % BK3; } else { kk = e / BM; row = e % BM; }
-    int m = m0 + row, k = k0 + kk;
-    float v = 0.f;
-    if (m < M && k < S.K) v = A[(long)m * S.sam + (long)k * S.sak];
-    r[q] = v * (float)S.sign;
-  }
-}
-
-template <int BN>
-__device__ inline void load3_b(float (&r)[BN * BK3 / NT], const float *B, const SegView &S, int n0, int k0, int N,
-                               int tid) {
-  const bool kc = (S.sbk == 1);
-#pragma unroll
-  for (int q = 0; q < BN * BK3 / NT; q++) {
-    int e = tid + q * NT;
-    int col, kk;
-    if (kc) { col = e / BK3; kk = e % BK3; } else { kk = e / BN; col = e % BN; }
-    int n = n0 + col, k = k0 + kk;
-    float v = 0.f;
-    if (n < N && k < S.K) v = B[(long)k * S.sbk + (long)n * S.sbn];
-    r[q] = v;
-  }
-}
-
-// Stores split values: Hs/Ls[row][k] (row-major, LD = BK3 + 8 halves).
-template <int BR, int LD>
-__device__ inline void store3(const float (&r)[BR * BK3 / NT], unsigned short (*Hs)[LD], unsigned short (*Ls)[LD],
-                              bool kc, int tid) {
-#pragma unroll
-  for (int q = 0; q < BR * BK3 / NT; q++) {
-    int e = tid + q * NT;
-    int row, kk;
-    if (kc) { row = e / BK3; kk = e % BK3; } else { kk = e / BR; row = e % BR; }
-    float x = r[q];
+  for (int q = 0; q < R * BK3 / NT; q++) {
+    int r, kk;
+    L.rk(L.kc, tid + q * NT, r, kk);
+    float x = v[q];
     unsigned short h = bf16_bits_rne(x);
-    unsigned short l = bf16_bits_rne(x - bf16_to_float(h));
-    Hs[row][kk] = h;
-    Ls[row][kk] = l;
+    H[r][kk] = h;
+    Lo[r][kk] = bf16_bits_rne(x - bf16_to_float(h));
   }
 }
 
 template <int WGM, int WGN, int TM, int TN>
 __global__ __launch_bounds__(NT) void k_gemm_bf16x3(GemmDesc d, const GemmPtrs *__restrict__ ptrs) {
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
-  constexpr int LD = BK3 + 8;
-  constexpr int RA = BM * BK3 / NT, RB = BN * BK3 / NT;
-  __shared__ __attribute__((aligned(16))) unsigned short Ah[BM][LD];
-  __shared__ __attribute__((aligned(16))) unsigned short Al[BM][LD];
-  __shared__ __attribute__((aligned(16))) unsigned short Bh[BN][LD];
-  __shared__ __attribute__((aligned(16))) unsigned short Bl[BN][LD];
+  __shared__ __attribute__((aligned(16))) unsigned short Ah[2][BM][LD3];
+  __shared__ __attribute__((aligned(16))) unsigned short Al[2][BM][LD3];
+  __shared__ __attribute__((aligned(16))) unsigned short Bh[2][BN][LD3];
+  __shared__ __attribute__((aligned(16))) unsigned short Bl[2][BN][LD3];
 
   const int M = d.M, N = d.N;
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
@@ -369,30 +339,21 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(GemmDesc d, const GemmPtrs *
   if (tm >= tiles_m || tn >= tiles_n) return;
   const int m0 = tm * BM, n0 = tn * BN;
   const long z = blockIdx.y;
-
-  SegView S[2];
-  const float *Ap[2];
-  const float *Bp[2];
-  int nt[2] = {0, 0};
-  for (int s = 0; s < 2; s++) {
-    if (s < d.nseg) {
-      S[s] = seg_view(d, ptrs, s, z);
-      if (ptrs) { Ap[s] = (const float *)S[s].A; Bp[s] = (const float *)S[s].B; }
-      else {
-        Ap[s] = (const float *)d.seg[s].A + z * d.seg[s].sAz;
-        Bp[s] = (const float *)d.seg[s].B + z * d.seg[s].sBz;
-      }
-      if (s == 0 && d.ktotal > 0) {
-        long rem = d.ktotal - z * (long)S[0].K;
-        if (rem < S[0].K) S[0].K = rem > 0 ? (int)rem : 0;
-      }
-      nt[s] = (S[s].K + BK3 - 1) / BK3;
-    } else {
-      S[s] = S[0]; Ap[s] = nullptr; Bp[s] = nullptr; nt[s] = 0;
-    }
-  }
-  const int ntiles = nt[0] + nt[1];
   const int tid = threadIdx.x;
+
+  Segs<float, BK3> S;
+  S.init(d, ptrs, z);
+  const int ntiles = S.ntiles;
+  OpLoader<float, BM, BK3> la;
+  OpLoader<float, BN, BK3> lb;
+  int cur_seg = -1;
+  auto seg_setup = [&](int s) {
+    const GemmSeg &g = d.seg[s];
+    la.setup(S.A[s], g.sam, g.sak, m0, M, S.K[s]);
+    lb.setup(S.B[s], g.sbn, g.sbk, n0, N, S.K[s]);
+    cur_seg = s;
+  };
+
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WGN, wn = wave % WGN;
   const int r16 = lane & 15, kq = lane >> 4;
@@ -403,50 +364,49 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(GemmDesc d, const GemmPtrs *
 #pragma unroll
     for (int j = 0; j < TN; j++) acc[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
 
-  float ra[RA], rb[RB];
+  float ra[OpLoader<float, BM, BK3>::N_], rb[OpLoader<float, BN, BK3>::N_];
   auto fetch = [&](int t) {
-    int s = (t < nt[0]) ? 0 : 1;
-    int k0 = (s == 0 ? t : t - nt[0]) * BK3;
-    load3_a<BM>(ra, Ap[s], S[s], m0, k0, M, tid);
-    load3_b<BN>(rb, Bp[s], S[s], n0, k0, N, tid);
-    return s;
+    int k0;
+    int s = S.locate(t, k0);
+    if (s != cur_seg) seg_setup(s);
+    la.load(ra, k0, S.sign[s], tid);
+    lb.load(rb, k0, 1.0f, tid);
+    la.advance();
+    lb.advance();
   };
   if (ntiles > 0) {
-    int s = fetch(0);
-    store3<BM, LD>(ra, Ah, Al, S[s].sak == 1, tid);
-    store3<BN, LD>(rb, Bh, Bl, S[s].sbk == 1, tid);
+    fetch(0);
+    store_split<BM>(ra, la, Ah[0], Al[0], tid);
+    store_split<BN>(rb, lb, Bh[0], Bl[0], tid);
+    if (ntiles > 1) fetch(1);
   }
   __syncthreads();
 
   for (int t = 0; t < ntiles; t++) {
-    const bool more = (t + 1 < ntiles);
-    int sn = 0;
-    if (more) sn = fetch(t + 1);
-    bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+    const int st = t & 1;
+    bf16x8 ah[TM], al[TM];
 #pragma unroll
     for (int i = 0; i < TM; i++) {
       int row = (wm * TM + i) * 16 + r16;
-      ah[i] = *reinterpret_cast<const bf16x8 *>(&Ah[row][8 * kq]);
-      al[i] = *reinterpret_cast<const bf16x8 *>(&Al[row][8 * kq]);
+      ah[i] = *reinterpret_cast<const bf16x8 *>(&Ah[st][row][8 * kq]);
+      al[i] = *reinterpret_cast<const bf16x8 *>(&Al[st][row][8 * kq]);
     }
 #pragma unroll
     for (int j = 0; j < TN; j++) {
       int col = (wn * TN + j) * 16 + r16;
-      bh[j] = *reinterpret_cast<const bf16x8 *>(&Bh[col][8 * kq]);
-      bl[j] = *reinterpret_cast<const bf16x8 *>(&Bl[col][8 * kq]);
-    }
+      bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&Bh[st][col][8 * kq]);
+      bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&Bl[st][col][8 * kq]);
 #pragma unroll
-    for (int i = 0; i < TM; i++)
-#pragma unroll
-      for (int j = 0; j < TN; j++) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      for (int i = 0; i < TM; i++) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
       }
-    __syncthreads();
-    if (more) {
-      store3<BM, LD>(ra, Ah, Al, S[sn].sak == 1, tid);
-      store3<BN, LD>(rb, Bh, Bl, S[sn].sbk == 1, tid);
+    }
+    if (t + 1 < ntiles) {
+      store_split<BM>(ra, la, Ah[st ^ 1], Al[st ^ 1], tid);
+      store_split<BN>(rb, lb, Bh[st ^ 1], Bl[st ^ 1], tid);
+      if (t + 2 < ntiles) fetch(t + 2);
     }
     __syncthreads();
   }
@@ -492,8 +452,7 @@ void launch_t(const GemmDesc &d, const GemmPtrs *ptrs, int prec, hipStream_t st)
   if (tiles > 0x7fffffffL || d.batch > 65535) throw std::runtime_error("ugx: GEMM grid too large");
   dim3 grid((unsigned)tiles, (unsigned)d.batch, 1);
   if (prec == PREC_BF16X3) {
-    if constexpr (BM * BK3 / NT >= 1 && BN * BK3 / NT >= 1)
-      hipLaunchKernelGGL((k_gemm_bf16x3<WGM, WGN, TM, TN>), grid, dim3(NT), 0, st, d, ptrs);
+    hipLaunchKernelGGL((k_gemm_bf16x3<WGM, WGN, TM, TN>), grid, dim3(NT), 0, st, d, ptrs);
   } else {
     hipLaunchKernelGGL((k_gemm<T, WGM, WGN, TM, TN>), grid, dim3(NT), 0, st, d, ptrs);
   }
@@ -544,7 +503,7 @@ const char *gemm_tile_name(int tile) { return (tile >= 0 && tile < kNumTiles) ? 
 
 void launch_gemm(const GemmDesc &d, const GemmPtrs *ptrs, int prec, int tile, hipStream_t st) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
-  if (d.nseg < 1 || d.nseg > 2) throw std::runtime_error("ugx: GEMM needs one or two K segments");
+  if (d.nseg < 1 || d.nseg > MAXSEG) throw std::runtime_error("ugx: GEMM needs one to four K segments");
   if (tile < 0) tile = pick_tile_impl(d);
   if (prec == PREC_FP64) launch_tile<double>(tile, d, ptrs, prec, st);
   else if (prec == PREC_FP32 || prec == PREC_BF16X3) launch_tile<float>(tile, d, ptrs, prec, st);

@@ -39,7 +39,7 @@ struct GemmSeg {
 struct GemmDesc {
   int M, N;
   int nseg;
-  GemmSeg seg[2];
+  GemmSeg seg[4];
   double *C;
   long scm, scn, scm1, scn1, sCz;
   int cdiv, ndiv;
@@ -51,10 +51,10 @@ struct GemmDesc {
 };
 
 // Per-batch pointers for grouped launches (used by the (T) build where each
-// batch entry is one triplet): A/B of both segments and C.
+// batch entry is one triplet): A/B of every segment and C.
 struct GemmPtrs {
-  const void *A[2];
-  const void *B[2];
+  const void *A[4];
+  const void *B[4];
   double *C;
 };
 
