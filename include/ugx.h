@@ -68,7 +68,9 @@ enum {
 enum {
   UGX_PREC_FP64 = 0,   /* FP64 MFMA, FP64 accumulate: the parity form */
   UGX_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulate */
-  UGX_PREC_BF16X3 = 2  /* split-BF16x3 MFMA (hi*hi + hi*lo + lo*hi), FP32 accumulate */
+  UGX_PREC_BF16X3 = 2, /* split-BF16x3 MFMA (hi*hi + hi*lo + lo*hi), FP32 accumulate */
+  UGX_PREC_BF16X3_PAIR = 3 /* ugx_gemm only: BF16x3 on operands already split into BF16 hi/lo
+                              halves (hi array, then lo array), A[m][k] and B[n][k] */
 };
 
 enum { UGX_GUESS_MP2 = 0, UGX_GUESS_USER = 1 };
@@ -98,6 +100,8 @@ typedef struct {
   int guess;               /* UGX_GUESS_MP2 (ccwfn.cc:81-87) or UGX_GUESS_USER */
   int energy_form;         /* UGX_ENERGY_REFERENCE or UGX_ENERGY_FULL */
   int print;               /* PRINT: 0 silent, >=1 iteration table on stdout */
+  int polish_keep_diis;    /* mixed forms: keep the DIIS subspace when switching
+                              to the FP64 polish (default 1); 0 restarts it */
 } ugx_options;
 
 typedef struct {

@@ -103,3 +103,26 @@ def test_triples_large_virtual_space(no, nv):
     W_o = orc.w3(no - 1, 0, 0)
     W_g = c.w3(no - 1, 0, 0)
     assert np.max(np.abs(W_g - W_o)) < 1e-13
+
+
+@pytest.mark.parametrize("no,nv", [(8, 40), (5, 19)])
+def test_triples_low_precision_forms(no, nv):
+    """(T) with the W3 build in FP32 and in BF16x3 (pre-split pair operands;
+    o, v multiples of 8 take the 16-byte load path) against the FP64 build:
+    per-triplet energies to 2e-6 relative of the largest, W3 to 1e-5
+    relative."""
+    announce("triples low precision o=%d v=%d" % (no, nv))
+    c, fock, ints = loaded(no, nv, seed=no * 100 + nv)
+    rng = np.random.default_rng(no + nv)
+    t1 = 0.01 * rng.standard_normal((no, nv))
+    t2 = 0.02 * rng.standard_normal((no, no, nv, nv))
+    t2 = 0.5 * (t2 + t2.transpose(1, 0, 3, 2))
+    c.set_amplitudes(t1, t2)
+    ref = c.triples_per_triplet(ugx.PREC_FP64)
+    W64 = c.w3(no - 1, 1, 0, ugx.PREC_FP64)
+    scale = np.max(np.abs(ref))
+    for prec in (ugx.PREC_FP32, ugx.PREC_BF16X3):
+        got = c.triples_per_triplet(prec)
+        assert np.max(np.abs(got - ref)) < 2e-6 * scale, prec
+        W = c.w3(no - 1, 1, 0, prec)
+        assert np.max(np.abs(W - W64)) < 1e-5 * np.max(np.abs(W64)), prec

@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -98,7 +99,10 @@ struct ugx_ctx {
   int diis_count = 0, diis_next = 0, diis_nvec = 0;
   // (T)
   DBuf tP, tQ, tWbuf, ttrip, tptrs, tE, tPart;
+  int tW_f32 = 0;  // the (T) W buffer holds FP32 values
   DBuf f_tP, f_tQ, f_t2, f_ooov;  // FP32 operand copies for the mixed forms
+  // BF16x3 (T): BF16 hi/lo pair operands, all k contiguous
+  DBuf b_P, b_Q, b_OVt, b_t2, b_T2t, b_T2tq;
   bool tables_ready = false;
   std::vector<int> trip_host;
   // residual of the last ugx_residual, for ugx_intermediates
@@ -434,7 +438,29 @@ void triples_prepare(ugx_ctx *c, int prec) {
     if (nt) check(hipMemcpy(c->ttrip.p, c->trip_host.data(), nt * 3 * 4, hipMemcpyHostToDevice), "triplet table");
     c->tables_ready = true;
   }
-  if (prec != PREC_FP64 && !c->f_tP.p) {
+  if (prec == PREC_BF16X3) {
+    Engine &E = *c->eng;
+    auto pair_of = [&](DBuf &dst, const std::string &spec, const double *src, std::vector<int> dims,
+                       std::vector<int> odims) {
+      long n = 1;
+      for (int x : dims) n *= x;
+      dst.alloc((size_t)n * 4);
+      TRef out = vref(nullptr, odims);
+      out.p = dst.p;
+      out.dtype = 2;
+      E.permute(spec, 1.0, vref(src, dims), 0.0, out);
+    };
+    if (!c->b_P.p) {
+      pair_of(c->b_P, "pxye->pxye", c->tP.d(), {o, v, v, v}, {o, v, v, v});
+      pair_of(c->b_Q, "pyxe->pyxe", c->tQ.d(), {o, v, v, v}, {o, v, v, v});
+      pair_of(c->b_OVt, "qrmz->qrzm", c->ooov.d(), {o, o, o, v}, {o, o, v, o});  // OVt[q][r][z][m]
+    }
+    // amplitude-dependent pairs, rebuilt every call
+    pair_of(c->b_t2, "ijab->ijab", c->t2.d(), {o, o, v, v}, {o, o, v, v});
+    pair_of(c->b_T2t, "pmxy->pxym", c->t2.d(), {o, o, v, v}, {o, v, v, o});   // T2t[p][x][y][m] = t2[p][m][x][y]
+    pair_of(c->b_T2tq, "pmxy->pyxm", c->t2.d(), {o, o, v, v}, {o, v, v, o});  // T2tq[p][y][x][m] = t2[p][m][x][y]
+  }
+  if (prec == PREC_FP32 && !c->f_tP.p) {
     auto conv = [&](DBuf &dst, const double *src, long n) {
       dst.alloc(n * 4);
       PermDesc pd{};
@@ -474,7 +500,7 @@ void triples_build_w(ugx_ctx *c, int prec, long t0, int nb, const std::vector<in
   std::vector<GemmPtrs> tab((size_t)3 * nb);
   for (int b = 0; b < nb; b++) {
     const long i = trip[3 * (t0 + b)], j = trip[3 * (t0 + b) + 1], k = trip[3 * (t0 + b) + 2];
-    double *W = c->tWbuf.d() + (size_t)b * v3;
+    double *W = (double *)((char *)c->tWbuf.p + (size_t)b * v3 * (c->tW_f32 ? 4 : 8));
     GemmPtrs &A = tab[b], &B = tab[(size_t)nb + b], &C = tab[(size_t)2 * nb + b];
     // z = c
     A.A[0] = at(P, i * v3);              A.B[0] = at(T2, (k * o + j) * v2);
@@ -530,8 +556,85 @@ void triples_build_w(ugx_ctx *c, int prec, long t0, int nb, const std::vector<in
     }
     d.alpha = 1.0;
     d.beta = (g == 0) ? 0.0 : 1.0;
+    d.c_f32 = c->tW_f32;
     c->eng->timed("triples_w", fl, [&]() {
       launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)g * nb, prec, tile, c->st);
+    });
+  }
+}
+
+// BF16x3 form of triples_build_w: the same three grouped GEMMs on BF16
+// hi/lo pair operands that all have k contiguous -- the hole terms read
+// T2t[p][x][y][m] = t2_pm^xy, T2tq[p][y][x][m] = t2_pm^xy and
+// OVt[q][r][z][m] = <qr|mz> instead of the row-contiguous t2 and ooov.
+void triples_build_w_pair(ugx_ctx *c, long t0, int nb, const std::vector<int> &trip) {
+  const int o = c->no, v = c->nv;
+  const long v2 = (long)v * v, v3 = v2 * v, ov2o = (long)o * v2, vo = (long)v * o;
+  const unsigned short *P = (const unsigned short *)c->b_P.p, *Q = (const unsigned short *)c->b_Q.p;
+  const unsigned short *T2 = (const unsigned short *)c->b_t2.p, *T2t = (const unsigned short *)c->b_T2t.p;
+  const unsigned short *T2q = (const unsigned short *)c->b_T2tq.p, *OVt = (const unsigned short *)c->b_OVt.p;
+  const long nP = (long)o * v3, nT = (long)o * o * v2, nOV = (long)o * o * o * v;
+  std::vector<GemmPtrs> tab((size_t)3 * nb);
+  for (int b = 0; b < nb; b++) {
+    const long i = trip[3 * (t0 + b)], j = trip[3 * (t0 + b) + 1], k = trip[3 * (t0 + b) + 2];
+    double *W = (double *)((char *)c->tWbuf.p + (size_t)b * v3 * (c->tW_f32 ? 4 : 8));
+    GemmPtrs &A = tab[b], &B = tab[(size_t)nb + b], &C = tab[(size_t)2 * nb + b];
+    A.A[0] = P + i * v3;              A.B[0] = T2 + (k * o + j) * v2;
+    A.A[1] = T2t + i * ov2o;          A.B[1] = OVt + (j * o + k) * vo;
+    A.A[2] = Q + j * v3;              A.B[2] = T2 + (k * o + i) * v2;
+    A.A[3] = T2q + j * ov2o;          A.B[3] = OVt + (i * o + k) * vo;
+    A.C = W;
+    B.A[0] = T2 + (j * o + k) * v2;   B.B[0] = P + i * v3;
+    B.A[1] = OVt + (k * o + j) * vo;  B.B[1] = T2t + i * ov2o;
+    B.A[2] = T2 + (j * o + i) * v2;   B.B[2] = Q + k * v3;
+    B.A[3] = OVt + (i * o + j) * vo;  B.B[3] = T2q + k * ov2o;
+    B.C = W;
+    C.A[0] = T2 + (i * o + k) * v2;   C.B[0] = P + j * v3;
+    C.A[1] = OVt + (k * o + i) * vo;  C.B[1] = T2t + j * ov2o;
+    C.A[2] = T2 + (i * o + j) * v2;   C.B[2] = Q + k * v3;
+    C.A[3] = OVt + (j * o + i) * vo;  C.B[3] = T2q + k * ov2o;
+    C.C = W;
+  }
+  c->tptrs.alloc(tab.size() * sizeof(GemmPtrs));
+  check(hipMemcpyAsync(c->tptrs.p, tab.data(), tab.size() * sizeof(GemmPtrs), hipMemcpyHostToDevice, c->st),
+        "pointer table copy");
+  auto tile_rows_v = [&](int n) { return n <= 32 ? 2 : n <= 64 ? 3 : n <= 80 ? 8 : n <= 128 ? 0 : n <= 160 ? 5 : 0; };
+  auto tile_cols_v = [&](int n) { return n <= 32 ? 2 : n <= 64 ? 4 : n <= 80 ? 10 : n <= 128 ? 0 : n <= 160 ? 9 : 0; };
+  const double fl = 4.0 * (double)nb * (double)v3 * (double)(v + o);
+  for (int g = 0; g < 3; g++) {
+    GemmDesc d{};
+    d.nseg = 4;
+    d.batch = nb;
+    d.vec_ok = (v % 8 == 0 && o % 8 == 0) ? 1 : 0;
+    for (int s = 0; s < 4; s++) {
+      const bool part = (s % 2 == 0);
+      GemmSeg &q = d.seg[s];
+      q.K = part ? v : o;
+      q.sign = part ? 1.0 : -1.0;
+      q.sak = 1; q.sbk = 1;
+      q.sam = part ? v : o;
+      q.sbn = part ? v : o;
+      if (g == 0) {
+        q.loA = part ? nP : nT;
+        q.loB = part ? nT : nOV;
+      } else {
+        q.loA = part ? nT : nOV;
+        q.loB = part ? nP : nT;
+      }
+    }
+    int tile;
+    if (g == 0) { d.M = (int)v2; d.N = v; d.scm = v; d.scn = 1; tile = tile_rows_v(v); }
+    else {
+      d.M = v; d.N = (int)v2;
+      if (g == 1) { d.scm = v; d.ndiv = v; d.scn1 = v2; d.scn = 1; }
+      else { d.scm = v2; d.scn = 1; }
+      tile = tile_cols_v(v);
+    }
+    d.alpha = 1.0;
+    d.beta = (g == 0) ? 0.0 : 1.0;
+    d.c_f32 = c->tW_f32;
+    c->eng->timed("triples_w", fl, [&]() {
+      launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)g * nb, PREC_BF16X3_PRESPLIT, tile, c->st);
     });
   }
 }
@@ -542,7 +645,7 @@ double triples(ugx_ctx *c, int prec, std::vector<double> *etrip_out) {
   const int o = c->no, v = c->nv;
   const long v3 = (long)v * v * v;
   triples_prepare(c, prec);
-  if (prec != PREC_FP64) {
+  if (prec == PREC_FP32) {
     const long n = c->oovv_n();
     c->f_t2.alloc(n * 4);
     PermDesc pd{};
@@ -551,17 +654,22 @@ double triples(ugx_ctx *c, int prec, std::vector<double> *etrip_out) {
     launch_permute(pd, c->st);
   }
   const long nt = (long)c->trip_host.size() / 3;
-  // batch: W of up to ~2 GB
-  long nb = std::max<long>(1, std::min<long>(nt, (2L << 30) / (v3 * 8)));
+  // W is FP32 in the low-precision forms.  Batch: W of up to ~2 GB
+  const int wf = (prec != PREC_FP64) ? 1 : 0;
+  const size_t wes = wf ? 4 : 8;
+  long nb = std::max<long>(1, std::min<long>(nt, (2L << 30) / (v3 * (long)wes)));
   nb = std::min<long>(nb, 4096);
-  c->tWbuf.alloc((size_t)nb * v3 * 8);
+  c->tW_f32 = wf;
+  c->tWbuf.alloc((size_t)nb * v3 * wes);
   c->tE.alloc(std::max<long>(nt, 1) * 8);
-  c->tPart.alloc((size_t)nb * v * 8);
+  const int npairs = triples_energy_npairs(v, wf);
+  c->tPart.alloc((size_t)nb * npairs * 8);
   for (long t0 = 0; t0 < nt; t0 += nb) {
     int b = (int)std::min<long>(nb, nt - t0);
-    triples_build_w(c, prec, t0, b, c->trip_host);
+    if (prec == PREC_BF16X3) triples_build_w_pair(c, t0, b, c->trip_host);
+    else triples_build_w(c, prec, t0, b, c->trip_host);
     c->eng->timed("triples_e", 0.0, [&]() {
-      launch_triples_energy(o, v, b, (const int *)c->ttrip.p + 3 * t0, c->tWbuf.d(), v3, c->eps.d(), c->oovv.d(),
+      launch_triples_energy(o, v, b, (const int *)c->ttrip.p + 3 * t0, c->tWbuf.p, wf, v3, c->eps.d(), c->oovv.d(),
                             c->t1.d(), c->tE.d() + t0, c->tPart.d(), c->st);
     });
   }
@@ -628,6 +736,7 @@ void ugx_default_options(ugx_options *opt) {
   opt->guess = UGX_GUESS_MP2;
   opt->energy_form = UGX_ENERGY_REFERENCE;
   opt->print = 0;
+  opt->polish_keep_diis = 1;
 }
 
 int ugx_create(ugx_ctx **out, int device) {
@@ -647,6 +756,8 @@ int ugx_create(ugx_ctx **out, int device) {
     return UGX_HIP_ERROR;
   }
   c->eng.reset(new Engine(c->st));
+  // tuning knob: LDS stages of the BF16x3 kernel (1 or 2)
+  if (const char *e = std::getenv("UGX_B3_STAGES")) gemm_set_b3_stages(std::atoi(e));
   *out = c;
   return UGX_OK;
 }
@@ -674,6 +785,7 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
     c->have_amps = false;
     c->have_intermediates = false;
     c->f_tP.release(); c->f_tQ.release(); c->f_ooov.release(); c->f_t2.release();
+  c->b_P.release(); c->b_Q.release(); c->b_OVt.release();
     c->diis_err.clear(); c->diis_amp.clear(); c->diis_nvec = 0;
     c->no = no; c->nv = nv;
     const long n = no + nv, o = no, v = nv;
@@ -776,7 +888,8 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
     double eccsd = emp2, rms = 0.0;
     int prec = opt.precision;
     c->diis_err.clear(); c->diis_amp.clear(); c->diis_nvec = 0;
-    int iter, niter = 0, nlow = 0;
+    int iter, niter = 0, nlow = 0, low_stall = 0;
+    double low_best = 1e300;
     for (iter = 1; iter <= opt.maxiter; iter++) {
       amp_save(c);
       residual(c, prec);
@@ -785,11 +898,18 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
       if (prec != PREC_FP64) nlow++;
       if (!std::isfinite(rms)) throw UgxError(UGX_NONFINITE, "non-finite residual norm");
       if (rms < opt.r_convergence && prec == PREC_FP64) break;
-      if (prec != PREC_FP64 && rms < opt.polish_threshold) {
+      // low precision has a residual floor (BF16x3: ~1e-6): switch to the
+      // FP64 polish when the threshold is reached or the residual stalls
+      if (prec != PREC_FP64) {
+        if (rms < 0.8 * low_best) { low_best = rms; low_stall = 0; }
+        else low_stall++;
+      }
+      if (prec != PREC_FP64 && (rms < opt.polish_threshold || low_stall >= 3)) {
         // polish: the remaining iterations run in FP64; the subspace built
-        // from low-precision residuals is dropped
+        // from low-precision residuals is kept (its error vectors are
+        // accurate to the low precision) unless asked otherwise
         prec = PREC_FP64;
-        diis_reset(c);
+        if (!opt.polish_keep_diis) diis_reset(c);
       }
       if (opt.diis) diis_step(c, opt.diis_nvec, opt.diis_policy, opt.diis_min_vecs);
       build_tau(c);
@@ -851,7 +971,7 @@ int ugx_w3(ugx_ctx *c, int i, int j, int k, int precision, double *W3) {
         precision > 2)
       throw UgxError(UGX_BAD_INPUT, "ugx_w3: bad arguments");
     triples_prepare(c, precision);
-    if (precision != PREC_FP64) {
+    if (precision == PREC_FP32) {
       const long n = c->oovv_n();
       c->f_t2.alloc(n * 4);
       PermDesc pd{};
@@ -860,11 +980,20 @@ int ugx_w3(ugx_ctx *c, int i, int j, int k, int precision, double *W3) {
       launch_permute(pd, c->st);
     }
     const long v3 = (long)c->nv * c->nv * c->nv;
+    c->tW_f32 = (precision != PREC_FP64) ? 1 : 0;
     if (c->tWbuf.bytes < (size_t)v3 * 8) c->tWbuf.alloc(v3 * 8);
     std::vector<int> one = {i, j, k};
-    triples_build_w(c, precision, 0, 1, one);
-    check(hipMemcpyAsync(W3, c->tWbuf.p, v3 * 8, hipMemcpyDeviceToHost, c->st), "W3 copy");
-    check(hipStreamSynchronize(c->st), "stream sync");
+    if (precision == PREC_BF16X3) triples_build_w_pair(c, 0, 1, one);
+    else triples_build_w(c, precision, 0, 1, one);
+    if (c->tW_f32) {
+      std::vector<float> wf(v3);
+      check(hipMemcpyAsync(wf.data(), c->tWbuf.p, v3 * 4, hipMemcpyDeviceToHost, c->st), "W3 copy");
+      check(hipStreamSynchronize(c->st), "stream sync");
+      for (long x = 0; x < v3; x++) W3[x] = wf[x];
+    } else {
+      check(hipMemcpyAsync(W3, c->tWbuf.p, v3 * 8, hipMemcpyDeviceToHost, c->st), "W3 copy");
+      check(hipStreamSynchronize(c->st), "stream sync");
+    }
     return UGX_OK;
   });
 }
@@ -953,16 +1082,40 @@ size_t ugx_kernel_stats(ugx_ctx *c, char *buf, size_t buflen) {
 int ugx_gemm(ugx_ctx *c, char opa, char opb, int M, int N, int K, double alpha, const void *A, long lda,
              const void *B, long ldb, double beta, double *C, long ldc, int precision, int tile) {
   return guarded(c, [&]() {
-    if (M < 0 || N < 0 || K < 0 || !A || !B || !C || precision < 0 || precision > 2 || tile >= gemm_num_tiles())
+    if (M < 0 || N < 0 || K < 0 || !A || !B || !C || precision < 0 || precision > 3 || tile >= gemm_num_tiles())
       throw UgxError(UGX_BAD_INPUT, "ugx_gemm: bad arguments");
+    if (precision == UGX_PREC_BF16X3_PAIR && !((opa == 'N' || opa == 'n') && (opb == 'T' || opb == 't')))
+      throw UgxError(UGX_BAD_INPUT, "ugx_gemm: BF16 pair operands must be A[m][k] and B[n][k] (opa N, opb T)");
     GemmDesc d{};
     d.M = M; d.N = N; d.nseg = 1; d.batch = 1;
     GemmSeg &g = d.seg[0];
     g.A = A; g.B = B; g.K = K; g.sign = 1.0;
-    if (opa == 'N' || opa == 'n') { g.sam = lda; g.sak = 1; } else { g.sam = 1; g.sak = lda; }
-    if (opb == 'N' || opb == 'n') { g.sbk = ldb; g.sbn = 1; } else { g.sbk = 1; g.sbn = ldb; }
+    const bool an = (opa == 'N' || opa == 'n'), bn = (opb == 'N' || opb == 'n');
+    if (an) { g.sam = lda; g.sak = 1; } else { g.sam = 1; g.sak = lda; }
+    if (bn) { g.sbk = ldb; g.sbn = 1; } else { g.sbk = 1; g.sbn = ldb; }
     d.C = C; d.scm = ldc; d.scn = 1; d.alpha = alpha; d.beta = beta;
-    launch_gemm(d, nullptr, precision, tile, c->st);
+    if (precision == UGX_PREC_BF16X3_PAIR) {
+      g.loA = (long)M * lda; g.loB = (long)N * ldb;
+      launch_gemm(d, nullptr, PREC_BF16X3_PRESPLIT, tile, c->st);
+    } else if (precision == UGX_PREC_BF16X3) {
+      // FP32 operands in any layout: split into BF16 hi/lo pairs A[m][k],
+      // B[n][k] first
+      DBuf pa, pb;
+      pa.alloc(std::max<size_t>(1, (size_t)M * K) * 4);
+      pb.alloc(std::max<size_t>(1, (size_t)N * K) * 4);
+      PermDesc pd{};
+      pd.rank = 2; pd.in_t = 1; pd.out_t = 2; pd.alpha = 1.0; pd.beta = 0.0;
+      pd.od[0] = M; pd.od[1] = K; pd.istr[0] = g.sam; pd.istr[1] = g.sak; pd.in = A; pd.out = pa.p;
+      if ((long)M * K > 0) launch_permute(pd, c->st);
+      pd.od[0] = N; pd.od[1] = K; pd.istr[0] = g.sbn; pd.istr[1] = g.sbk; pd.in = B; pd.out = pb.p;
+      if ((long)N * K > 0) launch_permute(pd, c->st);
+      g.A = pa.p; g.B = pb.p; g.sam = K; g.sak = 1; g.sbn = K; g.sbk = 1;
+      g.loA = (long)M * K; g.loB = (long)N * K;
+      launch_gemm(d, nullptr, PREC_BF16X3_PRESPLIT, tile, c->st);
+      check(hipStreamSynchronize(c->st), "stream sync");
+    } else {
+      launch_gemm(d, nullptr, precision, tile, c->st);
+    }
     check(hipGetLastError(), "gemm launch");
     return UGX_OK;
   });

@@ -56,6 +56,7 @@ bool is_perm_of(const std::string &x, const std::string &y) {
   return a == b;
 }
 
+// bytes per element: FP64, FP32, BF16 hi/lo pair
 size_t esize(int dtype) { return dtype == 0 ? 8 : 4; }
 
 }  // namespace
@@ -249,13 +250,22 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
           if (c < best.cost) best = Plan{c, sw == 1, m, n, k};
         }
 
-  const int dt = fp64 ? 0 : 1;
+  // operand element type: FP64, FP32, or (BF16x3) pre-split BF16 hi/lo pairs
+  // with k contiguous
+  const bool pair = (prec == PREC_BF16X3);
+  const int dt = fp64 ? 0 : (pair ? 2 : 1);
+  const int kprec = pair ? (int)PREC_BF16X3_PRESPLIT : prec;
   const long Me = extent_of(best.m, ext), Ne = extent_of(best.n, ext), Ke = extent_of(best.k, ext);
   // Operand A as matrix (m, k); B as (k, n).
   auto mat = [&](const TRef &T, const std::string &letters, const std::string &g1, const std::string &g2,
                  long e1, long e2, int slot_id, long &s1, long &s2) -> const void * {
     // returns pointer; strides s1 (along g1 index), s2 (along g2 index)
-    if (letters == g1 + g2 && (fp64 || T.dtype == 1) && T.dtype == dt) { s1 = e2; s2 = 1; return T.p; }
+    if (pair) {
+      s1 = e2; s2 = 1;
+      if (letters == g1 + g2 && T.dtype == 2) return T.p;
+      return prepared(T, letters, g1 + g2, 2, slot_id);
+    }
+    if (letters == g1 + g2 && T.dtype == dt) { s1 = e2; s2 = 1; return T.p; }
     if (letters == g2 + g1 && T.dtype == dt) { s1 = 1; s2 = e1; return T.p; }
     // permute/convert into g1+g2 unless the g2+g1 layout only needs a conversion
     if (letters == g2 + g1) {
@@ -271,15 +281,16 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
   long skb, snb;
   const void *Bp = mat(B, sp.b, best.n, best.k, Ne, Ke, 1, snb, skb);
   sbk = skb; sbn = snb;
+  const long loA = pair ? Me * Ke : 0, loB = pair ? Ne * Ke : 0;
 
   std::string rows = best.swap ? best.n : best.m, cols = best.swap ? best.m : best.n;
   const long Re = best.swap ? Ne : Me, Ce = best.swap ? Me : Ne;
   GemmSeg g{};
   if (!best.swap) {
-    g.A = Ap; g.sam = sam; g.sak = sak; g.B = Bp; g.sbk = sbk; g.sbn = sbn;
+    g.A = Ap; g.sam = sam; g.sak = sak; g.B = Bp; g.sbk = sbk; g.sbn = sbn; g.loA = loA; g.loB = loB;
   } else {
     // rows = n: A' (n,k) = B(k,n);  B' (k,m) = A(m,k)
-    g.A = Bp; g.sam = sbn; g.sak = sbk; g.B = Ap; g.sbk = sak; g.sbn = sam;
+    g.A = Bp; g.sam = sbn; g.sak = sbk; g.B = Ap; g.sbk = sak; g.sbn = sam; g.loA = loB; g.loB = loA;
   }
   g.K = (int)Ke; g.sign = 1.0;
   d.M = (int)Re; d.N = (int)Ce; d.nseg = 1; d.seg[0] = g; d.batch = 1;
@@ -301,6 +312,7 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
       S = std::min<long>(S, 4096);
       if (S > 1) {
         long chunk = (Ke + S - 1) / S;
+        chunk = (chunk + 31) / 32 * 32;  // whole slices (and 16-byte pair loads) per chunk
         S = (Ke + chunk - 1) / chunk;
         double *W = (double *)slot(3, (size_t)S * Re * Ce * 8);
         GemmDesc d2 = d;
@@ -310,7 +322,7 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
         d2.seg[0].sBz = chunk * d.seg[0].sbk;
         d2.ktotal = Ke;
         d2.C = W; d2.scm = Ce; d2.scn = 1; d2.scm1 = 0; d2.sCz = Re * Ce; d2.alpha = 1.0; d2.beta = 0.0;
-        launch_gemm(d2, nullptr, prec, tile, st_);
+        launch_gemm(d2, nullptr, kprec, tile, st_);
         if (direct) {
           launch_splitk_reduce((int)S, (int)Re, (int)Ce, W, alpha, beta, (double *)C.p, Ce, 1, st_);
         } else {
@@ -332,11 +344,11 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
   }
   if (direct) {
     d.C = (double *)C.p; d.scm = Ce; d.scn = 1; d.scm1 = 0; d.alpha = alpha; d.beta = beta;
-    launch_gemm(d, nullptr, prec, -1, st_);
+    launch_gemm(d, nullptr, kprec, -1, st_);
   } else {
     double *tmp = (double *)slot(2, (size_t)Re * Ce * 8);
     d.C = tmp; d.scm = Ce; d.scn = 1; d.scm1 = 0; d.alpha = 1.0; d.beta = 0.0;
-    launch_gemm(d, nullptr, prec, -1, st_);
+    launch_gemm(d, nullptr, kprec, -1, st_);
     TRef T = vref(tmp, {});
     for (char ch : rows + cols) T.dims.push_back(ext.at(ch));
     permute(rows + cols + "->" + sp.c, alpha, T, beta, C);

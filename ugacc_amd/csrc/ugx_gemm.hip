@@ -34,6 +34,7 @@
 //           (A: lane l -> A[l%16][8(l/16) .. 8(l/16)+7], B likewise along k)
 #include "ugx_internal.h"
 
+#include <cstdint>
 #include <stdexcept>
 
 namespace ugx {
@@ -124,14 +125,14 @@ struct Segs {
   const T *A[MAXSEG];
   const T *B[MAXSEG];
   int K[MAXSEG], nt[MAXSEG];
-  T sign[MAXSEG];
+  float sign[MAXSEG];  // +-1 (a float whatever T is: T may be an unsigned BF16 carrier)
   int n, ntiles;
   __device__ inline void init(const GemmDesc &d, const GemmPtrs *ptrs, long z) {
     n = d.nseg;
     ntiles = 0;
 #pragma unroll
     for (int s = 0; s < MAXSEG; s++) {
-      A[s] = nullptr; B[s] = nullptr; K[s] = 0; nt[s] = 0; sign[s] = T(1);
+      A[s] = nullptr; B[s] = nullptr; K[s] = 0; nt[s] = 0; sign[s] = 1.0f;
       if (s < n) {
         const GemmSeg &g = d.seg[s];
         if (ptrs) { A[s] = (const T *)ptrs[z].A[s]; B[s] = (const T *)ptrs[z].B[s]; }
@@ -143,7 +144,7 @@ struct Segs {
         }
         K[s] = k;
         nt[s] = (k + KB - 1) / KB;
-        sign[s] = T(g.sign);
+        sign[s] = g.sign < 0 ? -1.0f : 1.0f;
         ntiles += nt[s];
       }
     }
@@ -232,7 +233,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
     int k0;
     int s = S.locate(t, k0);
     if (s != cur_seg) seg_setup(s);
-    la.load(ra, k0, S.sign[s], tid);
+    la.load(ra, k0, T(S.sign[s]), tid);
     lb.load(rb, k0, T(1), tid);
     la.advance();
     lb.advance();
@@ -270,7 +271,10 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
     __syncthreads();
   }
 
-  double *C = ptrs ? ptrs[z].C : d.C + z * d.sCz;
+  // C is FP64, or FP32 when d.c_f32 (the (T) W buffer of the low-precision
+  // forms); per-batch offsets count elements of C's type
+  void *Cv = ptrs ? (void *)ptrs[z].C
+                  : (d.c_f32 ? (void *)((float *)d.C + z * d.sCz) : (void *)(d.C + z * d.sCz));
   const double alpha = d.alpha, beta = d.beta;
   const int cdiv = d.cdiv, ndiv = d.ndiv;
 #pragma unroll
@@ -285,19 +289,23 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
           long addr = (cdiv > 0 ? (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm : (long)m * d.scm) +
                       (ndiv > 0 ? (long)(n / ndiv) * d.scn1 + (long)(n % ndiv) * d.scn : (long)n * d.scn);
           double v = alpha * (double)acc[i][j][r];
-          if (beta != 0.0) v += beta * C[addr];
-          C[addr] = v;
+          if (d.c_f32) {
+            float *C = (float *)Cv;
+            if (beta != 0.0) v += beta * (double)C[addr];
+            C[addr] = (float)v;
+          } else {
+            double *C = (double *)Cv;
+            if (beta != 0.0) v += beta * C[addr];
+            C[addr] = v;
+          }
         }
       }
 }
 
 // ------------------------------------------------------------ BF16x3
-// FP32 operands are split on the LDS store into hi = bf16(x) and
-// lo = bf16(x - hi) (round to nearest even); the product is
-// hi*hi + hi*lo + lo*hi with FP32 accumulation (the dropped lo*lo term is
-// ~2^-16 relative).  LDS holds the splits row-major ([row][k], k fastest) so
-// each lane's eight consecutive k values are one 16-byte read.  Same
-// two-stage pipeline as the FP64/FP32 kernel, with 32-deep slices.
+// Each FP32 operand x is carried as hi = bf16(x) and lo = bf16(x - hi)
+// (round to nearest even); the product is hi*hi + hi*lo + lo*hi with FP32
+// accumulation (the dropped lo*lo term is ~2^-16 relative).
 constexpr int BK3 = 32;
 constexpr int LD3 = BK3 + 8;  // halves per LDS row (80 bytes: 16-byte aligned rows)
 
@@ -310,27 +318,79 @@ __device__ inline unsigned short bf16_bits_rne(float x) {
 }
 __device__ inline float bf16_to_float(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
+// ------------------------------------------------- BF16x3, pre-split operands
+// Operands arrive as BF16 hi/lo pairs with k contiguous (A[m][k], B[n][k]):
+// no conversion in the kernel, 16-byte global loads of eight k values when
+// every stride and K are multiples of 8 (VEC), element loads otherwise.
+// Two LDS stages, one barrier per 32-deep slice.
 template <int R>
-__device__ inline void store_split(const float (&v)[R * BK3 / NT], const OpLoader<float, R, BK3> &L,
-                                   unsigned short (*H)[LD3], unsigned short (*Lo)[LD3], int tid) {
-#pragma unroll
-  for (int q = 0; q < R * BK3 / NT; q++) {
-    int r, kk;
-    L.rk(L.kc, tid + q * NT, r, kk);
-    float x = v[q];
-    unsigned short h = bf16_bits_rne(x);
-    H[r][kk] = h;
-    Lo[r][kk] = bf16_bits_rne(x - bf16_to_float(h));
+struct PairLoader {
+  static constexpr int CH = R * (BK3 / 8);             // 16-byte chunks per array per slice
+  static constexpr int NQ = (CH + NT - 1) / NT;        // chunks per thread
+  const unsigned short *hi;                            // row0, k0 of the current slice
+  long lo, srow;
+  int rows_left, K;
+  __device__ inline void setup(const unsigned short *base, long lo_, long srow_, int row0, int rows, int K_) {
+    hi = base + (long)row0 * srow_;
+    lo = lo_; srow = srow_; rows_left = rows - row0; K = K_;
   }
-}
+  template <bool VEC>
+  __device__ inline void load(uint4 (&h)[NQ], uint4 (&l)[NQ], int k0, int tid) const {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      int c = tid + q * NT;
+      int r = c >> 2, kc = (c & 3) * 8;
+      uint4 vh = {0u, 0u, 0u, 0u}, vl = {0u, 0u, 0u, 0u};
+      if (c < CH && r < rows_left) {
+        const unsigned short *ph = hi + (long)r * srow + kc;
+        if (VEC) {
+          if (k0 + kc < K) {
+            vh = *reinterpret_cast<const uint4 *>(ph);
+            vl = *reinterpret_cast<const uint4 *>(ph + lo);
+          }
+        } else {
+          unsigned short th[8], tl[8];
+#pragma unroll
+          for (int e = 0; e < 8; e++) {
+            bool ok = (k0 + kc + e < K);
+            th[e] = ok ? ph[e] : (unsigned short)0;
+            tl[e] = ok ? ph[e + lo] : (unsigned short)0;
+          }
+          vh.x = th[0] | ((unsigned)th[1] << 16); vh.y = th[2] | ((unsigned)th[3] << 16);
+          vh.z = th[4] | ((unsigned)th[5] << 16); vh.w = th[6] | ((unsigned)th[7] << 16);
+          vl.x = tl[0] | ((unsigned)tl[1] << 16); vl.y = tl[2] | ((unsigned)tl[3] << 16);
+          vl.z = tl[4] | ((unsigned)tl[5] << 16); vl.w = tl[6] | ((unsigned)tl[7] << 16);
+        }
+      }
+      h[q] = vh;
+      l[q] = vl;
+    }
+  }
+  __device__ inline void advance() { hi += BK3; }
+  __device__ inline void store(const uint4 (&h)[NQ], const uint4 (&l)[NQ], unsigned short (*H)[LD3],
+                               unsigned short (*L)[LD3], int tid) const {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      int c = tid + q * NT;
+      if (c < CH) {
+        int r = c >> 2, kc = (c & 3) * 8;
+        *reinterpret_cast<uint4 *>(&H[r][kc]) = h[q];
+        *reinterpret_cast<uint4 *>(&L[r][kc]) = l[q];
+      }
+    }
+  }
+};
 
-template <int WGM, int WGN, int TM, int TN>
-__global__ __launch_bounds__(NT) void k_gemm_bf16x3(GemmDesc d, const GemmPtrs *__restrict__ ptrs) {
+template <int WGM, int WGN, int TM, int TN, bool VEC, int STAGES>
+__global__ __launch_bounds__(NT) void k_gemm_b3p(GemmDesc d, const GemmPtrs *__restrict__ ptrs) {
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
-  __shared__ __attribute__((aligned(16))) unsigned short Ah[2][BM][LD3];
-  __shared__ __attribute__((aligned(16))) unsigned short Al[2][BM][LD3];
-  __shared__ __attribute__((aligned(16))) unsigned short Bh[2][BN][LD3];
-  __shared__ __attribute__((aligned(16))) unsigned short Bl[2][BN][LD3];
+  // STAGES = 2: ping-pong LDS, one barrier per slice.  STAGES = 1: half the
+  // LDS (two workgroups per CU, so one's epilogue overlaps the other's
+  // MFMAs), two barriers per slice.
+  __shared__ __attribute__((aligned(16))) unsigned short Ah[STAGES][BM][LD3];
+  __shared__ __attribute__((aligned(16))) unsigned short Al[STAGES][BM][LD3];
+  __shared__ __attribute__((aligned(16))) unsigned short Bh[STAGES][BN][LD3];
+  __shared__ __attribute__((aligned(16))) unsigned short Bl[STAGES][BN][LD3];
 
   const int M = d.M, N = d.N;
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
@@ -341,16 +401,16 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(GemmDesc d, const GemmPtrs *
   const long z = blockIdx.y;
   const int tid = threadIdx.x;
 
-  Segs<float, BK3> S;
+  Segs<unsigned short, BK3> S;
   S.init(d, ptrs, z);
   const int ntiles = S.ntiles;
-  OpLoader<float, BM, BK3> la;
-  OpLoader<float, BN, BK3> lb;
+  PairLoader<BM> la;
+  PairLoader<BN> lb;
   int cur_seg = -1;
   auto seg_setup = [&](int s) {
     const GemmSeg &g = d.seg[s];
-    la.setup(S.A[s], g.sam, g.sak, m0, M, S.K[s]);
-    lb.setup(S.B[s], g.sbn, g.sbk, n0, N, S.K[s]);
+    la.setup(S.A[s], g.loA, g.sam, m0, M, S.K[s]);
+    lb.setup(S.B[s], g.loB, g.sbn, n0, N, S.K[s]);
     cur_seg = s;
   };
 
@@ -364,26 +424,34 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(GemmDesc d, const GemmPtrs *
 #pragma unroll
     for (int j = 0; j < TN; j++) acc[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
 
-  float ra[OpLoader<float, BM, BK3>::N_], rb[OpLoader<float, BN, BK3>::N_];
+  uint4 rah[PairLoader<BM>::NQ], ral[PairLoader<BM>::NQ], rbh[PairLoader<BN>::NQ], rbl[PairLoader<BN>::NQ];
+  // a negative segment sign flips the sign bits of its B halves (exact)
   auto fetch = [&](int t) {
     int k0;
     int s = S.locate(t, k0);
     if (s != cur_seg) seg_setup(s);
-    la.load(ra, k0, S.sign[s], tid);
-    lb.load(rb, k0, 1.0f, tid);
+    la.template load<VEC>(rah, ral, k0, tid);
+    lb.template load<VEC>(rbh, rbl, k0, tid);
+    if (S.sign[s] < 0) {
+#pragma unroll
+      for (int q = 0; q < PairLoader<BN>::NQ; q++) {
+        rbh[q].x ^= 0x80008000u; rbh[q].y ^= 0x80008000u; rbh[q].z ^= 0x80008000u; rbh[q].w ^= 0x80008000u;
+        rbl[q].x ^= 0x80008000u; rbl[q].y ^= 0x80008000u; rbl[q].z ^= 0x80008000u; rbl[q].w ^= 0x80008000u;
+      }
+    }
     la.advance();
     lb.advance();
   };
   if (ntiles > 0) {
     fetch(0);
-    store_split<BM>(ra, la, Ah[0], Al[0], tid);
-    store_split<BN>(rb, lb, Bh[0], Bl[0], tid);
+    la.store(rah, ral, Ah[0], Al[0], tid);
+    lb.store(rbh, rbl, Bh[0], Bl[0], tid);
     if (ntiles > 1) fetch(1);
   }
   __syncthreads();
 
   for (int t = 0; t < ntiles; t++) {
-    const int st = t & 1;
+    const int st = (STAGES == 2) ? (t & 1) : 0;
     bf16x8 ah[TM], al[TM];
 #pragma unroll
     for (int i = 0; i < TM; i++) {
@@ -403,15 +471,28 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(GemmDesc d, const GemmPtrs *
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
       }
     }
-    if (t + 1 < ntiles) {
-      store_split<BM>(ra, la, Ah[st ^ 1], Al[st ^ 1], tid);
-      store_split<BN>(rb, lb, Bh[st ^ 1], Bl[st ^ 1], tid);
-      if (t + 2 < ntiles) fetch(t + 2);
+    if (STAGES == 2) {
+      if (t + 1 < ntiles) {
+        la.store(rah, ral, Ah[st ^ 1], Al[st ^ 1], tid);
+        lb.store(rbh, rbl, Bh[st ^ 1], Bl[st ^ 1], tid);
+        if (t + 2 < ntiles) fetch(t + 2);
+      }
+      __syncthreads();
+    } else {
+      __syncthreads();
+      if (t + 1 < ntiles) {
+        la.store(rah, ral, Ah[0], Al[0], tid);
+        lb.store(rbh, rbl, Bh[0], Bl[0], tid);
+        if (t + 2 < ntiles) fetch(t + 2);
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 
-  double *C = ptrs ? ptrs[z].C : d.C + z * d.sCz;
+  // C is FP64, or FP32 when d.c_f32 (the (T) W buffer of the low-precision
+  // forms); per-batch offsets count elements of C's type
+  void *Cv = ptrs ? (void *)ptrs[z].C
+                  : (d.c_f32 ? (void *)((float *)d.C + z * d.sCz) : (void *)(d.C + z * d.sCz));
   const double alpha = d.alpha, beta = d.beta;
   const int cdiv = d.cdiv, ndiv = d.ndiv;
 #pragma unroll
@@ -426,8 +507,15 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(GemmDesc d, const GemmPtrs *
           long addr = (cdiv > 0 ? (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm : (long)m * d.scm) +
                       (ndiv > 0 ? (long)(n / ndiv) * d.scn1 + (long)(n % ndiv) * d.scn : (long)n * d.scn);
           double v = alpha * (double)acc[i][j][r];
-          if (beta != 0.0) v += beta * C[addr];
-          C[addr] = v;
+          if (d.c_f32) {
+            float *C = (float *)Cv;
+            if (beta != 0.0) v += beta * (double)C[addr];
+            C[addr] = (float)v;
+          } else {
+            double *C = (double *)Cv;
+            if (beta != 0.0) v += beta * C[addr];
+            C[addr] = v;
+          }
         }
       }
 }
@@ -445,14 +533,39 @@ const TileShape kTiles[] = {
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
+// Pre-split BF16x3 operands allow 16-byte loads when every segment's row
+// strides, K and hi/lo offsets are multiples of 8 and the bases are 16-byte
+// aligned (checked on the pointer table's entries by the caller via
+// d.flags bit 0).
+// LDS stages of the pre-split BF16x3 kernel (tunable, see gemm_set_b3_stages)
+int g_b3_stages = 1;
+
+bool b3_vectorizable(const GemmDesc &d) {
+  for (int s = 0; s < d.nseg; s++) {
+    const GemmSeg &g = d.seg[s];
+    if (g.sak != 1 || g.sbk != 1) throw std::runtime_error("ugx: pre-split BF16x3 needs k-contiguous operands");
+    if (g.sam % 8 || g.sbn % 8 || g.K % 8 || g.loA % 8 || g.loB % 8 || g.sAz % 8 || g.sBz % 8) return false;
+    if (g.A && (((uintptr_t)g.A) & 15)) return false;
+    if (g.B && (((uintptr_t)g.B) & 15)) return false;
+  }
+  return true;
+}
+
 template <typename T, int WGM, int WGN, int TM, int TN>
 void launch_t(const GemmDesc &d, const GemmPtrs *ptrs, int prec, hipStream_t st) {
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
   long tiles = (long)((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
   if (tiles > 0x7fffffffL || d.batch > 65535) throw std::runtime_error("ugx: GEMM grid too large");
   dim3 grid((unsigned)tiles, (unsigned)d.batch, 1);
-  if (prec == PREC_BF16X3) {
-    hipLaunchKernelGGL((k_gemm_bf16x3<WGM, WGN, TM, TN>), grid, dim3(NT), 0, st, d, ptrs);
+  if (prec == PREC_BF16X3_PRESPLIT) {
+    bool vec = b3_vectorizable(d) && (ptrs == nullptr || d.vec_ok);
+    if (g_b3_stages == 1) {
+      if (vec) hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, true, 1>), grid, dim3(NT), 0, st, d, ptrs);
+      else hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, false, 1>), grid, dim3(NT), 0, st, d, ptrs);
+    } else {
+      if (vec) hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, true, 2>), grid, dim3(NT), 0, st, d, ptrs);
+      else hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, false, 2>), grid, dim3(NT), 0, st, d, ptrs);
+    }
   } else {
     hipLaunchKernelGGL((k_gemm<T, WGM, WGN, TM, TN>), grid, dim3(NT), 0, st, d, ptrs);
   }
@@ -493,6 +606,7 @@ int pick_tile_impl(const GemmDesc &d) {
 }  // namespace
 
 int gemm_num_tiles() { return kNumTiles; }
+void gemm_set_b3_stages(int s) { g_b3_stages = (s == 1) ? 1 : 2; }
 int gemm_pick_tile(const GemmDesc &d) { return pick_tile_impl(d); }
 void gemm_tile_dims(int tile, int &bm, int &bn) {
   const TileShape &t = kTiles[tile];
@@ -506,7 +620,9 @@ void launch_gemm(const GemmDesc &d, const GemmPtrs *ptrs, int prec, int tile, hi
   if (d.nseg < 1 || d.nseg > MAXSEG) throw std::runtime_error("ugx: GEMM needs one to four K segments");
   if (tile < 0) tile = pick_tile_impl(d);
   if (prec == PREC_FP64) launch_tile<double>(tile, d, ptrs, prec, st);
-  else if (prec == PREC_FP32 || prec == PREC_BF16X3) launch_tile<float>(tile, d, ptrs, prec, st);
+  else if (prec == PREC_FP32 || prec == PREC_BF16X3_PRESPLIT) launch_tile<float>(tile, d, ptrs, prec, st);
+  else if (prec == PREC_BF16X3)
+    throw std::runtime_error("ugx: BF16x3 contractions take pre-split operands (PREC_BF16X3_PRESPLIT)");
   else throw std::runtime_error("ugx: unknown contraction precision");
 }
 

@@ -15,6 +15,10 @@ enum Prec : int {
   PREC_BF16X3 = 2,  // FP32 operands split hi+lo into BF16, three
                     // v_mfma_f32_16x16x32_bf16 products (hi*hi + hi*lo +
                     // lo*hi), FP32 accumulate
+  // Same arithmetic on operands that are already split into BF16 hi/lo
+  // arrays with k contiguous (A[m][k], B[n][k]); the form the engine and the
+  // (T) build use.  Internal launch mode, not an ABI precision.
+  PREC_BF16X3_PRESPLIT = 3,
 };
 
 // One K segment of a contraction C(m,n) += sign * sum_k A(m,k) B(k,n).
@@ -27,6 +31,9 @@ struct GemmSeg {
   long sAz, sBz;  // per-batch offsets (elements) when no pointer table
   int K;
   double sign;
+  // PREC_BF16X3_PRESPLIT: element offsets from the hi halves to the lo
+  // halves of A and B (operands stored as two BF16 arrays, k contiguous)
+  long loA, loB;
 };
 
 // C(m,n) lives at C[(m / cdiv) * scm1 + (m % cdiv) * scm
@@ -48,6 +55,11 @@ struct GemmDesc {
   // split-K: when > 0, batch entry z covers k in [z*K, min((z+1)*K, ktotal))
   // of segment 0 (its sAz/sBz advance by K*sak / K*sbk)
   long ktotal;
+  // pointer-table launches of pre-split BF16x3 operands: the caller
+  // asserts every table pointer is 16-byte aligned (enables 16-byte loads)
+  int vec_ok;
+  // C is float (its pointers hold float data despite their type)
+  int c_f32;
 };
 
 // Per-batch pointers for grouped launches (used by the (T) build where each
@@ -65,6 +77,7 @@ void launch_gemm(const GemmDesc &d, const GemmPtrs *ptrs, int prec, int tile, hi
 int gemm_num_tiles();
 const char *gemm_tile_name(int tile);
 int gemm_pick_tile(const GemmDesc &d);
+void gemm_set_b3_stages(int stages);  // 1 or 2 LDS stages in the BF16x3 kernel
 void gemm_tile_dims(int tile, int &bm, int &bn);
 // C[m*scm + n*scn] = alpha * sum_z W[z][m][n] + beta * C  (split-K reduce)
 void launch_splitk_reduce(int S, int M, int N, const double *W, double alpha, double beta, double *C, long scm,
@@ -73,7 +86,9 @@ void launch_splitk_reduce(int S, int M, int N, const double *W, double alpha, do
 // ------------------------------------------------------------ streaming
 // out[idx] = alpha * in[perm(idx)] + beta * out[idx] over a row-major output
 // of rank <= 6 with dims od[]; istr[] gives, per output dim, the stride of
-// the input element.  in/out element types: 0 = double, 1 = float.
+// the input element.  Element types: 0 = double, 1 = float (in or out),
+// 2 = split BF16 pair (out only, beta = 0): n halves hi = bf16(x), then n
+// halves lo = bf16(x - hi).
 struct PermDesc {
   int rank;
   int od[6];
@@ -119,8 +134,10 @@ int energy_nparts(int no, int nv);
 // ------------------------------------------------------------ (T)
 // Energy epilogue over a batch of triplets: for each triplet (i,j,k) with W
 // (v^3, [a][b][c]) already built, sums the (T) bracket over a>=b>=c.
-// etrip receives ntrip values; part is scratch of ntrip*nv doubles.
-void launch_triples_energy(int no, int nv, int ntrip, const int *trip_ijk, const double *W, long wstride,
+// W is FP64, or FP32 when w_f32.  etrip receives ntrip values; part is
+// scratch of ntrip * triples_energy_npairs(nv, w_f32) doubles.
+int triples_energy_npairs(int nv, int w_f32);
+void launch_triples_energy(int no, int nv, int ntrip, const int *trip_ijk, const void *W, int w_f32, long wstride,
                            const double *eps, const double *oovv, const double *t1, double *etrip, double *part,
                            hipStream_t st);
 

@@ -46,11 +46,41 @@ struct PermArgs {
   const void *in;
   void *out;
   double alpha, beta;
+  long n;  // total elements (offset of the lo half of a BF16 pair output)
+};
+
+// Output element policies: FP64, FP32, and the split BF16 pair (hi at
+// out[idx], lo at out[n + idx], hi = bf16(x), lo = bf16(x - hi), both round
+// to nearest even; accumulation is not offered for the pair).
+struct OutD {
+  typedef double tile_t;
+  __device__ static double get(const PermArgs &p, long i) { return ((const double *)p.out)[i]; }
+  __device__ static void put(const PermArgs &p, long i, double v) { ((double *)p.out)[i] = v; }
+};
+struct OutF {
+  typedef float tile_t;
+  __device__ static double get(const PermArgs &p, long i) { return ((const float *)p.out)[i]; }
+  __device__ static void put(const PermArgs &p, long i, double v) { ((float *)p.out)[i] = (float)v; }
+};
+__device__ inline unsigned short bf16_rne(float x) {
+  unsigned int u = __float_as_uint(x);
+  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+struct OutB3 {
+  typedef float tile_t;
+  __device__ static double get(const PermArgs &, long) { return 0.0; }
+  __device__ static void put(const PermArgs &p, long i, double v) {
+    float x = (float)v;
+    unsigned short h = bf16_rne(x);
+    float hf = __uint_as_float(((unsigned int)h) << 16);
+    ((unsigned short *)p.out)[i] = h;
+    ((unsigned short *)p.out)[p.n + i] = bf16_rne(x - hf);
+  }
 };
 
 // Linear form: used when the output's fastest index is also the input's
 // unit-stride index (or the tensor is small).
-template <typename TI, typename TO>
+template <typename TI, class OW>
 __global__ void k_perm_linear(PermArgs p, long n) {
   for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n; idx += (long)gridDim.x * NT) {
     long r = idx, off = 0;
@@ -60,16 +90,17 @@ __global__ void k_perm_linear(PermArgs p, long n) {
       off += c * p.istr[d];
     }
     double v = p.alpha * (double)((const TI *)p.in)[off];
-    if (p.beta != 0.0) v += p.beta * (double)((TO *)p.out)[idx];
-    ((TO *)p.out)[idx] = (TO)v;
+    if (p.beta != 0.0) v += p.beta * OW::get(p, idx);
+    OW::put(p, idx, v);
   }
 }
 
 // Tiled form: q = the output dim with input stride 1, l = the last output
 // dim.  A 64x64 (q,l) tile is read along q and written along l.
-template <typename TI, typename TO>
+template <typename TI, class OW>
 __global__ void k_perm_tiled(PermArgs p, int q, long nrest, int tq, int tl) {
-  __shared__ TO tile[64][65];
+  typedef typename OW::tile_t TT;
+  __shared__ TT tile[64][65];
   const int L = p.rank - 1;
   const int Dq = p.od[q], Dl = p.od[L];
   long ostr[6];
@@ -96,8 +127,8 @@ __global__ void k_perm_tiled(PermArgs p, int q, long nrest, int tq, int tl) {
     for (int e = threadIdx.x; e < 64 * 64; e += NT) {
       int lq = e & 63, ll = e >> 6;
       int gq = q0 + lq, gl = l0 + ll;
-      TO v = 0;
-      if (gq < Dq && gl < Dl) v = (TO)(p.alpha * (double)((const TI *)p.in)[ibase + gq + gl * p.istr[L]]);
+      TT v = 0;
+      if (gq < Dq && gl < Dl) v = (TT)(p.alpha * (double)((const TI *)p.in)[ibase + gq + gl * p.istr[L]]);
       tile[lq][ll] = v;
     }
     __syncthreads();
@@ -107,18 +138,17 @@ __global__ void k_perm_tiled(PermArgs p, int q, long nrest, int tq, int tl) {
       if (gq < Dq && gl < Dl) {
         long o = obase + gq * ostr[q] + gl;
         double v = (double)tile[lq][ll];
-        if (p.beta != 0.0) v += p.beta * (double)((TO *)p.out)[o];
-        ((TO *)p.out)[o] = (TO)v;
+        if (p.beta != 0.0) v += p.beta * OW::get(p, o);
+        OW::put(p, o, v);
       }
     }
     __syncthreads();
   }
 }
 
-template <typename TI, typename TO>
+template <typename TI, class OW>
 void perm_launch(const PermArgs &a, hipStream_t st) {
-  long n = 1;
-  for (int d = 0; d < a.rank; d++) n *= a.od[d];
+  long n = a.n;
   if (n <= 0) return;
   const int L = a.rank - 1;
   int q = -1;
@@ -126,12 +156,12 @@ void perm_launch(const PermArgs &a, hipStream_t st) {
     for (int d = 0; d < L; d++)
       if (a.istr[d] == 1 && a.od[d] > 1) q = d;
   if (q < 0 || n < 4096) {
-    hipLaunchKernelGGL((k_perm_linear<TI, TO>), dim3(nblocks(n)), dim3(NT), 0, st, a, n);
+    hipLaunchKernelGGL((k_perm_linear<TI, OW>), dim3(nblocks(n)), dim3(NT), 0, st, a, n);
   } else {
     int tq = (a.od[q] + 63) / 64, tl = (a.od[L] + 63) / 64;
     long nrest = n / ((long)a.od[q] * a.od[L]);
     long nb = (long)tq * tl * nrest;
-    hipLaunchKernelGGL((k_perm_tiled<TI, TO>), dim3((unsigned)(nb < 65536 ? nb : 65536)), dim3(NT), 0, st, a, q,
+    hipLaunchKernelGGL((k_perm_tiled<TI, OW>), dim3((unsigned)(nb < 65536 ? nb : 65536)), dim3(NT), 0, st, a, q,
                        nrest, tq, tl);
   }
 }
@@ -317,12 +347,23 @@ void launch_permute(const PermDesc &p, hipStream_t st) {
   if (p.rank < 1 || p.rank > 6) throw std::runtime_error("ugx: permute rank out of range");
   PermArgs a;
   a.rank = p.rank;
-  for (int d = 0; d < 6; d++) { a.od[d] = d < p.rank ? p.od[d] : 1; a.istr[d] = d < p.rank ? p.istr[d] : 0; }
+  a.n = 1;
+  for (int d = 0; d < 6; d++) {
+    a.od[d] = d < p.rank ? p.od[d] : 1;
+    a.istr[d] = d < p.rank ? p.istr[d] : 0;
+    a.n *= a.od[d];
+  }
   a.in = p.in; a.out = p.out; a.alpha = p.alpha; a.beta = p.beta;
-  if (p.in_t == 0 && p.out_t == 0) perm_launch<double, double>(a, st);
-  else if (p.in_t == 0 && p.out_t == 1) perm_launch<double, float>(a, st);
-  else if (p.in_t == 1 && p.out_t == 0) perm_launch<float, double>(a, st);
-  else perm_launch<float, float>(a, st);
+  if (p.out_t == 2 && p.beta != 0.0) throw std::runtime_error("ugx: BF16 pair output cannot accumulate");
+  if (p.in_t == 0) {
+    if (p.out_t == 0) perm_launch<double, OutD>(a, st);
+    else if (p.out_t == 1) perm_launch<double, OutF>(a, st);
+    else perm_launch<double, OutB3>(a, st);
+  } else {
+    if (p.out_t == 0) perm_launch<float, OutD>(a, st);
+    else if (p.out_t == 1) perm_launch<float, OutF>(a, st);
+    else perm_launch<float, OutB3>(a, st);
+  }
 }
 
 void launch_fill(long n, double v, double *y, hipStream_t st) {

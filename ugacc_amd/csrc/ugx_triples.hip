@@ -1,8 +1,7 @@
 // (T) energy epilogue, ccwfn.cc:795-852.
 //
 // For a batch of i>=j>=k triplets whose connected W3 (v^3, [a][b][c]) has
-// been built by the six permuted MFMA contractions, each workgroup takes one
-// (triplet, a) row and sums the bracket over b<=a, c<=b:
+// been built by the MFMA contractions, the bracket is summed over a>=b>=c:
 //   V_abc = (W_abc + <ij|ab> t1_kc + <ik|ac> t1_jb + <jk|bc> t1_ia)
 //           / (1 + d_ab + d_ac + d_bc)
 //   X = sum over the six permutations of W*V,  Y = V_abc + V_bca + V_cab,
@@ -10,8 +9,16 @@
 //   E += [(Y - 2Z)(W_abc + W_bca + W_cab) + (Z - 2Y)(W_acb + W_bac + W_cba)
 //         + 3X] (2 - d_ij - d_ik - d_jk) / D_ijk^abc
 // V is formed on the fly from W, the oovv block and t1, so only W is stored.
-// Sums are FP64; the per-(triplet, a) partials are reduced per triplet in a
-// fixed order, so the result is bitwise repeatable.
+//
+// Every point needs W at all six index permutations.  A workgroup owns one
+// triplet and a pair of E-wide index blocks A >= B, and walks the blocks
+// C <= B: for each (A, B, C) it stages the six permuted E^3 sub-cubes
+// W[A][B][C], W[A][C][B], W[B][A][C], W[B][C][A], W[C][A][B], W[C][B][A]
+// into LDS, each read along its own contiguous index so the loads are
+// coalesced, and every W element is read from memory about once.
+// W is FP64 (FP64 form) or FP32 (low-precision forms); all sums are FP64 and
+// the per-(triplet, block pair) partials are reduced per triplet in a fixed
+// order, so the result is bitwise repeatable.
 #include "ugx_internal.h"
 
 namespace ugx {
@@ -32,70 +39,153 @@ __device__ inline double block_sum(double v) {
   return s;
 }
 
+// Block pair index p -> (A, B) with B <= A.
+__device__ inline void pair_of(int p, int &A, int &B) {
+  A = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
+  while ((A + 1) * (A + 2) / 2 <= p) A++;
+  while (A * (A + 1) / 2 > p) A--;
+  B = p - A * (A + 1) / 2;
+}
+
+template <typename WT, int E>
 __global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, const int *__restrict__ trip,
-                                                       const double *__restrict__ Wall, long wstride,
+                                                       const WT *__restrict__ Wall, long wstride,
                                                        const double *__restrict__ eps,
                                                        const double *__restrict__ oovv,
-                                                       const double *__restrict__ t1, double *__restrict__ part) {
-  const int t = blockIdx.x;
-  const int a = blockIdx.y;
+                                                       const double *__restrict__ t1, double *__restrict__ part,
+                                                       int npairs) {
+  constexpr int E3 = E * E * E, E2 = E * E;
+  // padded cube layout idx(a,b,c) = a*SA + b*SB + c: the strides are odd, so
+  // the stores along a or b (and the reads along c) do not collide in banks
+  constexpr int SB = E + 1, SA = E * (E + 1) + 1;
+  __shared__ WT S[6][E * SA];
+  // disconnected-term operands of the three blocks X, Y in {A, B, C}:
+  // O[s][X][Y][x][y] = <pq|xy> for the slices s = (ij), (ik), (jk);
+  // T1[s][X][x] = t1 of k, j, i (the amplitude that multiplies slice s)
+  __shared__ double O[3][3][3][E2];
+  __shared__ double T1[3][3][E];
+  __shared__ double EPS[3][E];
+  const int t = blockIdx.y;
+  const int pidx = blockIdx.x;
+  int Ab, Bb;
+  pair_of(pidx, Ab, Bb);
   const int i = trip[3 * t], j = trip[3 * t + 1], k = trip[3 * t + 2];
-  const double *W = Wall + (long)t * wstride;
+  const WT *W = Wall + (long)t * wstride;
   const long v = nv, v2 = (long)nv * nv;
-  const double *Oij = oovv + ((long)i * no + j) * v2;  // <ij|xy>
-  const double *Oik = oovv + ((long)i * no + k) * v2;
-  const double *Ojk = oovv + ((long)j * no + k) * v2;
-  const double *t1i = t1 + (long)i * nv, *t1j = t1 + (long)j * nv, *t1k = t1 + (long)k * nv;
+  const double *Osl[3] = {oovv + ((long)i * no + j) * v2, oovv + ((long)i * no + k) * v2,
+                          oovv + ((long)j * no + k) * v2};
+  const double *T1sl[3] = {t1 + (long)k * nv, t1 + (long)j * nv, t1 + (long)i * nv};
   const double eo = eps[i] + eps[j] + eps[k];
   const double fac = 2.0 - ((i == j) + (i == k) + (j == k));
-  const double ea = eps[no + a];
+  int X0[3] = {Ab * E, Bb * E, 0};  // block origins of A, B, C (C set per step)
 
-  auto Wx = [&](int x, int y, int z) { return W[x * v2 + y * v + z]; };
-  // V at (x,y,z) for x,y,z in the roles of a,b,c of the reference
-  auto Vx = [&](int x, int y, int z, double w) {
-    double d = 1.0 + (x == y) + (x == z) + (y == z);
-    return (w + Oij[x * v + y] * t1k[z] + Oik[x * v + z] * t1j[y] + Ojk[y * v + z] * t1i[x]) / d;
+  // V at a permutation (x, y, z) of the point, given as block slots (bx, by,
+  // bz) in {0: A, 1: B, 2: C} and local offsets (lx, ly, lz)
+  auto Vx = [&](int bx, int lx, int by, int ly, int bz, int lz, double w, double inv) {
+    return (w + O[0][bx][by][lx * E + ly] * T1[0][bz][lz] + O[1][bx][bz][lx * E + lz] * T1[1][by][ly] +
+            O[2][by][bz][ly * E + lz] * T1[2][bx][lx]) * inv;
+  };
+  auto stage_pairs = [&](bool all) {
+    // O blocks for pairs touching C (or all pairs on the first step) and the
+    // t1 / eps entries of the blocks
+    for (int e = threadIdx.x; e < 3 * 9 * E2; e += NT) {
+      int s = e / (9 * E2), r = e % (9 * E2), pr = r / E2, xy = r % E2;
+      int bx = pr / 3, by = pr % 3;
+      if (!all && bx != 2 && by != 2) continue;
+      int gx = X0[bx] + xy / E, gy = X0[by] + xy % E;
+      O[s][bx][by][xy] = (gx < nv && gy < nv) ? Osl[s][gx * v + gy] : 0.0;
+    }
+    for (int e = threadIdx.x; e < 3 * 3 * E; e += NT) {
+      int s = e / (3 * E), r = e % (3 * E), bx = r / E, lx = r % E;
+      if (!all && bx != 2) continue;
+      int gx = X0[bx] + lx;
+      T1[s][bx][lx] = gx < nv ? T1sl[s][gx] : 0.0;
+      if (s == 0) EPS[bx][lx] = gx < nv ? eps[no + gx] : 0.0;
+    }
   };
 
   double s = 0.0;
-  const int npair = (a + 1) * (a + 2) / 2;  // (b, c) with c <= b <= a
-  for (int p = threadIdx.x; p < npair; p += NT) {
-    // p -> (b, c): b = floor((sqrt(8p+1)-1)/2)
-    int b = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
-    while ((b + 1) * (b + 2) / 2 <= p) b++;
-    while (b * (b + 1) / 2 > p) b--;
-    int c = p - b * (b + 1) / 2;
-    double wabc = Wx(a, b, c), wacb = Wx(a, c, b), wbac = Wx(b, a, c);
-    double wbca = Wx(b, c, a), wcab = Wx(c, a, b), wcba = Wx(c, b, a);
-    double vabc = Vx(a, b, c, wabc), vacb = Vx(a, c, b, wacb), vbac = Vx(b, a, c, wbac);
-    double vbca = Vx(b, c, a, wbca), vcab = Vx(c, a, b, wcab), vcba = Vx(c, b, a, wcba);
-    double X = wabc * vabc + wacb * vacb + wbac * vbac + wbca * vbca + wcab * vcab + wcba * vcba;
-    double Y = vabc + vbca + vcab;
-    double Z = vacb + vbac + vcba;
-    double denom = eo - (ea + eps[no + b] + eps[no + c]);
-    s += ((Y - 2.0 * Z) * (wabc + wbca + wcab) + (Z - 2.0 * Y) * (wacb + wbac + wcba) + 3.0 * X) * fac / denom;
+  const int A0 = X0[0], B0 = X0[1];
+  for (int Cb = 0; Cb <= Bb; Cb++) {
+    const int C0 = Cb * E;
+    X0[2] = C0;
+    auto ld = [&](int x, int y, int z) -> WT {
+      return (x < nv && y < nv && z < nv) ? W[x * v2 + y * v + z] : WT(0);
+    };
+    auto at = [](int a, int b, int c) { return a * SA + b * SB + c; };
+    // Stage the six sub-cubes, S[q][at(a,b,c)] = W at permutation q of
+    // (A0+a, B0+b, C0+c).  Thread e walks the cube with its fastest index
+    // equal to the sub-cube's contiguous index in W.
+    for (int e = threadIdx.x; e < E3; e += NT) {
+      int f = e % E, m = (e / E) % E, sl = e / (E * E);  // fastest, middle, slowest
+      S[0][at(sl, m, f)] = ld(A0 + sl, B0 + m, C0 + f);   // W[a][b][c], c contiguous
+      S[1][at(sl, f, m)] = ld(A0 + sl, C0 + m, B0 + f);   // W[a][c][b], b contiguous
+      S[2][at(m, sl, f)] = ld(B0 + sl, A0 + m, C0 + f);   // W[b][a][c], c contiguous
+      S[3][at(f, sl, m)] = ld(B0 + sl, C0 + m, A0 + f);   // W[b][c][a], a contiguous
+      S[4][at(m, f, sl)] = ld(C0 + sl, A0 + m, B0 + f);   // W[c][a][b], b contiguous
+      S[5][at(f, m, sl)] = ld(C0 + sl, B0 + m, A0 + f);   // W[c][b][a], a contiguous
+    }
+    stage_pairs(Cb == 0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < E3; e += NT) {
+      int cc = e % E, bb = (e / E) % E, aa = e / (E * E);
+      int a = A0 + aa, b = B0 + bb, c = C0 + cc;
+      if (a >= nv || b > a || c > b) continue;
+      const int x = at(aa, bb, cc);
+      // 1 / (1 + d_ab + d_ac + d_bc): 1, 1/2 or 1/4, exact
+      const double inv = (a == b && b == c) ? 0.25 : ((a == b || b == c || a == c) ? 0.5 : 1.0);
+      double wabc = (double)S[0][x], wacb = (double)S[1][x], wbac = (double)S[2][x];
+      double wbca = (double)S[3][x], wcab = (double)S[4][x], wcba = (double)S[5][x];
+      double vabc = Vx(0, aa, 1, bb, 2, cc, wabc, inv), vacb = Vx(0, aa, 2, cc, 1, bb, wacb, inv);
+      double vbac = Vx(1, bb, 0, aa, 2, cc, wbac, inv), vbca = Vx(1, bb, 2, cc, 0, aa, wbca, inv);
+      double vcab = Vx(2, cc, 0, aa, 1, bb, wcab, inv), vcba = Vx(2, cc, 1, bb, 0, aa, wcba, inv);
+      double X = wabc * vabc + wacb * vacb + wbac * vbac + wbca * vbca + wcab * vcab + wcba * vcba;
+      double Y = vabc + vbca + vcab;
+      double Z = vacb + vbac + vcba;
+      double denom = eo - (EPS[0][aa] + EPS[1][bb] + EPS[2][cc]);
+      s += ((Y - 2.0 * Z) * (wabc + wbca + wcab) + (Z - 2.0 * Y) * (wacb + wbac + wcba) + 3.0 * X) * fac / denom;
+    }
+    __syncthreads();
   }
   double tot = block_sum(s);
-  if (threadIdx.x == 0) part[(long)t * nv + a] = tot;
+  if (threadIdx.x == 0) part[(long)t * npairs + pidx] = tot;
 }
 
-__global__ void k_triples_reduce(int nv, int ntrip, const double *part, double *etrip) {
+__global__ void k_triples_reduce(int npairs, int ntrip, const double *part, double *etrip) {
   int t = blockIdx.x * NT + threadIdx.x;
   if (t >= ntrip) return;
   double s = 0.0;
-  for (int a = 0; a < nv; a++) s += part[(long)t * nv + a];
+  for (int p = 0; p < npairs; p++) s += part[(long)t * npairs + p];
   etrip[t] = s;
+}
+
+template <typename WT, int E>
+void launch_e(int no, int nv, int ntrip, const int *trip_ijk, const void *W, long wstride, const double *eps,
+              const double *oovv, const double *t1, double *etrip, double *part, hipStream_t st) {
+  int nb = (nv + E - 1) / E;
+  int npairs = nb * (nb + 1) / 2;
+  hipLaunchKernelGGL((k_triples_energy<WT, E>), dim3(npairs, ntrip), dim3(NT), 0, st, no, nv, trip_ijk,
+                     (const WT *)W, wstride, eps, oovv, t1, part, npairs);
+  hipLaunchKernelGGL(k_triples_reduce, dim3((ntrip + NT - 1) / NT), dim3(NT), 0, st, npairs, ntrip, part, etrip);
 }
 
 }  // namespace
 
-void launch_triples_energy(int no, int nv, int ntrip, const int *trip_ijk, const double *W, long wstride,
+// Sub-cube edge 12: six padded 12^3 cubes (45 KB FP32, 90 KB FP64) plus
+// 31 KB of staged <pq|xy> blocks.
+int triples_energy_npairs(int nv, int w_f32) {
+  int E = 12;
+  (void)w_f32;
+  int nb = (nv + E - 1) / E;
+  return nb * (nb + 1) / 2;
+}
+
+void launch_triples_energy(int no, int nv, int ntrip, const int *trip_ijk, const void *W, int w_f32, long wstride,
                            const double *eps, const double *oovv, const double *t1, double *etrip, double *part,
                            hipStream_t st) {
   if (ntrip <= 0) return;
-  hipLaunchKernelGGL(k_triples_energy, dim3(ntrip, nv), dim3(NT), 0, st, no, nv, trip_ijk, W, wstride, eps, oovv,
-                     t1, part);
-  hipLaunchKernelGGL(k_triples_reduce, dim3((ntrip + NT - 1) / NT), dim3(NT), 0, st, nv, ntrip, part, etrip);
+  if (w_f32) launch_e<float, 12>(no, nv, ntrip, trip_ijk, W, wstride, eps, oovv, t1, etrip, part, st);
+  else launch_e<double, 12>(no, nv, ntrip, trip_ijk, W, wstride, eps, oovv, t1, etrip, part, st);
 }
 
 }  // namespace ugx

@@ -40,7 +40,7 @@ class Options(ctypes.Structure):
         ("r_convergence", ctypes.c_double), ("ooc", ctypes.c_int), ("diis_nvec", ctypes.c_int),
         ("diis_min_vecs", ctypes.c_int), ("diis_policy", ctypes.c_int), ("precision", ctypes.c_int),
         ("polish_threshold", ctypes.c_double), ("guess", ctypes.c_int), ("energy_form", ctypes.c_int),
-        ("print", ctypes.c_int),
+        ("print", ctypes.c_int), ("polish_keep_diis", ctypes.c_int),
     ]
 
 
