@@ -69,11 +69,14 @@ enum {
   UGX_PREC_FP64 = 0,   /* FP64 MFMA, FP64 accumulate: the parity form */
   UGX_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulate */
   UGX_PREC_BF16X3 = 2, /* split-BF16x3 MFMA (hi*hi + hi*lo + lo*hi), FP32 accumulate */
-  UGX_PREC_BF16X3_PAIR = 3 /* ugx_gemm only: BF16x3 on operands already split into BF16 hi/lo
-                              halves (hi array, then lo array), A[m][k] and B[n][k] */
+  UGX_PREC_BF16X3_PAIR = 3, /* ugx_gemm only: BF16x3 on operands already split into BF16 hi/lo
+                               halves (hi array, then lo array), A[m][k] and B[n][k] */
+  UGX_PREC_BF16 = 4          /* ugx_mlp_forward only: BF16 MFMA inference, FP32 accumulate */
 };
 
-enum { UGX_GUESS_MP2 = 0, UGX_GUESS_USER = 1 };
+/* Initial amplitudes: MP2 (ccwfn.cc:75-87), the caller's (ugx_set_amplitudes),
+ * or the MP2 doubles corrected by the network set with ugx_set_guess_network. */
+enum { UGX_GUESS_MP2 = 0, UGX_GUESS_USER = 1, UGX_GUESS_MLP = 2 };
 
 /* Energy expression: the reference's (ccwfn.cc:272 assigns instead of
  * accumulating, so only the last (i,a) term of 2 f_ia t1_ia survives; zero
@@ -81,6 +84,7 @@ enum { UGX_GUESS_MP2 = 0, UGX_GUESS_USER = 1 };
 enum { UGX_ENERGY_REFERENCE = 0, UGX_ENERGY_FULL = 1 };
 
 typedef struct ugx_ctx ugx_ctx;
+typedef struct ugx_mlp ugx_mlp;
 
 typedef struct {
   int wfn_ccsd_t;          /* WFN: 0 = "CCSD", 1 = "CCSD_T" (ugacc.cc:60) */
@@ -97,7 +101,7 @@ typedef struct {
   int precision;           /* UGX_PREC_* of the low-precision phase */
   double polish_threshold; /* mixed forms: switch to FP64 once ||R|| drops
                               below this (e.g. 1e-5); ignored for FP64 */
-  int guess;               /* UGX_GUESS_MP2 (ccwfn.cc:81-87) or UGX_GUESS_USER */
+  int guess;               /* UGX_GUESS_MP2 (ccwfn.cc:81-87), UGX_GUESS_USER or UGX_GUESS_MLP */
   int energy_form;         /* UGX_ENERGY_REFERENCE or UGX_ENERGY_FULL */
   int print;               /* PRINT: 0 silent, >=1 iteration table on stdout */
   int polish_keep_diis;    /* mixed forms: keep the DIIS subspace when switching
@@ -181,6 +185,47 @@ int ugx_gemm(ugx_ctx *ctx, char opa, char opb, int M, int N, int K, double alpha
 
 /* Synchronises the context's stream. */
 int ugx_synchronize(ugx_ctx *ctx);
+
+/* ---------------------------------------------------------------------
+ * Learned T2 guess.  Replaces the MP2 doubles of CCWfn::CCWfn
+ * (ccwfn.cc:81-87) by t2 = t2(MP2) + y_scale (f(x_ijab) + f(x_jiba)) / 2,
+ * where f is a network 8 -> 128 -> 128 -> 128 -> 1 with GELU (tanh form)
+ * after the hidden layers and x are the amplitude's features (computed by
+ * ugx_features): asinh(1e3 t_MP2), log(-D), asinh(1e2 <ij|ab>),
+ * asinh(1e2 <ij|ba>), f_ii, f_jj, f_aa, f_bb.  The reference has no such
+ * guess; the fixed point of the iterations does not depend on it.
+ * Parameters are FP32 in the order W1[128][8], b1, W2[128][128], b2,
+ * W3[128][128], b3, W4[128], b4 (layers z = W h + b); the features are
+ * standardised as (x - shift) / scale before the first layer.
+ * --------------------------------------------------------------------- */
+
+/* Number of FP32 parameters (34305). */
+long ugx_mlp_num_params(void);
+/* New network on a device, initialised from seed (U(+-1/sqrt(fan_in))). */
+int ugx_mlp_create(ugx_mlp **net, int device, unsigned long long seed);
+void ugx_mlp_destroy(ugx_mlp *net);
+const char *ugx_mlp_last_error(const ugx_mlp *net);
+int ugx_mlp_set_params(ugx_mlp *net, const float *params);
+int ugx_mlp_get_params(ugx_mlp *net, float *params);
+/* Feature standardisation (8 shifts, 8 positive scales) and output scale. */
+int ugx_mlp_set_scaling(ugx_mlp *net, const float *x_shift, const float *x_scale, double y_scale);
+/* y[r] = f(X[r]) for raw feature rows X[n][8]; precision UGX_PREC_FP32 (the
+ * training forward, n <= 65536) or UGX_PREC_BF16 (the fused inference
+ * kernel the guess uses). */
+int ugx_mlp_forward(ugx_mlp *net, const float *X, long n, int precision, float *y);
+/* Loss (1/n) sum (f(X[r]) - y[r])^2 and its gradient with respect to the
+ * parameters (FP32 forward and backward on the device), n <= 65536. */
+int ugx_mlp_loss_grad(ugx_mlp *net, const float *X, const float *y, long n, double *loss, float *grad);
+/* Adam (beta1 0.9, beta2 0.999, eps 1e-8) over `epochs` passes of shuffled
+ * mini-batches (seeded) of the n rows; epoch_loss (may be NULL) receives
+ * the mean batch loss of each epoch. */
+int ugx_mlp_train(ugx_mlp *net, const float *X, const float *y, long n, int epochs, int batch, double lr,
+                  unsigned long long seed, double *epoch_loss);
+/* Raw features X[r][8] of the amplitudes idx[r] (flattened ijab, b fastest;
+ * idx NULL: the first n) of the loaded Hamiltonian. */
+int ugx_features(ugx_ctx *ctx, long n, const long *idx, float *X);
+/* Network used by UGX_GUESS_MLP (same device; it must outlive its use). */
+int ugx_set_guess_network(ugx_ctx *ctx, const ugx_mlp *net);
 
 #ifdef __cplusplus
 }

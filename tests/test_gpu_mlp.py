@@ -1,0 +1,144 @@
+"""Learned T2 guess on the GPU, through the C ABI, against the FP64 oracle
+(oracle/mlp.py): features, FP32 forward and gradient, the Adam step, the
+fused BF16 inference kernel, training, and the guess inside ugx_ccsd."""
+import numpy as np
+import pytest
+
+from oracle import mlp
+from tests.gpu_util import announce, loaded
+from ugacc_amd import ugx
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(n, seed):
+    """Feature-like rows (the spread of real features) and targets."""
+    rng = np.random.default_rng(seed)
+    X = np.column_stack([rng.normal(0, 2, n), rng.uniform(0, 4, n), rng.normal(0, 1, n), rng.normal(0, 1, n),
+                         rng.uniform(-20, 0, n), rng.uniform(-20, 0, n), rng.uniform(0, 4, n),
+                         rng.uniform(0, 4, n)]).astype(np.float32)
+    y = (np.tanh(X[:, 0]) * 0.5 + 0.1 * X[:, 2] * X[:, 3] + rng.normal(0, 0.05, n)).astype(np.float32)
+    return X, y
+
+
+def _net(seed=5):
+    net = ugx.Network(0, seed=seed)
+    X, _ = _data(4096, 99)
+    net.set_scaling(X.mean(0), X.std(0), 1e-3)
+    return net
+
+
+def test_features_match_oracle():
+    announce("mlp features o=4 v=9")
+    c, fock, ints = loaded(4, 9, seed=41)
+    eps = np.diag(fock)
+    ref = mlp.features(4, 9, eps, ints[:4, :4, 4:, 4:])
+    X = c.features()
+    np.testing.assert_allclose(X, ref.astype(np.float32), rtol=2e-6, atol=1e-6)
+    idx = np.array([5, 0, 1295, 17])
+    np.testing.assert_allclose(c.features(idx=idx), X[idx], rtol=0, atol=0)
+
+
+def test_forward_and_gradient_match_oracle():
+    announce("mlp fp32 forward + gradient, n=2000")
+    net = _net()
+    X, y = _data(2000, 7)
+    p = net.params().astype(np.float64)
+    f_ref = mlp.forward(p, X, net.shift.astype(np.float64), net.scale.astype(np.float64))
+    f = net.forward(X, ugx.PREC_FP32)
+    np.testing.assert_allclose(f, f_ref, rtol=1e-4, atol=1e-5 * np.abs(f_ref).max())
+    loss, g = net.loss_grad(X, y)
+    loss_ref, g_ref = mlp.loss_grad(p, X, y.astype(np.float64), net.shift.astype(np.float64),
+                                    net.scale.astype(np.float64))
+    assert loss == pytest.approx(loss_ref, rel=1e-5)
+    for name, (lo, hi) in _blocks().items():
+        err = np.abs(g[lo:hi] - g_ref[lo:hi]).max()
+        assert err <= 2e-4 * np.abs(g_ref[lo:hi]).max() + 1e-7, (name, err)
+
+
+def _blocks():
+    out, off = {}, 0
+    for name, shape in mlp.layout():
+        n = int(np.prod(shape))
+        out[name] = (off, off + n)
+        off += n
+    return out
+
+
+def test_bf16_inference_kernel_matches_fp32():
+    announce("mlp bf16 fused inference, n=100003")
+    net = _net(seed=11)
+    X, _ = _data(100003, 8)  # not a multiple of 16: ragged last tile
+    f32 = mlp.forward(net.params().astype(np.float64), X, net.shift.astype(np.float64),
+                      net.scale.astype(np.float64))
+    fb = net.forward(X, ugx.PREC_BF16)
+    err = np.abs(fb - f32)
+    assert err.max() <= 0.03 * np.abs(f32).max(), err.max()
+    assert np.sqrt(np.mean(err ** 2)) <= 0.01 * np.sqrt(np.mean(f32 ** 2))
+
+
+def test_adam_step_matches_oracle():
+    announce("mlp one Adam step, n=512")
+    net = _net(seed=3)
+    X, y = _data(512, 4)
+    p0 = net.params().astype(np.float64)
+    _, g = mlp.loss_grad(p0, X, y.astype(np.float64), net.shift.astype(np.float64), net.scale.astype(np.float64))
+    net.train(X, y, epochs=1, batch=512, lr=1e-3, seed=1)
+    p1 = net.params()
+    ref, _, _ = mlp.adam_step(p0, g, np.zeros_like(p0), np.zeros_like(p0), 1, lr=1e-3)
+    # the first step is lr * sign(g); components with |g| near eps may differ
+    ok = np.abs(g) > 1e-5
+    np.testing.assert_allclose(p1[ok], ref[ok], atol=2e-7)
+
+
+def test_training_reduces_loss_and_is_repeatable():
+    announce("mlp training 20 epochs, n=16384")
+    X, y = _data(16384, 21)
+    a = _net(seed=8)
+    h = a.train(X, y, epochs=20, batch=1024, lr=2e-3, seed=4)
+    assert h[-1] < 0.25 * h[0], h
+    b = _net(seed=8)
+    h2 = b.train(X, y, epochs=20, batch=1024, lr=2e-3, seed=4)
+    np.testing.assert_array_equal(h, h2)
+    np.testing.assert_array_equal(a.params(), b.params())
+
+
+def test_guess_inside_ccsd_matches_oracle():
+    announce("mlp guess inside ugx_ccsd o=4 v=12")
+    c, fock, ints = loaded(4, 12, seed=43)
+    net = _net(seed=9)
+    c.set_guess_network(net)
+    c.ccsd(maxiter=0, guess=ugx.GUESS_MLP, wfn_ccsd_t=0)
+    _, t2 = c.amplitudes()
+    ref = mlp.guess(net.params().astype(np.float64), 4, 12, np.diag(fock), ints[:4, :4, 4:, 4:],
+                    net.shift.astype(np.float64), net.scale.astype(np.float64), net.y_scale)
+    mp2 = ints[:4, :4, 4:, 4:] / _denom(np.diag(fock), 4, 12)
+    corr, corr_ref = t2 - mp2, ref - mp2
+    assert np.abs(corr - corr_ref).max() <= 0.03 * np.abs(corr_ref).max()
+    np.testing.assert_allclose(t2, t2.transpose(1, 0, 3, 2), atol=1e-15)
+    c.set_guess_network(None)
+
+
+def _denom(eps, no, nv):
+    eo, ev = eps[:no], eps[no:no + nv]
+    return eo[:, None, None, None] + eo[None, :, None, None] - ev[None, None, :, None] - ev[None, None, None, :]
+
+
+def test_guess_keeps_the_fixed_point():
+    announce("mlp guess: same CCSD energy as the MP2 guess, o=5 v=19")
+    c, fock, ints = loaded(5, 19, seed=44)
+    ra = c.ccsd(r_convergence=1e-11, wfn_ccsd_t=0)
+    net = _net(seed=10)
+    c.set_guess_network(net)
+    rb = c.ccsd(r_convergence=1e-11, wfn_ccsd_t=0, guess=ugx.GUESS_MLP)
+    c.set_guess_network(None)
+    assert ra["converged"] and rb["converged"]
+    assert abs(ra["e_ccsd_final"] - rb["e_ccsd_final"]) < 1e-10
+    assert rb["e_mp2"] != ra["e_mp2"]
+
+
+def test_guess_without_network_is_an_error():
+    c, _, _ = loaded(3, 6, seed=45)
+    c.set_guess_network(None)
+    with pytest.raises(ugx.UgxError, match="UGX_GUESS_MLP"):
+        c.ccsd(guess=ugx.GUESS_MLP, wfn_ccsd_t=0, maxiter=1)

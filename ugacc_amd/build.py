@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libugx.so")
-SOURCES = ["ugx_gemm.hip", "ugx_stream.hip", "ugx_triples.hip", "ugx_engine.cpp", "ugx_ccsd.cpp"]
+SOURCES = ["ugx_gemm.hip", "ugx_stream.hip", "ugx_triples.hip", "ugx_mlp.hip", "ugx_engine.cpp", "ugx_ccsd.cpp",
+           "ugx_mlp_api.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

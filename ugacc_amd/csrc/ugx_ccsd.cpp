@@ -28,50 +28,11 @@
 
 #include "../../include/ugx.h"
 #include "ugx_engine.h"
+#include "ugx_host.h"
 #include "ugx_internal.h"
 
 using namespace ugx;
 
-namespace {
-
-struct UgxError : std::runtime_error {
-  int code;
-  UgxError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-struct DBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-  DBuf() = default;
-  DBuf(const DBuf &) = delete;
-  DBuf &operator=(const DBuf &) = delete;
-  ~DBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  void alloc(size_t b) {
-    if (bytes >= b && p) return;
-    release();
-    if (b == 0) b = 8;
-    hipError_t e = hipMalloc(&p, b);
-    if (e != hipSuccess) {
-      p = nullptr;
-      (void)hipGetLastError();
-      throw UgxError(UGX_OOM, "device allocation of " + std::to_string(b) + " bytes failed");
-    }
-    bytes = b;
-  }
-  double *d() const { return (double *)p; }
-  float *f() const { return (float *)p; }
-};
-
-void check(hipError_t e, const char *what) {
-  if (e != hipSuccess) throw UgxError(UGX_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-}  // namespace
 
 struct ugx_ctx {
   int device = 0;
@@ -105,6 +66,9 @@ struct ugx_ctx {
   DBuf b_P, b_Q, b_OVt, b_t2, b_T2t, b_T2tq;
   bool tables_ready = false;
   std::vector<int> trip_host;
+  // learned guess: network (owned by the caller) and its output f(x_ijab)
+  const ugx_mlp *mlp = nullptr;
+  DBuf G;
   // residual of the last ugx_residual, for ugx_intermediates
   bool have_intermediates = false;
   double last_flops = 0.0;
@@ -405,6 +369,21 @@ void alloc_work(ugx_ctx *c) {
 
 void guess_mp2(ugx_ctx *c) {
   launch_guess(c->no, c->nv, c->fock.d(), c->oovv.d(), c->t1.d(), c->t2.d(), c->st);
+}
+
+// MP2 guess, then t2 += y_scale (f(x_ijab) + f(x_jiba)) / 2 with the network
+// evaluated by the fused BF16 kernel on features formed from oovv and eps
+void guess_mlp(ugx_ctx *c) {
+  MlpDevice P;
+  double ys = 0.0;
+  if (!mlp_inference_view(c->mlp, c->device, &P, &ys))
+    throw UgxError(UGX_BAD_INPUT, "UGX_GUESS_MLP: no network on this device (ugx_set_guess_network)");
+  guess_mp2(c);
+  c->G.alloc((size_t)c->oovv_n() * 4);
+  c->eng->timed("guess_mlp", 6.784e4 * (double)c->oovv_n(), [&]() {
+    launch_mlp_infer_ham(c->no, c->nv, c->eps.d(), c->oovv.d(), P, c->G.f(), c->st);
+  });
+  launch_apply_guess(c->no, c->nv, c->G.f(), ys, c->t2.d(), c->st);
 }
 
 // ------------------------------------------------------------------ (T)
@@ -872,14 +851,19 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
     if (opt.maxiter < 0 || opt.diis_nvec < 1 || opt.diis_nvec > 8 || opt.diis_min_vecs < 1 ||
         opt.precision < 0 || opt.precision > 2 || !(opt.r_convergence > 0))
       throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: invalid options");
+    if (opt.guess < UGX_GUESS_MP2 || opt.guess > UGX_GUESS_MLP)
+      throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: unknown guess");
     if (opt.guess == UGX_GUESS_USER && !c->have_amps)
       throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: UGX_GUESS_USER without ugx_set_amplitudes");
+    if (opt.guess == UGX_GUESS_MLP && !c->mlp)
+      throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: UGX_GUESS_MLP without ugx_set_guess_network");
     ugx_result r;
     std::memset(&r, 0, sizeof(r));
     Engine &E = *c->eng;
     E.reset_counters();
     auto t0 = std::chrono::steady_clock::now();
     if (opt.guess == UGX_GUESS_MP2) guess_mp2(c);
+    else if (opt.guess == UGX_GUESS_MLP) guess_mlp(c);
     build_tau(c);
     double emp2 = energy(c, opt.energy_form);
     r.e_mp2 = emp2;
@@ -1124,6 +1108,39 @@ int ugx_gemm(ugx_ctx *c, char opa, char opb, int M, int N, int K, double alpha, 
 int ugx_synchronize(ugx_ctx *c) {
   return guarded(c, [&]() {
     check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+int ugx_features(ugx_ctx *c, long n, const long *idx, float *X) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (n <= 0 || !X) throw UgxError(UGX_BAD_INPUT, "ugx_features: bad arguments");
+    const long total = c->oovv_n();
+    DBuf di, dx;
+    if (idx) {
+      for (long r = 0; r < n; r++)
+        if (idx[r] < 0 || idx[r] >= total) throw UgxError(UGX_BAD_INPUT, "ugx_features: index out of range");
+      di.alloc((size_t)n * 8);
+      check(hipMemcpyAsync(di.p, idx, (size_t)n * 8, hipMemcpyHostToDevice, c->st), "index copy");
+    } else if (n > total) {
+      throw UgxError(UGX_BAD_INPUT, "ugx_features: more rows than amplitudes");
+    }
+    dx.alloc((size_t)n * MLP_NFEAT * 4);
+    launch_features(c->no, c->nv, c->eps.d(), c->oovv.d(), n, idx ? (const long *)di.p : nullptr, dx.f(), c->st);
+    check(hipMemcpyAsync(X, dx.p, (size_t)n * MLP_NFEAT * 4, hipMemcpyDeviceToHost, c->st), "features copy");
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+int ugx_set_guess_network(ugx_ctx *c, const ugx_mlp *net) {
+  return guarded(c, [&]() {
+    MlpDevice P;
+    double ys;
+    if (net && !mlp_inference_view(net, c->device, &P, &ys))
+      throw UgxError(UGX_BAD_INPUT, "ugx_set_guess_network: the network lives on another device");
+    c->mlp = net;
     return UGX_OK;
   });
 }

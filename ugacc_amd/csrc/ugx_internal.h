@@ -141,4 +141,41 @@ void launch_triples_energy(int no, int nv, int ntrip, const int *trip_ijk, const
                            const double *eps, const double *oovv, const double *t1, double *etrip, double *part,
                            hipStream_t st);
 
+// ------------------------------------------------------------ learned guess
+constexpr int MLP_NFEAT = 8;
+constexpr int MLP_WIDTH = 128;
+// Inference view of a network: the first layer folded with the feature
+// standardisation (FP32), the two hidden 128x128 layers as BF16 MFMA A
+// fragments in the kernel's permuted K order (8 x 4 x 64 lanes x 16 bytes
+// each), biases and the output layer in FP32.
+struct MlpDevice {
+  const float *w1f, *b1f;
+  const void *w2p;
+  const float *b2;
+  const void *w3p;
+  const float *b3, *w4;
+  float b4;
+};
+// X[r][8] = features of amplitude idx[r] (flattened ijab; idx null: r)
+void launch_features(int no, int nv, const double *eps, const double *oovv, long n, const long *idx, float *X,
+                     hipStream_t st);
+void launch_mlp_infer_x(long N, const float *X, const MlpDevice &P, float *out, hipStream_t st);
+void launch_mlp_infer_ham(int no, int nv, const double *eps, const double *oovv, const MlpDevice &P, float *out,
+                          hipStream_t st);
+void launch_apply_guess(int no, int nv, const float *G, double ys, double *t2, hipStream_t st);
+void launch_mlp_normalize(long n, const float *X, const float *shift, const float *inv, float *Xn, hipStream_t st);
+void launch_mlp_gather(int B, const float *X, const float *y, const int *perm, float *Xb, float *yb, hipStream_t st);
+void launch_mlp_bias_gelu(long rows, const float *b, float *Z, float *H, hipStream_t st);
+void launch_mlp_head(int B, const float *H, const float *w4, const float *b4, const float *target, float *y, float *dy,
+                     double *sq, hipStream_t st);
+void launch_mlp_head_back(int B, const float *dy, const float *w4, const float *Z, float *dZ, hipStream_t st);
+void launch_mlp_gelu_back(long n, const float *dH, const float *Z, float *dZ, hipStream_t st);
+int mlp_colsum_chunks(int B);
+// out[f] = sum_r w[r] M[r][f] (w null: 1); P holds mlp_colsum_chunks(B) * width floats
+void launch_mlp_colsum(int B, int width, const float *M, const float *w, float *P, float *out, hipStream_t st);
+void launch_mlp_sum_parts(int S, long n, const float *P, float *out, hipStream_t st);
+void launch_mlp_loss(int n, const double *sq, double *out, hipStream_t st);
+void launch_mlp_adam(long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
+                     float bc1, float bc2, hipStream_t st);
+
 }  // namespace ugx

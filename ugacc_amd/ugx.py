@@ -14,8 +14,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libugx.so")
 
 OK, NOT_CONVERGED, BAD_INPUT, OOM, HIP_ERROR, NO_DEVICE, NOT_READY, NONFINITE = range(8)
-PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
-GUESS_MP2, GUESS_USER = 0, 1
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X3_PAIR, PREC_BF16 = 0, 1, 2, 3, 4
+GUESS_MP2, GUESS_USER, GUESS_MLP = 0, 1, 2
+MLP_NFEAT, MLP_WIDTH = 8, 128
 ENERGY_REFERENCE, ENERGY_FULL = 0, 1
 PREC_NAMES = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 
@@ -24,7 +25,9 @@ EXPORTS = [
     "ugx_default_options", "ugx_create", "ugx_destroy", "ugx_last_error", "ugx_load_hamiltonian",
     "ugx_set_amplitudes", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
     "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_set_profiling",
-    "ugx_kernel_stats", "ugx_gemm", "ugx_synchronize",
+    "ugx_kernel_stats", "ugx_gemm", "ugx_synchronize", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
+    "ugx_mlp_last_error", "ugx_mlp_set_params", "ugx_mlp_get_params", "ugx_mlp_set_scaling", "ugx_mlp_forward",
+    "ugx_mlp_loss_grad", "ugx_mlp_train", "ugx_features", "ugx_set_guess_network",
 ]
 
 
@@ -91,7 +94,24 @@ def load_library(path=LIBPATH):
     L.ugx_gemm.argtypes = [vp, ctypes.c_char, ctypes.c_char, ip, ip, ip, cd, vp, ctypes.c_long, vp, ctypes.c_long,
                            cd, vp, ctypes.c_long, ip, ip]
     L.ugx_synchronize.argtypes = [vp]
-    non_int = ("ugx_default_options", "ugx_destroy", "ugx_last_error", "ugx_set_profiling", "ugx_kernel_stats")
+    fp, lp, cl = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_long), ctypes.c_long
+    L.ugx_mlp_num_params.argtypes = []
+    L.ugx_mlp_num_params.restype = cl
+    L.ugx_mlp_create.argtypes = [ctypes.POINTER(vp), ip, ctypes.c_ulonglong]
+    L.ugx_mlp_destroy.argtypes = [vp]
+    L.ugx_mlp_destroy.restype = None
+    L.ugx_mlp_last_error.argtypes = [vp]
+    L.ugx_mlp_last_error.restype = ctypes.c_char_p
+    L.ugx_mlp_set_params.argtypes = [vp, fp]
+    L.ugx_mlp_get_params.argtypes = [vp, fp]
+    L.ugx_mlp_set_scaling.argtypes = [vp, fp, fp, cd]
+    L.ugx_mlp_forward.argtypes = [vp, fp, cl, ip, fp]
+    L.ugx_mlp_loss_grad.argtypes = [vp, fp, fp, cl, dp, fp]
+    L.ugx_mlp_train.argtypes = [vp, fp, fp, cl, ip, ip, cd, ctypes.c_ulonglong, dp]
+    L.ugx_features.argtypes = [vp, cl, lp, fp]
+    L.ugx_set_guess_network.argtypes = [vp, vp]
+    non_int = ("ugx_default_options", "ugx_destroy", "ugx_last_error", "ugx_set_profiling", "ugx_kernel_stats",
+               "ugx_mlp_num_params", "ugx_mlp_destroy", "ugx_mlp_last_error")
     for name in EXPORTS:
         if name not in non_int:
             getattr(L, name).restype = ctypes.c_int
@@ -105,6 +125,14 @@ def _p(a):
 
 def _c64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if a is not None else None
+
+
+def _c32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
 
 
 def default_options():
@@ -241,3 +269,93 @@ class Context:
 
     def synchronize(self):
         self._check(self.L.ugx_synchronize(self.h))
+
+    def features(self, idx=None, n=None):
+        """Raw learned-guess features of the amplitudes idx (flattened ijab)."""
+        if idx is not None:
+            idx = np.ascontiguousarray(idx, dtype=np.int64)
+            n = idx.size
+        elif n is None:
+            n = self.no * self.no * self.nv * self.nv
+        X = np.zeros((n, MLP_NFEAT), dtype=np.float32)
+        ip = idx.ctypes.data_as(ctypes.POINTER(ctypes.c_long)) if idx is not None else None
+        self._check(self.L.ugx_features(self.h, n, ip, _f(X)))
+        return X
+
+    def set_guess_network(self, net):
+        """Network used by GUESS_MLP (None clears it); the context keeps a reference."""
+        self._net = net
+        self._check(self.L.ugx_set_guess_network(self.h, net.h if net is not None else None))
+
+
+class Network:
+    """One ugx_mlp: the learned T2 guess network (parameters, Adam state) on a device."""
+
+    def __init__(self, device=0, seed=0):
+        self.L = load_library()
+        h = ctypes.c_void_p()
+        rc = self.L.ugx_mlp_create(ctypes.byref(h), device, seed)
+        if rc != OK:
+            raise UgxError(rc, "ugx_mlp_create failed with status %d" % rc)
+        self.h = h
+        self.nparam = int(self.L.ugx_mlp_num_params())
+        self.shift = np.zeros(MLP_NFEAT, np.float32)
+        self.scale = np.ones(MLP_NFEAT, np.float32)
+        self.y_scale = 1.0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ugx_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            raise UgxError(rc, self.L.ugx_mlp_last_error(self.h).decode())
+
+    def params(self):
+        p = np.zeros(self.nparam, np.float32)
+        self._check(self.L.ugx_mlp_get_params(self.h, _f(p)))
+        return p
+
+    def set_params(self, p):
+        p = _c32(p)
+        assert p.size == self.nparam
+        self._check(self.L.ugx_mlp_set_params(self.h, _f(p)))
+
+    def set_scaling(self, shift, scale, y_scale):
+        self.shift, self.scale, self.y_scale = _c32(shift), _c32(scale), float(y_scale)
+        self._check(self.L.ugx_mlp_set_scaling(self.h, _f(self.shift), _f(self.scale), self.y_scale))
+
+    def forward(self, X, precision=PREC_FP32):
+        X = _c32(X)
+        y = np.zeros(X.shape[0], np.float32)
+        self._check(self.L.ugx_mlp_forward(self.h, _f(X), X.shape[0], precision, _f(y)))
+        return y
+
+    def loss_grad(self, X, y):
+        X, y = _c32(X), _c32(y)
+        loss = ctypes.c_double()
+        g = np.zeros(self.nparam, np.float32)
+        self._check(self.L.ugx_mlp_loss_grad(self.h, _f(X), _f(y), X.shape[0], ctypes.byref(loss), _f(g)))
+        return loss.value, g
+
+    def train(self, X, y, epochs, batch, lr, seed=0):
+        X, y = _c32(X), _c32(y)
+        hist = np.zeros(epochs)
+        self._check(self.L.ugx_mlp_train(self.h, _f(X), _f(y), X.shape[0], epochs, batch, lr, seed, _p(hist)))
+        return hist
+
+    def save(self, path):
+        np.savez(path, params=self.params(), shift=self.shift, scale=self.scale, y_scale=np.float64(self.y_scale))
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            self.set_scaling(z["shift"], z["scale"], float(z["y_scale"]))
+            self.set_params(z["params"])
+        return self
