@@ -1,18 +1,28 @@
 #!/usr/bin/env python3
-"""CCSD(T) benchmark on one MI355X.
+"""CCSD(T) benchmark on one MI355X: the mixed-precision, learned-guess form
+against the plain FP64 form on the same Hamiltonian, in the same run.
 
-One step = one complete CCSD(T) solve (MP2 guess, CCSD iterations to
-R_CONVERGENCE, (T)) of the workload's Hamiltonian, which is resident on the
-device before the timed region starts.  Prints ONE JSON line.
+One step = one complete CCSD(T) solve (initial guess, CCSD iterations, (T))
+of the workload's Hamiltonian, which is resident on the device before the
+timed region starts.  Prints ONE JSON line.
 
     python bench.py [--steps K] [--warmup W] [--workload h2o8|h2o4|h2o]
-                    [--form fp64|fp32|bf16x3] [--no-cpu]
+                    [--form mixed|fp64|fp32|bf16x3] [--guess learned|mp2] [--no-cpu]
+
+--form mixed (the default, the headline): the CCSD iterations run with
+split-BF16x3 contractions (FP32 accumulation) until ||R|| < 1e-5 (or the
+low-precision residual stops falling), then in FP64 until ||R|| < 1e-7 (the
+reference's default R_CONVERGENCE); (T) uses BF16x3 for W3 with FP64
+energy sums; the initial T2 comes from the learned guess.  The FP64 form is
+solved to ||R|| < 1e-10 in the same run, and value is the mixed form's wall
+time; |dE_CCSD|, |dE_(T)| and |dE_total| against the FP64 form and the
+speedup are reported with it.  The other forms time one form alone.
 
 The CPU baseline is the FP64 oracle (a C restatement of the reference's
 loops, OpenMP over the outer loops) timed on a bounded sample of the same
-workload: one CCSD iteration restricted to its first occupied rows and the
-(T) of a few triplets, extrapolated to the full solve with the iteration
-count the GPU run needed.
+workload: one CCSD iteration restricted to its first occupied row and the
+(T) of a sample of triplets, extrapolated to the full solve with the FP64
+iteration count.
 """
 import argparse
 import json
@@ -33,7 +43,10 @@ WORKLOADS = {
     "h2o": dict(n=1, no=5, nv=19, name="H2O/cc-pVDZ CCSD(T), all-electron (24 MOs, o=5, v=19)"),
 }
 CLUSTER_SEED = 8
-PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}  # dense TFLOP/s
+# published dense MI355X peaks, TFLOP/s; a BF16x3 product costs three BF16
+# MFMAs, so its ceiling for algorithmic flops is a third of the BF16 peak
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}
+FP64_CONV = 1e-10
 
 
 def log(msg):
@@ -62,14 +75,11 @@ def cpu_baseline(no, nv, fock, blocks, niter, budget_s=25.0):
     orc.iteration_sample(rows)
     t_iter = (time.perf_counter() - t0) * no / rows
     ntrip = orc.ntriplets()
-    # triplets sampled: spread over the triplet list, within the budget
     t0 = time.perf_counter()
     k = 0
-    samples = []
     while True:
         t = (k * 7919) % ntrip
         orc.tcorr_tjl_range(t, t + 1)
-        samples.append(t)
         k += 1
         if time.perf_counter() - t0 > budget_s / 2 or k >= ntrip:
             break
@@ -82,14 +92,52 @@ def cpu_baseline(no, nv, fock, blocks, niter, budget_s=25.0):
                        % (threads, rows, no, no // rows, k, ntrip, niter))
 
 
+def run_form(ctx, opts, warmup, steps, label):
+    for w in range(warmup):
+        r = ctx.ccsd(opts)
+        log("%s warmup %d: %d iterations (%d low), %.2f s CCSD + %.2f s (T)" % (
+            label, w, r["niter"], r["niter_low"], r["seconds_ccsd"], r["seconds_triples"]))
+    ctx.set_profiling(True)
+    ctx.synchronize()
+    walls, res = [], None
+    for s in range(steps):
+        t0 = time.perf_counter()
+        res = ctx.ccsd(opts)
+        ctx.synchronize()
+        walls.append(time.perf_counter() - t0)
+        if not res["converged"]:
+            raise RuntimeError("%s form did not converge" % label)
+        log("%s step %d: %.3f s (%d iterations, %d low; CCSD %.3f s, (T) %.3f s)" % (
+            label, s, walls[-1], res["niter"], res["niter_low"], res["seconds_ccsd"], res["seconds_triples"]))
+    stats = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    return float(np.mean(walls)), res, stats
+
+
+def roofline(stats):
+    """The kernel label with the most device time, against its peak."""
+    dom = max((k for k in stats if stats[k]["gflop"] > 0), key=lambda k: stats[k]["ms"])
+    st = stats[dom]
+    form = dom.split(":")[1] if ":" in dom else "fp64"
+    ach = st["gflop"] / st["ms"]  # GFLOP/ms = TFLOP/s
+    names = {"triples_w": "k_gemm_b3p / k_gemm (T) W3 grouped GEMM", "ladder": "k_gemm particle-particle ladder"}
+    return dict(bound="mfma", achieved=ach, peak=PEAK[form], unit="TFLOP/s", frac=ach / PEAK[form], traffic=None,
+                kernel="%s (%s)" % (names.get(dom.split(":")[0], dom), dom), launches=st["launches"],
+                avg_ms=st["ms"] / max(1, st["launches"]), flop_per_launch=st["gflop"] * 1e9 / max(1, st["launches"]))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="h2o8", choices=sorted(WORKLOADS))
-    ap.add_argument("--form", default="fp64", choices=["fp64", "fp32", "bf16x3"])
-    ap.add_argument("--conv", type=float, default=1e-10)
+    ap.add_argument("--form", default="mixed", choices=["mixed", "fp64", "fp32", "bf16x3"])
+    ap.add_argument("--low", default="bf16x3", choices=["bf16x3", "fp32"],
+                    help="arithmetic of the low-precision phase of the mixed form")
+    ap.add_argument("--guess", default="learned", choices=["learned", "mp2"])
+    ap.add_argument("--mixed-conv", type=float, default=1e-7)
+    ap.add_argument("--polish", type=float, default=1e-5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=CLUSTER_SEED)
     ap.add_argument("--synthetic", action="store_true",
@@ -97,7 +145,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (the CCSD/(T) sharding is a "
-              "design note in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "design note); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from ugacc_amd import ugx
@@ -125,58 +173,68 @@ def main():
                 "H2O at 0.9572 A / 104.52 deg, cc-pVDZ, RHF from the project's host front end (untimed)")
     ctx = ugx.Context(0)
     ctx.load_hamiltonian(no, nv, fock, blocks)
-    prec = ugx.PREC_NAMES[args.form]
-    opts = ugx.default_options()
-    opts.wfn_ccsd_t = 1
-    opts.maxiter = 100
-    opts.r_convergence = args.conv
-    opts.precision = prec
 
-    for w in range(args.warmup):
-        log("warmup %d" % w)
-        r = ctx.ccsd(opts)
-        log("warmup done: %d iterations, %.2f s CCSD + %.2f s (T)" % (r["niter"], r["seconds_ccsd"],
-                                                                        r["seconds_triples"]))
-    ctx.set_profiling(True)
-    ctx.synchronize()
-    walls, res = [], None
-    for s in range(args.steps):
-        t0 = time.perf_counter()
-        res = ctx.ccsd(opts)
-        ctx.synchronize()
-        walls.append(time.perf_counter() - t0)
-        log("step %d: %.3f s (%d iterations)" % (s, walls[-1], res["niter"]))
-    stats = ctx.kernel_stats()
-    ctx.set_profiling(False)
-    wall = float(np.mean(walls))
-    fl_ccsd = res["niter"] * ccsd_flops(no, nv)
-    fl_t = triples_flops(no, nv)
-    tflops = (fl_ccsd + fl_t) / wall * 1e-12
+    def options(prec, conv, guess):
+        o = ugx.default_options()
+        o.wfn_ccsd_t = 1
+        o.maxiter = 100
+        o.r_convergence = conv
+        o.precision = prec
+        o.polish_threshold = args.polish
+        o.guess = guess
+        return o
 
-    # dominant kernel: the (T) W3 build GEMM (one launch per permutation and
-    # triplet batch, tile 128x160 used by nothing else)
-    dom = "triples_w"
-    st = stats.get(dom, {"launches": 1, "ms": float("nan"), "gflop": 0.0})
-    ach = st["gflop"] / st["ms"] if st["ms"] > 0 else float("nan")  # GFLOP/ms = TFLOP/s
-    peak = PEAK[args.form]
-    roof = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
-                traffic=None, kernel="k_gemm (T) W3 build, tile 128x160",
-                launches=st["launches"], avg_ms=st["ms"] / max(1, st["launches"]),
-                flop_per_launch=st["gflop"] * 1e9 / max(1, st["launches"]))
+    net = None
+    guess = ugx.GUESS_MP2
+    if args.form == "mixed" and args.guess == "learned":
+        from ugacc_amd.learned import WEIGHTS, load_network
+        net = load_network(WEIGHTS)
+        ctx.set_guess_network(net)
+        guess = ugx.GUESS_MLP
+
+    fp64_opts = options(ugx.PREC_FP64, FP64_CONV, ugx.GUESS_MP2)
+    out = {}
+    if args.form == "mixed":
+        wall64, res64, st64 = run_form(ctx, fp64_opts, args.warmup, args.steps, "fp64")
+        mixed_opts = options(ugx.PREC_NAMES[args.low], args.mixed_conv, guess)
+        wall, res, stats = run_form(ctx, mixed_opts, args.warmup, args.steps, "mixed")
+        d_ccsd = abs(res["e_ccsd_final"] - res64["e_ccsd_final"])
+        d_t = abs(res["e_t"] - res64["e_t"])
+        d_tot = abs(res["e_ccsd_final"] + res["e_t"] - res64["e_ccsd_final"] - res64["e_t"])
+        out.update(fp64_wall_s=wall64, fp64_iterations=res64["niter"], fp64_seconds_ccsd=res64["seconds_ccsd"],
+                   fp64_seconds_triples=res64["seconds_triples"], speedup_vs_fp64=wall64 / wall,
+                   dE_ccsd=d_ccsd, dE_t=d_t, dE_total=d_tot, dE_ok=bool(d_tot <= 1e-6),
+                   e_ccsd_fp64=res64["e_ccsd_final"], e_t_fp64=res64["e_t"],
+                   fp64_tflops=(res64["niter"] * ccsd_flops(no, nv) + triples_flops(no, nv)) / wall64 * 1e-12,
+                   fp64_kernels=st64)
+        niter64 = res64["niter"]
+        dtype = "%s/f64" % args.low
+        config = {"workload": wl["name"], "no": no, "nv": nv, "form": "mixed", "low_precision": args.low,
+                  "polish_threshold": args.polish, "r_convergence": args.mixed_conv, "guess": args.guess,
+                  "fp64_r_convergence": FP64_CONV}
+    else:
+        conv = FP64_CONV
+        wall, res, stats = run_form(ctx, options(ugx.PREC_NAMES[args.form], conv, ugx.GUESS_MP2), args.warmup,
+                                    args.steps, args.form)
+        niter64 = res["niter"]
+        dtype = "f64" if args.form == "fp64" else args.form
+        config = {"workload": wl["name"], "no": no, "nv": nv, "form": args.form, "guess": "mp2",
+                  "r_convergence": conv}
+    tflops = (res["niter"] * ccsd_flops(no, nv) + triples_flops(no, nv)) / wall * 1e-12
     cpu = None
     if not args.no_cpu:
         log("cpu baseline sample")
-        cpu = cpu_baseline(no, nv, fock, blocks, res["niter"])
+        cpu = cpu_baseline(no, nv, fock, blocks, niter64)
     line = {
         "metric": METRIC, "value": wall, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall * 1e3, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "f32", "data": data,
-        "config": {"workload": wl["name"], "no": no, "nv": nv, "form": args.form, "guess": "mp2",
-                   "r_convergence": args.conv},
-        "tflops": tflops, "iterations": res["niter"], "seconds_ccsd": res["seconds_ccsd"],
-        "seconds_triples": res["seconds_triples"], "e_ccsd": res["e_ccsd"], "e_t": res["e_t"],
-        "kernels": stats, "roofline": roof, "cpu_baseline": cpu,
+        "dtype": dtype, "data": data, "config": config,
+        "tflops": tflops, "iterations": res["niter"], "iterations_low": res["niter_low"],
+        "seconds_ccsd": res["seconds_ccsd"], "seconds_triples": res["seconds_triples"],
+        "e_ccsd": res["e_ccsd_final"], "e_t": res["e_t"],
     }
+    line.update(out)
+    line.update({"kernels": stats, "roofline": roofline(stats), "cpu_baseline": cpu})
     print(json.dumps(line), flush=True)
 
 

@@ -536,7 +536,7 @@ void triples_build_w(ugx_ctx *c, int prec, long t0, int nb, const std::vector<in
     d.alpha = 1.0;
     d.beta = (g == 0) ? 0.0 : 1.0;
     d.c_f32 = c->tW_f32;
-    c->eng->timed("triples_w", fl, [&]() {
+    c->eng->timed(prec == PREC_FP32 ? "triples_w:fp32" : "triples_w", fl, [&]() {
       launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)g * nb, prec, tile, c->st);
     });
   }
@@ -612,7 +612,7 @@ void triples_build_w_pair(ugx_ctx *c, long t0, int nb, const std::vector<int> &t
     d.alpha = 1.0;
     d.beta = (g == 0) ? 0.0 : 1.0;
     d.c_f32 = c->tW_f32;
-    c->eng->timed("triples_w", fl, [&]() {
+    c->eng->timed("triples_w:bf16x3", fl, [&]() {
       launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)g * nb, PREC_BF16X3_PRESPLIT, tile, c->st);
     });
   }

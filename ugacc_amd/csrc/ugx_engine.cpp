@@ -7,6 +7,18 @@
 
 namespace ugx {
 
+namespace {
+// Profiling label of a contraction: the low-precision forms get a suffix so
+// that their device time is reported apart from the FP64 polish.
+std::string prec_label(const char *label, int prec) {
+  std::string s(label);
+  if (prec == PREC_FP32) s += ":fp32";
+  else if (prec == PREC_BF16X3) s += ":bf16x3";
+  return s;
+}
+}  // namespace
+
+
 void hip_check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
 }
@@ -336,7 +348,7 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
         flops_ += fl;
         if (profiling_ && label) {
           hip_check(hipEventRecord(e1, st_), "event record");
-          pending_.push_back(Pending{label, e0, e1, fl});
+          pending_.push_back(Pending{prec_label(label, prec), e0, e1, fl});
         }
         return;
       }
@@ -357,7 +369,7 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
   flops_ += fl;
   if (profiling_ && label) {
     hip_check(hipEventRecord(e1, st_), "event record");
-    pending_.push_back(Pending{label, e0, e1, fl});
+    pending_.push_back(Pending{prec_label(label, prec), e0, e1, fl});
   }
 }
 

@@ -192,6 +192,7 @@ void ensure_work(ugx_mlp *n, int B) {
 // Forward of B normalised rows X (device); with a target, also the loss
 // (into loss_out, device) and the full gradient into n->g.
 void fwd_bwd(ugx_mlp *n, int B, const float *X, const float *target, double *loss_out) {
+  if (target && !loss_out) throw UgxError(UGX_BAD_INPUT, "internal: loss output missing");
   ensure_work(n, B);
   hipStream_t st = n->st;
   float *P = n->p.f(), *G = n->g.f();
@@ -369,7 +370,8 @@ int ugx_mlp_loss_grad(ugx_mlp *n, const float *X, const float *y, long rows, dou
     if (rows > MAX_BATCH) throw UgxError(UGX_BAD_INPUT, "ugx_mlp_loss_grad: at most 65536 rows");
     upload_normalized(n, X, rows);
     h2d(n->yb, y, (size_t)rows * 4, n->st);
-    fwd_bwd(n, (int)rows, n->Xn.f(), n->yb.f(), (double *)n->loss.p);
+    n->loss.alloc(8);  // before its address is taken
+    fwd_bwd(n, (int)rows, n->Xn.f(), n->yb.f(), n->loss.d());
     check(hipMemcpyAsync(loss, n->loss.p, 8, hipMemcpyDeviceToHost, n->st), "loss copy");
     check(hipMemcpyAsync(grad, n->g.p, NPARAM * 4, hipMemcpyDeviceToHost, n->st), "gradient copy");
     check(hipStreamSynchronize(n->st), "stream sync");
