@@ -47,6 +47,11 @@ CLUSTER_SEED = 8
 # MFMAs, so its ceiling for algorithmic flops is a third of the BF16 peak
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}
 FP64_CONV = 1e-10
+# HBM bytes per launch of a kernel label on a workload, from two rocprofv3
+# --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the counters' reading for a
+# plain copy of known size (fetch x2.0, write x1.0); the runs and their
+# commands are in profiles/triples_w_bf16x3_h2o8_hbm_traffic.md
+TRAFFIC = {("h2o8", "triples_w:bf16x3"): 12.46e9}
 
 
 def log(msg):
@@ -114,14 +119,15 @@ def run_form(ctx, opts, warmup, steps, label):
     return float(np.mean(walls)), res, stats
 
 
-def roofline(stats):
+def roofline(stats, workload):
     """The kernel label with the most device time, against its peak."""
     dom = max((k for k in stats if stats[k]["gflop"] > 0), key=lambda k: stats[k]["ms"])
     st = stats[dom]
     form = dom.split(":")[1] if ":" in dom else "fp64"
     ach = st["gflop"] / st["ms"]  # GFLOP/ms = TFLOP/s
     names = {"triples_w": "k_gemm_b3p / k_gemm (T) W3 grouped GEMM", "ladder": "k_gemm particle-particle ladder"}
-    return dict(bound="mfma", achieved=ach, peak=PEAK[form], unit="TFLOP/s", frac=ach / PEAK[form], traffic=None,
+    return dict(bound="mfma", achieved=ach, peak=PEAK[form], unit="TFLOP/s", frac=ach / PEAK[form],
+                traffic=TRAFFIC.get((workload, dom)),
                 kernel="%s (%s)" % (names.get(dom.split(":")[0], dom), dom), launches=st["launches"],
                 avg_ms=st["ms"] / max(1, st["launches"]), flop_per_launch=st["gflop"] * 1e9 / max(1, st["launches"]))
 
@@ -142,7 +148,19 @@ def main():
     ap.add_argument("--seed", type=int, default=CLUSTER_SEED)
     ap.add_argument("--synthetic", action="store_true",
                     help="seeded random Hamiltonian of the workload's dimensions instead of the water cluster")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (torch) five times: the reference kernel whose "
+                         "FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the library's kernels")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4, "launches": 5}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (the CCSD/(T) sharding is a "
               "design note); run with --gpus 1", file=sys.stderr)
@@ -234,7 +252,7 @@ def main():
         "e_ccsd": res["e_ccsd_final"], "e_t": res["e_t"],
     }
     line.update(out)
-    line.update({"kernels": stats, "roofline": roofline(stats), "cpu_baseline": cpu})
+    line.update({"kernels": stats, "roofline": roofline(stats, args.workload), "cpu_baseline": cpu})
     print(json.dumps(line), flush=True)
 
 
