@@ -145,6 +145,8 @@ def main():
     ap.add_argument("--mixed-conv", type=float, default=1e-7)
     ap.add_argument("--polish", type=float, default=1e-5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="size-heuristic GEMM tiles instead of the autotuned ones (tuning runs in the warmup)")
     ap.add_argument("--seed", type=int, default=CLUSTER_SEED)
     ap.add_argument("--synthetic", action="store_true",
                     help="seeded random Hamiltonian of the workload's dimensions instead of the water cluster")
@@ -191,6 +193,7 @@ def main():
                 "H2O at 0.9572 A / 104.52 deg, cc-pVDZ, RHF from the project's host front end (untimed)")
     ctx = ugx.Context(0)
     ctx.load_hamiltonian(no, nv, fock, blocks)
+    ctx.set_autotune(not args.no_autotune)
 
     def options(prec, conv, guess):
         o = ugx.default_options()
@@ -228,6 +231,7 @@ def main():
         niter64 = res64["niter"]
         dtype = "%s/f64" % args.low
         config = {"workload": wl["name"], "no": no, "nv": nv, "form": "mixed", "low_precision": args.low,
+                  "autotune": not args.no_autotune,
                   "polish_threshold": args.polish, "r_convergence": args.mixed_conv, "guess": args.guess,
                   "fp64_r_convergence": FP64_CONV}
     else:
@@ -237,6 +241,7 @@ def main():
         niter64 = res["niter"]
         dtype = "f64" if args.form == "fp64" else args.form
         config = {"workload": wl["name"], "no": no, "nv": nv, "form": args.form, "guess": "mp2",
+                  "autotune": not args.no_autotune,
                   "r_convergence": conv}
     tflops = (res["niter"] * ccsd_flops(no, nv) + triples_flops(no, nv)) / wall * 1e-12
     cpu = None

@@ -106,6 +106,9 @@ typedef struct {
   int print;               /* PRINT: 0 silent, >=1 iteration table on stdout */
   int polish_keep_diis;    /* mixed forms: keep the DIIS subspace when switching
                               to the FP64 polish (default 1); 0 restarts it */
+  int graph;               /* 1: replay each iteration's residual kernels from a
+                              captured hipGraph (launch-bound small systems);
+                              results are bitwise those of eager execution */
 } ugx_options;
 
 typedef struct {
@@ -185,6 +188,14 @@ int ugx_gemm(ugx_ctx *ctx, char opa, char opb, int M, int N, int K, double alpha
 
 /* Synchronises the context's stream. */
 int ugx_synchronize(ugx_ctx *ctx);
+
+/* GEMM tile autotuning (off by default): when on, the first launch of each
+ * contraction shape times every tile of the kernel's tile table on the
+ * real operands (into scratch) and keeps the fastest for the context's
+ * lifetime.  No counterpart in the reference (its loops have no tiles). */
+int ugx_set_autotune(ugx_ctx *ctx, int on);
+/* The tuned shapes as "shape -> tile ms" lines; returns the bytes needed. */
+size_t ugx_tile_table(ugx_ctx *ctx, char *buf, size_t buflen);
 
 /* ---------------------------------------------------------------------
  * Learned T2 guess.  Replaces the MP2 doubles of CCWfn::CCWfn

@@ -126,3 +126,46 @@ def test_triples_low_precision_forms(no, nv):
         assert np.max(np.abs(got - ref)) < 2e-6 * scale, prec
         W = c.w3(no - 1, 1, 0, prec)
         assert np.max(np.abs(W - W64)) < 1e-5 * np.max(np.abs(W64)), prec
+
+
+@pytest.mark.gpu
+def test_autotuned_tiles_give_the_same_results():
+    # the autotuner only changes which tile computes each contraction
+    announce("autotune on vs off, o=6 v=40, CCSD(T) FP64 and BF16x3")
+    c, fock, ints = loaded(6, 40, seed=61)
+    ref = c.ccsd(r_convergence=1e-10, wfn_ccsd_t=1)
+    ref3 = c.ccsd(r_convergence=1e-9, wfn_ccsd_t=1, precision=ugx.PREC_BF16X3)
+    c.set_autotune(True)
+    try:
+        r = c.ccsd(r_convergence=1e-10, wfn_ccsd_t=1)
+        r3 = c.ccsd(r_convergence=1e-9, wfn_ccsd_t=1, precision=ugx.PREC_BF16X3)
+        table = c.tile_table()
+    finally:
+        c.set_autotune(False)
+    assert abs(r["e_ccsd_final"] - ref["e_ccsd_final"]) < 1e-11
+    assert abs(r["e_t"] - ref["e_t"]) < 1e-12
+    assert abs(r3["e_ccsd_final"] - ref3["e_ccsd_final"]) < 1e-9
+    assert abs(r3["e_t"] - ref3["e_t"]) < 1e-10
+    assert table.count("->") >= 10, table
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [ugx.PREC_FP64, ugx.PREC_BF16X3])
+def test_graph_replayed_iterations_are_bitwise_eager(prec):
+    # every iteration after the first of each precision replays captured
+    # residual graphs, with the eager update / DIIS / energy work and host
+    # synchronisations between replays; the trajectory must be identical
+    announce("graph vs eager iterations o=5 v=19 prec=%d" % prec)
+    c, fock, ints = loaded(5, 19, seed=71)
+    ra = c.ccsd(r_convergence=1e-10, wfn_ccsd_t=1, precision=prec)
+    t1a, t2a = c.amplitudes()
+    rb = c.ccsd(r_convergence=1e-10, wfn_ccsd_t=1, precision=prec, graph=1)
+    t1b, t2b = c.amplitudes()
+    rc = c.ccsd(r_convergence=1e-10, wfn_ccsd_t=1, precision=prec, graph=1)  # replays the graphs of rb
+    t1c, t2c = c.amplitudes()
+    assert ra["niter"] == rb["niter"] == rc["niter"] and ra["niter"] > 4
+    for r in (rb, rc):
+        assert r["e_ccsd_final"] == ra["e_ccsd_final"] and r["e_t"] == ra["e_t"] and r["rms"] == ra["rms"]
+    np.testing.assert_array_equal(t2b, t2a)
+    np.testing.assert_array_equal(t2c, t2a)
+    np.testing.assert_array_equal(t1c, t1a)

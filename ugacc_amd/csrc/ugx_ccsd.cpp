@@ -60,6 +60,9 @@ struct ugx_ctx {
   int diis_count = 0, diis_next = 0, diis_nvec = 0;
   // (T)
   DBuf tP, tQ, tWbuf, ttrip, tptrs, tE, tPart;
+  DBuf tOff;  // blocked-W offset tables of the three (T) launches
+  DBuf tProb, tCounter;  // energy kernel: problem table (one entry), work-queue counter
+  int tOff_blk = 0, tOff_v = 0;
   int tW_f32 = 0;  // the (T) W buffer holds FP32 values
   DBuf f_tP, f_tQ, f_t2, f_ooov;  // FP32 operand copies for the mixed forms
   // BF16x3 (T): BF16 hi/lo pair operands, all k contiguous
@@ -69,6 +72,16 @@ struct ugx_ctx {
   // learned guess: network (owned by the caller) and its output f(x_ijab)
   const ugx_mlp *mlp = nullptr;
   DBuf G;
+  int t_batch = 0;  // cap on triplets per (T) batch (0: by the 2 GB W budget)
+  // hipGraphs of the residual, keyed by (amplitude buffer parity, precision)
+  struct ResidualGraph {
+    const void *t1o = nullptr;
+    int prec = -1;
+    hipGraphExec_t exec = nullptr;
+    double flops = 0.0;
+  };
+  std::vector<ResidualGraph> graphs;
+  int graph_eager[4] = {0, 0, 0, 0};  // eager residuals run per precision since the last reset
   // residual of the last ugx_residual, for ugx_intermediates
   bool have_intermediates = false;
   double last_flops = 0.0;
@@ -243,6 +256,59 @@ void residual(ugx_ctx *c, int prec) {
 }
 
 // t = t_old + R/D, returns ||R||_2 (ccwfn.cc:555-582)
+void drop_graphs(ugx_ctx *c) {
+  for (auto &g : c->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+  for (int &x : c->graph_eager) x = 0;
+}
+
+// One residual evaluation of the iteration.  With graphs on, the first
+// residual of a precision runs eagerly (it sizes the engine's workspaces and
+// fills its caches); the next one of each amplitude-buffer parity is
+// captured into a hipGraph, which later iterations replay.  The captured
+// work is the same kernels with the same arguments, so results are
+// bitwise identical to eager execution.
+void residual_step(ugx_ctx *c, int prec, bool use_graph) {
+  if (!use_graph || c->graph_eager[prec] == 0) {
+    residual(c, prec);
+    c->graph_eager[prec]++;
+    return;
+  }
+  for (auto &g : c->graphs)
+    if (g.t1o == c->t1o.p && g.prec == prec) {
+      check(hipGraphLaunch(g.exec, c->st), "graph launch");
+      c->eng->add_flops(g.flops);
+      return;
+    }
+  Engine &E = *c->eng;
+  const bool prof = E.profiling();
+  E.set_profiling(false);
+  const double f0 = E.flops_total();
+  hipGraph_t graph = nullptr;
+  check(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    residual(c, prec);
+  } catch (...) {
+    (void)hipStreamEndCapture(c->st, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    E.set_profiling(prof);
+    throw;
+  }
+  check(hipStreamEndCapture(c->st, &graph), "end capture");
+  E.set_profiling(prof);
+  ugx_ctx::ResidualGraph g;
+  g.t1o = c->t1o.p;
+  g.prec = prec;
+  g.flops = E.flops_total() - f0;
+  hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  check(e, "graph instantiate");
+  c->graphs.push_back(g);
+  // the capture recorded the work without running it
+  check(hipGraphLaunch(g.exec, c->st), "graph launch");
+}
+
 double increment(ugx_ctx *c, double *R2out) {
   int np = update_nparts(c->no, c->nv);
   launch_update(c->no, c->nv, c->eps.d(), c->R1.d(), c->X.d(), c->t1o.d(), c->t2o.d(), c->t1.d(), c->t2.d(), R2out,
@@ -412,9 +478,6 @@ void triples_prepare(ugx_ctx *c, int prec) {
           c->trip_host.push_back(j);
           c->trip_host.push_back(k);
         }
-    const long nt = (long)c->trip_host.size() / 3;
-    c->ttrip.alloc(std::max<long>(nt, 1) * 3 * 4);
-    if (nt) check(hipMemcpy(c->ttrip.p, c->trip_host.data(), nt * 3 * 4, hipMemcpyHostToDevice), "triplet table");
     c->tables_ready = true;
   }
   if (prec == PREC_BF16X3) {
@@ -453,6 +516,50 @@ void triples_prepare(ugx_ctx *c, int prec) {
   }
 }
 
+// One W3 to build: triplet (i, j, k) of the problem whose operands live in
+// context src (the (T) of one context uses src = itself; a batch groups the
+// triplets of several equally sized problems).
+struct TEnt {
+  const ugx_ctx *src;
+  int i, j, k;
+};
+
+// Output offset tables of the three W launches into the blocked layout of
+// block edge T:  z = c: rows (a,b), cols c;  z = b: rows b, cols (a,c);
+// z = a: rows a, cols (b,c).
+void ensure_w_tables(ugx_ctx *c, int T) {
+  const int v = c->nv;
+  if (c->tOff_blk == T && c->tOff_v == v && c->tOff.p) return;
+  const long vv = (long)v * v, nbk = tw_nb(v, T);
+  std::vector<long> off((size_t)3 * (vv + v));
+  long *g0m = off.data(), *g0n = g0m + vv, *g1m = g0n + v, *g1n = g1m + v, *g2m = g1n + vv, *g2n = g2m + v;
+  for (long x = 0; x < v; x++) {
+    g0n[x] = tw_off_c(x, T);
+    g1m[x] = tw_off_b(x, nbk, T);
+    g2m[x] = tw_off_a(x, nbk, T);
+    for (long y = 0; y < v; y++) {
+      g0m[x * v + y] = tw_off_a(x, nbk, T) + tw_off_b(y, nbk, T);
+      g1n[x * v + y] = tw_off_a(x, nbk, T) + tw_off_c(y, T);
+      g2n[x * v + y] = tw_off_b(x, nbk, T) + tw_off_c(y, T);
+    }
+  }
+  check(hipStreamSynchronize(c->st), "stream sync");  // the old tables may still be in use
+  c->tOff.alloc(off.size() * 8);
+  check(hipMemcpy(c->tOff.p, off.data(), off.size() * 8, hipMemcpyHostToDevice), "W offset tables");
+  c->tOff_blk = T;
+  c->tOff_v = v;
+}
+
+// Output tables of launch g (0: z = c, 1: z = b, 2: z = a) into the blocked W.
+void set_w_tables(ugx_ctx *c, GemmDesc &d, int g) {
+  ensure_w_tables(c, tw_blk(c->tW_f32));
+  const long v = c->nv, vv = v * v;
+  const long *t = (const long *)c->tOff.p;
+  const long *m0 = t, *n0 = m0 + vv, *m1 = n0 + v, *n1 = m1 + v, *m2 = n1 + vv, *n2 = m2 + v;
+  d.offm = g == 0 ? m0 : g == 1 ? m1 : m2;
+  d.offn = g == 0 ? n0 : g == 1 ? n1 : n2;
+}
+
 // Builds W for triplets [t0, t0 + nb) of the table into c->tWbuf.
 //
 // The six permutations pair up by their third output index z, and the two
@@ -466,20 +573,21 @@ void triples_prepare(ugx_ctx *c, int prec) {
 // member of a pair reads its integrals from Q[p][y][x][e] = <pe|xy> and its
 // hole amplitudes as t2_pm^xy = t2_mp^yx, which puts its rows/columns in
 // the first member's order.  W is written once and accumulated twice.
-void triples_build_w(ugx_ctx *c, int prec, long t0, int nb, const std::vector<int> &trip) {
+void triples_build_w(ugx_ctx *c, int prec, const TEnt *ents, int nb) {
   const int o = c->no, v = c->nv;
   const long v2 = (long)v * v, v3 = v2 * v, ov = (long)o * v, ov2 = (long)o * v2;
   const bool f32 = (prec != PREC_FP64);
   const size_t es = f32 ? 4 : 8;
-  const char *P = (const char *)(f32 ? c->f_tP.p : c->tP.p);
-  const char *Q = (const char *)(f32 ? c->f_tQ.p : c->tQ.p);
-  const char *T2 = (const char *)(f32 ? c->f_t2.p : c->t2.p);
-  const char *OV = (const char *)(f32 ? c->f_ooov.p : c->ooov.p);
   auto at = [&](const char *base, long off) { return (const void *)(base + (size_t)off * es); };
   std::vector<GemmPtrs> tab((size_t)3 * nb);
   for (int b = 0; b < nb; b++) {
-    const long i = trip[3 * (t0 + b)], j = trip[3 * (t0 + b) + 1], k = trip[3 * (t0 + b) + 2];
-    double *W = (double *)((char *)c->tWbuf.p + (size_t)b * v3 * (c->tW_f32 ? 4 : 8));
+    const ugx_ctx *src = ents[b].src;  // the entry's own operands (same no, nv)
+    const char *P = (const char *)(f32 ? src->f_tP.p : src->tP.p);
+    const char *Q = (const char *)(f32 ? src->f_tQ.p : src->tQ.p);
+    const char *T2 = (const char *)(f32 ? src->f_t2.p : src->t2.p);
+    const char *OV = (const char *)(f32 ? src->f_ooov.p : src->ooov.p);
+    const long i = ents[b].i, j = ents[b].j, k = ents[b].k;
+    double *W = (double *)((char *)c->tWbuf.p + (size_t)b * tw_stride(v, tw_blk(c->tW_f32)) * (c->tW_f32 ? 4 : 8));
     GemmPtrs &A = tab[b], &B = tab[(size_t)nb + b], &C = tab[(size_t)2 * nb + b];
     // z = c
     A.A[0] = at(P, i * v3);              A.B[0] = at(T2, (k * o + j) * v2);
@@ -536,6 +644,8 @@ void triples_build_w(ugx_ctx *c, int prec, long t0, int nb, const std::vector<in
     d.alpha = 1.0;
     d.beta = (g == 0) ? 0.0 : 1.0;
     d.c_f32 = c->tW_f32;
+    set_w_tables(c, d, g);
+    tile = c->eng->tile_for(d, tab.data() + (size_t)g * nb, prec, tile, tw_stride(v, tw_blk(c->tW_f32)));
     c->eng->timed(prec == PREC_FP32 ? "triples_w:fp32" : "triples_w", fl, [&]() {
       launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)g * nb, prec, tile, c->st);
     });
@@ -546,17 +656,18 @@ void triples_build_w(ugx_ctx *c, int prec, long t0, int nb, const std::vector<in
 // hi/lo pair operands that all have k contiguous -- the hole terms read
 // T2t[p][x][y][m] = t2_pm^xy, T2tq[p][y][x][m] = t2_pm^xy and
 // OVt[q][r][z][m] = <qr|mz> instead of the row-contiguous t2 and ooov.
-void triples_build_w_pair(ugx_ctx *c, long t0, int nb, const std::vector<int> &trip) {
+void triples_build_w_pair(ugx_ctx *c, const TEnt *ents, int nb) {
   const int o = c->no, v = c->nv;
   const long v2 = (long)v * v, v3 = v2 * v, ov2o = (long)o * v2, vo = (long)v * o;
-  const unsigned short *P = (const unsigned short *)c->b_P.p, *Q = (const unsigned short *)c->b_Q.p;
-  const unsigned short *T2 = (const unsigned short *)c->b_t2.p, *T2t = (const unsigned short *)c->b_T2t.p;
-  const unsigned short *T2q = (const unsigned short *)c->b_T2tq.p, *OVt = (const unsigned short *)c->b_OVt.p;
   const long nP = (long)o * v3, nT = (long)o * o * v2, nOV = (long)o * o * o * v;
   std::vector<GemmPtrs> tab((size_t)3 * nb);
   for (int b = 0; b < nb; b++) {
-    const long i = trip[3 * (t0 + b)], j = trip[3 * (t0 + b) + 1], k = trip[3 * (t0 + b) + 2];
-    double *W = (double *)((char *)c->tWbuf.p + (size_t)b * v3 * (c->tW_f32 ? 4 : 8));
+    const ugx_ctx *src = ents[b].src;  // the entry's own operands (same no, nv)
+    const unsigned short *P = (const unsigned short *)src->b_P.p, *Q = (const unsigned short *)src->b_Q.p;
+    const unsigned short *T2 = (const unsigned short *)src->b_t2.p, *T2t = (const unsigned short *)src->b_T2t.p;
+    const unsigned short *T2q = (const unsigned short *)src->b_T2tq.p, *OVt = (const unsigned short *)src->b_OVt.p;
+    const long i = ents[b].i, j = ents[b].j, k = ents[b].k;
+    double *W = (double *)((char *)c->tWbuf.p + (size_t)b * tw_stride(v, tw_blk(c->tW_f32)) * (c->tW_f32 ? 4 : 8));
     GemmPtrs &A = tab[b], &B = tab[(size_t)nb + b], &C = tab[(size_t)2 * nb + b];
     A.A[0] = P + i * v3;              A.B[0] = T2 + (k * o + j) * v2;
     A.A[1] = T2t + i * ov2o;          A.B[1] = OVt + (j * o + k) * vo;
@@ -612,17 +723,17 @@ void triples_build_w_pair(ugx_ctx *c, long t0, int nb, const std::vector<int> &t
     d.alpha = 1.0;
     d.beta = (g == 0) ? 0.0 : 1.0;
     d.c_f32 = c->tW_f32;
+    set_w_tables(c, d, g);
+    tile = c->eng->tile_for(d, tab.data() + (size_t)g * nb, PREC_BF16X3_PRESPLIT, tile, tw_stride(v, tw_blk(c->tW_f32)));
     c->eng->timed("triples_w:bf16x3", fl, [&]() {
       launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)g * nb, PREC_BF16X3_PRESPLIT, tile, c->st);
     });
   }
 }
 
-// (T) over all triplets; per-triplet energies (table order, i = j = k
-// entries reported as 0) into etrip_out when non-null.
-double triples(ugx_ctx *c, int prec, std::vector<double> *etrip_out) {
-  const int o = c->no, v = c->nv;
-  const long v3 = (long)v * v * v;
+// The (T) operands of a context at a precision: the integral tables and
+// the amplitude-dependent copies (FP32 t2, or the BF16 hi/lo pairs).
+void triples_operands(ugx_ctx *c, int prec) {
   triples_prepare(c, prec);
   if (prec == PREC_FP32) {
     const long n = c->oovv_n();
@@ -632,29 +743,66 @@ double triples(ugx_ctx *c, int prec, std::vector<double> *etrip_out) {
     pd.alpha = 1.0; pd.beta = 0.0;
     launch_permute(pd, c->st);
   }
-  const long nt = (long)c->trip_host.size() / 3;
-  // W is FP32 in the low-precision forms.  Batch: W of up to ~2 GB
-  const int wf = (prec != PREC_FP64) ? 1 : 0;
+}
+
+// (T) energies of a list of triplet entries, computed on the lead context's
+// stream and buffers.  Entry e is triplet (i, j, k) of the problem in
+// context ents[e].src (all of one size, operands prepared by
+// triples_operands and complete on the device); prob[e] indexes probs, the
+// matching (eps, oovv, t1) of each problem.  Batches of entries get one W
+// buffer of up to ~2 GB: three grouped W3 GEMM launches whose pointer
+// tables span every entry of the batch, then one persistent energy kernel
+// over all (entry, block pair) items.
+std::vector<double> triples_entries(ugx_ctx *c, int prec, const std::vector<TEnt> &ents,
+                                    const std::vector<int> &prob, const std::vector<TripProb> &probs) {
+  const int o = c->no, v = c->nv;
+  const long nt = (long)ents.size();
+  std::vector<double> e(nt);
+  if (nt == 0) return e;
+  const int wf = (prec != PREC_FP64) ? 1 : 0;  // W is FP32 in the low-precision forms
   const size_t wes = wf ? 4 : 8;
-  long nb = std::max<long>(1, std::min<long>(nt, (2L << 30) / (v3 * (long)wes)));
-  nb = std::min<long>(nb, 4096);
+  const long ws = tw_stride(v, tw_blk(wf));
+  long nb = std::max<long>(1, std::min<long>(nt, (2L << 30) / (ws * (long)wes)));
+  nb = std::min<long>(nb, c->t_batch > 0 ? c->t_batch : 4096);
   c->tW_f32 = wf;
-  c->tWbuf.alloc((size_t)nb * v3 * wes);
-  c->tE.alloc(std::max<long>(nt, 1) * 8);
+  c->tWbuf.alloc((size_t)nb * ws * wes);
+  c->tE.alloc((size_t)nt * 8);
   const int npairs = triples_energy_npairs(v, wf);
   c->tPart.alloc((size_t)nb * npairs * 8);
+  c->tProb.alloc(probs.size() * sizeof(TripProb));
+  check(hipMemcpyAsync(c->tProb.p, probs.data(), probs.size() * sizeof(TripProb), hipMemcpyHostToDevice, c->st),
+        "problem table");
+  std::vector<int> t4((size_t)nt * 4);
+  for (long t = 0; t < nt; t++) {
+    t4[4 * t] = ents[t].i; t4[4 * t + 1] = ents[t].j; t4[4 * t + 2] = ents[t].k; t4[4 * t + 3] = prob[t];
+  }
+  c->ttrip.alloc(t4.size() * 4);
+  check(hipMemcpyAsync(c->ttrip.p, t4.data(), t4.size() * 4, hipMemcpyHostToDevice, c->st), "entry table");
+  c->tCounter.alloc(64);
   for (long t0 = 0; t0 < nt; t0 += nb) {
     int b = (int)std::min<long>(nb, nt - t0);
-    if (prec == PREC_BF16X3) triples_build_w_pair(c, t0, b, c->trip_host);
-    else triples_build_w(c, prec, t0, b, c->trip_host);
+    if (prec == PREC_BF16X3) triples_build_w_pair(c, ents.data() + t0, b);
+    else triples_build_w(c, prec, ents.data() + t0, b);
     c->eng->timed("triples_e", 0.0, [&]() {
-      launch_triples_energy(o, v, b, (const int *)c->ttrip.p + 3 * t0, c->tWbuf.p, wf, v3, c->eps.d(), c->oovv.d(),
-                            c->t1.d(), c->tE.d() + t0, c->tPart.d(), c->st);
+      launch_triples_energy(o, v, b, (const int *)c->ttrip.p + 4 * t0, (const TripProb *)c->tProb.p, c->tWbuf.p, wf,
+                            ws, c->tE.d() + t0, c->tPart.d(), (int *)c->tCounter.p, c->st);
     });
   }
-  std::vector<double> e(nt);
-  if (nt) check(hipMemcpyAsync(e.data(), c->tE.p, nt * 8, hipMemcpyDeviceToHost, c->st), "triples copy");
+  check(hipMemcpyAsync(e.data(), c->tE.p, nt * 8, hipMemcpyDeviceToHost, c->st), "triples copy");
   check(hipStreamSynchronize(c->st), "stream sync");
+  return e;
+}
+
+// (T) over all triplets; per-triplet energies (table order, i = j = k
+// entries reported as 0) into etrip_out when non-null.
+double triples(ugx_ctx *c, int prec, std::vector<double> *etrip_out) {
+  const int o = c->no;
+  triples_operands(c, prec);
+  const long nt = (long)c->trip_host.size() / 3;
+  std::vector<TEnt> ents(nt);
+  for (long t = 0; t < nt; t++) ents[t] = TEnt{c, c->trip_host[3 * t], c->trip_host[3 * t + 1], c->trip_host[3 * t + 2]};
+  std::vector<int> prob(nt, 0);
+  std::vector<double> e = triples_entries(c, prec, ents, prob, {TripProb{c->eps.d(), c->oovv.d(), c->t1.d()}});
   double et = 0.0;
   for (double x : e) et += x;
   if (etrip_out) {
@@ -716,6 +864,7 @@ void ugx_default_options(ugx_options *opt) {
   opt->energy_form = UGX_ENERGY_REFERENCE;
   opt->print = 0;
   opt->polish_keep_diis = 1;
+  opt->graph = 0;
 }
 
 int ugx_create(ugx_ctx **out, int device) {
@@ -735,8 +884,10 @@ int ugx_create(ugx_ctx **out, int device) {
     return UGX_HIP_ERROR;
   }
   c->eng.reset(new Engine(c->st));
-  // tuning knob: LDS stages of the BF16x3 kernel (1 or 2)
+  // tuning knobs: LDS stages of the BF16x3 kernel (1 or 2); triplets per
+  // (T) batch
   if (const char *e = std::getenv("UGX_B3_STAGES")) gemm_set_b3_stages(std::atoi(e));
+  if (const char *e = std::getenv("UGX_T_BATCH")) c->t_batch = std::atoi(e);
   *out = c;
   return UGX_OK;
 }
@@ -745,6 +896,7 @@ void ugx_destroy(ugx_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
+  drop_graphs(c);
   c->eng.reset();
   hipStream_t st = c->st;
   delete c;
@@ -758,6 +910,8 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
   return guarded(c, [&]() {
     if (no <= 0 || nv <= 0 || !fock || !oooo || !ooov || !oovv || !ovov || !ovvv || !vvvv)
       throw UgxError(UGX_BAD_INPUT, "ugx_load_hamiltonian: sizes must be positive and all blocks given");
+    check(hipStreamSynchronize(c->st), "stream sync");
+    drop_graphs(c);
     c->eng->drop_cache();
     c->loaded = false;
     c->tables_ready = false;
@@ -876,7 +1030,7 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
     double low_best = 1e300;
     for (iter = 1; iter <= opt.maxiter; iter++) {
       amp_save(c);
-      residual(c, prec);
+      residual_step(c, prec, opt.graph != 0);
       rms = increment(c, nullptr);
       niter = iter;
       if (prec != PREC_FP64) nlow++;
@@ -954,30 +1108,29 @@ int ugx_w3(ugx_ctx *c, int i, int j, int k, int precision, double *W3) {
     if (!W3 || i < 0 || j < 0 || k < 0 || i >= c->no || j >= c->no || k >= c->no || precision < 0 ||
         precision > 2)
       throw UgxError(UGX_BAD_INPUT, "ugx_w3: bad arguments");
-    triples_prepare(c, precision);
-    if (precision == PREC_FP32) {
-      const long n = c->oovv_n();
-      c->f_t2.alloc(n * 4);
-      PermDesc pd{};
-      pd.rank = 1; pd.od[0] = (int)n; pd.istr[0] = 1; pd.in = c->t2.p; pd.out = c->f_t2.p; pd.in_t = 0;
-      pd.out_t = 1; pd.alpha = 1.0; pd.beta = 0.0;
-      launch_permute(pd, c->st);
-    }
-    const long v3 = (long)c->nv * c->nv * c->nv;
+    triples_operands(c, precision);
     c->tW_f32 = (precision != PREC_FP64) ? 1 : 0;
-    if (c->tWbuf.bytes < (size_t)v3 * 8) c->tWbuf.alloc(v3 * 8);
-    std::vector<int> one = {i, j, k};
-    if (precision == PREC_BF16X3) triples_build_w_pair(c, 0, 1, one);
-    else triples_build_w(c, precision, 0, 1, one);
+    const int T = tw_blk(c->tW_f32);
+    const long v = c->nv, ws = tw_stride(c->nv, T), nbk = tw_nb(c->nv, T);
+    c->tWbuf.alloc((size_t)ws * 8);
+    const TEnt one{c, i, j, k};
+    if (precision == PREC_BF16X3) triples_build_w_pair(c, &one, 1);
+    else triples_build_w(c, precision, &one, 1);
+    // blocked layout -> [a][b][c]
+    std::vector<double> wb(ws);
     if (c->tW_f32) {
-      std::vector<float> wf(v3);
-      check(hipMemcpyAsync(wf.data(), c->tWbuf.p, v3 * 4, hipMemcpyDeviceToHost, c->st), "W3 copy");
+      std::vector<float> wf(ws);
+      check(hipMemcpyAsync(wf.data(), c->tWbuf.p, ws * 4, hipMemcpyDeviceToHost, c->st), "W3 copy");
       check(hipStreamSynchronize(c->st), "stream sync");
-      for (long x = 0; x < v3; x++) W3[x] = wf[x];
+      for (long x = 0; x < ws; x++) wb[x] = wf[x];
     } else {
-      check(hipMemcpyAsync(W3, c->tWbuf.p, v3 * 8, hipMemcpyDeviceToHost, c->st), "W3 copy");
+      check(hipMemcpyAsync(wb.data(), c->tWbuf.p, ws * 8, hipMemcpyDeviceToHost, c->st), "W3 copy");
       check(hipStreamSynchronize(c->st), "stream sync");
     }
+    for (long a = 0; a < v; a++)
+      for (long b = 0; b < v; b++)
+        for (long cc = 0; cc < v; cc++)
+          W3[(a * v + b) * v + cc] = wb[tw_off_a(a, nbk, T) + tw_off_b(b, nbk, T) + tw_off_c(cc, T)];
     return UGX_OK;
   });
 }
@@ -1110,6 +1263,24 @@ int ugx_synchronize(ugx_ctx *c) {
     check(hipStreamSynchronize(c->st), "stream sync");
     return UGX_OK;
   });
+}
+
+int ugx_set_autotune(ugx_ctx *c, int on) {
+  return guarded(c, [&]() {
+    c->eng->set_autotune(on != 0);
+    return UGX_OK;
+  });
+}
+
+size_t ugx_tile_table(ugx_ctx *c, char *buf, size_t buflen) {
+  if (!c) return 0;
+  std::string s = c->eng->tile_table();
+  if (buf && buflen) {
+    size_t n = std::min(buflen - 1, s.size());
+    std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
+  }
+  return s.size() + 1;
 }
 
 int ugx_features(ugx_ctx *c, long n, const long *idx, float *X) {

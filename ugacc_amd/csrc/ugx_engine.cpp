@@ -334,7 +334,7 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
         d2.seg[0].sBz = chunk * d.seg[0].sbk;
         d2.ktotal = Ke;
         d2.C = W; d2.scm = Ce; d2.scn = 1; d2.scm1 = 0; d2.sCz = Re * Ce; d2.alpha = 1.0; d2.beta = 0.0;
-        launch_gemm(d2, nullptr, kprec, tile, st_);
+        launch_gemm(d2, nullptr, kprec, tile_for(d2, nullptr, kprec, tile), st_);
         if (direct) {
           launch_splitk_reduce((int)S, (int)Re, (int)Ce, W, alpha, beta, (double *)C.p, Ce, 1, st_);
         } else {
@@ -356,11 +356,11 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
   }
   if (direct) {
     d.C = (double *)C.p; d.scm = Ce; d.scn = 1; d.scm1 = 0; d.alpha = alpha; d.beta = beta;
-    launch_gemm(d, nullptr, kprec, -1, st_);
+    launch_gemm(d, nullptr, kprec, tile_for(d, nullptr, kprec, -1), st_);
   } else {
     double *tmp = (double *)slot(2, (size_t)Re * Ce * 8);
     d.C = tmp; d.scm = Ce; d.scn = 1; d.scm1 = 0; d.alpha = 1.0; d.beta = 0.0;
-    launch_gemm(d, nullptr, kprec, -1, st_);
+    launch_gemm(d, nullptr, kprec, tile_for(d, nullptr, kprec, -1), st_);
     TRef T = vref(tmp, {});
     for (char ch : rows + cols) T.dims.push_back(ext.at(ch));
     permute(rows + cols + "->" + sp.c, alpha, T, beta, C);
@@ -371,6 +371,94 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
     hip_check(hipEventRecord(e1, st_), "event record");
     pending_.push_back(Pending{prec_label(label, prec), e0, e1, fl});
   }
+}
+
+// ------------------------------------------------------------ autotuner
+namespace {
+
+// Largest element offset of one batch entry's output, plus one.
+long out_extent(const GemmDesc &d, long table_extent) {
+  if (d.offm) return table_extent;
+  auto part = [](int n, int div, long s, long s1) -> long {
+    if (div > 0 && div < n) return (long)((n - 1) / div) * s1 + (long)(div - 1) * s;
+    return (long)(n - 1) * s;
+  };
+  return part(d.M, d.cdiv, d.scm, d.scm1) + part(d.N, d.ndiv, d.scn, d.scn1) + 1;
+}
+
+}  // namespace
+
+int Engine::tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kprec, int fallback) {
+  if (!autotune_) return fallback;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return fallback;
+  std::string key = "M" + std::to_string(d.M) + " N" + std::to_string(d.N) + " p" + std::to_string(kprec) +
+                    (d.c_f32 ? " f" : " d") + (d.beta != 0.0 ? " acc" : "") + (d.cdiv > 0 ? " cd" : "") +
+                    (d.ndiv > 0 ? " nd" : "") + (d.offm ? " tab" : "") + (ptrs_host ? " grouped" : " b" + std::to_string(d.batch)) +
+                    (d.ktotal > 0 ? " kt" + std::to_string(d.ktotal) : "");
+  for (int s = 0; s < d.nseg; s++) {
+    const GemmSeg &g = d.seg[s];
+    key += " [K" + std::to_string(g.K) + " " + std::to_string(g.sam) + "," + std::to_string(g.sak) + "," +
+           std::to_string(g.sbk) + "," + std::to_string(g.sbn) + "]";
+  }
+  auto it = tiles_.find(key);
+  if (it != tiles_.end()) return it->second.tile;
+
+  const size_t es = d.c_f32 ? 4 : 8;
+  const long ext = out_extent(d, out_extent_hint_);
+  GemmDesc t = d;
+  const GemmPtrs *tab_dev = nullptr;
+  std::vector<GemmPtrs> tab;
+  size_t bytes;
+  if (ptrs_host) {
+    // up to 8 entries of the table, outputs laid end to end in scratch
+    const int nb = std::min(d.batch, 8);
+    t.batch = nb;
+    bytes = (size_t)nb * ext * es;
+    char *out = (char *)slot(4, bytes);
+    tab.assign(ptrs_host, ptrs_host + nb);
+    for (int z = 0; z < nb; z++) tab[z].C = (double *)(out + (size_t)z * ext * es);
+    void *dt = slot(5, tab.size() * sizeof(GemmPtrs));
+    hip_check(hipMemcpyAsync(dt, tab.data(), tab.size() * sizeof(GemmPtrs), hipMemcpyHostToDevice, st_),
+              "tuning table copy");
+    tab_dev = (const GemmPtrs *)dt;
+  } else {
+    // the whole batch (strided outputs); cap the scratch at 1 GiB by
+    // tuning on fewer batch entries
+    long nb = d.batch;
+    if (nb > 1 && d.sCz > 0) nb = std::max<long>(1, std::min<long>(nb, ((1L << 30) / (long)es - ext) / d.sCz + 1));
+    t.batch = (int)nb;
+    if (d.ktotal > 0) t.ktotal = std::min<long>(d.ktotal, (long)nb * d.seg[0].K);
+    bytes = ((size_t)(nb - 1) * (size_t)d.sCz + (size_t)ext) * es;
+    t.C = (double *)slot(4, bytes);
+  }
+  bytes = (bytes + 7) / 8 * 8;
+  launch_fill((long)(bytes / 8), 0.0, (double *)slot(4, bytes), st_);
+  hipEvent_t e0 = event(), e1 = event();
+  int best = fallback >= 0 ? fallback : gemm_pick_tile(d);
+  double best_ms = 1e300;
+  const int reps = 3;
+  for (int tile = 0; tile < gemm_num_tiles(); tile++) {
+    launch_gemm(t, tab_dev, kprec, tile, st_);  // warm
+    hip_check(hipEventRecord(e0, st_), "event record");
+    for (int r = 0; r < reps; r++) launch_gemm(t, tab_dev, kprec, tile, st_);
+    hip_check(hipEventRecord(e1, st_), "event record");
+    hip_check(hipEventSynchronize(e1), "event sync");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+    if (ms / reps < best_ms) { best_ms = ms / reps; best = tile; }
+  }
+  free_events_.push_back(e0);
+  free_events_.push_back(e1);
+  tiles_[key] = Tuned{best, best_ms};
+  return best;
+}
+
+std::string Engine::tile_table() const {
+  std::string s;
+  for (auto &kv : tiles_)
+    s += kv.first + " -> " + gemm_tile_name(kv.second.tile) + " " + std::to_string(kv.second.ms) + " ms\n";
+  return s;
 }
 
 }  // namespace ugx

@@ -61,10 +61,29 @@ class Engine {
   void collect_stats();  // waits for the stream, folds event pairs into stats_
   std::map<std::string, KernelStat> &stats() { return stats_; }
   double flops_total() const { return flops_; }
+  void add_flops(double f) { flops_ += f; }
+  bool profiling() const { return profiling_; }
   void reset_counters() { flops_ = 0.0; stats_.clear(); }
   void drop_cache();
   size_t cached_bytes() const;
   hipStream_t stream() const { return st_; }
+
+  // Tile of a GEMM launch.  Autotuning off: `fallback` (-1 = the size
+  // heuristic).  On: the first launch of each shape times every tile into a
+  // scratch output (the real operands, the same batch addressing) and the
+  // fastest is kept for the context's lifetime.  ptrs_host (grouped
+  // launches) is the host copy of the launch's pointer table; tuning then
+  // uses its first entries with their outputs redirected to scratch.
+  // (outputs addressed through offset tables: `extent` is one entry's
+  // largest offset plus one)
+  int tile_for(const GemmDesc &d, const GemmPtrs *ptrs_host, int kprec, int fallback, long extent = 0) {
+    out_extent_hint_ = extent;
+    return tile_for_impl(d, ptrs_host, kprec, fallback);
+  }
+  void set_autotune(bool on) { autotune_ = on; }
+  bool autotune() const { return autotune_; }
+  // "key tile ms" lines of the tuned shapes
+  std::string tile_table() const;
 
  private:
   struct Plan;
@@ -72,7 +91,15 @@ class Engine {
   const void *prepared(const TRef &T, const std::string &letters, const std::string &want, int dtype,
                        int slot_id);
   hipStream_t st_;
-  std::vector<std::pair<void *, size_t>> slots_{4, {nullptr, 0}};
+  std::vector<std::pair<void *, size_t>> slots_{6, {nullptr, 0}};
+  bool autotune_ = false;
+  long out_extent_hint_ = 0;
+  int tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kprec, int fallback);
+  struct Tuned {
+    int tile;
+    double ms;
+  };
+  std::map<std::string, Tuned> tiles_;
   std::map<std::tuple<const void *, std::string, int>, std::pair<void *, size_t>> cache_;
   bool profiling_ = false;
   struct Pending {

@@ -286,8 +286,10 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
         int m = m0 + (wm * TM + i) * 16 + MF::row(kq, r);
         int n = n0 + (wn * TN + j) * 16 + r16;
         if (m < M && n < N) {
-          long addr = (cdiv > 0 ? (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm : (long)m * d.scm) +
-                      (ndiv > 0 ? (long)(n / ndiv) * d.scn1 + (long)(n % ndiv) * d.scn : (long)n * d.scn);
+          long addr = d.offm ? d.offm[m] + d.offn[n]
+                             : (cdiv > 0 ? (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm : (long)m * d.scm) +
+                                   (ndiv > 0 ? (long)(n / ndiv) * d.scn1 + (long)(n % ndiv) * d.scn
+                                             : (long)n * d.scn);
           double v = alpha * (double)acc[i][j][r];
           if (d.c_f32) {
             float *C = (float *)Cv;
@@ -504,8 +506,10 @@ __global__ __launch_bounds__(NT) void k_gemm_b3p(GemmDesc d, const GemmPtrs *__r
         int m = m0 + (wm * TM + i) * 16 + 4 * kq + r;
         int n = n0 + (wn * TN + j) * 16 + r16;
         if (m < M && n < N) {
-          long addr = (cdiv > 0 ? (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm : (long)m * d.scm) +
-                      (ndiv > 0 ? (long)(n / ndiv) * d.scn1 + (long)(n % ndiv) * d.scn : (long)n * d.scn);
+          long addr = d.offm ? d.offm[m] + d.offn[n]
+                             : (cdiv > 0 ? (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm : (long)m * d.scm) +
+                                   (ndiv > 0 ? (long)(n / ndiv) * d.scn1 + (long)(n % ndiv) * d.scn
+                                             : (long)n * d.scn);
           double v = alpha * (double)acc[i][j][r];
           if (d.c_f32) {
             float *C = (float *)Cv;

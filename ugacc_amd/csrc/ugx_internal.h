@@ -60,6 +60,9 @@ struct GemmDesc {
   int vec_ok;
   // C is float (its pointers hold float data despite their type)
   int c_f32;
+  // when offm is non-null, C(m,n) lives at C[offm[m] + offn[n]] (device
+  // tables of M and N entries; any separable layout, e.g. the blocked (T) W)
+  const long *offm, *offn;
 };
 
 // Per-batch pointers for grouped launches (used by the (T) build where each
@@ -132,14 +135,37 @@ void launch_energy(int no, int nv, const double *fov, const double *t1, const do
 int energy_nparts(int no, int nv);
 
 // ------------------------------------------------------------ (T)
-// Energy epilogue over a batch of triplets: for each triplet (i,j,k) with W
-// (v^3, [a][b][c]) already built, sums the (T) bracket over a>=b>=c.
-// W is FP64, or FP32 when w_f32.  etrip receives ntrip values; part is
-// scratch of ntrip * triples_energy_npairs(nv, w_f32) doubles.
+// W3 of a triplet is stored blocked: T^3 sub-cubes (T = tw_blk), each
+// contiguous in [a][b][c] order, the sub-cubes in [A][B][C] order over
+// nb = ceil(nv / T) blocks per index, so W[a][b][c] lives at
+//   tw_off_a(a) + tw_off_b(b) + tw_off_c(c)
+// with tw_off_a(x) = (x / T) nb^2 T^3 + (x % T) T^2,
+//      tw_off_b(x) = (x / T) nb T^3 + (x % T) T,  tw_off_c(x) = (x / T) T^3 + x % T,
+// and one triplet occupies (nb T)^3 elements (positions past nv unused).
+// T = 16 for FP32 W (16 lanes of an MFMA output row write one aligned
+// 64-byte run), 12 for FP64 W (the energy kernel's LDS holds six padded
+// 12^3 FP64 sub-cubes).
+inline int tw_blk(int w_f32) { return w_f32 ? 16 : 12; }
+inline long tw_nb(int nv, int T) { return (nv + T - 1) / T; }
+inline long tw_stride(int nv, int T) { const long e = tw_nb(nv, T) * T; return e * e * e; }
+inline long tw_off_a(long x, long nb, int T) { return (x / T) * nb * nb * T * T * T + (x % T) * T * T; }
+inline long tw_off_b(long x, long nb, int T) { return (x / T) * nb * T * T * T + (x % T) * T; }
+inline long tw_off_c(long x, int T) { return (x / T) * T * T * T + x % T; }
+// Energy epilogue over a batch of triplet entries: entry t is (i, j, k,
+// problem) = trip4[4t .. 4t+3]; its W (blocked layout above) starts at
+// W + t * wstride; probs[problem] gives that problem's Fock diagonal, oovv
+// block and t1 (all problems share no and nv).  Sums the (T) bracket over
+// a>=b>=c into etrip[t].  W is FP64, or FP32 when w_f32.  A persistent
+// kernel: as many workgroups as fit on the device drain a queue of
+// (entry, block pair) items; counter is one device int (reset by the launch
+// with a kernel), part holds ntrip * triples_energy_npairs(nv, w_f32)
+// doubles.
+struct TripProb {
+  const double *eps, *oovv, *t1;
+};
 int triples_energy_npairs(int nv, int w_f32);
-void launch_triples_energy(int no, int nv, int ntrip, const int *trip_ijk, const void *W, int w_f32, long wstride,
-                           const double *eps, const double *oovv, const double *t1, double *etrip, double *part,
-                           hipStream_t st);
+void launch_triples_energy(int no, int nv, int ntrip, const int *trip4, const TripProb *probs, const void *W,
+                           int w_f32, long wstride, double *etrip, double *part, int *counter, hipStream_t st);
 
 // ------------------------------------------------------------ learned guess
 constexpr int MLP_NFEAT = 8;

@@ -14,12 +14,15 @@
 // triplet and a pair of E-wide index blocks A >= B, and walks the blocks
 // C <= B: for each (A, B, C) it stages the six permuted E^3 sub-cubes
 // W[A][B][C], W[A][C][B], W[B][A][C], W[B][C][A], W[C][A][B], W[C][B][A]
-// into LDS, each read along its own contiguous index so the loads are
-// coalesced, and every W element is read from memory about once.
+// into LDS.  W is stored in the blocked layout the W3 GEMMs write (E^3
+// sub-cubes, each contiguous), so every staged sub-cube is one contiguous
+// read and every W element is read from memory about once.
 // W is FP64 (FP64 form) or FP32 (low-precision forms); all sums are FP64 and
 // the per-(triplet, block pair) partials are reduced per triplet in a fixed
 // order, so the result is bitwise repeatable.
 #include "ugx_internal.h"
+
+#include <algorithm>
 
 namespace ugx {
 namespace {
@@ -47,29 +50,41 @@ __device__ inline void pair_of(int p, int &A, int &B) {
   B = p - A * (A + 1) / 2;
 }
 
-template <typename WT, int E>
-__global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, const int *__restrict__ trip,
+template <typename WT, int E, typename OT>
+__global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, int ntrip, const int *__restrict__ trip,
+                                                       const TripProb *__restrict__ probs,
                                                        const WT *__restrict__ Wall, long wstride,
-                                                       const double *__restrict__ eps,
-                                                       const double *__restrict__ oovv,
-                                                       const double *__restrict__ t1, double *__restrict__ part,
-                                                       int npairs) {
+                                                       double *__restrict__ part, int npairs,
+                                                       int *__restrict__ counter) {
   constexpr int E3 = E * E * E, E2 = E * E;
   // padded cube layout idx(a,b,c) = a*SA + b*SB + c: the strides are odd, so
   // the stores along a or b (and the reads along c) do not collide in banks
   constexpr int SB = E + 1, SA = E * (E + 1) + 1;
   __shared__ WT S[6][E * SA];
   // disconnected-term operands of the three blocks X, Y in {A, B, C}:
-  // O[s][X][Y][x][y] = <pq|xy> for the slices s = (ij), (ik), (jk);
+  // O[s][X][Y][x][y] = <pq|xy> for the slices s = (ij), (ik), (jk) (FP32
+  // alongside FP32 W, FP64 otherwise);
   // T1[s][X][x] = t1 of k, j, i (the amplitude that multiplies slice s)
-  __shared__ double O[3][3][3][E2];
+  __shared__ OT O[3][3][3][E2];
   __shared__ double T1[3][3][E];
   __shared__ double EPS[3][E];
-  const int t = blockIdx.y;
-  const int pidx = blockIdx.x;
+  __shared__ int item_sh;
+  const long total = (long)ntrip * npairs;
+  // persistent: each workgroup takes (entry, block pair) items from the
+  // queue until it is empty; every workgroup reaches the exit test
+  for (;;) {
+  __syncthreads();  // the previous item is done with LDS and item_sh
+  if (threadIdx.x == 0) item_sh = atomicAdd(counter, 1);
+  __syncthreads();
+  const long item = item_sh;
+  if (item >= total) break;
+  const int t = (int)(item / npairs);
+  const int pidx = (int)(item % npairs);
   int Ab, Bb;
   pair_of(pidx, Ab, Bb);
-  const int i = trip[3 * t], j = trip[3 * t + 1], k = trip[3 * t + 2];
+  const int i = trip[4 * t], j = trip[4 * t + 1], k = trip[4 * t + 2];
+  const TripProb pr = probs[trip[4 * t + 3]];
+  const double *__restrict__ eps = pr.eps, *__restrict__ oovv = pr.oovv, *__restrict__ t1 = pr.t1;
   const WT *W = Wall + (long)t * wstride;
   const long v = nv, v2 = (long)nv * nv;
   const double *Osl[3] = {oovv + ((long)i * no + j) * v2, oovv + ((long)i * no + k) * v2,
@@ -82,8 +97,8 @@ __global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, const int
   // V at a permutation (x, y, z) of the point, given as block slots (bx, by,
   // bz) in {0: A, 1: B, 2: C} and local offsets (lx, ly, lz)
   auto Vx = [&](int bx, int lx, int by, int ly, int bz, int lz, double w, double inv) {
-    return (w + O[0][bx][by][lx * E + ly] * T1[0][bz][lz] + O[1][bx][bz][lx * E + lz] * T1[1][by][ly] +
-            O[2][by][bz][ly * E + lz] * T1[2][bx][lx]) * inv;
+    return (w + (double)O[0][bx][by][lx * E + ly] * T1[0][bz][lz] + (double)O[1][bx][bz][lx * E + lz] * T1[1][by][ly] +
+            (double)O[2][by][bz][ly * E + lz] * T1[2][bx][lx]) * inv;
   };
   auto stage_pairs = [&](bool all) {
     // O blocks for pairs touching C (or all pairs on the first step) and the
@@ -93,7 +108,7 @@ __global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, const int
       int bx = pr / 3, by = pr % 3;
       if (!all && bx != 2 && by != 2) continue;
       int gx = X0[bx] + xy / E, gy = X0[by] + xy % E;
-      O[s][bx][by][xy] = (gx < nv && gy < nv) ? Osl[s][gx * v + gy] : 0.0;
+      O[s][bx][by][xy] = (OT)((gx < nv && gy < nv) ? Osl[s][gx * v + gy] : 0.0);
     }
     for (int e = threadIdx.x; e < 3 * 3 * E; e += NT) {
       int s = e / (3 * E), r = e % (3 * E), bx = r / E, lx = r % E;
@@ -106,24 +121,28 @@ __global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, const int
 
   double s = 0.0;
   const int A0 = X0[0], B0 = X0[1];
+  // W is blocked (E^3 sub-cubes, each contiguous in its own [x][y][z]
+  // order): sub-cube (X, Y, Z) starts at ((X nb + Y) nb + Z) E^3
+  const long nbk = (nv + E - 1) / E;
+  auto cube = [&](int X, int Y, int Z) { return W + ((X * nbk + Y) * nbk + Z) * (long)E3; };
   for (int Cb = 0; Cb <= Bb; Cb++) {
     const int C0 = Cb * E;
     X0[2] = C0;
-    auto ld = [&](int x, int y, int z) -> WT {
-      return (x < nv && y < nv && z < nv) ? W[x * v2 + y * v + z] : WT(0);
-    };
     auto at = [](int a, int b, int c) { return a * SA + b * SB + c; };
     // Stage the six sub-cubes, S[q][at(a,b,c)] = W at permutation q of
-    // (A0+a, B0+b, C0+c).  Thread e walks the cube with its fastest index
-    // equal to the sub-cube's contiguous index in W.
+    // (A0+a, B0+b, C0+c): each is one contiguous E^3 chunk of W, read in
+    // order (entries past nv are never used: the points that would need
+    // them are skipped below).
+    const WT *c0 = cube(Ab, Bb, Cb), *c1 = cube(Ab, Cb, Bb), *c2 = cube(Bb, Ab, Cb);
+    const WT *c3 = cube(Bb, Cb, Ab), *c4 = cube(Cb, Ab, Bb), *c5 = cube(Cb, Bb, Ab);
     for (int e = threadIdx.x; e < E3; e += NT) {
       int f = e % E, m = (e / E) % E, sl = e / (E * E);  // fastest, middle, slowest
-      S[0][at(sl, m, f)] = ld(A0 + sl, B0 + m, C0 + f);   // W[a][b][c], c contiguous
-      S[1][at(sl, f, m)] = ld(A0 + sl, C0 + m, B0 + f);   // W[a][c][b], b contiguous
-      S[2][at(m, sl, f)] = ld(B0 + sl, A0 + m, C0 + f);   // W[b][a][c], c contiguous
-      S[3][at(f, sl, m)] = ld(B0 + sl, C0 + m, A0 + f);   // W[b][c][a], a contiguous
-      S[4][at(m, f, sl)] = ld(C0 + sl, A0 + m, B0 + f);   // W[c][a][b], b contiguous
-      S[5][at(f, m, sl)] = ld(C0 + sl, B0 + m, A0 + f);   // W[c][b][a], a contiguous
+      S[0][at(sl, m, f)] = c0[e];   // W[a][b][c]
+      S[1][at(sl, f, m)] = c1[e];   // W[a][c][b]
+      S[2][at(m, sl, f)] = c2[e];   // W[b][a][c]
+      S[3][at(f, sl, m)] = c3[e];   // W[b][c][a]
+      S[4][at(m, f, sl)] = c4[e];   // W[c][a][b]
+      S[5][at(f, m, sl)] = c5[e];   // W[c][b][a]
     }
     stage_pairs(Cb == 0);
     __syncthreads();
@@ -149,6 +168,11 @@ __global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, const int
   }
   double tot = block_sum(s);
   if (threadIdx.x == 0) part[(long)t * npairs + pidx] = tot;
+  }
+}
+
+__global__ void k_zero_counter(int *counter) {
+  if (threadIdx.x == 0) counter[0] = 0;
 }
 
 __global__ void k_triples_reduce(int npairs, int ntrip, const double *part, double *etrip) {
@@ -159,33 +183,40 @@ __global__ void k_triples_reduce(int npairs, int ntrip, const double *part, doub
   etrip[t] = s;
 }
 
-template <typename WT, int E>
-void launch_e(int no, int nv, int ntrip, const int *trip_ijk, const void *W, long wstride, const double *eps,
-              const double *oovv, const double *t1, double *etrip, double *part, hipStream_t st) {
+template <typename WT, int E, typename OT>
+void launch_e(int no, int nv, int ntrip, const int *trip4, const TripProb *probs, const void *W, long wstride,
+              double *etrip, double *part, int *counter, hipStream_t st) {
   int nb = (nv + E - 1) / E;
   int npairs = nb * (nb + 1) / 2;
-  hipLaunchKernelGGL((k_triples_energy<WT, E>), dim3(npairs, ntrip), dim3(NT), 0, st, no, nv, trip_ijk,
-                     (const WT *)W, wstride, eps, oovv, t1, part, npairs);
+  // as many workgroups as can be resident at once
+  int dev = 0, ncu = 0, per = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_triples_energy<WT, E, OT>, NT, 0);
+  long grid = (long)std::max(ncu, 1) * std::max(per, 1);
+  grid = std::min<long>(grid, (long)ntrip * npairs);
+  hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, counter);
+  hipLaunchKernelGGL((k_triples_energy<WT, E, OT>), dim3((unsigned)grid), dim3(NT), 0, st, no, nv, ntrip, trip4,
+                     probs, (const WT *)W, wstride, part, npairs, counter);
   hipLaunchKernelGGL(k_triples_reduce, dim3((ntrip + NT - 1) / NT), dim3(NT), 0, st, npairs, ntrip, part, etrip);
 }
 
 }  // namespace
 
-// Sub-cube edge 12: six padded 12^3 cubes (45 KB FP32, 90 KB FP64) plus
-// 31 KB of staged <pq|xy> blocks.
+// Sub-cube edge = the W block edge: 16 for FP32 W (six padded 16^3 FP32
+// sub-cubes, 105 KB, plus 28 KB of FP32 <pq|xy> blocks), 12 for FP64 W
+// (90 KB plus 31 KB of FP64 blocks).
 int triples_energy_npairs(int nv, int w_f32) {
-  int E = 12;
-  (void)w_f32;
+  const int E = tw_blk(w_f32);
   int nb = (nv + E - 1) / E;
   return nb * (nb + 1) / 2;
 }
 
-void launch_triples_energy(int no, int nv, int ntrip, const int *trip_ijk, const void *W, int w_f32, long wstride,
-                           const double *eps, const double *oovv, const double *t1, double *etrip, double *part,
-                           hipStream_t st) {
+void launch_triples_energy(int no, int nv, int ntrip, const int *trip4, const TripProb *probs, const void *W,
+                           int w_f32, long wstride, double *etrip, double *part, int *counter, hipStream_t st) {
   if (ntrip <= 0) return;
-  if (w_f32) launch_e<float, 12>(no, nv, ntrip, trip_ijk, W, wstride, eps, oovv, t1, etrip, part, st);
-  else launch_e<double, 12>(no, nv, ntrip, trip_ijk, W, wstride, eps, oovv, t1, etrip, part, st);
+  if (w_f32) launch_e<float, 16, float>(no, nv, ntrip, trip4, probs, W, wstride, etrip, part, counter, st);
+  else launch_e<double, 12, double>(no, nv, ntrip, trip4, probs, W, wstride, etrip, part, counter, st);
 }
 
 }  // namespace ugx

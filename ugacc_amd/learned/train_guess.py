@@ -154,10 +154,17 @@ def evaluate(args):
                 row["s_" + key] = r["seconds_ccsd"]
                 row["e_" + key] = r["e_ccsd_final"]
                 if fname == "fp64" and gname == "mp2":
-                    _, t2c = ctx.amplitudes()
+                    t1c, t2c = ctx.amplitudes()
         ctx.ccsd(maxiter=0, wfn_ccsd_t=0, guess=ugx.GUESS_MLP)
         _, t2g = ctx.amplitudes()
         mp2 = ref.blocks["oovv"] / denominators(np.diag(ref.fock), no, nv)
+        if args.ceiling:
+            # what a perfect guess could save: the converged T2 with T1 = 0,
+            # and the converged T1 with the MP2 T2 (FP64, same convergence)
+            for key, a1, a2 in (("exact_t2", np.zeros_like(t1c), t2c), ("exact_t1", t1c, mp2)):
+                ctx.set_amplitudes(a1, a2)
+                r = ctx.ccsd(r_convergence=args.conv, wfn_ccsd_t=0, maxiter=100, guess=ugx.GUESS_USER)
+                row["iter_fp64_" + key] = r["niter"]
         row["t2_err_mp2"] = float(np.linalg.norm(t2c - mp2))
         row["t2_err_learned"] = float(np.linalg.norm(t2c - t2g))
         row["dE_fp64"] = abs(row["e_fp64_mp2"] - row["e_fp64_learned"])
@@ -171,8 +178,10 @@ def evaluate(args):
                 row["s_mixed_mp2"], row["s_mixed_learned"], row["dE_fp64"], row["dE_mixed"]))
     summary = dict(clusters=len(rows), conv=args.conv, mixed_conv=args.mixed_conv)
     for k in ("iter_fp64_mp2", "iter_fp64_learned", "iter_mixed_mp2", "iter_mixed_learned", "s_fp64_mp2",
-              "s_fp64_learned", "s_mixed_mp2", "s_mixed_learned", "t2_err_mp2", "t2_err_learned"):
-        summary["mean_" + k] = float(np.mean([r[k] for r in rows]))
+              "s_fp64_learned", "s_mixed_mp2", "s_mixed_learned", "t2_err_mp2", "t2_err_learned",
+              "iter_fp64_exact_t2", "iter_fp64_exact_t1"):
+        if k in rows[0]:
+            summary["mean_" + k] = float(np.mean([r[k] for r in rows]))
     summary["max_dE_fp64"] = float(max(r["dE_fp64"] for r in rows))
     summary["max_dE_mixed"] = float(max(r["dE_mixed"] for r in rows))
     summary["rows"] = rows
@@ -197,6 +206,8 @@ def main(argv=None):
     e.add_argument("--clusters", type=int, default=16)
     e.add_argument("--conv", type=float, default=1e-10)
     e.add_argument("--mixed-conv", type=float, default=1e-7)
+    e.add_argument("--ceiling", action="store_true",
+                   help="also solve from the exact converged T2 (T1 = 0) and from the exact T1 (MP2 T2)")
     e.add_argument("--weights", default=WEIGHTS)
     e.add_argument("--out", default=None)
     args = ap.parse_args(argv)

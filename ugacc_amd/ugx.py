@@ -25,7 +25,7 @@ EXPORTS = [
     "ugx_default_options", "ugx_create", "ugx_destroy", "ugx_last_error", "ugx_load_hamiltonian",
     "ugx_set_amplitudes", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
     "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_set_profiling",
-    "ugx_kernel_stats", "ugx_gemm", "ugx_synchronize", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
+    "ugx_kernel_stats", "ugx_gemm", "ugx_synchronize", "ugx_set_autotune", "ugx_tile_table", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
     "ugx_mlp_last_error", "ugx_mlp_set_params", "ugx_mlp_get_params", "ugx_mlp_set_scaling", "ugx_mlp_forward",
     "ugx_mlp_loss_grad", "ugx_mlp_train", "ugx_features", "ugx_set_guess_network",
 ]
@@ -43,7 +43,7 @@ class Options(ctypes.Structure):
         ("r_convergence", ctypes.c_double), ("ooc", ctypes.c_int), ("diis_nvec", ctypes.c_int),
         ("diis_min_vecs", ctypes.c_int), ("diis_policy", ctypes.c_int), ("precision", ctypes.c_int),
         ("polish_threshold", ctypes.c_double), ("guess", ctypes.c_int), ("energy_form", ctypes.c_int),
-        ("print", ctypes.c_int), ("polish_keep_diis", ctypes.c_int),
+        ("print", ctypes.c_int), ("polish_keep_diis", ctypes.c_int), ("graph", ctypes.c_int),
     ]
 
 
@@ -94,6 +94,9 @@ def load_library(path=LIBPATH):
     L.ugx_gemm.argtypes = [vp, ctypes.c_char, ctypes.c_char, ip, ip, ip, cd, vp, ctypes.c_long, vp, ctypes.c_long,
                            cd, vp, ctypes.c_long, ip, ip]
     L.ugx_synchronize.argtypes = [vp]
+    L.ugx_set_autotune.argtypes = [vp, ip]
+    L.ugx_tile_table.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+    L.ugx_tile_table.restype = ctypes.c_size_t
     fp, lp, cl = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_long), ctypes.c_long
     L.ugx_mlp_num_params.argtypes = []
     L.ugx_mlp_num_params.restype = cl
@@ -111,6 +114,7 @@ def load_library(path=LIBPATH):
     L.ugx_features.argtypes = [vp, cl, lp, fp]
     L.ugx_set_guess_network.argtypes = [vp, vp]
     non_int = ("ugx_default_options", "ugx_destroy", "ugx_last_error", "ugx_set_profiling", "ugx_kernel_stats",
+               "ugx_tile_table",
                "ugx_mlp_num_params", "ugx_mlp_destroy", "ugx_mlp_last_error")
     for name in EXPORTS:
         if name not in non_int:
@@ -269,6 +273,15 @@ class Context:
 
     def synchronize(self):
         self._check(self.L.ugx_synchronize(self.h))
+
+    def set_autotune(self, on=True):
+        self._check(self.L.ugx_set_autotune(self.h, 1 if on else 0))
+
+    def tile_table(self):
+        n = self.L.ugx_tile_table(self.h, None, 0)
+        buf = ctypes.create_string_buffer(max(n, 1))
+        self.L.ugx_tile_table(self.h, buf, n)
+        return buf.value.decode()
 
     def features(self, idx=None, n=None):
         """Raw learned-guess features of the amplitudes idx (flattened ijab)."""
