@@ -179,6 +179,17 @@ __device__ inline void tile_of(int bid, int nblk, int tiles_m, int tiles_n, int 
   tn = in / rows;
 }
 
+// Address parts of C(m, n): C[c_row_off(m) + c_col_off(n)] (an offset table,
+// a two-level split, or a plain stride per dimension).
+__device__ inline long c_row_off(const GemmDesc &d, int m) {
+  if (d.offm) return d.offm[m];
+  return d.cdiv > 0 ? (long)(m / d.cdiv) * d.scm1 + (long)(m % d.cdiv) * d.scm : (long)m * d.scm;
+}
+__device__ inline long c_col_off(const GemmDesc &d, int n) {
+  if (d.offn) return d.offn[n];
+  return d.ndiv > 0 ? (long)(n / d.ndiv) * d.scn1 + (long)(n % d.ndiv) * d.scn : (long)n * d.scn;
+}
+
 // FP64 / FP32 main kernel.  Two LDS stages: while the MFMAs consume slice t
 // from one stage, slice t+1 (loaded into registers one iteration earlier) is
 // written to the other stage and the global loads of slice t+2 are issued,
@@ -276,20 +287,28 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
   void *Cv = ptrs ? (void *)ptrs[z].C
                   : (d.c_f32 ? (void *)((float *)d.C + z * d.sCz) : (void *)(d.C + z * d.sCz));
   const double alpha = d.alpha, beta = d.beta;
-  const int cdiv = d.cdiv, ndiv = d.ndiv;
+  // this thread's TM*4 output rows and TN columns, addressed once
+  long ro[TM][4], co[TN];
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int m = m0 + (wm * TM + i) * 16 + MF::row(kq, r);
+      ro[i][r] = m < M ? c_row_off(d, m) : -1;
+    }
+#pragma unroll
+  for (int j = 0; j < TN; j++) {
+    const int n = n0 + (wn * TN + j) * 16 + r16;
+    co[j] = n < N ? c_col_off(d, n) : -1;
+  }
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
     for (int j = 0; j < TN; j++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int m = m0 + (wm * TM + i) * 16 + MF::row(kq, r);
-        int n = n0 + (wn * TN + j) * 16 + r16;
-        if (m < M && n < N) {
-          long addr = d.offm ? d.offm[m] + d.offn[n]
-                             : (cdiv > 0 ? (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm : (long)m * d.scm) +
-                                   (ndiv > 0 ? (long)(n / ndiv) * d.scn1 + (long)(n % ndiv) * d.scn
-                                             : (long)n * d.scn);
+        if (ro[i][r] >= 0 && co[j] >= 0) {
+          const long addr = ro[i][r] + co[j];
           double v = alpha * (double)acc[i][j][r];
           if (d.c_f32) {
             float *C = (float *)Cv;
@@ -496,20 +515,28 @@ __global__ __launch_bounds__(NT) void k_gemm_b3p(GemmDesc d, const GemmPtrs *__r
   void *Cv = ptrs ? (void *)ptrs[z].C
                   : (d.c_f32 ? (void *)((float *)d.C + z * d.sCz) : (void *)(d.C + z * d.sCz));
   const double alpha = d.alpha, beta = d.beta;
-  const int cdiv = d.cdiv, ndiv = d.ndiv;
+  // this thread's TM*4 output rows and TN columns, addressed once
+  long ro[TM][4], co[TN];
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int m = m0 + (wm * TM + i) * 16 + 4 * kq + r;
+      ro[i][r] = m < M ? c_row_off(d, m) : -1;
+    }
+#pragma unroll
+  for (int j = 0; j < TN; j++) {
+    const int n = n0 + (wn * TN + j) * 16 + r16;
+    co[j] = n < N ? c_col_off(d, n) : -1;
+  }
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
     for (int j = 0; j < TN; j++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int m = m0 + (wm * TM + i) * 16 + 4 * kq + r;
-        int n = n0 + (wn * TN + j) * 16 + r16;
-        if (m < M && n < N) {
-          long addr = d.offm ? d.offm[m] + d.offn[n]
-                             : (cdiv > 0 ? (long)(m / cdiv) * d.scm1 + (long)(m % cdiv) * d.scm : (long)m * d.scm) +
-                                   (ndiv > 0 ? (long)(n / ndiv) * d.scn1 + (long)(n % ndiv) * d.scn
-                                             : (long)n * d.scn);
+        if (ro[i][r] >= 0 && co[j] >= 0) {
+          const long addr = ro[i][r] + co[j];
           double v = alpha * (double)acc[i][j][r];
           if (d.c_f32) {
             float *C = (float *)Cv;

@@ -142,10 +142,10 @@ int energy_nparts(int no, int nv);
 // with tw_off_a(x) = (x / T) nb^2 T^3 + (x % T) T^2,
 //      tw_off_b(x) = (x / T) nb T^3 + (x % T) T,  tw_off_c(x) = (x / T) T^3 + x % T,
 // and one triplet occupies (nb T)^3 elements (positions past nv unused).
-// T = 16 for FP32 W (16 lanes of an MFMA output row write one aligned
-// 64-byte run), 12 for FP64 W (the energy kernel's LDS holds six padded
-// 12^3 FP64 sub-cubes).
-inline int tw_blk(int w_f32) { return w_f32 ? 16 : 12; }
+// T = 12 for both element types: the energy kernel's LDS holds six padded
+// 12^3 sub-cubes plus the <pq|xy> blocks with room for two workgroups per
+// CU at FP32 (measured faster than T = 16 with one).
+inline int tw_blk(int w_f32) { (void)w_f32; return 12; }
 inline long tw_nb(int nv, int T) { return (nv + T - 1) / T; }
 inline long tw_stride(int nv, int T) { const long e = tw_nb(nv, T) * T; return e * e * e; }
 inline long tw_off_a(long x, long nb, int T) { return (x / T) * nb * nb * T * T * T + (x % T) * T * T; }

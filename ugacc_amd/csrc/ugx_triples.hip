@@ -203,9 +203,8 @@ void launch_e(int no, int nv, int ntrip, const int *trip4, const TripProb *probs
 
 }  // namespace
 
-// Sub-cube edge = the W block edge: 16 for FP32 W (six padded 16^3 FP32
-// sub-cubes, 105 KB, plus 28 KB of FP32 <pq|xy> blocks), 12 for FP64 W
-// (90 KB plus 31 KB of FP64 blocks).
+// Sub-cube edge = the W block edge, 12: six padded 12^3 sub-cubes (45 KB
+// FP32, 90 KB FP64) plus 31 KB of FP64 <pq|xy> blocks.
 int triples_energy_npairs(int nv, int w_f32) {
   const int E = tw_blk(w_f32);
   int nb = (nv + E - 1) / E;
@@ -215,7 +214,7 @@ int triples_energy_npairs(int nv, int w_f32) {
 void launch_triples_energy(int no, int nv, int ntrip, const int *trip4, const TripProb *probs, const void *W,
                            int w_f32, long wstride, double *etrip, double *part, int *counter, hipStream_t st) {
   if (ntrip <= 0) return;
-  if (w_f32) launch_e<float, 16, float>(no, nv, ntrip, trip4, probs, W, wstride, etrip, part, counter, st);
+  if (w_f32) launch_e<float, 12, double>(no, nv, ntrip, trip4, probs, W, wstride, etrip, part, counter, st);
   else launch_e<double, 12, double>(no, nv, ntrip, trip4, probs, W, wstride, etrip, part, counter, st);
 }
 
