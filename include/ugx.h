@@ -189,6 +189,30 @@ int ugx_gemm(ugx_ctx *ctx, char opa, char opb, int M, int N, int K, double alpha
 /* Synchronises the context's stream. */
 int ugx_synchronize(ugx_ctx *ctx);
 
+/* One closed-shell problem of a batch: sizes, Fock matrix and the six
+ * Dirac blocks, in the layouts of ugx_load_hamiltonian. */
+typedef struct {
+  int no, nv;
+  const double *fock, *oooo, *ooov, *oovv, *ovov, *ovvv, *vvvv;
+} ugx_problem;
+
+/* Solves nprob independent problems of any sizes (the reference solves one
+ * at a time through CCWfn::compute_energy, ccwfn.cc:188-258, and
+ * tcorr_ooc_TJL, :795-852).  Each problem gets its own stream and device
+ * buffers; nworkers host threads take problems largest first and run their
+ * CCSD iterations concurrently (opt->graph replays each iteration's
+ * residual from a hipGraph).  With opt->wfn_ccsd_t, (T) then runs once per
+ * (no, nv) class: grouped W3 GEMMs whose pointer tables span every triplet
+ * of every problem of the class, and one persistent energy kernel.
+ * results[p] are those of ugx_ccsd for problem p; seconds_ccsd is the
+ * problem's CCSD wall time on its worker, seconds_triples its share of the
+ * class's (T) time.  *wall_seconds receives the whole batch's wall time.
+ * The context supplies the device, the autotuning setting and the guess
+ * network.  Returns UGX_NOT_CONVERGED when any problem did not converge
+ * (its (T) is then skipped). */
+int ugx_batch_solve(ugx_ctx *ctx, int nprob, const ugx_problem *probs, const ugx_options *opt, int nworkers,
+                    ugx_result *results, double *wall_seconds);
+
 /* GEMM tile autotuning (off by default): when on, the first launch of each
  * contraction shape times every tile of the kernel's tile table on the
  * real operands (into scratch) and keeps the fastest for the context's

@@ -169,3 +169,31 @@ def test_graph_replayed_iterations_are_bitwise_eager(prec):
     np.testing.assert_array_equal(t2b, t2a)
     np.testing.assert_array_equal(t2c, t2a)
     np.testing.assert_array_equal(t1c, t1a)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [ugx.PREC_FP64, ugx.PREC_BF16X3])
+def test_batch_matches_single_solves(prec):
+    # uneven sizes in one batch; each problem's result must be the one its
+    # own single solve gives (the per-problem kernels are the same, and the
+    # batched (T) builds each triplet's W3 with the same GEMM shapes)
+    from ugacc_amd.synthetic import random_hamiltonian
+    announce("batch of 7 uneven problems, prec=%d" % prec)
+    shapes = [(3, 8, 81), (4, 13, 82), (3, 8, 83), (2, 20, 84), (4, 13, 85), (3, 8, 86), (5, 10, 87)]
+    problems = []
+    for no, nv, seed in shapes:
+        fock, ints = random_hamiltonian(no, nv, seed=seed)
+        problems.append((no, nv, fock, ugx.split_blocks(ints, no)))
+    c, _, _ = loaded(3, 6, seed=80)
+    kw = dict(r_convergence=1e-10, wfn_ccsd_t=1, precision=prec)
+    single = []
+    for no, nv, fock, blocks in problems:
+        c.load_hamiltonian(no, nv, fock, blocks)
+        single.append(c.ccsd(**kw))
+    batch, wall = c.batch_solve(problems, nworkers=4, graph=1, **kw)
+    assert wall > 0
+    for s, b in zip(single, batch):
+        assert b["converged"] and s["niter"] == b["niter"]
+        assert abs(s["e_ccsd_final"] - b["e_ccsd_final"]) < 1e-13
+        assert abs(s["e_t"] - b["e_t"]) < 1e-13
+        assert b["seconds_ccsd"] > 0 and b["seconds_triples"] > 0

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -24,6 +25,8 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <map>
+#include <thread>
 #include <vector>
 
 #include "../../include/ugx.h"
@@ -1262,6 +1265,114 @@ int ugx_synchronize(ugx_ctx *c) {
   return guarded(c, [&]() {
     check(hipStreamSynchronize(c->st), "stream sync");
     return UGX_OK;
+  });
+}
+
+int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_options *opt_in, int nworkers,
+                    ugx_result *results, double *wall_seconds) {
+  return guarded(c, [&]() {
+    if (nprob <= 0 || !probs || !results || nworkers < 1 || nworkers > 64)
+      throw UgxError(UGX_BAD_INPUT, "ugx_batch_solve: bad arguments");
+    for (int p = 0; p < nprob; p++) {
+      const ugx_problem &q = probs[p];
+      if (q.no <= 0 || q.nv <= 0 || !q.fock || !q.oooo || !q.ooov || !q.oovv || !q.ovov || !q.ovvv || !q.vvvv)
+        throw UgxError(UGX_BAD_INPUT, "ugx_batch_solve: problem " + std::to_string(p) + " is incomplete");
+    }
+    ugx_options opt;
+    if (opt_in) opt = *opt_in; else ugx_default_options(&opt);
+    if (opt.guess == UGX_GUESS_USER) throw UgxError(UGX_BAD_INPUT, "ugx_batch_solve: UGX_GUESS_USER is per context");
+    const bool want_t = opt.wfn_ccsd_t != 0;
+    opt.wfn_ccsd_t = 0;  // (T) runs batched per size class below
+    auto t_start = std::chrono::steady_clock::now();
+    // one context (stream, engine, buffers) per problem
+    std::vector<std::unique_ptr<ugx_ctx, void (*)(ugx_ctx *)>> sub;
+    sub.reserve(nprob);
+    for (int p = 0; p < nprob; p++) {
+      ugx_ctx *s = nullptr;
+      int rc = ugx_create(&s, c->device);
+      if (rc != UGX_OK) throw UgxError(rc, "ugx_batch_solve: context creation failed");
+      sub.emplace_back(s, ugx_destroy);
+      s->eng->set_autotune(c->eng->autotune());
+      s->mlp = c->mlp;
+      s->t_batch = c->t_batch;
+    }
+    // CCSD: workers take problems largest first (o^2 v^4 ladder cost)
+    std::vector<int> order(nprob);
+    for (int p = 0; p < nprob; p++) order[p] = p;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      auto cost = [&](int p) { return (double)probs[p].no * probs[p].no * std::pow((double)probs[p].nv, 4); };
+      return cost(a) > cost(b);
+    });
+    std::vector<int> rc(nprob, UGX_OK);
+    std::vector<double> t_ccsd(nprob, 0.0);
+    std::atomic<int> next(0);
+    auto worker = [&]() {
+      for (;;) {
+        const int q = next.fetch_add(1);
+        if (q >= nprob) break;
+        const int p = order[q];
+        const ugx_problem &pr = probs[p];
+        auto t0 = std::chrono::steady_clock::now();
+        int r = ugx_load_hamiltonian(sub[p].get(), pr.no, pr.nv, pr.fock, pr.oooo, pr.ooov, pr.oovv, pr.ovov,
+                                     pr.ovvv, pr.vvvv);
+        if (r == UGX_OK) r = ugx_ccsd(sub[p].get(), &opt, &results[p]);
+        t_ccsd[p] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        rc[p] = r;
+      }
+    };
+    std::vector<std::thread> pool;
+    const int nthr = std::min(nworkers, nprob);
+    try {
+      for (int w = 0; w < nthr; w++) pool.emplace_back(worker);
+    } catch (...) {
+      next.store(nprob);  // the started workers stop after their current problem
+      for (auto &th : pool) th.join();
+      throw;
+    }
+    for (auto &th : pool) th.join();
+    int status = UGX_OK;
+    for (int p = 0; p < nprob; p++) {
+      if (rc[p] == UGX_NOT_CONVERGED) status = UGX_NOT_CONVERGED;
+      else if (rc[p] != UGX_OK)
+        throw UgxError(rc[p], "ugx_batch_solve: problem " + std::to_string(p) + ": " + sub[p]->err);
+      results[p].seconds_ccsd = t_ccsd[p];
+      results[p].e_t = 0.0;
+      results[p].seconds_triples = 0.0;
+    }
+    // (T): one batched computation per (no, nv) class over every triplet of
+    // every converged problem of the class, on the class lead's stream
+    if (want_t) {
+      std::map<std::pair<int, int>, std::vector<int>> classes;
+      for (int p = 0; p < nprob; p++)
+        if (rc[p] == UGX_OK) classes[{probs[p].no, probs[p].nv}].push_back(p);
+      for (auto &kv : classes) {
+        const std::vector<int> &mem = kv.second;
+        auto t0 = std::chrono::steady_clock::now();
+        for (int p : mem) triples_operands(sub[p].get(), opt.precision);
+        for (int p : mem) check(hipStreamSynchronize(sub[p]->st), "stream sync");
+        ugx_ctx *lead = sub[mem[0]].get();
+        std::vector<TEnt> ents;
+        std::vector<int> prob;
+        std::vector<TripProb> tp;
+        for (size_t q = 0; q < mem.size(); q++) {
+          ugx_ctx *s = sub[mem[q]].get();
+          tp.push_back(TripProb{s->eps.d(), s->oovv.d(), s->t1.d()});
+          const std::vector<int> &tr = s->trip_host;
+          for (size_t t = 0; t + 2 < tr.size(); t += 3) {
+            ents.push_back(TEnt{s, tr[t], tr[t + 1], tr[t + 2]});
+            prob.push_back((int)q);
+          }
+        }
+        std::vector<double> e = triples_entries(lead, opt.precision, ents, prob, tp);
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (size_t t = 0; t < e.size(); t++) results[mem[prob[t]]].e_t += e[t];
+        for (int p : mem) results[p].seconds_triples = dt / (double)mem.size();
+      }
+    }
+    if (wall_seconds)
+      *wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    if (status == UGX_NOT_CONVERGED) c->err = "Computation has not converged.";
+    return status;
   });
 }
 

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 
 namespace ugx {
@@ -376,6 +377,15 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
 // ------------------------------------------------------------ autotuner
 namespace {
 
+// Tuned tiles, shared by every engine of the process (keyed by device and
+// shape; the best tile is a property of the device and the shape).
+struct Tuned {
+  int tile;
+  double ms;
+};
+std::mutex g_tiles_mu;
+std::map<std::string, Tuned> g_tiles;
+
 // Largest element offset of one batch entry's output, plus one.
 long out_extent(const GemmDesc &d, long table_extent) {
   if (d.offm) return table_extent;
@@ -401,8 +411,14 @@ int Engine::tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kpre
     key += " [K" + std::to_string(g.K) + " " + std::to_string(g.sam) + "," + std::to_string(g.sak) + "," +
            std::to_string(g.sbk) + "," + std::to_string(g.sbn) + "]";
   }
-  auto it = tiles_.find(key);
-  if (it != tiles_.end()) return it->second.tile;
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "get device");
+  key = "dev" + std::to_string(dev) + " " + key;
+  {
+    std::lock_guard<std::mutex> lk(g_tiles_mu);
+    auto it = g_tiles.find(key);
+    if (it != g_tiles.end()) return it->second.tile;
+  }
 
   const size_t es = d.c_f32 ? 4 : 8;
   const long ext = out_extent(d, out_extent_hint_);
@@ -450,13 +466,17 @@ int Engine::tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kpre
   }
   free_events_.push_back(e0);
   free_events_.push_back(e1);
-  tiles_[key] = Tuned{best, best_ms};
+  {
+    std::lock_guard<std::mutex> lk(g_tiles_mu);
+    g_tiles[key] = Tuned{best, best_ms};
+  }
   return best;
 }
 
 std::string Engine::tile_table() const {
   std::string s;
-  for (auto &kv : tiles_)
+  std::lock_guard<std::mutex> lk(g_tiles_mu);
+  for (auto &kv : g_tiles)
     s += kv.first + " -> " + gemm_tile_name(kv.second.tile) + " " + std::to_string(kv.second.ms) + " ms\n";
   return s;
 }

@@ -82,7 +82,8 @@ class Engine {
   }
   void set_autotune(bool on) { autotune_ = on; }
   bool autotune() const { return autotune_; }
-  // "key tile ms" lines of the tuned shapes
+  // "key tile ms" lines of the tuned shapes (process-wide: contexts on one
+  // device share the tuned tiles)
   std::string tile_table() const;
 
  private:
@@ -95,11 +96,6 @@ class Engine {
   bool autotune_ = false;
   long out_extent_hint_ = 0;
   int tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kprec, int fallback);
-  struct Tuned {
-    int tile;
-    double ms;
-  };
-  std::map<std::string, Tuned> tiles_;
   std::map<std::tuple<const void *, std::string, int>, std::pair<void *, size_t>> cache_;
   bool profiling_ = false;
   struct Pending {

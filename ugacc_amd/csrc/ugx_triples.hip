@@ -62,8 +62,8 @@ __global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, int ntrip
   constexpr int SB = E + 1, SA = E * (E + 1) + 1;
   __shared__ WT S[6][E * SA];
   // disconnected-term operands of the three blocks X, Y in {A, B, C}:
-  // O[s][X][Y][x][y] = <pq|xy> for the slices s = (ij), (ik), (jk) (FP32
-  // alongside FP32 W, FP64 otherwise);
+  // O[s][X][Y][x][y] = <pq|xy> for the slices s = (ij), (ik), (jk) (OT,
+  // FP64 for both W types);
   // T1[s][X][x] = t1 of k, j, i (the amplitude that multiplies slice s)
   __shared__ OT O[3][3][3][E2];
   __shared__ double T1[3][3][E];

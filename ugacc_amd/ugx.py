@@ -25,7 +25,7 @@ EXPORTS = [
     "ugx_default_options", "ugx_create", "ugx_destroy", "ugx_last_error", "ugx_load_hamiltonian",
     "ugx_set_amplitudes", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
     "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_set_profiling",
-    "ugx_kernel_stats", "ugx_gemm", "ugx_synchronize", "ugx_set_autotune", "ugx_tile_table", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
+    "ugx_kernel_stats", "ugx_gemm", "ugx_synchronize", "ugx_batch_solve", "ugx_set_autotune", "ugx_tile_table", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
     "ugx_mlp_last_error", "ugx_mlp_set_params", "ugx_mlp_get_params", "ugx_mlp_set_scaling", "ugx_mlp_forward",
     "ugx_mlp_loss_grad", "ugx_mlp_train", "ugx_features", "ugx_set_guess_network",
 ]
@@ -56,6 +56,11 @@ class Result(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("no", ctypes.c_int), ("nv", ctypes.c_int)] + [
+        (k, ctypes.POINTER(ctypes.c_double)) for k in ("fock", "oooo", "ooov", "oovv", "ovov", "ovvv", "vvvv")]
 
 
 _lib = None
@@ -94,6 +99,8 @@ def load_library(path=LIBPATH):
     L.ugx_gemm.argtypes = [vp, ctypes.c_char, ctypes.c_char, ip, ip, ip, cd, vp, ctypes.c_long, vp, ctypes.c_long,
                            cd, vp, ctypes.c_long, ip, ip]
     L.ugx_synchronize.argtypes = [vp]
+    L.ugx_batch_solve.argtypes = [vp, ip, ctypes.POINTER(Problem), ctypes.POINTER(Options), ip,
+                                  ctypes.POINTER(Result), dp]
     L.ugx_set_autotune.argtypes = [vp, ip]
     L.ugx_tile_table.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
     L.ugx_tile_table.restype = ctypes.c_size_t
@@ -273,6 +280,31 @@ class Context:
 
     def synchronize(self):
         self._check(self.L.ugx_synchronize(self.h))
+
+    def batch_solve(self, problems, options=None, nworkers=8, **kw):
+        """Solves [(no, nv, fock, blocks), ...] with ugx_batch_solve; returns
+        (list of result dicts, batch wall seconds)."""
+        o = options if options is not None else default_options()
+        for k, v in kw.items():
+            setattr(o, k, v)
+        keep = []
+        arr = (Problem * len(problems))()
+        for q, (no, nv, fock, blocks) in enumerate(problems):
+            bufs = [_c64(fock)] + [_c64(blocks[k]) for k in ("oooo", "ooov", "oovv", "ovov", "ovvv", "vvvv")]
+            keep.append(bufs)
+            arr[q].no, arr[q].nv = no, nv
+            for name, b in zip(("fock", "oooo", "ooov", "oovv", "ovov", "ovvv", "vvvv"), bufs):
+                setattr(arr[q], name, _p(b))
+        res = (Result * len(problems))()
+        wall = ctypes.c_double()
+        rc = self._check(self.L.ugx_batch_solve(self.h, len(problems), arr, ctypes.byref(o), nworkers, res,
+                                                ctypes.byref(wall)), allow=(OK, NOT_CONVERGED))
+        out = []
+        for r in res:
+            d = r.as_dict()
+            d["converged"] = rc == OK
+            out.append(d)
+        return out, wall.value
 
     def set_autotune(self, on=True):
         self._check(self.L.ugx_set_autotune(self.h, 1 if on else 0))
