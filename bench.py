@@ -50,8 +50,9 @@ FP64_CONV = 1e-10
 # HBM bytes per launch of a kernel label on a workload, from two rocprofv3
 # --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the counters' reading for a
 # plain copy of known size (fetch x2.0, write x1.0); the runs and their
-# commands are in profiles/triples_w_bf16x3_h2o8_hbm_traffic.md
-TRAFFIC = {("h2o8", "triples_w:bf16x3"): 12.46e9}
+# commands are in profiles/ (filled in when measured for the current
+# kernels)
+TRAFFIC = {}
 
 
 def log(msg):
@@ -161,16 +162,17 @@ def run_batch(args):
             single.append(r["e_ccsd_final"] + r["e_t"])
         t_seq = time.perf_counter() - t0
     log("sequential single FP64 solves: %.3f s" % t_seq)
+    graph = 0 if args.no_graph else 1
     forms = {"fp64": dict(precision=ugx.PREC_FP64, r_convergence=FP64_CONV),
              "mixed": dict(precision=ugx.PREC_NAMES[args.low], r_convergence=args.mixed_conv,
                            polish_threshold=args.polish)}
     out = {}
     for name, kw in forms.items():
         for w in range(args.warmup):
-            ctx.batch_solve(problems, nworkers=args.workers, wfn_ccsd_t=1, maxiter=100, graph=1, **kw)
+            ctx.batch_solve(problems, nworkers=args.workers, wfn_ccsd_t=1, maxiter=100, graph=graph, **kw)
         walls, per = [], []
         for st in range(args.steps):
-            res, wall = ctx.batch_solve(problems, nworkers=args.workers, wfn_ccsd_t=1, maxiter=100, graph=1, **kw)
+            res, wall = ctx.batch_solve(problems, nworkers=args.workers, wfn_ccsd_t=1, maxiter=100, graph=graph, **kw)
             if not all(r["converged"] for r in res):
                 raise RuntimeError("batch form %s did not converge" % name)
             walls.append(wall)
@@ -188,7 +190,7 @@ def run_batch(args):
             "data": "seeded distorted monomers (O-H U[0.90,1.10] A, angle U[95,115] deg, seeds 100-131) and dimers "
                     "(O-O U[2.7,3.2] A, random rotations, seeds 200-231), cc-pVDZ, host front end (untimed)",
             "config": {"workload": "64 uneven geometries: 32 x (o=5, v=19) + 32 x (o=10, v=38)",
-                       "workers": args.workers, "graph": True, "autotune": not args.no_autotune,
+                       "workers": args.workers, "graph": bool(graph), "autotune": not args.no_autotune,
                        "mixed_r_convergence": args.mixed_conv, "fp64_r_convergence": FP64_CONV},
             "forms": out, "sequential_single_fp64_s": t_seq, "roofline": None, "cpu_baseline": None}
     print(json.dumps(line), flush=True)
@@ -202,6 +204,7 @@ def main():
     ap.add_argument("--workload", default="h2o8", choices=sorted(WORKLOADS) + ["batch64"],
                     help="batch64: 32 distorted H2O monomers + 32 (H2O)2 dimers solved as one batch")
     ap.add_argument("--workers", type=int, default=8, help="batch64: host worker threads (streams in flight)")
+    ap.add_argument("--no-graph", action="store_true", help="batch64: eager iterations instead of hipGraph replays")
     ap.add_argument("--form", default="mixed", choices=["mixed", "fp64", "fp32", "bf16x3"])
     ap.add_argument("--low", default="bf16x3", choices=["bf16x3", "fp32"],
                     help="arithmetic of the low-precision phase of the mixed form")

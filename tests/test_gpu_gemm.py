@@ -43,7 +43,7 @@ def test_gemm_forms(prec, opa, opb, M, N, K):
 
 @pytest.mark.parametrize("prec", [ugx.PREC_FP64, ugx.PREC_FP32])
 @pytest.mark.parametrize("layout", ["NT", "TN"])
-@pytest.mark.parametrize("tile", range(11))
+@pytest.mark.parametrize("tile", range(12))
 def test_gemm_tiles(tile, layout, prec):
     """Every tile shape (including the 80- and 160-row ones) with both the
     k-contiguous (NT) and the row-contiguous (TN) operand loaders."""

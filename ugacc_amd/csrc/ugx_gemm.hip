@@ -42,7 +42,7 @@ namespace {
 
 constexpr int NT = 256;
 constexpr int BK = 16;
-constexpr int MAXSEG = 4;
+constexpr int MAXSEG = GEMM_MAXSEG;
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
@@ -560,7 +560,7 @@ struct TileShape {
 const TileShape kTiles[] = {
     {2, 2, 4, 4, "128x128"}, {2, 2, 2, 2, "64x64"},  {2, 2, 1, 1, "32x32"},  {4, 1, 2, 4, "128x64"},
     {1, 4, 4, 2, "64x128"},  {2, 2, 4, 5, "128x160"}, {4, 1, 1, 4, "64x64w"}, {2, 2, 2, 5, "64x160"},
-    {4, 1, 2, 5, "128x80"},  {2, 2, 5, 4, "160x128"}, {1, 4, 5, 2, "80x128"},
+    {4, 1, 2, 5, "128x80"},  {2, 2, 5, 4, "160x128"}, {1, 4, 5, 2, "80x128"},  {2, 2, 5, 5, "160x160"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
@@ -616,6 +616,7 @@ void launch_tile(int tile, const GemmDesc &d, const GemmPtrs *ptrs, int prec, hi
     case 8: launch_t<T, 4, 1, 2, 5>(d, ptrs, prec, st); break;
     case 9: launch_t<T, 2, 2, 5, 4>(d, ptrs, prec, st); break;
     case 10: launch_t<T, 1, 4, 5, 2>(d, ptrs, prec, st); break;
+    case 11: launch_t<T, 2, 2, 5, 5>(d, ptrs, prec, st); break;
     default: throw std::runtime_error("ugx: bad GEMM tile id");
   }
 }

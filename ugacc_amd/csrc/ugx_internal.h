@@ -21,6 +21,8 @@ enum Prec : int {
   PREC_BF16X3_PRESPLIT = 3,
 };
 
+constexpr int GEMM_MAXSEG = 4;
+
 // One K segment of a contraction C(m,n) += sign * sum_k A(m,k) B(k,n).
 //   A(m,k) = A[m*sam + k*sak]       B(k,n) = B[k*sbk + n*sbn]
 // Element type is double for PREC_FP64 and float otherwise.
@@ -46,7 +48,7 @@ struct GemmSeg {
 struct GemmDesc {
   int M, N;
   int nseg;
-  GemmSeg seg[4];
+  GemmSeg seg[GEMM_MAXSEG];
   double *C;
   long scm, scn, scm1, scn1, sCz;
   int cdiv, ndiv;
@@ -68,8 +70,8 @@ struct GemmDesc {
 // Per-batch pointers for grouped launches (used by the (T) build where each
 // batch entry is one triplet): A/B of every segment and C.
 struct GemmPtrs {
-  const void *A[4];
-  const void *B[4];
+  const void *A[GEMM_MAXSEG];
+  const void *B[GEMM_MAXSEG];
   double *C;
 };
 
