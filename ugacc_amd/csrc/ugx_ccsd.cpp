@@ -647,6 +647,7 @@ void triples_build_w(ugx_ctx *c, int prec, const TEnt *ents, int nb) {
     d.alpha = 1.0;
     d.beta = (g == 0) ? 0.0 : 1.0;
     d.c_f32 = c->tW_f32;
+    d.tag = 1;
     set_w_tables(c, d, g);
     tile = c->eng->tile_for(d, tab.data() + (size_t)g * nb, prec, tile, tw_stride(v, tw_blk(c->tW_f32)));
     c->eng->timed(prec == PREC_FP32 ? "triples_w:fp32" : "triples_w", fl, [&]() {
@@ -726,6 +727,7 @@ void triples_build_w_pair(ugx_ctx *c, const TEnt *ents, int nb) {
     d.alpha = 1.0;
     d.beta = (g == 0) ? 0.0 : 1.0;
     d.c_f32 = c->tW_f32;
+    d.tag = 1;
     set_w_tables(c, d, g);
     tile = c->eng->tile_for(d, tab.data() + (size_t)g * nb, PREC_BF16X3_PRESPLIT, tile, tw_stride(v, tw_blk(c->tW_f32)));
     c->eng->timed("triples_w:bf16x3", fl, [&]() {

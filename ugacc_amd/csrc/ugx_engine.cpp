@@ -404,7 +404,7 @@ int Engine::tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kpre
   if (hipStreamIsCapturing(st_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return fallback;
   std::string key = "M" + std::to_string(d.M) + " N" + std::to_string(d.N) + " p" + std::to_string(kprec) +
                     (d.c_f32 ? " f" : " d") + (d.beta != 0.0 ? " acc" : "") + (d.cdiv > 0 ? " cd" : "") +
-                    (d.ndiv > 0 ? " nd" : "") + (d.offm ? " tab" : "") + (ptrs_host ? " grouped" : " b" + std::to_string(d.batch)) +
+                    (d.ndiv > 0 ? " nd" : "") + (d.offm ? " tab" : "") + (d.tag ? " t" : "") + (ptrs_host ? " grouped" : " b" + std::to_string(d.batch)) +
                     (d.ktotal > 0 ? " kt" + std::to_string(d.ktotal) : "");
   for (int s = 0; s < d.nseg; s++) {
     const GemmSeg &g = d.seg[s];

@@ -195,7 +195,9 @@ __device__ inline long c_col_off(const GemmDesc &d, int n) {
 // written to the other stage and the global loads of slice t+2 are issued,
 // so each global load has a full slice of MFMA work to land and there is one
 // barrier per slice.
-template <typename T, int WGM, int WGN, int TM, int TN>
+// TAG only distinguishes instantiations (the (T) W3 launches use TAG = 1,
+// so profiles report them apart from the CCSD contractions).
+template <typename T, int WGM, int WGN, int TM, int TN, int TAG>
 __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restrict__ ptrs) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
@@ -402,7 +404,7 @@ struct PairLoader {
   }
 };
 
-template <int WGM, int WGN, int TM, int TN, bool VEC, int STAGES>
+template <int WGM, int WGN, int TM, int TN, bool VEC, int STAGES, int TAG>
 __global__ __launch_bounds__(NT) void k_gemm_b3p(GemmDesc d, const GemmPtrs *__restrict__ ptrs) {
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
   // STAGES = 2: ping-pong LDS, one barrier per slice.  STAGES = 1: half the
@@ -591,14 +593,20 @@ void launch_t(const GemmDesc &d, const GemmPtrs *ptrs, int prec, hipStream_t st)
   if (prec == PREC_BF16X3_PRESPLIT) {
     bool vec = b3_vectorizable(d) && (ptrs == nullptr || d.vec_ok);
     if (g_b3_stages == 1) {
-      if (vec) hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, true, 1>), grid, dim3(NT), 0, st, d, ptrs);
-      else hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, false, 1>), grid, dim3(NT), 0, st, d, ptrs);
+      if (d.tag) {
+        if (vec) hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, true, 1, 1>), grid, dim3(NT), 0, st, d, ptrs);
+        else hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, false, 1, 1>), grid, dim3(NT), 0, st, d, ptrs);
+      } else {
+        if (vec) hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, true, 1, 0>), grid, dim3(NT), 0, st, d, ptrs);
+        else hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, false, 1, 0>), grid, dim3(NT), 0, st, d, ptrs);
+      }
     } else {
-      if (vec) hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, true, 2>), grid, dim3(NT), 0, st, d, ptrs);
-      else hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, false, 2>), grid, dim3(NT), 0, st, d, ptrs);
+      if (vec) hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, true, 2, 0>), grid, dim3(NT), 0, st, d, ptrs);
+      else hipLaunchKernelGGL((k_gemm_b3p<WGM, WGN, TM, TN, false, 2, 0>), grid, dim3(NT), 0, st, d, ptrs);
     }
   } else {
-    hipLaunchKernelGGL((k_gemm<T, WGM, WGN, TM, TN>), grid, dim3(NT), 0, st, d, ptrs);
+    if (d.tag) hipLaunchKernelGGL((k_gemm<T, WGM, WGN, TM, TN, 1>), grid, dim3(NT), 0, st, d, ptrs);
+    else hipLaunchKernelGGL((k_gemm<T, WGM, WGN, TM, TN, 0>), grid, dim3(NT), 0, st, d, ptrs);
   }
 }
 

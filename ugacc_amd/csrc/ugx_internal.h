@@ -65,6 +65,9 @@ struct GemmDesc {
   // when offm is non-null, C(m,n) lives at C[offm[m] + offn[n]] (device
   // tables of M and N entries; any separable layout, e.g. the blocked (T) W)
   const long *offm, *offn;
+  // 1: launch the TAG = 1 instantiation (same code; the (T) W3 launches use
+  // it so that profiles attribute them apart)
+  int tag;
 };
 
 // Per-batch pointers for grouped launches (used by the (T) build where each
