@@ -84,3 +84,44 @@ def test_random_blocks_match_full_construction():
     np.testing.assert_allclose(g, g.transpose(1, 0, 3, 2), atol=1e-15)
     np.testing.assert_allclose(g, g.transpose(2, 3, 0, 1), atol=1e-15)
     np.testing.assert_allclose(g, g.transpose(2, 1, 0, 3), atol=1e-15)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_network_without_gpu_reports_no_device():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(ugx.UgxError) as e:
+        ugx.Network(0)
+    assert e.value.code == ugx.NO_DEVICE
+
+
+def test_trained_guess_weights_file():
+    """The committed network: safe to load (no pickles), the parameter count
+    of include/ugx.h, finite values, positive feature scales, and the
+    training log that produced it (256 geometries, held-out validation)."""
+    import json
+    from oracle import mlp
+    from ugacc_amd.learned import WEIGHTS
+    from ugacc_amd.learned.train_guess import heldout_geometries, training_geometries
+    with np.load(WEIGHTS, allow_pickle=False) as z:
+        p, shift, scale, ys = z["params"], z["shift"], z["scale"], float(z["y_scale"])
+    assert p.dtype == np.float32 and p.size == 34305 == sum(int(np.prod(s)) for _, s in mlp.layout())
+    assert np.all(np.isfinite(p)) and np.all(scale > 0) and shift.size == scale.size == 8 and ys > 0
+    log = json.load(open(os.path.splitext(WEIGHTS)[0] + ".json"))
+    assert len(log["geometries"]) == 256 and log["val_loss_final"] < 0.2 * log["val_loss_initial"]
+    assert [(g["n"], g["seed"]) for g in log["geometries"]] == training_geometries(256)
+    train_seeds = {s for _, s in training_geometries(256)}
+    assert not train_seeds & {s for _, s in heldout_geometries(16)}
+    np.testing.assert_allclose(log["shift"], shift, rtol=1e-6)
