@@ -48,6 +48,7 @@ struct ugx_ctx {
 
   // Hamiltonian (device, constant)
   DBuf fock, eps, fov, fvo, foo, fvv;
+  std::vector<double> h_fov;  // host copy of f_ov for the one-electron energy
   DBuf oooo, ooov, oovv, ovov, ovvv, vvvv;
   DBuf Loovv, Looov, Lovvv;
   // amplitudes and intermediates
@@ -119,15 +120,19 @@ int npart_max(ugx_ctx *c) {
 
 // -------------------------------------------------------------- energy
 // ccwfn.cc:260-282 for the amplitudes in t1 and tau.
-double energy(ugx_ctx *c, int form) {
+double energy_of(ugx_ctx *c, int form, const double *t1dev) {
   const int no = c->no, nv = c->nv;
   int np = energy_nparts(no, nv);
-  launch_energy(no, nv, c->fov.d(), c->t1.d(), c->tau.d(), c->Loovv.d(), c->part.d(), np, c->st);
-  double two = host_sum(c, np);
-  std::vector<double> t1(c->ov()), fov(c->ov());
-  check(hipMemcpyAsync(t1.data(), c->t1.p, c->ov() * 8, hipMemcpyDeviceToHost, c->st), "t1 copy");
-  check(hipMemcpyAsync(fov.data(), c->fov.p, c->ov() * 8, hipMemcpyDeviceToHost, c->st), "fov copy");
+  launch_energy(no, nv, c->fov.d(), t1dev, c->tau.d(), c->Loovv.d(), c->part.d(), np, c->st);
+  // the tau.L partials and t1 come back with one synchronisation; f_ov
+  // is kept on the host since load
+  std::vector<double> h(np), t1(c->ov());
+  check(hipMemcpyAsync(h.data(), c->part.p, np * 8, hipMemcpyDeviceToHost, c->st), "partials copy");
+  check(hipMemcpyAsync(t1.data(), t1dev, c->ov() * 8, hipMemcpyDeviceToHost, c->st), "t1 copy");
   check(hipStreamSynchronize(c->st), "stream sync");
+  double two = 0.0;
+  for (int i = 0; i < np; i++) two += h[i];
+  const std::vector<double> &fov = c->h_fov;
   double one = 0.0;
   if (form == UGX_ENERGY_REFERENCE) {
     // the '=' at ccwfn.cc:272 keeps only the last (i,a) term
@@ -137,6 +142,9 @@ double energy(ugx_ctx *c, int form) {
   }
   return one + two;
 }
+
+// Energy of the current amplitudes (t1, tau).
+double energy(ugx_ctx *c, int form) { return energy_of(c, form, c->t1.d()); }
 
 double t1norm(ugx_ctx *c) {
   std::vector<double> t1(c->ov());
@@ -941,6 +949,7 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
       for (long b = 0; b < v; b++) fvv[a * v + b] = fock[(o + a) * n + o + b];
     upload(c->eps, eps.data(), n, c->st);
     upload(c->fov, fov.data(), o * v, c->st);
+    c->h_fov = fov;
     upload(c->fvo, fvo.data(), o * v, c->st);
     upload(c->foo, foo.data(), o * o, c->st);
     upload(c->fvv, fvv.data(), v * v, c->st);
@@ -1040,7 +1049,13 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
       niter = iter;
       if (prec != PREC_FP64) nlow++;
       if (!std::isfinite(rms)) throw UgxError(UGX_NONFINITE, "non-finite residual norm");
-      if (rms < opt.r_convergence && prec == PREC_FP64) break;
+      if (rms < opt.r_convergence && prec == PREC_FP64) {
+        // the reported energy is that of the amplitudes this iteration
+        // started from (ccwfn.cc:216, 223-224): t1_old and the tau built
+        // from them, both still in place
+        if (!opt.print) eccsd = energy_of(c, opt.energy_form, c->t1o.d());
+        break;
+      }
       // low precision has a residual floor (BF16x3: ~1e-6): switch to the
       // FP64 polish when the threshold is reached or the residual stalls
       if (prec != PREC_FP64) {
@@ -1056,8 +1071,14 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
       }
       if (opt.diis) diis_step(c, opt.diis_nvec, opt.diis_policy, opt.diis_min_vecs);
       build_tau(c);
-      eccsd = energy(c, opt.energy_form);
-      if (opt.print) std::printf("\t  %3d  %20.15f  %8.6f  %8.6e\n", iter, eccsd, t1norm(c), rms);
+      // only printed per iteration; otherwise evaluated once, for the
+      // iteration that ends the loop
+      if (opt.print) {
+        eccsd = energy(c, opt.energy_form);
+        std::printf("\t  %3d  %20.15f  %8.6f  %8.6e\n", iter, eccsd, t1norm(c), rms);
+      } else if (iter == opt.maxiter) {
+        eccsd = energy(c, opt.energy_form);
+      }
     }
     build_tau(c);
     check(hipStreamSynchronize(c->st), "stream sync");

@@ -134,7 +134,8 @@ void launch_diis_dots(long len, const double *e, const double *const *vecs, int 
                       int npart, hipStream_t st);
 void launch_diis_combine(long n1, long n2, const double *const *amps, const double *coef, int nvec, double *t1,
                          double *t2, hipStream_t st);
-// E = 2 sum f_ia t1_ia + sum tau_ijab L_ijab into partials
+// partials of sum tau_ijab L_ijab (the one-electron term 2 f_ia t1_ia is
+// added on the host)
 void launch_energy(int no, int nv, const double *fov, const double *t1, const double *tau, const double *Loovv,
                    double *partials, int npart, hipStream_t st);
 int energy_nparts(int no, int nv);
