@@ -67,6 +67,9 @@ struct ugx_ctx {
   DBuf tOff;  // blocked-W offset tables of the three (T) launches
   DBuf tProb, tCounter;  // energy kernel: problem table (one entry), work-queue counter
   int tOff_blk = 0, tOff_v = 0;
+  // per launch g: the column table of the old orientation, used as the row
+  // table of the transposed BF16x3 launch, holds aligned runs of 4
+  bool tOff_row4[3] = {false, false, false};
   int tW_f32 = 0;  // the (T) W buffer holds FP32 values
   DBuf f_tP, f_tQ, f_t2, f_ooov;  // FP32 operand copies for the mixed forms
   // BF16x3 (T): BF16 hi/lo pair operands, all k contiguous
@@ -504,8 +507,13 @@ void triples_prepare(ugx_ctx *c, int prec) {
       E.permute(spec, 1.0, vref(src, dims), 0.0, out);
     };
     if (!c->b_P.p) {
-      pair_of(c->b_P, "pxye->pxye", c->tP.d(), {o, v, v, v}, {o, v, v, v});
-      pair_of(c->b_Q, "pyxe->pyxe", c->tQ.d(), {o, v, v, v}, {o, v, v, v});
+      // P and Q, the streamed operands of the particle terms, as MFMA panels
+      // (rows (x,y), k = e): each 16-row, 32-deep slice one contiguous block
+      const long v2 = (long)v * v, np = panel_size(o, v2, v);
+      c->b_P.alloc((size_t)np * 4);
+      launch_pack_panel(c->tP.d(), v3, v, 1, o, v2, v, (unsigned short *)c->b_P.p, c->st);
+      c->b_Q.alloc((size_t)np * 4);
+      launch_pack_panel(c->tQ.d(), v3, v, 1, o, v2, v, (unsigned short *)c->b_Q.p, c->st);
       pair_of(c->b_OVt, "qrmz->qrzm", c->ooov.d(), {o, o, o, v}, {o, o, v, o});  // OVt[q][r][z][m]
     }
     // amplitude-dependent pairs, rebuilt every call
@@ -554,6 +562,19 @@ void ensure_w_tables(ugx_ctx *c, int T) {
       g2n[x * v + y] = tw_off_b(x, nbk, T) + tw_off_c(y, T);
     }
   }
+  // rows r..r+3 (r % 4 == 0) of table rt contiguous and 4-aligned, and
+  // every entry of table ct a multiple of 4
+  auto runs4 = [](const long *rt, long nr, const long *ct, long nc) {
+    if (nr % 4) return false;
+    for (long r = 0; r < nr; r += 4)
+      if (rt[r] % 4 || rt[r + 1] != rt[r] + 1 || rt[r + 2] != rt[r] + 2 || rt[r + 3] != rt[r] + 3) return false;
+    for (long x = 0; x < nc; x++)
+      if (ct[x] % 4) return false;
+    return true;
+  };
+  c->tOff_row4[0] = runs4(g0n, v, g0m, vv);
+  c->tOff_row4[1] = runs4(g1n, vv, g1m, v);
+  c->tOff_row4[2] = runs4(g2n, vv, g2m, v);
   check(hipStreamSynchronize(c->st), "stream sync");  // the old tables may still be in use
   c->tOff.alloc(off.size() * 8);
   check(hipMemcpy(c->tOff.p, off.data(), off.size() * 8, hipMemcpyHostToDevice), "W offset tables");
@@ -561,14 +582,16 @@ void ensure_w_tables(ugx_ctx *c, int T) {
   c->tOff_v = v;
 }
 
-// Output tables of launch g (0: z = c, 1: z = b, 2: z = a) into the blocked W.
-void set_w_tables(ugx_ctx *c, GemmDesc &d, int g) {
+// Output tables of launch g (0: z = c, 1: z = b, 2: z = a) into the blocked
+// W; transposed launches swap the row and column tables.
+void set_w_tables(ugx_ctx *c, GemmDesc &d, int g, bool transposed = false) {
   ensure_w_tables(c, tw_blk(c->tW_f32));
   const long v = c->nv, vv = v * v;
   const long *t = (const long *)c->tOff.p;
   const long *m0 = t, *n0 = m0 + vv, *m1 = n0 + v, *n1 = m1 + v, *m2 = n1 + vv, *n2 = m2 + v;
   d.offm = g == 0 ? m0 : g == 1 ? m1 : m2;
   d.offn = g == 0 ? n0 : g == 1 ? n1 : n2;
+  if (transposed) std::swap(d.offm, d.offn);
 }
 
 // Builds W for triplets [t0, t0 + nb) of the table into c->tWbuf.
@@ -667,11 +690,19 @@ void triples_build_w(ugx_ctx *c, int prec, const TEnt *ents, int nb) {
 // BF16x3 form of triples_build_w: the same three grouped GEMMs on BF16
 // hi/lo pair operands that all have k contiguous -- the hole terms read
 // T2t[p][x][y][m] = t2_pm^xy, T2tq[p][y][x][m] = t2_pm^xy and
-// OVt[q][r][z][m] = <qr|mz> instead of the row-contiguous t2 and ooov.
+// OVt[q][r][z][m] = <qr|mz> instead of the row-contiguous t2 and ooov, and
+// P and Q are MFMA panels.  Each GEMM is the transpose of its
+// triples_build_w counterpart, so that its rows carry W's contiguous index c:
+//   z = c: rows c, columns (a,b);  z = b: rows (a,c), columns b;
+//   z = a: rows (b,c), columns a,
+// and the four rows an MFMA lane accumulates are four consecutive floats of
+// the blocked FP32 W (one 16-byte read-modify-write, GemmDesc::row4).
 void triples_build_w_pair(ugx_ctx *c, const TEnt *ents, int nb) {
   const int o = c->no, v = c->nv;
   const long v2 = (long)v * v, v3 = v2 * v, ov2o = (long)o * v2, vo = (long)v * o;
-  const long nP = (long)o * v3, nT = (long)o * o * v2, nOV = (long)o * o * o * v;
+  // P and Q are panels (launch_pack_panel): pP halves per p, nP per array
+  const long pP = panel_size(1, v2, v), nP = panel_size(o, v2, v), pblk = 16L * ((v + 31) / 32 * 32);
+  const long nT = (long)o * o * v2, nOV = (long)o * o * o * v;
   std::vector<GemmPtrs> tab((size_t)3 * nb);
   for (int b = 0; b < nb; b++) {
     const ugx_ctx *src = ents[b].src;  // the entry's own operands (same no, nv)
@@ -681,20 +712,20 @@ void triples_build_w_pair(ugx_ctx *c, const TEnt *ents, int nb) {
     const long i = ents[b].i, j = ents[b].j, k = ents[b].k;
     double *W = (double *)((char *)c->tWbuf.p + (size_t)b * tw_stride(v, tw_blk(c->tW_f32)) * (c->tW_f32 ? 4 : 8));
     GemmPtrs &A = tab[b], &B = tab[(size_t)nb + b], &C = tab[(size_t)2 * nb + b];
-    A.A[0] = P + i * v3;              A.B[0] = T2 + (k * o + j) * v2;
-    A.A[1] = T2t + i * ov2o;          A.B[1] = OVt + (j * o + k) * vo;
-    A.A[2] = Q + j * v3;              A.B[2] = T2 + (k * o + i) * v2;
-    A.A[3] = T2q + j * ov2o;          A.B[3] = OVt + (i * o + k) * vo;
+    A.A[0] = T2 + (k * o + j) * v2;   A.B[0] = P + i * pP;
+    A.A[1] = OVt + (j * o + k) * vo;  A.B[1] = T2t + i * ov2o;
+    A.A[2] = T2 + (k * o + i) * v2;   A.B[2] = Q + j * pP;
+    A.A[3] = OVt + (i * o + k) * vo;  A.B[3] = T2q + j * ov2o;
     A.C = W;
-    B.A[0] = T2 + (j * o + k) * v2;   B.B[0] = P + i * v3;
-    B.A[1] = OVt + (k * o + j) * vo;  B.B[1] = T2t + i * ov2o;
-    B.A[2] = T2 + (j * o + i) * v2;   B.B[2] = Q + k * v3;
-    B.A[3] = OVt + (i * o + j) * vo;  B.B[3] = T2q + k * ov2o;
+    B.A[0] = P + i * pP;              B.B[0] = T2 + (j * o + k) * v2;
+    B.A[1] = T2t + i * ov2o;          B.B[1] = OVt + (k * o + j) * vo;
+    B.A[2] = Q + k * pP;              B.B[2] = T2 + (j * o + i) * v2;
+    B.A[3] = T2q + k * ov2o;          B.B[3] = OVt + (i * o + j) * vo;
     B.C = W;
-    C.A[0] = T2 + (i * o + k) * v2;   C.B[0] = P + j * v3;
-    C.A[1] = OVt + (k * o + i) * vo;  C.B[1] = T2t + j * ov2o;
-    C.A[2] = T2 + (i * o + j) * v2;   C.B[2] = Q + k * v3;
-    C.A[3] = OVt + (j * o + i) * vo;  C.B[3] = T2q + k * ov2o;
+    C.A[0] = P + j * pP;              C.B[0] = T2 + (i * o + k) * v2;
+    C.A[1] = T2t + j * ov2o;          C.B[1] = OVt + (k * o + i) * vo;
+    C.A[2] = Q + k * pP;              C.B[2] = T2 + (i * o + j) * v2;
+    C.A[3] = T2q + k * ov2o;          C.B[3] = OVt + (j * o + i) * vo;
     C.C = W;
   }
   c->tptrs.alloc(tab.size() * sizeof(GemmPtrs));
@@ -708,35 +739,35 @@ void triples_build_w_pair(ugx_ctx *c, const TEnt *ents, int nb) {
     d.nseg = 4;
     d.batch = nb;
     d.vec_ok = (v % 8 == 0 && o % 8 == 0) ? 1 : 0;
+    // g = 0: A = the t2 / OVt rows of c, B = the panel / T2t columns (a,b);
+    // g > 0: A = the panel / T2t rows (x,c), B = the t2 / OVt columns
     for (int s = 0; s < 4; s++) {
       const bool part = (s % 2 == 0);
       GemmSeg &q = d.seg[s];
       q.K = part ? v : o;
       q.sign = part ? 1.0 : -1.0;
       q.sak = 1; q.sbk = 1;
-      q.sam = part ? v : o;
-      q.sbn = part ? v : o;
+      const long lo_small = part ? nT : nOV, lo_big = part ? nP : nT;
+      const long ld_small = part ? v : o;
       if (g == 0) {
-        q.loA = part ? nP : nT;
-        q.loB = part ? nT : nOV;
+        q.sam = ld_small; q.loA = lo_small;
+        q.loB = lo_big;
+        if (part) { q.pblkB = pblk; q.sbn = BK_PANEL; } else q.sbn = o;
       } else {
-        q.loA = part ? nT : nOV;
-        q.loB = part ? nP : nT;
+        q.sbn = ld_small; q.loB = lo_small;
+        q.loA = lo_big;
+        if (part) { q.pblkA = pblk; q.sam = BK_PANEL; } else q.sam = o;
       }
     }
     int tile;
-    if (g == 0) { d.M = (int)v2; d.N = v; d.scm = v; d.scn = 1; tile = tile_rows_v(v); }
-    else {
-      d.M = v; d.N = (int)v2;
-      if (g == 1) { d.scm = v; d.ndiv = v; d.scn1 = v2; d.scn = 1; }
-      else { d.scm = v2; d.scn = 1; }
-      tile = tile_cols_v(v);
-    }
+    if (g == 0) { d.M = v; d.N = (int)v2; tile = tile_cols_v(v); }
+    else { d.M = (int)v2; d.N = v; tile = tile_rows_v(v); }
     d.alpha = 1.0;
     d.beta = (g == 0) ? 0.0 : 1.0;
     d.c_f32 = c->tW_f32;
     d.tag = 1;
-    set_w_tables(c, d, g);
+    set_w_tables(c, d, g, true);
+    d.row4 = (c->tW_f32 && c->tOff_row4[g]) ? 1 : 0;
     tile = c->eng->tile_for(d, tab.data() + (size_t)g * nb, PREC_BF16X3_PRESPLIT, tile, tw_stride(v, tw_blk(c->tW_f32)));
     c->eng->timed("triples_w:bf16x3", fl, [&]() {
       launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)g * nb, PREC_BF16X3_PRESPLIT, tile, c->st);

@@ -118,41 +118,46 @@ struct OpLoader {
   }
 };
 
-// Segment bookkeeping shared by the kernels: operand bases of batch entry z,
-// K per segment (the split-K tail applied to segment 0) and slice counts.
+// Segment bookkeeping shared by the kernels: slice counts per segment (the
+// split-K tail applied to segment 0).  The operand bases, K and sign of a
+// segment are read from the kernel argument and the pointer table when the
+// segment starts: a runtime index into a local array would place it in
+// scratch memory, so only nt[] is kept, and only constant indices touch it.
 template <typename T, int KB>
 struct Segs {
-  const T *A[MAXSEG];
-  const T *B[MAXSEG];
-  int K[MAXSEG], nt[MAXSEG];
-  float sign[MAXSEG];  // +-1 (a float whatever T is: T may be an unsigned BF16 carrier)
+  int nt[MAXSEG];
   int n, ntiles;
-  __device__ inline void init(const GemmDesc &d, const GemmPtrs *ptrs, long z) {
+  __device__ inline void init(const GemmDesc &d, long z) {
     n = d.nseg;
     ntiles = 0;
 #pragma unroll
     for (int s = 0; s < MAXSEG; s++) {
-      A[s] = nullptr; B[s] = nullptr; K[s] = 0; nt[s] = 0; sign[s] = 1.0f;
-      if (s < n) {
-        const GemmSeg &g = d.seg[s];
-        if (ptrs) { A[s] = (const T *)ptrs[z].A[s]; B[s] = (const T *)ptrs[z].B[s]; }
-        else { A[s] = (const T *)g.A + z * g.sAz; B[s] = (const T *)g.B + z * g.sBz; }
-        int k = g.K;
-        if (s == 0 && d.ktotal > 0) {
-          long rem = d.ktotal - z * (long)g.K;
-          if (rem < k) k = rem > 0 ? (int)rem : 0;
-        }
-        K[s] = k;
-        nt[s] = (k + KB - 1) / KB;
-        sign[s] = g.sign < 0 ? -1.0f : 1.0f;
-        ntiles += nt[s];
-      }
+      nt[s] = s < n ? (kof(d, z, s) + KB - 1) / KB : 0;
+      ntiles += nt[s];
     }
   }
+  __device__ static inline int kof(const GemmDesc &d, long z, int s) {
+    int k = d.seg[s].K;
+    if (s == 0 && d.ktotal > 0) {
+      long rem = d.ktotal - z * (long)k;
+      if (rem < k) k = rem > 0 ? (int)rem : 0;
+    }
+    return k;
+  }
+  __device__ static inline const T *a(const GemmDesc &d, const GemmPtrs *ptrs, long z, int s) {
+    return ptrs ? (const T *)ptrs[z].A[s] : (const T *)d.seg[s].A + z * d.seg[s].sAz;
+  }
+  __device__ static inline const T *b(const GemmDesc &d, const GemmPtrs *ptrs, long z, int s) {
+    return ptrs ? (const T *)ptrs[z].B[s] : (const T *)d.seg[s].B + z * d.seg[s].sBz;
+  }
+  // +-1 as a float whatever T is (T may be an unsigned BF16 carrier)
+  __device__ static inline float sg(const GemmDesc &d, int s) { return d.seg[s].sign < 0 ? -1.0f : 1.0f; }
   // slice t -> segment and its k0
   __device__ inline int locate(int t, int &k0) const {
     int s = 0;
-    while (s < n - 1 && t >= nt[s]) { t -= nt[s]; s++; }
+#pragma unroll
+    for (int q = 0; q < MAXSEG - 1; q++)
+      if (q < n - 1 && s == q && t >= nt[q]) { t -= nt[q]; s = q + 1; }
     k0 = t * KB;
     return s;
   }
@@ -217,7 +222,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
   const int tid = threadIdx.x;
 
   Segs<T, BK> S;
-  S.init(d, ptrs, z);
+  S.init(d, z);
   const int ntiles = S.ntiles;
 
   OpLoader<T, BM, BK> la;
@@ -225,8 +230,8 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
   int cur_seg = -1;
   auto seg_setup = [&](int s) {
     const GemmSeg &g = d.seg[s];
-    la.setup(S.A[s], g.sam, g.sak, m0, M, S.K[s]);
-    lb.setup(S.B[s], g.sbn, g.sbk, n0, N, S.K[s]);
+    la.setup(S.a(d, ptrs, z, s), g.sam, g.sak, m0, M, S.kof(d, z, s));
+    lb.setup(S.b(d, ptrs, z, s), g.sbn, g.sbk, n0, N, S.kof(d, z, s));
     cur_seg = s;
   };
 
@@ -246,7 +251,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
     int k0;
     int s = S.locate(t, k0);
     if (s != cur_seg) seg_setup(s);
-    la.load(ra, k0, T(S.sign[s]), tid);
+    la.load(ra, k0, T(S.sg(d, s)), tid);
     lb.load(rb, k0, T(1), tid);
     la.advance();
     lb.advance();
@@ -330,6 +335,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
 // (round to nearest even); the product is hi*hi + hi*lo + lo*hi with FP32
 // accumulation (the dropped lo*lo term is ~2^-16 relative).
 constexpr int BK3 = 32;
+static_assert(BK3 == BK_PANEL, "MFMA panels are packed one BF16x3 slice deep");
 constexpr int LD3 = BK3 + 8;  // halves per LDS row (80 bytes: 16-byte aligned rows)
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -345,17 +351,23 @@ __device__ inline float bf16_to_float(unsigned short h) { return __uint_as_float
 // Operands arrive as BF16 hi/lo pairs with k contiguous (A[m][k], B[n][k]):
 // no conversion in the kernel, 16-byte global loads of eight k values when
 // every stride and K are multiples of 8 (VEC), element loads otherwise.
-// Two LDS stages, one barrier per 32-deep slice.
+// Element (r, k) of a slice lies at hi + (r/16)*sblk + (r%16)*srow + k%32:
+// row-major operands have sblk = 16*srow and step BK3 halves per slice;
+// MFMA panels (GemmSeg::pblkA/B) have srow = 32, sblk = pblk and step 512,
+// so the 16 rows x 32 k of a slice are one contiguous kilobyte.
 template <int R>
 struct PairLoader {
   static constexpr int CH = R * (BK3 / 8);             // 16-byte chunks per array per slice
   static constexpr int NQ = (CH + NT - 1) / NT;        // chunks per thread
   const unsigned short *hi;                            // row0, k0 of the current slice
-  long lo, srow;
+  long lo, srow, sblk, kstep;
   int rows_left, K;
-  __device__ inline void setup(const unsigned short *base, long lo_, long srow_, int row0, int rows, int K_) {
-    hi = base + (long)row0 * srow_;
-    lo = lo_; srow = srow_; rows_left = rows - row0; K = K_;
+  __device__ inline void setup(const unsigned short *base, long lo_, long srow_, long pblk, int row0, int rows,
+                               int K_) {
+    if (pblk > 0) { srow = BK3; sblk = pblk; kstep = 16 * BK3; }
+    else { srow = srow_; sblk = 16 * srow_; kstep = BK3; }
+    hi = base + (long)(row0 >> 4) * sblk;  // row0 is a multiple of 16
+    lo = lo_; rows_left = rows - row0; K = K_;
   }
   template <bool VEC>
   __device__ inline void load(uint4 (&h)[NQ], uint4 (&l)[NQ], int k0, int tid) const {
@@ -365,7 +377,7 @@ struct PairLoader {
       int r = c >> 2, kc = (c & 3) * 8;
       uint4 vh = {0u, 0u, 0u, 0u}, vl = {0u, 0u, 0u, 0u};
       if (c < CH && r < rows_left) {
-        const unsigned short *ph = hi + (long)r * srow + kc;
+        const unsigned short *ph = hi + (long)(r >> 4) * sblk + (long)(r & 15) * srow + kc;
         if (VEC) {
           if (k0 + kc < K) {
             vh = *reinterpret_cast<const uint4 *>(ph);
@@ -389,7 +401,7 @@ struct PairLoader {
       l[q] = vl;
     }
   }
-  __device__ inline void advance() { hi += BK3; }
+  __device__ inline void advance() { hi += kstep; }
   __device__ inline void store(const uint4 (&h)[NQ], const uint4 (&l)[NQ], unsigned short (*H)[LD3],
                                unsigned short (*L)[LD3], int tid) const {
 #pragma unroll
@@ -425,15 +437,15 @@ __global__ __launch_bounds__(NT) void k_gemm_b3p(GemmDesc d, const GemmPtrs *__r
   const int tid = threadIdx.x;
 
   Segs<unsigned short, BK3> S;
-  S.init(d, ptrs, z);
+  S.init(d, z);
   const int ntiles = S.ntiles;
   PairLoader<BM> la;
   PairLoader<BN> lb;
   int cur_seg = -1;
   auto seg_setup = [&](int s) {
     const GemmSeg &g = d.seg[s];
-    la.setup(S.A[s], g.loA, g.sam, m0, M, S.K[s]);
-    lb.setup(S.B[s], g.loB, g.sbn, n0, N, S.K[s]);
+    la.setup(S.a(d, ptrs, z, s), g.loA, g.sam, g.pblkA, m0, M, S.kof(d, z, s));
+    lb.setup(S.b(d, ptrs, z, s), g.loB, g.sbn, g.pblkB, n0, N, S.kof(d, z, s));
     cur_seg = s;
   };
 
@@ -455,7 +467,7 @@ __global__ __launch_bounds__(NT) void k_gemm_b3p(GemmDesc d, const GemmPtrs *__r
     if (s != cur_seg) seg_setup(s);
     la.template load<VEC>(rah, ral, k0, tid);
     lb.template load<VEC>(rbh, rbl, k0, tid);
-    if (S.sign[s] < 0) {
+    if (S.sg(d, s) < 0) {
 #pragma unroll
       for (int q = 0; q < PairLoader<BN>::NQ; q++) {
         rbh[q].x ^= 0x80008000u; rbh[q].y ^= 0x80008000u; rbh[q].z ^= 0x80008000u; rbh[q].w ^= 0x80008000u;
@@ -531,6 +543,29 @@ __global__ __launch_bounds__(NT) void k_gemm_b3p(GemmDesc d, const GemmPtrs *__r
     const int n = n0 + (wn * TN + j) * 16 + r16;
     co[j] = n < N ? c_col_off(d, n) : -1;
   }
+  if (d.row4 && d.c_f32) {
+    // the caller guarantees that a lane's four rows (4*kq + r, r = 0..3)
+    // are four consecutive, 16-byte aligned floats of C: one 16-byte
+    // read-modify-write per accumulator instead of four
+    float *C = (float *)Cv;
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+      for (int j = 0; j < TN; j++) {
+        if (ro[i][0] >= 0 && co[j] >= 0) {
+          float4 *p = reinterpret_cast<float4 *>(C + ro[i][0] + co[j]);
+          float4 o = {0.f, 0.f, 0.f, 0.f};
+          if (beta != 0.0) o = *p;
+          float4 w;
+          w.x = (float)(alpha * (double)acc[i][j][0] + beta * (double)o.x);
+          w.y = (float)(alpha * (double)acc[i][j][1] + beta * (double)o.y);
+          w.z = (float)(alpha * (double)acc[i][j][2] + beta * (double)o.z);
+          w.w = (float)(alpha * (double)acc[i][j][3] + beta * (double)o.w);
+          *p = w;
+        }
+      }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < TM; i++)
 #pragma unroll
@@ -578,6 +613,7 @@ bool b3_vectorizable(const GemmDesc &d) {
     const GemmSeg &g = d.seg[s];
     if (g.sak != 1 || g.sbk != 1) throw std::runtime_error("ugx: pre-split BF16x3 needs k-contiguous operands");
     if (g.sam % 8 || g.sbn % 8 || g.K % 8 || g.loA % 8 || g.loB % 8 || g.sAz % 8 || g.sBz % 8) return false;
+    if (g.pblkA % 512 || g.pblkB % 512) return false;
     if (g.A && (((uintptr_t)g.A) & 15)) return false;
     if (g.B && (((uintptr_t)g.B) & 15)) return false;
   }

@@ -22,6 +22,7 @@ enum Prec : int {
 };
 
 constexpr int GEMM_MAXSEG = 4;
+constexpr int BK_PANEL = 32;  // k depth of one MFMA panel slice (GemmSeg::pblkA/B)
 
 // One K segment of a contraction C(m,n) += sign * sum_k A(m,k) B(k,n).
 //   A(m,k) = A[m*sam + k*sak]       B(k,n) = B[k*sbk + n*sbn]
@@ -36,6 +37,11 @@ struct GemmSeg {
   // PREC_BF16X3_PRESPLIT: element offsets from the hi halves to the lo
   // halves of A and B (operands stored as two BF16 arrays, k contiguous)
   long loA, loB;
+  // PREC_BF16X3_PRESPLIT: > 0 marks an operand packed as MFMA panels (see
+  // launch_pack_panel): element (r, k) at (r/16)*pblk + (k/32)*512 +
+  // (r%16)*32 + k%32, pblk = 16 * K rounded up to 32; its sam / sbn are
+  // then ignored.  0 = row-major with stride sam / sbn.
+  long pblkA, pblkB;
 };
 
 // C(m,n) lives at C[(m / cdiv) * scm1 + (m % cdiv) * scm
@@ -68,6 +74,11 @@ struct GemmDesc {
   // 1: launch the TAG = 1 instantiation (same code; the (T) W3 launches use
   // it so that profiles attribute them apart)
   int tag;
+  // PREC_BF16X3_PRESPLIT with c_f32 and offm: the caller asserts M % 4 == 0
+  // and, for every m % 4 == 0, offm[m + r] = offm[m] + r (r < 4) with
+  // offm[m] + offn[n] a multiple of 4 and C 16-byte aligned, so the epilogue
+  // moves four rows of C as one 16-byte access
+  int row4;
 };
 
 // Per-batch pointers for grouped launches (used by the (T) build where each
@@ -107,6 +118,13 @@ struct PermDesc {
   double alpha, beta;
 };
 void launch_permute(const PermDesc &p, hipStream_t st);
+// Packs np FP64 matrices in[p*sp + r*sr + k*sk] (r < R, k < K) as split
+// BF16 MFMA panels: hi halves first, then lo halves (at offset
+// panel_size(np, R, K)), each [p][r/16][k/32][r%16][k%32] with R rounded up
+// to 16 and K to 32, zeros in the padding.
+long panel_size(int np, long R, int K);
+void launch_pack_panel(const double *in, long sp, long sr, long sk, int np, long R, int K, unsigned short *out,
+                       hipStream_t st);
 
 void launch_fill(long n, double v, double *y, hipStream_t st);
 void launch_axpby(long n, double a, const double *x, double b, double *y, hipStream_t st);  // y = a x + b y

@@ -341,7 +341,35 @@ __global__ void k_diis_combine(long n1, long n2, VecList vl, int nvec, double *t
   }
 }
 
+// One thread per panel element: i -> (p, row block, k slice, row, k).
+__global__ void k_pack_panel(const double *in, long sp, long sr, long sk, long R, int K, int nrb, int nks, long n,
+                             unsigned short *out) {
+  for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const int kk = (int)(i & 31), rr = (int)((i >> 5) & 15);
+    const long rest = i >> 9;
+    const int ks = (int)(rest % nks);
+    const long rb = (rest / nks) % nrb, p = rest / ((long)nks * nrb);
+    const long r = rb * 16 + rr;
+    const int k = ks * 32 + kk;
+    float x = 0.0f;
+    if (r < R && k < K) x = (float)in[p * sp + r * sr + k * sk];
+    const unsigned short h = bf16_rne(x);
+    out[i] = h;
+    out[n + i] = bf16_rne(x - __uint_as_float(((unsigned int)h) << 16));
+  }
+}
+
 }  // namespace
+
+long panel_size(int np, long R, int K) { return (long)np * ((R + 15) / 16 * 16) * ((K + 31) / 32 * 32); }
+
+void launch_pack_panel(const double *in, long sp, long sr, long sk, int np, long R, int K, unsigned short *out,
+                       hipStream_t st) {
+  const long n = panel_size(np, R, K);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pack_panel, dim3(nblocks(n)), dim3(NT), 0, st, in, sp, sr, sk, R, K, (int)((R + 15) / 16),
+                     (K + 31) / 32, n, out);
+}
 
 void launch_permute(const PermDesc &p, hipStream_t st) {
   if (p.rank < 1 || p.rank > 6) throw std::runtime_error("ugx: permute rank out of range");
