@@ -195,6 +195,89 @@ __device__ inline long c_col_off(const GemmDesc &d, int n) {
   return d.ndiv > 0 ? (long)(n / d.ndiv) * d.scn1 + (long)(n % d.ndiv) * d.scn : (long)n * d.scn;
 }
 
+// Epilogue: C = alpha * acc + beta * C over this thread's accumulators,
+// rows ro[i][r] (item r of block i) and columns co[j] (-1 = outside C).
+// With beta != 0 the old values are read first and combined afterwards,
+// each masked element reading C's first element instead (always in bounds,
+// the value is discarded): a load behind a per-element branch would wait for
+// its own round trip before the next one is issued.  One row block (TN*4
+// values) is read at a time, which bounds the registers the epilogue holds.
+// The arithmetic is the same on every path.
+template <int TM, int TN, typename ACC>
+__device__ inline void store_tile(const GemmDesc &d, void *Cv, const long (&ro)[TM][4], const long (&co)[TN],
+                                  const ACC (&acc)[TM][TN]) {
+  const double alpha = d.alpha, beta = d.beta;
+  if (d.c_f32 && d.row4) {
+    // a lane's four rows are four consecutive, 16-byte aligned floats
+    float *C = (float *)Cv;
+#pragma unroll
+    for (int i = 0; i < TM; i++) {
+      float4 old[TN];
+      if (beta != 0.0) {
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+          const bool ok = ro[i][0] >= 0 && co[j] >= 0;
+          old[j] = *reinterpret_cast<const float4 *>(C + (ok ? ro[i][0] + co[j] : 0));
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < TN; j++)
+        if (ro[i][0] >= 0 && co[j] >= 0) {
+          double v0 = alpha * (double)acc[i][j][0], v1 = alpha * (double)acc[i][j][1];
+          double v2 = alpha * (double)acc[i][j][2], v3 = alpha * (double)acc[i][j][3];
+          if (beta != 0.0) {
+            v0 += beta * (double)old[j].x; v1 += beta * (double)old[j].y;
+            v2 += beta * (double)old[j].z; v3 += beta * (double)old[j].w;
+          }
+          float4 w;
+          w.x = (float)v0; w.y = (float)v1; w.z = (float)v2; w.w = (float)v3;
+          *reinterpret_cast<float4 *>(C + ro[i][0] + co[j]) = w;
+        }
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < TM; i++) {
+    if (d.c_f32) {
+      float *C = (float *)Cv;
+      float old[TN][4];
+      if (beta != 0.0) {
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) old[j][r] = C[(ro[i][r] >= 0 && co[j] >= 0) ? ro[i][r] + co[j] : 0];
+      }
+#pragma unroll
+      for (int j = 0; j < TN; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (ro[i][r] >= 0 && co[j] >= 0) {
+            double v = alpha * (double)acc[i][j][r];
+            if (beta != 0.0) v += beta * (double)old[j][r];
+            C[ro[i][r] + co[j]] = (float)v;
+          }
+    } else {
+      double *C = (double *)Cv;
+      double old[TN][4];
+      if (beta != 0.0) {
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) old[j][r] = C[(ro[i][r] >= 0 && co[j] >= 0) ? ro[i][r] + co[j] : 0];
+      }
+#pragma unroll
+      for (int j = 0; j < TN; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (ro[i][r] >= 0 && co[j] >= 0) {
+            double v = alpha * (double)acc[i][j][r];
+            if (beta != 0.0) v += beta * old[j][r];
+            C[ro[i][r] + co[j]] = v;
+          }
+    }
+  }
+}
+
 // FP64 / FP32 main kernel.  Two LDS stages: while the MFMAs consume slice t
 // from one stage, slice t+1 (loaded into registers one iteration earlier) is
 // written to the other stage and the global loads of slice t+2 are issued,
@@ -293,7 +376,6 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
   // forms); per-batch offsets count elements of C's type
   void *Cv = ptrs ? (void *)ptrs[z].C
                   : (d.c_f32 ? (void *)((float *)d.C + z * d.sCz) : (void *)(d.C + z * d.sCz));
-  const double alpha = d.alpha, beta = d.beta;
   // this thread's TM*4 output rows and TN columns, addressed once
   long ro[TM][4], co[TN];
 #pragma unroll
@@ -308,26 +390,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restr
     const int n = n0 + (wn * TN + j) * 16 + r16;
     co[j] = n < N ? c_col_off(d, n) : -1;
   }
-#pragma unroll
-  for (int i = 0; i < TM; i++)
-#pragma unroll
-    for (int j = 0; j < TN; j++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        if (ro[i][r] >= 0 && co[j] >= 0) {
-          const long addr = ro[i][r] + co[j];
-          double v = alpha * (double)acc[i][j][r];
-          if (d.c_f32) {
-            float *C = (float *)Cv;
-            if (beta != 0.0) v += beta * (double)C[addr];
-            C[addr] = (float)v;
-          } else {
-            double *C = (double *)Cv;
-            if (beta != 0.0) v += beta * C[addr];
-            C[addr] = v;
-          }
-        }
-      }
+  store_tile<TM, TN>(d, Cv, ro, co, acc);
 }
 
 // ------------------------------------------------------------ BF16x3
@@ -357,6 +420,7 @@ __device__ inline float bf16_to_float(unsigned short h) { return __uint_as_float
 // so the 16 rows x 32 k of a slice are one contiguous kilobyte.
 template <int R>
 struct PairLoader {
+  static_assert(R % 16 == 0, "tile rows must be a multiple of 16");
   static constexpr int CH = R * (BK3 / 8);             // 16-byte chunks per array per slice
   static constexpr int NQ = (CH + NT - 1) / NT;        // chunks per thread
   const unsigned short *hi;                            // row0, k0 of the current slice
@@ -374,7 +438,8 @@ struct PairLoader {
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
       int c = tid + q * NT;
-      int r = c >> 2, kc = (c & 3) * 8;
+      int r, kc;
+      chunk(c, r, kc);
       uint4 vh = {0u, 0u, 0u, 0u}, vl = {0u, 0u, 0u, 0u};
       if (c < CH && r < rows_left) {
         const unsigned short *ph = hi + (long)(r >> 4) * sblk + (long)(r & 15) * srow + kc;
@@ -401,6 +466,16 @@ struct PairLoader {
       l[q] = vl;
     }
   }
+  // 16-byte chunk c of a slice -> (row, k offset): each run of 64 chunks
+  // (one wave) covers 16 rows x 4 chunks with the row fastest, so the 8
+  // lanes an LDS write serves per cycle hit 8 rows at one k (rows are 80
+  // bytes apart: 8 disjoint 4-bank groups) -- the same pattern as the MFMA
+  // fragment reads; with the chunk fastest, two rows share one cycle and
+  // collide on a bank group
+  __device__ static inline void chunk(int c, int &r, int &kc) {
+    r = ((c >> 6) << 4) + (c & 15);
+    kc = ((c >> 4) & 3) * 8;
+  }
   __device__ inline void advance() { hi += kstep; }
   __device__ inline void store(const uint4 (&h)[NQ], const uint4 (&l)[NQ], unsigned short (*H)[LD3],
                                unsigned short (*L)[LD3], int tid) const {
@@ -408,7 +483,8 @@ struct PairLoader {
     for (int q = 0; q < NQ; q++) {
       int c = tid + q * NT;
       if (c < CH) {
-        int r = c >> 2, kc = (c & 3) * 8;
+        int r, kc;
+        chunk(c, r, kc);
         *reinterpret_cast<uint4 *>(&H[r][kc]) = h[q];
         *reinterpret_cast<uint4 *>(&L[r][kc]) = l[q];
       }
@@ -417,7 +493,8 @@ struct PairLoader {
 };
 
 template <int WGM, int WGN, int TM, int TN, bool VEC, int STAGES, int TAG>
-__global__ __launch_bounds__(NT) void k_gemm_b3p(GemmDesc d, const GemmPtrs *__restrict__ ptrs) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_b3p(GemmDesc d,
+                                                                                const GemmPtrs *__restrict__ ptrs) {
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
   // STAGES = 2: ping-pong LDS, one barrier per slice.  STAGES = 1: half the
   // LDS (two workgroups per CU, so one's epilogue overlaps the other's
@@ -528,7 +605,6 @@ __global__ __launch_bounds__(NT) void k_gemm_b3p(GemmDesc d, const GemmPtrs *__r
   // forms); per-batch offsets count elements of C's type
   void *Cv = ptrs ? (void *)ptrs[z].C
                   : (d.c_f32 ? (void *)((float *)d.C + z * d.sCz) : (void *)(d.C + z * d.sCz));
-  const double alpha = d.alpha, beta = d.beta;
   // this thread's TM*4 output rows and TN columns, addressed once
   long ro[TM][4], co[TN];
 #pragma unroll
@@ -543,49 +619,7 @@ __global__ __launch_bounds__(NT) void k_gemm_b3p(GemmDesc d, const GemmPtrs *__r
     const int n = n0 + (wn * TN + j) * 16 + r16;
     co[j] = n < N ? c_col_off(d, n) : -1;
   }
-  if (d.row4 && d.c_f32) {
-    // the caller guarantees that a lane's four rows (4*kq + r, r = 0..3)
-    // are four consecutive, 16-byte aligned floats of C: one 16-byte
-    // read-modify-write per accumulator instead of four
-    float *C = (float *)Cv;
-#pragma unroll
-    for (int i = 0; i < TM; i++)
-#pragma unroll
-      for (int j = 0; j < TN; j++) {
-        if (ro[i][0] >= 0 && co[j] >= 0) {
-          float4 *p = reinterpret_cast<float4 *>(C + ro[i][0] + co[j]);
-          float4 o = {0.f, 0.f, 0.f, 0.f};
-          if (beta != 0.0) o = *p;
-          float4 w;
-          w.x = (float)(alpha * (double)acc[i][j][0] + beta * (double)o.x);
-          w.y = (float)(alpha * (double)acc[i][j][1] + beta * (double)o.y);
-          w.z = (float)(alpha * (double)acc[i][j][2] + beta * (double)o.z);
-          w.w = (float)(alpha * (double)acc[i][j][3] + beta * (double)o.w);
-          *p = w;
-        }
-      }
-    return;
-  }
-#pragma unroll
-  for (int i = 0; i < TM; i++)
-#pragma unroll
-    for (int j = 0; j < TN; j++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        if (ro[i][r] >= 0 && co[j] >= 0) {
-          const long addr = ro[i][r] + co[j];
-          double v = alpha * (double)acc[i][j][r];
-          if (d.c_f32) {
-            float *C = (float *)Cv;
-            if (beta != 0.0) v += beta * (double)C[addr];
-            C[addr] = (float)v;
-          } else {
-            double *C = (double *)Cv;
-            if (beta != 0.0) v += beta * C[addr];
-            C[addr] = v;
-          }
-        }
-      }
+  store_tile<TM, TN>(d, Cv, ro, co, acc);
 }
 
 // ------------------------------------------------------------ dispatch
