@@ -327,6 +327,9 @@ def main():
     }
     line.update(out)
     line.update({"kernels": stats, "roofline": roofline(stats, args.workload), "cpu_baseline": cpu})
+    for t in ctx.tile_table().splitlines():
+        if " t " in t:  # the (T) W3 launches' tuned tiles
+            log("tile " + t)
     print(json.dumps(line), flush=True)
 
 

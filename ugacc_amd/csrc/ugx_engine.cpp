@@ -427,8 +427,12 @@ int Engine::tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kpre
   std::vector<GemmPtrs> tab;
   size_t bytes;
   if (ptrs_host) {
-    // up to 8 entries of the table, outputs laid end to end in scratch
-    const int nb = std::min(d.batch, 8);
+    // up to 64 entries of the table (capped at 1 GiB of output), outputs
+    // laid end to end in scratch: enough workgroups in flight that a tile
+    // which streams an operand twice pays for it in HBM traffic, as it does
+    // on a full batch, rather than hitting cache
+    const long fit = std::max<long>(1, (1L << 30) / std::max<long>(1, ext * (long)es));
+    const int nb = (int)std::min<long>(std::min<long>(d.batch, 64), fit);
     t.batch = nb;
     bytes = (size_t)nb * ext * es;
     char *out = (char *)slot(4, bytes);
