@@ -16,13 +16,16 @@
 // W[A][B][C], W[A][C][B], W[B][A][C], W[B][C][A], W[C][A][B], W[C][B][A]
 // into LDS.  W is stored in the blocked layout the W3 GEMMs write (E^3
 // sub-cubes, each contiguous), so every staged sub-cube is one contiguous
-// read and every W element is read from memory about once.
+// read (16-byte loads) and every W element is read from memory about once.
+// The sub-cubes and <pq|xy> blocks of step C+1 are loaded into registers
+// while step C computes, and stored to LDS after the step's barrier.
 // W is FP64 (FP64 form) or FP32 (low-precision forms); all sums are FP64 and
 // the per-(triplet, block pair) partials are reduced per triplet in a fixed
 // order, so the result is bitwise repeatable.
 #include "ugx_internal.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace ugx {
 namespace {
@@ -51,7 +54,7 @@ __device__ inline void pair_of(int p, int &A, int &B) {
 }
 
 template <typename WT, int E, typename OT>
-__global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, int ntrip, const int *__restrict__ trip,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) == 4 ? 2 : 1))) void k_triples_energy(int no, int nv, int ntrip, const int *__restrict__ trip,
                                                        const TripProb *__restrict__ probs,
                                                        const WT *__restrict__ Wall, long wstride,
                                                        double *__restrict__ part, int npairs,
@@ -60,6 +63,17 @@ __global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, int ntrip
   // padded cube layout idx(a,b,c) = a*SA + b*SB + c: the strides are odd, so
   // the stores along a or b (and the reads along c) do not collide in banks
   constexpr int SB = E + 1, SA = E * (E + 1) + 1;
+  // per-thread shares of what one step stages: the six sub-cubes (PW
+  // elements each), the <pq|xy> blocks of the five block pairs that touch C
+  // (PC), and, once per item, the four pairs within {A, B} (PF)
+  // W moves in 16-byte chunks (EPC elements; sub-cubes are contiguous and
+  // 16-byte aligned)
+  typedef typename std::conditional<sizeof(WT) == 4, float4, double2>::type W16;
+  constexpr int EPC = 16 / (int)sizeof(WT), NCH = E3 / EPC;
+  static_assert(E3 % EPC == 0, "sub-cubes are whole 16-byte chunks");
+  constexpr int NPC = 3 * 5 * E2, NPF = 3 * 4 * E2;
+  constexpr int PW = (NCH + NT - 1) / NT, PC = (NPC + NT - 1) / NT, PF = (NPF + NT - 1) / NT;
+  static_assert(4 * E <= NT, "t1 / eps staging takes one element per thread");
   __shared__ WT S[6][E * SA];
   // disconnected-term operands of the three blocks X, Y in {A, B, C}:
   // O[s][X][Y][x][y] = <pq|xy> for the slices s = (ij), (ik), (jk) (OT,
@@ -69,12 +83,13 @@ __global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, int ntrip
   __shared__ double T1[3][3][E];
   __shared__ double EPS[3][E];
   __shared__ int item_sh;
+  const int tid = threadIdx.x;
   const long total = (long)ntrip * npairs;
   // persistent: each workgroup takes (entry, block pair) items from the
   // queue until it is empty; every workgroup reaches the exit test
   for (;;) {
   __syncthreads();  // the previous item is done with LDS and item_sh
-  if (threadIdx.x == 0) item_sh = atomicAdd(counter, 1);
+  if (tid == 0) item_sh = atomicAdd(counter, 1);
   __syncthreads();
   const long item = item_sh;
   if (item >= total) break;
@@ -87,12 +102,16 @@ __global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, int ntrip
   const double *__restrict__ eps = pr.eps, *__restrict__ oovv = pr.oovv, *__restrict__ t1 = pr.t1;
   const WT *W = Wall + (long)t * wstride;
   const long v = nv, v2 = (long)nv * nv;
-  const double *Osl[3] = {oovv + ((long)i * no + j) * v2, oovv + ((long)i * no + k) * v2,
-                          oovv + ((long)j * no + k) * v2};
-  const double *T1sl[3] = {t1 + (long)k * nv, t1 + (long)j * nv, t1 + (long)i * nv};
+  // slices selected by value (a runtime index into a pointer array would
+  // place the array in scratch memory)
+  const double *O0 = oovv + ((long)i * no + j) * v2, *O1 = oovv + ((long)i * no + k) * v2,
+               *O2 = oovv + ((long)j * no + k) * v2;
+  const double *T10 = t1 + (long)k * nv, *T11 = t1 + (long)j * nv, *T12 = t1 + (long)i * nv;
+  auto osl = [&](int sl) { return sl == 0 ? O0 : (sl == 1 ? O1 : O2); };
+  auto t1sl = [&](int sl) { return sl == 0 ? T10 : (sl == 1 ? T11 : T12); };
   const double eo = eps[i] + eps[j] + eps[k];
   const double fac = 2.0 - ((i == j) + (i == k) + (j == k));
-  int X0[3] = {Ab * E, Bb * E, 0};  // block origins of A, B, C (C set per step)
+  const int A0 = Ab * E, B0 = Bb * E;
 
   // V at a permutation (x, y, z) of the point, given as block slots (bx, by,
   // bz) in {0: A, 1: B, 2: C} and local offsets (lx, ly, lz)
@@ -100,61 +119,163 @@ __global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, int ntrip
     return (w + (double)O[0][bx][by][lx * E + ly] * T1[0][bz][lz] + (double)O[1][bx][bz][lx * E + lz] * T1[1][by][ly] +
             (double)O[2][by][bz][ly * E + lz] * T1[2][bx][lx]) * inv;
   };
-  auto stage_pairs = [&](bool all) {
-    // O blocks for pairs touching C (or all pairs on the first step) and the
-    // t1 / eps entries of the blocks
-    for (int e = threadIdx.x; e < 3 * 9 * E2; e += NT) {
-      int s = e / (9 * E2), r = e % (9 * E2), pr = r / E2, xy = r % E2;
-      int bx = pr / 3, by = pr % 3;
-      if (!all && bx != 2 && by != 2) continue;
-      int gx = X0[bx] + xy / E, gy = X0[by] + xy % E;
-      O[s][bx][by][xy] = (OT)((gx < nv && gy < nv) ? Osl[s][gx * v + gy] : 0.0);
-    }
-    for (int e = threadIdx.x; e < 3 * 3 * E; e += NT) {
-      int s = e / (3 * E), r = e % (3 * E), bx = r / E, lx = r % E;
-      if (!all && bx != 2) continue;
-      int gx = X0[bx] + lx;
-      T1[s][bx][lx] = gx < nv ? T1sl[s][gx] : 0.0;
-      if (s == 0) EPS[bx][lx] = gx < nv ? eps[no + gx] : 0.0;
-    }
+  // <pq|xy> of slice s at block-local (lx, ly) of blocks with origins gx0, gy0
+  auto oget = [&](int s, int gx0, int lx, int gy0, int ly) {
+    const int gx = gx0 + lx, gy = gy0 + ly;
+    return (OT)((gx < nv && gy < nv) ? osl(s)[gx * v + gy] : 0.0);
   };
 
-  double s = 0.0;
-  const int A0 = X0[0], B0 = X0[1];
+  // Once per item: the pairs within {A, B} and the t1 / eps entries of A and
+  // B.  Every load is issued before any LDS store, so the loads overlap.
+  {
+    OT rf[PF];
+#pragma unroll
+    for (int q = 0; q < PF; q++) {
+      const int e = tid + q * NT;
+      rf[q] = 0;
+      if (e < NPF) {
+        const int sl = e / (4 * E2), r = e % (4 * E2), pp = r / E2, xy = r % E2;
+        const int bx = pp >> 1, by = pp & 1;
+        rf[q] = oget(sl, bx ? B0 : A0, xy / E, by ? B0 : A0, xy % E);
+      }
+    }
+    double rt = 0.0;
+    if (tid < 4 * E) {  // T1 of slices 0 and 1 for blocks A, B
+      const int sl = tid / (2 * E), bx = (tid / E) & 1, lx = tid % E, gx = (bx ? B0 : A0) + lx;
+      if (gx < nv) rt = t1sl(sl)[gx];
+    }
+    double rt2 = 0.0, re = 0.0;
+    if (tid < 2 * E) {  // T1 of slice 2 and eps for blocks A, B
+      const int bx = tid / E, lx = tid % E, gx = (bx ? B0 : A0) + lx;
+      if (gx < nv) { rt2 = T12[gx]; re = eps[no + gx]; }
+    }
+#pragma unroll
+    for (int q = 0; q < PF; q++) {
+      const int e = tid + q * NT;
+      if (e < NPF) {
+        const int sl = e / (4 * E2), r = e % (4 * E2), pp = r / E2, xy = r % E2;
+        O[sl][pp >> 1][pp & 1][xy] = rf[q];
+      }
+    }
+    if (tid < 4 * E) {
+      const int sl = tid / (2 * E), bx = (tid / E) & 1, lx = tid % E;
+      T1[sl][bx][lx] = rt;
+    }
+    if (tid < 2 * E) {
+      const int bx = tid / E, lx = tid % E;
+      T1[2][bx][lx] = rt2;
+      EPS[bx][lx] = re;
+    }
+  }
+
   // W is blocked (E^3 sub-cubes, each contiguous in its own [x][y][z]
   // order): sub-cube (X, Y, Z) starts at ((X nb + Y) nb + Z) E^3
   const long nbk = (nv + E - 1) / E;
   auto cube = [&](int X, int Y, int Z) { return W + ((X * nbk + Y) * nbk + Z) * (long)E3; };
-  for (int Cb = 0; Cb <= Bb; Cb++) {
+  auto at = [](int a, int b, int c) { return a * SA + b * SB + c; };
+  // the five block pairs that touch C: (A,C), (B,C), (C,A), (C,B), (C,C)
+  auto cpair = [](int p, int &bx, int &by) {
+    bx = p < 2 ? p : 2;
+    by = p < 2 ? 2 : (p == 2 ? 0 : (p == 3 ? 1 : 2));
+  };
+  // Step data in registers: loaded for step C + 1 while step C computes,
+  // committed to LDS after the step's closing barrier.
+  W16 rw[6][PW];
+  OT rc[PC];
+  double rtc = 0.0;
+  auto fetch = [&](int Cb) {
     const int C0 = Cb * E;
-    X0[2] = C0;
-    auto at = [](int a, int b, int c) { return a * SA + b * SB + c; };
-    // Stage the six sub-cubes, S[q][at(a,b,c)] = W at permutation q of
-    // (A0+a, B0+b, C0+c): each is one contiguous E^3 chunk of W, read in
-    // order (entries past nv are never used: the points that would need
-    // them are skipped below).
     const WT *c0 = cube(Ab, Bb, Cb), *c1 = cube(Ab, Cb, Bb), *c2 = cube(Bb, Ab, Cb);
     const WT *c3 = cube(Bb, Cb, Ab), *c4 = cube(Cb, Ab, Bb), *c5 = cube(Cb, Bb, Ab);
-    for (int e = threadIdx.x; e < E3; e += NT) {
-      int f = e % E, m = (e / E) % E, sl = e / (E * E);  // fastest, middle, slowest
-      S[0][at(sl, m, f)] = c0[e];   // W[a][b][c]
-      S[1][at(sl, f, m)] = c1[e];   // W[a][c][b]
-      S[2][at(m, sl, f)] = c2[e];   // W[b][a][c]
-      S[3][at(f, sl, m)] = c3[e];   // W[b][c][a]
-      S[4][at(m, f, sl)] = c4[e];   // W[c][a][b]
-      S[5][at(f, m, sl)] = c5[e];   // W[c][b][a]
+#pragma unroll
+    for (int q = 0; q < PW; q++) {
+      const int ch = tid + q * NT;
+      if (ch < NCH) {
+        rw[0][q] = reinterpret_cast<const W16 *>(c0)[ch]; rw[1][q] = reinterpret_cast<const W16 *>(c1)[ch];
+        rw[2][q] = reinterpret_cast<const W16 *>(c2)[ch]; rw[3][q] = reinterpret_cast<const W16 *>(c3)[ch];
+        rw[4][q] = reinterpret_cast<const W16 *>(c4)[ch]; rw[5][q] = reinterpret_cast<const W16 *>(c5)[ch];
+      }
     }
-    stage_pairs(Cb == 0);
+#pragma unroll
+    for (int q = 0; q < PC; q++) {
+      const int e = tid + q * NT;
+      rc[q] = 0;
+      if (e < NPC) {
+        const int sl = e / (5 * E2), r = e % (5 * E2), p = r / E2, xy = r % E2;
+        int bx, by;
+        cpair(p, bx, by);
+        const int gx0 = bx == 0 ? A0 : (bx == 1 ? B0 : C0), gy0 = by == 0 ? A0 : (by == 1 ? B0 : C0);
+        rc[q] = oget(sl, gx0, xy / E, gy0, xy % E);
+      }
+    }
+    rtc = 0.0;
+    if (tid < 4 * E) {  // T1 of slices 0..2 at block C, then eps
+      const int sl = tid / E, lx = tid % E, gx = C0 + lx;
+      if (gx < nv) rtc = sl < 3 ? t1sl(sl)[gx] : eps[no + gx];
+    }
+  };
+  auto commit = [&]() {
+    // S[q] = sub-cube q in its own index order (one contiguous E^3 chunk of
+    // W, read in order), all six at the same LDS offsets: sub-cube 0 is
+    // W[A][B][C], 1 W[A][C][B], 2 W[B][A][C], 3 W[B][C][A], 4 W[C][A][B],
+    // 5 W[C][B][A], and the compute reads each at its permutation of the
+    // point.  (Permuting on the store side would give every thread 6x the
+    // LDS addresses to hold.)  Entries past nv are never used: the points
+    // that would need them are skipped.
+#pragma unroll
+    for (int q = 0; q < PW; q++) {
+      const int ch = tid + q * NT;
+      if (ch < NCH) {
+        const WT *w0 = reinterpret_cast<const WT *>(&rw[0][q]), *w1 = reinterpret_cast<const WT *>(&rw[1][q]);
+        const WT *w2 = reinterpret_cast<const WT *>(&rw[2][q]), *w3 = reinterpret_cast<const WT *>(&rw[3][q]);
+        const WT *w4 = reinterpret_cast<const WT *>(&rw[4][q]), *w5 = reinterpret_cast<const WT *>(&rw[5][q]);
+#pragma unroll
+        for (int u = 0; u < EPC; u++) {
+          const int e = ch * EPC + u;
+          const int x = at(e / (E * E), (e / E) % E, e % E);  // slowest, middle, fastest
+          S[0][x] = w0[u];
+          S[1][x] = w1[u];
+          S[2][x] = w2[u];
+          S[3][x] = w3[u];
+          S[4][x] = w4[u];
+          S[5][x] = w5[u];
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < PC; q++) {
+      const int e = tid + q * NT;
+      if (e < NPC) {
+        const int sl = e / (5 * E2), r = e % (5 * E2), p = r / E2, xy = r % E2;
+        int bx, by;
+        cpair(p, bx, by);
+        O[sl][bx][by][xy] = rc[q];
+      }
+    }
+    if (tid < 4 * E) {
+      const int sl = tid / E, lx = tid % E;
+      if (sl < 3) T1[sl][2][lx] = rtc;
+      else EPS[2][lx] = rtc;
+    }
+  };
+
+  double s = 0.0;
+  fetch(0);
+  for (int Cb = 0; Cb <= Bb; Cb++) {
+    const int C0 = Cb * E;
+    commit();
     __syncthreads();
-    for (int e = threadIdx.x; e < E3; e += NT) {
+    if (Cb < Bb) fetch(Cb + 1);  // in flight while this step computes
+#pragma unroll 1
+    for (int e = tid; e < E3; e += NT) {
       int cc = e % E, bb = (e / E) % E, aa = e / (E * E);
       int a = A0 + aa, b = B0 + bb, c = C0 + cc;
       if (a >= nv || b > a || c > b) continue;
-      const int x = at(aa, bb, cc);
       // 1 / (1 + d_ab + d_ac + d_bc): 1, 1/2 or 1/4, exact
       const double inv = (a == b && b == c) ? 0.25 : ((a == b || b == c || a == c) ? 0.5 : 1.0);
-      double wabc = (double)S[0][x], wacb = (double)S[1][x], wbac = (double)S[2][x];
-      double wbca = (double)S[3][x], wcab = (double)S[4][x], wcba = (double)S[5][x];
+      double wabc = (double)S[0][at(aa, bb, cc)], wacb = (double)S[1][at(aa, cc, bb)];
+      double wbac = (double)S[2][at(bb, aa, cc)], wbca = (double)S[3][at(bb, cc, aa)];
+      double wcab = (double)S[4][at(cc, aa, bb)], wcba = (double)S[5][at(cc, bb, aa)];
       double vabc = Vx(0, aa, 1, bb, 2, cc, wabc, inv), vacb = Vx(0, aa, 2, cc, 1, bb, wacb, inv);
       double vbac = Vx(1, bb, 0, aa, 2, cc, wbac, inv), vbca = Vx(1, bb, 2, cc, 0, aa, wbca, inv);
       double vcab = Vx(2, cc, 0, aa, 1, bb, wcab, inv), vcba = Vx(2, cc, 1, bb, 0, aa, wcba, inv);
@@ -167,7 +288,7 @@ __global__ __launch_bounds__(NT) void k_triples_energy(int no, int nv, int ntrip
     __syncthreads();
   }
   double tot = block_sum(s);
-  if (threadIdx.x == 0) part[(long)t * npairs + pidx] = tot;
+  if (tid == 0) part[(long)t * npairs + pidx] = tot;
   }
 }
 
