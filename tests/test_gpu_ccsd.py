@@ -129,6 +129,37 @@ def test_triples_low_precision_forms(no, nv):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("no,nv,stages", [(8, 40, "3"), (5, 19, "0"), (3, 70, "0")])
+def test_triples_bf16x3_lds_copy_matches_register_path(no, nv, stages, monkeypatch):
+    """The BF16x3 W3 build with its panel operands copied straight into LDS
+    (the default: two stages) and the other W3 kernels (UGX_T_DMA=0:
+    register-staged operands; 3: three LDS stages; read when a context is
+    created) run the same MFMA products in the same order: per-triplet
+    energies and W3 agree bitwise.  v = 19 and 70 pad the panels' rows and
+    k; o = 3 and 5 leave the hole terms' k mostly padding."""
+    announce("triples bf16x3 lds copy vs UGX_T_DMA=%s o=%d v=%d" % (stages, no, nv))
+    c, fock, ints = loaded(no, nv, seed=no * 31 + nv)
+    rng = np.random.default_rng(no * nv)
+    t1 = 0.01 * rng.standard_normal((no, nv))
+    t2 = 0.02 * rng.standard_normal((no, no, nv, nv))
+    t2 = 0.5 * (t2 + t2.transpose(1, 0, 3, 2))
+    c.set_amplitudes(t1, t2)
+    e_copy = c.triples_per_triplet(ugx.PREC_BF16X3)
+    w_copy = c.w3(no - 1, 1, 0, ugx.PREC_BF16X3)
+    monkeypatch.setenv("UGX_T_DMA", stages)
+    other = ugx.Context(0)
+    other.load_hamiltonian(no, nv, fock, ugx.split_blocks(ints, no))
+    other.set_amplitudes(t1, t2)
+    e_other = other.triples_per_triplet(ugx.PREC_BF16X3)
+    w_other = other.w3(no - 1, 1, 0, ugx.PREC_BF16X3)
+    other.close()
+    np.testing.assert_array_equal(e_copy, e_other)
+    np.testing.assert_array_equal(w_copy, w_other)
+    ref = c.triples_per_triplet(ugx.PREC_FP64)
+    assert np.max(np.abs(e_copy - ref)) < 2e-6 * np.max(np.abs(ref))
+
+
+@pytest.mark.gpu
 def test_autotuned_tiles_give_the_same_results():
     # the autotuner only changes which tile computes each contraction
     announce("autotune on vs off, o=6 v=40, CCSD(T) FP64 and BF16x3")

@@ -74,12 +74,16 @@ struct ugx_ctx {
   DBuf f_tP, f_tQ, f_t2, f_ooov;  // FP32 operand copies for the mixed forms
   // BF16x3 (T): BF16 hi/lo pair operands, all k contiguous
   DBuf b_P, b_Q, b_OVt, b_t2, b_T2t, b_T2tq;
+  // the same operands as MFMA panels, for the LDS-copy W3 kernel
+  DBuf p_OV, p_t2, p_T2t, p_T2tq;
   bool tables_ready = false;
   std::vector<int> trip_host;
   // learned guess: network (owned by the caller) and its output f(x_ijab)
   const ugx_mlp *mlp = nullptr;
   DBuf G;
   int t_batch = 0;  // cap on triplets per (T) batch (0: by the 2 GB W budget)
+  int t_zgroup = 4;  // consecutive triplets per output tile in the BF16x3 W3 launches
+  int t_dma = 2;     // > 1: BF16x3 W3 launches copy panels into LDS, with that many stages (0: registers)
   // hipGraphs of the residual, keyed by (amplitude buffer parity, precision)
   struct ResidualGraph {
     const void *t1o = nullptr;
@@ -516,6 +520,23 @@ void triples_prepare(ugx_ctx *c, int prec) {
       launch_pack_panel(c->tQ.d(), v3, v, 1, o, v2, v, (unsigned short *)c->b_Q.p, c->st);
       pair_of(c->b_OVt, "qrmz->qrzm", c->ooov.d(), {o, o, o, v}, {o, o, v, o});  // OVt[q][r][z][m]
     }
+    if (c->t_dma > 1) {
+      // every operand as a panel: OV[q][r] (rows z, k = m) from ooov once,
+      // the t2 slices (rows x, k = y) and T2t / T2tq (rows (x,y) / (y,x),
+      // k = m) every call
+      const long v2 = (long)v * v;
+      if (!c->p_OV.p) {
+        c->p_OV.alloc((size_t)panel_size(o * o, v, o) * 4);
+        launch_pack_panel(c->ooov.d(), (long)o * v, 1, v, o * o, v, o, (unsigned short *)c->p_OV.p, c->st);
+      }
+      c->p_t2.alloc((size_t)panel_size(o * o, v, v) * 4);
+      launch_pack_panel(c->t2.d(), v2, v, 1, o * o, v, v, (unsigned short *)c->p_t2.p, c->st);
+      c->p_T2t.alloc((size_t)panel_size(o, v2, o) * 4);
+      launch_pack_panel(c->t2.d(), (long)o * v2, 1, v2, o, v2, o, (unsigned short *)c->p_T2t.p, c->st);
+      c->p_T2tq.alloc((size_t)panel_size(o, v2, o) * 4);
+      launch_pack_panel2(c->t2.d(), (long)o * v2, 1, v, v, v2, o, v2, o, (unsigned short *)c->p_T2tq.p, c->st);
+      return;
+    }
     // amplitude-dependent pairs, rebuilt every call
     pair_of(c->b_t2, "ijab->ijab", c->t2.d(), {o, o, v, v}, {o, o, v, v});
     pair_of(c->b_T2t, "pmxy->pxym", c->t2.d(), {o, o, v, v}, {o, v, v, o});   // T2t[p][x][y][m] = t2[p][m][x][y]
@@ -768,6 +789,86 @@ void triples_build_w_pair(ugx_ctx *c, const TEnt *ents, int nb) {
     d.tag = 1;
     set_w_tables(c, d, g, true);
     d.row4 = (c->tW_f32 && c->tOff_row4[g]) ? 1 : 0;
+    // consecutive triplets share i (and often j): run them tile by tile so
+    // their common <pe|xy> panel tiles come from L2
+    d.zgroup = c->t_zgroup;
+    tile = c->eng->tile_for(d, tab.data() + (size_t)g * nb, PREC_BF16X3_PRESPLIT, tile, tw_stride(v, tw_blk(c->tW_f32)));
+    c->eng->timed("triples_w:bf16x3", fl, [&]() {
+      launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)g * nb, PREC_BF16X3_PRESPLIT, tile, c->st);
+    });
+  }
+}
+
+// triples_build_w_pair with every operand an MFMA panel, for the kernel
+// that copies panel blocks straight into LDS (GemmDesc::dma): the same
+// three transposed grouped GEMMs, pointer choices and K segments.
+void triples_build_w_dma(ugx_ctx *c, const TEnt *ents, int nb) {
+  const int o = c->no, v = c->nv;
+  const long v2 = (long)v * v, v3 = v2 * v;
+  const long pP = panel_size(1, v2, v), nP = panel_size(o, v2, v);          // P, Q
+  const long pT = panel_size(1, v, v), nT = panel_size(o * o, v, v);         // t2 slices
+  const long pO = panel_size(1, v, o), nO = panel_size(o * o, v, o);         // OV slices
+  const long pTT = panel_size(1, v2, o), nTT = panel_size(o, v2, o);         // T2t, T2tq
+  const long blk_v = 16L * ((v + 31) / 32 * 32), blk_o = 16L * ((o + 31) / 32 * 32);
+  std::vector<GemmPtrs> tab((size_t)3 * nb);
+  for (int b = 0; b < nb; b++) {
+    const ugx_ctx *src = ents[b].src;  // the entry's own operands (same no, nv)
+    const unsigned short *P = (const unsigned short *)src->b_P.p, *Q = (const unsigned short *)src->b_Q.p;
+    const unsigned short *T2 = (const unsigned short *)src->p_t2.p, *OV = (const unsigned short *)src->p_OV.p;
+    const unsigned short *T2t = (const unsigned short *)src->p_T2t.p, *T2q = (const unsigned short *)src->p_T2tq.p;
+    const long i = ents[b].i, j = ents[b].j, k = ents[b].k;
+    double *W = (double *)((char *)c->tWbuf.p + (size_t)b * tw_stride(v, tw_blk(c->tW_f32)) * (c->tW_f32 ? 4 : 8));
+    GemmPtrs &A = tab[b], &B = tab[(size_t)nb + b], &C = tab[(size_t)2 * nb + b];
+    A.A[0] = T2 + (k * o + j) * pT;   A.B[0] = P + i * pP;
+    A.A[1] = OV + (j * o + k) * pO;   A.B[1] = T2t + i * pTT;
+    A.A[2] = T2 + (k * o + i) * pT;   A.B[2] = Q + j * pP;
+    A.A[3] = OV + (i * o + k) * pO;   A.B[3] = T2q + j * pTT;
+    A.C = W;
+    B.A[0] = P + i * pP;              B.B[0] = T2 + (j * o + k) * pT;
+    B.A[1] = T2t + i * pTT;           B.B[1] = OV + (k * o + j) * pO;
+    B.A[2] = Q + k * pP;              B.B[2] = T2 + (j * o + i) * pT;
+    B.A[3] = T2q + k * pTT;           B.B[3] = OV + (i * o + j) * pO;
+    B.C = W;
+    C.A[0] = P + j * pP;              C.B[0] = T2 + (i * o + k) * pT;
+    C.A[1] = T2t + j * pTT;           C.B[1] = OV + (k * o + i) * pO;
+    C.A[2] = Q + k * pP;              C.B[2] = T2 + (i * o + j) * pT;
+    C.A[3] = T2q + k * pTT;           C.B[3] = OV + (j * o + i) * pO;
+    C.C = W;
+  }
+  c->tptrs.alloc(tab.size() * sizeof(GemmPtrs));
+  check(hipMemcpyAsync(c->tptrs.p, tab.data(), tab.size() * sizeof(GemmPtrs), hipMemcpyHostToDevice, c->st),
+        "pointer table copy");
+  auto tile_rows_v = [&](int n) { return n <= 32 ? 2 : n <= 64 ? 3 : n <= 80 ? 8 : n <= 128 ? 0 : n <= 160 ? 5 : 0; };
+  auto tile_cols_v = [&](int n) { return n <= 32 ? 2 : n <= 64 ? 4 : n <= 80 ? 10 : n <= 128 ? 0 : n <= 160 ? 9 : 0; };
+  const double fl = 4.0 * (double)nb * (double)v3 * (double)(v + o);
+  for (int g = 0; g < 3; g++) {
+    GemmDesc d{};
+    d.nseg = 4;
+    d.batch = nb;
+    d.vec_ok = 1;  // panel bases are whole kilobytes apart
+    for (int s = 0; s < 4; s++) {
+      const bool part = (s % 2 == 0);
+      GemmSeg &q = d.seg[s];
+      q.K = part ? v : o;
+      q.sign = part ? 1.0 : -1.0;
+      q.sak = 1; q.sbk = 1;
+      q.sam = BK_PANEL; q.sbn = BK_PANEL;
+      const long lo_small = part ? nT : nO, lo_big = part ? nP : nTT, blk = part ? blk_v : blk_o;
+      q.pblkA = blk; q.pblkB = blk;
+      if (g == 0) { q.loA = lo_small; q.loB = lo_big; }
+      else { q.loA = lo_big; q.loB = lo_small; }
+    }
+    int tile;
+    if (g == 0) { d.M = v; d.N = (int)v2; tile = tile_cols_v(v); }
+    else { d.M = (int)v2; d.N = v; tile = tile_rows_v(v); }
+    d.alpha = 1.0;
+    d.beta = (g == 0) ? 0.0 : 1.0;
+    d.c_f32 = c->tW_f32;
+    d.tag = 1;
+    set_w_tables(c, d, g, true);
+    d.row4 = (c->tW_f32 && c->tOff_row4[g]) ? 1 : 0;
+    d.zgroup = c->t_zgroup;
+    d.dma = c->t_dma;
     tile = c->eng->tile_for(d, tab.data() + (size_t)g * nb, PREC_BF16X3_PRESPLIT, tile, tw_stride(v, tw_blk(c->tW_f32)));
     c->eng->timed("triples_w:bf16x3", fl, [&]() {
       launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)g * nb, PREC_BF16X3_PRESPLIT, tile, c->st);
@@ -825,7 +926,8 @@ std::vector<double> triples_entries(ugx_ctx *c, int prec, const std::vector<TEnt
   c->tCounter.alloc(64);
   for (long t0 = 0; t0 < nt; t0 += nb) {
     int b = (int)std::min<long>(nb, nt - t0);
-    if (prec == PREC_BF16X3) triples_build_w_pair(c, ents.data() + t0, b);
+    if (prec == PREC_BF16X3 && c->t_dma > 1) triples_build_w_dma(c, ents.data() + t0, b);
+    else if (prec == PREC_BF16X3) triples_build_w_pair(c, ents.data() + t0, b);
     else triples_build_w(c, prec, ents.data() + t0, b);
     c->eng->timed("triples_e", 0.0, [&]() {
       launch_triples_energy(o, v, b, (const int *)c->ttrip.p + 4 * t0, (const TripProb *)c->tProb.p, c->tWbuf.p, wf,
@@ -929,9 +1031,14 @@ int ugx_create(ugx_ctx **out, int device) {
   }
   c->eng.reset(new Engine(c->st));
   // tuning knobs: LDS stages of the BF16x3 kernel (1 or 2); triplets per
-  // (T) batch
+  // (T) batch; triplets per output tile of a W3 launch; the W3 kernel (0:
+  // register-staged operands, 2 or 3: panels copied into LDS, with that many
+  // stages)
   if (const char *e = std::getenv("UGX_B3_STAGES")) gemm_set_b3_stages(std::atoi(e));
   if (const char *e = std::getenv("UGX_T_BATCH")) c->t_batch = std::atoi(e);
+  if (const char *e = std::getenv("UGX_T_ZGROUP")) c->t_zgroup = std::atoi(e);
+  if (const char *e = std::getenv("UGX_T_DMA")) c->t_dma = std::atoi(e);
+  if (c->t_dma != 0 && c->t_dma != 2 && c->t_dma != 3) c->t_dma = 0;
   *out = c;
   return UGX_OK;
 }
@@ -963,6 +1070,7 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
     c->have_intermediates = false;
     c->f_tP.release(); c->f_tQ.release(); c->f_ooov.release(); c->f_t2.release();
   c->b_P.release(); c->b_Q.release(); c->b_OVt.release();
+  c->p_OV.release(); c->p_t2.release(); c->p_T2t.release(); c->p_T2tq.release();
     c->diis_err.clear(); c->diis_amp.clear(); c->diis_nvec = 0;
     c->no = no; c->nv = nv;
     const long n = no + nv, o = no, v = nv;
@@ -1171,7 +1279,8 @@ int ugx_w3(ugx_ctx *c, int i, int j, int k, int precision, double *W3) {
     const long v = c->nv, ws = tw_stride(c->nv, T), nbk = tw_nb(c->nv, T);
     c->tWbuf.alloc((size_t)ws * 8);
     const TEnt one{c, i, j, k};
-    if (precision == PREC_BF16X3) triples_build_w_pair(c, &one, 1);
+    if (precision == PREC_BF16X3 && c->t_dma > 1) triples_build_w_dma(c, &one, 1);
+    else if (precision == PREC_BF16X3) triples_build_w_pair(c, &one, 1);
     else triples_build_w(c, precision, &one, 1);
     // blocked layout -> [a][b][c]
     std::vector<double> wb(ws);
@@ -1349,6 +1458,8 @@ int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_o
       s->eng->set_autotune(c->eng->autotune());
       s->mlp = c->mlp;
       s->t_batch = c->t_batch;
+      s->t_zgroup = c->t_zgroup;
+      s->t_dma = c->t_dma;
     }
     // CCSD: workers take problems largest first (o^2 v^4 ladder cost)
     std::vector<int> order(nprob);

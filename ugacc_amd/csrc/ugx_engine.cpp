@@ -405,7 +405,7 @@ int Engine::tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kpre
   std::string key = "M" + std::to_string(d.M) + " N" + std::to_string(d.N) + " p" + std::to_string(kprec) +
                     (d.c_f32 ? " f" : " d") + (d.beta != 0.0 ? " acc" : "") + (d.cdiv > 0 ? " cd" : "") +
                     (d.ndiv > 0 ? " nd" : "") + (d.offm ? " tab" : "") + (d.tag ? " t" : "") + (ptrs_host ? " grouped" : " b" + std::to_string(d.batch)) +
-                    (d.ktotal > 0 ? " kt" + std::to_string(d.ktotal) : "");
+                    (d.ktotal > 0 ? " kt" + std::to_string(d.ktotal) : "") + (d.dma > 1 ? " dma" + std::to_string(d.dma) : "");
   for (int s = 0; s < d.nseg; s++) {
     const GemmSeg &g = d.seg[s];
     key += " [K" + std::to_string(g.K) + " " + std::to_string(g.sam) + "," + std::to_string(g.sak) + "," +

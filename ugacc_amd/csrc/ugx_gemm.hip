@@ -184,6 +184,26 @@ __device__ inline void tile_of(int bid, int nblk, int tiles_m, int tiles_n, int 
   tn = in / rows;
 }
 
+// Batched launches with d.zgroup = G > 1: block (x, y) of the (tiles x
+// batch) grid -> (tile, batch entry).  The hardware deals linear block ids
+// round-robin to the 8 XCDs; XCD x is given the contiguous range of work
+// items [start_x, start_x + count_x) in the order (group of G entries, tile,
+// entry), so G consecutive entries of one output tile run back to back on
+// one XCD and share, through its L2, the operand tiles they have in common
+// (the (T) entries of a batch share i and often j).  Any grid size: XCD x
+// holds count_x = q + (x < rem) ids, start_x = x q + min(x, rem).
+__device__ inline void zgroup_of(int G, int &bx, long &z) {
+  constexpr long NXCD = 8;
+  const long tiles = gridDim.x, nb = gridDim.y, total = tiles * nb;
+  const long L = (long)blockIdx.y * tiles + blockIdx.x;
+  const long q = total / NXCD, rem = total % NXCD, x = L % NXCD;
+  const long w = x * q + (x < rem ? x : rem) + L / NXCD;  // work item
+  const long g = w / (G * tiles), r = w % (G * tiles);
+  const long gz = (nb - g * G < G) ? nb - g * G : G;      // entries in group g
+  bx = (int)(r / gz);
+  z = g * G + r % gz;
+}
+
 // Address parts of C(m, n): C[c_row_off(m) + c_col_off(n)] (an offset table,
 // a two-level split, or a plain stride per dimension).
 __device__ inline long c_row_off(const GemmDesc &d, int m) {
@@ -417,7 +437,8 @@ __device__ inline float bf16_to_float(unsigned short h) { return __uint_as_float
 // Element (r, k) of a slice lies at hi + (r/16)*sblk + (r%16)*srow + k%32:
 // row-major operands have sblk = 16*srow and step BK3 halves per slice;
 // MFMA panels (GemmSeg::pblkA/B) have srow = 32, sblk = pblk and step 512,
-// so the 16 rows x 32 k of a slice are one contiguous kilobyte.
+// so the 16 rows x 32 k of a slice are one contiguous kilobyte, and their
+// 8-k chunks are swizzled (panel_chunk).
 template <int R>
 struct PairLoader {
   static_assert(R % 16 == 0, "tile rows must be a multiple of 16");
@@ -426,10 +447,11 @@ struct PairLoader {
   const unsigned short *hi;                            // row0, k0 of the current slice
   long lo, srow, sblk, kstep;
   int rows_left, K;
+  bool swz;
   __device__ inline void setup(const unsigned short *base, long lo_, long srow_, long pblk, int row0, int rows,
                                int K_) {
-    if (pblk > 0) { srow = BK3; sblk = pblk; kstep = 16 * BK3; }
-    else { srow = srow_; sblk = 16 * srow_; kstep = BK3; }
+    if (pblk > 0) { srow = BK3; sblk = pblk; kstep = 16 * BK3; swz = true; }
+    else { srow = srow_; sblk = 16 * srow_; kstep = BK3; swz = false; }
     hi = base + (long)(row0 >> 4) * sblk;  // row0 is a multiple of 16
     lo = lo_; rows_left = rows - row0; K = K_;
   }
@@ -442,7 +464,8 @@ struct PairLoader {
       chunk(c, r, kc);
       uint4 vh = {0u, 0u, 0u, 0u}, vl = {0u, 0u, 0u, 0u};
       if (c < CH && r < rows_left) {
-        const unsigned short *ph = hi + (long)(r >> 4) * sblk + (long)(r & 15) * srow + kc;
+        const int kp = swz ? panel_chunk(r & 15, kc >> 3) << 3 : kc;  // stored chunk of a panel row
+        const unsigned short *ph = hi + (long)(r >> 4) * sblk + (long)(r & 15) * srow + kp;
         if (VEC) {
           if (k0 + kc < K) {
             vh = *reinterpret_cast<const uint4 *>(ph);
@@ -506,11 +529,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
 
   const int M = d.M, N = d.N;
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
-  int tm, tn;
-  tile_of(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+  int tm, tn, bx = blockIdx.x;
+  long z = blockIdx.y;
+  if (d.zgroup > 1) {
+    zgroup_of(d.zgroup, bx, z);
+    tm = bx % tiles_m;  // the XCD mapping is zgroup_of's
+    tn = bx / tiles_m;
+  } else {
+    tile_of(bx, gridDim.x, tiles_m, tiles_n, tm, tn);
+  }
   if (tm >= tiles_m || tn >= tiles_n) return;
   const int m0 = tm * BM, n0 = tn * BN;
-  const long z = blockIdx.y;
   const int tid = threadIdx.x;
 
   Segs<unsigned short, BK3> S;
@@ -622,6 +651,179 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void k_
   store_tile<TM, TN>(d, Cv, ro, co, acc);
 }
 
+// ------------------------------------------ BF16x3, panels copied into LDS
+// Every operand of every segment is an MFMA panel (GemmSeg::pblkA/B > 0).
+// A slice of the tile is (BM + BN) / 16 row blocks of each operand, hi and
+// lo: OPS one-kilobyte panel blocks, copied verbatim into LDS by
+// global_load_lds_dwordx4 (one block per wave instruction, lane l moving
+// bytes 16l..16l+15), so no registers stage the tile and no LDS store
+// instructions run.  The panel swizzle (panel_chunk) makes the fragment
+// reads of the copied 64-byte rows conflict-free.  STAGES LDS stages; the
+// blocks of slice t + STAGES - 1 are issued right after the barrier of
+// slice t, so each copy has STAGES - 1 slices of MFMA work to land; one
+// barrier per slice.  Every wave issues the same OPS / 4 copies per slice
+// (blocks past the operand's last row block repeat that block, and the
+// rows they feed are never stored), so "slice t has landed" is an exact
+// vmcnt wait.  A negative segment sign flips the sign bits of the B
+// fragments in registers.
+template <int N>
+__device__ inline void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt is six bits");
+  // gfx9 s_waitcnt: vmcnt[3:0], expcnt[6:4] (7: no wait), lgkmcnt[11:8]
+  // (15: no wait), vmcnt[5:4] in bits [15:14]
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+template <int WGM, int WGN, int TM, int TN, int STAGES, int TAG>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(STAGES == 2 ? 2 : 1))) void k_gemm_b3d(
+    GemmDesc d, const GemmPtrs *__restrict__ ptrs) {
+  static_assert(STAGES == 2 || STAGES == 3, "two or three LDS stages");
+  constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
+  constexpr int RA = BM / 16, RB = BN / 16, OPS = 2 * (RA + RB), OPW = OPS / 4;
+  static_assert(OPS % 4 == 0, "every wave issues the same number of block copies");
+  // One array per stage, each named statically (the loop is unrolled by
+  // STAGES): the compiler can then tell that fragment reads of one stage do
+  // not alias copies in flight into another, and does not drain them.
+  // Slots per stage: [A hi: RA][A lo: RA][B hi: RB][B lo: RB], 512 halves.
+  __shared__ __attribute__((aligned(1024))) unsigned short L0[OPS][512];
+  __shared__ __attribute__((aligned(1024))) unsigned short L1[OPS][512];
+  __shared__ __attribute__((aligned(1024))) unsigned short L2[STAGES == 3 ? OPS : 1][512];
+
+  const int M = d.M, N = d.N;
+  const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+  int tm, tn, bx = blockIdx.x;
+  long z = blockIdx.y;
+  if (d.zgroup > 1) {
+    zgroup_of(d.zgroup, bx, z);
+    tm = bx % tiles_m;
+    tn = bx / tiles_m;
+  } else {
+    tile_of(bx, gridDim.x, tiles_m, tiles_n, tm, tn);
+  }
+  if (tm >= tiles_m || tn >= tiles_n) return;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave / WGN, wn = wave % WGN;
+  const int r16 = lane & 15, kq = lane >> 4;
+  // fragment offset in a block: row r16, logical chunk kq
+  const int foff = r16 * BK3 + panel_chunk(r16, kq) * 8;
+
+  Segs<unsigned short, BK3> S;
+  S.init(d, z);
+  const int ntiles = S.ntiles;
+  const int lastA = (M + 15) / 16 - 1, lastB = (N + 15) / 16 - 1;
+
+  int cur_seg = -1;
+  const unsigned short *bA = nullptr, *bB = nullptr;
+  long loA = 0, loB = 0, pbA = 0, pbB = 0;
+  auto issue = [&](int t, unsigned short (*Ls)[512]) {
+    int k0;
+    const int s = S.locate(t, k0);
+    if (s != cur_seg) {
+      bA = S.a(d, ptrs, z, s); bB = S.b(d, ptrs, z, s);
+      loA = d.seg[s].loA; loB = d.seg[s].loB; pbA = d.seg[s].pblkA; pbB = d.seg[s].pblkB;
+      cur_seg = s;
+    }
+    const long ks = k0 / BK3;
+#pragma unroll
+    for (int q = 0; q < OPW; q++) {
+      const int op = wave + 4 * q;
+      const unsigned short *src;
+      if (op < 2 * RA) {
+        const int h = op / RA, rb = min(m0 / 16 + op % RA, lastA);
+        src = bA + (h ? loA : 0) + rb * pbA + ks * 512;
+      } else {
+        const int o2 = op - 2 * RA, h = o2 / RB, rb = min(n0 / 16 + o2 % RB, lastB);
+        src = bB + (h ? loB : 0) + rb * pbB + ks * 512;
+      }
+      __builtin_amdgcn_global_load_lds((const void *)(src + lane * 8),
+                                       (__attribute__((address_space(3))) void *)&Ls[op][0], 16, 0, 0);
+    }
+  };
+
+  flt4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++) acc[i][j] = flt4{0.f, 0.f, 0.f, 0.f};
+
+  // One slice: wait until this wave's copies of slice t have landed (at most
+  // the copies of the slices after it outstanding), barrier (every wave's
+  // copies are in; the stage read in slice t - 1 is free), read slice t's
+  // fragments, refill the free stage with slice t + STAGES - 1, then the
+  // MFMAs.  The barrier is a plain s_barrier: the explicit vmcnt wait is the
+  // only ordering the copies need (a fenced barrier would drain the copies
+  // just issued).  The copies are issued after the fragment reads because
+  // the compiler waits for every copy in flight before an LDS read.
+  constexpr int PRE = STAGES - 1;
+  auto step = [&](int t, const unsigned short (*Lr)[512], unsigned short (*Li)[512]) {
+    if (STAGES == 3 && t + 1 < ntiles) wait_vmcnt<OPW>();
+    else wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    int k0;
+    const int s = S.locate(t, k0);
+    const bool neg = S.sg(d, s) < 0;
+    bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++) {
+      const int blk = wm * TM + i;
+      ah[i] = *reinterpret_cast<const bf16x8 *>(&Lr[blk][foff]);
+      al[i] = *reinterpret_cast<const bf16x8 *>(&Lr[RA + blk][foff]);
+    }
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+      const int blk = wn * TN + j;
+      uint4 uh = *reinterpret_cast<const uint4 *>(&Lr[2 * RA + blk][foff]);
+      uint4 ul = *reinterpret_cast<const uint4 *>(&Lr[2 * RA + RB + blk][foff]);
+      if (neg) {
+        uh.x ^= 0x80008000u; uh.y ^= 0x80008000u; uh.z ^= 0x80008000u; uh.w ^= 0x80008000u;
+        ul.x ^= 0x80008000u; ul.y ^= 0x80008000u; ul.z ^= 0x80008000u; ul.w ^= 0x80008000u;
+      }
+      bh[j] = *reinterpret_cast<const bf16x8 *>(&uh);
+      bl[j] = *reinterpret_cast<const bf16x8 *>(&ul);
+    }
+    if (t + PRE < ntiles) issue(t + PRE, Li);
+#pragma unroll
+    for (int j = 0; j < TN; j++)
+#pragma unroll
+      for (int i = 0; i < TM; i++) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+      }
+  };
+  issue(0, L0);
+  if (STAGES == 3 && ntiles > 1) issue(1, L1);
+  for (int t = 0; t < ntiles; t += STAGES) {
+    if (STAGES == 2) {
+      step(t, L0, L1);
+      if (t + 1 < ntiles) step(t + 1, L1, L0);
+    } else {
+      step(t, L0, L2);
+      if (t + 1 < ntiles) step(t + 1, L1, L0);
+      if (t + 2 < ntiles) step(t + 2, L2, L1);
+    }
+  }
+
+  void *Cv = ptrs ? (void *)ptrs[z].C
+                  : (d.c_f32 ? (void *)((float *)d.C + z * d.sCz) : (void *)(d.C + z * d.sCz));
+  long ro[TM][4], co[TN];
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int m = m0 + (wm * TM + i) * 16 + 4 * kq + r;
+      ro[i][r] = m < M ? c_row_off(d, m) : -1;
+    }
+#pragma unroll
+  for (int j = 0; j < TN; j++) {
+    const int n = n0 + (wn * TN + j) * 16 + r16;
+    co[j] = n < N ? c_col_off(d, n) : -1;
+  }
+  store_tile<TM, TN>(d, Cv, ro, co, acc);
+}
+
 // ------------------------------------------------------------ dispatch
 struct TileShape {
   int wgm, wgn, tm, tn;
@@ -661,6 +863,23 @@ void launch_t(const GemmDesc &d, const GemmPtrs *ptrs, int prec, hipStream_t st)
   if (tiles > 0x7fffffffL || d.batch > 65535) throw std::runtime_error("ugx: GEMM grid too large");
   dim3 grid((unsigned)tiles, (unsigned)d.batch, 1);
   if (prec == PREC_BF16X3_PRESPLIT) {
+    // tiles whose block copies split evenly over the 4 waves can take the
+    // LDS-copy kernel; the others fall back to the register-staged one
+    if constexpr ((2 * (BM + BN) / 16) % 4 == 0) {
+      if (d.dma > 1) {
+        for (int q = 0; q < d.nseg; q++)
+          if (d.seg[q].pblkA <= 0 || d.seg[q].pblkB <= 0)
+            throw std::runtime_error("ugx: the LDS-copy BF16x3 kernel needs panel operands");
+        if (d.dma == 2) {
+          if (d.tag) hipLaunchKernelGGL((k_gemm_b3d<WGM, WGN, TM, TN, 2, 1>), grid, dim3(NT), 0, st, d, ptrs);
+          else hipLaunchKernelGGL((k_gemm_b3d<WGM, WGN, TM, TN, 2, 0>), grid, dim3(NT), 0, st, d, ptrs);
+        } else {
+          if (d.tag) hipLaunchKernelGGL((k_gemm_b3d<WGM, WGN, TM, TN, 3, 1>), grid, dim3(NT), 0, st, d, ptrs);
+          else hipLaunchKernelGGL((k_gemm_b3d<WGM, WGN, TM, TN, 3, 0>), grid, dim3(NT), 0, st, d, ptrs);
+        }
+        return;
+      }
+    }
     bool vec = b3_vectorizable(d) && (ptrs == nullptr || d.vec_ok);
     if (g_b3_stages == 1) {
       if (d.tag) {

@@ -39,10 +39,17 @@ struct GemmSeg {
   long loA, loB;
   // PREC_BF16X3_PRESPLIT: > 0 marks an operand packed as MFMA panels (see
   // launch_pack_panel): element (r, k) at (r/16)*pblk + (k/32)*512 +
-  // (r%16)*32 + k%32, pblk = 16 * K rounded up to 32; its sam / sbn are
-  // then ignored.  0 = row-major with stride sam / sbn.
+  // (r%16)*32 + 8*panel_chunk(r%16, (k%32)/8) + k%8, pblk = 16 * K rounded
+  // up to 32; its sam / sbn are then ignored.  0 = row-major with stride
+  // sam / sbn.
   long pblkA, pblkB;
 };
+
+// The 8-k chunk c of panel row rr (0..15) is stored at chunk position
+// c ^ ((rr >> 1) & 3) (an involution): a panel block copied verbatim into
+// LDS (64-byte rows) is then read conflict-free by MFMA fragment loads,
+// whose 8 lanes per cycle take rows 0..7 at one logical chunk.
+__host__ __device__ inline int panel_chunk(int rr, int c) { return c ^ ((rr >> 1) & 3); }
 
 // C(m,n) lives at C[(m / cdiv) * scm1 + (m % cdiv) * scm
 //                   + (n / ndiv) * scn1 + (n % ndiv) * scn];
@@ -79,6 +86,14 @@ struct GemmDesc {
   // offm[m] + offn[n] a multiple of 4 and C 16-byte aligned, so the epilogue
   // moves four rows of C as one 16-byte access
   int row4;
+  // PREC_BF16X3_PRESPLIT pointer-table launches: > 1 runs groups of zgroup
+  // consecutive batch entries of one output tile back to back on one XCD
+  // (entries that share operand tiles then reuse them from L2)
+  int zgroup;
+  // PREC_BF16X3_PRESPLIT with every operand of every segment a panel: > 1
+  // selects the kernel that copies panel blocks straight into LDS (global
+  // loads to LDS, no register staging) with that many LDS stages (2 or 3)
+  int dma;
 };
 
 // Per-batch pointers for grouped launches (used by the (T) build where each
@@ -120,11 +135,16 @@ struct PermDesc {
 void launch_permute(const PermDesc &p, hipStream_t st);
 // Packs np FP64 matrices in[p*sp + r*sr + k*sk] (r < R, k < K) as split
 // BF16 MFMA panels: hi halves first, then lo halves (at offset
-// panel_size(np, R, K)), each [p][r/16][k/32][r%16][k%32] with R rounded up
-// to 16 and K to 32, zeros in the padding.
+// panel_size(np, R, K)), each [p][r/16][k/32][r%16][k%32] (8-k chunks
+// swizzled by panel_chunk) with R rounded up to 16 and K to 32, zeros in
+// the padding.
 long panel_size(int np, long R, int K);
 void launch_pack_panel(const double *in, long sp, long sr, long sk, int np, long R, int K, unsigned short *out,
                        hipStream_t st);
+// The same with a two-level row index: row r reads in[p*sp + (r / R2)*sr1
+// + (r % R2)*sr2 + k*sk].
+void launch_pack_panel2(const double *in, long sp, long sr1, long sr2, long R2, long sk, int np, long R, int K,
+                        unsigned short *out, hipStream_t st);
 
 void launch_fill(long n, double v, double *y, hipStream_t st);
 void launch_axpby(long n, double a, const double *x, double b, double *y, hipStream_t st);  // y = a x + b y
