@@ -81,7 +81,8 @@ struct ugx_ctx {
   // learned guess: network (owned by the caller) and its output f(x_ijab)
   const ugx_mlp *mlp = nullptr;
   DBuf G;
-  int t_batch = 0;  // cap on triplets per (T) batch (0: by the 2 GB W budget)
+  int t_batch = 0;  // cap on triplets per (T) batch (0: by the W budget)
+  long t_wbytes = 12L << 30;  // (T) W buffer budget, bytes (at most a quarter of the free HBM)
   int t_zgroup = 4;  // consecutive triplets per output tile in the BF16x3 W3 launches
   int t_dma = 2;     // > 1: BF16x3 W3 launches copy panels into LDS, with that many stages (0: registers)
   // hipGraphs of the residual, keyed by (amplitude buffer parity, precision)
@@ -895,7 +896,7 @@ void triples_operands(ugx_ctx *c, int prec) {
 // context ents[e].src (all of one size, operands prepared by
 // triples_operands and complete on the device); prob[e] indexes probs, the
 // matching (eps, oovv, t1) of each problem.  Batches of entries get one W
-// buffer of up to ~2 GB: three grouped W3 GEMM launches whose pointer
+// buffer of up to t_wbytes (12 GiB by default): three grouped W3 GEMM launches whose pointer
 // tables span every entry of the batch, then one persistent energy kernel
 // over all (entry, block pair) items.
 std::vector<double> triples_entries(ugx_ctx *c, int prec, const std::vector<TEnt> &ents,
@@ -907,7 +908,12 @@ std::vector<double> triples_entries(ugx_ctx *c, int prec, const std::vector<TEnt
   const int wf = (prec != PREC_FP64) ? 1 : 0;  // W is FP32 in the low-precision forms
   const size_t wes = wf ? 4 : 8;
   const long ws = tw_stride(v, tw_blk(wf));
-  long nb = std::max<long>(1, std::min<long>(nt, (2L << 30) / (ws * (long)wes)));
+  // fewer, larger batches: fewer launches and energy-kernel tails (12 GiB
+  // holds 564 (H2O)8 triplets: 42 batches instead of 82 at 2 GiB)
+  size_t mem_free = 0, mem_total = 0;
+  check(hipMemGetInfo(&mem_free, &mem_total), "memory info");
+  const long budget = std::min<long>(c->t_wbytes, (long)(mem_free / 4));
+  long nb = std::max<long>(1, std::min<long>(nt, budget / (ws * (long)wes)));
   nb = std::min<long>(nb, c->t_batch > 0 ? c->t_batch : 4096);
   c->tW_f32 = wf;
   c->tWbuf.alloc((size_t)nb * ws * wes);
@@ -1033,11 +1039,12 @@ int ugx_create(ugx_ctx **out, int device) {
   // tuning knobs: LDS stages of the BF16x3 kernel (1 or 2); triplets per
   // (T) batch; triplets per output tile of a W3 launch; the W3 kernel (0:
   // register-staged operands, 2 or 3: panels copied into LDS, with that many
-  // stages)
+  // stages); the (T) W buffer budget in GiB
   if (const char *e = std::getenv("UGX_B3_STAGES")) gemm_set_b3_stages(std::atoi(e));
   if (const char *e = std::getenv("UGX_T_BATCH")) c->t_batch = std::atoi(e);
   if (const char *e = std::getenv("UGX_T_ZGROUP")) c->t_zgroup = std::atoi(e);
   if (const char *e = std::getenv("UGX_T_DMA")) c->t_dma = std::atoi(e);
+  if (const char *e = std::getenv("UGX_T_WGIB")) c->t_wbytes = std::max(1L, std::atol(e)) << 30;
   if (c->t_dma != 0 && c->t_dma != 2 && c->t_dma != 3) c->t_dma = 0;
   *out = c;
   return UGX_OK;
@@ -1460,6 +1467,7 @@ int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_o
       s->t_batch = c->t_batch;
       s->t_zgroup = c->t_zgroup;
       s->t_dma = c->t_dma;
+      s->t_wbytes = c->t_wbytes;
     }
     // CCSD: workers take problems largest first (o^2 v^4 ladder cost)
     std::vector<int> order(nprob);
