@@ -52,7 +52,7 @@ FP64_CONV = 1e-10
 # plain copy of known size (fetch x2.0, write x1.0); the runs and their
 # commands are in profiles/ (filled in when measured for the current
 # kernels)
-TRAFFIC = {("h2o8", "triples_w:bf16x3"): 11.57e9}
+TRAFFIC = {("h2o8", "triples_w:bf16x3"): 45.9e9}
 
 
 def log(msg):
@@ -126,7 +126,7 @@ def roofline(stats, workload):
     st = stats[dom]
     form = dom.split(":")[1] if ":" in dom else "fp64"
     ach = st["gflop"] / st["ms"]  # GFLOP/ms = TFLOP/s
-    names = {"triples_w": "k_gemm_b3p / k_gemm (T) W3 grouped GEMM", "ladder": "k_gemm particle-particle ladder"}
+    names = {"triples_w": "k_gemm_b3d (BF16x3) / k_gemm (FP64, FP32) (T) W3 grouped GEMM", "ladder": "k_gemm particle-particle ladder"}
     return dict(bound="mfma", achieved=ach, peak=PEAK[form], unit="TFLOP/s", frac=ach / PEAK[form],
                 traffic=TRAFFIC.get((workload, dom)),
                 kernel="%s (%s)" % (names.get(dom.split(":")[0], dom), dom), launches=st["launches"],
