@@ -128,6 +128,34 @@ def test_triples_low_precision_forms(no, nv):
         assert np.max(np.abs(W - W64)) < 1e-5 * np.max(np.abs(W64)), prec
 
 
+@pytest.mark.parametrize("no,nv", [(5, 19), (1, 9), (6, 33)])
+def test_packed_ladder_matches_full_contraction(no, nv, monkeypatch):
+    """The ladder as two packed GEMMs over the pairs i>=j, a>=b, e>=f (the
+    default for real-orbital integrals) and as the full contraction
+    (UGX_LADDER=full, read when a context is created): R2 in FP64 to
+    rounding, and in BF16x3 to its precision.  o = 1 has no antisymmetric
+    part."""
+    announce("packed vs full ladder o=%d v=%d" % (no, nv))
+    c, fock, ints = loaded(no, nv, seed=no * 13 + nv)
+    rng = np.random.default_rng(no + 7 * nv)
+    t1 = 0.02 * rng.standard_normal((no, nv))
+    t2 = 0.02 * rng.standard_normal((no, no, nv, nv))
+    t2 = 0.5 * (t2 + t2.transpose(1, 0, 3, 2))
+    R1p, R2p = c.residual(t1, t2)
+    R1b, R2b = c.residual(t1, t2, ugx.PREC_BF16X3)
+    monkeypatch.setenv("UGX_LADDER", "full")
+    full = ugx.Context(0)
+    full.load_hamiltonian(no, nv, fock, ugx.split_blocks(ints, no))
+    R1f, R2f = full.residual(t1, t2)
+    R1fb, R2fb = full.residual(t1, t2, ugx.PREC_BF16X3)
+    full.close()
+    s = max(1.0, np.max(np.abs(R2f)))
+    assert np.max(np.abs(R2p - R2f)) < 1e-13 * s
+    np.testing.assert_array_equal(R1p, R1f)
+    assert np.max(np.abs(R2b - R2fb)) < 2e-6 * s
+    np.testing.assert_allclose(R2p, R2p.transpose(1, 0, 3, 2), atol=1e-15 * s)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("no,nv,stages", [(8, 40, "3"), (5, 19, "0"), (3, 70, "0")])
 def test_triples_bf16x3_lds_copy_matches_register_path(no, nv, stages, monkeypatch):

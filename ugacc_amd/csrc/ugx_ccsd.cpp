@@ -76,6 +76,11 @@ struct ugx_ctx {
   DBuf b_P, b_Q, b_OVt, b_t2, b_T2t, b_T2tq;
   // the same operands as MFMA panels, for the LDS-copy W3 kernel
   DBuf p_OV, p_t2, p_T2t, p_T2tq;
+  // packed symmetric / antisymmetric ladder (launch_ladder_*): V+, V- once
+  // per Hamiltonian; S, A and the two GEMM outputs per residual
+  bool lad_packed = false;
+  bool lad_full = false;  // UGX_LADDER=full: always the full contraction
+  DBuf lad_vp, lad_vm, lad_S, lad_A, lad_LP, lad_LM;
   bool tables_ready = false;
   std::vector<int> trip_host;
   // learned guess: network (owned by the caller) and its output f(x_ijab)
@@ -254,7 +259,21 @@ void residual(ugx_ctx *c, int prec) {
   E.contract("ijae,be->ijab", 1.0, T2, FAEP, 1.0, XX, prec, "t2_F");
   E.contract("imab,mj->ijab", -1.0, T2, FMIP, 1.0, XX, prec, "t2_F");
   E.contract("mnab,mnij->ijab", 0.5, TAU, WO, 1.0, XX, prec, "t2_Wmnij");
-  E.contract("ijef,abef->ijab", 0.5, TAU, VVVV, 1.0, XX, prec, "ladder");
+  if (c->lad_packed) {
+    // sum_ef tau_ij^ef <ab|ef> = (L+ + L-)/2 over the pairs i>=j, a>=b, e>=f
+    // (tau_ij^ef = tau_ji^fe, <ab|ef> = <ba|fe>): a quarter of the flops
+    const int Q = o * (o + 1) / 2, Qs = o * (o - 1) / 2, P = v * (v + 1) / 2, Ps = v * (v - 1) / 2;
+    const int ldp = (int)ladder_ld(P), ldm = (int)ladder_ld(Ps);  // k padded with zeros
+    launch_ladder_pack(o, v, tau, c->lad_S.d(), c->lad_A.d(), st);
+    E.contract("qe,ae->qa", 1.0, V(c->lad_S.d(), {Q, ldp}), Cn(c->lad_vp.d(), {P, ldp}), 0.0,
+               V(c->lad_LP.d(), {Q, P}), prec, "ladder");
+    if (Qs > 0 && Ps > 0)
+      E.contract("qe,ae->qa", 1.0, V(c->lad_A.d(), {Qs, ldm}), Cn(c->lad_vm.d(), {Ps, ldm}), 0.0,
+                 V(c->lad_LM.d(), {Qs, Ps}), prec, "ladder");
+    launch_ladder_unpack(o, v, c->lad_LP.d(), c->lad_LM.d(), 0.25, c->X.d(), st);
+  } else {
+    E.contract("ijef,abef->ijab", 0.5, TAU, VVVV, 1.0, XX, prec, "ladder");
+  }
   E.contract("ie,jeba->ijab", 1.0, T1, OVVV, 1.0, XX, P64, "t2_t1");  // <ab|ej> = <je|ba>
   // Z' = Zmbij + sum_e t1_ie <mb|ej> + <mb|ij>  (the t1 t1 <mb|ej> and
   // t1 <mb|ij> terms of :512-515, :528-535 share Zmbij's t1_ma contraction)
@@ -1039,12 +1058,14 @@ int ugx_create(ugx_ctx **out, int device) {
   // tuning knobs: LDS stages of the BF16x3 kernel (1 or 2); triplets per
   // (T) batch; triplets per output tile of a W3 launch; the W3 kernel (0:
   // register-staged operands, 2 or 3: panels copied into LDS, with that many
-  // stages); the (T) W buffer budget in GiB
+  // stages); the (T) W buffer budget in GiB; UGX_LADDER=full keeps the
+  // full ladder contraction
   if (const char *e = std::getenv("UGX_B3_STAGES")) gemm_set_b3_stages(std::atoi(e));
   if (const char *e = std::getenv("UGX_T_BATCH")) c->t_batch = std::atoi(e);
   if (const char *e = std::getenv("UGX_T_ZGROUP")) c->t_zgroup = std::atoi(e);
   if (const char *e = std::getenv("UGX_T_DMA")) c->t_dma = std::atoi(e);
   if (const char *e = std::getenv("UGX_T_WGIB")) c->t_wbytes = std::max(1L, std::atol(e)) << 30;
+  if (const char *e = std::getenv("UGX_LADDER")) c->lad_full = std::string(e) == "full";
   if (c->t_dma != 0 && c->t_dma != 2 && c->t_dma != 3) c->t_dma = 0;
   *out = c;
   return UGX_OK;
@@ -1106,6 +1127,35 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
     upload(c->ovvv, ovvv, o * v * v * v, c->st);
     upload(c->vvvv, vvvv, v * v * v * v, c->st);
     Engine &E = *c->eng;
+    // The packed ladder needs <ab|ef> = <ba|fe> (real orbitals), checked on
+    // a deterministic sample of the block, and grids within limits;
+    // otherwise the full contraction runs.
+    {
+      bool sym = true;
+      double vmax = 0.0;
+      unsigned long long x = 0x9E3779B97F4A7C15ull;
+      auto rnd = [&](long m) { x = x * 6364136223846793005ull + 1442695040888963407ull; return (long)((x >> 33) % (unsigned long long)m); };
+      std::vector<std::pair<double, double>> smp;
+      for (int q = 0; q < 4096; q++) {
+        const long a = rnd(v), b = rnd(v), e = rnd(v), f = rnd(v);
+        const double p1 = vvvv[((a * v + b) * v + e) * v + f], p2 = vvvv[((b * v + a) * v + f) * v + e];
+        vmax = std::max(vmax, std::fabs(p1));
+        smp.emplace_back(p1, p2);
+      }
+      for (auto &pr : smp)
+        if (std::fabs(pr.first - pr.second) > 1e-12 * vmax + 1e-300) sym = false;
+      const long P = v * (v + 1) / 2, Ps = v * (v - 1) / 2, Q = o * (o + 1) / 2, Qs = o * (o - 1) / 2;
+      c->lad_packed = sym && !c->lad_full && P <= 65535 && Q <= 65535;
+      if (c->lad_packed) {
+        c->lad_vp.alloc((size_t)P * ladder_ld(P) * 8);
+        c->lad_vm.alloc((size_t)std::max<long>(1, Ps * ladder_ld(Ps)) * 8);
+        c->lad_S.alloc((size_t)Q * ladder_ld(P) * 8);
+        c->lad_A.alloc((size_t)std::max<long>(1, Qs * ladder_ld(Ps)) * 8);
+        c->lad_LP.alloc((size_t)Q * P * 8);
+        c->lad_LM.alloc((size_t)std::max<long>(1, Qs * Ps) * 8);
+        launch_ladder_vpm(nv, c->vvvv.d(), c->lad_vp.d(), c->lad_vm.d(), c->st);
+      }
+    }
     // L = 2<pq|rs> - <pq|sr> for the blocks the equations read (hamiltonian.cc:96-102)
     c->Loovv.alloc(o * o * v * v * 8);
     E.permute("mnef->mnef", 2.0, cref(c->oovv.d(), {no, no, nv, nv}), 0.0, vref(c->Loovv.d(), {no, no, nv, nv}));
@@ -1468,6 +1518,7 @@ int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_o
       s->t_zgroup = c->t_zgroup;
       s->t_dma = c->t_dma;
       s->t_wbytes = c->t_wbytes;
+      s->lad_full = c->lad_full;
     }
     // CCSD: workers take problems largest first (o^2 v^4 ladder cost)
     std::vector<int> order(nprob);

@@ -147,6 +147,19 @@ void launch_pack_panel2(const double *in, long sp, long sr1, long sr2, long R2, 
                         unsigned short *out, hipStream_t st);
 
 void launch_fill(long n, double v, double *y, hipStream_t st);
+// Packed symmetric / antisymmetric ladder.  Pairs p(a,b) = a(a+1)/2 + b for
+// a >= b (P = v(v+1)/2 of them) and s(a,b) = a(a-1)/2 + b for a > b
+// (P' = v(v-1)/2); likewise over (i,j).  The (e,f) index -- the GEMMs'
+// k -- is stored with leading dimension ladder_ld(pairs), a multiple of 32
+// with zeros in the padding (aligned rows for the BF16x3 kernels).
+long ladder_ld(long pairs);
+//   V+[p(ab)][p(ef)] = <ab|ef> + <ab|fe>, V-[s(ab)][s(ef)] = <ab|ef> - <ab|fe>
+void launch_ladder_vpm(int nv, const double *vvvv, double *vp, double *vm, hipStream_t st);
+//   S[p(ij)][p(ef)] = (tau_ij^ef + tau_ij^fe)(1/2 if e = f), A[s(ij)][s(ef)] = tau_ij^ef - tau_ij^fe
+void launch_ladder_pack(int no, int nv, const double *tau, double *S, double *A, hipStream_t st);
+//   X_ijab += scale (L+[p(ij)][p(ab)] +- L-[s(ij)][s(ab)]), the sign s_ij s_ab
+void launch_ladder_unpack(int no, int nv, const double *LP, const double *LM, double scale, double *X,
+                          hipStream_t st);
 void launch_axpby(long n, double a, const double *x, double b, double *y, hipStream_t st);  // y = a x + b y
 // tau = t2 + t1 (x) t1, ttau = t2 + 0.5 t1 (x) t1 (ttau may be null)
 void launch_tau(int no, int nv, const double *t1, const double *t2, double *tau, double *ttau, hipStream_t st);

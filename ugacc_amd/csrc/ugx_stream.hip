@@ -8,6 +8,7 @@
 // by one single-block pass, all in FP64.
 #include "ugx_internal.h"
 
+#include <algorithm>
 #include <stdexcept>
 
 namespace ugx {
@@ -361,9 +362,124 @@ __global__ void k_pack_panel(const double *in, long sp, long sr1, long sr2, long
   }
 }
 
+// ---- packed symmetric / antisymmetric ladder
+// Pair indices: p(a,b) = a(a+1)/2 + b for a >= b, s(a,b) = a(a-1)/2 + b for
+// a > b.  pair_of_index inverts them.
+__device__ inline void pair_of_index(long p, bool strict, int &a, int &b) {
+  // x = floor((sqrt(8p+1)-1)/2), corrected for rounding; a strict pair
+  // s(a,b) is the pair p(a-1, b)
+  int x = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+  while ((long)(x + 1) * (x + 2) / 2 <= p) x++;
+  while ((long)x * (x + 1) / 2 > p) x--;
+  if (strict) { a = x + 1; b = (int)(p - (long)x * (x + 1) / 2); }
+  else { a = x; b = (int)(p - (long)x * (x + 1) / 2); }
+}
+
+// V+[p(ab)][p(ef)] = <ab|ef> + <ab|fe>,  V-[s(ab)][s(ef)] = <ab|ef> - <ab|fe>,
+// rows ldk elements apart with zeros past the last pair
+__global__ void k_ladder_vpm(int nv, const double *__restrict__ vvvv, double *__restrict__ vp,
+                             double *__restrict__ vm, long ldp, long ldm) {
+  const long P = (long)nv * (nv + 1) / 2, Ps = (long)nv * (nv - 1) / 2, v = nv;
+  const long row = blockIdx.y;
+  int a, b;
+  if (row < P) {
+    pair_of_index(row, false, a, b);
+    const double *Vab = vvvv + ((long)a * v + b) * v * v;
+    for (long col = blockIdx.x * (long)NT + threadIdx.x; col < ldp; col += (long)gridDim.x * NT) {
+      int e, f;
+      pair_of_index(col < P ? col : 0, false, e, f);
+      vp[row * ldp + col] = col < P ? Vab[(long)e * v + f] + Vab[(long)f * v + e] : 0.0;
+    }
+  }
+  if (row < Ps) {
+    pair_of_index(row, true, a, b);
+    const double *Vab = vvvv + ((long)a * v + b) * v * v;
+    for (long col = blockIdx.x * (long)NT + threadIdx.x; col < ldm; col += (long)gridDim.x * NT) {
+      int e, f;
+      pair_of_index(col < Ps ? col : 0, true, e, f);
+      vm[row * ldm + col] = col < Ps ? Vab[(long)e * v + f] - Vab[(long)f * v + e] : 0.0;
+    }
+  }
+}
+
+// S[p(ij)][p(ef)] = (tau_ij^ef + tau_ij^fe) (1/2 if e = f),
+// A[s(ij)][s(ef)] = tau_ij^ef - tau_ij^fe, rows ldp / ldm elements apart with
+// zeros past the last pair
+__global__ void k_ladder_pack(int no, int nv, const double *__restrict__ tau, double *__restrict__ S,
+                              double *__restrict__ A, long ldp, long ldm) {
+  const long P = (long)nv * (nv + 1) / 2, Ps = (long)nv * (nv - 1) / 2, v = nv;
+  const long Q = (long)no * (no + 1) / 2, Qs = (long)no * (no - 1) / 2;
+  const long row = blockIdx.y;
+  int i, j;
+  if (row < Q) {
+    pair_of_index(row, false, i, j);
+    const double *T = tau + ((long)i * no + j) * v * v;
+    for (long col = blockIdx.x * (long)NT + threadIdx.x; col < ldp; col += (long)gridDim.x * NT) {
+      int e, f;
+      pair_of_index(col < P ? col : 0, false, e, f);
+      const double x = T[(long)e * v + f] + T[(long)f * v + e];
+      S[row * ldp + col] = col < P ? (e == f ? 0.5 * x : x) : 0.0;
+    }
+  }
+  if (row < Qs) {
+    pair_of_index(row, true, i, j);
+    const double *T = tau + ((long)i * no + j) * v * v;
+    for (long col = blockIdx.x * (long)NT + threadIdx.x; col < ldm; col += (long)gridDim.x * NT) {
+      int e, f;
+      pair_of_index(col < Ps ? col : 0, true, e, f);
+      A[row * ldm + col] = col < Ps ? T[(long)e * v + f] - T[(long)f * v + e] : 0.0;
+    }
+  }
+}
+
+// X[i][j][a][b] += scale (L+[p(ij)][p(ab)] + s_ij s_ab L-[s(ij)][s(ab)]),
+// indices sorted into pairs, s = +1 for i > j (a > b), -1 below, and no
+// L- term on a diagonal
+__global__ void k_ladder_unpack(int no, int nv, const double *__restrict__ LP, const double *__restrict__ LM,
+                                double scale, double *__restrict__ X) {
+  const long v = nv, P = (long)nv * (nv + 1) / 2, Ps = (long)nv * (nv - 1) / 2;
+  const long n = (long)no * no * v * v;
+  for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n; idx += (long)gridDim.x * NT) {
+    const int b = (int)(idx % v), a = (int)((idx / v) % v);
+    const long ij = idx / (v * v);
+    const int j = (int)(ij % no), i = (int)(ij / no);
+    const int hi = i > j ? i : j, lo = i > j ? j : i, ha = a > b ? a : b, la = a > b ? b : a;
+    double x = LP[((long)hi * (hi + 1) / 2 + lo) * P + (long)ha * (ha + 1) / 2 + la];
+    if (i != j && a != b) {
+      const double m = LM[((long)hi * (hi - 1) / 2 + lo) * Ps + (long)ha * (ha - 1) / 2 + la];
+      x += ((i > j) == (a > b)) ? m : -m;
+    }
+    X[idx] += scale * x;
+  }
+}
+
 }  // namespace
 
 long panel_size(int np, long R, int K) { return (long)np * ((R + 15) / 16 * 16) * ((K + 31) / 32 * 32); }
+
+long ladder_ld(long pairs) { return (pairs + 31) / 32 * 32; }
+
+void launch_ladder_vpm(int nv, const double *vvvv, double *vp, double *vm, hipStream_t st) {
+  const long P = (long)nv * (nv + 1) / 2, Ps = (long)nv * (nv - 1) / 2;
+  if (P > 65535) throw std::runtime_error("ugx: packed ladder needs v(v+1)/2 <= 65535");
+  const int gx = (int)std::min<long>((ladder_ld(P) + NT - 1) / NT, 64);
+  hipLaunchKernelGGL(k_ladder_vpm, dim3(gx, (unsigned)P), dim3(NT), 0, st, nv, vvvv, vp, vm, ladder_ld(P),
+                     ladder_ld(Ps));
+}
+
+void launch_ladder_pack(int no, int nv, const double *tau, double *S, double *A, hipStream_t st) {
+  const long P = (long)nv * (nv + 1) / 2, Ps = (long)nv * (nv - 1) / 2, Q = (long)no * (no + 1) / 2;
+  if (Q > 65535) throw std::runtime_error("ugx: packed ladder needs o(o+1)/2 <= 65535");
+  const int gx = (int)std::min<long>((ladder_ld(P) + NT - 1) / NT, 64);
+  hipLaunchKernelGGL(k_ladder_pack, dim3(gx, (unsigned)Q), dim3(NT), 0, st, no, nv, tau, S, A, ladder_ld(P),
+                     ladder_ld(Ps));
+}
+
+void launch_ladder_unpack(int no, int nv, const double *LP, const double *LM, double scale, double *X,
+                          hipStream_t st) {
+  const long n = (long)no * no * nv * nv;
+  hipLaunchKernelGGL(k_ladder_unpack, dim3(nblocks(n)), dim3(NT), 0, st, no, nv, LP, LM, scale, X);
+}
 
 void launch_pack_panel(const double *in, long sp, long sr, long sk, int np, long R, int K, unsigned short *out,
                        hipStream_t st) {
