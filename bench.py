@@ -67,10 +67,18 @@ def ladder_flops(o, v):
     return 2.0 * (q * p * p + qs * ps * ps)
 
 
+def zmbij_flops(o, v):
+    """Zmbij = sum_ef <mb|ef> tau_ij^ef on the same pairs: ~o^3 v^3 (half the
+    full 2 o^3 v^3)."""
+    q, p, qs, ps = o * (o + 1) / 2.0, v * (v + 1) / 2.0, o * (o - 1) / 2.0, v * (v - 1) / 2.0
+    return 2.0 * o * v * (q * p + qs * ps)
+
+
 def ccsd_flops(o, v):
     """Algorithmic flops of one CCSD iteration (2 per multiply-add), with the
-    ladder counted as executed (packed)."""
-    return ladder_flops(o, v) + 14.0 * o**3 * v**3 + 4.0 * o**4 * v**2 + 20.0 * o**2 * v**3
+    ladder and Zmbij counted as executed (packed pairs)."""
+    return (ladder_flops(o, v) + zmbij_flops(o, v) + 12.0 * o**3 * v**3 + 4.0 * o**4 * v**2
+            + 20.0 * o**2 * v**3)
 
 
 def triples_flops(o, v):

@@ -79,6 +79,9 @@ struct ugx_ctx {
   // packed symmetric / antisymmetric ladder (launch_ladder_*): V+, V- once
   // per Hamiltonian; S, A and the two GEMM outputs per residual
   bool lad_packed = false;
+  // Zmbij = sum_ef <mb|ef> tau_ij^ef on the same pairs: <mb|ef> +- <mb|fe>
+  // once per Hamiltonian, the two GEMM outputs per residual
+  DBuf zm_vp, zm_vm, zm_P, zm_M;
   bool lad_full = false;  // UGX_LADDER=full: always the full contraction
   DBuf lad_vp, lad_vm, lad_S, lad_A, lad_LP, lad_LM;
   bool tables_ready = false;
@@ -233,7 +236,25 @@ void residual(ugx_ctx *c, int prec) {
   E.contract("mnef,njfb->mejb", 0.5, LOOVV, T2, 1.0, WB, prec, "Wmbej");
   // Zmbij = sum_ef <mb|ef> tau_ijef
   const auto ZZ = V(c->Z.d(), {o, v, o, o});
-  E.contract("mbef,ijef->mbij", 1.0, OVVV, TAU, 0.0, ZZ, prec, "Zmbij");
+  if (c->lad_packed) {
+    // tau -> the pair matrices S, A (shared with the ladder below); Zmbij on
+    // pairs i>=j / i>j and e>=f / e>f: half the flops
+    const int Q = o * (o + 1) / 2, Qs = o * (o - 1) / 2, P = v * (v + 1) / 2, Ps = v * (v - 1) / 2;
+    const int ldp = (int)ladder_ld(P), ldm = (int)ladder_ld(Ps), ov = o * v;
+    launch_ladder_pack(o, v, tau, c->lad_S.d(), c->lad_A.d(), st);
+    E.contract("ne,qe->nq", 1.0, Cn(c->zm_vp.d(), {ov, ldp}), V(c->lad_S.d(), {Q, ldp}), 0.0,
+               V(c->zm_P.d(), {ov, Q}), prec, "Zmbij");
+    if (Qs > 0) {
+      if (Ps > 0)
+        E.contract("ne,qe->nq", 1.0, Cn(c->zm_vm.d(), {ov, ldm}), V(c->lad_A.d(), {Qs, ldm}), 0.0,
+                   V(c->zm_M.d(), {ov, Qs}), prec, "Zmbij");
+      else
+        launch_fill((long)ov * Qs, 0.0, c->zm_M.d(), st);
+    }
+    launch_pairs_unpack(ov, o, c->zm_P.d(), c->zm_M.d(), c->Z.d(), st);
+  } else {
+    E.contract("mbef,ijef->mbij", 1.0, OVVV, TAU, 0.0, ZZ, prec, "Zmbij");
+  }
 
   // ---- build_t1, ccwfn.cc:432-469
   const auto RR1 = V(c->R1.d(), {o, v});
@@ -264,7 +285,7 @@ void residual(ugx_ctx *c, int prec) {
     // (tau_ij^ef = tau_ji^fe, <ab|ef> = <ba|fe>): a quarter of the flops
     const int Q = o * (o + 1) / 2, Qs = o * (o - 1) / 2, P = v * (v + 1) / 2, Ps = v * (v - 1) / 2;
     const int ldp = (int)ladder_ld(P), ldm = (int)ladder_ld(Ps);  // k padded with zeros
-    launch_ladder_pack(o, v, tau, c->lad_S.d(), c->lad_A.d(), st);
+    // S and A were packed from tau for Zmbij above
     E.contract("qe,ae->qa", 1.0, V(c->lad_S.d(), {Q, ldp}), Cn(c->lad_vp.d(), {P, ldp}), 0.0,
                V(c->lad_LP.d(), {Q, P}), prec, "ladder");
     if (Qs > 0 && Ps > 0)
@@ -1154,6 +1175,11 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
         c->lad_LP.alloc((size_t)Q * P * 8);
         c->lad_LM.alloc((size_t)std::max<long>(1, Qs * Ps) * 8);
         launch_ladder_vpm(nv, c->vvvv.d(), c->lad_vp.d(), c->lad_vm.d(), c->st);
+        c->zm_vp.alloc((size_t)o * v * ladder_ld(P) * 8);
+        c->zm_vm.alloc((size_t)std::max<long>(1, o * v * ladder_ld(Ps)) * 8);
+        c->zm_P.alloc((size_t)o * v * Q * 8);
+        c->zm_M.alloc((size_t)std::max<long>(1, o * v * Qs) * 8);
+        launch_rows_vpm(o * v, nv, c->ovvv.d(), c->zm_vp.d(), c->zm_vm.d(), c->st);
       }
     }
     // L = 2<pq|rs> - <pq|sr> for the blocks the equations read (hamiltonian.cc:96-102)

@@ -432,6 +432,46 @@ __global__ void k_ladder_pack(int no, int nv, const double *__restrict__ tau, do
   }
 }
 
+// Row r of in (a v x v slab at in + r v^2, e.g. <mb|..> of ovvv):
+// Vp[r][p(ef)] = in_r[e][f] + in_r[f][e], Vm[r][s(ef)] = in_r[e][f] - in_r[f][e],
+// rows ldp / ldm elements apart with zeros past the last pair
+__global__ void k_rows_vpm(long nrows, int nv, const double *__restrict__ in, double *__restrict__ vp,
+                           double *__restrict__ vm, long ldp, long ldm) {
+  const long P = (long)nv * (nv + 1) / 2, Ps = (long)nv * (nv - 1) / 2, v = nv;
+  for (long row = blockIdx.y; row < nrows; row += gridDim.y) {
+    const double *R = in + row * v * v;
+    for (long col = blockIdx.x * (long)NT + threadIdx.x; col < ldp; col += (long)gridDim.x * NT) {
+      int e, f;
+      pair_of_index(col < P ? col : 0, false, e, f);
+      vp[row * ldp + col] = col < P ? R[(long)e * v + f] + R[(long)f * v + e] : 0.0;
+    }
+    for (long col = blockIdx.x * (long)NT + threadIdx.x; col < ldm; col += (long)gridDim.x * NT) {
+      int e, f;
+      pair_of_index(col < Ps ? col : 0, true, e, f);
+      vm[row * ldm + col] = col < Ps ? R[(long)e * v + f] - R[(long)f * v + e] : 0.0;
+    }
+  }
+}
+
+// Z[r][i][j] = (Z+[r][p(ij)] + s_ij Z-[r][s(ij)]) / 2 over nrows rows r
+// (no Z- term for i = j)
+__global__ void k_pairs_unpack(long nrows, int no, const double *__restrict__ ZP, const double *__restrict__ ZM,
+                               double *__restrict__ Z) {
+  const long Q = (long)no * (no + 1) / 2, Qs = (long)no * (no - 1) / 2, oo = (long)no * no;
+  const long n = nrows * oo;
+  for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n; idx += (long)gridDim.x * NT) {
+    const long r = idx / oo;
+    const int j = (int)(idx % no), i = (int)((idx / no) % no);
+    const int hi = i > j ? i : j, lo = i > j ? j : i;
+    double x = ZP[r * Q + (long)hi * (hi + 1) / 2 + lo];
+    if (i != j) {
+      const double m = ZM[r * Qs + (long)hi * (hi - 1) / 2 + lo];
+      x += i > j ? m : -m;
+    }
+    Z[idx] = 0.5 * x;
+  }
+}
+
 // X[i][j][a][b] += scale (L+[p(ij)][p(ab)] + s_ij s_ab L-[s(ij)][s(ab)]),
 // indices sorted into pairs, s = +1 for i > j (a > b), -1 below, and no
 // L- term on a diagonal
@@ -473,6 +513,20 @@ void launch_ladder_pack(int no, int nv, const double *tau, double *S, double *A,
   const int gx = (int)std::min<long>((ladder_ld(P) + NT - 1) / NT, 64);
   hipLaunchKernelGGL(k_ladder_pack, dim3(gx, (unsigned)Q), dim3(NT), 0, st, no, nv, tau, S, A, ladder_ld(P),
                      ladder_ld(Ps));
+}
+
+void launch_rows_vpm(long nrows, int nv, const double *in, double *vp, double *vm, hipStream_t st) {
+  const long P = (long)nv * (nv + 1) / 2, Ps = (long)nv * (nv - 1) / 2;
+  const int gx = (int)std::min<long>((ladder_ld(P) + NT - 1) / NT, 64);
+  const unsigned gy = (unsigned)std::min<long>(nrows, 65535);
+  if (nrows <= 0) return;
+  hipLaunchKernelGGL(k_rows_vpm, dim3(gx, gy), dim3(NT), 0, st, nrows, nv, in, vp, vm, ladder_ld(P), ladder_ld(Ps));
+}
+
+void launch_pairs_unpack(long nrows, int no, const double *ZP, const double *ZM, double *Z, hipStream_t st) {
+  const long n = nrows * no * no;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pairs_unpack, dim3(nblocks(n)), dim3(NT), 0, st, nrows, no, ZP, ZM, Z);
 }
 
 void launch_ladder_unpack(int no, int nv, const double *LP, const double *LM, double scale, double *X,
