@@ -74,11 +74,19 @@ def zmbij_flops(o, v):
     return 2.0 * o * v * (q * p + qs * ps)
 
 
+def woooo_tau_flops(o, v):
+    """The tau term of Woooo, sum_ef <mn|ef> tau_ij^ef, on the same pairs:
+    ~o^4 v^2 (half the full 2 o^4 v^2)."""
+    q, p, qs, ps = o * (o + 1) / 2.0, v * (v + 1) / 2.0, o * (o - 1) / 2.0, v * (v - 1) / 2.0
+    return 2.0 * o * o * (q * p + qs * ps)
+
+
 def ccsd_flops(o, v):
     """Algorithmic flops of one CCSD iteration (2 per multiply-add), with the
-    ladder and Zmbij counted as executed (packed pairs)."""
-    return (ladder_flops(o, v) + zmbij_flops(o, v) + 12.0 * o**3 * v**3 + 4.0 * o**4 * v**2
-            + 20.0 * o**2 * v**3)
+    ladder, Zmbij and the tau term of Woooo counted as executed (packed
+    pairs)."""
+    return (ladder_flops(o, v) + zmbij_flops(o, v) + woooo_tau_flops(o, v) + 12.0 * o**3 * v**3
+            + 2.0 * o**4 * v**2 + 20.0 * o**2 * v**3)
 
 
 def triples_flops(o, v):

@@ -82,6 +82,8 @@ struct ugx_ctx {
   // Zmbij = sum_ef <mb|ef> tau_ij^ef on the same pairs: <mb|ef> +- <mb|fe>
   // once per Hamiltonian, the two GEMM outputs per residual
   DBuf zm_vp, zm_vm, zm_P, zm_M;
+  // the tau term of Woooo, sum_ef <mn|ef> tau_ij^ef, likewise (rows (m,n))
+  DBuf wo_vp, wo_vm, wo_P, wo_M;
   bool lad_full = false;  // UGX_LADDER=full: always the full contraction
   DBuf lad_vp, lad_vm, lad_S, lad_A, lad_LP, lad_LM;
   bool tables_ready = false;
@@ -216,7 +218,31 @@ void residual(ugx_ctx *c, int prec) {
   check(hipMemcpyAsync(c->Woooo.p, c->oooo.p, (size_t)o * o * o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
   E.contract("je,mnie->mnij", 1.0, T1, OOOV, 1.0, WO, P64, "Woooo");
   E.contract("ie,nmje->mnij", 1.0, T1, OOOV, 1.0, WO, P64, "Woooo");  // <mn|ej> = <nm|je>
-  E.contract("ijef,mnef->mnij", 1.0, TAU, OOVV, 1.0, WO, prec, "Woooo");
+  // the pair matrices S, A of tau (launch_ladder_pack), shared by the tau
+  // terms of Woooo, Zmbij and the ladder
+  const int Q = o * (o + 1) / 2, Qs = o * (o - 1) / 2, P = v * (v + 1) / 2, Ps = v * (v - 1) / 2;
+  const int ldp = (int)ladder_ld(P), ldm = (int)ladder_ld(Ps);  // k padded with zeros
+  // rows x (e,f) pairs of a constant block times S / A, unpacked into out:
+  // sum_ef in[r][e][f] tau_ij^ef on pairs i>=j / i>j and e>=f / e>f
+  auto pair_term = [&](int rows, DBuf &vp, DBuf &vm, DBuf &zp, DBuf &zm, double beta, double *out,
+                       const char *label) {
+    E.contract("ne,qe->nq", 1.0, Cn(vp.d(), {rows, ldp}), V(c->lad_S.d(), {Q, ldp}), 0.0, V(zp.d(), {rows, Q}),
+               prec, label);
+    if (Qs > 0) {
+      if (Ps > 0)
+        E.contract("ne,qe->nq", 1.0, Cn(vm.d(), {rows, ldm}), V(c->lad_A.d(), {Qs, ldm}), 0.0,
+                   V(zm.d(), {rows, Qs}), prec, label);
+      else
+        launch_fill((long)rows * Qs, 0.0, zm.d(), st);
+    }
+    launch_pairs_unpack(rows, o, zp.d(), zm.d(), beta, out, st);
+  };
+  if (c->lad_packed) {
+    launch_ladder_pack(o, v, tau, c->lad_S.d(), c->lad_A.d(), st);
+    pair_term(o * o, c->wo_vp, c->wo_vm, c->wo_P, c->wo_M, 1.0, c->Woooo.d(), "Woooo");
+  } else {
+    E.contract("ijef,mnef->mnij", 1.0, TAU, OOVV, 1.0, WO, prec, "Woooo");
+  }
 
   // Y[n][f][j][b] = 0.5 t2[j][n][f][b] + t1[j][f] t1[n][b]
   launch_build_Y(o, v, t1, t2, c->Y.d(), st);
@@ -237,21 +263,8 @@ void residual(ugx_ctx *c, int prec) {
   // Zmbij = sum_ef <mb|ef> tau_ijef
   const auto ZZ = V(c->Z.d(), {o, v, o, o});
   if (c->lad_packed) {
-    // tau -> the pair matrices S, A (shared with the ladder below); Zmbij on
-    // pairs i>=j / i>j and e>=f / e>f: half the flops
-    const int Q = o * (o + 1) / 2, Qs = o * (o - 1) / 2, P = v * (v + 1) / 2, Ps = v * (v - 1) / 2;
-    const int ldp = (int)ladder_ld(P), ldm = (int)ladder_ld(Ps), ov = o * v;
-    launch_ladder_pack(o, v, tau, c->lad_S.d(), c->lad_A.d(), st);
-    E.contract("ne,qe->nq", 1.0, Cn(c->zm_vp.d(), {ov, ldp}), V(c->lad_S.d(), {Q, ldp}), 0.0,
-               V(c->zm_P.d(), {ov, Q}), prec, "Zmbij");
-    if (Qs > 0) {
-      if (Ps > 0)
-        E.contract("ne,qe->nq", 1.0, Cn(c->zm_vm.d(), {ov, ldm}), V(c->lad_A.d(), {Qs, ldm}), 0.0,
-                   V(c->zm_M.d(), {ov, Qs}), prec, "Zmbij");
-      else
-        launch_fill((long)ov * Qs, 0.0, c->zm_M.d(), st);
-    }
-    launch_pairs_unpack(ov, o, c->zm_P.d(), c->zm_M.d(), c->Z.d(), st);
+    // Zmbij on pairs: half the flops
+    pair_term(o * v, c->zm_vp, c->zm_vm, c->zm_P, c->zm_M, 0.0, c->Z.d(), "Zmbij");
   } else {
     E.contract("mbef,ijef->mbij", 1.0, OVVV, TAU, 0.0, ZZ, prec, "Zmbij");
   }
@@ -283,9 +296,7 @@ void residual(ugx_ctx *c, int prec) {
   if (c->lad_packed) {
     // sum_ef tau_ij^ef <ab|ef> = (L+ + L-)/2 over the pairs i>=j, a>=b, e>=f
     // (tau_ij^ef = tau_ji^fe, <ab|ef> = <ba|fe>): a quarter of the flops
-    const int Q = o * (o + 1) / 2, Qs = o * (o - 1) / 2, P = v * (v + 1) / 2, Ps = v * (v - 1) / 2;
-    const int ldp = (int)ladder_ld(P), ldm = (int)ladder_ld(Ps);  // k padded with zeros
-    // S and A were packed from tau for Zmbij above
+    // S and A were packed from tau above
     E.contract("qe,ae->qa", 1.0, V(c->lad_S.d(), {Q, ldp}), Cn(c->lad_vp.d(), {P, ldp}), 0.0,
                V(c->lad_LP.d(), {Q, P}), prec, "ladder");
     if (Qs > 0 && Ps > 0)
@@ -1180,6 +1191,11 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
         c->zm_P.alloc((size_t)o * v * Q * 8);
         c->zm_M.alloc((size_t)std::max<long>(1, o * v * Qs) * 8);
         launch_rows_vpm(o * v, nv, c->ovvv.d(), c->zm_vp.d(), c->zm_vm.d(), c->st);
+        c->wo_vp.alloc((size_t)o * o * ladder_ld(P) * 8);
+        c->wo_vm.alloc((size_t)std::max<long>(1, o * o * ladder_ld(Ps)) * 8);
+        c->wo_P.alloc((size_t)o * o * Q * 8);
+        c->wo_M.alloc((size_t)std::max<long>(1, o * o * Qs) * 8);
+        launch_rows_vpm(o * o, nv, c->oovv.d(), c->wo_vp.d(), c->wo_vm.d(), c->st);
       }
     }
     // L = 2<pq|rs> - <pq|sr> for the blocks the equations read (hamiltonian.cc:96-102)

@@ -160,8 +160,10 @@ void launch_ladder_pack(int no, int nv, const double *tau, double *S, double *A,
 // The same pair split of the (e,f) index of nrows slabs in[r][e][f]:
 //   Vp[r][p(ef)] = in[r][e][f] + in[r][f][e], Vm[r][s(ef)] = in[r][e][f] - in[r][f][e]
 void launch_rows_vpm(long nrows, int nv, const double *in, double *vp, double *vm, hipStream_t st);
-//   Z[r][i][j] = (Z+[r][p(ij)] + s_ij Z-[r][s(ij)]) / 2, s_ij = +1 for i > j, -1 below
-void launch_pairs_unpack(long nrows, int no, const double *ZP, const double *ZM, double *Z, hipStream_t st);
+//   Z[r][i][j] = beta Z[r][i][j] + (Z+[r][p(ij)] + s_ij Z-[r][s(ij)]) / 2,
+//   s_ij = +1 for i > j, -1 below; beta 0 or 1
+void launch_pairs_unpack(long nrows, int no, const double *ZP, const double *ZM, double beta, double *Z,
+                         hipStream_t st);
 //   X_ijab += scale (L+[p(ij)][p(ab)] +- L-[s(ij)][s(ab)]), the sign s_ij s_ab
 void launch_ladder_unpack(int no, int nv, const double *LP, const double *LM, double scale, double *X,
                           hipStream_t st);

@@ -453,10 +453,10 @@ __global__ void k_rows_vpm(long nrows, int nv, const double *__restrict__ in, do
   }
 }
 
-// Z[r][i][j] = (Z+[r][p(ij)] + s_ij Z-[r][s(ij)]) / 2 over nrows rows r
-// (no Z- term for i = j)
+// Z[r][i][j] = beta Z[r][i][j] + (Z+[r][p(ij)] + s_ij Z-[r][s(ij)]) / 2 over
+// nrows rows r (no Z- term for i = j; beta 0 or 1)
 __global__ void k_pairs_unpack(long nrows, int no, const double *__restrict__ ZP, const double *__restrict__ ZM,
-                               double *__restrict__ Z) {
+                               double beta, double *__restrict__ Z) {
   const long Q = (long)no * (no + 1) / 2, Qs = (long)no * (no - 1) / 2, oo = (long)no * no;
   const long n = nrows * oo;
   for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n; idx += (long)gridDim.x * NT) {
@@ -468,7 +468,7 @@ __global__ void k_pairs_unpack(long nrows, int no, const double *__restrict__ ZP
       const double m = ZM[r * Qs + (long)hi * (hi - 1) / 2 + lo];
       x += i > j ? m : -m;
     }
-    Z[idx] = 0.5 * x;
+    Z[idx] = beta != 0.0 ? Z[idx] + 0.5 * x : 0.5 * x;
   }
 }
 
@@ -523,10 +523,11 @@ void launch_rows_vpm(long nrows, int nv, const double *in, double *vp, double *v
   hipLaunchKernelGGL(k_rows_vpm, dim3(gx, gy), dim3(NT), 0, st, nrows, nv, in, vp, vm, ladder_ld(P), ladder_ld(Ps));
 }
 
-void launch_pairs_unpack(long nrows, int no, const double *ZP, const double *ZM, double *Z, hipStream_t st) {
+void launch_pairs_unpack(long nrows, int no, const double *ZP, const double *ZM, double beta, double *Z,
+                         hipStream_t st) {
   const long n = nrows * no * no;
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_pairs_unpack, dim3(nblocks(n)), dim3(NT), 0, st, nrows, no, ZP, ZM, Z);
+  hipLaunchKernelGGL(k_pairs_unpack, dim3(nblocks(n)), dim3(NT), 0, st, nrows, no, ZP, ZM, beta, Z);
 }
 
 void launch_ladder_unpack(int no, int nv, const double *LP, const double *LM, double scale, double *X,
