@@ -305,8 +305,11 @@ __device__ inline void store_tile(const GemmDesc &d, void *Cv, const long (&ro)[
 // barrier per slice.
 // TAG only distinguishes instantiations (the (T) W3 launches use TAG = 1,
 // so profiles report them apart from the CCSD contractions).
+// Tiles of up to 4 x 4 blocks per wave are held at two waves per SIMD
+// (the 128 x 128 FP64 tile fits without spills; larger ones would spill).
 template <typename T, int WGM, int WGN, int TM, int TN, int TAG>
-__global__ __launch_bounds__(NT) void k_gemm(GemmDesc d, const GemmPtrs *__restrict__ ptrs) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TM * TN <= 16 ? 2 : 1))) void k_gemm(
+    GemmDesc d, const GemmPtrs *__restrict__ ptrs) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
