@@ -237,6 +237,11 @@ def main():
     ap.add_argument("--guess", default="learned", choices=["learned", "mp2"])
     ap.add_argument("--mixed-conv", type=float, default=1e-7)
     ap.add_argument("--polish", type=float, default=1e-5)
+    ap.add_argument("--polish-mode", default="refine", choices=["refine", "plain"],
+                    help="mixed form after the switch: refinement polish (FP64 residual at the switch point, then "
+                         "low-precision difference residuals) or plain FP64 residuals")
+    ap.add_argument("--no-plain-compare", action="store_true",
+                    help="mixed form: skip the extra timed solve with the plain FP64 polish")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="size-heuristic GEMM tiles instead of the autotuned ones (tuning runs in the warmup)")
@@ -290,13 +295,14 @@ def main():
     ctx.load_hamiltonian(no, nv, fock, blocks)
     ctx.set_autotune(not args.no_autotune)
 
-    def options(prec, conv, guess):
+    def options(prec, conv, guess, polish_mode=ugx.POLISH_PLAIN):
         o = ugx.default_options()
         o.wfn_ccsd_t = 1
         o.maxiter = 100
         o.r_convergence = conv
         o.precision = prec
         o.polish_threshold = args.polish
+        o.polish_mode = polish_mode
         o.guess = guess
         return o
 
@@ -312,7 +318,16 @@ def main():
     out = {}
     if args.form == "mixed":
         wall64, res64, st64 = run_form(ctx, fp64_opts, args.warmup, args.steps, "fp64")
-        mixed_opts = options(ugx.PREC_NAMES[args.low], args.mixed_conv, guess)
+        pmode = ugx.POLISH_REFINE if args.polish_mode == "refine" else ugx.POLISH_PLAIN
+        if pmode == ugx.POLISH_REFINE and not args.no_plain_compare:
+            # the same mixed form with the plain FP64 polish, for comparison
+            wallp, resp, _ = run_form(ctx, options(ugx.PREC_NAMES[args.low], args.mixed_conv, guess), args.warmup,
+                                      args.steps, "mixed, plain FP64 polish")
+            d_p = abs(resp["e_ccsd_final"] + resp["e_t"] - res64["e_ccsd_final"] - res64["e_t"])
+            out.update(plain_polish_wall_s=wallp, plain_polish_iterations=resp["niter"],
+                       plain_polish_seconds_ccsd=resp["seconds_ccsd"], plain_polish_speedup_vs_fp64=wall64 / wallp,
+                       plain_polish_dE_total=d_p)
+        mixed_opts = options(ugx.PREC_NAMES[args.low], args.mixed_conv, guess, pmode)
         wall, res, stats = run_form(ctx, mixed_opts, args.warmup, args.steps, "mixed")
         d_ccsd = abs(res["e_ccsd_final"] - res64["e_ccsd_final"])
         d_t = abs(res["e_t"] - res64["e_t"])
@@ -327,7 +342,8 @@ def main():
         dtype = "%s/f64" % args.low
         config = {"workload": wl["name"], "no": no, "nv": nv, "form": "mixed", "low_precision": args.low,
                   "autotune": not args.no_autotune,
-                  "polish_threshold": args.polish, "r_convergence": args.mixed_conv, "guess": args.guess,
+                  "polish_threshold": args.polish, "polish": args.polish_mode, "r_convergence": args.mixed_conv,
+                  "guess": args.guess,
                   "fp64_r_convergence": FP64_CONV}
     else:
         conv = FP64_CONV

@@ -78,6 +78,16 @@ enum {
  * or the MP2 doubles corrected by the network set with ugx_set_guess_network. */
 enum { UGX_GUESS_MP2 = 0, UGX_GUESS_USER = 1, UGX_GUESS_MLP = 2 };
 
+/* Polish of the mixed forms, after the switch to FP64 accuracy:
+ *  PLAIN:  every residual evaluated in FP64;
+ *  REFINE: one FP64 residual at the switch point T_s, then residuals
+ *          R(T) = R(T_s) + [R(T) - R(T_s)], where each o^2v^4 / o^3v^3
+ *          contraction A.B of the difference is evaluated as
+ *          A.(B - B_s) + (A - A_s).B_s in the low precision and every other
+ *          term in FP64.  The low precision's error then scales with the
+ *          size of T - T_s instead of T, well below R_CONVERGENCE = 1e-7. */
+enum { UGX_POLISH_PLAIN = 0, UGX_POLISH_REFINE = 1 };
+
 /* Energy expression: the reference's (ccwfn.cc:272 assigns instead of
  * accumulating, so only the last (i,a) term of 2 f_ia t1_ia survives; zero
  * for canonical RHF orbitals) or the full sum. */
@@ -109,6 +119,8 @@ typedef struct {
   int graph;               /* 1: replay each iteration's residual kernels from a
                               captured hipGraph (launch-bound small systems);
                               results are bitwise those of eager execution */
+  int polish_mode;         /* mixed forms: UGX_POLISH_PLAIN (default) or
+                              UGX_POLISH_REFINE */
 } ugx_options;
 
 typedef struct {

@@ -209,18 +209,22 @@ def test_autotuned_tiles_give_the_same_results():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", [ugx.PREC_FP64, ugx.PREC_BF16X3])
-def test_graph_replayed_iterations_are_bitwise_eager(prec):
-    # every iteration after the first of each precision replays captured
-    # residual graphs, with the eager update / DIIS / energy work and host
-    # synchronisations between replays; the trajectory must be identical
-    announce("graph vs eager iterations o=5 v=19 prec=%d" % prec)
+@pytest.mark.parametrize("prec,polish", [(ugx.PREC_FP64, ugx.POLISH_PLAIN), (ugx.PREC_BF16X3, ugx.POLISH_PLAIN),
+                                         (ugx.PREC_BF16X3, ugx.POLISH_REFINE)])
+def test_graph_replayed_iterations_are_bitwise_eager(prec, polish):
+    # every iteration after the first of each precision (and polish mode)
+    # replays captured residual graphs, with the eager update / DIIS / energy
+    # work and host synchronisations between replays; the trajectory must be
+    # identical.  The refinement polish's base residual always runs eagerly.
+    announce("graph vs eager iterations o=5 v=19 prec=%d polish=%d" % (prec, polish))
     c, fock, ints = loaded(5, 19, seed=71)
-    ra = c.ccsd(r_convergence=1e-10, wfn_ccsd_t=1, precision=prec)
+    conv = 1e-10 if polish == ugx.POLISH_PLAIN else 1e-9
+    kw = dict(r_convergence=conv, wfn_ccsd_t=1, precision=prec, polish_mode=polish)
+    ra = c.ccsd(**kw)
     t1a, t2a = c.amplitudes()
-    rb = c.ccsd(r_convergence=1e-10, wfn_ccsd_t=1, precision=prec, graph=1)
+    rb = c.ccsd(graph=1, **kw)
     t1b, t2b = c.amplitudes()
-    rc = c.ccsd(r_convergence=1e-10, wfn_ccsd_t=1, precision=prec, graph=1)  # replays the graphs of rb
+    rc = c.ccsd(graph=1, **kw)  # replays the graphs of rb
     t1c, t2c = c.amplitudes()
     assert ra["niter"] == rb["niter"] == rc["niter"] and ra["niter"] > 4
     for r in (rb, rc):

@@ -93,6 +93,31 @@ def test_h2o4_full_size_sampled_parity():
         assert abs(per[t] - o.tcorr_tjl_range(t, t + 1)) < 1e-13, t
 
 
+def test_refinement_polish_reaches_the_fp64_fixed_point():
+    """BF16x3 iterations to ||R|| < 1e-5, then the refinement polish: one
+    FP64 residual at the switch point T_s, then residuals R(T_s) + [R(T) -
+    R(T_s)] whose large contractions are BF16x3 products with a difference
+    from T_s as one operand.  At the converged amplitudes the FP64 residual,
+    evaluated independently by ugx_residual, must be below the threshold; the
+    energy is the FP64 form's and the iteration count the plain FP64
+    polish's (+-1)."""
+    announce("refinement polish (h2o)2")
+    ref = Reference(molecules.water_cluster(2, seed=3), "cc-pvdz")
+    c = context()
+    c.load_hamiltonian(ref.no, ref.nv, ref.fock, ref.blocks)
+    r64 = c.ccsd(maxiter=100, r_convergence=1e-10, wfn_ccsd_t=0)
+    common = dict(maxiter=100, r_convergence=1e-9, wfn_ccsd_t=0, precision=ugx.PREC_BF16X3, polish_threshold=1e-5)
+    rp = c.ccsd(**common)
+    rr = c.ccsd(polish_mode=ugx.POLISH_REFINE, **common)
+    t1, t2 = c.amplitudes()
+    R1, R2 = c.residual(t1, t2, ugx.PREC_FP64)
+    assert r64["converged"] and rp["converged"] and rr["converged"]
+    assert rr["niter_low"] == rp["niter_low"] > 0
+    assert abs(rr["niter"] - rp["niter"]) <= 1, (rr["niter"], rp["niter"])
+    assert np.sqrt((R1 ** 2).sum() + (R2 ** 2).sum()) < 2e-9
+    assert abs(rr["e_ccsd_final"] - r64["e_ccsd_final"]) < 1e-9
+
+
 @pytest.mark.parametrize("form", ["fp32", "bf16x3"])
 def test_mixed_precision_forms_reach_the_fp64_fixed_point(form):
     """Low-precision iterations to ||R|| < 1e-5, then FP64 polish: the same

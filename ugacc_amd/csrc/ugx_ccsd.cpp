@@ -86,6 +86,12 @@ struct ugx_ctx {
   DBuf wo_vp, wo_vm, wo_P, wo_M;
   bool lad_full = false;  // UGX_LADDER=full: always the full contraction
   DBuf lad_vp, lad_vm, lad_S, lad_A, lad_LP, lad_LM;
+  // refinement polish (UGX_POLISH_REFINE): the operands of the large
+  // contractions at the base point (rf_*), each output's sum of its large
+  // contractions there (rs_*), and the differences from the base (rd_*)
+  DBuf rf_t2, rf_tau, rf_ttau, rf_U, rf_Y, rf_S, rf_A, rf_WB, rf_WJ, rf_WO, rf_FaeP, rf_FmiP;
+  DBuf rs_Fvv, rs_Foo, rs_WO, rs_WJ, rs_WB, rs_Z, rs_R1, rs_X;
+  DBuf rd_t2, rd_tau, rd_ttau, rd_U, rd_Y, rd_S, rd_A, rd_WB, rd_WJ, rd_WO, rd_FaeP, rd_FmiP;
   bool tables_ready = false;
   std::vector<int> trip_host;
   // learned guess: network (owned by the caller) and its output f(x_ijab)
@@ -95,7 +101,8 @@ struct ugx_ctx {
   long t_wbytes = 12L << 30;  // (T) W buffer budget, bytes (at most a quarter of the free HBM)
   int t_zgroup = 4;  // consecutive triplets per output tile in the BF16x3 W3 launches
   int t_dma = 2;     // > 1: BF16x3 W3 launches copy panels into LDS, with that many stages (0: registers)
-  // hipGraphs of the residual, keyed by (amplitude buffer parity, precision)
+  // hipGraphs of the residual, keyed by (amplitude buffer parity, precision
+  // + 4 for the refinement polish's difference residuals)
   struct ResidualGraph {
     const void *t1o = nullptr;
     int prec = -1;
@@ -103,7 +110,7 @@ struct ugx_ctx {
     double flops = 0.0;
   };
   std::vector<ResidualGraph> graphs;
-  int graph_eager[4] = {0, 0, 0, 0};  // eager residuals run per precision since the last reset
+  int graph_eager[8] = {};  // eager residuals run per (precision, mode) since the last reset
   // residual of the last ugx_residual, for ugx_intermediates
   bool have_intermediates = false;
   double last_flops = 0.0;
@@ -174,15 +181,47 @@ double t1norm(ugx_ctx *c) {
 
 void build_tau(ugx_ctx *c) { launch_tau(c->no, c->nv, c->t1.d(), c->t2.d(), c->tau.d(), c->ttau.d(), c->st); }
 
+// Residual evaluation modes.  RES_BASE and RES_DELTA are the refinement
+// polish: R(T) = R(T_s) + [R(T) - R(T_s)] around a base point T_s.
+enum { RES_PLAIN = 0, RES_BASE = 1, RES_DELTA = 2 };
+
+void ensure_refine(ugx_ctx *c) {
+  const long o = c->no, v = c->nv, ov = o * v, oovv = ov * ov;
+  for (DBuf *b : {&c->rf_t2, &c->rf_tau, &c->rf_ttau, &c->rf_U, &c->rf_Y, &c->rf_WB, &c->rf_WJ, &c->rs_WJ,
+                  &c->rs_WB, &c->rs_X, &c->rd_t2, &c->rd_tau, &c->rd_ttau, &c->rd_U, &c->rd_Y, &c->rd_WB, &c->rd_WJ})
+    b->alloc(oovv * 8);
+  for (DBuf *b : {&c->rf_FaeP, &c->rs_Fvv, &c->rd_FaeP}) b->alloc(v * v * 8);
+  for (DBuf *b : {&c->rf_FmiP, &c->rs_Foo, &c->rd_FmiP}) b->alloc(o * o * 8);
+  for (DBuf *b : {&c->rf_WO, &c->rs_WO, &c->rd_WO}) b->alloc(o * o * o * o * 8);
+  c->rs_Z.alloc(o * v * o * o * 8);
+  c->rs_R1.alloc(ov * 8);
+  if (c->lad_packed) {
+    for (DBuf *b : {&c->rf_S, &c->rd_S}) b->alloc(c->lad_S.bytes);
+    for (DBuf *b : {&c->rf_A, &c->rd_A}) b->alloc(c->lad_A.bytes);
+  }
+}
+
 // ------------------------------------------------------------ residual
 // Residuals of the amplitudes (t1o, t2o) with tau/ttau built from them:
 // R1 -> c->R1, the T2 half X -> c->X.  prec is the arithmetic of the
-// o^2v^4 / o^3v^3 contractions; the small terms stay FP64.
-void residual(ugx_ctx *c, int prec) {
+// o^2v^4 / o^3v^3 contractions ("large" below); the small terms stay FP64.
+//   RES_PLAIN: every term at the current amplitudes.
+//   RES_BASE:  the same in FP64 (prec must be FP64); it also keeps the base
+//              point: the operands of the large contractions (rf_*) and,
+//              per output, the sum of its large contractions (rs_*).
+//   RES_DELTA: each output's large contractions as its base sum plus the
+//              contraction of the differences from the base point,
+//              A.B - A_s.B_s = A.(B - B_s) + (A - A_s).B_s, in prec; the
+//              small terms are evaluated at the current amplitudes in FP64.
+//              The error of prec then scales with |T - T_s|, not |T|.
+void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   Engine &E = *c->eng;
   const int o = c->no, v = c->nv;
   const hipStream_t st = c->st;
   const int P64 = PREC_FP64;
+  const bool base = mode == RES_BASE, delta = mode == RES_DELTA, plain = mode == RES_PLAIN;
+  if (base && prec != P64) throw UgxError(UGX_BAD_INPUT, "residual: the base point is evaluated in FP64");
+  if (!plain) ensure_refine(c);
   const double *t1 = c->t1o.d(), *t2 = c->t2o.d(), *tau = c->tau.d(), *ttau = c->ttau.d();
   auto V = [](const double *p, std::vector<int> d) { return vref(p, std::move(d)); };
   auto Cn = [](const double *p, std::vector<int> d) { return cref(p, std::move(d)); };
@@ -193,80 +232,164 @@ void residual(ugx_ctx *c, int prec) {
   const auto LOOVV = Cn(c->Loovv.d(), {o, o, v, v}), LOOOV = Cn(c->Looov.d(), {o, o, o, v});
   const auto LOVVV = Cn(c->Lovvv.d(), {o, v, v, v});
   const auto FOV = Cn(c->fov.d(), {o, v});
+  const long n4 = c->oovv_n();
+
+  // base point: keep a copy of an operand; differences from it
+  auto keep = [&](DBuf &dst, const void *src, size_t bytes) {
+    check(hipMemcpyAsync(dst.p, src, bytes, hipMemcpyDeviceToDevice, st), "copy");
+  };
+  auto dif = [&](DBuf &dst, const double *x, const DBuf &xs, long n) { launch_diff(n, x, xs.d(), dst.d(), st); };
+  // adds an output's base sum of large contractions (RES_BASE, RES_DELTA)
+  auto add_sum = [&](const DBuf &sum, double *out, long n) {
+    if (!plain) launch_axpby(n, 1.0, sum.d(), 1.0, out, st);
+  };
+  // one large contraction out += alpha A.B; rs is the output's base sum
+  // (first: the first large term of that output).  dA / dB are the
+  // differences of variable operands (null for constants), Bs the base
+  // value of a variable B.
+  auto big = [&](const char *spec, double alpha, const TRef &A, const TRef *dA, const TRef &B, const TRef *dB,
+                 const TRef *Bs, const TRef &out, const TRef &rs, bool first, const char *label) {
+    if (plain) {
+      E.contract(spec, alpha, A, B, 1.0, out, prec, label);
+    } else if (base) {
+      E.contract(spec, alpha, A, B, first ? 0.0 : 1.0, rs, P64, label);
+    } else {
+      if (dB) E.contract(spec, alpha, A, *dB, 1.0, out, prec, label);
+      if (dA) E.contract(spec, alpha, *dA, Bs ? *Bs : B, 1.0, out, prec, label);
+    }
+  };
 
   // U = 2 t2 - t2(ab swapped), the spin-adapted combination used by build_t1
   // (ccwfn.cc:455,461) and the Wmbej ring term (ccwfn.cc:518-519).
   E.permute("ijab->ijab", 2.0, T2, 0.0, V(c->U.d(), {o, o, v, v}));
   E.permute("ijba->ijab", -1.0, T2, 1.0, V(c->U.d(), {o, o, v, v}));
   const auto UU = V(c->U.d(), {o, o, v, v});
+  if (base) {
+    keep(c->rf_t2, t2, n4 * 8); keep(c->rf_tau, tau, n4 * 8); keep(c->rf_ttau, ttau, n4 * 8);
+    keep(c->rf_U, c->U.p, n4 * 8);
+  } else if (delta) {
+    dif(c->rd_t2, t2, c->rf_t2, n4); dif(c->rd_tau, tau, c->rf_tau, n4); dif(c->rd_ttau, ttau, c->rf_ttau, n4);
+    dif(c->rd_U, c->U.d(), c->rf_U, n4);
+  }
+  const auto dT2 = V(c->rd_t2.d(), {o, o, v, v}), dTAU = V(c->rd_tau.d(), {o, o, v, v});
+  const auto dTTAU = V(c->rd_ttau.d(), {o, o, v, v}), dU = V(c->rd_U.d(), {o, o, v, v});
 
   // ---- build_F, ccwfn.cc:311-356
   const auto FVV = V(c->Fvv.d(), {v, v}), FOO = V(c->Foo.d(), {o, o}), FOVI = V(c->Fov.d(), {o, v});
   check(hipMemcpyAsync(c->Fvv.p, c->fvv.p, (size_t)v * v * 8, hipMemcpyDeviceToDevice, st), "copy");
   E.contract("me,ma->ae", -0.5, FOV, T1, 1.0, FVV, P64, "F");
   E.contract("mf,mafe->ae", 1.0, T1, LOVVV, 1.0, FVV, P64, "F");
-  E.contract("mnaf,mnef->ae", -1.0, TTAU, LOOVV, 1.0, FVV, prec, "F");
+  big("mnaf,mnef->ae", -1.0, TTAU, &dTTAU, LOOVV, nullptr, nullptr, FVV, V(c->rs_Fvv.d(), {v, v}), true, "F");
+  add_sum(c->rs_Fvv, c->Fvv.d(), (long)v * v);
   check(hipMemcpyAsync(c->Foo.p, c->foo.p, (size_t)o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
   E.contract("ie,me->mi", 0.5, T1, FOV, 1.0, FOO, P64, "F");
   E.contract("ne,mnie->mi", 1.0, T1, LOOOV, 1.0, FOO, P64, "F");
-  E.contract("inef,mnef->mi", 1.0, TTAU, LOOVV, 1.0, FOO, prec, "F");
+  big("inef,mnef->mi", 1.0, TTAU, &dTTAU, LOOVV, nullptr, nullptr, FOO, V(c->rs_Foo.d(), {o, o}), true, "F");
+  add_sum(c->rs_Foo, c->Foo.d(), (long)o * o);
   check(hipMemcpyAsync(c->Fov.p, c->fov.p, (size_t)o * v * 8, hipMemcpyDeviceToDevice, st), "copy");
   E.contract("nf,mnef->me", 1.0, T1, LOOVV, 1.0, FOVI, P64, "F");
 
   // ---- build_W, ccwfn.cc:358-430
+  const long n_oooo = (long)o * o * o * o;
   const auto WO = V(c->Woooo.d(), {o, o, o, o});
-  check(hipMemcpyAsync(c->Woooo.p, c->oooo.p, (size_t)o * o * o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
+  check(hipMemcpyAsync(c->Woooo.p, c->oooo.p, (size_t)n_oooo * 8, hipMemcpyDeviceToDevice, st), "copy");
   E.contract("je,mnie->mnij", 1.0, T1, OOOV, 1.0, WO, P64, "Woooo");
   E.contract("ie,nmje->mnij", 1.0, T1, OOOV, 1.0, WO, P64, "Woooo");  // <mn|ej> = <nm|je>
   // the pair matrices S, A of tau (launch_ladder_pack), shared by the tau
   // terms of Woooo, Zmbij and the ladder
   const int Q = o * (o + 1) / 2, Qs = o * (o - 1) / 2, P = v * (v + 1) / 2, Ps = v * (v - 1) / 2;
   const int ldp = (int)ladder_ld(P), ldm = (int)ladder_ld(Ps);  // k padded with zeros
-  // rows x (e,f) pairs of a constant block times S / A, unpacked into out:
-  // sum_ef in[r][e][f] tau_ij^ef on pairs i>=j / i>j and e>=f / e>f
-  auto pair_term = [&](int rows, DBuf &vp, DBuf &vm, DBuf &zp, DBuf &zm, double beta, double *out,
-                       const char *label) {
-    E.contract("ne,qe->nq", 1.0, Cn(vp.d(), {rows, ldp}), V(c->lad_S.d(), {Q, ldp}), 0.0, V(zp.d(), {rows, Q}),
-               prec, label);
+  // rows x (e,f) pairs of a constant block times the pair matrices S / A,
+  // unpacked into out: sum_ef in[r][e][f] tau_ij^ef on pairs i>=j / i>j and
+  // e>=f / e>f
+  auto pair_term = [&](int rows, DBuf &vp, DBuf &vm, DBuf &zp, DBuf &zm, const double *S, const double *A,
+                       double beta, double *out, int pr, const char *label) {
+    E.contract("ne,qe->nq", 1.0, Cn(vp.d(), {rows, ldp}), V(S, {Q, ldp}), 0.0, V(zp.d(), {rows, Q}), pr, label);
     if (Qs > 0) {
       if (Ps > 0)
-        E.contract("ne,qe->nq", 1.0, Cn(vm.d(), {rows, ldm}), V(c->lad_A.d(), {Qs, ldm}), 0.0,
-                   V(zm.d(), {rows, Qs}), prec, label);
+        E.contract("ne,qe->nq", 1.0, Cn(vm.d(), {rows, ldm}), V(A, {Qs, ldm}), 0.0, V(zm.d(), {rows, Qs}), pr,
+                   label);
       else
         launch_fill((long)rows * Qs, 0.0, zm.d(), st);
     }
     launch_pairs_unpack(rows, o, zp.d(), zm.d(), beta, out, st);
   };
+  // the pair matrices this evaluation contracts: S, A (plain, base) or
+  // their differences from the base point (delta)
+  const double *pS = c->lad_S.d(), *pA = c->lad_A.d();
   if (c->lad_packed) {
     launch_ladder_pack(o, v, tau, c->lad_S.d(), c->lad_A.d(), st);
-    pair_term(o * o, c->wo_vp, c->wo_vm, c->wo_P, c->wo_M, 1.0, c->Woooo.d(), "Woooo");
+    if (base) {
+      keep(c->rf_S, c->lad_S.p, c->lad_S.bytes); keep(c->rf_A, c->lad_A.p, c->lad_A.bytes);
+    } else if (delta) {
+      dif(c->rd_S, c->lad_S.d(), c->rf_S, (long)(c->lad_S.bytes / 8));
+      dif(c->rd_A, c->lad_A.d(), c->rf_A, (long)(c->lad_A.bytes / 8));
+      pS = c->rd_S.d(); pA = c->rd_A.d();
+    }
+    if (plain)
+      pair_term(o * o, c->wo_vp, c->wo_vm, c->wo_P, c->wo_M, pS, pA, 1.0, c->Woooo.d(), prec, "Woooo");
+    else if (base)
+      pair_term(o * o, c->wo_vp, c->wo_vm, c->wo_P, c->wo_M, pS, pA, 0.0, c->rs_WO.d(), P64, "Woooo");
+    else
+      pair_term(o * o, c->wo_vp, c->wo_vm, c->wo_P, c->wo_M, pS, pA, 1.0, c->Woooo.d(), prec, "Woooo");
   } else {
-    E.contract("ijef,mnef->mnij", 1.0, TAU, OOVV, 1.0, WO, prec, "Woooo");
+    big("ijef,mnef->mnij", 1.0, TAU, &dTAU, OOVV, nullptr, nullptr, WO, V(c->rs_WO.d(), {o, o, o, o}), true,
+        "Woooo");
   }
+  add_sum(c->rs_WO, c->Woooo.d(), n_oooo);
+  if (base) keep(c->rf_WO, c->Woooo.p, n_oooo * 8);
+  else if (delta) dif(c->rd_WO, c->Woooo.d(), c->rf_WO, n_oooo);
 
   // Y[n][f][j][b] = 0.5 t2[j][n][f][b] + t1[j][f] t1[n][b]
   launch_build_Y(o, v, t1, t2, c->Y.d(), st);
   const auto YY = V(c->Y.d(), {o, v, o, v});
+  if (base) keep(c->rf_Y, c->Y.p, n4 * 8);
+  else if (delta) dif(c->rd_Y, c->Y.d(), c->rf_Y, n4);
+  const auto dY = V(c->rd_Y.d(), {o, v, o, v});
   // Wmbje stored as WJ[m][e][j][b]
   const auto WJ = V(c->WJ.d(), {o, v, o, v});
   E.permute("mbje->mejb", -1.0, OVOV, 0.0, WJ);
   E.contract("jf,mbfe->mejb", -1.0, T1, OVVV, 1.0, WJ, P64, "Wmbje");
   E.contract("nb,mnje->mejb", 1.0, T1, OOOV, 1.0, WJ, P64, "Wmbje");
-  E.contract("mnfe,nfjb->mejb", 1.0, OOVV, YY, 1.0, WJ, prec, "Wmbje");
+  const auto RSJ = V(c->rs_WJ.d(), {o, v, o, v});
+  big("mnfe,nfjb->mejb", 1.0, OOVV, nullptr, YY, &dY, nullptr, WJ, RSJ, true, "Wmbje");
+  add_sum(c->rs_WJ, c->WJ.d(), n4);
   // Wmbej stored as WB[m][e][j][b];  <mb|ej> = <mj|eb>
   const auto WB = V(c->WB.d(), {o, v, o, v});
   E.permute("mjeb->mejb", 1.0, OOVV, 0.0, WB);
   E.contract("jf,mbef->mejb", 1.0, T1, OVVV, 1.0, WB, P64, "Wmbej");
   E.contract("nb,nmje->mejb", -1.0, T1, OOOV, 1.0, WB, P64, "Wmbej");
-  E.contract("mnef,nfjb->mejb", -1.0, OOVV, YY, 1.0, WB, prec, "Wmbej");
-  E.contract("mnef,njfb->mejb", 0.5, LOOVV, T2, 1.0, WB, prec, "Wmbej");
+  const auto RSB = V(c->rs_WB.d(), {o, v, o, v});
+  big("mnef,nfjb->mejb", -1.0, OOVV, nullptr, YY, &dY, nullptr, WB, RSB, true, "Wmbej");
+  big("mnef,njfb->mejb", 0.5, LOOVV, nullptr, T2, &dT2, nullptr, WB, RSB, false, "Wmbej");
+  add_sum(c->rs_WB, c->WB.d(), n4);
+  if (base) {
+    keep(c->rf_WJ, c->WJ.p, n4 * 8); keep(c->rf_WB, c->WB.p, n4 * 8);
+  } else if (delta) {
+    dif(c->rd_WJ, c->WJ.d(), c->rf_WJ, n4); dif(c->rd_WB, c->WB.d(), c->rf_WB, n4);
+  }
   // Zmbij = sum_ef <mb|ef> tau_ijef
+  const long n_ovoo = (long)o * v * o * o;
   const auto ZZ = V(c->Z.d(), {o, v, o, o});
   if (c->lad_packed) {
     // Zmbij on pairs: half the flops
-    pair_term(o * v, c->zm_vp, c->zm_vm, c->zm_P, c->zm_M, 0.0, c->Z.d(), "Zmbij");
+    if (base) {
+      pair_term(o * v, c->zm_vp, c->zm_vm, c->zm_P, c->zm_M, pS, pA, 0.0, c->rs_Z.d(), P64, "Zmbij");
+      launch_axpby(n_ovoo, 1.0, c->rs_Z.d(), 0.0, c->Z.d(), st);
+    } else {
+      pair_term(o * v, c->zm_vp, c->zm_vm, c->zm_P, c->zm_M, pS, pA, 0.0, c->Z.d(), prec, "Zmbij");
+      if (delta) add_sum(c->rs_Z, c->Z.d(), n_ovoo);
+    }
   } else {
-    E.contract("mbef,ijef->mbij", 1.0, OVVV, TAU, 0.0, ZZ, prec, "Zmbij");
+    if (plain) {
+      E.contract("mbef,ijef->mbij", 1.0, OVVV, TAU, 0.0, ZZ, prec, "Zmbij");
+    } else if (base) {
+      E.contract("mbef,ijef->mbij", 1.0, OVVV, TAU, 0.0, V(c->rs_Z.d(), {o, v, o, o}), P64, "Zmbij");
+      launch_axpby(n_ovoo, 1.0, c->rs_Z.d(), 0.0, c->Z.d(), st);
+    } else {
+      E.contract("mbef,ijef->mbij", 1.0, OVVV, dTAU, 0.0, ZZ, prec, "Zmbij");
+      add_sum(c->rs_Z, c->Z.d(), n_ovoo);
+    }
   }
 
   // ---- build_t1, ccwfn.cc:432-469
@@ -277,12 +400,14 @@ void residual(ugx_ctx *c, int prec) {
   E.contract("imae,me->ia", 1.0, UU, FOVI, 1.0, RR1, P64, "t1");
   E.contract("nf,nifa->ia", 2.0, T1, OOVV, 1.0, RR1, P64, "t1");   // 2<na|fi> = 2<ni|fa>
   E.contract("nf,naif->ia", -1.0, T1, OVOV, 1.0, RR1, P64, "t1");  // -<na|if>
-  E.contract("mief,maef->ia", 1.0, UU, OVVV, 1.0, RR1, prec, "t1");
+  big("mief,maef->ia", 1.0, UU, &dU, OVVV, nullptr, nullptr, RR1, V(c->rs_R1.d(), {o, v}), true, "t1");
+  add_sum(c->rs_R1, c->R1.d(), (long)o * v);
   E.contract("mnae,mnie->ia", -2.0, T2, OOOV, 1.0, RR1, P64, "t1");  // L<nm|ei> = 2<mn|ie> - <nm|ie>
   E.contract("mnae,nmie->ia", 1.0, T2, OOOV, 1.0, RR1, P64, "t1");
 
   // ---- build_t2 (half X), ccwfn.cc:471-539
   const auto XX = V(c->X.d(), {o, o, v, v});
+  const auto RSX = V(c->rs_X.d(), {o, o, v, v});
   E.permute("ijab->ijab", 0.5, OOVV, 0.0, XX);
   // Fae - 0.5 t1.Fme and Fmi + 0.5 t1.Fme fold the t1*Fme terms (:489-491, :496-498)
   const auto FAEP = V(c->FaeP.d(), {v, v}), FMIP = V(c->FmiP.d(), {o, o});
@@ -290,21 +415,30 @@ void residual(ugx_ctx *c, int prec) {
   E.contract("mb,me->be", -0.5, T1, FOVI, 1.0, FAEP, P64, "t2");
   check(hipMemcpyAsync(c->FmiP.p, c->Foo.p, (size_t)o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
   E.contract("je,me->mj", 0.5, T1, FOVI, 1.0, FMIP, P64, "t2");
-  E.contract("ijae,be->ijab", 1.0, T2, FAEP, 1.0, XX, prec, "t2_F");
-  E.contract("imab,mj->ijab", -1.0, T2, FMIP, 1.0, XX, prec, "t2_F");
-  E.contract("mnab,mnij->ijab", 0.5, TAU, WO, 1.0, XX, prec, "t2_Wmnij");
+  if (base) {
+    keep(c->rf_FaeP, c->FaeP.p, (size_t)v * v * 8); keep(c->rf_FmiP, c->FmiP.p, (size_t)o * o * 8);
+  } else if (delta) {
+    dif(c->rd_FaeP, c->FaeP.d(), c->rf_FaeP, (long)v * v); dif(c->rd_FmiP, c->FmiP.d(), c->rf_FmiP, (long)o * o);
+  }
+  const auto dFAEP = V(c->rd_FaeP.d(), {v, v}), sFAEP = V(c->rf_FaeP.d(), {v, v});
+  const auto dFMIP = V(c->rd_FmiP.d(), {o, o}), sFMIP = V(c->rf_FmiP.d(), {o, o});
+  const auto dWO = V(c->rd_WO.d(), {o, o, o, o}), sWO = V(c->rf_WO.d(), {o, o, o, o});
+  big("ijae,be->ijab", 1.0, T2, &dT2, FAEP, &dFAEP, &sFAEP, XX, RSX, true, "t2_F");
+  big("imab,mj->ijab", -1.0, T2, &dT2, FMIP, &dFMIP, &sFMIP, XX, RSX, false, "t2_F");
+  big("mnab,mnij->ijab", 0.5, TAU, &dTAU, WO, &dWO, &sWO, XX, RSX, false, "t2_Wmnij");
   if (c->lad_packed) {
     // sum_ef tau_ij^ef <ab|ef> = (L+ + L-)/2 over the pairs i>=j, a>=b, e>=f
     // (tau_ij^ef = tau_ji^fe, <ab|ef> = <ba|fe>): a quarter of the flops
-    // S and A were packed from tau above
-    E.contract("qe,ae->qa", 1.0, V(c->lad_S.d(), {Q, ldp}), Cn(c->lad_vp.d(), {P, ldp}), 0.0,
-               V(c->lad_LP.d(), {Q, P}), prec, "ladder");
+    // S and A (or their differences) were packed from tau above
+    const int pr = base ? P64 : prec;
+    E.contract("qe,ae->qa", 1.0, V(pS, {Q, ldp}), Cn(c->lad_vp.d(), {P, ldp}), 0.0, V(c->lad_LP.d(), {Q, P}), pr,
+               "ladder");
     if (Qs > 0 && Ps > 0)
-      E.contract("qe,ae->qa", 1.0, V(c->lad_A.d(), {Qs, ldm}), Cn(c->lad_vm.d(), {Ps, ldm}), 0.0,
-                 V(c->lad_LM.d(), {Qs, Ps}), prec, "ladder");
-    launch_ladder_unpack(o, v, c->lad_LP.d(), c->lad_LM.d(), 0.25, c->X.d(), st);
+      E.contract("qe,ae->qa", 1.0, V(pA, {Qs, ldm}), Cn(c->lad_vm.d(), {Ps, ldm}), 0.0,
+                 V(c->lad_LM.d(), {Qs, Ps}), pr, "ladder");
+    launch_ladder_unpack(o, v, c->lad_LP.d(), c->lad_LM.d(), 0.25, base ? c->rs_X.d() : c->X.d(), st);
   } else {
-    E.contract("ijef,abef->ijab", 0.5, TAU, VVVV, 1.0, XX, prec, "ladder");
+    big("ijef,abef->ijab", 0.5, TAU, &dTAU, VVVV, nullptr, nullptr, XX, RSX, false, "ladder");
   }
   E.contract("ie,jeba->ijab", 1.0, T1, OVVV, 1.0, XX, P64, "t2_t1");  // <ab|ej> = <je|ba>
   // Z' = Zmbij + sum_e t1_ie <mb|ej> + <mb|ij>  (the t1 t1 <mb|ej> and
@@ -319,9 +453,12 @@ void residual(ugx_ctx *c, int prec) {
   E.contract("ie,maje->maji", 1.0, T1, OVOV, 0.0, PP2, P64, "t2_t1");
   E.contract("mb,maji->ijab", -1.0, T1, PP2, 1.0, XX, P64, "t2_t1");
   // ring terms, ccwfn.cc:516-526
-  E.contract("imae,mejb->ijab", 1.0, UU, WB, 1.0, XX, prec, "ring");
-  E.contract("imae,mejb->ijab", 1.0, T2, WJ, 1.0, XX, prec, "ring");
-  E.contract("mjae,meib->ijab", 1.0, T2, WJ, 1.0, XX, prec, "ring");
+  const auto dWB = V(c->rd_WB.d(), {o, v, o, v}), sWB = V(c->rf_WB.d(), {o, v, o, v});
+  const auto dWJ = V(c->rd_WJ.d(), {o, v, o, v}), sWJ = V(c->rf_WJ.d(), {o, v, o, v});
+  big("imae,mejb->ijab", 1.0, UU, &dU, WB, &dWB, &sWB, XX, RSX, false, "ring");
+  big("imae,mejb->ijab", 1.0, T2, &dT2, WJ, &dWJ, &sWJ, XX, RSX, false, "ring");
+  big("mjae,meib->ijab", 1.0, T2, &dT2, WJ, &dWJ, &sWJ, XX, RSX, false, "ring");
+  add_sum(c->rs_X, c->X.d(), n4);
   c->have_intermediates = true;
 }
 
@@ -339,14 +476,17 @@ void drop_graphs(ugx_ctx *c) {
 // captured into a hipGraph, which later iterations replay.  The captured
 // work is the same kernels with the same arguments, so results are
 // bitwise identical to eager execution.
-void residual_step(ugx_ctx *c, int prec, bool use_graph) {
-  if (!use_graph || c->graph_eager[prec] == 0) {
-    residual(c, prec);
-    c->graph_eager[prec]++;
+// The base point of the refinement polish always runs eagerly; graphs of the
+// other modes are keyed by (amplitude buffer, precision, mode).
+void residual_step(ugx_ctx *c, int prec, bool use_graph, int mode = RES_PLAIN) {
+  const int key = prec + 4 * (mode == RES_DELTA);
+  if (!use_graph || mode == RES_BASE || c->graph_eager[key] == 0) {
+    residual(c, prec, mode);
+    if (mode != RES_BASE) c->graph_eager[key]++;
     return;
   }
   for (auto &g : c->graphs)
-    if (g.t1o == c->t1o.p && g.prec == prec) {
+    if (g.t1o == c->t1o.p && g.prec == key) {
       check(hipGraphLaunch(g.exec, c->st), "graph launch");
       c->eng->add_flops(g.flops);
       return;
@@ -358,7 +498,7 @@ void residual_step(ugx_ctx *c, int prec, bool use_graph) {
   hipGraph_t graph = nullptr;
   check(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal), "begin capture");
   try {
-    residual(c, prec);
+    residual(c, prec, mode);
   } catch (...) {
     (void)hipStreamEndCapture(c->st, &graph);
     if (graph) (void)hipGraphDestroy(graph);
@@ -369,7 +509,7 @@ void residual_step(ugx_ctx *c, int prec, bool use_graph) {
   E.set_profiling(prof);
   ugx_ctx::ResidualGraph g;
   g.t1o = c->t1o.p;
-  g.prec = prec;
+  g.prec = key;
   g.flops = E.flops_total() - f0;
   hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
@@ -1068,6 +1208,7 @@ void ugx_default_options(ugx_options *opt) {
   opt->print = 0;
   opt->polish_keep_diis = 1;
   opt->graph = 0;
+  opt->polish_mode = UGX_POLISH_PLAIN;
 }
 
 int ugx_create(ugx_ctx **out, int device) {
@@ -1255,7 +1396,8 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
     ugx_options opt;
     if (opt_in) opt = *opt_in; else ugx_default_options(&opt);
     if (opt.maxiter < 0 || opt.diis_nvec < 1 || opt.diis_nvec > 8 || opt.diis_min_vecs < 1 ||
-        opt.precision < 0 || opt.precision > 2 || !(opt.r_convergence > 0))
+        opt.precision < 0 || opt.precision > 2 || !(opt.r_convergence > 0) ||
+        (opt.polish_mode != UGX_POLISH_PLAIN && opt.polish_mode != UGX_POLISH_REFINE))
       throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: invalid options");
     if (opt.guess < UGX_GUESS_MP2 || opt.guess > UGX_GUESS_MLP)
       throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: unknown guess");
@@ -1280,14 +1422,19 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
     c->diis_err.clear(); c->diis_amp.clear(); c->diis_nvec = 0;
     int iter, niter = 0, nlow = 0, low_stall = 0;
     double low_best = 1e300;
+    // mode of the next residual; polished: the residuals are FP64-accurate
+    // (FP64, or the refinement polish's difference form)
+    int mode = RES_PLAIN;
+    bool polished = prec == PREC_FP64;
     for (iter = 1; iter <= opt.maxiter; iter++) {
       amp_save(c);
-      residual_step(c, prec, opt.graph != 0);
+      residual_step(c, mode == RES_BASE ? (int)PREC_FP64 : prec, opt.graph != 0, mode);
+      if (mode == RES_BASE) mode = RES_DELTA;
       rms = increment(c, nullptr);
       niter = iter;
-      if (prec != PREC_FP64) nlow++;
+      if (!polished) nlow++;
       if (!std::isfinite(rms)) throw UgxError(UGX_NONFINITE, "non-finite residual norm");
-      if (rms < opt.r_convergence && prec == PREC_FP64) {
+      if (rms < opt.r_convergence && polished) {
         // the reported energy is that of the amplitudes this iteration
         // started from (ccwfn.cc:216, 223-224): t1_old and the tau built
         // from them, both still in place
@@ -1296,15 +1443,19 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
       }
       // low precision has a residual floor (BF16x3: ~1e-6): switch to the
       // FP64 polish when the threshold is reached or the residual stalls
-      if (prec != PREC_FP64) {
+      if (!polished) {
         if (rms < 0.8 * low_best) { low_best = rms; low_stall = 0; }
         else low_stall++;
       }
-      if (prec != PREC_FP64 && (rms < opt.polish_threshold || low_stall >= 3)) {
-        // polish: the remaining iterations run in FP64; the subspace built
-        // from low-precision residuals is kept (its error vectors are
-        // accurate to the low precision) unless asked otherwise
-        prec = PREC_FP64;
+      if (!polished && (rms < opt.polish_threshold || low_stall >= 3)) {
+        // polish: the remaining residuals are FP64-accurate, evaluated in
+        // FP64 or (refinement) as the FP64 residual of the next amplitudes
+        // plus low-precision differences from them; the subspace built from
+        // low-precision residuals is kept (its error vectors are accurate to
+        // the low precision) unless asked otherwise
+        polished = true;
+        if (opt.polish_mode == UGX_POLISH_REFINE) mode = RES_BASE;
+        else prec = PREC_FP64;
         if (!opt.polish_keep_diis) diis_reset(c);
       }
       if (opt.diis) diis_step(c, opt.diis_nvec, opt.diis_policy, opt.diis_min_vecs);

@@ -173,6 +173,10 @@ __global__ void k_axpby(long n, double a, const double *x, double b, double *y) 
     y[i] = a * x[i] + (b != 0.0 ? b * y[i] : 0.0);
 }
 
+__global__ void k_diff(long n, const double *__restrict__ x, const double *__restrict__ y, double *__restrict__ z) {
+  for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) z[i] = x[i] - y[i];
+}
+
 __global__ void k_fill(long n, double v, double *y) {
   for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) y[i] = v;
 }
@@ -581,6 +585,11 @@ void launch_fill(long n, double v, double *y, hipStream_t st) {
 void launch_axpby(long n, double a, const double *x, double b, double *y, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(NT), 0, st, n, a, x, b, y);
+}
+
+void launch_diff(long n, const double *x, const double *y, double *z, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_diff, dim3(nblocks(n)), dim3(NT), 0, st, n, x, y, z);
 }
 
 void launch_tau(int no, int nv, const double *t1, const double *t2, double *tau, double *ttau, hipStream_t st) {

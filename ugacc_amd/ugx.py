@@ -16,6 +16,7 @@ LIBPATH = os.path.join(HERE, "libugx.so")
 OK, NOT_CONVERGED, BAD_INPUT, OOM, HIP_ERROR, NO_DEVICE, NOT_READY, NONFINITE = range(8)
 PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X3_PAIR, PREC_BF16 = 0, 1, 2, 3, 4
 GUESS_MP2, GUESS_USER, GUESS_MLP = 0, 1, 2
+POLISH_PLAIN, POLISH_REFINE = 0, 1
 MLP_NFEAT, MLP_WIDTH = 8, 128
 ENERGY_REFERENCE, ENERGY_FULL = 0, 1
 PREC_NAMES = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
@@ -44,6 +45,7 @@ class Options(ctypes.Structure):
         ("diis_min_vecs", ctypes.c_int), ("diis_policy", ctypes.c_int), ("precision", ctypes.c_int),
         ("polish_threshold", ctypes.c_double), ("guess", ctypes.c_int), ("energy_form", ctypes.c_int),
         ("print", ctypes.c_int), ("polish_keep_diis", ctypes.c_int), ("graph", ctypes.c_int),
+        ("polish_mode", ctypes.c_int),
     ]
 
 
