@@ -103,7 +103,7 @@ typedef struct {
   double r_convergence;    /* R_CONVERGENCE, default 1e-7 (ugacc.cc:65) */
   int ooc;                 /* OOC, default 0 (ugacc.cc:67); reported only, the
                               (T) value is the same for both settings */
-  int diis_nvec;           /* subspace size, 8 (ccwfn.cc:205) */
+  int diis_nvec;           /* subspace size, 8 (ccwfn.cc:205); at most 16 */
   int diis_min_vecs;       /* extrapolate once this many vectors are stored;
                               3 = "subspace_size() > 2" (ccwfn.cc:220) */
   int diis_policy;         /* 0: replace the oldest vector once full;

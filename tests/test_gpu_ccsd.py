@@ -235,6 +235,22 @@ def test_graph_replayed_iterations_are_bitwise_eager(prec, polish):
 
 
 @pytest.mark.gpu
+def test_larger_diis_subspaces_reach_the_same_fixed_point():
+    # subspaces above the reference's 8 vectors (up to 16): the DIIS dots
+    # then write 512 partials for each of up to 16 stored vectors
+    announce("diis subspace 8 / 12 / 16 o=5 v=19")
+    c, fock, ints = loaded(5, 19, seed=71)
+    e = {}
+    for n in (8, 12, 16):
+        r = c.ccsd(r_convergence=1e-10, diis_nvec=n)
+        assert r["converged"], n
+        e[n] = r["e_ccsd_final"]
+    assert abs(e[12] - e[8]) < 1e-10 and abs(e[16] - e[8]) < 1e-10
+    with pytest.raises(ugx.UgxError):
+        c.ccsd(diis_nvec=17)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("prec", [ugx.PREC_FP64, ugx.PREC_BF16X3])
 def test_batch_matches_single_solves(prec):
     # uneven sizes in one batch; each problem's result must be the one its

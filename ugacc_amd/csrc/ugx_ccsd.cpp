@@ -204,8 +204,8 @@ void ensure_refine(ugx_ctx *c) {
 // ------------------------------------------------------------ residual
 // Residuals of the amplitudes (t1o, t2o) with tau/ttau built from them:
 // R1 -> c->R1, the T2 half X -> c->X.  prec is the arithmetic of the
-// o^2v^4 / o^3v^3 contractions ("large" below); the small terms stay FP64.
-//   RES_PLAIN: every term at the current amplitudes.
+// contractions; "large" below are the o^2v^4 / o^3v^3 ones.
+//   RES_PLAIN: every term at the current amplitudes, in prec.
 //   RES_BASE:  the same in FP64 (prec must be FP64); it also keeps the base
 //              point: the operands of the large contractions (rf_*) and,
 //              per output, the sum of its large contractions (rs_*).
@@ -222,6 +222,8 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   const bool base = mode == RES_BASE, delta = mode == RES_DELTA, plain = mode == RES_PLAIN;
   if (base && prec != P64) throw UgxError(UGX_BAD_INPUT, "residual: the base point is evaluated in FP64");
   if (!plain) ensure_refine(c);
+  // the small (t1) terms: prec in a plain evaluation, FP64 in the polish
+  const int SP = plain ? prec : P64;
   const double *t1 = c->t1o.d(), *t2 = c->t2o.d(), *tau = c->tau.d(), *ttau = c->ttau.d();
   auto V = [](const double *p, std::vector<int> d) { return vref(p, std::move(d)); };
   auto Cn = [](const double *p, std::vector<int> d) { return cref(p, std::move(d)); };
@@ -277,24 +279,24 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   // ---- build_F, ccwfn.cc:311-356
   const auto FVV = V(c->Fvv.d(), {v, v}), FOO = V(c->Foo.d(), {o, o}), FOVI = V(c->Fov.d(), {o, v});
   check(hipMemcpyAsync(c->Fvv.p, c->fvv.p, (size_t)v * v * 8, hipMemcpyDeviceToDevice, st), "copy");
-  E.contract("me,ma->ae", -0.5, FOV, T1, 1.0, FVV, P64, "F");
-  E.contract("mf,mafe->ae", 1.0, T1, LOVVV, 1.0, FVV, P64, "F");
+  E.contract("me,ma->ae", -0.5, FOV, T1, 1.0, FVV, SP, "F");
+  E.contract("mf,mafe->ae", 1.0, T1, LOVVV, 1.0, FVV, SP, "F");
   big("mnaf,mnef->ae", -1.0, TTAU, &dTTAU, LOOVV, nullptr, nullptr, FVV, V(c->rs_Fvv.d(), {v, v}), true, "F");
   add_sum(c->rs_Fvv, c->Fvv.d(), (long)v * v);
   check(hipMemcpyAsync(c->Foo.p, c->foo.p, (size_t)o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
-  E.contract("ie,me->mi", 0.5, T1, FOV, 1.0, FOO, P64, "F");
-  E.contract("ne,mnie->mi", 1.0, T1, LOOOV, 1.0, FOO, P64, "F");
+  E.contract("ie,me->mi", 0.5, T1, FOV, 1.0, FOO, SP, "F");
+  E.contract("ne,mnie->mi", 1.0, T1, LOOOV, 1.0, FOO, SP, "F");
   big("inef,mnef->mi", 1.0, TTAU, &dTTAU, LOOVV, nullptr, nullptr, FOO, V(c->rs_Foo.d(), {o, o}), true, "F");
   add_sum(c->rs_Foo, c->Foo.d(), (long)o * o);
   check(hipMemcpyAsync(c->Fov.p, c->fov.p, (size_t)o * v * 8, hipMemcpyDeviceToDevice, st), "copy");
-  E.contract("nf,mnef->me", 1.0, T1, LOOVV, 1.0, FOVI, P64, "F");
+  E.contract("nf,mnef->me", 1.0, T1, LOOVV, 1.0, FOVI, SP, "F");
 
   // ---- build_W, ccwfn.cc:358-430
   const long n_oooo = (long)o * o * o * o;
   const auto WO = V(c->Woooo.d(), {o, o, o, o});
   check(hipMemcpyAsync(c->Woooo.p, c->oooo.p, (size_t)n_oooo * 8, hipMemcpyDeviceToDevice, st), "copy");
-  E.contract("je,mnie->mnij", 1.0, T1, OOOV, 1.0, WO, P64, "Woooo");
-  E.contract("ie,nmje->mnij", 1.0, T1, OOOV, 1.0, WO, P64, "Woooo");  // <mn|ej> = <nm|je>
+  E.contract("je,mnie->mnij", 1.0, T1, OOOV, 1.0, WO, SP, "Woooo");
+  E.contract("ie,nmje->mnij", 1.0, T1, OOOV, 1.0, WO, SP, "Woooo");  // <mn|ej> = <nm|je>
   // the pair matrices S, A of tau (launch_ladder_pack), shared by the tau
   // terms of Woooo, Zmbij and the ladder
   const int Q = o * (o + 1) / 2, Qs = o * (o - 1) / 2, P = v * (v + 1) / 2, Ps = v * (v - 1) / 2;
@@ -349,16 +351,16 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   // Wmbje stored as WJ[m][e][j][b]
   const auto WJ = V(c->WJ.d(), {o, v, o, v});
   E.permute("mbje->mejb", -1.0, OVOV, 0.0, WJ);
-  E.contract("jf,mbfe->mejb", -1.0, T1, OVVV, 1.0, WJ, P64, "Wmbje");
-  E.contract("nb,mnje->mejb", 1.0, T1, OOOV, 1.0, WJ, P64, "Wmbje");
+  E.contract("jf,mbfe->mejb", -1.0, T1, OVVV, 1.0, WJ, SP, "Wmbje");
+  E.contract("nb,mnje->mejb", 1.0, T1, OOOV, 1.0, WJ, SP, "Wmbje");
   const auto RSJ = V(c->rs_WJ.d(), {o, v, o, v});
   big("mnfe,nfjb->mejb", 1.0, OOVV, nullptr, YY, &dY, nullptr, WJ, RSJ, true, "Wmbje");
   add_sum(c->rs_WJ, c->WJ.d(), n4);
   // Wmbej stored as WB[m][e][j][b];  <mb|ej> = <mj|eb>
   const auto WB = V(c->WB.d(), {o, v, o, v});
   E.permute("mjeb->mejb", 1.0, OOVV, 0.0, WB);
-  E.contract("jf,mbef->mejb", 1.0, T1, OVVV, 1.0, WB, P64, "Wmbej");
-  E.contract("nb,nmje->mejb", -1.0, T1, OOOV, 1.0, WB, P64, "Wmbej");
+  E.contract("jf,mbef->mejb", 1.0, T1, OVVV, 1.0, WB, SP, "Wmbej");
+  E.contract("nb,nmje->mejb", -1.0, T1, OOOV, 1.0, WB, SP, "Wmbej");
   const auto RSB = V(c->rs_WB.d(), {o, v, o, v});
   big("mnef,nfjb->mejb", -1.0, OOVV, nullptr, YY, &dY, nullptr, WB, RSB, true, "Wmbej");
   big("mnef,njfb->mejb", 0.5, LOOVV, nullptr, T2, &dT2, nullptr, WB, RSB, false, "Wmbej");
@@ -395,15 +397,15 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   // ---- build_t1, ccwfn.cc:432-469
   const auto RR1 = V(c->R1.d(), {o, v});
   E.permute("ai->ia", 1.0, Cn(c->fvo.d(), {v, o}), 0.0, RR1);
-  E.contract("ie,ae->ia", 1.0, T1, FVV, 1.0, RR1, P64, "t1");
-  E.contract("ma,mi->ia", -1.0, T1, FOO, 1.0, RR1, P64, "t1");
-  E.contract("imae,me->ia", 1.0, UU, FOVI, 1.0, RR1, P64, "t1");
-  E.contract("nf,nifa->ia", 2.0, T1, OOVV, 1.0, RR1, P64, "t1");   // 2<na|fi> = 2<ni|fa>
-  E.contract("nf,naif->ia", -1.0, T1, OVOV, 1.0, RR1, P64, "t1");  // -<na|if>
+  E.contract("ie,ae->ia", 1.0, T1, FVV, 1.0, RR1, SP, "t1");
+  E.contract("ma,mi->ia", -1.0, T1, FOO, 1.0, RR1, SP, "t1");
+  E.contract("imae,me->ia", 1.0, UU, FOVI, 1.0, RR1, SP, "t1");
+  E.contract("nf,nifa->ia", 2.0, T1, OOVV, 1.0, RR1, SP, "t1");   // 2<na|fi> = 2<ni|fa>
+  E.contract("nf,naif->ia", -1.0, T1, OVOV, 1.0, RR1, SP, "t1");  // -<na|if>
   big("mief,maef->ia", 1.0, UU, &dU, OVVV, nullptr, nullptr, RR1, V(c->rs_R1.d(), {o, v}), true, "t1");
   add_sum(c->rs_R1, c->R1.d(), (long)o * v);
-  E.contract("mnae,mnie->ia", -2.0, T2, OOOV, 1.0, RR1, P64, "t1");  // L<nm|ei> = 2<mn|ie> - <nm|ie>
-  E.contract("mnae,nmie->ia", 1.0, T2, OOOV, 1.0, RR1, P64, "t1");
+  E.contract("mnae,mnie->ia", -2.0, T2, OOOV, 1.0, RR1, SP, "t1");  // L<nm|ei> = 2<mn|ie> - <nm|ie>
+  E.contract("mnae,nmie->ia", 1.0, T2, OOOV, 1.0, RR1, SP, "t1");
 
   // ---- build_t2 (half X), ccwfn.cc:471-539
   const auto XX = V(c->X.d(), {o, o, v, v});
@@ -412,9 +414,9 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   // Fae - 0.5 t1.Fme and Fmi + 0.5 t1.Fme fold the t1*Fme terms (:489-491, :496-498)
   const auto FAEP = V(c->FaeP.d(), {v, v}), FMIP = V(c->FmiP.d(), {o, o});
   check(hipMemcpyAsync(c->FaeP.p, c->Fvv.p, (size_t)v * v * 8, hipMemcpyDeviceToDevice, st), "copy");
-  E.contract("mb,me->be", -0.5, T1, FOVI, 1.0, FAEP, P64, "t2");
+  E.contract("mb,me->be", -0.5, T1, FOVI, 1.0, FAEP, SP, "t2");
   check(hipMemcpyAsync(c->FmiP.p, c->Foo.p, (size_t)o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
-  E.contract("je,me->mj", 0.5, T1, FOVI, 1.0, FMIP, P64, "t2");
+  E.contract("je,me->mj", 0.5, T1, FOVI, 1.0, FMIP, SP, "t2");
   if (base) {
     keep(c->rf_FaeP, c->FaeP.p, (size_t)v * v * 8); keep(c->rf_FmiP, c->FmiP.p, (size_t)o * o * 8);
   } else if (delta) {
@@ -440,18 +442,18 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   } else {
     big("ijef,abef->ijab", 0.5, TAU, &dTAU, VVVV, nullptr, nullptr, XX, RSX, false, "ladder");
   }
-  E.contract("ie,jeba->ijab", 1.0, T1, OVVV, 1.0, XX, P64, "t2_t1");  // <ab|ej> = <je|ba>
+  E.contract("ie,jeba->ijab", 1.0, T1, OVVV, 1.0, XX, SP, "t2_t1");  // <ab|ej> = <je|ba>
   // Z' = Zmbij + sum_e t1_ie <mb|ej> + <mb|ij>  (the t1 t1 <mb|ej> and
   // t1 <mb|ij> terms of :512-515, :528-535 share Zmbij's t1_ma contraction)
   const auto ZX = V(c->Zx.d(), {o, v, o, o});
   E.permute("mbij->mbij", 1.0, ZZ, 0.0, ZX);
-  E.contract("ie,mjeb->mbij", 1.0, T1, OOVV, 1.0, ZX, P64, "t2_t1");
+  E.contract("ie,mjeb->mbij", 1.0, T1, OOVV, 1.0, ZX, SP, "t2_t1");
   E.permute("ijmb->mbij", 1.0, OOOV, 1.0, ZX);
-  E.contract("ma,mbij->ijab", -1.0, T1, ZX, 1.0, XX, P64, "t2_t1");
+  E.contract("ma,mbij->ijab", -1.0, T1, ZX, 1.0, XX, SP, "t2_t1");
   // P2[m][a][j][i] = sum_e t1_ie <ma|je>
   const auto PP2 = V(c->P2.d(), {o, v, o, o});
-  E.contract("ie,maje->maji", 1.0, T1, OVOV, 0.0, PP2, P64, "t2_t1");
-  E.contract("mb,maji->ijab", -1.0, T1, PP2, 1.0, XX, P64, "t2_t1");
+  E.contract("ie,maje->maji", 1.0, T1, OVOV, 0.0, PP2, SP, "t2_t1");
+  E.contract("mb,maji->ijab", -1.0, T1, PP2, 1.0, XX, SP, "t2_t1");
   // ring terms, ccwfn.cc:516-526
   const auto dWB = V(c->rd_WB.d(), {o, v, o, v}), sWB = V(c->rf_WB.d(), {o, v, o, v});
   const auto dWJ = V(c->rd_WJ.d(), {o, v, o, v}), sWJ = V(c->rf_WJ.d(), {o, v, o, v});
@@ -640,7 +642,8 @@ void alloc_work(ugx_ctx *c) {
   c->Fov.alloc(ov * 8);
   c->Woooo.alloc(o * o * o * o * 8);
   c->Z.alloc(o * v * o * o * 8); c->Zx.alloc(o * v * o * o * 8); c->P2.alloc(o * v * o * o * 8);
-  c->part.alloc((size_t)std::max(npart_max(c), 512 * 8) * 8);
+  // the DIIS dots write 512 partials per stored vector
+  c->part.alloc((size_t)std::max(npart_max(c), 512 * kDiisMax) * 8);
 }
 
 void guess_mp2(ugx_ctx *c) {
@@ -1395,7 +1398,7 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
     require_loaded(c);
     ugx_options opt;
     if (opt_in) opt = *opt_in; else ugx_default_options(&opt);
-    if (opt.maxiter < 0 || opt.diis_nvec < 1 || opt.diis_nvec > 8 || opt.diis_min_vecs < 1 ||
+    if (opt.maxiter < 0 || opt.diis_nvec < 1 || opt.diis_nvec > kDiisMax || opt.diis_min_vecs < 1 ||
         opt.precision < 0 || opt.precision > 2 || !(opt.r_convergence > 0) ||
         (opt.polish_mode != UGX_POLISH_PLAIN && opt.polish_mode != UGX_POLISH_REFINE))
       throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: invalid options");

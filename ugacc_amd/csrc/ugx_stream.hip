@@ -316,18 +316,18 @@ __global__ void k_diis_store(long n1, long n2, const double *t1, const double *t
 }
 
 struct VecList {
-  const double *v[8];
-  double c[8];
+  const double *v[kDiisMax];
+  double c[kDiisMax];
 };
 
 __global__ void k_diis_dots(long len, const double *e, VecList vl, int nvec, double *partials, int npart) {
-  double s[8];
+  double s[kDiisMax];
 #pragma unroll
-  for (int p = 0; p < 8; p++) s[p] = 0.0;
+  for (int p = 0; p < kDiisMax; p++) s[p] = 0.0;
   for (long i = blockIdx.x * (long)NT + threadIdx.x; i < len; i += (long)gridDim.x * NT) {
     double x = e[i];
 #pragma unroll
-    for (int p = 0; p < 8; p++)
+    for (int p = 0; p < kDiisMax; p++)
       if (p < nvec) s[p] += x * vl.v[p][i];
   }
   for (int p = 0; p < nvec; p++) {
@@ -340,7 +340,7 @@ __global__ void k_diis_combine(long n1, long n2, VecList vl, int nvec, double *t
   for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n1 + n2; i += (long)gridDim.x * NT) {
     double s = 0.0;
 #pragma unroll
-    for (int p = 0; p < 8; p++)
+    for (int p = 0; p < kDiisMax; p++)
       if (p < nvec) s += vl.c[p] * vl.v[p][i];
     if (i < n1) t1[i] = s; else t2[i - n1] = s;
   }
@@ -652,7 +652,7 @@ void launch_diis_store(long n1, long n2, const double *t1, const double *t1old, 
 
 void launch_diis_dots(long len, const double *e, const double *const *vecs, int nvec, double *partials,
                       int npart, hipStream_t st) {
-  if (nvec > 8) throw std::runtime_error("ugx: DIIS subspace above 8");
+  if (nvec > kDiisMax) throw std::runtime_error("ugx: DIIS subspace above 16");
   VecList vl{};
   for (int p = 0; p < nvec; p++) vl.v[p] = vecs[p];
   hipLaunchKernelGGL(k_diis_dots, dim3(npart), dim3(NT), 0, st, len, e, vl, nvec, partials, npart);
@@ -660,7 +660,7 @@ void launch_diis_dots(long len, const double *e, const double *const *vecs, int 
 
 void launch_diis_combine(long n1, long n2, const double *const *amps, const double *coef, int nvec, double *t1,
                          double *t2, hipStream_t st) {
-  if (nvec > 8) throw std::runtime_error("ugx: DIIS subspace above 8");
+  if (nvec > kDiisMax) throw std::runtime_error("ugx: DIIS subspace above 16");
   VecList vl{};
   for (int p = 0; p < nvec; p++) { vl.v[p] = amps[p]; vl.c[p] = coef[p]; }
   hipLaunchKernelGGL(k_diis_combine, dim3(nblocks(n1 + n2)), dim3(NT), 0, st, n1, n2, vl, nvec, t1, t2);
