@@ -222,8 +222,10 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   const bool base = mode == RES_BASE, delta = mode == RES_DELTA, plain = mode == RES_PLAIN;
   if (base && prec != P64) throw UgxError(UGX_BAD_INPUT, "residual: the base point is evaluated in FP64");
   if (!plain) ensure_refine(c);
-  // the small (t1) terms: prec in a plain evaluation, FP64 in the polish
-  const int SP = plain ? prec : P64;
+  // the small (t1) terms: FP64 in the polish; in a plain evaluation prec,
+  // except for small systems (o v < 1024), where the operand conversions of
+  // the low precision cost more launches than the FP64 arithmetic saves
+  const int SP = (plain && c->ov() >= 1024) ? prec : P64;
   const double *t1 = c->t1o.d(), *t2 = c->t2o.d(), *tau = c->tau.d(), *ttau = c->ttau.d();
   auto V = [](const double *p, std::vector<int> d) { return vref(p, std::move(d)); };
   auto Cn = [](const double *p, std::vector<int> d) { return cref(p, std::move(d)); };
