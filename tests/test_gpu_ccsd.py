@@ -251,20 +251,22 @@ def test_larger_diis_subspaces_reach_the_same_fixed_point():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", [ugx.PREC_FP64, ugx.PREC_BF16X3])
-def test_batch_matches_single_solves(prec):
+@pytest.mark.parametrize("prec,polish", [(ugx.PREC_FP64, ugx.POLISH_PLAIN), (ugx.PREC_BF16X3, ugx.POLISH_PLAIN),
+                                         (ugx.PREC_BF16X3, ugx.POLISH_REFINE)])
+def test_batch_matches_single_solves(prec, polish):
     # uneven sizes in one batch; each problem's result must be the one its
     # own single solve gives (the per-problem kernels are the same, and the
     # batched (T) builds each triplet's W3 with the same GEMM shapes)
     from ugacc_amd.synthetic import random_hamiltonian
-    announce("batch of 7 uneven problems, prec=%d" % prec)
+    announce("batch of 7 uneven problems, prec=%d polish=%d" % (prec, polish))
     shapes = [(3, 8, 81), (4, 13, 82), (3, 8, 83), (2, 20, 84), (4, 13, 85), (3, 8, 86), (5, 10, 87)]
     problems = []
     for no, nv, seed in shapes:
         fock, ints = random_hamiltonian(no, nv, seed=seed)
         problems.append((no, nv, fock, ugx.split_blocks(ints, no)))
     c, _, _ = loaded(3, 6, seed=80)
-    kw = dict(r_convergence=1e-10, wfn_ccsd_t=1, precision=prec)
+    kw = dict(r_convergence=1e-10 if polish == ugx.POLISH_PLAIN else 1e-9, wfn_ccsd_t=1, precision=prec,
+              polish_mode=polish)
     single = []
     for no, nv, fock, blocks in problems:
         c.load_hamiltonian(no, nv, fock, blocks)

@@ -93,20 +93,27 @@ def test_h2o4_full_size_sampled_parity():
         assert abs(per[t] - o.tcorr_tjl_range(t, t + 1)) < 1e-13, t
 
 
-def test_refinement_polish_reaches_the_fp64_fixed_point():
-    """BF16x3 iterations to ||R|| < 1e-5, then the refinement polish: one
-    FP64 residual at the switch point T_s, then residuals R(T_s) + [R(T) -
-    R(T_s)] whose large contractions are BF16x3 products with a difference
-    from T_s as one operand.  At the converged amplitudes the FP64 residual,
-    evaluated independently by ugx_residual, must be below the threshold; the
-    energy is the FP64 form's and the iteration count the plain FP64
-    polish's (+-1)."""
-    announce("refinement polish (h2o)2")
+@pytest.mark.parametrize("form,ladder", [("bf16x3", "packed"), ("fp32", "packed"), ("bf16x3", "full")])
+def test_refinement_polish_reaches_the_fp64_fixed_point(form, ladder, monkeypatch):
+    """Low-precision iterations to ||R|| < 1e-5, then the refinement polish:
+    one FP64 residual at the switch point T_s, then residuals R(T_s) + [R(T) -
+    R(T_s)] whose large contractions are low-precision products with a
+    difference from T_s as one operand.  At the converged amplitudes the FP64
+    residual, evaluated independently by ugx_residual, must be below the
+    threshold; the energy is the FP64 form's and the iteration count the
+    plain FP64 polish's (+-1).  Both the pair-packed tau terms and the full
+    contractions (UGX_LADDER=full, read when a context is created)."""
+    announce("refinement polish (h2o)2 %s %s" % (form, ladder))
     ref = Reference(molecules.water_cluster(2, seed=3), "cc-pvdz")
-    c = context()
+    if ladder == "full":
+        monkeypatch.setenv("UGX_LADDER", "full")
+        c = ugx.Context(0)
+    else:
+        c = context()
     c.load_hamiltonian(ref.no, ref.nv, ref.fock, ref.blocks)
     r64 = c.ccsd(maxiter=100, r_convergence=1e-10, wfn_ccsd_t=0)
-    common = dict(maxiter=100, r_convergence=1e-9, wfn_ccsd_t=0, precision=ugx.PREC_BF16X3, polish_threshold=1e-5)
+    common = dict(maxiter=100, r_convergence=1e-9, wfn_ccsd_t=0, precision=ugx.PREC_NAMES[form],
+                  polish_threshold=1e-5)
     rp = c.ccsd(**common)
     rr = c.ccsd(polish_mode=ugx.POLISH_REFINE, **common)
     t1, t2 = c.amplitudes()
@@ -116,6 +123,8 @@ def test_refinement_polish_reaches_the_fp64_fixed_point():
     assert abs(rr["niter"] - rp["niter"]) <= 1, (rr["niter"], rp["niter"])
     assert np.sqrt((R1 ** 2).sum() + (R2 ** 2).sum()) < 2e-9
     assert abs(rr["e_ccsd_final"] - r64["e_ccsd_final"]) < 1e-9
+    if ladder == "full":
+        c.close()
 
 
 @pytest.mark.parametrize("form", ["fp32", "bf16x3"])
