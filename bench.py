@@ -191,7 +191,9 @@ def run_batch(args):
     graph = 0 if args.no_graph else 1
     forms = {"fp64": dict(precision=ugx.PREC_FP64, r_convergence=FP64_CONV),
              "mixed": dict(precision=ugx.PREC_NAMES[args.low], r_convergence=args.mixed_conv,
-                           polish_threshold=args.polish)}
+                           polish_threshold=args.polish,
+                           polish_mode=ugx.POLISH_REFINE if args.polish_mode == "refine" else ugx.POLISH_PLAIN)}
+    log("batch64 mixed form: %s polish" % (args.polish_mode or "plain"))
     out = {}
     for name, kw in forms.items():
         for w in range(args.warmup):
@@ -217,7 +219,8 @@ def run_batch(args):
                     "(O-O U[2.7,3.2] A, random rotations, seeds 200-231), cc-pVDZ, host front end (untimed)",
             "config": {"workload": "64 uneven geometries: 32 x (o=5, v=19) + 32 x (o=10, v=38)",
                        "workers": args.workers, "graph": bool(graph), "autotune": not args.no_autotune,
-                       "mixed_r_convergence": args.mixed_conv, "fp64_r_convergence": FP64_CONV},
+                       "mixed_r_convergence": args.mixed_conv, "mixed_polish": args.polish_mode or "plain",
+                       "fp64_r_convergence": FP64_CONV},
             "forms": out, "sequential_single_fp64_s": t_seq, "roofline": None, "cpu_baseline": None}
     print(json.dumps(line), flush=True)
 
@@ -237,9 +240,11 @@ def main():
     ap.add_argument("--guess", default="learned", choices=["learned", "mp2"])
     ap.add_argument("--mixed-conv", type=float, default=1e-7)
     ap.add_argument("--polish", type=float, default=1e-5)
-    ap.add_argument("--polish-mode", default="refine", choices=["refine", "plain"],
+    ap.add_argument("--polish-mode", default=None, choices=["refine", "plain"],
                     help="mixed form after the switch: refinement polish (FP64 residual at the switch point, then "
-                         "low-precision difference residuals) or plain FP64 residuals")
+                         "low-precision difference residuals) or plain FP64 residuals; default refine for the "
+                         "single-geometry workloads, plain for batch64 (launch-bound small systems, where the "
+                         "refinement's extra launches cost more than its FP64 savings)")
     ap.add_argument("--no-plain-compare", action="store_true",
                     help="mixed form: skip the extra timed solve with the plain FP64 polish")
     ap.add_argument("--no-cpu", action="store_true")
@@ -318,6 +323,8 @@ def main():
     out = {}
     if args.form == "mixed":
         wall64, res64, st64 = run_form(ctx, fp64_opts, args.warmup, args.steps, "fp64")
+        if args.polish_mode is None:
+            args.polish_mode = "refine"
         pmode = ugx.POLISH_REFINE if args.polish_mode == "refine" else ugx.POLISH_PLAIN
         if pmode == ugx.POLISH_REFINE and not args.no_plain_compare:
             # the same mixed form with the plain FP64 polish, for comparison
