@@ -249,6 +249,42 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
     if (!needs) return 0.0;
     return T.constant ? 0.01 * size : 2.0 * size;
   };
+  // C written straight from the GEMM epilogue: each of the GEMM's row and
+  // column index groups maps to C with at most two stride levels (outer
+  // letters, inner letters; GemmDesc cdiv / ndiv), the columns' innermost
+  // stride being 1 so that a wave's stores stay contiguous.  Otherwise the
+  // GEMM writes a workspace that is permuted into C.
+  struct OutMap {
+    bool ok = false;
+    long div = 0, s1 = 0, s0 = 0;
+  };
+  const auto cstr = strides_of(sp.c, ext);
+  auto flat = [&](const std::string &g) {
+    for (size_t q = 0; q + 1 < g.size(); q++)
+      if (cstr.at(g[q]) != cstr.at(g[q + 1]) * ext.at(g[q + 1])) return false;
+    return true;
+  };
+  auto outmap = [&](const std::string &g) {
+    OutMap o;
+    if (g.empty()) { o.ok = true; return o; }
+    if (flat(g)) { o.ok = true; o.s0 = cstr.at(g.back()); return o; }
+    for (size_t p = 1; p < g.size(); p++) {
+      const std::string outer = g.substr(0, p), inner = g.substr(p);
+      if (flat(outer) && flat(inner)) {
+        o.ok = true;
+        o.div = extent_of(inner, ext);
+        o.s0 = cstr.at(inner.back());
+        o.s1 = cstr.at(outer.back());
+        o.ok = o.div <= 0x7fffffffL;
+        return o;
+      }
+    }
+    return o;
+  };
+  auto two_level = [&](const std::string &rows, const std::string &cols) {
+    const OutMap rm = outmap(rows), cm = outmap(cols);
+    return rm.ok && cm.ok && (cols.empty() || cm.s0 == 1);
+  };
   Plan best{1e300, false, "", "", ""};
   for (int sw = 0; sw < 2; sw++)
     for (int mi = 0; mi < 2; mi++)
@@ -258,7 +294,7 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
           double c = opcost(A, sp.a, m, k, sA) + opcost(B, sp.b, k, n, sB);
           // C = A B with rows m, cols n (sw = 0) or rows n, cols m (sw = 1)
           std::string rows = sw ? n : m, cols = sw ? m : n;
-          if (sp.c == rows + cols) c += 0.0;
+          if (sp.c == rows + cols || two_level(rows, cols)) c += 0.0;
           else c += 3.0 * sC;
           if (c < best.cost) best = Plan{c, sw == 1, m, n, k};
         }
@@ -357,6 +393,12 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
   }
   if (direct) {
     d.C = (double *)C.p; d.scm = Ce; d.scn = 1; d.scm1 = 0; d.alpha = alpha; d.beta = beta;
+    launch_gemm(d, nullptr, kprec, tile_for(d, nullptr, kprec, -1), st_);
+  } else if (two_level(rows, cols)) {
+    const OutMap rm = outmap(rows), cm = outmap(cols);
+    d.C = (double *)C.p; d.alpha = alpha; d.beta = beta;
+    d.scm = rm.s0; d.cdiv = (int)rm.div; d.scm1 = rm.s1;
+    d.scn = cm.s0; d.ndiv = (int)cm.div; d.scn1 = cm.s1;
     launch_gemm(d, nullptr, kprec, tile_for(d, nullptr, kprec, -1), st_);
   } else {
     double *tmp = (double *)slot(2, (size_t)Re * Ce * 8);
