@@ -404,10 +404,12 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   E.contract("imae,me->ia", 1.0, UU, FOVI, 1.0, RR1, SP, "t1");
   E.contract("nf,nifa->ia", 2.0, T1, OOVV, 1.0, RR1, SP, "t1");   // 2<na|fi> = 2<ni|fa>
   E.contract("nf,naif->ia", -1.0, T1, OVOV, 1.0, RR1, SP, "t1");  // -<na|if>
-  big("mief,maef->ia", 1.0, UU, &dU, OVVV, nullptr, nullptr, RR1, V(c->rs_R1.d(), {o, v}), true, "t1");
+  // sum_mef U_mief <ma|ef>, read as U_imfe (U_mief = U_imfe, the pair
+  // symmetry of t2): the stored order, so U needs no permute
+  big("imfe,maef->ia", 1.0, UU, &dU, OVVV, nullptr, nullptr, RR1, V(c->rs_R1.d(), {o, v}), true, "t1");
   add_sum(c->rs_R1, c->R1.d(), (long)o * v);
-  E.contract("mnae,mnie->ia", -2.0, T2, OOOV, 1.0, RR1, SP, "t1");  // L<nm|ei> = 2<mn|ie> - <nm|ie>
-  E.contract("mnae,nmie->ia", 1.0, T2, OOOV, 1.0, RR1, SP, "t1");
+  // -sum_mne t2_mnae L<nm|ei>, L<nm|ei> = 2<mn|ie> - <nm|ie> = Looov[m][n][i][e]
+  E.contract("mnae,mnie->ia", -1.0, T2, LOOOV, 1.0, RR1, SP, "t1");
 
   // ---- build_t2 (half X), ccwfn.cc:471-539
   const auto XX = V(c->X.d(), {o, o, v, v});
