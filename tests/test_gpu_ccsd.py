@@ -156,6 +156,29 @@ def test_packed_ladder_matches_full_contraction(no, nv, monkeypatch):
     np.testing.assert_allclose(R2p, R2p.transpose(1, 0, 3, 2), atol=1e-15 * s)
 
 
+@pytest.mark.parametrize("no,nv", [(8, 32), (6, 40)])
+def test_bf16x3_panel_operands_match_pair_arrays(no, nv, monkeypatch):
+    """BF16x3 contractions whose K is whole 32-slices take their operands as
+    MFMA panels (the LDS-copy kernel); UGX_CCSD_PANELS=0 (read when a context
+    is created) keeps the hi/lo pair arrays and the register-staged kernel.
+    Same split values, same MFMA order: the residuals agree to rounding."""
+    announce("panel vs pair BF16x3 residual o=%d v=%d" % (no, nv))
+    c, fock, ints = loaded(no, nv, seed=no * 7 + nv)
+    rng = np.random.default_rng(no + nv)
+    t1 = 0.02 * rng.standard_normal((no, nv))
+    t2 = 0.02 * rng.standard_normal((no, no, nv, nv))
+    t2 = 0.5 * (t2 + t2.transpose(1, 0, 3, 2))
+    R1p, R2p = c.residual(t1, t2, ugx.PREC_BF16X3)
+    monkeypatch.setenv("UGX_CCSD_PANELS", "0")
+    pairs = ugx.Context(0)
+    pairs.load_hamiltonian(no, nv, fock, ugx.split_blocks(ints, no))
+    R1q, R2q = pairs.residual(t1, t2, ugx.PREC_BF16X3)
+    pairs.close()
+    s = max(1.0, np.max(np.abs(R2q)))
+    assert np.max(np.abs(R2p - R2q)) < 1e-12 * s
+    assert np.max(np.abs(R1p - R1q)) < 1e-12 * s
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("no,nv,stages", [(8, 40, "3"), (5, 19, "0"), (3, 70, "0")])
 def test_triples_bf16x3_lds_copy_matches_register_path(no, nv, stages, monkeypatch):

@@ -1,6 +1,8 @@
 // Contraction engine (see ugx_engine.h).
 #include "ugx_engine.h"
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -72,6 +74,7 @@ bool is_perm_of(const std::string &x, const std::string &y) {
 // bytes per element: FP64, FP32, BF16 hi/lo pair
 size_t esize(int dtype) { return dtype == 0 ? 8 : 4; }
 
+
 }  // namespace
 
 Engine::~Engine() {
@@ -137,11 +140,20 @@ void Engine::collect_stats() {
 // with element type dtype.  No copy when T already is that; constants are
 // permuted/converted once and cached; variables go to workspace slot_id.
 const void *Engine::prepared(const TRef &T, const std::string &letters, const std::string &want, int dtype,
-                             int slot_id) {
+                             int slot_id, long pk) {
   if (letters == want && T.dtype == dtype) return T.p;
   std::map<char, int> ext;
   for (size_t d = 0; d < letters.size(); d++) ext[letters[d]] = T.dims[d];
   long n = extent_of(letters, ext);
+  size_t bytes = n * esize(dtype);
+  long pblk = 0, plo = 0;
+  if (dtype == 3) {
+    if (pk <= 0 || n % pk) throw std::runtime_error("ugx: bad panel operand shape");
+    const long R = n / pk, Rp = (R + 15) / 16 * 16, Kp = (pk + 31) / 32 * 32;
+    pblk = 16 * Kp;
+    plo = Rp * Kp;
+    bytes = (size_t)plo * 2 * 2;  // hi and lo halves
+  }
   void *dst;
   if (T.constant) {
     // the key names the permutation (source letters -> wanted order), not
@@ -150,10 +162,10 @@ const void *Engine::prepared(const TRef &T, const std::string &letters, const st
     auto key = std::make_tuple(T.p, letters + "->" + want, dtype);
     auto it = cache_.find(key);
     if (it != cache_.end()) return it->second.first;
-    hip_check(hipMalloc(&dst, n * esize(dtype)), "cache alloc");
-    cache_[key] = {dst, n * esize(dtype)};
+    hip_check(hipMalloc(&dst, bytes), "cache alloc");
+    cache_[key] = {dst, bytes};
   } else {
-    dst = slot(slot_id, n * esize(dtype));
+    dst = slot(slot_id, bytes);
   }
   auto is = strides_of(letters, ext);
   PermDesc pd{};
@@ -161,6 +173,7 @@ const void *Engine::prepared(const TRef &T, const std::string &letters, const st
   if (pd.rank == 0) throw std::runtime_error("ugx: empty operand");
   for (int d = 0; d < pd.rank; d++) { pd.od[d] = ext.at(want[d]); pd.istr[d] = is.at(want[d]); }
   pd.in = T.p; pd.out = dst; pd.in_t = T.dtype; pd.out_t = dtype; pd.alpha = 1.0; pd.beta = 0.0;
+  pd.pk = pk; pd.pblk = pblk; pd.plo = plo;
   launch_permute(pd, st_);
   return dst;
 }
@@ -306,9 +319,26 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
   const int kprec = pair ? (int)PREC_BF16X3_PRESPLIT : prec;
   const long Me = extent_of(best.m, ext), Ne = extent_of(best.n, ext), Ke = extent_of(best.k, ext);
   // Operand A as matrix (m, k); B as (k, n).
+  const long Re = best.swap ? Ne : Me, Ce = best.swap ? Me : Ne;
+  // BF16x3 operands as MFMA panels (LDS-copy kernel, GemmDesc::dma) when K
+  // is whole panel slices (no k padding to clear) and the GEMM does not
+  // take the split-K path below
+  bool panel = false;
+  if (pair && panels_ && Ke % BK_PANEL == 0) {
+    GemmDesc t{};
+    t.M = (int)std::min<long>(Re, 0x7fffffffL); t.N = (int)std::min<long>(Ce, 0x7fffffffL); t.batch = 1; t.nseg = 1;
+    int bm, bn;
+    gemm_tile_dims(gemm_pick_tile(t), bm, bn);
+    const long tiles = ((Re + bm - 1) / bm) * ((Ce + bn - 1) / bn);
+    panel = !(tiles < 256 && Ke >= 2048);
+  }
   auto mat = [&](const TRef &T, const std::string &letters, const std::string &g1, const std::string &g2,
                  long e1, long e2, int slot_id, long &s1, long &s2) -> const void * {
     // returns pointer; strides s1 (along g1 index), s2 (along g2 index)
+    if (panel) {
+      s1 = BK_PANEL; s2 = 1;
+      return prepared(T, letters, g1 + g2, 3, slot_id, e2);
+    }
     if (pair) {
       s1 = e2; s2 = 1;
       if (letters == g1 + g2 && T.dtype == 2) return T.p;
@@ -330,10 +360,12 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
   long skb, snb;
   const void *Bp = mat(B, sp.b, best.n, best.k, Ne, Ke, 1, snb, skb);
   sbk = skb; sbn = snb;
-  const long loA = pair ? Me * Ke : 0, loB = pair ? Ne * Ke : 0;
+  const long Kp = (Ke + BK_PANEL - 1) / BK_PANEL * BK_PANEL;
+  const long loA = panel ? (Me + 15) / 16 * 16 * Kp : (pair ? Me * Ke : 0);
+  const long loB = panel ? (Ne + 15) / 16 * 16 * Kp : (pair ? Ne * Ke : 0);
+  const long pblk = panel ? 16 * Kp : 0;
 
   std::string rows = best.swap ? best.n : best.m, cols = best.swap ? best.m : best.n;
-  const long Re = best.swap ? Ne : Me, Ce = best.swap ? Me : Ne;
   GemmSeg g{};
   if (!best.swap) {
     g.A = Ap; g.sam = sam; g.sak = sak; g.B = Bp; g.sbk = sbk; g.sbn = sbn; g.loA = loA; g.loB = loB;
@@ -341,6 +373,8 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
     // rows = n: A' (n,k) = B(k,n);  B' (k,m) = A(m,k)
     g.A = Bp; g.sam = sbn; g.sak = sbk; g.B = Ap; g.sbk = sak; g.sbn = sam; g.loA = loB; g.loB = loA;
   }
+  g.pblkA = pblk; g.pblkB = pblk;
+  if (panel) d.dma = 2;
   g.K = (int)Ke; g.sign = 1.0;
   d.M = (int)Re; d.N = (int)Ce; d.nseg = 1; d.seg[0] = g; d.batch = 1;
   if (Re > 0x7fffffffL || Ce > 0x7fffffffL || Ke > 0x7fffffffL) throw std::runtime_error("ugx: GEMM too large");
@@ -360,6 +394,7 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
       long S = std::min<long>((512 + tiles - 1) / tiles, Ke / 512);
       S = std::min<long>(S, 4096);
       if (S > 1) {
+        if (panel) throw std::runtime_error("ugx: split-K on panel operands");
         long chunk = (Ke + S - 1) / S;
         chunk = (chunk + 31) / 32 * 32;  // whole slices (and 16-byte pair loads) per chunk
         S = (Ke + chunk - 1) / chunk;

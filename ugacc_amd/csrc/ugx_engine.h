@@ -17,6 +17,7 @@
 #include <map>
 #include <string>
 #include <tuple>
+#include <cstdlib>
 #include <vector>
 
 #include "ugx_internal.h"
@@ -41,7 +42,12 @@ struct KernelStat {
 
 class Engine {
  public:
-  explicit Engine(hipStream_t st) : st_(st) {}
+  // UGX_CCSD_PANELS=0 (read here, per engine): BF16x3 operands as hi/lo
+  // pair arrays only, never as MFMA panels
+  explicit Engine(hipStream_t st) : st_(st) {
+    const char *e = std::getenv("UGX_CCSD_PANELS");
+    panels_ = !(e && e[0] == '0' && e[1] == 0);
+  }
   ~Engine();
   Engine(const Engine &) = delete;
   Engine &operator=(const Engine &) = delete;
@@ -89,9 +95,11 @@ class Engine {
  private:
   struct Plan;
   void *slot(int i, size_t bytes);
+  // dtype 3: BF16 hi/lo MFMA panels of the [rows][pk] view (pk = the k extent)
   const void *prepared(const TRef &T, const std::string &letters, const std::string &want, int dtype,
-                       int slot_id);
+                       int slot_id, long pk = 0);
   hipStream_t st_;
+  bool panels_ = true;
   std::vector<std::pair<void *, size_t>> slots_{6, {nullptr, 0}};
   bool autotune_ = false;
   long out_extent_hint_ = 0;

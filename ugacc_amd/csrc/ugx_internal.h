@@ -123,6 +123,9 @@ void launch_splitk_reduce(int S, int M, int N, const double *W, double alpha, do
 // the input element.  Element types: 0 = double, 1 = float (in or out),
 // 2 = split BF16 pair (out only, beta = 0): n halves hi = bf16(x), then n
 // halves lo = bf16(x - hi).
+// out_t: 0 FP64, 1 FP32, 2 BF16 hi/lo pair arrays, 3 BF16 hi/lo MFMA panels
+// of the output viewed as [n / pk][pk] (rows x k; row blocks pblk apart, lo
+// halves at plo; see launch_pack_panel; k padding is not written)
 struct PermDesc {
   int rank;
   int od[6];
@@ -131,6 +134,7 @@ struct PermDesc {
   void *out;
   int in_t, out_t;
   double alpha, beta;
+  long pk, pblk, plo;
 };
 void launch_permute(const PermDesc &p, hipStream_t st);
 // Packs np FP64 matrices in[p*sp + r*sr + k*sk] (r < R, k < K) as split

@@ -48,6 +48,7 @@ struct PermArgs {
   void *out;
   double alpha, beta;
   long n;  // total elements (offset of the lo half of a BF16 pair output)
+  long pk, pblk, plo;  // MFMA panel output: k extent, row-block stride, lo offset
 };
 
 // Output element policies: FP64, FP32, and the split BF16 pair (hi at
@@ -76,6 +77,24 @@ struct OutB3 {
     float hf = __uint_as_float(((unsigned int)h) << 16);
     ((unsigned short *)p.out)[i] = h;
     ((unsigned short *)p.out)[p.n + i] = bf16_rne(x - hf);
+  }
+};
+
+// Split BF16 pair into MFMA panels: element i of the [n / pk][pk] output
+// (row r, k) goes to panel block r / 16, k slice k / 32, row r % 16, stored
+// 8-k chunk panel_chunk(r % 16, k % 32 / 8) (the layout of launch_pack_panel)
+struct OutP {
+  typedef float tile_t;
+  __device__ static double get(const PermArgs &, long) { return 0.0; }
+  __device__ static void put(const PermArgs &p, long i, double v) {
+    const long r = i / p.pk, k = i - r * p.pk;
+    const int rr = (int)(r & 15), kk = (int)(k & 31);
+    const long a = (r >> 4) * p.pblk + (k >> 5) * 512 + rr * 32 + 8 * panel_chunk(rr, kk >> 3) + (kk & 7);
+    float x = (float)v;
+    unsigned short h = bf16_rne(x);
+    float hf = __uint_as_float(((unsigned int)h) << 16);
+    ((unsigned short *)p.out)[a] = h;
+    ((unsigned short *)p.out)[p.plo + a] = bf16_rne(x - hf);
   }
 };
 
@@ -565,15 +584,20 @@ void launch_permute(const PermDesc &p, hipStream_t st) {
     a.n *= a.od[d];
   }
   a.in = p.in; a.out = p.out; a.alpha = p.alpha; a.beta = p.beta;
-  if (p.out_t == 2 && p.beta != 0.0) throw std::runtime_error("ugx: BF16 pair output cannot accumulate");
+  a.pk = p.pk; a.pblk = p.pblk; a.plo = p.plo;
+  if (p.out_t >= 2 && p.beta != 0.0) throw std::runtime_error("ugx: BF16 pair output cannot accumulate");
+  if (p.out_t == 3 && (p.pk <= 0 || a.n % p.pk != 0 || p.pblk != 16 * ((p.pk + 31) / 32 * 32)))
+    throw std::runtime_error("ugx: bad MFMA panel output shape");
   if (p.in_t == 0) {
     if (p.out_t == 0) perm_launch<double, OutD>(a, st);
     else if (p.out_t == 1) perm_launch<double, OutF>(a, st);
-    else perm_launch<double, OutB3>(a, st);
+    else if (p.out_t == 2) perm_launch<double, OutB3>(a, st);
+    else perm_launch<double, OutP>(a, st);
   } else {
     if (p.out_t == 0) perm_launch<float, OutD>(a, st);
     else if (p.out_t == 1) perm_launch<float, OutF>(a, st);
-    else perm_launch<float, OutB3>(a, st);
+    else if (p.out_t == 2) perm_launch<float, OutB3>(a, st);
+    else perm_launch<float, OutP>(a, st);
   }
 }
 
