@@ -156,7 +156,7 @@ def test_packed_ladder_matches_full_contraction(no, nv, monkeypatch):
     np.testing.assert_allclose(R2p, R2p.transpose(1, 0, 3, 2), atol=1e-15 * s)
 
 
-@pytest.mark.parametrize("no,nv", [(8, 32), (6, 40)])
+@pytest.mark.parametrize("no,nv", [(8, 32), (6, 40), (4, 24), (5, 32)])
 def test_bf16x3_panel_operands_match_pair_arrays(no, nv, monkeypatch):
     """BF16x3 contractions whose K is whole 32-slices take their operands as
     MFMA panels (the LDS-copy kernel); UGX_CCSD_PANELS=0 (read when a context
