@@ -50,6 +50,14 @@ __device__ inline float gelu_f(float z) {
   return z / (1.0f + __expf(-2.0f * u));
 }
 
+// Inference form: the hardware reciprocal (1 ulp) instead of an IEEE
+// division; its output is rounded to BF16 (or summed in FP32 by the output
+// layer), far coarser than the difference.
+__device__ inline float gelu_infer_f(float z) {
+  const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+}
+
 __device__ inline float gelu_grad_f(float z) {
   const float z2 = z * z;
   const float u = 0.7978845608028654f * (z + 0.044715f * z2 * z);
@@ -147,7 +155,7 @@ __global__ __launch_bounds__(NT) void k_mlp_infer(long N, const float *__restric
         float z = sb1[f];
 #pragma unroll
         for (int k = 0; k < NF; k++) z += sW1[f * NF + k] * x[k];
-        hb[jj] = bf16_rne(gelu_f(z));
+        hb[jj] = bf16_rne(gelu_infer_f(z));
       }
       bop[s] = u32x4{hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16), hb[6] | (hb[7] << 16)};
     }
@@ -172,8 +180,8 @@ __global__ __launch_bounds__(NT) void k_mlp_infer(long N, const float *__restric
           unsigned int hb[8];
 #pragma unroll
           for (int r = 0; r < 4; r++) {
-            hb[r] = bf16_rne(gelu_f(acc[2 * s][r] + sb[32 * s + 4 * q + r]));
-            hb[4 + r] = bf16_rne(gelu_f(acc[2 * s + 1][r] + sb[32 * s + 16 + 4 * q + r]));
+            hb[r] = bf16_rne(gelu_infer_f(acc[2 * s][r] + sb[32 * s + 4 * q + r]));
+            hb[4 + r] = bf16_rne(gelu_infer_f(acc[2 * s + 1][r] + sb[32 * s + 16 + 4 * q + r]));
           }
           bop[s] = u32x4{hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16),
                          hb[6] | (hb[7] << 16)};
@@ -186,7 +194,7 @@ __global__ __launch_bounds__(NT) void k_mlp_infer(long N, const float *__restric
 #pragma unroll
           for (int r = 0; r < 4; r++) {
             const int f = 16 * F + 4 * q + r;
-            part += sW4[f] * gelu_f(acc[F][r] + sb[f]);
+            part += sW4[f] * gelu_infer_f(acc[F][r] + sb[f]);
           }
       }
     }
