@@ -71,6 +71,9 @@ __device__ inline unsigned int bf16_rne(float x) {
   return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
 }
 
+// F32: the transcendental functions in FP32 (the inference kernel; the
+// features feed a BF16 network), else in FP64 (the training features)
+template <bool F32 = false>
 __device__ inline void amp_features(int no, int nv, const double *__restrict__ eps,
                                     const double *__restrict__ oovv, long g, float *x) {
   const long v = nv;
@@ -84,10 +87,17 @@ __device__ inline void amp_features(int no, int nv, const double *__restrict__ e
   const double D = ei + ej - ea - eb;
   const double I = oovv[g];
   const double K = oovv[(((long)i * no + j) * v + b) * v + a];
-  x[0] = (float)asinh(1e3 * (I / D));
-  x[1] = (float)log(fmax(-D, 1e-6));
-  x[2] = (float)asinh(1e2 * I);
-  x[3] = (float)asinh(1e2 * K);
+  if (F32) {
+    x[0] = asinhf((float)(1e3 * I / D));
+    x[1] = logf(fmaxf((float)-D, 1e-6f));
+    x[2] = asinhf((float)(1e2 * I));
+    x[3] = asinhf((float)(1e2 * K));
+  } else {
+    x[0] = (float)asinh(1e3 * (I / D));
+    x[1] = (float)log(fmax(-D, 1e-6));
+    x[2] = (float)asinh(1e2 * I);
+    x[3] = (float)asinh(1e2 * K);
+  }
   x[4] = (float)ei;
   x[5] = (float)ej;
   x[6] = (float)ea;
@@ -135,7 +145,7 @@ __global__ __launch_bounds__(NT) void k_mlp_infer(long N, const float *__restric
     const bool valid = g < N;
     float x[NF];
     if (valid) {
-      if (HAM) amp_features(no, nv, eps, oovv, g, x);
+      if (HAM) amp_features<true>(no, nv, eps, oovv, g, x);
       else {
 #pragma unroll
         for (int k = 0; k < NF; k++) x[k] = Xraw[g * NF + k];
