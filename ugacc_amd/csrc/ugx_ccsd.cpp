@@ -341,7 +341,10 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
         "Woooo");
   }
   add_sum(c->rs_WO, c->Woooo.d(), n_oooo);
-  if (base) keep(c->rf_WO, c->Woooo.p, n_oooo * 8);
+  if (base) {
+    keep(c->rf_WO, c->Woooo.p, n_oooo * 8);
+    E.refresh(c->rf_WO.p);
+  }
   else if (delta) dif(c->rd_WO, c->Woooo.d(), c->rf_WO, n_oooo);
 
   // Y[n][f][j][b] = 0.5 t2[j][n][f][b] + t1[j][f] t1[n][b]
@@ -369,6 +372,7 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   add_sum(c->rs_WB, c->WB.d(), n4);
   if (base) {
     keep(c->rf_WJ, c->WJ.p, n4 * 8); keep(c->rf_WB, c->WB.p, n4 * 8);
+    E.refresh(c->rf_WJ.p); E.refresh(c->rf_WB.p);
   } else if (delta) {
     dif(c->rd_WJ, c->WJ.d(), c->rf_WJ, n4); dif(c->rd_WB, c->WB.d(), c->rf_WB, n4);
   }
@@ -423,12 +427,15 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   E.contract("je,me->mj", 0.5, T1, FOVI, 1.0, FMIP, SP, "t2");
   if (base) {
     keep(c->rf_FaeP, c->FaeP.p, (size_t)v * v * 8); keep(c->rf_FmiP, c->FmiP.p, (size_t)o * o * 8);
+    E.refresh(c->rf_FaeP.p); E.refresh(c->rf_FmiP.p);
   } else if (delta) {
     dif(c->rd_FaeP, c->FaeP.d(), c->rf_FaeP, (long)v * v); dif(c->rd_FmiP, c->FmiP.d(), c->rf_FmiP, (long)o * o);
   }
-  const auto dFAEP = V(c->rd_FaeP.d(), {v, v}), sFAEP = V(c->rf_FaeP.d(), {v, v});
-  const auto dFMIP = V(c->rd_FmiP.d(), {o, o}), sFMIP = V(c->rf_FmiP.d(), {o, o});
-  const auto dWO = V(c->rd_WO.d(), {o, o, o, o}), sWO = V(c->rf_WO.d(), {o, o, o, o});
+  // the base-point operands are constant during the polish: their
+  // conversions are cached, and re-converted in place at each base point
+  const auto dFAEP = V(c->rd_FaeP.d(), {v, v}), sFAEP = Cn(c->rf_FaeP.d(), {v, v});
+  const auto dFMIP = V(c->rd_FmiP.d(), {o, o}), sFMIP = Cn(c->rf_FmiP.d(), {o, o});
+  const auto dWO = V(c->rd_WO.d(), {o, o, o, o}), sWO = Cn(c->rf_WO.d(), {o, o, o, o});
   big("ijae,be->ijab", 1.0, T2, &dT2, FAEP, &dFAEP, &sFAEP, XX, RSX, true, "t2_F");
   big("imab,mj->ijab", -1.0, T2, &dT2, FMIP, &dFMIP, &sFMIP, XX, RSX, false, "t2_F");
   big("mnab,mnij->ijab", 0.5, TAU, &dTAU, WO, &dWO, &sWO, XX, RSX, false, "t2_Wmnij");
@@ -459,8 +466,8 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   E.contract("ie,maje->maji", 1.0, T1, OVOV, 0.0, PP2, SP, "t2_t1");
   E.contract("mb,maji->ijab", -1.0, T1, PP2, 1.0, XX, SP, "t2_t1");
   // ring terms, ccwfn.cc:516-526
-  const auto dWB = V(c->rd_WB.d(), {o, v, o, v}), sWB = V(c->rf_WB.d(), {o, v, o, v});
-  const auto dWJ = V(c->rd_WJ.d(), {o, v, o, v}), sWJ = V(c->rf_WJ.d(), {o, v, o, v});
+  const auto dWB = V(c->rd_WB.d(), {o, v, o, v}), sWB = Cn(c->rf_WB.d(), {o, v, o, v});
+  const auto dWJ = V(c->rd_WJ.d(), {o, v, o, v}), sWJ = Cn(c->rf_WJ.d(), {o, v, o, v});
   big("imae,mejb->ijab", 1.0, UU, &dU, WB, &dWB, &sWB, XX, RSX, false, "ring");
   big("imae,mejb->ijab", 1.0, T2, &dT2, WJ, &dWJ, &sWJ, XX, RSX, false, "ring");
   big("mjae,meib->ijab", 1.0, T2, &dT2, WJ, &dWJ, &sWJ, XX, RSX, false, "ring");

@@ -89,6 +89,12 @@ void Engine::drop_cache() {
   hip_check(hipStreamSynchronize(st_), "drop_cache sync");
   for (auto &kv : cache_) (void)hipFree(kv.second.first);
   cache_.clear();
+  cache_pd_.clear();
+}
+
+void Engine::refresh(const void *p) {
+  for (auto &kv : cache_pd_)
+    if (std::get<0>(kv.first) == p) launch_permute(kv.second, st_);
 }
 
 size_t Engine::cached_bytes() const {
@@ -175,6 +181,7 @@ const void *Engine::prepared(const TRef &T, const std::string &letters, const st
   pd.in = T.p; pd.out = dst; pd.in_t = T.dtype; pd.out_t = dtype; pd.alpha = 1.0; pd.beta = 0.0;
   pd.pk = pk; pd.pblk = pblk; pd.plo = plo;
   launch_permute(pd, st_);
+  if (T.constant) cache_pd_[std::make_tuple(T.p, letters + "->" + want, dtype)] = pd;
   return dst;
 }
 

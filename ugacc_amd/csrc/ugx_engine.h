@@ -71,6 +71,10 @@ class Engine {
   bool profiling() const { return profiling_; }
   void reset_counters() { flops_ = 0.0; stats_.clear(); }
   void drop_cache();
+  // Re-converts, in place, every cached conversion of the constant at p
+  // (whose contents the caller has just replaced): buffers keep their
+  // addresses, so captured graphs that read them stay valid.
+  void refresh(const void *p);
   size_t cached_bytes() const;
   hipStream_t stream() const { return st_; }
 
@@ -105,6 +109,7 @@ class Engine {
   long out_extent_hint_ = 0;
   int tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kprec, int fallback);
   std::map<std::tuple<const void *, std::string, int>, std::pair<void *, size_t>> cache_;
+  std::map<std::tuple<const void *, std::string, int>, PermDesc> cache_pd_;  // how each entry was filled
   bool profiling_ = false;
   struct Pending {
     std::string label;
