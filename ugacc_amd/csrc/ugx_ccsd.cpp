@@ -238,6 +238,13 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   const auto FOV = Cn(c->fov.d(), {o, v});
   const long n4 = c->oovv_n();
 
+  // Operands of the large contractions that several of them read (Y, the
+  // pair matrices S and A, their differences) are cached like constants:
+  // each is re-converted in place, for the element types of prec, as soon as
+  // this evaluation has rebuilt it, and every later read is a cache hit.
+  const int cdt_lo = prec == P64 ? 0 : (prec == PREC_FP32 ? 1 : 2);
+  const int cdt_hi = prec == P64 ? 0 : (prec == PREC_FP32 ? 1 : 3);
+  auto rebuilt = [&](const void *p) { E.refresh(p, cdt_lo, cdt_hi); };
   // base point: keep a copy of an operand; differences from it
   auto keep = [&](DBuf &dst, const void *src, size_t bytes) {
     check(hipMemcpyAsync(dst.p, src, bytes, hipMemcpyDeviceToDevice, st), "copy");
@@ -308,10 +315,10 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   // e>=f / e>f
   auto pair_term = [&](int rows, DBuf &vp, DBuf &vm, DBuf &zp, DBuf &zm, const double *S, const double *A,
                        double beta, double *out, int pr, const char *label) {
-    E.contract("ne,qe->nq", 1.0, Cn(vp.d(), {rows, ldp}), V(S, {Q, ldp}), 0.0, V(zp.d(), {rows, Q}), pr, label);
+    E.contract("ne,qe->nq", 1.0, Cn(vp.d(), {rows, ldp}), Cn(S, {Q, ldp}), 0.0, V(zp.d(), {rows, Q}), pr, label);
     if (Qs > 0) {
       if (Ps > 0)
-        E.contract("ne,qe->nq", 1.0, Cn(vm.d(), {rows, ldm}), V(A, {Qs, ldm}), 0.0, V(zm.d(), {rows, Qs}), pr,
+        E.contract("ne,qe->nq", 1.0, Cn(vm.d(), {rows, ldm}), Cn(A, {Qs, ldm}), 0.0, V(zm.d(), {rows, Qs}), pr,
                    label);
       else
         launch_fill((long)rows * Qs, 0.0, zm.d(), st);
@@ -330,6 +337,8 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
       dif(c->rd_A, c->lad_A.d(), c->rf_A, (long)(c->lad_A.bytes / 8));
       pS = c->rd_S.d(); pA = c->rd_A.d();
     }
+    rebuilt(pS);
+    rebuilt(pA);
     if (plain)
       pair_term(o * o, c->wo_vp, c->wo_vm, c->wo_P, c->wo_M, pS, pA, 1.0, c->Woooo.d(), prec, "Woooo");
     else if (base)
@@ -349,10 +358,12 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
 
   // Y[n][f][j][b] = 0.5 t2[j][n][f][b] + t1[j][f] t1[n][b]
   launch_build_Y(o, v, t1, t2, c->Y.d(), st);
-  const auto YY = V(c->Y.d(), {o, v, o, v});
+  rebuilt(c->Y.p);
+  const auto YY = Cn(c->Y.d(), {o, v, o, v});
   if (base) keep(c->rf_Y, c->Y.p, n4 * 8);
   else if (delta) dif(c->rd_Y, c->Y.d(), c->rf_Y, n4);
-  const auto dY = V(c->rd_Y.d(), {o, v, o, v});
+  if (delta) rebuilt(c->rd_Y.p);
+  const auto dY = Cn(c->rd_Y.d(), {o, v, o, v});
   // Wmbje stored as WJ[m][e][j][b]
   const auto WJ = V(c->WJ.d(), {o, v, o, v});
   E.permute("mbje->mejb", -1.0, OVOV, 0.0, WJ);
@@ -444,10 +455,10 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
     // (tau_ij^ef = tau_ji^fe, <ab|ef> = <ba|fe>): a quarter of the flops
     // S and A (or their differences) were packed from tau above
     const int pr = base ? P64 : prec;
-    E.contract("qe,ae->qa", 1.0, V(pS, {Q, ldp}), Cn(c->lad_vp.d(), {P, ldp}), 0.0, V(c->lad_LP.d(), {Q, P}), pr,
+    E.contract("qe,ae->qa", 1.0, Cn(pS, {Q, ldp}), Cn(c->lad_vp.d(), {P, ldp}), 0.0, V(c->lad_LP.d(), {Q, P}), pr,
                "ladder");
     if (Qs > 0 && Ps > 0)
-      E.contract("qe,ae->qa", 1.0, V(pA, {Qs, ldm}), Cn(c->lad_vm.d(), {Ps, ldm}), 0.0,
+      E.contract("qe,ae->qa", 1.0, Cn(pA, {Qs, ldm}), Cn(c->lad_vm.d(), {Ps, ldm}), 0.0,
                  V(c->lad_LM.d(), {Qs, Ps}), pr, "ladder");
     launch_ladder_unpack(o, v, c->lad_LP.d(), c->lad_LM.d(), 0.25, base ? c->rs_X.d() : c->X.d(), st);
   } else {

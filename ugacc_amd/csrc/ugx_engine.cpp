@@ -92,9 +92,11 @@ void Engine::drop_cache() {
   cache_pd_.clear();
 }
 
-void Engine::refresh(const void *p) {
-  for (auto &kv : cache_pd_)
-    if (std::get<0>(kv.first) == p) launch_permute(kv.second, st_);
+void Engine::refresh(const void *p, int dmin, int dmax) {
+  for (auto &kv : cache_pd_) {
+    const int dt = std::get<2>(kv.first);
+    if (std::get<0>(kv.first) == p && dt >= dmin && dt <= dmax) launch_permute(kv.second, st_);
+  }
 }
 
 size_t Engine::cached_bytes() const {

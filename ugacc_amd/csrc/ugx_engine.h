@@ -72,9 +72,11 @@ class Engine {
   void reset_counters() { flops_ = 0.0; stats_.clear(); }
   void drop_cache();
   // Re-converts, in place, every cached conversion of the constant at p
-  // (whose contents the caller has just replaced): buffers keep their
-  // addresses, so captured graphs that read them stay valid.
-  void refresh(const void *p);
+  // with element type in [dmin, dmax] (whose contents the caller has just
+  // replaced): buffers keep their addresses, so captured graphs that read
+  // them stay valid.  A caller that replaces p's contents must refresh every
+  // type it will read before reading it.
+  void refresh(const void *p, int dmin = 0, int dmax = 3);
   size_t cached_bytes() const;
   hipStream_t stream() const { return st_; }
 
