@@ -272,8 +272,7 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
 
   // U = 2 t2 - t2(ab swapped), the spin-adapted combination used by build_t1
   // (ccwfn.cc:455,461) and the Wmbej ring term (ccwfn.cc:518-519).
-  E.permute("ijab->ijab", 2.0, T2, 0.0, V(c->U.d(), {o, o, v, v}));
-  E.permute("ijba->ijab", -1.0, T2, 1.0, V(c->U.d(), {o, o, v, v}));
+  launch_build_U(o, v, t2, c->U.d(), st);
   const auto UU = V(c->U.d(), {o, o, v, v});
   if (base) {
     keep(c->rf_t2, t2, n4 * 8); keep(c->rf_tau, tau, n4 * 8); keep(c->rf_ttau, ttau, n4 * 8);

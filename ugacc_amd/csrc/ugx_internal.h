@@ -176,6 +176,8 @@ void launch_diff(long n, const double *x, const double *y, double *z, hipStream_
 // tau = t2 + t1 (x) t1, ttau = t2 + 0.5 t1 (x) t1 (ttau may be null)
 void launch_tau(int no, int nv, const double *t1, const double *t2, double *tau, double *ttau, hipStream_t st);
 // Y[n][f][j][b] = 0.5 t2[j][n][f][b] + t1[j][f] t1[n][b]
+// U[i][j][a][b] = 2 t2[i][j][a][b] - t2[i][j][b][a]
+void launch_build_U(int no, int nv, const double *t2, double *U, hipStream_t st);
 void launch_build_Y(int no, int nv, const double *t1, const double *t2, double *Y, hipStream_t st);
 // R2 = X + X^T(ij,ab);  rsq partials of |R1|^2 + |R2|^2;  t = t_old + R/D
 // with D from the Fock diagonal eps (nact).  R1/R2 are written out when the

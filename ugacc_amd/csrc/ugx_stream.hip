@@ -200,6 +200,35 @@ __global__ void k_fill(long n, double v, double *y) {
   for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) y[i] = v;
 }
 
+// U_ijab = 2 t2_ijab - t2_ijba (ccwfn.cc:455, 518-519), one 32 x 32 (a, b)
+// tile of one (i, j) plane per block: the tile and its mirror are read
+// along b into LDS, so both global reads and the write are coalesced.
+__global__ void k_build_U(int nv, long nplane, const double *__restrict__ t2, double *__restrict__ U) {
+  __shared__ double T[32][33], Tm[32][33];
+  const int nt = (nv + 31) / 32;
+  const long ntile = nplane * nt * nt;
+  const long v2 = (long)nv * nv;
+  for (long blk = blockIdx.x; blk < ntile; blk += gridDim.x) {
+    const long ij = blk / (nt * nt);
+    const int ta = (int)((blk / nt) % nt), tb = (int)(blk % nt);
+    const double *P = t2 + ij * v2;
+    for (int e = threadIdx.x; e < 32 * 32; e += NT) {
+      const int r = e >> 5, q = e & 31;
+      const int a = ta * 32 + r, b = tb * 32 + q;     // tile (a, b)
+      const int am = tb * 32 + r, bm = ta * 32 + q;   // mirror tile (b, a), rows b
+      T[r][q] = (a < nv && b < nv) ? P[(long)a * nv + b] : 0.0;
+      Tm[r][q] = (am < nv && bm < nv) ? P[(long)am * nv + bm] : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * 32; e += NT) {
+      const int r = e >> 5, q = e & 31;
+      const int a = ta * 32 + r, b = tb * 32 + q;
+      if (a < nv && b < nv) U[ij * v2 + (long)a * nv + b] = 2.0 * T[r][q] - Tm[q][r];
+    }
+    __syncthreads();
+  }
+}
+
 // ccwfn.cc:284-298
 __global__ void k_tau(int no, int nv, const double *t1, const double *t2, double *tau, double *ttau) {
   long n = (long)no * no * nv * nv;
@@ -619,6 +648,12 @@ void launch_diff(long n, const double *x, const double *y, double *z, hipStream_
 void launch_tau(int no, int nv, const double *t1, const double *t2, double *tau, double *ttau, hipStream_t st) {
   long n = (long)no * no * nv * nv;
   hipLaunchKernelGGL(k_tau, dim3(nblocks(n)), dim3(NT), 0, st, no, nv, t1, t2, tau, ttau);
+}
+
+void launch_build_U(int no, int nv, const double *t2, double *U, hipStream_t st) {
+  const long nplane = (long)no * no, nt = (nv + 31) / 32, ntile = nplane * nt * nt;
+  if (ntile <= 0) return;
+  hipLaunchKernelGGL(k_build_U, dim3((unsigned)std::min<long>(ntile, 65536)), dim3(NT), 0, st, nv, nplane, t2, U);
 }
 
 void launch_build_Y(int no, int nv, const double *t1, const double *t2, double *Y, hipStream_t st) {
