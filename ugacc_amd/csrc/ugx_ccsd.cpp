@@ -51,6 +51,9 @@ struct ugx_ctx {
   std::vector<double> h_fov;  // host copy of f_ov for the one-electron energy
   DBuf oooo, ooov, oovv, ovov, ovvv, vvvv;
   DBuf Loovv, Looov, Lovvv;
+  // the integral parts of Wmbje and Wmbej in their [m][e][j][b] order:
+  // -<mb|je> and <mj|eb>, copied into WJ / WB at the start of each residual
+  DBuf WJ0, WB0;
   // amplitudes and intermediates
   DBuf t1, t1o, t2, t2o, tau, ttau, U, Y;
   DBuf Fvv, Foo, Fov, FaeP, FmiP;
@@ -365,7 +368,7 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   const auto dY = Cn(c->rd_Y.d(), {o, v, o, v});
   // Wmbje stored as WJ[m][e][j][b]
   const auto WJ = V(c->WJ.d(), {o, v, o, v});
-  E.permute("mbje->mejb", -1.0, OVOV, 0.0, WJ);
+  check(hipMemcpyAsync(c->WJ.p, c->WJ0.p, (size_t)n4 * 8, hipMemcpyDeviceToDevice, st), "copy");  // -<mb|je>
   E.contract("jf,mbfe->mejb", -1.0, T1, OVVV, 1.0, WJ, SP, "Wmbje");
   E.contract("nb,mnje->mejb", 1.0, T1, OOOV, 1.0, WJ, SP, "Wmbje");
   const auto RSJ = V(c->rs_WJ.d(), {o, v, o, v});
@@ -373,7 +376,7 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   add_sum(c->rs_WJ, c->WJ.d(), n4);
   // Wmbej stored as WB[m][e][j][b];  <mb|ej> = <mj|eb>
   const auto WB = V(c->WB.d(), {o, v, o, v});
-  E.permute("mjeb->mejb", 1.0, OOVV, 0.0, WB);
+  check(hipMemcpyAsync(c->WB.p, c->WB0.p, (size_t)n4 * 8, hipMemcpyDeviceToDevice, st), "copy");  // <mj|eb>
   E.contract("jf,mbef->mejb", 1.0, T1, OVVV, 1.0, WB, SP, "Wmbej");
   E.contract("nb,nmje->mejb", -1.0, T1, OOOV, 1.0, WB, SP, "Wmbej");
   const auto RSB = V(c->rs_WB.d(), {o, v, o, v});
@@ -1367,6 +1370,10 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
     c->Loovv.alloc(o * o * v * v * 8);
     E.permute("mnef->mnef", 2.0, cref(c->oovv.d(), {no, no, nv, nv}), 0.0, vref(c->Loovv.d(), {no, no, nv, nv}));
     E.permute("mnfe->mnef", -1.0, cref(c->oovv.d(), {no, no, nv, nv}), 1.0, vref(c->Loovv.d(), {no, no, nv, nv}));
+    c->WJ0.alloc(o * o * v * v * 8);
+    c->WB0.alloc(o * o * v * v * 8);
+    E.permute("mbje->mejb", -1.0, cref(c->ovov.d(), {no, nv, no, nv}), 0.0, vref(c->WJ0.d(), {no, nv, no, nv}));
+    E.permute("mjeb->mejb", 1.0, cref(c->oovv.d(), {no, no, nv, nv}), 0.0, vref(c->WB0.d(), {no, nv, no, nv}));
     // L<mn|ie> = 2<mn|ie> - <mn|ei> = 2 ooov[m][n][i][e] - ooov[n][m][i][e]
     c->Looov.alloc(o * o * o * v * 8);
     E.permute("mnie->mnie", 2.0, cref(c->ooov.d(), {no, no, no, nv}), 0.0, vref(c->Looov.d(), {no, no, no, nv}));
