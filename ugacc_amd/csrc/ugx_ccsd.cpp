@@ -625,7 +625,8 @@ int diis_step(ugx_ctx *c, int nvec, int policy, int min_vecs) {
   // dots of the new error with every stored error (slot order)
   std::vector<const double *> vecs(n);
   for (int p = 0; p < n; p++) vecs[p] = c->diis_err[p]->d();
-  int np = 512;
+  // about 64 elements per thread, between 64 and kDiisDotBlocks blocks
+  const int np = (int)std::min<long>(kDiisDotBlocks, std::max<long>(64, (len + 256 * 64 - 1) / (256 * 64)));
   launch_diis_dots(len, e.d(), vecs.data(), n, c->part.d(), np, c->st);
   std::vector<double> h((size_t)n * np);
   check(hipMemcpyAsync(h.data(), c->part.p, h.size() * 8, hipMemcpyDeviceToHost, c->st), "diis dots copy");
@@ -666,8 +667,8 @@ void alloc_work(ugx_ctx *c) {
   c->Fov.alloc(ov * 8);
   c->Woooo.alloc(o * o * o * o * 8);
   c->Z.alloc(o * v * o * o * 8); c->Zx.alloc(o * v * o * o * 8); c->P2.alloc(o * v * o * o * 8);
-  // the DIIS dots write 512 partials per stored vector
-  c->part.alloc((size_t)std::max(npart_max(c), 512 * kDiisMax) * 8);
+  // the DIIS dots write one partial per block for each stored vector
+  c->part.alloc((size_t)std::max(npart_max(c), kDiisDotBlocks * kDiisMax) * 8);
 }
 
 void guess_mp2(ugx_ctx *c) {

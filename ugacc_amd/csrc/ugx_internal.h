@@ -196,6 +196,7 @@ void launch_diis_store(long n1, long n2, const double *t1, const double *t1old, 
                        const double *t2old, double *err, double *amp, hipStream_t st);
 // partials[p*npart + blk] = sum err_new . err_p over nvec stored vectors
 constexpr int kDiisMax = 16;  // largest DIIS subspace
+constexpr int kDiisDotBlocks = 2048;  // blocks (and partials per vector) of the DIIS dots
 void launch_diis_dots(long len, const double *e, const double *const *vecs, int nvec, double *partials,
                       int npart, hipStream_t st);
 void launch_diis_combine(long n1, long n2, const double *const *amps, const double *coef, int nvec, double *t1,
