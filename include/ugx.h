@@ -15,9 +15,11 @@
  *   ugx_ccsd              <- CCWfn::CCWfn (denominators and MP2 guess,
  *                            ccwfn.cc:56-94) + CCWfn::compute_energy's
  *                            iterations, ccwfn.cc:188-231
- *   ugx_triples           <- CCWfn::tcorr_ooc_TJL, ccwfn.cc:795-852 with
- *                            W3_ijk :1388-1418 (the same number tcorr and
- *                            tcorr_ooc return, :647-793)
+ *   ugx_triples           <- CCWfn::tcorr / tcorr_ooc, ccwfn.cc:647-793 (the
+ *                            value compute_energy returns, :239-248),
+ *                            evaluated as tcorr_ooc_TJL's i>=j>=k bracket
+ *                            (:795-852, W3_ijk :1388-1418) with the f_ov t3
+ *                            terms of tcorr's X2 (:706-709) folded into V
  *   ugx_energy            <- CCWfn::energy, ccwfn.cc:260-282
  *   ugx_residual          <- build_F/build_W/build_t1/build_t2,
  *                            ccwfn.cc:311-539
@@ -71,7 +73,9 @@ enum {
   UGX_PREC_BF16X3 = 2, /* split-BF16x3 MFMA (hi*hi + hi*lo + lo*hi), FP32 accumulate */
   UGX_PREC_BF16X3_PAIR = 3, /* ugx_gemm only: BF16x3 on operands already split into BF16 hi/lo
                                halves (hi array, then lo array), A[m][k] and B[n][k] */
-  UGX_PREC_BF16 = 4          /* ugx_mlp_forward only: BF16 MFMA inference, FP32 accumulate */
+  UGX_PREC_BF16 = 4,         /* ugx_mlp_forward only: BF16 MFMA inference, FP32 accumulate */
+  UGX_PREC_BF16X3_PANEL = 5  /* ugx_gemm only: FP64 operands packed into BF16 hi/lo MFMA panels,
+                                multiplied by the LDS-copy BF16x3 kernel of the (T) W3 build */
 };
 
 /* Initial amplitudes: MP2 (ccwfn.cc:75-87), the caller's (ugx_set_amplitudes),
@@ -101,8 +105,10 @@ typedef struct {
   int maxiter;             /* MAXITER, default 100 (ugacc.cc:63) */
   int diis;                /* DIIS, default 1 (ugacc.cc:64) */
   double r_convergence;    /* R_CONVERGENCE, default 1e-7 (ugacc.cc:65) */
-  int ooc;                 /* OOC, default 0 (ugacc.cc:67); reported only, the
-                              (T) value is the same for both settings */
+  int ooc;                 /* OOC, default 0 (ugacc.cc:67); accepted for the
+                              drop-in: the reference returns tcorr (0) or
+                              tcorr_ooc (1), equal values, which ugx_triples
+                              computes either way */
   int diis_nvec;           /* subspace size, 8 (ccwfn.cc:205); at most 16 */
   int diis_min_vecs;       /* extrapolate once this many vectors are stored;
                               3 = "subspace_size() > 2" (ccwfn.cc:220) */
@@ -134,6 +140,9 @@ typedef struct {
   double seconds_ccsd;     /* wall time of the iterations */
   double seconds_triples;  /* wall time of (T) */
   double flops_ccsd;       /* algorithmic flops executed by the contractions */
+  int device_allocs;       /* device allocations made by the iterations: 0 when
+                              ugx_prepare (which ugx_ccsd calls first) left every
+                              workspace and cached conversion in place */
 } ugx_result;
 
 /* Fills *opt with the reference's defaults (ugacc.cc:58-68). */
@@ -152,12 +161,23 @@ int ugx_load_hamiltonian(ugx_ctx *ctx, int no, int nv, const double *fock, const
 /* Initial amplitudes for UGX_GUESS_USER (for example a learned guess). */
 int ugx_set_amplitudes(ugx_ctx *ctx, const double *t1, const double *t2);
 
+/* Allocates and converts, for the loaded Hamiltonian, everything a
+ * ugx_ccsd with these options needs before its first iteration: DIIS
+ * vectors, refinement and guess buffers, the contraction workspaces, the
+ * cached integral conversions of every precision the solve uses and (with
+ * autotuning on) the tuned tiles.  ugx_ccsd calls it itself when the
+ * options differ from the last prepared ones; calling it beforehand moves
+ * that one-time cost out of the solve.  (The reference allocates in the
+ * CCWfn constructor, ccwfn.cc:96-106.) */
+int ugx_prepare(ugx_ctx *ctx, const ugx_options *opt);
+
 /* Solves the CCSD equations (ccwfn.cc:188-231), then (T) when
  * opt->wfn_ccsd_t.  Returns UGX_OK or UGX_NOT_CONVERGED; res is filled in
  * either case. */
 int ugx_ccsd(ugx_ctx *ctx, const ugx_options *opt, ugx_result *res);
 
-/* (T) correction from the current amplitudes (ccwfn.cc:795-852). */
+/* (T) correction from the current amplitudes: the value of tcorr and
+ * tcorr_ooc (ccwfn.cc:647-793), f_ov terms included. */
 int ugx_triples(ugx_ctx *ctx, int precision, double *e_t);
 
 /* Per-triplet (T) energies in the i>=j>=k loop order of ccwfn.cc:810-812
@@ -193,7 +213,8 @@ size_t ugx_kernel_stats(ugx_ctx *ctx, char *buf, size_t buflen);
 
 /* General GEMM on device buffers with the library's MFMA kernel:
  * C = alpha op(A) op(B) + beta C (row-major, op 'N' or 'T'); pointers are
- * device addresses; A/B are double for UGX_PREC_FP64 and float otherwise.
+ * device addresses; A/B are double for UGX_PREC_FP64 and UGX_PREC_BF16X3_PANEL
+ * and float otherwise.
  * tile = -1 picks the tile by size.  For kernel tests and benchmarks. */
 int ugx_gemm(ugx_ctx *ctx, char opa, char opb, int M, int N, int K, double alpha, const void *A, long lda,
              const void *B, long ldb, double beta, double *C, long ldc, int precision, int tile);
@@ -211,9 +232,10 @@ typedef struct {
 /* Solves nprob independent problems of any sizes (the reference solves one
  * at a time through CCWfn::compute_energy, ccwfn.cc:188-258, and
  * tcorr_ooc_TJL, :795-852).  Each problem gets its own stream and device
- * buffers; nworkers host threads take problems largest first and run their
- * CCSD iterations concurrently (opt->graph replays each iteration's
- * residual from a hipGraph).  With opt->wfn_ccsd_t, (T) then runs once per
+ * buffers; nworkers host threads first load and prepare (ugx_prepare)
+ * every problem, then take problems largest first and run their CCSD
+ * iterations concurrently, allocating no device memory (opt->graph replays
+ * each iteration's residual from a hipGraph).  With opt->wfn_ccsd_t, (T) then runs once per
  * (no, nv) class: grouped W3 GEMMs whose pointer tables span every triplet
  * of every problem of the class, and one persistent energy kernel.
  * results[p] are those of ugx_ccsd for problem p; seconds_ccsd is the
@@ -232,6 +254,13 @@ int ugx_batch_solve(ugx_ctx *ctx, int nprob, const ugx_problem *probs, const ugx
 int ugx_set_autotune(ugx_ctx *ctx, int on);
 /* The tuned shapes as "shape -> tile ms" lines; returns the bytes needed. */
 size_t ugx_tile_table(ugx_ctx *ctx, char *buf, size_t buflen);
+/* The process's tuned tiles to / from a text file, keyed by GPU
+ * architecture, CU count and contraction shape, so a later process skips
+ * the tuning (loading merges; entries of other GPUs are kept, unused).
+ * ugx_create loads the file named by the UGX_TILE_FILE environment
+ * variable, once per process. */
+int ugx_save_tile_table(ugx_ctx *ctx, const char *path);
+int ugx_load_tile_table(ugx_ctx *ctx, const char *path);
 
 /* ---------------------------------------------------------------------
  * Learned T2 guess.  Replaces the MP2 doubles of CCWfn::CCWfn

@@ -39,6 +39,8 @@ def _lib(omp=False):
             getattr(L, fn).argtypes = [vp]
         L.orc_tcorr_tjl_range.restype = ctypes.c_double
         L.orc_tcorr_tjl_range.argtypes = [vp, ctypes.c_long, ctypes.c_long]
+        L.orc_tjl_fov_range.restype = ctypes.c_double
+        L.orc_tjl_fov_range.argtypes = [vp, ctypes.c_long, ctypes.c_long]
         L.orc_ntriplets.restype = ctypes.c_long
         L.orc_ntriplets.argtypes = [ctypes.c_int]
         L.orc_set_amps.argtypes = [vp, dp, dp]
@@ -53,6 +55,7 @@ def _lib(omp=False):
         L.orc_ccsd.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                ctypes.c_int, dp, dp]
         L.orc_tjl_per_triplet.argtypes = [vp, dp]
+        L.orc_tjl_fov_per_triplet.argtypes = [vp, dp]
         L.orc_w3.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.orc_iteration_sample.argtypes = [vp, ctypes.c_int]
         L.orc_num_threads.restype = ctypes.c_int
@@ -164,12 +167,24 @@ class Oracle:
     def tcorr_tjl_range(self, t0, t1):
         return self.L.orc_tcorr_tjl_range(self.h, t0, t1)
 
+    def tjl_fov_range(self, t0, t1):
+        """Triplets [t0, t1) of the TJL bracket with tcorr's f_ov terms
+        folded in (the full range is tcorr's value)."""
+        return self.L.orc_tjl_fov_range(self.h, t0, t1)
+
     def ntriplets(self):
         return self.L.orc_ntriplets(self.no)
 
     def tjl_per_triplet(self):
         e = np.zeros(self.ntriplets())
         self.L.orc_tjl_per_triplet(self.h, _p(e))
+        return e
+
+    def tjl_fov_per_triplet(self):
+        """Per-triplet TJL energies with tcorr's f_ov t3 terms folded into V
+        (their sum is tcorr's value for any f_ov)."""
+        e = np.zeros(self.ntriplets())
+        self.L.orc_tjl_fov_per_triplet(self.h, _p(e))
         return e
 
     def w3(self, i, j, k):

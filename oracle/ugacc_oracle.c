@@ -735,8 +735,14 @@ double orc_tcorr_ooc(void *p) {
 }
 
 /* Energy of one triplet (i >= j >= k) in ccwfn.cc:795-852, W3 from
- * W3_ijk (:1388-1418).  Buffers W3, V3 are nv^3 each. */
-static double tjl_triplet(const orc_state *s, int i, int j, int k, double *W3, double *V3) {
+ * W3_ijk (:1388-1418).  Buffers W3, V3 are nv^3 each.
+ * fov != 0 adds t2_ij^ab f_kc + t2_ik^ac f_jb + t2_jk^bc f_ia to V: the
+ * f_kc t3 part of tcorr's X2 (ccwfn.cc:706-709), which the TJL routine
+ * omits, has exactly the form of the <ij|ab> t1 disconnected term with
+ * <pq|xy> -> t2_pq^xy and t1 -> f_ov (both have the pair symmetry
+ * X_pq^xy = X_qp^yx).  With it the sum over triplets is the value tcorr
+ * and tcorr_ooc return (pinned in tests/test_oracle_pins.py). */
+static double tjl_triplet_x(const orc_state *s, int i, int j, int k, double *W3, double *V3, int fov) {
   int no = s->no, nv = s->nv;
   const double *t1 = s->t1, *t2 = s->t2;
   for (int a = 0; a < nv; a++)
@@ -744,10 +750,14 @@ static double tjl_triplet(const orc_state *s, int i, int j, int k, double *W3, d
       for (int c = 0; c < nv; c++) W3[V3(a, b, c)] = w3_elem(s, t2, i, j, k, a, b, c);
   for (int a = 0; a < nv; a++)
     for (int b = 0; b < nv; b++)
-      for (int c = 0; c < nv; c++)
-        V3[V3(a, b, c)] = (W3[V3(a, b, c)] + INT(i, j, a + no, b + no) * t1[O2(k, c)] +
-                           INT(i, k, a + no, c + no) * t1[O2(j, b)] + INT(j, k, b + no, c + no) * t1[O2(i, a)]) /
-                          (1.0 + (a == b) + (a == c) + (b == c));
+      for (int c = 0; c < nv; c++) {
+        double d = INT(i, j, a + no, b + no) * t1[O2(k, c)] + INT(i, k, a + no, c + no) * t1[O2(j, b)] +
+                   INT(j, k, b + no, c + no) * t1[O2(i, a)];
+        if (fov)
+          d += t2[T4(i, j, a, b)] * F(k, c + no) + t2[T4(i, k, a, c)] * F(j, b + no) +
+               t2[T4(j, k, b, c)] * F(i, a + no);
+        V3[V3(a, b, c)] = (W3[V3(a, b, c)] + d) / (1.0 + (a == b) + (a == c) + (b == c));
+      }
   double et = 0.0;
   for (int a = 0; a < nv; a++)
     for (int b = 0; b <= a; b++)
@@ -766,6 +776,10 @@ static double tjl_triplet(const orc_state *s, int i, int j, int k, double *W3, d
   return et;
 }
 
+static double tjl_triplet(const orc_state *s, int i, int j, int k, double *W3, double *V3) {
+  return tjl_triplet_x(s, i, j, k, W3, V3, 0);
+}
+
 /* Number of i >= j >= k triplets, o(o+1)(o+2)/6. */
 long orc_ntriplets(int no) { return (long)no * (no + 1) * (no + 2) / 6; }
 
@@ -781,8 +795,9 @@ static void triplet_of(long t, int *pi, int *pj, int *pk) {
 }
 
 /* ccwfn.cc:795-852 over triplets [t0, t1) of the i>=j>=k order (the full
- * range gives tcorr_ooc_TJL; sub-ranges serve the bounded CPU sample). */
-double orc_tcorr_tjl_range(void *p, long t0, long t1) {
+ * range gives tcorr_ooc_TJL; sub-ranges serve the bounded CPU sample);
+ * fov != 0 folds tcorr's f_ov t3 terms in (tjl_triplet_x). */
+static double tjl_range(void *p, long t0, long t1, int fov) {
   orc_state *s = (orc_state *)p;
   size_t vvv = (size_t)s->nv * s->nv * s->nv;
   double et = 0.0;
@@ -793,12 +808,18 @@ double orc_tcorr_tjl_range(void *p, long t0, long t1) {
     for (long t = t0; t < t1; t++) {
       int i, j, k;
       triplet_of(t, &i, &j, &k);
-      et += tjl_triplet(s, i, j, k, W3, V3b);
+      et += tjl_triplet_x(s, i, j, k, W3, V3b, fov);
     }
     free(W3); free(V3b);
   }
   return et;
 }
+
+double orc_tcorr_tjl_range(void *p, long t0, long t1) { return tjl_range(p, t0, t1, 0); }
+
+/* Triplets [t0, t1) of the folded bracket: the full range gives tcorr's
+ * value (ccwfn.cc:647-743) for any f_ov. */
+double orc_tjl_fov_range(void *p, long t0, long t1) { return tjl_range(p, t0, t1, 1); }
 
 double orc_tcorr_tjl(void *p) {
   orc_state *s = (orc_state *)p;
@@ -818,6 +839,25 @@ void orc_tjl_per_triplet(void *p, double *e_out) {
     e_out[t] = tjl_triplet(s, i, j, k, W3, V3b);
   }
   free(W3); free(V3b);
+}
+
+/* Per-triplet energies of the TJL bracket with the f_ov t3 terms folded in
+ * (tjl_triplet_x with fov): their sum is tcorr (ccwfn.cc:647-743). */
+void orc_tjl_fov_per_triplet(void *p, double *e_out) {
+  orc_state *s = (orc_state *)p;
+  size_t vvv = (size_t)s->nv * s->nv * s->nv;
+  long nt = orc_ntriplets(s->no);
+#pragma omp parallel
+  {
+    double *W3 = zalloc(vvv), *V3b = zalloc(vvv);
+#pragma omp for schedule(dynamic, 1)
+    for (long t = 0; t < nt; t++) {
+      int i, j, k;
+      triplet_of(t, &i, &j, &k);
+      e_out[t] = tjl_triplet_x(s, i, j, k, W3, V3b, 1);
+    }
+    free(W3); free(V3b);
+  }
 }
 
 /* W3_ijk for one triplet (ccwfn.cc:1388-1418), for kernel parity. */

@@ -68,16 +68,18 @@ def test_ccsd_and_triples_parity(case):
     t1, t2 = c.amplitudes()
     assert np.max(np.abs(t1 - t1o)) < 1e-9
     assert np.max(np.abs(t2 - t2o)) < 1e-9
-    # (T) from the same (oracle) amplitudes: per-triplet and total
+    # (T) from the same (oracle) amplitudes: per-triplet and total.  The
+    # value the reference returns is tcorr (OOC false) or tcorr_ooc (OOC
+    # true), f_ov terms included (the fov case); per triplet it is the TJL
+    # bracket with those terms folded in (equal to plain TJL when f_ov = 0)
     c.set_amplitudes(t1o, t2o)
-    et_o = orc.tcorr_tjl()
-    per_o = orc.tjl_per_triplet()
+    et_ref = orc.tcorr()
+    per_o = orc.tjl_fov_per_triplet()
     per_g = c.triples_per_triplet()
     assert np.max(np.abs(per_g - per_o)) < 1e-13
-    assert abs(c.triples() - et_o) < 1e-12
-    if not case.get("offdiag") and not case.get("fov"):
-        # the three reference (T) algorithms agree for canonical orbitals
-        assert abs(et_o - orc.tcorr_ooc()) < 1e-12
+    et_g = c.triples()
+    assert abs(et_g - et_ref) < 1e-12
+    assert abs(et_g - orc.tcorr_ooc()) < 1e-12
     W_o = orc.w3(no - 1, no // 2, 0)
     W_g = c.w3(no - 1, no // 2, 0)
     assert np.max(np.abs(W_g - W_o)) < 1e-12 * max(1.0, np.max(np.abs(W_o)))

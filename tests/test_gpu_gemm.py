@@ -69,3 +69,47 @@ def test_gemm_tiles(tile, layout, prec):
     c.synchronize()
     tol = 1e-12 if prec == ugx.PREC_FP64 else 1e-4
     np.testing.assert_allclose(dC.cpu().numpy(), A @ B.T, rtol=0, atol=tol)
+
+
+@pytest.mark.parametrize("form", ["pair", "panel"])
+@pytest.mark.parametrize("layout", ["NT", "TN"])
+@pytest.mark.parametrize("tile", range(12))
+@pytest.mark.parametrize("K", [152, 64])
+def test_gemm_bf16x3_tiles(tile, layout, form, K):
+    """Every tile of the two BF16x3 kernels against a float64 product:
+    "pair" splits FP32 operands into BF16 hi/lo arrays (k_gemm_b3p, the
+    CCSD contractions' register-staged kernel); "panel" packs FP64 operands
+    into MFMA panels and runs the LDS-copy kernel of the (T) W3 build
+    (k_gemm_b3d; tiles whose block copies do not split over the four waves
+    take k_gemm_b3p on the same panels).  K = 152 ends in a partial 32-deep
+    slice (zero padding), as the (T) W3's K = v does at (H2O)8; M and N are
+    not multiples of the tiles.  BF16x3 drops only the lo*lo product
+    (~2^-16 relative per product), well inside 2e-5 of sum |a||b|."""
+    from tests.gpu_util import context, announce
+    announce("gemm bf16x3 %s tile=%d %s K=%d" % (form, tile, layout, K))
+    c = context()
+    rng = np.random.default_rng(1000 + tile * 7 + K)
+    M, N = 203, 170
+    A = rng.standard_normal((M, K))
+    B = rng.standard_normal((N, K))
+    C0 = rng.standard_normal((M, N))
+    tdt = torch.float32 if form == "pair" else torch.float64
+    prec = ugx.PREC_BF16X3 if form == "pair" else ugx.PREC_BF16X3_PANEL
+    if form == "pair":
+        A, B = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
+    dC = _dev(C0, torch.float64)
+    if layout == "NT":
+        dA, dB = _dev(A, tdt), _dev(B, tdt)
+        torch.cuda.synchronize()
+        c.gemm("N", "T", M, N, K, 0.5, dA.data_ptr(), K, dB.data_ptr(), K, 2.0, dC.data_ptr(), N, precision=prec,
+               tile=tile)
+    else:
+        dA, dB = _dev(A.T.copy(), tdt), _dev(B.T.copy(), tdt)
+        torch.cuda.synchronize()
+        c.gemm("T", "N", M, N, K, 0.5, dA.data_ptr(), M, dB.data_ptr(), N, 2.0, dC.data_ptr(), N, precision=prec,
+               tile=tile)
+    c.synchronize()
+    ref = 0.5 * A @ B.T + 2.0 * C0
+    scale = 0.5 * np.abs(A) @ np.abs(B).T + 1.0
+    err = np.max(np.abs(dC.cpu().numpy() - ref) / scale)
+    assert err < 2e-5, err

@@ -113,3 +113,27 @@ def test_amplitude_symmetry():
     o, _ = _ccsd(fock, ints, 3)
     _, t2 = o.amps()
     assert np.max(np.abs(t2 - t2.transpose(1, 0, 3, 2))) < 1e-12
+
+
+@pytest.mark.parametrize("no,nv,seed,fov,offdiag", [(3, 7, 5, 0.01, 0.0), (3, 6, 9, 0.02, 0.01), (4, 5, 2, 0.03, 0.0)])
+def test_triples_fov_terms_fold_into_the_bracket(no, nv, seed, fov, offdiag):
+    """With f_ov != 0 the reference returns tcorr (OOC false) or tcorr_ooc
+    (OOC true), whose X2 carries f_kc t3 terms (ccwfn.cc:706-709, 725) that
+    tcorr_ooc_TJL lacks.  The TJL bracket with t2_pq^xy f_r^z terms added to
+    V (the form the GPU kernel evaluates) reproduces both reference values;
+    the plain TJL value visibly differs."""
+    fock, ints = random_hamiltonian(no, nv, seed=seed, fov=fov, offdiag=offdiag)
+    o, _ = _ccsd(fock, ints, no, conv=1e-11)
+    a, b = o.tcorr(), o.tcorr_ooc()
+    per = o.tjl_fov_per_triplet()
+    scale = abs(a)
+    assert abs(a - b) < 1e-12 * scale
+    assert abs(per.sum() - a) < 1e-12 * scale
+    assert abs(o.tcorr_tjl() - a) > 1e-3 * scale
+    # with f_ov = 0 the folded and plain per-triplet energies coincide
+    f0 = fock.copy()
+    f0[:no, no:] = 0.0
+    f0[no:, :no] = 0.0
+    o0 = Oracle(f0, ints, no)
+    o0.set_amps(*o.amps())
+    np.testing.assert_allclose(o0.tjl_fov_per_triplet(), o0.tjl_per_triplet(), rtol=0, atol=1e-18)

@@ -21,11 +21,13 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
 #include <map>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -49,6 +51,7 @@ struct ugx_ctx {
   // Hamiltonian (device, constant)
   DBuf fock, eps, fov, fvo, foo, fvv;
   std::vector<double> h_fov;  // host copy of f_ov for the one-electron energy
+  bool has_fov = false;       // some f_ov element is non-zero
   DBuf oooo, ooov, oovv, ovov, ovvv, vvvv;
   DBuf Loovv, Looov, Lovvv;
   // the integral parts of Wmbje and Wmbej in their [m][e][j][b] order:
@@ -114,6 +117,9 @@ struct ugx_ctx {
   };
   std::vector<ResidualGraph> graphs;
   int graph_eager[8] = {};  // eager residuals run per (precision, mode) since the last reset
+  // options signature the context's workspaces, cached conversions and
+  // buffers were prepared for (prepare_solve); empty after a load
+  std::string prepared_for;
   // residual of the last ugx_residual, for ugx_intermediates
   bool have_intermediates = false;
   double last_flops = 0.0;
@@ -592,8 +598,10 @@ int lin_solve(std::vector<double> &A, std::vector<double> &b, int n) {
   return 0;
 }
 
-int diis_step(ugx_ctx *c, int nvec, int policy, int min_vecs) {
-  const long n1 = c->ov(), n2 = c->oovv_n(), len = n1 + n2;
+// Error and amplitude vectors of an nvec-vector subspace over (t1 | t2),
+// allocated once per Hamiltonian and subspace size.
+void diis_alloc(ugx_ctx *c, int nvec) {
+  const long len = c->ov() + c->oovv_n();
   if (c->diis_nvec != nvec || (int)c->diis_err.size() != nvec) {
     c->diis_err.clear();
     c->diis_amp.clear();
@@ -604,8 +612,17 @@ int diis_step(ugx_ctx *c, int nvec, int policy, int min_vecs) {
     c->diis_nvec = nvec;
     c->diis_B.assign((size_t)nvec * nvec, 0.0);
     c->diis_norm.assign(nvec, 0.0);
-    diis_reset(c);
   }
+  for (int p = 0; p < nvec; p++) {
+    c->diis_err[p]->alloc(len * 8);
+    c->diis_amp[p]->alloc(len * 8);
+  }
+  diis_reset(c);
+}
+
+int diis_step(ugx_ctx *c, int nvec, int policy, int min_vecs) {
+  const long n1 = c->ov(), n2 = c->oovv_n(), len = n1 + n2;
+  if (c->diis_nvec != nvec || (int)c->diis_err.size() != nvec) diis_alloc(c, nvec);
   int slot;
   if (c->diis_count < nvec) slot = c->diis_count;
   else if (policy == 0) slot = c->diis_next;
@@ -616,8 +633,6 @@ int diis_step(ugx_ctx *c, int nvec, int policy, int min_vecs) {
   }
   DBuf &e = *c->diis_err[slot];
   DBuf &x = *c->diis_amp[slot];
-  e.alloc(len * 8);
-  x.alloc(len * 8);
   launch_diis_store(n1, n2, c->t1.d(), c->t1o.d(), c->t2.d(), c->t2o.d(), e.d(), x.d(), c->st);
   if (c->diis_count < nvec) c->diis_count++;
   c->diis_next = (slot + 1) % nvec;
@@ -669,6 +684,46 @@ void alloc_work(ugx_ctx *c) {
   c->Z.alloc(o * v * o * o * 8); c->Zx.alloc(o * v * o * o * 8); c->P2.alloc(o * v * o * o * 8);
   // the DIIS dots write one partial per block for each stored vector
   c->part.alloc((size_t)std::max(npart_max(c), kDiisDotBlocks * kDiisMax) * 8);
+}
+
+// Everything a ugx_ccsd with options opt needs, allocated and converted
+// before its iterations: the DIIS vectors, the refinement buffers, the
+// learned guess's output, and the engine's workspaces, cached integral
+// conversions and tuned tiles for each residual form the solve evaluates
+// (one evaluation of each in the engine's sizing mode, which runs no
+// contraction; the streaming kernels of those evaluations write work
+// buffers that every real evaluation rewrites).  Done once per Hamiltonian
+// and options signature.
+void prepare_solve(ugx_ctx *c, const ugx_options &opt) {
+  const int prec = opt.precision;
+  const bool refine = prec != PREC_FP64 && opt.polish_mode == UGX_POLISH_REFINE;
+  const std::string key = std::to_string(prec) + "/" + std::to_string(refine) + "/" +
+                          std::to_string(opt.diis ? opt.diis_nvec : 0) + "/" + std::to_string(opt.guess) + "/" +
+                          std::to_string(c->eng->autotune());
+  if (c->prepared_for == key) return;
+  if (opt.diis) diis_alloc(c, opt.diis_nvec);
+  if (opt.guess == UGX_GUESS_MLP) c->G.alloc((size_t)c->oovv_n() * 4);
+  if (refine) ensure_refine(c);
+  Engine &E = *c->eng;
+  E.set_sizing(true);
+  try {
+    residual(c, prec, RES_PLAIN);
+    if (prec != PREC_FP64) {
+      if (refine) {
+        residual(c, PREC_FP64, RES_BASE);
+        residual(c, prec, RES_DELTA);
+      } else {
+        residual(c, PREC_FP64, RES_PLAIN);
+      }
+    }
+  } catch (...) {
+    E.set_sizing(false);
+    throw;
+  }
+  E.set_sizing(false);
+  check(hipStreamSynchronize(c->st), "stream sync");
+  (void)E.take_allocations();
+  c->prepared_for = key;
 }
 
 void guess_mp2(ugx_ctx *c) {
@@ -1149,6 +1204,11 @@ std::vector<double> triples_entries(ugx_ctx *c, int prec, const std::vector<TEnt
   c->ttrip.alloc(t4.size() * 4);
   check(hipMemcpyAsync(c->ttrip.p, t4.data(), t4.size() * 4, hipMemcpyHostToDevice, c->st), "entry table");
   c->tCounter.alloc(64);
+  // the f_ov t3 terms of the reference's tcorr (ccwfn.cc:706-709) whenever
+  // any entry's problem has a non-zero occupied-virtual Fock element
+  int fov = 0;
+  for (const TEnt &en : ents)
+    if (en.src->has_fov) { fov = 1; break; }
   for (long t0 = 0; t0 < nt; t0 += nb) {
     int b = (int)std::min<long>(nb, nt - t0);
     if (prec == PREC_BF16X3 && c->t_dma > 1) triples_build_w_dma(c, ents.data() + t0, b);
@@ -1156,7 +1216,7 @@ std::vector<double> triples_entries(ugx_ctx *c, int prec, const std::vector<TEnt
     else triples_build_w(c, prec, ents.data() + t0, b);
     c->eng->timed("triples_e", 0.0, [&]() {
       launch_triples_energy(o, v, b, (const int *)c->ttrip.p + 4 * t0, (const TripProb *)c->tProb.p, c->tWbuf.p, wf,
-                            ws, c->tE.d() + t0, c->tPart.d(), (int *)c->tCounter.p, c->st);
+                            ws, fov, c->tE.d() + t0, c->tPart.d(), (int *)c->tCounter.p, c->st);
     });
   }
   check(hipMemcpyAsync(e.data(), c->tE.p, nt * 8, hipMemcpyDeviceToHost, c->st), "triples copy");
@@ -1173,7 +1233,7 @@ double triples(ugx_ctx *c, int prec, std::vector<double> *etrip_out) {
   std::vector<TEnt> ents(nt);
   for (long t = 0; t < nt; t++) ents[t] = TEnt{c, c->trip_host[3 * t], c->trip_host[3 * t + 1], c->trip_host[3 * t + 2]};
   std::vector<int> prob(nt, 0);
-  std::vector<double> e = triples_entries(c, prec, ents, prob, {TripProb{c->eps.d(), c->oovv.d(), c->t1.d()}});
+  std::vector<double> e = triples_entries(c, prec, ents, prob, {TripProb{c->eps.d(), c->oovv.d(), c->t1.d(), c->t2.d(), c->fov.d()}});
   double et = 0.0;
   for (double x : e) et += x;
   if (etrip_out) {
@@ -1192,6 +1252,19 @@ double triples(ugx_ctx *c, int prec, std::vector<double> *etrip_out) {
 double triples_flops(int o, int v) {
   double ntrip = (double)o * (o + 1) * (o + 2) / 6.0 - o;
   return 12.0 * ntrip * ((double)v * v * v * v + (double)o * v * v * v);
+}
+
+void validate_options(ugx_ctx *c, const ugx_options &opt, const char *who) {
+  const std::string w(who);
+  if (opt.maxiter < 0 || opt.diis_nvec < 1 || opt.diis_nvec > kDiisMax || opt.diis_min_vecs < 1 ||
+      opt.precision < 0 || opt.precision > 2 || !(opt.r_convergence > 0) ||
+      (opt.polish_mode != UGX_POLISH_PLAIN && opt.polish_mode != UGX_POLISH_REFINE))
+    throw UgxError(UGX_BAD_INPUT, w + ": invalid options");
+  if (opt.guess < UGX_GUESS_MP2 || opt.guess > UGX_GUESS_MLP) throw UgxError(UGX_BAD_INPUT, w + ": unknown guess");
+  if (opt.guess == UGX_GUESS_USER && !c->have_amps)
+    throw UgxError(UGX_BAD_INPUT, w + ": UGX_GUESS_USER without ugx_set_amplitudes");
+  if (opt.guess == UGX_GUESS_MLP && !c->mlp)
+    throw UgxError(UGX_BAD_INPUT, w + ": UGX_GUESS_MLP without ugx_set_guess_network");
 }
 
 template <typename F>
@@ -1268,6 +1341,11 @@ int ugx_create(ugx_ctx **out, int device) {
   if (const char *e = std::getenv("UGX_T_WGIB")) c->t_wbytes = std::max(1L, std::atol(e)) << 30;
   if (const char *e = std::getenv("UGX_LADDER")) c->lad_full = std::string(e) == "full";
   if (c->t_dma != 0 && c->t_dma != 2 && c->t_dma != 3) c->t_dma = 0;
+  // a tuned tile table to start from (once per process)
+  static std::once_flag tiles_once;
+  std::call_once(tiles_once, []() {
+    if (const char *e = std::getenv("UGX_TILE_FILE")) (void)tiles_load(e);
+  });
   *out = c;
   return UGX_OK;
 }
@@ -1301,6 +1379,7 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
   c->b_P.release(); c->b_Q.release(); c->b_OVt.release();
   c->p_OV.release(); c->p_t2.release(); c->p_T2t.release(); c->p_T2tq.release();
     c->diis_err.clear(); c->diis_amp.clear(); c->diis_nvec = 0;
+    c->prepared_for.clear();
     c->no = no; c->nv = nv;
     const long n = no + nv, o = no, v = nv;
     for (long k = 0; k < n * n; k++)
@@ -1318,6 +1397,9 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
     upload(c->eps, eps.data(), n, c->st);
     upload(c->fov, fov.data(), o * v, c->st);
     c->h_fov = fov;
+    c->has_fov = false;
+    for (double f : fov)
+      if (f != 0.0) c->has_fov = true;
     upload(c->fvo, fvo.data(), o * v, c->st);
     upload(c->foo, foo.data(), o * o, c->st);
     upload(c->fvv, fvv.data(), v * v, c->st);
@@ -1327,6 +1409,21 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
     upload(c->ovov, ovov, o * v * o * v, c->st);
     upload(c->ovvv, ovvv, o * v * v * v, c->st);
     upload(c->vvvv, vvvv, v * v * v * v, c->st);
+    // non-finite integrals would only surface as a non-finite residual
+    // norm iterations later: one scan of the blocks on the device
+    {
+      DBuf flag;
+      flag.alloc(8);
+      launch_fill(1, 0.0, flag.d(), c->st);
+      const std::pair<const DBuf *, long> blk[] = {{&c->oooo, o * o * o * o}, {&c->ooov, o * o * o * v},
+                                                   {&c->oovv, o * o * v * v}, {&c->ovov, o * v * o * v},
+                                                   {&c->ovvv, o * v * v * v}, {&c->vvvv, v * v * v * v}};
+      for (auto &b : blk) launch_nonfinite(b.second, b.first->d(), flag.d(), c->st);
+      double h = 0.0;
+      check(hipMemcpyAsync(&h, flag.p, 8, hipMemcpyDeviceToHost, c->st), "flag copy");
+      check(hipStreamSynchronize(c->st), "stream sync");
+      if (h != 0.0) throw UgxError(UGX_NONFINITE, "non-finite two-electron integral");
+    }
     Engine &E = *c->eng;
     // The packed ladder needs <ab|ef> = <ba|fe> (real orbitals), checked on
     // a deterministic sample of the block, and grids within limits;
@@ -1422,24 +1519,27 @@ int ugx_energy(ugx_ctx *c, int form, double *e) {
   });
 }
 
+int ugx_prepare(ugx_ctx *c, const ugx_options *opt_in) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    ugx_options opt;
+    if (opt_in) opt = *opt_in; else ugx_default_options(&opt);
+    validate_options(c, opt, "ugx_prepare");
+    prepare_solve(c, opt);
+    return UGX_OK;
+  });
+}
+
 int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
   return guarded(c, [&]() {
     require_loaded(c);
     ugx_options opt;
     if (opt_in) opt = *opt_in; else ugx_default_options(&opt);
-    if (opt.maxiter < 0 || opt.diis_nvec < 1 || opt.diis_nvec > kDiisMax || opt.diis_min_vecs < 1 ||
-        opt.precision < 0 || opt.precision > 2 || !(opt.r_convergence > 0) ||
-        (opt.polish_mode != UGX_POLISH_PLAIN && opt.polish_mode != UGX_POLISH_REFINE))
-      throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: invalid options");
-    if (opt.guess < UGX_GUESS_MP2 || opt.guess > UGX_GUESS_MLP)
-      throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: unknown guess");
-    if (opt.guess == UGX_GUESS_USER && !c->have_amps)
-      throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: UGX_GUESS_USER without ugx_set_amplitudes");
-    if (opt.guess == UGX_GUESS_MLP && !c->mlp)
-      throw UgxError(UGX_BAD_INPUT, "ugx_ccsd: UGX_GUESS_MLP without ugx_set_guess_network");
+    validate_options(c, opt, "ugx_ccsd");
     ugx_result r;
     std::memset(&r, 0, sizeof(r));
     Engine &E = *c->eng;
+    prepare_solve(c, opt);
     E.reset_counters();
     auto t0 = std::chrono::steady_clock::now();
     if (opt.guess == UGX_GUESS_MP2) guess_mp2(c);
@@ -1451,7 +1551,7 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
       std::printf("\t Iter   Correlation Energy   T1 Norm      RMS   \n\t  %3d  %20.15f\n", 0, emp2);
     double eccsd = emp2, rms = 0.0;
     int prec = opt.precision;
-    c->diis_err.clear(); c->diis_amp.clear(); c->diis_nvec = 0;
+    diis_reset(c);
     int iter, niter = 0, nlow = 0, low_stall = 0;
     double low_best = 1e300;
     // mode of the next residual; polished: the residuals are FP64-accurate
@@ -1511,6 +1611,7 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
     r.niter_low = nlow;
     r.seconds_ccsd = std::chrono::duration<double>(t1c - t0).count();
     r.flops_ccsd = E.flops_total();
+    r.device_allocs = (int)std::min<long>(E.take_allocations(), 1L << 30);
     E.collect_stats();
     const bool conv = (rms < opt.r_convergence);
     if (conv && opt.wfn_ccsd_t) {
@@ -1667,7 +1768,8 @@ size_t ugx_kernel_stats(ugx_ctx *c, char *buf, size_t buflen) {
 int ugx_gemm(ugx_ctx *c, char opa, char opb, int M, int N, int K, double alpha, const void *A, long lda,
              const void *B, long ldb, double beta, double *C, long ldc, int precision, int tile) {
   return guarded(c, [&]() {
-    if (M < 0 || N < 0 || K < 0 || !A || !B || !C || precision < 0 || precision > 3 || tile >= gemm_num_tiles())
+    if (M < 0 || N < 0 || K < 0 || !A || !B || !C || precision < 0 || precision > UGX_PREC_BF16X3_PANEL ||
+        precision == UGX_PREC_BF16 || tile >= gemm_num_tiles())
       throw UgxError(UGX_BAD_INPUT, "ugx_gemm: bad arguments");
     if (precision == UGX_PREC_BF16X3_PAIR && !((opa == 'N' || opa == 'n') && (opb == 'T' || opb == 't')))
       throw UgxError(UGX_BAD_INPUT, "ugx_gemm: BF16 pair operands must be A[m][k] and B[n][k] (opa N, opb T)");
@@ -1682,6 +1784,26 @@ int ugx_gemm(ugx_ctx *c, char opa, char opb, int M, int N, int K, double alpha, 
     if (precision == UGX_PREC_BF16X3_PAIR) {
       g.loA = (long)M * lda; g.loB = (long)N * ldb;
       launch_gemm(d, nullptr, PREC_BF16X3_PRESPLIT, tile, c->st);
+    } else if (precision == UGX_PREC_BF16X3_PANEL) {
+      // FP64 operands in any layout packed into BF16 hi/lo MFMA panels
+      // (A as [m][k], B as [n][k]), then the LDS-copy kernel of the (T) W3
+      // build (tiles whose block copies do not split over the four waves
+      // take the register-staged kernel on the same panels)
+      if (M == 0 || N == 0) return UGX_OK;
+      const long Kp = (K + 31) / 32 * 32;
+      DBuf pa, pb;
+      pa.alloc((size_t)panel_size(1, M, K) * 4);
+      pb.alloc((size_t)panel_size(1, N, K) * 4);
+      launch_pack_panel((const double *)A, 0, an ? lda : 1, an ? 1 : lda, 1, M, K, (unsigned short *)pa.p, c->st);
+      launch_pack_panel((const double *)B, 0, bn ? 1 : ldb, bn ? ldb : 1, 1, N, K, (unsigned short *)pb.p, c->st);
+      g.A = pa.p; g.B = pb.p;
+      g.sam = BK_PANEL; g.sak = 1; g.sbn = BK_PANEL; g.sbk = 1;
+      g.loA = panel_size(1, M, K); g.loB = panel_size(1, N, K);
+      g.pblkA = 16 * Kp; g.pblkB = 16 * Kp;
+      d.dma = 2;
+      d.vec_ok = 1;
+      launch_gemm(d, nullptr, PREC_BF16X3_PRESPLIT, tile, c->st);
+      check(hipStreamSynchronize(c->st), "stream sync");
     } else if (precision == UGX_PREC_BF16X3) {
       // FP32 operands in any layout: split into BF16 hi/lo pairs A[m][k],
       // B[n][k] first
@@ -1753,9 +1875,13 @@ int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_o
       return cost(a) > cost(b);
     });
     std::vector<int> rc(nprob, UGX_OK);
-    std::vector<double> t_ccsd(nprob, 0.0);
+    std::vector<double> t_ccsd(nprob, 0.0), t_load(nprob, 0.0);
     std::atomic<int> next(0);
-    auto worker = [&]() {
+    // Phase 1 loads and prepares every problem (all device allocation of
+    // the CCSD happens here); phase 2 iterates and allocates nothing, so no
+    // worker allocates or frees device memory while another captures or
+    // replays its graphs.
+    auto loader = [&]() {
       for (;;) {
         const int q = next.fetch_add(1);
         if (q >= nprob) break;
@@ -1764,21 +1890,37 @@ int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_o
         auto t0 = std::chrono::steady_clock::now();
         int r = ugx_load_hamiltonian(sub[p].get(), pr.no, pr.nv, pr.fock, pr.oooo, pr.ooov, pr.oovv, pr.ovov,
                                      pr.ovvv, pr.vvvv);
-        if (r == UGX_OK) r = ugx_ccsd(sub[p].get(), &opt, &results[p]);
-        t_ccsd[p] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (r == UGX_OK) r = ugx_prepare(sub[p].get(), &opt);
+        t_load[p] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         rc[p] = r;
       }
     };
-    std::vector<std::thread> pool;
+    auto solver = [&]() {
+      for (;;) {
+        const int q = next.fetch_add(1);
+        if (q >= nprob) break;
+        const int p = order[q];
+        if (rc[p] != UGX_OK) continue;
+        auto t0 = std::chrono::steady_clock::now();
+        rc[p] = ugx_ccsd(sub[p].get(), &opt, &results[p]);
+        t_ccsd[p] = t_load[p] + std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      }
+    };
     const int nthr = std::min(nworkers, nprob);
-    try {
-      for (int w = 0; w < nthr; w++) pool.emplace_back(worker);
-    } catch (...) {
-      next.store(nprob);  // the started workers stop after their current problem
+    auto run_pool = [&](const std::function<void()> &fn) {
+      std::vector<std::thread> pool;
+      next.store(0);
+      try {
+        for (int w = 0; w < nthr; w++) pool.emplace_back(fn);
+      } catch (...) {
+        next.store(nprob);  // the started workers stop after their current problem
+        for (auto &th : pool) th.join();
+        throw;
+      }
       for (auto &th : pool) th.join();
-      throw;
-    }
-    for (auto &th : pool) th.join();
+    };
+    run_pool(loader);
+    run_pool(solver);
     int status = UGX_OK;
     for (int p = 0; p < nprob; p++) {
       if (rc[p] == UGX_NOT_CONVERGED) status = UGX_NOT_CONVERGED;
@@ -1805,7 +1947,7 @@ int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_o
         std::vector<TripProb> tp;
         for (size_t q = 0; q < mem.size(); q++) {
           ugx_ctx *s = sub[mem[q]].get();
-          tp.push_back(TripProb{s->eps.d(), s->oovv.d(), s->t1.d()});
+          tp.push_back(TripProb{s->eps.d(), s->oovv.d(), s->t1.d(), s->t2.d(), s->fov.d()});
           const std::vector<int> &tr = s->trip_host;
           for (size_t t = 0; t + 2 < tr.size(); t += 3) {
             ents.push_back(TEnt{s, tr[t], tr[t + 1], tr[t + 2]});
@@ -1828,6 +1970,22 @@ int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_o
 int ugx_set_autotune(ugx_ctx *c, int on) {
   return guarded(c, [&]() {
     c->eng->set_autotune(on != 0);
+    return UGX_OK;
+  });
+}
+
+int ugx_save_tile_table(ugx_ctx *c, const char *path) {
+  return guarded(c, [&]() {
+    if (!path) throw UgxError(UGX_BAD_INPUT, "ugx_save_tile_table: null path");
+    if (!tiles_save(path)) throw UgxError(UGX_BAD_INPUT, std::string("cannot write the tile table to ") + path);
+    return UGX_OK;
+  });
+}
+
+int ugx_load_tile_table(ugx_ctx *c, const char *path) {
+  return guarded(c, [&]() {
+    if (!path) throw UgxError(UGX_BAD_INPUT, "ugx_load_tile_table: null path");
+    if (tiles_load(path) < 0) throw UgxError(UGX_BAD_INPUT, std::string("cannot read the tile table ") + path);
     return UGX_OK;
   });
 }

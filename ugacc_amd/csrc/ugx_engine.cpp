@@ -4,6 +4,7 @@
 #include <cstdlib>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -80,6 +81,7 @@ size_t esize(int dtype) { return dtype == 0 ? 8 : 4; }
 Engine::~Engine() {
   for (auto &s : slots_)
     if (s.first) (void)hipFree(s.first);
+  for (void *p : retired_) (void)hipFree(p);
   for (auto &kv : cache_) (void)hipFree(kv.second.first);
   for (auto &p : pending_) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   for (auto e : free_events_) (void)hipEventDestroy(e);
@@ -90,9 +92,17 @@ void Engine::drop_cache() {
   for (auto &kv : cache_) (void)hipFree(kv.second.first);
   cache_.clear();
   cache_pd_.clear();
+  for (void *p : retired_) (void)hipFree(p);
+  retired_.clear();
+  for (auto &s : slots_) {
+    if (s.first) (void)hipFree(s.first);
+    s = {nullptr, 0};
+  }
+  gen_++;
 }
 
 void Engine::refresh(const void *p, int dmin, int dmax) {
+  if (sizing_) return;
   for (auto &kv : cache_pd_) {
     const int dt = std::get<2>(kv.first);
     if (std::get<0>(kv.first) == p && dt >= dmin && dt <= dmax) launch_permute(kv.second, st_);
@@ -108,11 +118,16 @@ size_t Engine::cached_bytes() const {
 void *Engine::slot(int i, size_t bytes) {
   auto &s = slots_.at(i);
   if (s.second < bytes) {
-    hip_check(hipStreamSynchronize(st_), "workspace sync");
-    if (s.first) hip_check(hipFree(s.first), "workspace free");
-    s.first = nullptr;
-    hip_check(hipMalloc(&s.first, bytes), "workspace alloc");
+    // the old workspace may be addressed by queued or captured work: retire
+    // it (freed by drop_cache) instead of freeing it here, which would also
+    // synchronise the device under other threads' streams
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, bytes), "workspace alloc");
+    if (s.first) retired_.push_back(s.first);
+    s.first = p;
     s.second = bytes;
+    gen_++;
+    if (!sizing_) allocs_++;
   }
   return s.first;
 }
@@ -172,8 +187,10 @@ const void *Engine::prepared(const TRef &T, const std::string &letters, const st
     if (it != cache_.end()) return it->second.first;
     hip_check(hipMalloc(&dst, bytes), "cache alloc");
     cache_[key] = {dst, bytes};
+    if (!sizing_) allocs_++;
   } else {
     dst = slot(slot_id, bytes);
+    if (sizing_) return dst;  // sized; its contents come with the real evaluation
   }
   auto is = strides_of(letters, ext);
   PermDesc pd{};
@@ -188,6 +205,7 @@ const void *Engine::prepared(const TRef &T, const std::string &letters, const st
 }
 
 void Engine::permute(const std::string &s, double alpha, const TRef &A, double beta, const TRef &C) {
+  if (sizing_) return;
   Spec sp = parse(s);
   if (!sp.b.empty()) throw std::runtime_error("ugx: permute takes one operand");
   if (!is_perm_of(sp.a, sp.c)) throw std::runtime_error("ugx: permute letters differ: " + s);
@@ -389,7 +407,9 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
   if (Re > 0x7fffffffL || Ce > 0x7fffffffL || Ke > 0x7fffffffL) throw std::runtime_error("ugx: GEMM too large");
 
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (profiling_ && label) { e0 = event(); e1 = event(); hip_check(hipEventRecord(e0, st_), "event record"); }
+  if (profiling_ && label && !sizing_) {
+    e0 = event(); e1 = event(); hip_check(hipEventRecord(e0, st_), "event record");
+  }
 
   const bool direct = (sp.c == rows + cols);
   // Split-K: a small output with a long contraction (e.g. Foo = ttau.L over
@@ -415,7 +435,12 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
         d2.seg[0].sBz = chunk * d.seg[0].sbk;
         d2.ktotal = Ke;
         d2.C = W; d2.scm = Ce; d2.scn = 1; d2.scm1 = 0; d2.sCz = Re * Ce; d2.alpha = 1.0; d2.beta = 0.0;
-        launch_gemm(d2, nullptr, kprec, tile_for(d2, nullptr, kprec, tile), st_);
+        const int t2 = tile_for(d2, nullptr, kprec, tile);
+        if (sizing_) {
+          if (!direct) (void)slot(2, (size_t)Re * Ce * 8);
+          return;
+        }
+        launch_gemm(d2, nullptr, kprec, t2, st_);
         if (direct) {
           launch_splitk_reduce((int)S, (int)Re, (int)Ce, W, alpha, beta, (double *)C.p, Ce, 1, st_);
         } else {
@@ -437,17 +462,23 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
   }
   if (direct) {
     d.C = (double *)C.p; d.scm = Ce; d.scn = 1; d.scm1 = 0; d.alpha = alpha; d.beta = beta;
-    launch_gemm(d, nullptr, kprec, tile_for(d, nullptr, kprec, -1), st_);
+    const int tile = tile_for(d, nullptr, kprec, -1);
+    if (sizing_) return;
+    launch_gemm(d, nullptr, kprec, tile, st_);
   } else if (two_level(rows, cols)) {
     const OutMap rm = outmap(rows), cm = outmap(cols);
     d.C = (double *)C.p; d.alpha = alpha; d.beta = beta;
     d.scm = rm.s0; d.cdiv = (int)rm.div; d.scm1 = rm.s1;
     d.scn = cm.s0; d.ndiv = (int)cm.div; d.scn1 = cm.s1;
-    launch_gemm(d, nullptr, kprec, tile_for(d, nullptr, kprec, -1), st_);
+    const int tile = tile_for(d, nullptr, kprec, -1);
+    if (sizing_) return;
+    launch_gemm(d, nullptr, kprec, tile, st_);
   } else {
     double *tmp = (double *)slot(2, (size_t)Re * Ce * 8);
     d.C = tmp; d.scm = Ce; d.scn = 1; d.scm1 = 0; d.alpha = 1.0; d.beta = 0.0;
-    launch_gemm(d, nullptr, kprec, tile_for(d, nullptr, kprec, -1), st_);
+    const int tile = tile_for(d, nullptr, kprec, -1);
+    if (sizing_) return;
+    launch_gemm(d, nullptr, kprec, tile, st_);
     TRef T = vref(tmp, {});
     for (char ch : rows + cols) T.dims.push_back(ext.at(ch));
     permute(rows + cols + "->" + sp.c, alpha, T, beta, C);
@@ -472,6 +503,25 @@ struct Tuned {
 std::mutex g_tiles_mu;
 std::map<std::string, Tuned> g_tiles;
 
+// The tuned tile depends on the GPU model, not on its index: keys start
+// with the architecture name and the CU count of the current device.
+std::string device_tag() {
+  static std::mutex mu;
+  static std::map<int, std::string> tags;
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "get device");
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = tags.find(dev);
+  if (it != tags.end()) return it->second;
+  hipDeviceProp_t prop;
+  hip_check(hipGetDeviceProperties(&prop, dev), "device properties");
+  std::string arch(prop.gcnArchName);
+  arch = arch.substr(0, arch.find(':'));
+  std::string tag = arch + "/" + std::to_string(prop.multiProcessorCount) + "cu";
+  tags[dev] = tag;
+  return tag;
+}
+
 // Largest element offset of one batch entry's output, plus one.
 long out_extent(const GemmDesc &d, long table_extent) {
   if (d.offm) return table_extent;
@@ -486,8 +536,6 @@ long out_extent(const GemmDesc &d, long table_extent) {
 
 int Engine::tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kprec, int fallback) {
   if (!autotune_) return fallback;
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(st_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return fallback;
   std::string key = "M" + std::to_string(d.M) + " N" + std::to_string(d.N) + " p" + std::to_string(kprec) +
                     (d.c_f32 ? " f" : " d") + (d.beta != 0.0 ? " acc" : "") + (d.cdiv > 0 ? " cd" : "") +
                     (d.ndiv > 0 ? " nd" : "") + (d.offm ? " tab" : "") + (d.tag ? " t" : "") + (ptrs_host ? " grouped" : " b" + std::to_string(d.batch)) +
@@ -497,14 +545,16 @@ int Engine::tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kpre
     key += " [K" + std::to_string(g.K) + " " + std::to_string(g.sam) + "," + std::to_string(g.sak) + "," +
            std::to_string(g.sbk) + "," + std::to_string(g.sbn) + "]";
   }
-  int dev = 0;
-  hip_check(hipGetDevice(&dev), "get device");
-  key = "dev" + std::to_string(dev) + " " + key;
+  key = device_tag() + " " + key;
   {
     std::lock_guard<std::mutex> lk(g_tiles_mu);
     auto it = g_tiles.find(key);
     if (it != g_tiles.end()) return it->second.tile;
   }
+  // a shape first seen while the stream is being captured into a graph
+  // keeps the fallback tile: timing needs synchronisation
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st_, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return fallback;
 
   const size_t es = d.c_f32 ? 4 : 8;
   const long ext = out_extent(d, out_extent_hint_);
@@ -520,7 +570,7 @@ int Engine::tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kpre
     const long fit = std::max<long>(1, (1L << 30) / std::max<long>(1, ext * (long)es));
     const int nb = (int)std::min<long>(std::min<long>(d.batch, 64), fit);
     t.batch = nb;
-    bytes = (size_t)nb * ext * es;
+    bytes = ((size_t)nb * ext * es + 7) / 8 * 8;  // whole doubles: the fill below clears it as such
     char *out = (char *)slot(4, bytes);
     tab.assign(ptrs_host, ptrs_host + nb);
     for (int z = 0; z < nb; z++) tab[z].C = (double *)(out + (size_t)z * ext * es);
@@ -535,10 +585,9 @@ int Engine::tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kpre
     if (nb > 1 && d.sCz > 0) nb = std::max<long>(1, std::min<long>(nb, ((1L << 30) / (long)es - ext) / d.sCz + 1));
     t.batch = (int)nb;
     if (d.ktotal > 0) t.ktotal = std::min<long>(d.ktotal, (long)nb * d.seg[0].K);
-    bytes = ((size_t)(nb - 1) * (size_t)d.sCz + (size_t)ext) * es;
+    bytes = (((size_t)(nb - 1) * (size_t)d.sCz + (size_t)ext) * es + 7) / 8 * 8;
     t.C = (double *)slot(4, bytes);
   }
-  bytes = (bytes + 7) / 8 * 8;
   launch_fill((long)(bytes / 8), 0.0, (double *)slot(4, bytes), st_);
   hipEvent_t e0 = event(), e1 = event();
   int best = fallback >= 0 ? fallback : gemm_pick_tile(d);
@@ -569,6 +618,46 @@ std::string Engine::tile_table() const {
   for (auto &kv : g_tiles)
     s += kv.first + " -> " + gemm_tile_name(kv.second.tile) + " " + std::to_string(kv.second.ms) + " ms\n";
   return s;
+}
+
+// File form of the tuned table: one "key<TAB>tile name<TAB>ms" line per
+// shape (tiles by name, so a table survives changes of the tile ids).
+bool tiles_save(const std::string &path) {
+  std::string s;
+  {
+    std::lock_guard<std::mutex> lk(g_tiles_mu);
+    for (auto &kv : g_tiles)
+      s += kv.first + "\t" + gemm_tile_name(kv.second.tile) + "\t" + std::to_string(kv.second.ms) + "\n";
+  }
+  const std::string tmp = path + ".tmp";
+  FILE *f = std::fopen(tmp.c_str(), "w");
+  if (!f) return false;
+  const bool ok = std::fwrite(s.data(), 1, s.size(), f) == s.size();
+  if (std::fclose(f) != 0 || !ok) return false;
+  return std::rename(tmp.c_str(), path.c_str()) == 0;
+}
+
+long tiles_load(const std::string &path) {
+  FILE *f = std::fopen(path.c_str(), "r");
+  if (!f) return -1;
+  std::map<std::string, int> by_name;
+  for (int t = 0; t < gemm_num_tiles(); t++) by_name[gemm_tile_name(t)] = t;
+  long n = 0;
+  std::string line;
+  char buf[4096];
+  std::lock_guard<std::mutex> lk(g_tiles_mu);
+  while (std::fgets(buf, sizeof(buf), f)) {
+    line = buf;
+    while (!line.empty() && (line.back() == '\n' || line.back() == '\r')) line.pop_back();
+    const size_t t1 = line.find('\t'), t2 = t1 == std::string::npos ? t1 : line.find('\t', t1 + 1);
+    if (t2 == std::string::npos) continue;
+    auto it = by_name.find(line.substr(t1 + 1, t2 - t1 - 1));
+    if (it == by_name.end()) continue;
+    g_tiles[line.substr(0, t1)] = Tuned{it->second, std::atof(line.c_str() + t2 + 1)};
+    n++;
+  }
+  std::fclose(f);
+  return n;
 }
 
 }  // namespace ugx

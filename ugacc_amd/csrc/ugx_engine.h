@@ -70,7 +70,27 @@ class Engine {
   void add_flops(double f) { flops_ += f; }
   bool profiling() const { return profiling_; }
   void reset_counters() { flops_ = 0.0; stats_.clear(); }
+  // Frees the cached conversions and every workspace (retired ones too).
   void drop_cache();
+
+  // Sizing mode: contract() plans each contraction, allocates its
+  // workspaces, converts and caches its constant operands and tunes its
+  // tiles (when autotuning is on), but launches no GEMM, no variable
+  // operand conversion and no output permute; permute() and refresh() do
+  // nothing.  A solve's contractions evaluated once in this mode leave the
+  // engine holding everything the same contractions need, so the solve's
+  // iterations allocate no device memory.
+  void set_sizing(bool on) { sizing_ = on; }
+  bool sizing() const { return sizing_; }
+  // Device allocations made outside sizing mode since the last call
+  // (workspace growth and new cached conversions).
+  long take_allocations() { long n = allocs_; allocs_ = 0; return n; }
+  // Incremented whenever a workspace is replaced by a larger one.  A
+  // replaced workspace is retired, not freed, until drop_cache(), so work
+  // captured into a graph before the change still addresses valid memory;
+  // callers that cache graphs compare generations to re-capture against the
+  // current workspaces.
+  unsigned generation() const { return gen_; }
   // Re-converts, in place, every cached conversion of the constant at p
   // with element type in [dmin, dmax] (whose contents the caller has just
   // replaced): buffers keep their addresses, so captured graphs that read
@@ -106,7 +126,11 @@ class Engine {
                        int slot_id, long pk = 0);
   hipStream_t st_;
   bool panels_ = true;
+  bool sizing_ = false;
+  long allocs_ = 0;
+  unsigned gen_ = 0;
   std::vector<std::pair<void *, size_t>> slots_{6, {nullptr, 0}};
+  std::vector<void *> retired_;
   bool autotune_ = false;
   long out_extent_hint_ = 0;
   int tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kprec, int fallback);
@@ -126,5 +150,9 @@ class Engine {
 };
 
 void hip_check(hipError_t e, const char *what);
+// The process-wide tuned tile table to / from a file (returns false / -1
+// when the file cannot be written / read; load returns the entries merged).
+bool tiles_save(const std::string &path);
+long tiles_load(const std::string &path);
 
 }  // namespace ugx

@@ -151,6 +151,8 @@ void launch_pack_panel2(const double *in, long sp, long sr1, long sr2, long R2, 
                         unsigned short *out, hipStream_t st);
 
 void launch_fill(long n, double v, double *y, hipStream_t st);
+// flag[0] = 1 when any of x[0..n) is NaN or infinite (flag untouched otherwise)
+void launch_nonfinite(long n, const double *x, double *flag, hipStream_t st);
 // Packed symmetric / antisymmetric ladder.  Pairs p(a,b) = a(a+1)/2 + b for
 // a >= b (P = v(v+1)/2 of them) and s(a,b) = a(a-1)/2 + b for a > b
 // (P' = v(v-1)/2); likewise over (i,j).  The (e,f) index -- the GEMMs'
@@ -233,12 +235,17 @@ inline long tw_off_c(long x, int T) { return (x / T) * T * T * T + x % T; }
 // (entry, block pair) items; counter is one device int (reset by the launch
 // with a kernel), part holds ntrip * triples_energy_npairs(nv, w_f32)
 // doubles.
+// Per-problem operands of the (T) energy kernel: orbital energies, the oovv
+// block, t1, and (read only by the f_ov variant) t2 and f_ov[i][a].
 struct TripProb {
-  const double *eps, *oovv, *t1;
+  const double *eps, *oovv, *t1, *t2, *fov;
 };
 int triples_energy_npairs(int nv, int w_f32);
+// fov != 0: includes the f_ov t3 terms of the reference's tcorr (any
+// problem of the batch with a non-zero occupied-virtual Fock block)
 void launch_triples_energy(int no, int nv, int ntrip, const int *trip4, const TripProb *probs, const void *W,
-                           int w_f32, long wstride, double *etrip, double *part, int *counter, hipStream_t st);
+                           int w_f32, long wstride, int fov, double *etrip, double *part, int *counter,
+                           hipStream_t st);
 
 // ------------------------------------------------------------ learned guess
 constexpr int MLP_NFEAT = 8;

@@ -200,6 +200,13 @@ __global__ void k_fill(long n, double v, double *y) {
   for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) y[i] = v;
 }
 
+// flag[0] |= 1 when any of the n values is NaN or infinite
+__global__ void k_nonfinite(long n, const double *__restrict__ x, double *flag) {
+  bool bad = false;
+  for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) bad |= !isfinite(x[i]);
+  if (bad) flag[0] = 1.0;  // every writer stores the same value
+}
+
 // U_ijab = 2 t2_ijab - t2_ijba (ccwfn.cc:455, 518-519), one 32 x 32 (a, b)
 // tile of one (i, j) plane per block: the tile and its mirror are read
 // along b into LDS, so both global reads and the write are coalesced.
@@ -633,6 +640,11 @@ void launch_permute(const PermDesc &p, hipStream_t st) {
 void launch_fill(long n, double v, double *y, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(NT), 0, st, n, v, y);
+}
+
+void launch_nonfinite(long n, const double *x, double *flag, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_nonfinite, dim3(nblocks(n, NT * 8, 4096)), dim3(NT), 0, st, n, x, flag);
 }
 
 void launch_axpby(long n, double a, const double *x, double b, double *y, hipStream_t st) {

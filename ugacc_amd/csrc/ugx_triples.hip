@@ -2,13 +2,21 @@
 //
 // For a batch of i>=j>=k triplets whose connected W3 (v^3, [a][b][c]) has
 // been built by the MFMA contractions, the bracket is summed over a>=b>=c:
-//   V_abc = (W_abc + <ij|ab> t1_kc + <ik|ac> t1_jb + <jk|bc> t1_ia)
+//   V_abc = (W_abc + <ij|ab> t1_kc + <ik|ac> t1_jb + <jk|bc> t1_ia
+//            [+ t2_ij^ab f_kc + t2_ik^ac f_jb + t2_jk^bc f_ia])
 //           / (1 + d_ab + d_ac + d_bc)
 //   X = sum over the six permutations of W*V,  Y = V_abc + V_bca + V_cab,
 //   Z = V_acb + V_bac + V_cba
 //   E += [(Y - 2Z)(W_abc + W_bca + W_cab) + (Z - 2Y)(W_acb + W_bac + W_cba)
 //         + 3X] (2 - d_ij - d_ik - d_jk) / D_ijk^abc
 // V is formed on the fly from W, the oovv block and t1, so only W is stored.
+// The bracketed f_ov terms are the f_kc t3 part of X2 in tcorr / tcorr_ooc
+// (ccwfn.cc:706-709, 725), the values the reference returns: since t2 has
+// the pair symmetry of <ij|ab>, that part of the energy is the bracket's
+// disconnected term with <pq|xy> -> t2_pq^xy and t1 -> f_ov.  The FOV
+// variant of the kernel adds it (any f_ov != 0); it stages each <pq|xy>
+// element with its t2 and each t1 with its f_ov as one vector pair, so the
+// number of LDS accesses stays that of the plain kernel.
 //
 // Every point needs W at all six index permutations.  A workgroup owns one
 // triplet and a pair of E-wide index blocks A >= B, and walks the blocks
@@ -53,7 +61,12 @@ __device__ inline void pair_of(int p, int &A, int &B) {
   B = p - A * (A + 1) / 2;
 }
 
-template <typename WT, int E, typename OT>
+// (<pq|xy>, t2_pq^xy) and (t1, f_ov) pairs of the FOV variant
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { typedef double2 type; };
+template <> struct Vec2<float> { typedef float2 type; };
+
+template <typename WT, int E, typename OT, bool FOV>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) == 4 ? 2 : 1))) void k_triples_energy(int no, int nv, int ntrip, const int *__restrict__ trip,
                                                        const TripProb *__restrict__ probs,
                                                        const WT *__restrict__ Wall, long wstride,
@@ -76,11 +89,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
   static_assert(4 * E <= NT, "t1 / eps staging takes one element per thread");
   __shared__ WT S[6][E * SA];
   // disconnected-term operands of the three blocks X, Y in {A, B, C}:
-  // O[s][X][Y][x][y] = <pq|xy> for the slices s = (ij), (ik), (jk) (OT,
-  // FP64 for both W types);
-  // T1[s][X][x] = t1 of k, j, i (the amplitude that multiplies slice s)
-  __shared__ OT O[3][3][3][E2];
-  __shared__ double T1[3][3][E];
+  // O[s][X][Y][x][y] = <pq|xy> for the slices s = (ij), (ik), (jk) (OT),
+  // with t2_pq^xy beside it in the FOV variant;
+  // T1[s][X][x] = t1 of k, j, i (the amplitude that multiplies slice s),
+  // with f_ov of the same row beside it in the FOV variant
+  typedef typename std::conditional<FOV, typename Vec2<OT>::type, OT>::type OE;
+  typedef typename std::conditional<FOV, typename Vec2<OT>::type, double>::type TE;
+  __shared__ OE O[3][3][3][E2];
+  __shared__ TE T1[3][3][E];
   __shared__ double EPS[3][E];
   __shared__ int item_sh;
   const int tid = threadIdx.x;
@@ -100,6 +116,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
   const int i = trip[4 * t], j = trip[4 * t + 1], k = trip[4 * t + 2];
   const TripProb pr = probs[trip[4 * t + 3]];
   const double *__restrict__ eps = pr.eps, *__restrict__ oovv = pr.oovv, *__restrict__ t1 = pr.t1;
+  const double *__restrict__ t2 = pr.t2, *__restrict__ fov = pr.fov;
   const WT *W = Wall + (long)t * wstride;
   const long v = nv, v2 = (long)nv * nv;
   // slices selected by value (a runtime index into a pointer array would
@@ -109,6 +126,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
   const double *T10 = t1 + (long)k * nv, *T11 = t1 + (long)j * nv, *T12 = t1 + (long)i * nv;
   auto osl = [&](int sl) { return sl == 0 ? O0 : (sl == 1 ? O1 : O2); };
   auto t1sl = [&](int sl) { return sl == 0 ? T10 : (sl == 1 ? T11 : T12); };
+  // FOV: the t2 slices (ij), (ik), (jk) and the f_ov rows k, j, i
+  const double *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr, *F0 = nullptr, *F1 = nullptr, *F2 = nullptr;
+  if constexpr (FOV) {
+    Q0 = t2 + ((long)i * no + j) * v2; Q1 = t2 + ((long)i * no + k) * v2; Q2 = t2 + ((long)j * no + k) * v2;
+    F0 = fov + (long)k * nv; F1 = fov + (long)j * nv; F2 = fov + (long)i * nv;
+  }
+  auto qsl = [&](int sl) { return sl == 0 ? Q0 : (sl == 1 ? Q1 : Q2); };
+  auto fsl = [&](int sl) { return sl == 0 ? F0 : (sl == 1 ? F1 : F2); };
   const double eo = eps[i] + eps[j] + eps[k];
   const double fac = 2.0 - ((i == j) + (i == k) + (j == k));
   const int A0 = Ab * E, B0 = Bb * E;
@@ -116,38 +141,81 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
   // V at a permutation (x, y, z) of the point, given as block slots (bx, by,
   // bz) in {0: A, 1: B, 2: C} and local offsets (lx, ly, lz)
   auto Vx = [&](int bx, int lx, int by, int ly, int bz, int lz, double w, double inv) {
-    return (w + (double)O[0][bx][by][lx * E + ly] * T1[0][bz][lz] + (double)O[1][bx][bz][lx * E + lz] * T1[1][by][ly] +
-            (double)O[2][by][bz][ly * E + lz] * T1[2][bx][lx]) * inv;
+    if constexpr (FOV) {
+      // the disconnected sum in OT (FP32 pairs: FP32 like their operands)
+      const OE o0 = O[0][bx][by][lx * E + ly], o1 = O[1][bx][bz][lx * E + lz], o2 = O[2][by][bz][ly * E + lz];
+      const TE u0 = T1[0][bz][lz], u1 = T1[1][by][ly], u2 = T1[2][bx][lx];
+      const OT d = o0.x * u0.x + o0.y * u0.y + o1.x * u1.x + o1.y * u1.y + o2.x * u2.x + o2.y * u2.y;
+      return (w + (double)d) * inv;
+    } else {
+      return (w + (double)O[0][bx][by][lx * E + ly] * T1[0][bz][lz] +
+              (double)O[1][bx][bz][lx * E + lz] * T1[1][by][ly] + (double)O[2][by][bz][ly * E + lz] * T1[2][bx][lx]) *
+             inv;
+    }
   };
-  // <pq|xy> of slice s at block-local (lx, ly) of blocks with origins gx0, gy0
+  // <pq|xy> (and t2_pq^xy) of slice s at block-local (lx, ly) of blocks
+  // with origins gx0, gy0
   auto oget = [&](int s, int gx0, int lx, int gy0, int ly) {
     const int gx = gx0 + lx, gy = gy0 + ly;
-    return (OT)((gx < nv && gy < nv) ? osl(s)[gx * v + gy] : 0.0);
+    const bool in = gx < nv && gy < nv;
+    if constexpr (FOV) {
+      OE r;
+      r.x = (OT)(in ? osl(s)[gx * v + gy] : 0.0);
+      r.y = (OT)(in ? qsl(s)[gx * v + gy] : 0.0);
+      return r;
+    } else {
+      return (OE)(in ? osl(s)[gx * v + gy] : 0.0);
+    }
+  };
+  // t1 (and f_ov) of slice s at global virtual index gx (< nv)
+  auto tget = [&](int s, int gx) {
+    if constexpr (FOV) {
+      TE r;
+      r.x = (OT)t1sl(s)[gx];
+      r.y = (OT)fsl(s)[gx];
+      return r;
+    } else {
+      return t1sl(s)[gx];
+    }
+  };
+  auto tzero = []() { return TE{}; };
+  // an orbital energy carried in a TE register, bit for bit (an FP32 pair
+  // holds the FP64 value's eight bytes: the denominators stay FP64)
+  auto tval = [](double x) {
+    if constexpr (FOV && sizeof(OT) == 4) return __builtin_bit_cast(TE, x);
+    else if constexpr (FOV) { TE r{}; r.x = x; return r; }
+    else return x;
+  };
+  auto tfirst = [](const TE &x) {
+    if constexpr (FOV && sizeof(OT) == 4) return __builtin_bit_cast(double, x);
+    else if constexpr (FOV) return (double)x.x;
+    else return x;
   };
 
   // Once per item: the pairs within {A, B} and the t1 / eps entries of A and
   // B.  Every load is issued before any LDS store, so the loads overlap.
   {
-    OT rf[PF];
+    OE rf[PF];
 #pragma unroll
     for (int q = 0; q < PF; q++) {
       const int e = tid + q * NT;
-      rf[q] = 0;
+      rf[q] = OE{};
       if (e < NPF) {
         const int sl = e / (4 * E2), r = e % (4 * E2), pp = r / E2, xy = r % E2;
         const int bx = pp >> 1, by = pp & 1;
         rf[q] = oget(sl, bx ? B0 : A0, xy / E, by ? B0 : A0, xy % E);
       }
     }
-    double rt = 0.0;
+    TE rt = tzero();
     if (tid < 4 * E) {  // T1 of slices 0 and 1 for blocks A, B
       const int sl = tid / (2 * E), bx = (tid / E) & 1, lx = tid % E, gx = (bx ? B0 : A0) + lx;
-      if (gx < nv) rt = t1sl(sl)[gx];
+      if (gx < nv) rt = tget(sl, gx);
     }
-    double rt2 = 0.0, re = 0.0;
+    TE rt2 = tzero();
+    double re = 0.0;
     if (tid < 2 * E) {  // T1 of slice 2 and eps for blocks A, B
       const int bx = tid / E, lx = tid % E, gx = (bx ? B0 : A0) + lx;
-      if (gx < nv) { rt2 = T12[gx]; re = eps[no + gx]; }
+      if (gx < nv) { rt2 = tget(2, gx); re = eps[no + gx]; }
     }
 #pragma unroll
     for (int q = 0; q < PF; q++) {
@@ -181,8 +249,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
   // Step data in registers: loaded for step C + 1 while step C computes,
   // committed to LDS after the step's closing barrier.
   W16 rw[6][PW];
-  OT rc[PC];
-  double rtc = 0.0;
+  OE rc[PC];
+  TE rtc = tzero();
   auto fetch = [&](int Cb) {
     const int C0 = Cb * E;
     const WT *c0 = cube(Ab, Bb, Cb), *c1 = cube(Ab, Cb, Bb), *c2 = cube(Bb, Ab, Cb);
@@ -199,7 +267,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
 #pragma unroll
     for (int q = 0; q < PC; q++) {
       const int e = tid + q * NT;
-      rc[q] = 0;
+      rc[q] = OE{};
       if (e < NPC) {
         const int sl = e / (5 * E2), r = e % (5 * E2), p = r / E2, xy = r % E2;
         int bx, by;
@@ -208,10 +276,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
         rc[q] = oget(sl, gx0, xy / E, gy0, xy % E);
       }
     }
-    rtc = 0.0;
+    rtc = tzero();
     if (tid < 4 * E) {  // T1 of slices 0..2 at block C, then eps
       const int sl = tid / E, lx = tid % E, gx = C0 + lx;
-      if (gx < nv) rtc = sl < 3 ? t1sl(sl)[gx] : eps[no + gx];
+      if (gx < nv) rtc = sl < 3 ? tget(sl, gx) : tval(eps[no + gx]);
     }
   };
   auto commit = [&]() {
@@ -255,7 +323,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     if (tid < 4 * E) {
       const int sl = tid / E, lx = tid % E;
       if (sl < 3) T1[sl][2][lx] = rtc;
-      else EPS[2][lx] = rtc;
+      else EPS[2][lx] = tfirst(rtc);
     }
   };
 
@@ -304,7 +372,7 @@ __global__ void k_triples_reduce(int npairs, int ntrip, const double *part, doub
   etrip[t] = s;
 }
 
-template <typename WT, int E, typename OT>
+template <typename WT, int E, typename OT, bool FOV>
 void launch_e(int no, int nv, int ntrip, const int *trip4, const TripProb *probs, const void *W, long wstride,
               double *etrip, double *part, int *counter, hipStream_t st) {
   int nb = (nv + E - 1) / E;
@@ -313,11 +381,11 @@ void launch_e(int no, int nv, int ntrip, const int *trip4, const TripProb *probs
   int dev = 0, ncu = 0, per = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_triples_energy<WT, E, OT>, NT, 0);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_triples_energy<WT, E, OT, FOV>, NT, 0);
   long grid = (long)std::max(ncu, 1) * std::max(per, 1);
   grid = std::min<long>(grid, (long)ntrip * npairs);
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, counter);
-  hipLaunchKernelGGL((k_triples_energy<WT, E, OT>), dim3((unsigned)grid), dim3(NT), 0, st, no, nv, ntrip, trip4,
+  hipLaunchKernelGGL((k_triples_energy<WT, E, OT, FOV>), dim3((unsigned)grid), dim3(NT), 0, st, no, nv, ntrip, trip4,
                      probs, (const WT *)W, wstride, part, npairs, counter);
   hipLaunchKernelGGL(k_triples_reduce, dim3((ntrip + NT - 1) / NT), dim3(NT), 0, st, npairs, ntrip, part, etrip);
 }
@@ -333,10 +401,18 @@ int triples_energy_npairs(int nv, int w_f32) {
 }
 
 void launch_triples_energy(int no, int nv, int ntrip, const int *trip4, const TripProb *probs, const void *W,
-                           int w_f32, long wstride, double *etrip, double *part, int *counter, hipStream_t st) {
+                           int w_f32, long wstride, int fov, double *etrip, double *part, int *counter,
+                           hipStream_t st) {
   if (ntrip <= 0) return;
-  if (w_f32) launch_e<float, 12, double>(no, nv, ntrip, trip4, probs, W, wstride, etrip, part, counter, st);
-  else launch_e<double, 12, double>(no, nv, ntrip, trip4, probs, W, wstride, etrip, part, counter, st);
+  // FOV with an FP32 W: the (<pq|xy>, t2) pairs in FP32, so that the LDS
+  // footprint (and two workgroups per CU) is that of the plain kernel
+  if (w_f32) {
+    if (fov) launch_e<float, 12, float, true>(no, nv, ntrip, trip4, probs, W, wstride, etrip, part, counter, st);
+    else launch_e<float, 12, double, false>(no, nv, ntrip, trip4, probs, W, wstride, etrip, part, counter, st);
+  } else {
+    if (fov) launch_e<double, 12, double, true>(no, nv, ntrip, trip4, probs, W, wstride, etrip, part, counter, st);
+    else launch_e<double, 12, double, false>(no, nv, ntrip, trip4, probs, W, wstride, etrip, part, counter, st);
+  }
 }
 
 }  // namespace ugx

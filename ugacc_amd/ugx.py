@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libugx.so")
 
 OK, NOT_CONVERGED, BAD_INPUT, OOM, HIP_ERROR, NO_DEVICE, NOT_READY, NONFINITE = range(8)
-PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X3_PAIR, PREC_BF16 = 0, 1, 2, 3, 4
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X3_PAIR, PREC_BF16, PREC_BF16X3_PANEL = 0, 1, 2, 3, 4, 5
 GUESS_MP2, GUESS_USER, GUESS_MLP = 0, 1, 2
 POLISH_PLAIN, POLISH_REFINE = 0, 1
 MLP_NFEAT, MLP_WIDTH = 8, 128
@@ -24,9 +24,10 @@ PREC_NAMES = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 # Every function include/ugx.h declares, in declaration order.
 EXPORTS = [
     "ugx_default_options", "ugx_create", "ugx_destroy", "ugx_last_error", "ugx_load_hamiltonian",
-    "ugx_set_amplitudes", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
+    "ugx_set_amplitudes", "ugx_prepare", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
     "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_set_profiling",
-    "ugx_kernel_stats", "ugx_gemm", "ugx_synchronize", "ugx_batch_solve", "ugx_set_autotune", "ugx_tile_table", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
+    "ugx_kernel_stats", "ugx_gemm", "ugx_synchronize", "ugx_batch_solve", "ugx_set_autotune", "ugx_tile_table",
+    "ugx_save_tile_table", "ugx_load_tile_table", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
     "ugx_mlp_last_error", "ugx_mlp_set_params", "ugx_mlp_get_params", "ugx_mlp_set_scaling", "ugx_mlp_forward",
     "ugx_mlp_loss_grad", "ugx_mlp_train", "ugx_features", "ugx_set_guess_network",
 ]
@@ -54,6 +55,7 @@ class Result(ctypes.Structure):
         ("e_mp2", ctypes.c_double), ("e_ccsd", ctypes.c_double), ("e_ccsd_final", ctypes.c_double),
         ("e_t", ctypes.c_double), ("rms", ctypes.c_double), ("niter", ctypes.c_int), ("niter_low", ctypes.c_int),
         ("seconds_ccsd", ctypes.c_double), ("seconds_triples", ctypes.c_double), ("flops_ccsd", ctypes.c_double),
+        ("device_allocs", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -86,6 +88,7 @@ def load_library(path=LIBPATH):
     L.ugx_last_error.restype = ctypes.c_char_p
     L.ugx_load_hamiltonian.argtypes = [vp, ip, ip] + [dp] * 7
     L.ugx_set_amplitudes.argtypes = [vp, dp, dp]
+    L.ugx_prepare.argtypes = [vp, ctypes.POINTER(Options)]
     L.ugx_ccsd.argtypes = [vp, ctypes.POINTER(Options), ctypes.POINTER(Result)]
     L.ugx_triples.argtypes = [vp, ip, dp]
     L.ugx_triples_per_triplet.argtypes = [vp, ip, dp]
@@ -106,6 +109,8 @@ def load_library(path=LIBPATH):
     L.ugx_set_autotune.argtypes = [vp, ip]
     L.ugx_tile_table.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
     L.ugx_tile_table.restype = ctypes.c_size_t
+    L.ugx_save_tile_table.argtypes = [vp, ctypes.c_char_p]
+    L.ugx_load_tile_table.argtypes = [vp, ctypes.c_char_p]
     fp, lp, cl = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_long), ctypes.c_long
     L.ugx_mlp_num_params.argtypes = []
     L.ugx_mlp_num_params.restype = cl
@@ -215,6 +220,13 @@ class Context:
         self._check(self.L.ugx_get_amplitudes(self.h, _p(t1), _p(t2)))
         return t1, t2
 
+    def prepare(self, options=None, **kw):
+        """ugx_prepare: allocations and conversions of a solve with these options."""
+        o = options if options is not None else default_options()
+        for k, v in kw.items():
+            setattr(o, k, v)
+        self._check(self.L.ugx_prepare(self.h, ctypes.byref(o)))
+
     def ccsd(self, options=None, **kw):
         o = options if options is not None else default_options()
         for k, v in kw.items():
@@ -316,6 +328,12 @@ class Context:
         buf = ctypes.create_string_buffer(max(n, 1))
         self.L.ugx_tile_table(self.h, buf, n)
         return buf.value.decode()
+
+    def save_tile_table(self, path):
+        self._check(self.L.ugx_save_tile_table(self.h, os.fsencode(path)))
+
+    def load_tile_table(self, path):
+        self._check(self.L.ugx_load_tile_table(self.h, os.fsencode(path)))
 
     def features(self, idx=None, n=None):
         """Raw learned-guess features of the amplitudes idx (flattened ijab)."""
