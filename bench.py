@@ -43,9 +43,12 @@ WORKLOADS = {
     "h2o": dict(n=1, no=5, nv=19, name="H2O/cc-pVDZ CCSD(T), all-electron (24 MOs, o=5, v=19)"),
 }
 CLUSTER_SEED = 8
-# published dense MI355X peaks, TFLOP/s; a BF16x3 product costs three BF16
-# MFMAs, so its ceiling for algorithmic flops is a third of the BF16 peak
-PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}
+# published dense MI355X peaks of the arithmetic each form executes,
+# TFLOP/s: BF16x3 runs on the BF16 MFMAs (three BF16 products per
+# algorithmic product, so algorithmic flops can reach at most a third of the
+# BF16 peak; that ceiling is reported beside the published-peak fraction)
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}
+MFMA_PER_PRODUCT = {"fp64": 1, "fp32": 1, "bf16x3": 3}
 FP64_CONV = 1e-10
 # HBM bytes per launch of a kernel label on a workload, from two rocprofv3
 # --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by the counters' reading for a
@@ -116,12 +119,80 @@ def cpu_baseline(no, nv, fock, blocks, niter, budget_s=25.0):
         if time.perf_counter() - t0 > budget_s / 2 or k >= ntrip:
             break
     t_trip = (time.perf_counter() - t0) / k
+    # serial leg (the reference is single-threaded): the same loops on one
+    # thread over a few triplets; the serial CCSD iteration is the OpenMP
+    # one scaled by the measured serial / OpenMP ratio of the (T) loops
+    orc.set_threads(1)
+    ks = 2
+    t0 = time.perf_counter()
+    for q in range(ks):
+        t = (q * 7919 + 13) % ntrip
+        orc.tcorr_tjl_range(t, t + 1)
+    t_trip1 = (time.perf_counter() - t0) / ks
+    orc.set_threads(threads)
     orc.close()
     total = niter * t_iter + ntrip * t_trip
+    ratio = t_trip1 / t_trip
+    serial = dict(value=niter * t_iter * ratio + ntrip * t_trip1, unit="s", cores=1, kind="port",
+                  sample="FP64 oracle on 1 thread: (T) over %d triplets (%.2f s each, x%.1f the %d-thread time); "
+                         "CCSD iterations estimated as the %d-thread row sample times that ratio; extrapolated "
+                         "to %d iterations + all %d triplets" % (ks, t_trip1, ratio, threads, threads, niter, ntrip))
     return dict(value=total, unit="s", cores=threads, kind="port",
                 sample="FP64 oracle (OpenMP, %d threads): one CCSD iteration restricted to %d of %d occupied "
                        "rows (x%d) and (T) over %d of %d triplets, extrapolated to %d iterations + all triplets"
-                       % (threads, rows, no, no // rows, k, ntrip, niter))
+                       % (threads, rows, no, no // rows, k, ntrip, niter),
+                serial=serial, full_h2o=cpu_full_h2o())
+
+
+def cpu_full_h2o():
+    """The single-water configuration (o=5, v=19) solved completely by the
+    oracle: CCSD to 1e-10 and (T), on one thread and on all."""
+    from oracle import Oracle
+    from ugacc_amd.chem import molecules
+    from ugacc_amd.chem.scf import Reference
+    ref = Reference(molecules.to_bohr(molecules.water()), "cc-pvdz")
+    out = {}
+    for label in ("serial", "openmp"):
+        orc = Oracle(ref.fock, ref.full_ints(), ref.no, omp=True)
+        threads = orc.threads
+        if label == "serial":
+            orc.set_threads(1)
+        t0 = time.perf_counter()
+        r = orc.ccsd(maxiter=100, conv=FP64_CONV)
+        et = orc.tcorr_tjl()
+        out[label + "_s"] = time.perf_counter() - t0
+        out[label + "_threads"] = 1 if label == "serial" else threads
+        orc.set_threads(threads)
+        orc.close()
+    out.update(iterations=r["niter"], e_ccsd=r["eccsd"], e_t=et,
+               sample="complete FP64 oracle CCSD(T) of H2O/cc-pVDZ (o=5, v=19), R_CONVERGENCE 1e-10")
+    return out
+
+
+def cold_solve(ugx, no, nv, fock, blocks, opts, net, label):
+    """The first ugx_ccsd after ugx_load_hamiltonian on a fresh context:
+    every per-Hamiltonian conversion (integral copies in each precision,
+    BF16 panels, (T) operand tables) and allocation happens inside it.  The
+    tuned tiles come from the process's table (ugx_load_tile_table makes
+    that the case in a new process)."""
+    c = ugx.Context(0)
+    c.set_autotune(True)
+    if net is not None:
+        c.set_guess_network(net)
+    t0 = time.perf_counter()
+    c.load_hamiltonian(no, nv, fock, blocks)
+    c.synchronize()
+    t1 = time.perf_counter()
+    c.prepare(opts)
+    c.synchronize()
+    t2 = time.perf_counter()
+    r = c.ccsd(opts)
+    c.synchronize()
+    t3 = time.perf_counter()
+    c.close()
+    log("%s cold: load %.2f s, prepare %.2f s, solve %.2f s" % (label, t1 - t0, t2 - t1, t3 - t2))
+    return dict(load_s=t1 - t0, prepare_s=t2 - t1, first_solve_s=t3 - t1, converged=bool(r["converged"]),
+                device_allocs_in_iterations=r["device_allocs"])
 
 
 def run_form(ctx, opts, warmup, steps, label):
@@ -147,33 +218,67 @@ def run_form(ctx, opts, warmup, steps, label):
 
 
 def roofline(stats, workload):
-    """The kernel label with the most device time, against its peak."""
+    """The kernel label with the most device time, against the published
+    peak of the arithmetic it executes: achieved = algorithmic flops per
+    launch / average launch time (HIP events on the library's stream)."""
     dom = max((k for k in stats if stats[k]["gflop"] > 0), key=lambda k: stats[k]["ms"])
     st = stats[dom]
     form = dom.split(":")[1] if ":" in dom else "fp64"
     ach = st["gflop"] / st["ms"]  # GFLOP/ms = TFLOP/s
-    names = {"triples_w": "k_gemm_b3d (BF16x3) / k_gemm (FP64, FP32) (T) W3 grouped GEMM", "ladder": "k_gemm particle-particle ladder"}
+    names = {"triples_w": "k_gemm_b3d (BF16x3) / k_gemm (FP64, FP32) (T) W3 grouped GEMM",
+             "ladder": "k_gemm particle-particle ladder", "ring": "k_gemm ring contraction"}
+    m = MFMA_PER_PRODUCT[form]
     return dict(bound="mfma", achieved=ach, peak=PEAK[form], unit="TFLOP/s", frac=ach / PEAK[form],
                 traffic=TRAFFIC.get((workload, dom)),
                 kernel="%s (%s)" % (names.get(dom.split(":")[0], dom), dom), launches=st["launches"],
-                avg_ms=st["ms"] / max(1, st["launches"]), flop_per_launch=st["gflop"] * 1e9 / max(1, st["launches"]))
+                avg_ms=st["ms"] / max(1, st["launches"]), flop_per_launch=st["gflop"] * 1e9 / max(1, st["launches"]),
+                mfma_products_per_flop=m, executed_tflops=ach * m, frac_of_form_ceiling=ach * m / PEAK[form])
 
 
-def run_batch(args):
-    """64 uneven geometries solved by ugx_batch_solve, FP64 and mixed forms:
-    solves/s and per-solve seconds (p50, p99), each result checked against
-    the problem's own single-geometry FP64 solve."""
-    from ugacc_amd import ugx
+def batch_problems():
+    """The 64 uneven geometries: 32 distorted H2O monomers (seeds 100-131)
+    and 32 (H2O)2 dimers (seeds 200-231), cc-pVDZ, host front end."""
     from ugacc_amd.chem import molecules
     from ugacc_amd.chem.scf import Reference
     t0 = time.perf_counter()
-    geoms = [("H2O distorted seed %d" % (100 + k), molecules.distorted_monomer(100 + k)) for k in range(32)]
-    geoms += [("(H2O)2 seed %d" % (200 + k), molecules.random_dimer(200 + k)) for k in range(32)]
+    geoms = [molecules.distorted_monomer(100 + k) for k in range(32)]
+    geoms += [molecules.random_dimer(200 + k) for k in range(32)]
     problems = []
-    for name, atoms in geoms:
+    for atoms in geoms:
         ref = Reference(atoms, "cc-pvdz", log=None)
         problems.append((ref.no, ref.nv, ref.fock, ref.blocks))
     log("front end for %d geometries (untimed): %.1f s" % (len(problems), time.perf_counter() - t0))
+    return problems
+
+
+def batch_cpu_baseline(problems):
+    """Full FP64 oracle solves (OpenMP) of the first monomer and the first
+    dimer, CCSD to 1e-10 and (T), extrapolated to the 64 problems."""
+    from oracle import Oracle
+    from ugacc_amd.hamiltonian import full_from_blocks
+    times = {}
+    threads = 1
+    for q in (0, 32):
+        no, nv, fock, blocks = problems[q]
+        orc = Oracle(fock, full_from_blocks(no, nv, blocks), no, omp=True)
+        threads = orc.threads
+        t0 = time.perf_counter()
+        orc.ccsd(maxiter=100, conv=FP64_CONV)
+        orc.tcorr_tjl()
+        times[q] = time.perf_counter() - t0
+        orc.close()
+    total = 32 * times[0] + 32 * times[32]
+    return dict(value=len(problems) / total, unit="solves/s", cores=threads, kind="port",
+                seconds_monomer=times[0], seconds_dimer=times[32],
+                sample="FP64 oracle (OpenMP, %d threads): complete CCSD (1e-10) + (T) of one monomer (%.2f s) and one "
+                       "dimer (%.2f s), extrapolated to 32 of each" % (threads, times[0], times[32]))
+
+
+def batch_measure(args, problems, warmup, steps, cpu=True):
+    """ugx_batch_solve of the 64 geometries, FP64 and mixed forms: solves/s,
+    per-solve seconds (p50, p99), each result checked against the problem's
+    own single-geometry FP64 solve; kernel statistics of the mixed form."""
+    from ugacc_amd import ugx
     ctx = ugx.Context(0)
     ctx.set_autotune(not args.no_autotune)
     # single-geometry FP64 references, solved one after another on one
@@ -189,39 +294,60 @@ def run_batch(args):
         t_seq = time.perf_counter() - t0
     log("sequential single FP64 solves: %.3f s" % t_seq)
     graph = 0 if args.no_graph else 1
+    pmode = args.batch_polish_mode
     forms = {"fp64": dict(precision=ugx.PREC_FP64, r_convergence=FP64_CONV),
              "mixed": dict(precision=ugx.PREC_NAMES[args.low], r_convergence=args.mixed_conv,
                            polish_threshold=args.polish,
-                           polish_mode=ugx.POLISH_REFINE if args.polish_mode == "refine" else ugx.POLISH_PLAIN)}
-    log("batch64 mixed form: %s polish" % (args.polish_mode or "plain"))
-    out = {}
+                           polish_mode=ugx.POLISH_REFINE if pmode == "refine" else ugx.POLISH_PLAIN)}
+    out, stats = {}, None
     for name, kw in forms.items():
-        for w in range(args.warmup):
+        for w in range(warmup):
             ctx.batch_solve(problems, nworkers=args.workers, wfn_ccsd_t=1, maxiter=100, graph=graph, **kw)
-        walls, per = [], []
-        for st in range(args.steps):
+        ctx.set_profiling(name == "mixed")
+        walls, per, allocs = [], [], 0
+        for st in range(steps):
             res, wall = ctx.batch_solve(problems, nworkers=args.workers, wfn_ccsd_t=1, maxiter=100, graph=graph, **kw)
             if not all(r["converged"] for r in res):
                 raise RuntimeError("batch form %s did not converge" % name)
             walls.append(wall)
             per += [r["seconds_ccsd"] + r["seconds_triples"] for r in res]
+            allocs = max(allocs, max(r["device_allocs"] for r in res))
             dE = max(abs(r["e_ccsd_final"] + r["e_t"] - e) for r, e in zip(res, single))
-            log("%s step %d: %.3f s for %d solves, max |dE| vs single FP64 solves %.1e" % (name, st, wall, len(res), dE))
+            log("batch %s step %d: %.3f s for %d solves, max |dE| vs single FP64 solves %.1e" % (
+                name, st, wall, len(res), dE))
+        if name == "mixed":
+            stats = ctx.kernel_stats()
+        ctx.set_profiling(False)
         wall = float(np.mean(walls))
         out[name] = dict(wall_s=wall, solves_per_s=len(problems) / wall, p50_s=float(np.percentile(per, 50)),
                          p99_s=float(np.percentile(per, 99)), max_dE_vs_single_fp64=dE,
-                         iterations_mean=float(np.mean([r["niter"] for r in res])))
-    line = {"metric": "batched CCSD(T) solves/s, 32 distorted H2O + 32 (H2O)2, cc-pVDZ", "value":
-            out["mixed"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": out["mixed"]["wall_s"] * 1e3, "higher_is_better": True,
+                         iterations_mean=float(np.mean([r["niter"] for r in res])),
+                         device_allocs_in_iterations=allocs)
+    ctx.close()
+    res = dict(metric="batched CCSD(T) solves/s, 32 distorted H2O + 32 (H2O)2, cc-pVDZ",
+               value=out["mixed"]["solves_per_s"], unit="solves/s", steps=steps, warmup=warmup,
+               workload="64 uneven geometries: 32 x (o=5, v=19) + 32 x (o=10, v=38)",
+               workers=args.workers, graph=bool(graph), autotune=not args.no_autotune,
+               mixed_r_convergence=args.mixed_conv, mixed_polish=pmode, fp64_r_convergence=FP64_CONV,
+               forms=out, sequential_single_fp64_s=t_seq, kernels=stats,
+               roofline=roofline(stats, "batch64") if stats else None,
+               cpu_baseline=batch_cpu_baseline(problems) if cpu else None)
+    return res
+
+
+def run_batch(args):
+    """--workload batch64: the batch line alone."""
+    problems = batch_problems()
+    b = batch_measure(args, problems, args.warmup, args.steps, cpu=not args.no_cpu)
+    line = {"metric": b["metric"], "value": b["value"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": b["forms"]["mixed"]["wall_s"] * 1e3, "higher_is_better": True,
             "scaling": "strong", "vs_baseline": None, "dtype": "%s/f64" % args.low,
             "data": "seeded distorted monomers (O-H U[0.90,1.10] A, angle U[95,115] deg, seeds 100-131) and dimers "
                     "(O-O U[2.7,3.2] A, random rotations, seeds 200-231), cc-pVDZ, host front end (untimed)",
-            "config": {"workload": "64 uneven geometries: 32 x (o=5, v=19) + 32 x (o=10, v=38)",
-                       "workers": args.workers, "graph": bool(graph), "autotune": not args.no_autotune,
-                       "mixed_r_convergence": args.mixed_conv, "mixed_polish": args.polish_mode or "plain",
-                       "fp64_r_convergence": FP64_CONV},
-            "forms": out, "sequential_single_fp64_s": t_seq, "roofline": None, "cpu_baseline": None}
+            "config": {k: b[k] for k in ("workload", "workers", "graph", "autotune", "mixed_r_convergence",
+                                         "mixed_polish", "fp64_r_convergence")},
+            "forms": b["forms"], "sequential_single_fp64_s": b["sequential_single_fp64_s"],
+            "kernels": b["kernels"], "roofline": b["roofline"], "cpu_baseline": b["cpu_baseline"]}
     print(json.dumps(line), flush=True)
 
 
@@ -247,6 +373,15 @@ def main():
                          "refinement's extra launches cost more than its FP64 savings)")
     ap.add_argument("--no-plain-compare", action="store_true",
                     help="mixed form: skip the extra timed solve with the plain FP64 polish")
+    ap.add_argument("--ref-steps", type=int, default=None,
+                    help="timed solves of each comparison form (FP64 at 1e-10, FP64 at the mixed threshold, the "
+                         "plain polish); default min(steps, 3)")
+    ap.add_argument("--ref-warmup", type=int, default=1)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batch64 object of the default line")
+    ap.add_argument("--no-cold", action="store_true", help="skip the cold (first-solve) measurements")
+    ap.add_argument("--batch-polish-mode", default="plain", choices=["refine", "plain"])
+    ap.add_argument("--tiles", default=None, help="tuned tile table to load first (ugx_load_tile_table)")
+    ap.add_argument("--save-tiles", default=None, help="write the tuned tile table here after the run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="size-heuristic GEMM tiles instead of the autotuned ones (tuning runs in the warmup)")
@@ -274,7 +409,10 @@ def main():
     from ugacc_amd import ugx
     from ugacc_amd.synthetic import random_blocks
 
+    if args.ref_steps is None:
+        args.ref_steps = max(1, min(args.steps, 3))
     if args.workload == "batch64":
+        args.batch_polish_mode = args.polish_mode or args.batch_polish_mode
         return run_batch(args)
     wl = WORKLOADS[args.workload]
     no, nv = wl["no"], wl["nv"]
@@ -297,6 +435,8 @@ def main():
                 "project's host front end (untimed)" % args.seed if wl["n"] > 1 else
                 "H2O at 0.9572 A / 104.52 deg, cc-pVDZ, RHF from the project's host front end (untimed)")
     ctx = ugx.Context(0)
+    if args.tiles:
+        ctx.load_tile_table(args.tiles)
     ctx.load_hamiltonian(no, nv, fock, blocks)
     ctx.set_autotune(not args.no_autotune)
 
@@ -321,15 +461,20 @@ def main():
 
     fp64_opts = options(ugx.PREC_FP64, FP64_CONV, ugx.GUESS_MP2)
     out = {}
+    rs, rw = args.ref_steps, args.ref_warmup
     if args.form == "mixed":
-        wall64, res64, st64 = run_form(ctx, fp64_opts, args.warmup, args.steps, "fp64")
+        # the comparison forms: FP64 at the parity threshold 1e-10 and at the
+        # mixed form's threshold (the equal-threshold ratio)
+        wall64, res64, st64 = run_form(ctx, fp64_opts, rw, rs, "fp64 (1e-10)")
+        fp64_eq = options(ugx.PREC_FP64, args.mixed_conv, ugx.GUESS_MP2)
+        wall64e, res64e, _ = run_form(ctx, fp64_eq, rw, rs, "fp64 (%.0e)" % args.mixed_conv)
         if args.polish_mode is None:
             args.polish_mode = "refine"
         pmode = ugx.POLISH_REFINE if args.polish_mode == "refine" else ugx.POLISH_PLAIN
         if pmode == ugx.POLISH_REFINE and not args.no_plain_compare:
             # the same mixed form with the plain FP64 polish, for comparison
-            wallp, resp, _ = run_form(ctx, options(ugx.PREC_NAMES[args.low], args.mixed_conv, guess), args.warmup,
-                                      args.steps, "mixed, plain FP64 polish")
+            wallp, resp, _ = run_form(ctx, options(ugx.PREC_NAMES[args.low], args.mixed_conv, guess), rw,
+                                      rs, "mixed, plain FP64 polish")
             d_p = abs(resp["e_ccsd_final"] + resp["e_t"] - res64["e_ccsd_final"] - res64["e_t"])
             out.update(plain_polish_wall_s=wallp, plain_polish_iterations=resp["niter"],
                        plain_polish_seconds_ccsd=resp["seconds_ccsd"], plain_polish_speedup_vs_fp64=wall64 / wallp,
@@ -339,19 +484,32 @@ def main():
         d_ccsd = abs(res["e_ccsd_final"] - res64["e_ccsd_final"])
         d_t = abs(res["e_t"] - res64["e_t"])
         d_tot = abs(res["e_ccsd_final"] + res["e_t"] - res64["e_ccsd_final"] - res64["e_t"])
+        d_eq = abs(res64e["e_ccsd_final"] + res64e["e_t"] - res64["e_ccsd_final"] - res64["e_t"])
+        out.update(fp64_same_threshold_wall_s=wall64e, fp64_same_threshold_iterations=res64e["niter"],
+                   fp64_same_threshold_r_convergence=args.mixed_conv,
+                   fp64_same_threshold_dE_total_vs_1e10=d_eq,
+                   speedup_vs_fp64_same_threshold=wall64e / wall)
         out.update(fp64_wall_s=wall64, fp64_iterations=res64["niter"], fp64_seconds_ccsd=res64["seconds_ccsd"],
                    fp64_seconds_triples=res64["seconds_triples"], speedup_vs_fp64=wall64 / wall,
                    dE_ccsd=d_ccsd, dE_t=d_t, dE_total=d_tot, dE_ok=bool(d_tot <= 1e-6),
                    e_ccsd_fp64=res64["e_ccsd_final"], e_t_fp64=res64["e_t"],
                    fp64_tflops=(res64["niter"] * ccsd_flops(no, nv) + triples_flops(no, nv)) / wall64 * 1e-12,
                    fp64_kernels=st64)
+        if args.save_tiles:
+            ctx.save_tile_table(args.save_tiles)
+        if not args.no_cold:
+            cm = cold_solve(ugx, no, nv, fock, blocks, mixed_opts, net, "mixed")
+            cf = cold_solve(ugx, no, nv, fock, blocks, fp64_opts, None, "fp64 (1e-10)")
+            out["cold"] = dict(mixed=cm, fp64=cf, speedup_vs_fp64=cf["first_solve_s"] / cm["first_solve_s"],
+                               note="first ugx_ccsd after ugx_load_hamiltonian on a fresh context (tuned tiles "
+                                    "from the process's table): prepare (conversions, allocations) + solve")
         niter64 = res64["niter"]
         dtype = "%s/f64" % args.low
         config = {"workload": wl["name"], "no": no, "nv": nv, "form": "mixed", "low_precision": args.low,
                   "autotune": not args.no_autotune,
                   "polish_threshold": args.polish, "polish": args.polish_mode, "r_convergence": args.mixed_conv,
                   "guess": args.guess,
-                  "fp64_r_convergence": FP64_CONV}
+                  "fp64_r_convergence": FP64_CONV, "comparison_form_steps": rs, "comparison_form_warmup": rw}
     else:
         conv = FP64_CONV
         wall, res, stats = run_form(ctx, options(ugx.PREC_NAMES[args.form], conv, ugx.GUESS_MP2), args.warmup,
@@ -376,9 +534,10 @@ def main():
     }
     line.update(out)
     line.update({"kernels": stats, "roofline": roofline(stats, args.workload), "cpu_baseline": cpu})
-    for t in ctx.tile_table().splitlines():
-        if " t " in t:  # the (T) W3 launches' tuned tiles
-            log("tile " + t)
+    if args.form == "mixed" and args.workload == "h2o8" and not args.no_batch:
+        ctx.close()
+        args.batch_polish_mode = args.batch_polish_mode or "plain"
+        line["batch64"] = batch_measure(args, batch_problems(), rw, rs, cpu=not args.no_cpu)
     print(json.dumps(line), flush=True)
 
 

@@ -241,8 +241,9 @@ typedef struct {
  * results[p] are those of ugx_ccsd for problem p; seconds_ccsd is the
  * problem's CCSD wall time on its worker, seconds_triples its share of the
  * class's (T) time.  *wall_seconds receives the whole batch's wall time.
- * The context supplies the device, the autotuning setting and the guess
- * network.  Returns UGX_NOT_CONVERGED when any problem did not converge
+ * The context supplies the device, the autotuning and profiling settings
+ * and the guess network; with profiling on, its ugx_kernel_stats then
+ * covers every problem of the batch.  Returns UGX_NOT_CONVERGED when any problem did not converge
  * (its (T) is then skipped). */
 int ugx_batch_solve(ugx_ctx *ctx, int nprob, const ugx_problem *probs, const ugx_options *opt, int nworkers,
                     ugx_result *results, double *wall_seconds);

@@ -56,9 +56,11 @@ def _lib(omp=False):
                                ctypes.c_int, dp, dp]
         L.orc_tjl_per_triplet.argtypes = [vp, dp]
         L.orc_tjl_fov_per_triplet.argtypes = [vp, dp]
+        L.orc_tjl_fov_list.argtypes = [vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long), dp]
         L.orc_w3.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.orc_iteration_sample.argtypes = [vp, ctypes.c_int]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_residual_rows.argtypes = [vp, ctypes.c_int, dp, dp]
         _libs[omp] = L
     return _libs[omp]
@@ -93,6 +95,10 @@ class Oracle:
     @property
     def threads(self):
         return self.L.orc_num_threads()
+
+    def set_threads(self, n):
+        """OpenMP threads of this library's loops (process-wide)."""
+        self.L.orc_set_threads(int(n))
 
     def energy(self):
         return self.L.orc_energy(self.h)
@@ -185,6 +191,13 @@ class Oracle:
         (their sum is tcorr's value for any f_ov)."""
         e = np.zeros(self.ntriplets())
         self.L.orc_tjl_fov_per_triplet(self.h, _p(e))
+        return e
+
+    def tjl_fov_list(self, idx):
+        """Folded per-triplet energies of the triplets idx (i>=j>=k order)."""
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        e = np.zeros(idx.size)
+        self.L.orc_tjl_fov_list(self.h, idx.size, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), _p(e))
         return e
 
     def w3(self, i, j, k):

@@ -860,6 +860,25 @@ void orc_tjl_fov_per_triplet(void *p, double *e_out) {
   }
 }
 
+/* Folded per-triplet energies (tjl_triplet_x with fov) of the n triplets
+ * idx[0..n) of the i>=j>=k order, in parallel over the list (sampled
+ * parity at sizes where the whole (T) is too slow). */
+void orc_tjl_fov_list(void *p, long n, const long *idx, double *e_out) {
+  orc_state *s = (orc_state *)p;
+  size_t vvv = (size_t)s->nv * s->nv * s->nv;
+#pragma omp parallel
+  {
+    double *W3 = zalloc(vvv), *V3b = zalloc(vvv);
+#pragma omp for schedule(dynamic, 1)
+    for (long q = 0; q < n; q++) {
+      int i, j, k;
+      triplet_of(idx[q], &i, &j, &k);
+      e_out[q] = tjl_triplet_x(s, i, j, k, W3, V3b, 1);
+    }
+    free(W3); free(V3b);
+  }
+}
+
 /* W3_ijk for one triplet (ccwfn.cc:1388-1418), for kernel parity. */
 void orc_w3(void *p, int i, int j, int k, double *W3) {
   orc_state *s = (orc_state *)p;
@@ -907,6 +926,16 @@ void orc_residual_rows(void *p, int rows, double *R1, double *R2) {
   memcpy(R2, s->t2, (size_t)rows * no * nv * nv * sizeof(double));
   amp_save(s);
   orc_build_tau(s);
+}
+
+/* Threads of the OpenMP build's parallel loops (1 times the serial
+ * reference loop nest on the same state). */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
 }
 
 int orc_num_threads(void) {

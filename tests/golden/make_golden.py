@@ -19,7 +19,7 @@ Fixtures (each records its generator and seed):
   energies stored, the integral blocks described by checksums (rebuilt by
   the front end).
 - h2o4_seed1: the seeded (H2O)4 cluster (o=20, v=76), solved to 1e-10 by the
-  oracle (OpenMP): energies, t1 and the t2 rows i < 4 stored, fock and the
+  oracle (OpenMP): energies, t1 and the t2 row i = 0 stored, fock and the
   blocks described by checksums.
 """
 import os
@@ -113,7 +113,7 @@ def make_h2o4():
     from ugacc_amd.chem import molecules
     ref = _molecule(molecules.water_cluster(4, seed=1), "cc-pvdz")
     t0 = time.time()
-    sc, out = _oracle_results(ref.fock, ref.full_ints(), ref.no, 1e-10, omp=True, rows=4)
+    sc, out = _oracle_results(ref.fock, ref.full_ints(), ref.no, 1e-10, omp=True, rows=1)
     sc.update(escf=ref.escf, efzc=ref.efzc, enuc=ref.enuc, r_convergence=1e-10, oracle_seconds=time.time() - t0)
     sizes = dict(nmo=ref.nmo, nact=ref.no + ref.nv, no=ref.no, nv=ref.nv, nfzc=0, nfzv=0)
     gen = dict(kind="molecule", builder="ugacc_amd.chem.molecules.water_cluster", n=4, seed=1, basis="cc-pvdz",

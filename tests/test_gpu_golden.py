@@ -104,7 +104,7 @@ def test_h2o_fixture():
 def test_h2o4_fixture_converged_to_1e10():
     """(H2O)4 at the configuration's R_CONVERGENCE of 1e-10 against the
     oracle's complete solve (E_MP2, E_CCSD, E_(T) to 1e-10; t1 and the t2
-    rows i < 4 to 1e-9)."""
+    row i = 0 to 1e-9)."""
     announce("golden h2o4_seed1 (FP64, 1e-10)")
     d = _read("h2o4_seed1")
     ref = _rebuild(d)
@@ -117,7 +117,7 @@ def test_h2o4_fixture_converged_to_1e10():
     assert abs(r["e_t"] - d.scalars["e_t_fov_tjl"]) < 1e-10
     t1, t2 = c.amplitudes()
     assert np.max(np.abs(t1 - d["t1"])) < 1e-9
-    rows = d["t2_rows4"]
+    rows = d["t2_rows1"]
     assert np.max(np.abs(t2[: rows.shape[0]] - rows)) < 1e-9
     print("(H2O)4: E_CCSD %.12f (oracle %.12f), E_(T) %.12f (oracle %.12f), %d iterations (oracle %d)"
           % (r["e_ccsd"], d.scalars["eccsd"], r["e_t"], d.scalars["e_t_fov_tjl"], r["niter"],

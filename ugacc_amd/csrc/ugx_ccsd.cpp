@@ -71,6 +71,7 @@ struct ugx_ctx {
   // (T)
   DBuf tP, tQ, tWbuf, ttrip, tptrs, tE, tPart;
   DBuf tOff;  // blocked-W offset tables of the three (T) launches
+  DBuf tOT2, tT1F;  // f_ov energy kernel: (<ij|ab>, t2) and (t1, f_ov) pair arrays
   DBuf tProb, tCounter;  // energy kernel: problem table (one entry), work-queue counter
   int tOff_blk = 0, tOff_v = 0;
   // per launch g: the column table of the old orientation, used as the row
@@ -229,6 +230,9 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   const hipStream_t st = c->st;
   const int P64 = PREC_FP64;
   const bool base = mode == RES_BASE, delta = mode == RES_DELTA, plain = mode == RES_PLAIN;
+  // the engine's sizing mode (prepare_solve): contractions are only planned,
+  // and the streaming kernels and copies between them are skipped
+  const bool dry = E.sizing();
   if (base && prec != P64) throw UgxError(UGX_BAD_INPUT, "residual: the base point is evaluated in FP64");
   if (!plain) ensure_refine(c);
   // the small (t1) terms: FP64 in the polish; in a plain evaluation prec,
@@ -256,12 +260,14 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   auto rebuilt = [&](const void *p) { E.refresh(p, cdt_lo, cdt_hi); };
   // base point: keep a copy of an operand; differences from it
   auto keep = [&](DBuf &dst, const void *src, size_t bytes) {
-    check(hipMemcpyAsync(dst.p, src, bytes, hipMemcpyDeviceToDevice, st), "copy");
+    if (!dry) check(hipMemcpyAsync(dst.p, src, bytes, hipMemcpyDeviceToDevice, st), "copy");
   };
-  auto dif = [&](DBuf &dst, const double *x, const DBuf &xs, long n) { launch_diff(n, x, xs.d(), dst.d(), st); };
+  auto dif = [&](DBuf &dst, const double *x, const DBuf &xs, long n) {
+    if (!dry) launch_diff(n, x, xs.d(), dst.d(), st);
+  };
   // adds an output's base sum of large contractions (RES_BASE, RES_DELTA)
   auto add_sum = [&](const DBuf &sum, double *out, long n) {
-    if (!plain) launch_axpby(n, 1.0, sum.d(), 1.0, out, st);
+    if (!plain && !dry) launch_axpby(n, 1.0, sum.d(), 1.0, out, st);
   };
   // one large contraction out += alpha A.B; rs is the output's base sum
   // (first: the first large term of that output).  dA / dB are the
@@ -281,7 +287,7 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
 
   // U = 2 t2 - t2(ab swapped), the spin-adapted combination used by build_t1
   // (ccwfn.cc:455,461) and the Wmbej ring term (ccwfn.cc:518-519).
-  launch_build_U(o, v, t2, c->U.d(), st);
+  if (!dry) launch_build_U(o, v, t2, c->U.d(), st);
   const auto UU = V(c->U.d(), {o, o, v, v});
   if (base) {
     keep(c->rf_t2, t2, n4 * 8); keep(c->rf_tau, tau, n4 * 8); keep(c->rf_ttau, ttau, n4 * 8);
@@ -295,23 +301,23 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
 
   // ---- build_F, ccwfn.cc:311-356
   const auto FVV = V(c->Fvv.d(), {v, v}), FOO = V(c->Foo.d(), {o, o}), FOVI = V(c->Fov.d(), {o, v});
-  check(hipMemcpyAsync(c->Fvv.p, c->fvv.p, (size_t)v * v * 8, hipMemcpyDeviceToDevice, st), "copy");
+  if (!dry) check(hipMemcpyAsync(c->Fvv.p, c->fvv.p, (size_t)v * v * 8, hipMemcpyDeviceToDevice, st), "copy");
   E.contract("me,ma->ae", -0.5, FOV, T1, 1.0, FVV, SP, "F");
   E.contract("mf,mafe->ae", 1.0, T1, LOVVV, 1.0, FVV, SP, "F");
   big("mnaf,mnef->ae", -1.0, TTAU, &dTTAU, LOOVV, nullptr, nullptr, FVV, V(c->rs_Fvv.d(), {v, v}), true, "F");
   add_sum(c->rs_Fvv, c->Fvv.d(), (long)v * v);
-  check(hipMemcpyAsync(c->Foo.p, c->foo.p, (size_t)o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
+  if (!dry) check(hipMemcpyAsync(c->Foo.p, c->foo.p, (size_t)o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
   E.contract("ie,me->mi", 0.5, T1, FOV, 1.0, FOO, SP, "F");
   E.contract("ne,mnie->mi", 1.0, T1, LOOOV, 1.0, FOO, SP, "F");
   big("inef,mnef->mi", 1.0, TTAU, &dTTAU, LOOVV, nullptr, nullptr, FOO, V(c->rs_Foo.d(), {o, o}), true, "F");
   add_sum(c->rs_Foo, c->Foo.d(), (long)o * o);
-  check(hipMemcpyAsync(c->Fov.p, c->fov.p, (size_t)o * v * 8, hipMemcpyDeviceToDevice, st), "copy");
+  if (!dry) check(hipMemcpyAsync(c->Fov.p, c->fov.p, (size_t)o * v * 8, hipMemcpyDeviceToDevice, st), "copy");
   E.contract("nf,mnef->me", 1.0, T1, LOOVV, 1.0, FOVI, SP, "F");
 
   // ---- build_W, ccwfn.cc:358-430
   const long n_oooo = (long)o * o * o * o;
   const auto WO = V(c->Woooo.d(), {o, o, o, o});
-  check(hipMemcpyAsync(c->Woooo.p, c->oooo.p, (size_t)n_oooo * 8, hipMemcpyDeviceToDevice, st), "copy");
+  if (!dry) check(hipMemcpyAsync(c->Woooo.p, c->oooo.p, (size_t)n_oooo * 8, hipMemcpyDeviceToDevice, st), "copy");
   E.contract("je,mnie->mnij", 1.0, T1, OOOV, 1.0, WO, SP, "Woooo");
   E.contract("ie,nmje->mnij", 1.0, T1, OOOV, 1.0, WO, SP, "Woooo");  // <mn|ej> = <nm|je>
   // the pair matrices S, A of tau (launch_ladder_pack), shared by the tau
@@ -329,15 +335,15 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
         E.contract("ne,qe->nq", 1.0, Cn(vm.d(), {rows, ldm}), Cn(A, {Qs, ldm}), 0.0, V(zm.d(), {rows, Qs}), pr,
                    label);
       else
-        launch_fill((long)rows * Qs, 0.0, zm.d(), st);
+        if (!dry) launch_fill((long)rows * Qs, 0.0, zm.d(), st);
     }
-    launch_pairs_unpack(rows, o, zp.d(), zm.d(), beta, out, st);
+    if (!dry) launch_pairs_unpack(rows, o, zp.d(), zm.d(), beta, out, st);
   };
   // the pair matrices this evaluation contracts: S, A (plain, base) or
   // their differences from the base point (delta)
   const double *pS = c->lad_S.d(), *pA = c->lad_A.d();
   if (c->lad_packed) {
-    launch_ladder_pack(o, v, tau, c->lad_S.d(), c->lad_A.d(), st);
+    if (!dry) launch_ladder_pack(o, v, tau, c->lad_S.d(), c->lad_A.d(), st);
     if (base) {
       keep(c->rf_S, c->lad_S.p, c->lad_S.bytes); keep(c->rf_A, c->lad_A.p, c->lad_A.bytes);
     } else if (delta) {
@@ -365,7 +371,7 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   else if (delta) dif(c->rd_WO, c->Woooo.d(), c->rf_WO, n_oooo);
 
   // Y[n][f][j][b] = 0.5 t2[j][n][f][b] + t1[j][f] t1[n][b]
-  launch_build_Y(o, v, t1, t2, c->Y.d(), st);
+  if (!dry) launch_build_Y(o, v, t1, t2, c->Y.d(), st);
   rebuilt(c->Y.p);
   const auto YY = Cn(c->Y.d(), {o, v, o, v});
   if (base) keep(c->rf_Y, c->Y.p, n4 * 8);
@@ -374,7 +380,7 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   const auto dY = Cn(c->rd_Y.d(), {o, v, o, v});
   // Wmbje stored as WJ[m][e][j][b]
   const auto WJ = V(c->WJ.d(), {o, v, o, v});
-  check(hipMemcpyAsync(c->WJ.p, c->WJ0.p, (size_t)n4 * 8, hipMemcpyDeviceToDevice, st), "copy");  // -<mb|je>
+  if (!dry) check(hipMemcpyAsync(c->WJ.p, c->WJ0.p, (size_t)n4 * 8, hipMemcpyDeviceToDevice, st), "copy");  // -<mb|je>
   E.contract("jf,mbfe->mejb", -1.0, T1, OVVV, 1.0, WJ, SP, "Wmbje");
   E.contract("nb,mnje->mejb", 1.0, T1, OOOV, 1.0, WJ, SP, "Wmbje");
   const auto RSJ = V(c->rs_WJ.d(), {o, v, o, v});
@@ -382,7 +388,7 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   add_sum(c->rs_WJ, c->WJ.d(), n4);
   // Wmbej stored as WB[m][e][j][b];  <mb|ej> = <mj|eb>
   const auto WB = V(c->WB.d(), {o, v, o, v});
-  check(hipMemcpyAsync(c->WB.p, c->WB0.p, (size_t)n4 * 8, hipMemcpyDeviceToDevice, st), "copy");  // <mj|eb>
+  if (!dry) check(hipMemcpyAsync(c->WB.p, c->WB0.p, (size_t)n4 * 8, hipMemcpyDeviceToDevice, st), "copy");  // <mj|eb>
   E.contract("jf,mbef->mejb", 1.0, T1, OVVV, 1.0, WB, SP, "Wmbej");
   E.contract("nb,nmje->mejb", -1.0, T1, OOOV, 1.0, WB, SP, "Wmbej");
   const auto RSB = V(c->rs_WB.d(), {o, v, o, v});
@@ -402,7 +408,7 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
     // Zmbij on pairs: half the flops
     if (base) {
       pair_term(o * v, c->zm_vp, c->zm_vm, c->zm_P, c->zm_M, pS, pA, 0.0, c->rs_Z.d(), P64, "Zmbij");
-      launch_axpby(n_ovoo, 1.0, c->rs_Z.d(), 0.0, c->Z.d(), st);
+      if (!dry) launch_axpby(n_ovoo, 1.0, c->rs_Z.d(), 0.0, c->Z.d(), st);
     } else {
       pair_term(o * v, c->zm_vp, c->zm_vm, c->zm_P, c->zm_M, pS, pA, 0.0, c->Z.d(), prec, "Zmbij");
       if (delta) add_sum(c->rs_Z, c->Z.d(), n_ovoo);
@@ -412,7 +418,7 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
       E.contract("mbef,ijef->mbij", 1.0, OVVV, TAU, 0.0, ZZ, prec, "Zmbij");
     } else if (base) {
       E.contract("mbef,ijef->mbij", 1.0, OVVV, TAU, 0.0, V(c->rs_Z.d(), {o, v, o, o}), P64, "Zmbij");
-      launch_axpby(n_ovoo, 1.0, c->rs_Z.d(), 0.0, c->Z.d(), st);
+      if (!dry) launch_axpby(n_ovoo, 1.0, c->rs_Z.d(), 0.0, c->Z.d(), st);
     } else {
       E.contract("mbef,ijef->mbij", 1.0, OVVV, dTAU, 0.0, ZZ, prec, "Zmbij");
       add_sum(c->rs_Z, c->Z.d(), n_ovoo);
@@ -440,9 +446,9 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
   E.permute("ijab->ijab", 0.5, OOVV, 0.0, XX);
   // Fae - 0.5 t1.Fme and Fmi + 0.5 t1.Fme fold the t1*Fme terms (:489-491, :496-498)
   const auto FAEP = V(c->FaeP.d(), {v, v}), FMIP = V(c->FmiP.d(), {o, o});
-  check(hipMemcpyAsync(c->FaeP.p, c->Fvv.p, (size_t)v * v * 8, hipMemcpyDeviceToDevice, st), "copy");
+  if (!dry) check(hipMemcpyAsync(c->FaeP.p, c->Fvv.p, (size_t)v * v * 8, hipMemcpyDeviceToDevice, st), "copy");
   E.contract("mb,me->be", -0.5, T1, FOVI, 1.0, FAEP, SP, "t2");
-  check(hipMemcpyAsync(c->FmiP.p, c->Foo.p, (size_t)o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
+  if (!dry) check(hipMemcpyAsync(c->FmiP.p, c->Foo.p, (size_t)o * o * 8, hipMemcpyDeviceToDevice, st), "copy");
   E.contract("je,me->mj", 0.5, T1, FOVI, 1.0, FMIP, SP, "t2");
   if (base) {
     keep(c->rf_FaeP, c->FaeP.p, (size_t)v * v * 8); keep(c->rf_FmiP, c->FmiP.p, (size_t)o * o * 8);
@@ -468,7 +474,7 @@ void residual(ugx_ctx *c, int prec, int mode = RES_PLAIN) {
     if (Qs > 0 && Ps > 0)
       E.contract("qe,ae->qa", 1.0, Cn(pA, {Qs, ldm}), Cn(c->lad_vm.d(), {Ps, ldm}), 0.0,
                  V(c->lad_LM.d(), {Qs, Ps}), pr, "ladder");
-    launch_ladder_unpack(o, v, c->lad_LP.d(), c->lad_LM.d(), 0.25, base ? c->rs_X.d() : c->X.d(), st);
+    if (!dry) launch_ladder_unpack(o, v, c->lad_LP.d(), c->lad_LM.d(), 0.25, base ? c->rs_X.d() : c->X.d(), st);
   } else {
     big("ijef,abef->ijab", 0.5, TAU, &dTAU, VVVV, nullptr, nullptr, XX, RSX, false, "ladder");
   }
@@ -1209,6 +1215,26 @@ std::vector<double> triples_entries(ugx_ctx *c, int prec, const std::vector<TEnt
   int fov = 0;
   for (const TEnt &en : ents)
     if (en.src->has_fov) { fov = 1; break; }
+  // the pair arrays of the f_ov kernel, per problem, from its final t2 / t1
+  std::vector<TripProb> pv = probs;
+  if (fov) {
+    for (size_t q = 0; q < probs.size(); q++) {
+      ugx_ctx *s = nullptr;
+      for (const TEnt &en : ents)
+        if (en.src->eps.p == probs[q].eps) { s = const_cast<ugx_ctx *>(en.src); break; }
+      if (!s) continue;
+      const size_t pe = wf ? 8 : 16;
+      s->tOT2.alloc((size_t)s->oovv_n() * pe);
+      s->tT1F.alloc((size_t)s->ov() * pe);
+      launch_interleave(s->oovv_n(), s->oovv.d(), s->t2.d(), s->tOT2.p, wf, s->st);
+      launch_interleave(s->ov(), s->t1.d(), s->fov.d(), s->tT1F.p, wf, s->st);
+      if (s != c) check(hipStreamSynchronize(s->st), "stream sync");
+      pv[q].ot2 = s->tOT2.p;
+      pv[q].t1f = s->tT1F.p;
+    }
+    check(hipMemcpyAsync(c->tProb.p, pv.data(), pv.size() * sizeof(TripProb), hipMemcpyHostToDevice, c->st),
+          "problem table");
+  }
   for (long t0 = 0; t0 < nt; t0 += nb) {
     int b = (int)std::min<long>(nb, nt - t0);
     if (prec == PREC_BF16X3 && c->t_dma > 1) triples_build_w_dma(c, ents.data() + t0, b);
@@ -1233,7 +1259,7 @@ double triples(ugx_ctx *c, int prec, std::vector<double> *etrip_out) {
   std::vector<TEnt> ents(nt);
   for (long t = 0; t < nt; t++) ents[t] = TEnt{c, c->trip_host[3 * t], c->trip_host[3 * t + 1], c->trip_host[3 * t + 2]};
   std::vector<int> prob(nt, 0);
-  std::vector<double> e = triples_entries(c, prec, ents, prob, {TripProb{c->eps.d(), c->oovv.d(), c->t1.d(), c->t2.d(), c->fov.d()}});
+  std::vector<double> e = triples_entries(c, prec, ents, prob, {TripProb{c->eps.d(), c->oovv.d(), c->t1.d(), nullptr, nullptr}});
   double et = 0.0;
   for (double x : e) et += x;
   if (etrip_out) {
@@ -1860,6 +1886,7 @@ int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_o
       if (rc != UGX_OK) throw UgxError(rc, "ugx_batch_solve: context creation failed");
       sub.emplace_back(s, ugx_destroy);
       s->eng->set_autotune(c->eng->autotune());
+      s->eng->set_profiling(c->eng->profiling());
       s->mlp = c->mlp;
       s->t_batch = c->t_batch;
       s->t_zgroup = c->t_zgroup;
@@ -1947,7 +1974,7 @@ int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_o
         std::vector<TripProb> tp;
         for (size_t q = 0; q < mem.size(); q++) {
           ugx_ctx *s = sub[mem[q]].get();
-          tp.push_back(TripProb{s->eps.d(), s->oovv.d(), s->t1.d(), s->t2.d(), s->fov.d()});
+          tp.push_back(TripProb{s->eps.d(), s->oovv.d(), s->t1.d(), nullptr, nullptr});
           const std::vector<int> &tr = s->trip_host;
           for (size_t t = 0; t + 2 < tr.size(); t += 3) {
             ents.push_back(TEnt{s, tr[t], tr[t + 1], tr[t + 2]});
@@ -1962,6 +1989,15 @@ int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_o
     }
     if (wall_seconds)
       *wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    // profiling on the calling context: its statistics gain every
+    // problem's device time per label (ugx_kernel_stats)
+    if (c->eng->profiling()) {
+      c->eng->reset_counters();
+      for (auto &s : sub) {
+        s->eng->collect_stats();
+        c->eng->merge_stats(s->eng->stats());
+      }
+    }
     if (status == UGX_NOT_CONVERGED) c->err = "Computation has not converged.";
     return status;
   });

@@ -66,6 +66,15 @@ class Engine {
   void set_profiling(bool on) { profiling_ = on; }
   void collect_stats();  // waits for the stream, folds event pairs into stats_
   std::map<std::string, KernelStat> &stats() { return stats_; }
+  // Adds another engine's collected per-label statistics to this one's.
+  void merge_stats(const std::map<std::string, KernelStat> &other) {
+    for (auto &kv : other) {
+      KernelStat &k = stats_[kv.first];
+      k.launches += kv.second.launches;
+      k.ms += kv.second.ms;
+      k.flops += kv.second.flops;
+    }
+  }
   double flops_total() const { return flops_; }
   void add_flops(double f) { flops_ += f; }
   bool profiling() const { return profiling_; }

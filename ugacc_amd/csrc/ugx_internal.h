@@ -236,10 +236,16 @@ inline long tw_off_c(long x, int T) { return (x / T) * T * T * T + x % T; }
 // with a kernel), part holds ntrip * triples_energy_npairs(nv, w_f32)
 // doubles.
 // Per-problem operands of the (T) energy kernel: orbital energies, the oovv
-// block, t1, and (read only by the f_ov variant) t2 and f_ov[i][a].
+// block and t1; for the f_ov variant instead of oovv and t1 the pair arrays
+// ot2[i][j][a][b] = (<ij|ab>, t2_ij^ab) and t1f[i][a] = (t1_ia, f_ia) in the
+// kernel's pair type (float2 with an FP32 W, double2 with an FP64 W; see
+// launch_interleave).
 struct TripProb {
-  const double *eps, *oovv, *t1, *t2, *fov;
+  const double *eps, *oovv, *t1;
+  const void *ot2, *t1f;
 };
+// out[k] = (x[k], y[k]) for k < n, as float2 (f32) or double2
+void launch_interleave(long n, const double *x, const double *y, void *out, int f32, hipStream_t st);
 int triples_energy_npairs(int nv, int w_f32);
 // fov != 0: includes the f_ov t3 terms of the reference's tcorr (any
 // problem of the batch with a non-zero occupied-virtual Fock block)

@@ -200,6 +200,16 @@ __global__ void k_fill(long n, double v, double *y) {
   for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) y[i] = v;
 }
 
+template <typename P>
+__global__ void k_interleave(long n, const double *__restrict__ x, const double *__restrict__ y, P *__restrict__ out) {
+  for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    P p;
+    p.x = x[i];
+    p.y = y[i];
+    out[i] = p;
+  }
+}
+
 // flag[0] |= 1 when any of the n values is NaN or infinite
 __global__ void k_nonfinite(long n, const double *__restrict__ x, double *flag) {
   bool bad = false;
@@ -640,6 +650,12 @@ void launch_permute(const PermDesc &p, hipStream_t st) {
 void launch_fill(long n, double v, double *y, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(NT), 0, st, n, v, y);
+}
+
+void launch_interleave(long n, const double *x, const double *y, void *out, int f32, hipStream_t st) {
+  if (n <= 0) return;
+  if (f32) hipLaunchKernelGGL(k_interleave<float2>, dim3(nblocks(n)), dim3(NT), 0, st, n, x, y, (float2 *)out);
+  else hipLaunchKernelGGL(k_interleave<double2>, dim3(nblocks(n)), dim3(NT), 0, st, n, x, y, (double2 *)out);
 }
 
 void launch_nonfinite(long n, const double *x, double *flag, hipStream_t st) {

@@ -14,9 +14,11 @@
 // (ccwfn.cc:706-709, 725), the values the reference returns: since t2 has
 // the pair symmetry of <ij|ab>, that part of the energy is the bracket's
 // disconnected term with <pq|xy> -> t2_pq^xy and t1 -> f_ov.  The FOV
-// variant of the kernel adds it (any f_ov != 0); it stages each <pq|xy>
-// element with its t2 and each t1 with its f_ov as one vector pair, so the
-// number of LDS accesses stays that of the plain kernel.
+// variant of the kernel adds it (any f_ov != 0); it reads <pq|xy> with its
+// t2 and t1 with its f_ov as one vector pair (arrays interleaved once per
+// (T) call in the kernel's element type, so a prefetch is one wide load
+// with no conversion), and stages them in LDS as such, so the numbers of
+// loads and LDS accesses stay those of the plain kernel.
 //
 // Every point needs W at all six index permutations.  A workgroup owns one
 // triplet and a pair of E-wide index blocks A >= B, and walks the blocks
@@ -116,7 +118,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
   const int i = trip[4 * t], j = trip[4 * t + 1], k = trip[4 * t + 2];
   const TripProb pr = probs[trip[4 * t + 3]];
   const double *__restrict__ eps = pr.eps, *__restrict__ oovv = pr.oovv, *__restrict__ t1 = pr.t1;
-  const double *__restrict__ t2 = pr.t2, *__restrict__ fov = pr.fov;
+  const OE *__restrict__ ot2 = (const OE *)pr.ot2;
+  const TE *__restrict__ t1f = (const TE *)pr.t1f;
   const WT *W = Wall + (long)t * wstride;
   const long v = nv, v2 = (long)nv * nv;
   // slices selected by value (a runtime index into a pointer array would
@@ -126,11 +129,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
   const double *T10 = t1 + (long)k * nv, *T11 = t1 + (long)j * nv, *T12 = t1 + (long)i * nv;
   auto osl = [&](int sl) { return sl == 0 ? O0 : (sl == 1 ? O1 : O2); };
   auto t1sl = [&](int sl) { return sl == 0 ? T10 : (sl == 1 ? T11 : T12); };
-  // FOV: the t2 slices (ij), (ik), (jk) and the f_ov rows k, j, i
-  const double *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr, *F0 = nullptr, *F1 = nullptr, *F2 = nullptr;
+  // FOV: the (<pq|xy>, t2) pair slices (ij), (ik), (jk) and the (t1, f_ov)
+  // pair rows k, j, i, in the kernel's pair types (one load each)
+  const OE *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr;
+  const TE *F0 = nullptr, *F1 = nullptr, *F2 = nullptr;
   if constexpr (FOV) {
-    Q0 = t2 + ((long)i * no + j) * v2; Q1 = t2 + ((long)i * no + k) * v2; Q2 = t2 + ((long)j * no + k) * v2;
-    F0 = fov + (long)k * nv; F1 = fov + (long)j * nv; F2 = fov + (long)i * nv;
+    Q0 = ot2 + ((long)i * no + j) * v2; Q1 = ot2 + ((long)i * no + k) * v2; Q2 = ot2 + ((long)j * no + k) * v2;
+    F0 = t1f + (long)k * nv; F1 = t1f + (long)j * nv; F2 = t1f + (long)i * nv;
   }
   auto qsl = [&](int sl) { return sl == 0 ? Q0 : (sl == 1 ? Q1 : Q2); };
   auto fsl = [&](int sl) { return sl == 0 ? F0 : (sl == 1 ? F1 : F2); };
@@ -159,10 +164,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     const int gx = gx0 + lx, gy = gy0 + ly;
     const bool in = gx < nv && gy < nv;
     if constexpr (FOV) {
-      OE r;
-      r.x = (OT)(in ? osl(s)[gx * v + gy] : 0.0);
-      r.y = (OT)(in ? qsl(s)[gx * v + gy] : 0.0);
-      return r;
+      return in ? qsl(s)[gx * v + gy] : OE{};
     } else {
       return (OE)(in ? osl(s)[gx * v + gy] : 0.0);
     }
@@ -170,10 +172,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
   // t1 (and f_ov) of slice s at global virtual index gx (< nv)
   auto tget = [&](int s, int gx) {
     if constexpr (FOV) {
-      TE r;
-      r.x = (OT)t1sl(s)[gx];
-      r.y = (OT)fsl(s)[gx];
-      return r;
+      return fsl(s)[gx];
     } else {
       return t1sl(s)[gx];
     }
