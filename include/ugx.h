@@ -219,6 +219,19 @@ size_t ugx_kernel_stats(ugx_ctx *ctx, char *buf, size_t buflen);
 int ugx_gemm(ugx_ctx *ctx, char opa, char opb, int M, int N, int K, double alpha, const void *A, long lda,
              const void *B, long ldb, double beta, double *C, long ldc, int precision, int tile);
 
+/* ugx_gemm with UGX_PREC_BF16X3 (FP32 operands A, B in any layout), without
+ * any device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The BF16 hi/lo staging of A and B lives
+ * in a 64 MiB device array of the library, one per GPU, held until the
+ * process ends: (M + N) K must stay within 16 Mi (2^24) elements, otherwise
+ * UGX_BAD_INPUT.  The work is queued on the context's stream (ugx_synchronize
+ * waits for it); every call on one GPU shares that array, so calls that
+ * overlap on one GPU (several contexts or threads) race.  Results are
+ * bitwise those of ugx_gemm(..., UGX_PREC_BF16X3, tile). */
+int ugx_gemm_bf16x3_static(ugx_ctx *ctx, char opa, char opb, int M, int N, int K, double alpha, const float *A,
+                           long lda, const float *B, long ldb, double beta, double *C, long ldc, int tile);
+
 /* Synchronises the context's stream. */
 int ugx_synchronize(ugx_ctx *ctx);
 

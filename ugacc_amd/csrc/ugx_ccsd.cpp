@@ -1293,6 +1293,58 @@ void validate_options(ugx_ctx *c, const ugx_options &opt, const char *who) {
     throw UgxError(UGX_BAD_INPUT, w + ": UGX_GUESS_MLP without ugx_set_guess_network");
 }
 
+// BF16x3 GEMM of FP32 operands in any layout (d's segment 0 as built by
+// ugx_gemm): the operands are split into BF16 hi/lo pairs A[m][k] at pa
+// and B[n][k] at pb (M K and N K pairs, 4 bytes each), then multiplied by
+// the pre-split kernel.  Nothing is allocated here.
+void gemm_bf16x3_split(ugx_ctx *c, GemmDesc &d, void *pa, void *pb, int tile) {
+  GemmSeg &g = d.seg[0];
+  const int M = d.M, N = d.N, K = g.K;
+  PermDesc pd{};
+  pd.rank = 2; pd.in_t = 1; pd.out_t = 2; pd.alpha = 1.0; pd.beta = 0.0;
+  pd.od[0] = M; pd.od[1] = K; pd.istr[0] = g.sam; pd.istr[1] = g.sak; pd.in = g.A; pd.out = pa;
+  if ((long)M * K > 0) launch_permute(pd, c->st);
+  pd.od[0] = N; pd.od[1] = K; pd.istr[0] = g.sbn; pd.istr[1] = g.sbk; pd.in = g.B; pd.out = pb;
+  if ((long)N * K > 0) launch_permute(pd, c->st);
+  g.A = pa; g.B = pb; g.sam = K; g.sak = 1; g.sbn = K; g.sbk = 1;
+  g.loA = (long)M * K; g.loB = (long)N * K;
+  launch_gemm(d, nullptr, PREC_BF16X3_PRESPLIT, tile, c->st);
+}
+
+// ------------------------------------------------------- static scratch
+// Staging of ugx_gemm_bf16x3_static (the BF16 hi/lo pairs of its operands):
+// a file-scope device array of kStaticScratchBytes, one copy per GPU, held
+// until the process ends.  In host code the array's name is not a device
+// address: ugx_create fetches each device's address of it with
+// hipGetSymbolAddress (with that device current) into g_scratch, and only
+// those addresses are used.
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+constexpr int kStaticScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+std::mutex g_scratch_mu;
+void *g_scratch[kStaticScratchDevices] = {};
+bool g_scratch_tried[kStaticScratchDevices] = {};
+
+// Once per device, with that device current.
+void static_scratch_setup(int device) {
+  if (device < 0 || device >= kStaticScratchDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_tried[device]) return;
+  g_scratch_tried[device] = true;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+    (void)hipGetLastError();
+    p = nullptr;
+  }
+  g_scratch[device] = p;
+}
+
+void *static_scratch(int device) {
+  if (device < 0 || device >= kStaticScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[device];
+}
+
 template <typename F>
 int guarded(ugx_ctx *c, F &&f) {
   if (!c) return UGX_BAD_INPUT;
@@ -1355,6 +1407,7 @@ int ugx_create(ugx_ctx **out, int device) {
     return UGX_HIP_ERROR;
   }
   c->eng.reset(new Engine(c->st));
+  static_scratch_setup(device);  // the device's address of the static scratch (device current)
   // tuning knobs: LDS stages of the BF16x3 kernel (1 or 2); triplets per
   // (T) batch; triplets per output tile of a W3 launch; the W3 kernel (0:
   // register-staged operands, 2 or 3: panels copied into LDS, with that many
@@ -1832,23 +1885,42 @@ int ugx_gemm(ugx_ctx *c, char opa, char opb, int M, int N, int K, double alpha, 
       check(hipStreamSynchronize(c->st), "stream sync");
     } else if (precision == UGX_PREC_BF16X3) {
       // FP32 operands in any layout: split into BF16 hi/lo pairs A[m][k],
-      // B[n][k] first
+      // B[n][k] in two staging buffers first
       DBuf pa, pb;
       pa.alloc(std::max<size_t>(1, (size_t)M * K) * 4);
       pb.alloc(std::max<size_t>(1, (size_t)N * K) * 4);
-      PermDesc pd{};
-      pd.rank = 2; pd.in_t = 1; pd.out_t = 2; pd.alpha = 1.0; pd.beta = 0.0;
-      pd.od[0] = M; pd.od[1] = K; pd.istr[0] = g.sam; pd.istr[1] = g.sak; pd.in = A; pd.out = pa.p;
-      if ((long)M * K > 0) launch_permute(pd, c->st);
-      pd.od[0] = N; pd.od[1] = K; pd.istr[0] = g.sbn; pd.istr[1] = g.sbk; pd.in = B; pd.out = pb.p;
-      if ((long)N * K > 0) launch_permute(pd, c->st);
-      g.A = pa.p; g.B = pb.p; g.sam = K; g.sak = 1; g.sbn = K; g.sbk = 1;
-      g.loA = (long)M * K; g.loB = (long)N * K;
-      launch_gemm(d, nullptr, PREC_BF16X3_PRESPLIT, tile, c->st);
+      gemm_bf16x3_split(c, d, pa.p, pb.p, tile);
       check(hipStreamSynchronize(c->st), "stream sync");
     } else {
       launch_gemm(d, nullptr, precision, tile, c->st);
     }
+    check(hipGetLastError(), "gemm launch");
+    return UGX_OK;
+  });
+}
+
+int ugx_gemm_bf16x3_static(ugx_ctx *c, char opa, char opb, int M, int N, int K, double alpha, const float *A,
+                           long lda, const float *B, long ldb, double beta, double *C, long ldc, int tile) {
+  return guarded(c, [&]() {
+    if (M < 0 || N < 0 || K < 0 || !A || !B || !C || tile >= gemm_num_tiles())
+      throw UgxError(UGX_BAD_INPUT, "ugx_gemm_bf16x3_static: bad arguments");
+    // pairs of A from offset 0, pairs of B from the next 256-byte boundary
+    const size_t bytes_a = ((size_t)M * K * 4 + 255) / 256 * 256, bytes_b = (size_t)N * K * 4;
+    if (bytes_a + bytes_b > kStaticScratchBytes)
+      throw UgxError(UGX_BAD_INPUT, "ugx_gemm_bf16x3_static: operands need " + std::to_string(bytes_a + bytes_b) +
+                                        " bytes of staging; the static scratch holds " +
+                                        std::to_string(kStaticScratchBytes) + " (64 MiB): (M + N) K <= 16 Mi");
+    unsigned char *base = (unsigned char *)static_scratch(c->device);
+    if (!base) throw UgxError(UGX_HIP_ERROR, "ugx_gemm_bf16x3_static: no static scratch on this device");
+    GemmDesc d{};
+    d.M = M; d.N = N; d.nseg = 1; d.batch = 1;
+    GemmSeg &g = d.seg[0];
+    g.A = A; g.B = B; g.K = K; g.sign = 1.0;
+    const bool an = (opa == 'N' || opa == 'n'), bn = (opb == 'N' || opb == 'n');
+    if (an) { g.sam = lda; g.sak = 1; } else { g.sam = 1; g.sak = lda; }
+    if (bn) { g.sbk = ldb; g.sbn = 1; } else { g.sbk = 1; g.sbn = ldb; }
+    d.C = C; d.scm = ldc; d.scn = 1; d.alpha = alpha; d.beta = beta;
+    gemm_bf16x3_split(c, d, base, base + bytes_a, tile);
     check(hipGetLastError(), "gemm launch");
     return UGX_OK;
   });

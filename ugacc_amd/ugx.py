@@ -26,7 +26,7 @@ EXPORTS = [
     "ugx_default_options", "ugx_create", "ugx_destroy", "ugx_last_error", "ugx_load_hamiltonian",
     "ugx_set_amplitudes", "ugx_prepare", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
     "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_set_profiling",
-    "ugx_kernel_stats", "ugx_gemm", "ugx_synchronize", "ugx_batch_solve", "ugx_set_autotune", "ugx_tile_table",
+    "ugx_kernel_stats", "ugx_gemm", "ugx_gemm_bf16x3_static", "ugx_synchronize", "ugx_batch_solve", "ugx_set_autotune", "ugx_tile_table",
     "ugx_save_tile_table", "ugx_load_tile_table", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
     "ugx_mlp_last_error", "ugx_mlp_set_params", "ugx_mlp_get_params", "ugx_mlp_set_scaling", "ugx_mlp_forward",
     "ugx_mlp_loss_grad", "ugx_mlp_train", "ugx_features", "ugx_set_guess_network",
@@ -103,6 +103,8 @@ def load_library(path=LIBPATH):
     L.ugx_kernel_stats.restype = ctypes.c_size_t
     L.ugx_gemm.argtypes = [vp, ctypes.c_char, ctypes.c_char, ip, ip, ip, cd, vp, ctypes.c_long, vp, ctypes.c_long,
                            cd, vp, ctypes.c_long, ip, ip]
+    L.ugx_gemm_bf16x3_static.argtypes = [vp, ctypes.c_char, ctypes.c_char, ip, ip, ip, cd, vp, ctypes.c_long, vp,
+                                         ctypes.c_long, cd, vp, ctypes.c_long, ip]
     L.ugx_synchronize.argtypes = [vp]
     L.ugx_batch_solve.argtypes = [vp, ip, ctypes.POINTER(Problem), ctypes.POINTER(Options), ip,
                                   ctypes.POINTER(Result), dp]
@@ -291,6 +293,12 @@ class Context:
              tile=-1):
         self._check(self.L.ugx_gemm(self.h, opa.encode(), opb.encode(), M, N, K, alpha, A_ptr, lda, B_ptr, ldb,
                                     beta, C_ptr, ldc, precision, tile))
+
+    def gemm_bf16x3_static(self, opa, opb, M, N, K, alpha, A_ptr, lda, B_ptr, ldb, beta, C_ptr, ldc, tile=-1):
+        """ugx_gemm_bf16x3_static: BF16x3 GEMM of FP32 device operands with the
+        library's static staging array (no device allocation)."""
+        self._check(self.L.ugx_gemm_bf16x3_static(self.h, opa.encode(), opb.encode(), M, N, K, alpha, A_ptr, lda,
+                                                  B_ptr, ldb, beta, C_ptr, ldc, tile))
 
     def synchronize(self):
         self._check(self.L.ugx_synchronize(self.h))
