@@ -63,6 +63,9 @@ struct ugx_ctx {
   DBuf Woooo, WJ, WB, Z, Zx, P2;
   DBuf R1, X;
   DBuf part;
+  // pinned landing buffer of the per-iteration scalar readbacks (partial
+  // sums, DIIS dots, t1 for the energy)
+  HBuf hpin;
   // DIIS
   std::vector<std::unique_ptr<DBuf>> diis_err, diis_amp;
   std::vector<double> diis_B;  // nvec x nvec, slot order
@@ -105,9 +108,11 @@ struct ugx_ctx {
   const ugx_mlp *mlp = nullptr;
   DBuf G;
   int t_batch = 0;  // cap on triplets per (T) batch (0: by the W budget)
-  long t_wbytes = 12L << 30;  // (T) W buffer budget, bytes (at most a quarter of the free HBM)
+  long t_wbytes = 48L << 30;  // (T) W buffer budget, bytes (at most a quarter of the free HBM)
   int t_zgroup = 4;  // consecutive triplets per output tile in the BF16x3 W3 launches
   int t_dma = 2;     // > 1: BF16x3 W3 launches copy panels into LDS, with that many stages (0: registers)
+  int t_sort = 1;    // BF16x3 W3 launches: each launch's entries ordered by its own panel pair
+  int t_align = 1;   // (T) batches end at a triplet-block boundary (all triplets of one i)
   // hipGraphs of the residual, keyed by (amplitude buffer parity, precision
   // + 4 for the refinement polish's difference residuals)
   struct ResidualGraph {
@@ -141,8 +146,8 @@ void upload(DBuf &b, const double *h, size_t n, hipStream_t st) {
 }
 
 double host_sum(ugx_ctx *c, int n) {
-  std::vector<double> h(n);
-  check(hipMemcpyAsync(h.data(), c->part.p, n * 8, hipMemcpyDeviceToHost, c->st), "partials copy");
+  double *h = c->hpin.p;
+  check(hipMemcpyAsync(h, c->part.p, n * 8, hipMemcpyDeviceToHost, c->st), "partials copy");
   check(hipStreamSynchronize(c->st), "stream sync");
   double s = 0.0;
   for (int i = 0; i < n; i++) s += h[i];
@@ -161,9 +166,9 @@ double energy_of(ugx_ctx *c, int form, const double *t1dev) {
   launch_energy(no, nv, c->fov.d(), t1dev, c->tau.d(), c->Loovv.d(), c->part.d(), np, c->st);
   // the tau.L partials and t1 come back with one synchronisation; f_ov
   // is kept on the host since load
-  std::vector<double> h(np), t1(c->ov());
-  check(hipMemcpyAsync(h.data(), c->part.p, np * 8, hipMemcpyDeviceToHost, c->st), "partials copy");
-  check(hipMemcpyAsync(t1.data(), t1dev, c->ov() * 8, hipMemcpyDeviceToHost, c->st), "t1 copy");
+  double *h = c->hpin.p, *t1 = c->hpin.p + np;
+  check(hipMemcpyAsync(h, c->part.p, np * 8, hipMemcpyDeviceToHost, c->st), "partials copy");
+  check(hipMemcpyAsync(t1, t1dev, c->ov() * 8, hipMemcpyDeviceToHost, c->st), "t1 copy");
   check(hipStreamSynchronize(c->st), "stream sync");
   double two = 0.0;
   for (int i = 0; i < np; i++) two += h[i];
@@ -649,8 +654,8 @@ int diis_step(ugx_ctx *c, int nvec, int policy, int min_vecs) {
   // about 64 elements per thread, between 64 and kDiisDotBlocks blocks
   const int np = (int)std::min<long>(kDiisDotBlocks, std::max<long>(64, (len + 256 * 64 - 1) / (256 * 64)));
   launch_diis_dots(len, e.d(), vecs.data(), n, c->part.d(), np, c->st);
-  std::vector<double> h((size_t)n * np);
-  check(hipMemcpyAsync(h.data(), c->part.p, h.size() * 8, hipMemcpyDeviceToHost, c->st), "diis dots copy");
+  const double *h = c->hpin.p;
+  check(hipMemcpyAsync(c->hpin.p, c->part.p, (size_t)n * np * 8, hipMemcpyDeviceToHost, c->st), "diis dots copy");
   check(hipStreamSynchronize(c->st), "stream sync");
   for (int p = 0; p < n; p++) {
     double s = 0.0;
@@ -689,7 +694,10 @@ void alloc_work(ugx_ctx *c) {
   c->Woooo.alloc(o * o * o * o * 8);
   c->Z.alloc(o * v * o * o * 8); c->Zx.alloc(o * v * o * o * 8); c->P2.alloc(o * v * o * o * 8);
   // the DIIS dots write one partial per block for each stored vector
-  c->part.alloc((size_t)std::max(npart_max(c), kDiisDotBlocks * kDiisMax) * 8);
+  const size_t npart = (size_t)std::max(npart_max(c), kDiisDotBlocks * kDiisMax);
+  c->part.alloc(npart * 8);
+  // readbacks: the partials, and the energy's partials followed by t1
+  c->hpin.alloc(std::max<size_t>(npart, (size_t)energy_nparts(c->no, c->nv) + (size_t)ov));
 }
 
 // Everything a ugx_ccsd with options opt needs, allocated and converted
@@ -1092,6 +1100,27 @@ void triples_build_w_dma(ugx_ctx *c, const TEnt *ents, int nb) {
   const long pTT = panel_size(1, v2, o), nTT = panel_size(o, v2, o);         // T2t, T2tq
   const long blk_v = 16L * ((v + 31) / 32 * 32), blk_o = 16L * ((o + 31) / 32 * 32);
   std::vector<GemmPtrs> tab((size_t)3 * nb);
+  // Table position of entry b in each launch.  The big panel operands of
+  // launch z = c are P_i and Q_j, of z = b P_i and Q_k, of z = a P_j and Q_k:
+  // ordering each launch's entries by its own pair makes the zgroup
+  // neighbours (consecutive positions, run tile by tile on one XCD) share
+  // both panels through L2.  The order does not change any entry's result.
+  std::vector<int> pos[3];
+  for (int g = 0; g < 3; g++) {
+    std::vector<int> ord(nb);
+    for (int b = 0; b < nb; b++) ord[b] = b;
+    if (c->t_sort && g > 0) {
+      std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+        const TEnt &X = ents[x], &Y = ents[y];
+        if (X.src != Y.src) return X.src < Y.src;
+        const int x0 = g == 1 ? X.i : X.j, y0 = g == 1 ? Y.i : Y.j;
+        if (x0 != y0) return x0 < y0;
+        return X.k < Y.k;
+      });
+    }
+    pos[g].assign(nb, 0);
+    for (int q = 0; q < nb; q++) pos[g][ord[q]] = q;
+  }
   for (int b = 0; b < nb; b++) {
     const ugx_ctx *src = ents[b].src;  // the entry's own operands (same no, nv)
     const unsigned short *P = (const unsigned short *)src->b_P.p, *Q = (const unsigned short *)src->b_Q.p;
@@ -1099,7 +1128,7 @@ void triples_build_w_dma(ugx_ctx *c, const TEnt *ents, int nb) {
     const unsigned short *T2t = (const unsigned short *)src->p_T2t.p, *T2q = (const unsigned short *)src->p_T2tq.p;
     const long i = ents[b].i, j = ents[b].j, k = ents[b].k;
     double *W = (double *)((char *)c->tWbuf.p + (size_t)b * tw_stride(v, tw_blk(c->tW_f32)) * (c->tW_f32 ? 4 : 8));
-    GemmPtrs &A = tab[b], &B = tab[(size_t)nb + b], &C = tab[(size_t)2 * nb + b];
+    GemmPtrs &A = tab[pos[0][b]], &B = tab[(size_t)nb + pos[1][b]], &C = tab[(size_t)2 * nb + pos[2][b]];
     A.A[0] = T2 + (k * o + j) * pT;   A.B[0] = P + i * pP;
     A.A[1] = OV + (j * o + k) * pO;   A.B[1] = T2t + i * pTT;
     A.A[2] = T2 + (k * o + i) * pT;   A.B[2] = Q + j * pP;
@@ -1176,7 +1205,7 @@ void triples_operands(ugx_ctx *c, int prec) {
 // context ents[e].src (all of one size, operands prepared by
 // triples_operands and complete on the device); prob[e] indexes probs, the
 // matching (eps, oovv, t1) of each problem.  Batches of entries get one W
-// buffer of up to t_wbytes (12 GiB by default): three grouped W3 GEMM launches whose pointer
+// buffer of up to t_wbytes (48 GiB by default): three grouped W3 GEMM launches whose pointer
 // tables span every entry of the batch, then one persistent energy kernel
 // over all (entry, block pair) items.
 std::vector<double> triples_entries(ugx_ctx *c, int prec, const std::vector<TEnt> &ents,
@@ -1188,8 +1217,8 @@ std::vector<double> triples_entries(ugx_ctx *c, int prec, const std::vector<TEnt
   const int wf = (prec != PREC_FP64) ? 1 : 0;  // W is FP32 in the low-precision forms
   const size_t wes = wf ? 4 : 8;
   const long ws = tw_stride(v, tw_blk(wf));
-  // fewer, larger batches: fewer launches and energy-kernel tails (12 GiB
-  // holds 564 (H2O)8 triplets: 42 batches instead of 82 at 2 GiB)
+  // fewer, larger batches: fewer launches and energy-kernel tails (48 GiB
+  // holds ~3400 (H2O)8 triplets: 4 batches of the 11480)
   size_t mem_free = 0, mem_total = 0;
   check(hipMemGetInfo(&mem_free, &mem_total), "memory info");
   const long budget = std::min<long>(c->t_wbytes, (long)(mem_free / 4));
@@ -1235,8 +1264,23 @@ std::vector<double> triples_entries(ugx_ctx *c, int prec, const std::vector<TEnt
     check(hipMemcpyAsync(c->tProb.p, pv.data(), pv.size() * sizeof(TripProb), hipMemcpyHostToDevice, c->st),
           "problem table");
   }
-  for (long t0 = 0; t0 < nt; t0 += nb) {
-    int b = (int)std::min<long>(nb, nt - t0);
+  // batch ranges of at most nb entries, ending at a block boundary (all
+  // triplets of one i of one problem) when one is within the range: the
+  // triplets a launch orders by (i, k) or (j, k) then share their panels
+  std::vector<std::pair<long, long>> ranges;
+  for (long t0 = 0; t0 < nt;) {
+    long t1 = std::min<long>(nt, t0 + nb);
+    if (c->t_align && t1 < nt) {
+      long t = t1;
+      while (t > t0 && ents[t - 1].src == ents[t].src && ents[t - 1].i == ents[t].i) t--;
+      if (t > t0) t1 = t;
+    }
+    ranges.emplace_back(t0, t1);
+    t0 = t1;
+  }
+  for (auto &rg : ranges) {
+    const long t0 = rg.first;
+    int b = (int)(rg.second - rg.first);
     if (prec == PREC_BF16X3 && c->t_dma > 1) triples_build_w_dma(c, ents.data() + t0, b);
     else if (prec == PREC_BF16X3) triples_build_w_pair(c, ents.data() + t0, b);
     else triples_build_w(c, prec, ents.data() + t0, b);
@@ -1417,6 +1461,8 @@ int ugx_create(ugx_ctx **out, int device) {
   if (const char *e = std::getenv("UGX_T_BATCH")) c->t_batch = std::atoi(e);
   if (const char *e = std::getenv("UGX_T_ZGROUP")) c->t_zgroup = std::atoi(e);
   if (const char *e = std::getenv("UGX_T_DMA")) c->t_dma = std::atoi(e);
+  if (const char *e = std::getenv("UGX_T_SORT")) c->t_sort = std::atoi(e);
+  if (const char *e = std::getenv("UGX_T_ALIGN")) c->t_align = std::atoi(e);
   if (const char *e = std::getenv("UGX_T_WGIB")) c->t_wbytes = std::max(1L, std::atol(e)) << 30;
   if (const char *e = std::getenv("UGX_LADDER")) c->lad_full = std::string(e) == "full";
   if (c->t_dma != 0 && c->t_dma != 2 && c->t_dma != 3) c->t_dma = 0;
@@ -1963,6 +2009,8 @@ int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_o
       s->t_batch = c->t_batch;
       s->t_zgroup = c->t_zgroup;
       s->t_dma = c->t_dma;
+      s->t_sort = c->t_sort;
+      s->t_align = c->t_align;
       s->t_wbytes = c->t_wbytes;
       s->lad_full = c->lad_full;
     }

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 
@@ -47,6 +48,33 @@ struct DBuf {
   }
   double *d() const { return (double *)p; }
   float *f() const { return (float *)p; }
+};
+
+// Owning pinned host allocation (device-to-host copies land in it directly,
+// with no staging copy); alloc() only grows.
+struct HBuf {
+  double *p = nullptr;
+  size_t n = 0;  // doubles
+  HBuf() = default;
+  HBuf(const HBuf &) = delete;
+  HBuf &operator=(const HBuf &) = delete;
+  ~HBuf() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (n >= count && p) return;
+    release();
+    void *q = nullptr;
+    if (hipHostMalloc(&q, std::max<size_t>(count, 1) * 8, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      throw UgxError(UGX_OOM, "pinned host allocation of " + std::to_string(count * 8) + " bytes failed");
+    }
+    p = (double *)q;
+    n = count;
+  }
 };
 
 // Inference view of a learned-guess network for a context on `device`
