@@ -1,0 +1,171 @@
+/*
+ * ugacc_mi355x.cc -- the ugacc Psi4 plugin entry with its CCSD / CCSD(T)
+ * hot path on the MI355X library (include/ugx.h, libugx.so).
+ *
+ * It exports the two symbols the reference plugin exports (ugacc.cc:54-76),
+ * from a plugin of the same name (ugacc.so), so psi4.core.plugin_load /
+ * plugin("ugacc.so", scf_wfn) call it unchanged:
+ *   read_options  -- the same option names and defaults (ugacc.cc:58-68)
+ *   ugacc         -- the same header, the same RHF error trapping
+ *                    (ugacc.cc:95-99), then the Hamiltonian, CCWfn and
+ *                    compute_energy steps (ugacc.cc:104-115) on the GPU:
+ *                    the six Dirac integral blocks and the active Fock
+ *                    matrix are built on the host and handed to ugx_ccsd.
+ * Energies are printed in the layout of ccwfn.cc:234-247.  The post-CC
+ * steps of DERTYPE FIRST / SECOND (HBAR, Lambda, densities, response,
+ * ugacc.cc:117-) are not part of this library: such a request fails with
+ * a PSIEXCEPTION instead of returning wrong numbers.
+ *
+ * Integrals: MintsHelper::mo_eri over the active occupied / virtual AO
+ * coefficients gives chemist (pq|rs); the blocks are <pq|rs> = (pr|qs).
+ * Orbitals are ordered by energy within the occupied and virtual spaces
+ * (the reference uses QT order; every energy is invariant to the order).
+ */
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "psi4/libmints/matrix.h"
+#include "psi4/libmints/mintshelper.h"
+#include "psi4/libmints/wavefunction.h"
+#include "psi4/liboptions/liboptions.h"
+#include "psi4/libpsi4util/PsiOutStream.h"
+#include "psi4/libpsi4util/exception.h"
+#include "psi4/psi4-dec.h"
+
+#include "ugx.h"
+
+namespace psi {
+namespace ugacc {
+
+extern "C" PSI_API int read_options(std::string name, Options &options) {
+  if (name == "UGACC" || options.read_globals()) {
+    options.add_int("PRINT", 1);
+    options.add_str("REFERENCE", "RHF");
+    options.add_str("WFN", "CCSD");
+    options.add_str("DERTYPE", "NONE");
+    options.add_str("MYHAND", "RIGHT");
+    options.add_int("MAXITER", 100);
+    options.add_bool("DIIS", true);
+    options.add_double("R_CONVERGENCE", 1e-7);
+    options.add_double("MY_OMEGA", 0.00);
+    options.add_bool("OOC", false);
+    options.add_str("PROPERTY", "POLARIZABILITY", "POLARIZABILITY ROTATION ROA ROA_TENSOR ALL");
+    // MI355X forms (no counterpart in the reference): arithmetic of the
+    // iterations before the FP64 polish, and the GPU to run on
+    options.add_str("UGX_PRECISION", "FP64", "FP64 FP32 BF16X3");
+    options.add_int("UGX_DEVICE", 0);
+  }
+  return true;
+}
+
+namespace {
+
+// Dirac block X[p][q][r][s] = <pq|rs> = (pr|qs) from the chemist matrix
+// G[(p r)][(q s)] of mo_eri(C_p, C_r, C_q, C_s).
+std::vector<double> dirac_block(const SharedMatrix &G, int np, int nq, int nr, int ns) {
+  std::vector<double> X((size_t)np * nq * nr * ns);
+  double **g = G->pointer();
+  for (int p = 0; p < np; p++)
+    for (int q = 0; q < nq; q++)
+      for (int r = 0; r < nr; r++)
+        for (int s = 0; s < ns; s++)
+          X[(((size_t)p * nq + q) * nr + r) * ns + s] = g[(size_t)p * nr + r][(size_t)q * ns + s];
+  return X;
+}
+
+struct Context {
+  ugx_ctx *c = nullptr;
+  ~Context() {
+    if (c) ugx_destroy(c);
+  }
+};
+
+}  // namespace
+
+extern "C" PSI_API SharedWavefunction ugacc(SharedWavefunction ref, Options &options) {
+  outfile->Printf("\t**************************\n");
+  outfile->Printf("\t*                        *\n");
+  outfile->Printf("\t*    UGA-CC (MI355X)     *\n");
+  outfile->Printf("\t*                        *\n");
+  outfile->Printf("\t**************************\n\n");
+  outfile->Printf("\tWave function  = %s\n", options.get_str("WFN").c_str());
+  outfile->Printf("\tMaxiter        = %d\n", options.get_int("MAXITER"));
+  outfile->Printf("\tConvergence    = %3.1e\n", options.get_double("R_CONVERGENCE"));
+  outfile->Printf("\tDIIS           = %s\n", options.get_bool("DIIS") ? "Yes" : "No");
+  outfile->Printf("\tOut-of-core    = %s\n", options.get_bool("OOC") ? "Yes" : "No");
+  outfile->Printf("\tDertype        = %s\n", options.get_str("DERTYPE").c_str());
+  outfile->Printf("\tPrecision      = %s\n", options.get_str("UGX_PRECISION").c_str());
+
+  // the reference's error trapping (closed-shell RHF in place)
+  if (!ref) throw PSIEXCEPTION("SCF has not been run yet!");
+  if (options.get_str("REFERENCE") != "RHF") throw PSIEXCEPTION("Only for use with RHF references.");
+  for (int h = 0; h < ref->nirrep(); h++)
+    if (ref->soccpi()[h]) throw PSIEXCEPTION("UGACC is for closed-shell systems only.");
+  if (options.get_str("DERTYPE") != "NONE")
+    throw PSIEXCEPTION("ugacc (MI355X): DERTYPE " + options.get_str("DERTYPE") +
+                       " needs HBAR, Lambda and densities, which this library does not provide");
+
+  // active orbitals (frozen core and frozen virtuals removed), AO basis
+  SharedMatrix Co = ref->Ca_subset("AO", "ACTIVE_OCC");
+  SharedMatrix Cv = ref->Ca_subset("AO", "ACTIVE_VIR");
+  const int no = Co->colspi()[0], nv = Cv->colspi()[0], nact = no + nv;
+
+  // active Fock matrix, occupied first (hamiltonian.cc:44-56)
+  SharedMatrix Fao = ref->Fa_subset("AO");
+  std::vector<SharedMatrix> cs = {Co, Cv};
+  SharedMatrix Ca = linalg::horzcat(cs);
+  SharedMatrix Fmo = linalg::triplet(Ca, Fao, Ca, true, false, false);
+  std::vector<double> fock((size_t)nact * nact);
+  for (int p = 0; p < nact; p++)
+    for (int q = 0; q < nact; q++) fock[(size_t)p * nact + q] = Fmo->get(p, q);
+
+  // the six blocks (include/ugx.h layouts)
+  auto mints = std::make_shared<MintsHelper>(ref->basisset(), options, 0);
+  const std::vector<double> oooo = dirac_block(mints->mo_eri(Co, Co, Co, Co), no, no, no, no);
+  const std::vector<double> ooov = dirac_block(mints->mo_eri(Co, Co, Co, Cv), no, no, no, nv);
+  const std::vector<double> oovv = dirac_block(mints->mo_eri(Co, Cv, Co, Cv), no, no, nv, nv);
+  const std::vector<double> ovov = dirac_block(mints->mo_eri(Co, Co, Cv, Cv), no, nv, no, nv);
+  const std::vector<double> ovvv = dirac_block(mints->mo_eri(Co, Cv, Cv, Cv), no, nv, nv, nv);
+  const std::vector<double> vvvv = dirac_block(mints->mo_eri(Cv, Cv, Cv, Cv), nv, nv, nv, nv);
+
+  Context ctx;
+  int rc = ugx_create(&ctx.c, options.get_int("UGX_DEVICE"));
+  if (rc != UGX_OK) throw PSIEXCEPTION("ugacc (MI355X): no usable MI355X device");
+  rc = ugx_load_hamiltonian(ctx.c, no, nv, fock.data(), oooo.data(), ooov.data(), oovv.data(), ovov.data(),
+                            ovvv.data(), vvvv.data());
+  if (rc != UGX_OK) throw PSIEXCEPTION(std::string("ugacc (MI355X): ") + ugx_last_error(ctx.c));
+
+  ugx_options opt;
+  ugx_default_options(&opt);
+  opt.wfn_ccsd_t = options.get_str("WFN") == "CCSD_T";
+  opt.maxiter = options.get_int("MAXITER");
+  opt.diis = options.get_bool("DIIS");
+  opt.r_convergence = options.get_double("R_CONVERGENCE");
+  opt.ooc = options.get_bool("OOC");
+  opt.print = options.get_int("PRINT") > 1;
+  const std::string prec = options.get_str("UGX_PRECISION");
+  opt.precision = prec == "BF16X3" ? UGX_PREC_BF16X3 : (prec == "FP32" ? UGX_PREC_FP32 : UGX_PREC_FP64);
+  if (opt.precision != UGX_PREC_FP64) opt.polish_mode = UGX_POLISH_REFINE;
+
+  ugx_result res;
+  rc = ugx_ccsd(ctx.c, &opt, &res);
+  if (rc == UGX_NOT_CONVERGED) throw PSIEXCEPTION("Computation has not converged.");
+  if (rc != UGX_OK) throw PSIEXCEPTION(std::string("ugacc (MI355X): ") + ugx_last_error(ctx.c));
+
+  const double eref = ref->energy();
+  outfile->Printf("\n\tMP2 Correlation Energy     = %20.14f\n", res.e_mp2);
+  outfile->Printf("\tCCSD Correlation Energy    = %20.14f\n", res.e_ccsd);
+  outfile->Printf("\tCCSD Total Energy          = %20.14f\n", res.e_ccsd + eref);
+  if (opt.wfn_ccsd_t) {
+    outfile->Printf("\t(T) Correction             = %20.14f\n", res.e_t);
+    outfile->Printf("\tCCSD(T) Correlation Energy = %20.14f\n", res.e_ccsd + res.e_t);
+    outfile->Printf("\tCCSD(T) Total Energy       = %20.14f\n", res.e_ccsd + res.e_t + eref);
+  }
+  outfile->Printf("\t%d iterations (%d before the FP64 polish), CCSD %.3f s, (T) %.3f s on the GPU\n", res.niter,
+                  res.niter_low, res.seconds_ccsd, res.seconds_triples);
+  return ref;
+}
+
+}  // namespace ugacc
+}  // namespace psi
