@@ -79,7 +79,10 @@ enum {
 };
 
 /* Initial amplitudes: MP2 (ccwfn.cc:75-87), the caller's (ugx_set_amplitudes),
- * or the MP2 doubles corrected by the network set with ugx_set_guess_network. */
+ * or the learned guess: the MP2 doubles corrected by the network set with
+ * ugx_set_guess_network, with the first-order singles at those doubles,
+ * t1 = R1(t1 = 0, t2) / D1 (one Jacobi step of build_t1, ccwfn.cc:432-469,
+ * from t1 = 0). */
 enum { UGX_GUESS_MP2 = 0, UGX_GUESS_USER = 1, UGX_GUESS_MLP = 2 };
 
 /* Polish of the mixed forms, after the switch to FP64 accuracy:
@@ -277,9 +280,10 @@ int ugx_save_tile_table(ugx_ctx *ctx, const char *path);
 int ugx_load_tile_table(ugx_ctx *ctx, const char *path);
 
 /* ---------------------------------------------------------------------
- * Learned T2 guess.  Replaces the MP2 doubles of CCWfn::CCWfn
+ * Learned guess.  Replaces the MP2 doubles of CCWfn::CCWfn
  * (ccwfn.cc:81-87) by t2 = t2(MP2) + y_scale (f(x_ijab) + f(x_jiba)) / 2,
- * where f is a network 8 -> 128 -> 128 -> 128 -> 1 with GELU (tanh form)
+ * and the singles f_ia / D_ia (:75-79) by the first-order singles at that
+ * t2 (see UGX_GUESS_MLP), where f is a network 8 -> 128 -> 128 -> 128 -> 1 with GELU (tanh form)
  * after the hidden layers and x are the amplitude's features (computed by
  * ugx_features): asinh(1e3 t_MP2), log(-D), asinh(1e2 <ij|ab>),
  * asinh(1e2 <ij|ba>), f_ii, f_jj, f_aa, f_bb.  The reference has no such

@@ -121,7 +121,16 @@ def test_guess_inside_ccsd_matches_oracle():
     net = _net(seed=9)
     c.set_guess_network(net)
     c.ccsd(maxiter=0, guess=ugx.GUESS_MLP, wfn_ccsd_t=0)
-    _, t2 = c.amplitudes()
+    t1, t2 = c.amplitudes()
+    # the singles are the first-order T1 at the guess doubles: one Jacobi
+    # step of the oracle's build_t1 from t1 = 0
+    from oracle import Oracle
+    o = Oracle(fock, ints, 4)
+    o.set_amps(np.zeros_like(t1), t2)
+    o.iterate()
+    t1o, _ = o.amps()
+    o.close()
+    assert np.max(np.abs(t1 - t1o)) < 1e-13 and np.max(np.abs(t1o)) > 1e-4
     ref = mlp.guess(net.params().astype(np.float64), 4, 12, np.diag(fock), ints[:4, :4, 4:, 4:],
                     net.shift.astype(np.float64), net.scale.astype(np.float64), net.y_scale)
     mp2 = ints[:4, :4, 4:, 4:] / _denom(np.diag(fock), 4, 12)
@@ -189,3 +198,49 @@ def test_trained_network_gain_on_a_held_out_cluster():
     assert r_mp2["converged"] and r_ml["converged"]
     assert abs(r_mp2["e_ccsd_final"] - r_ml["e_ccsd_final"]) < 1e-9
     assert r_ml["niter"] <= r_mp2["niter"]
+
+
+def test_learned_guess_cuts_iterations_on_held_out_h2o8():
+    """The first two held-out (H2O)8 clusters (seeds 5000, 5001; training
+    used (H2O)2-(H2O)4 seeds 1000-1255): the learned guess (network T2 and
+    its first-order T1) starts closer to the converged amplitudes than the
+    MP2 guess in both T1 and T2, reaches the same fixed point, and needs no
+    more iterations in either form (FP64 to 1e-10; BF16x3 with the
+    refinement polish to 1e-7) and fewer in total."""
+    from ugacc_amd.chem import molecules
+    from ugacc_amd.chem.scf import Reference
+    from ugacc_amd.learned import load_network
+    net = load_network()
+    c = ugx.Context(0)
+    c.set_guess_network(net)
+    total = {"mp2": 0, "learned": 0}
+    for seed in (5000, 5001):
+        announce("learned guess on held-out (H2O)8 seed %d" % seed)
+        ref = Reference(molecules.water_cluster(8, seed), "cc-pvdz", log=None)
+        no, nv = ref.no, ref.nv
+        c.load_hamiltonian(no, nv, ref.fock, ref.blocks)
+        out = {}
+        for form, kw in (("fp64", dict(r_convergence=1e-10)),
+                         ("mixed", dict(r_convergence=1e-7, precision=ugx.PREC_BF16X3, polish_threshold=1e-5,
+                                        polish_mode=ugx.POLISH_REFINE))):
+            for g, guess in (("mp2", ugx.GUESS_MP2), ("learned", ugx.GUESS_MLP)):
+                r = c.ccsd(wfn_ccsd_t=0, maxiter=100, guess=guess, **kw)
+                assert r["converged"]
+                out[form, g] = r
+                total[g] += r["niter"]
+                if form == "fp64" and g == "mp2":
+                    t1c, t2c = c.amplitudes()
+        c.ccsd(maxiter=0, wfn_ccsd_t=0, guess=ugx.GUESS_MLP)
+        t1g, t2g = c.amplitudes()
+        mp2 = ref.blocks["oovv"] / _denom(np.diag(ref.fock), no, nv)
+        announce("seed %d: |dt1| %.3e -> %.3e  |dt2| %.3e -> %.3e  fp64 %d -> %d  mixed %d -> %d" % (
+            seed, np.linalg.norm(t1c), np.linalg.norm(t1c - t1g), np.linalg.norm(t2c - mp2),
+            np.linalg.norm(t2c - t2g), out["fp64", "mp2"]["niter"], out["fp64", "learned"]["niter"],
+            out["mixed", "mp2"]["niter"], out["mixed", "learned"]["niter"]))
+        assert np.linalg.norm(t1c - t1g) < 0.3 * np.linalg.norm(t1c)
+        assert np.linalg.norm(t2c - t2g) < 0.6 * np.linalg.norm(t2c - mp2)
+        assert abs(out["fp64", "learned"]["e_ccsd_final"] - out["fp64", "mp2"]["e_ccsd_final"]) < 1e-10
+        for form in ("fp64", "mixed"):
+            assert out[form, "learned"]["niter"] <= out[form, "mp2"]["niter"], form
+    c.close()
+    assert total["learned"] < total["mp2"], total

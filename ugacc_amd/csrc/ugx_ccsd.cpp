@@ -708,6 +708,8 @@ void alloc_work(ugx_ctx *c) {
 // contraction; the streaming kernels of those evaluations write work
 // buffers that every real evaluation rewrites).  Done once per Hamiltonian
 // and options signature.
+void guess_t1_first_order(ugx_ctx *c);
+
 void prepare_solve(ugx_ctx *c, const ugx_options &opt) {
   const int prec = opt.precision;
   const bool refine = prec != PREC_FP64 && opt.polish_mode == UGX_POLISH_REFINE;
@@ -721,6 +723,7 @@ void prepare_solve(ugx_ctx *c, const ugx_options &opt) {
   Engine &E = *c->eng;
   E.set_sizing(true);
   try {
+    if (opt.guess == UGX_GUESS_MLP) guess_t1_first_order(c);
     residual(c, prec, RES_PLAIN);
     if (prec != PREC_FP64) {
       if (refine) {
@@ -744,8 +747,31 @@ void guess_mp2(ugx_ctx *c) {
   launch_guess(c->no, c->nv, c->fock.d(), c->oovv.d(), c->t1.d(), c->t2.d(), c->st);
 }
 
+// First-order T1 at the current t2: one Jacobi step of the T1 equation
+// from t1 = 0, t1 = R1(t1 = 0, t2) / D1.  With t1 = 0 every t1 term of
+// build_t1 (ccwfn.cc:432-469) vanishes and Fme = f_me, which leaves
+//   R1_ia = f_ai + sum_me U_imae f_me + sum_mef U_imfe <ma|ef>
+//           - sum_mne t2_mnae L<mn|ie>
+// (U = 2 t2 - t2 with a and b exchanged): three FP64 contractions,
+// 2 o^2 v^3 flops at most.  The learned guess pairs its T2 with this T1.
+void guess_t1_first_order(ugx_ctx *c) {
+  Engine &E = *c->eng;
+  const int o = c->no, v = c->nv;
+  const int P64 = PREC_FP64;
+  const bool dry = E.sizing();
+  if (!dry) launch_build_U(o, v, c->t2.d(), c->U.d(), c->st);
+  const auto UU = vref(c->U.d(), {o, o, v, v}), T2 = vref(c->t2.d(), {o, o, v, v});
+  const auto RR1 = vref(c->R1.d(), {o, v});
+  E.permute("ai->ia", 1.0, cref(c->fvo.d(), {v, o}), 0.0, RR1);
+  E.contract("imae,me->ia", 1.0, UU, cref(c->fov.d(), {o, v}), 1.0, RR1, P64, "guess_t1");
+  E.contract("imfe,maef->ia", 1.0, UU, cref(c->ovvv.d(), {o, v, v, v}), 1.0, RR1, P64, "guess_t1");
+  E.contract("mnae,mnie->ia", -1.0, T2, cref(c->Looov.d(), {o, o, o, v}), 1.0, RR1, P64, "guess_t1");
+  if (!dry) launch_t1_divide(o, v, c->eps.d(), c->R1.d(), c->t1.d(), c->st);
+}
+
 // MP2 guess, then t2 += y_scale (f(x_ijab) + f(x_jiba)) / 2 with the network
-// evaluated by the fused BF16 kernel on features formed from oovv and eps
+// evaluated by the fused BF16 kernel on features formed from oovv and eps,
+// and t1 the first-order T1 at that t2 (guess_t1_first_order)
 void guess_mlp(ugx_ctx *c) {
   MlpDevice P;
   double ys = 0.0;
@@ -757,6 +783,7 @@ void guess_mlp(ugx_ctx *c) {
     launch_mlp_infer_ham(c->no, c->nv, c->eps.d(), c->oovv.d(), P, c->G.f(), c->st);
   });
   launch_apply_guess(c->no, c->nv, c->G.f(), ys, c->t2.d(), c->st);
+  guess_t1_first_order(c);
 }
 
 // ------------------------------------------------------------------ (T)

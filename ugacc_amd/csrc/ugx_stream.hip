@@ -210,6 +210,16 @@ __global__ void k_interleave(long n, const double *__restrict__ x, const double 
   }
 }
 
+// t1_ia = R1_ia / (f_ii - f_aa)
+__global__ void k_t1_divide(int no, int nv, const double *__restrict__ eps, const double *__restrict__ R1,
+                            double *__restrict__ t1) {
+  const long n = (long)no * nv;
+  for (long q = blockIdx.x * (long)NT + threadIdx.x; q < n; q += (long)gridDim.x * NT) {
+    const int i = (int)(q / nv), a = (int)(q % nv);
+    t1[q] = R1[q] / (eps[i] - eps[no + a]);
+  }
+}
+
 // flag[0] |= 1 when any of the n values is NaN or infinite
 __global__ void k_nonfinite(long n, const double *__restrict__ x, double *flag) {
   bool bad = false;
@@ -656,6 +666,12 @@ void launch_interleave(long n, const double *x, const double *y, void *out, int 
   if (n <= 0) return;
   if (f32) hipLaunchKernelGGL(k_interleave<float2>, dim3(nblocks(n)), dim3(NT), 0, st, n, x, y, (float2 *)out);
   else hipLaunchKernelGGL(k_interleave<double2>, dim3(nblocks(n)), dim3(NT), 0, st, n, x, y, (double2 *)out);
+}
+
+void launch_t1_divide(int no, int nv, const double *eps, const double *R1, double *t1, hipStream_t st) {
+  const long n = (long)no * nv;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_t1_divide, dim3(nblocks(n)), dim3(NT), 0, st, no, nv, eps, R1, t1);
 }
 
 void launch_nonfinite(long n, const double *x, double *flag, hipStream_t st) {

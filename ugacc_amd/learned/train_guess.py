@@ -146,7 +146,8 @@ def evaluate(args):
         row = dict(seed=seed, no=no, nv=nv, front_end_s=tf)
         for fname, prec, conv in forms:
             for gname, guess in (("mp2", ugx.GUESS_MP2), ("learned", ugx.GUESS_MLP)):
-                r = ctx.ccsd(r_convergence=conv, wfn_ccsd_t=0, maxiter=100, guess=guess, precision=prec)
+                r = ctx.ccsd(r_convergence=conv, wfn_ccsd_t=0, maxiter=100, guess=guess, precision=prec,
+                             polish_mode=ugx.POLISH_REFINE if prec != ugx.PREC_FP64 else ugx.POLISH_PLAIN)
                 if not r["converged"]:
                     raise RuntimeError("held-out cluster %d did not converge" % seed)
                 key = "%s_%s" % (fname, gname)
@@ -156,7 +157,7 @@ def evaluate(args):
                 if fname == "fp64" and gname == "mp2":
                     t1c, t2c = ctx.amplitudes()
         ctx.ccsd(maxiter=0, wfn_ccsd_t=0, guess=ugx.GUESS_MLP)
-        _, t2g = ctx.amplitudes()
+        t1g, t2g = ctx.amplitudes()
         mp2 = ref.blocks["oovv"] / denominators(np.diag(ref.fock), no, nv)
         if args.ceiling:
             # what a perfect guess could save: the converged T2 with T1 = 0,
@@ -167,18 +168,22 @@ def evaluate(args):
                 row["iter_fp64_" + key] = r["niter"]
         row["t2_err_mp2"] = float(np.linalg.norm(t2c - mp2))
         row["t2_err_learned"] = float(np.linalg.norm(t2c - t2g))
+        row["t1_err_mp2"] = float(np.linalg.norm(t1c))  # canonical RHF: the MP2 guess has t1 = 0
+        row["t1_err_learned"] = float(np.linalg.norm(t1c - t1g))
         row["dE_fp64"] = abs(row["e_fp64_mp2"] - row["e_fp64_learned"])
         row["dE_mixed"] = max(abs(row["e_mixed_mp2"] - row["e_fp64_mp2"]),
                               abs(row["e_mixed_learned"] - row["e_fp64_mp2"]))
         rows.append(row)
-        log("seed %d: |dt2| %.3e -> %.3e; fp64 iterations %d -> %d (%.2f -> %.2f s); mixed %d -> %d "
-            "(%.2f -> %.2f s); |dE| %.1e / %.1e" % (
-                seed, row["t2_err_mp2"], row["t2_err_learned"], row["iter_fp64_mp2"], row["iter_fp64_learned"],
+        log("seed %d: |dt1| %.3e -> %.3e, |dt2| %.3e -> %.3e; fp64 iterations %d -> %d (%.2f -> %.2f s); mixed "
+            "%d -> %d (%.2f -> %.2f s); |dE| %.1e / %.1e" % (
+                seed, row["t1_err_mp2"], row["t1_err_learned"], row["t2_err_mp2"], row["t2_err_learned"],
+                row["iter_fp64_mp2"], row["iter_fp64_learned"],
                 row["s_fp64_mp2"], row["s_fp64_learned"], row["iter_mixed_mp2"], row["iter_mixed_learned"],
                 row["s_mixed_mp2"], row["s_mixed_learned"], row["dE_fp64"], row["dE_mixed"]))
     summary = dict(clusters=len(rows), conv=args.conv, mixed_conv=args.mixed_conv)
     for k in ("iter_fp64_mp2", "iter_fp64_learned", "iter_mixed_mp2", "iter_mixed_learned", "s_fp64_mp2",
-              "s_fp64_learned", "s_mixed_mp2", "s_mixed_learned", "t2_err_mp2", "t2_err_learned",
+              "s_fp64_learned", "s_mixed_mp2", "s_mixed_learned", "t2_err_mp2", "t2_err_learned", "t1_err_mp2",
+              "t1_err_learned",
               "iter_fp64_exact_t2", "iter_fp64_exact_t1"):
         if k in rows[0]:
             summary["mean_" + k] = float(np.mean([r[k] for r in rows]))
