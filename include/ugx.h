@@ -208,6 +208,17 @@ int ugx_residual(ugx_ctx *ctx, const double *t1, const double *t2, int precision
 int ugx_intermediates(ugx_ctx *ctx, double *Fvv, double *Foo, double *Fov, double *Woooo, double *Wovov,
                       double *Wovvo, double *Wovoo);
 
+/* HBAR::HBAR (hbar.cc:8-228): the one- and two-body blocks of the
+ * similarity-transformed Hamiltonian built on the device in FP64 from the
+ * context's current amplitudes (after ugx_ccsd, or ugx_set_amplitudes) and
+ * kept there; the blocks asked for (non-NULL) are copied out in the
+ * reference's layouts: Hov[m][e], Hoo[m][i], Hvv[a][e], Hoooo[m][n][i][j],
+ * Hvvvv[a][b][e][f], Hvovv[a][m][e][f], Hooov[m][n][i][e], Hovvo[m][b][e][j],
+ * Hovov[m][b][j][e], Hvvvo[a][b][e][i], Hovoo[m][b][i][j].  Hoooo and Hvvvv
+ * carry the reference's doubled single-t1 term (hbar.cc:80, :93). */
+int ugx_hbar(ugx_ctx *ctx, double *Hov, double *Hoo, double *Hvv, double *Hoooo, double *Hvvvv, double *Hvovv,
+             double *Hooov, double *Hovvo, double *Hovov, double *Hvvvo, double *Hovoo);
+
 /* Device time per contraction label of the last ugx_ccsd / ugx_triples
  * call when profiling is on, as "label launches ms gflop\n" lines.  Returns
  * the number of bytes needed (including the terminating zero). */

@@ -59,6 +59,7 @@ def _lib(omp=False):
         L.orc_tjl_fov_list.argtypes = [vp, ctypes.c_long, ctypes.POINTER(ctypes.c_long), dp]
         L.orc_w3.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.orc_iteration_sample.argtypes = [vp, ctypes.c_int]
+        L.orc_hbar.argtypes = [vp] + [dp] * 11
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_residual_rows.argtypes = [vp, ctypes.c_int, dp, dp]
@@ -204,6 +205,23 @@ class Oracle:
         W = np.zeros((self.nv,) * 3)
         self.L.orc_w3(self.h, i, j, k, _p(W))
         return W
+
+    HBAR_BLOCKS = ("Hov", "Hoo", "Hvv", "Hoooo", "Hvvvv", "Hvovv", "Hooov", "Hovvo", "Hovov", "Hvvvo", "Hovoo")
+
+    @staticmethod
+    def hbar_shapes(no, nv):
+        o, v = no, nv
+        return dict(Hov=(o, v), Hoo=(o, o), Hvv=(v, v), Hoooo=(o, o, o, o), Hvvvv=(v, v, v, v), Hvovv=(v, o, v, v),
+                    Hooov=(o, o, o, v), Hovvo=(o, v, v, o), Hovov=(o, v, o, v), Hvvvo=(v, v, v, o),
+                    Hovoo=(o, v, o, o))
+
+    def hbar(self):
+        """HBAR::HBAR (hbar.cc:8-228) from the current amplitudes: the eleven
+        blocks by name, in the reference's layouts."""
+        sh = self.hbar_shapes(self.no, self.nv)
+        out = {k: np.zeros(sh[k]) for k in self.HBAR_BLOCKS}
+        self.L.orc_hbar(self.h, *[_p(out[k]) for k in self.HBAR_BLOCKS])
+        return out
 
     def residual_rows(self, rows):
         """(R1, R2[:rows]) of the current amplitudes."""

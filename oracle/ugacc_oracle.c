@@ -945,3 +945,189 @@ int orc_num_threads(void) {
   return 1;
 #endif
 }
+
+/* HBAR::HBAR, hbar.cc:8-228: the similarity-transformed Hamiltonian's
+ * one- and two-body blocks from the current amplitudes (tau rebuilt from
+ * them first), in the reference's layouts:
+ *   Hov[m][e] Hoo[m][i] Hvv[a][e] Hoooo[m][n][i][j] Hvvvv[a][b][e][f]
+ *   Hvovv[a][m][e][f] Hooov[m][n][i][e] Hovvo[m][b][e][j] Hovov[m][b][j][e]
+ *   Hvvvo[a][b][e][i] Hovoo[m][b][i][j]
+ * Loop by loop the reference's expressions (hbar.cc:38-223). */
+#define O4(a, b, c, d, n2, n3, n4) ((((size_t)(a) * (n2) + (b)) * (n3) + (c)) * (n4) + (d))
+void orc_hbar(void *p, double *Hov, double *Hoo, double *Hvv, double *Hoooo, double *Hvvvv, double *Hvovv,
+              double *Hooov, double *Hovvo, double *Hovov, double *Hvvvo, double *Hovoo) {
+  orc_state *s = (orc_state *)p;
+  const int no = s->no, nv = s->nv;
+  const double *t1 = s->t1, *t2 = s->t2, *tau;
+  orc_build_tau(s);
+  tau = s->tau;
+#define T1(i, a) t1[O2(i, a)]
+#define T2(i, j, a, b) t2[T4(i, j, a, b)]
+#define TAU(i, j, a, b) tau[T4(i, j, a, b)]
+  /* hbar.cc:38-45 */
+  for (int m = 0; m < no; m++)
+    for (int e = 0; e < nv; e++) {
+      double v = F(m, e + no);
+      for (int n = 0; n < no; n++)
+        for (int f = 0; f < nv; f++) v += T1(n, f) * LL(m, n, e + no, f + no);
+      Hov[O2(m, e)] = v;
+    }
+  /* :47-59 */
+  for (int m = 0; m < no; m++)
+    for (int i = 0; i < no; i++) {
+      double v = F(m, i);
+      for (int e = 0; e < nv; e++) {
+        v += T1(i, e) * F(m, e + no);
+        for (int n = 0; n < no; n++) {
+          v += T1(n, e) * LL(m, n, i, e + no);
+          for (int f = 0; f < nv; f++) v += TAU(i, n, e, f) * LL(m, n, e + no, f + no);
+        }
+      }
+      Hoo[(size_t)m * no + i] = v;
+    }
+  /* :61-73 */
+  for (int a = 0; a < nv; a++)
+    for (int e = 0; e < nv; e++) {
+      double v = F(a + no, e + no);
+      for (int m = 0; m < no; m++) {
+        v -= F(m, e + no) * T1(m, a);
+        for (int f = 0; f < nv; f++) {
+          v += T1(m, f) * LL(a + no, m, e + no, f + no);
+          for (int n = 0; n < no; n++) v -= TAU(m, n, f, a) * LL(m, n, f + no, e + no);
+        }
+      }
+      Hvv[(size_t)a * nv + e] = v;
+    }
+  /* :75-86 */
+  for (int m = 0; m < no; m++)
+    for (int n = 0; n < no; n++)
+      for (int i = 0; i < no; i++)
+        for (int j = 0; j < no; j++) {
+          double v = INT(m, n, i, j);
+          for (int e = 0; e < nv; e++) {
+            v += 2.0 * T1(j, e) * INT(m, n, i, e + no);
+            for (int f = 0; f < nv; f++) v += TAU(i, j, e, f) * INT(m, n, e + no, f + no);
+          }
+          Hoooo[O4(m, n, i, j, no, no, no)] = v;
+        }
+  /* :88-99 */
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++)
+      for (int e = 0; e < nv; e++)
+        for (int f = 0; f < nv; f++) {
+          double v = INT(a + no, b + no, e + no, f + no);
+          for (int m = 0; m < no; m++) {
+            v -= 2.0 * T1(m, b) * INT(a + no, m, e + no, f + no);
+            for (int n = 0; n < no; n++) v += TAU(m, n, a, b) * INT(m, n, e + no, f + no);
+          }
+          Hvvvv[O4(a, b, e, f, nv, nv, nv)] = v;
+        }
+  /* :101-108 */
+  for (int a = 0; a < nv; a++)
+    for (int m = 0; m < no; m++)
+      for (int e = 0; e < nv; e++)
+        for (int f = 0; f < nv; f++) {
+          double v = INT(a + no, m, e + no, f + no);
+          for (int n = 0; n < no; n++) v -= T1(n, a) * INT(n, m, e + no, f + no);
+          Hvovv[O4(a, m, e, f, no, nv, nv)] = v;
+        }
+  /* :110-117 */
+  for (int m = 0; m < no; m++)
+    for (int n = 0; n < no; n++)
+      for (int i = 0; i < no; i++)
+        for (int e = 0; e < nv; e++) {
+          double v = INT(m, n, i, e + no);
+          for (int f = 0; f < nv; f++) v += T1(i, f) * INT(n, m, e + no, f + no);
+          Hooov[O4(m, n, i, e, no, no, nv)] = v;
+        }
+  /* :119-134 */
+  for (int m = 0; m < no; m++)
+    for (int b = 0; b < nv; b++)
+      for (int e = 0; e < nv; e++)
+        for (int j = 0; j < no; j++) {
+          double v = INT(m, b + no, e + no, j);
+          for (int f = 0; f < nv; f++) v += T1(j, f) * INT(m, b + no, e + no, f + no);
+          for (int n = 0; n < no; n++) v -= T1(n, b) * INT(m, n, e + no, j);
+          for (int n = 0; n < no; n++)
+            for (int f = 0; f < nv; f++) v -= INT(m, n, e + no, f + no) * TAU(n, j, b, f);
+          for (int n = 0; n < no; n++)
+            for (int f = 0; f < nv; f++) v += T2(n, j, f, b) * LL(m, n, e + no, f + no);
+          Hovvo[O4(m, b, e, j, nv, nv, no)] = v;
+        }
+  /* :136-149 */
+  for (int m = 0; m < no; m++)
+    for (int b = 0; b < nv; b++)
+      for (int j = 0; j < no; j++)
+        for (int e = 0; e < nv; e++) {
+          double v = INT(m, b + no, j, e + no);
+          for (int f = 0; f < nv; f++) v += T1(j, f) * INT(b + no, m, e + no, f + no);
+          for (int n = 0; n < no; n++) v -= T1(n, b) * INT(m, n, j, e + no);
+          for (int n = 0; n < no; n++)
+            for (int f = 0; f < nv; f++) v -= INT(n, m, e + no, f + no) * TAU(j, n, f, b);
+          Hovov[O4(m, b, j, e, nv, no, nv)] = v;
+        }
+  /* :151-187 */
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++)
+      for (int e = 0; e < nv; e++)
+        for (int i = 0; i < no; i++) {
+          double v = INT(a + no, b + no, e + no, i);
+          for (int f = 0; f < nv; f++) v += T1(i, f) * INT(a + no, b + no, e + no, f + no);
+          for (int m = 0; m < no; m++) v -= T1(m, b) * INT(a + no, m, e + no, i) + T1(m, a) * INT(b + no, m, i, e + no);
+          for (int m = 0; m < no; m++)
+            for (int f = 0; f < nv; f++)
+              v -= TAU(i, m, f, a) * INT(m, b + no, e + no, f + no) + TAU(i, m, f, b) * INT(a + no, m, e + no, f + no);
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++) v += TAU(m, n, a, b) * INT(m, n, e + no, i);
+          for (int m = 0; m < no; m++) v -= F(m, e + no) * T2(m, i, a, b);
+          for (int m = 0; m < no; m++)
+            for (int f = 0; f < nv; f++) v += T2(m, i, f, b) * LL(a + no, m, e + no, f + no);
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++)
+              for (int f = 0; f < nv; f++)
+                v += INT(m, n, e + no, f + no) *
+                     (T1(i, f) * T2(m, n, a, b) + T1(m, a) * T2(i, n, f, b) + T1(n, b) * T2(m, i, a, f));
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++)
+              for (int f = 0; f < nv; f++)
+                v -= (T1(m, f) * T2(n, i, a, b) + T1(n, a) * T2(m, i, f, b)) * LL(m, n, f + no, e + no);
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++)
+              for (int f = 0; f < nv; f++) v += T1(i, f) * T1(m, a) * T1(n, b) * INT(m, n, e + no, f + no);
+          Hvvvo[O4(a, b, e, i, nv, nv, no)] = v;
+        }
+  /* :189-223 */
+  for (int m = 0; m < no; m++)
+    for (int b = 0; b < nv; b++)
+      for (int i = 0; i < no; i++)
+        for (int j = 0; j < no; j++) {
+          double v = INT(m, b + no, i, j);
+          for (int e = 0; e < nv; e++) v += T1(j, e) * INT(m, b + no, i, e + no) + T1(i, e) * INT(b + no, m, j, e + no);
+          for (int n = 0; n < no; n++) v -= T1(n, b) * INT(m, n, i, j);
+          for (int n = 0; n < no; n++)
+            for (int e = 0; e < nv; e++)
+              v -= TAU(i, n, e, b) * INT(n, m, j, e + no) + TAU(j, n, e, b) * INT(m, n, i, e + no);
+          for (int e = 0; e < nv; e++)
+            for (int f = 0; f < nv; f++) v += INT(m, b + no, e + no, f + no) * TAU(i, j, e, f);
+          for (int e = 0; e < nv; e++) v += F(m, e + no) * T2(i, j, e, b);
+          for (int n = 0; n < no; n++)
+            for (int e = 0; e < nv; e++) v += LL(m, n, i, e + no) * T2(n, j, e, b);
+          for (int e = 0; e < nv; e++)
+            for (int n = 0; n < no; n++)
+              for (int f = 0; f < nv; f++)
+                v -= INT(m, n, e + no, f + no) *
+                     (T1(j, f) * T2(i, n, e, b) + T1(i, e) * T2(j, n, f, b) + T1(n, b) * T2(i, j, e, f));
+          for (int e = 0; e < nv; e++)
+            for (int n = 0; n < no; n++)
+              for (int f = 0; f < nv; f++)
+                v += (T1(i, e) * T2(n, j, f, b) + T1(n, f) * T2(i, j, e, b)) * LL(m, n, e + no, f + no);
+          for (int e = 0; e < nv; e++)
+            for (int n = 0; n < no; n++)
+              for (int f = 0; f < nv; f++) v -= T1(j, f) * T1(i, e) * T1(n, b) * INT(m, n, e + no, f + no);
+          Hovoo[O4(m, b, i, j, nv, no, no)] = v;
+        }
+#undef T1
+#undef T2
+#undef TAU
+}
+#undef O4

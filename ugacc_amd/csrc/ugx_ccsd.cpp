@@ -126,6 +126,9 @@ struct ugx_ctx {
   // options signature the context's workspaces, cached conversions and
   // buffers were prepared for (prepare_solve); empty after a load
   std::string prepared_for;
+  // HBAR blocks (hbar.cc:8-228), built by ugx_hbar, and its intermediates
+  DBuf hb_ov, hb_oo, hb_vv, hb_oooo, hb_vvvv, hb_vovv, hb_ooov, hb_ovvo, hb_ovov, hb_vvvo, hb_ovoo;
+  DBuf hb_B, hb_Bp, hb_Lt, hb_C, hb_D, hb_X, hb_E, hb_G, hb_t1L, hb_Woo, hb_Q;
   // residual of the last ugx_residual, for ugx_intermediates
   bool have_intermediates = false;
   double last_flops = 0.0;
@@ -784,6 +787,150 @@ void guess_mlp(ugx_ctx *c) {
   });
   launch_apply_guess(c->no, c->nv, c->G.f(), ys, c->t2.d(), c->st);
   guess_t1_first_order(c);
+}
+
+// ----------------------------------------------------------------- HBAR
+// HBAR::HBAR (hbar.cc:8-228) from the current amplitudes, in FP64, into the
+// context's hb_* blocks (the reference's layouts).  Every term of the
+// reference is one engine contraction on the Dirac blocks and L; a term with
+// two amplitudes and one integral (t1 t2 <mn|ef>, t1 t1 t1 <mn|ef>) goes
+// through one intermediate of at most o^2 v^2 elements:
+//   B[m][n][e][i]  = sum_f <mn|ef> t1_if      Bp[m][n][i][f] = sum_e <mn|ef> t1_ie
+//   Lt[m][n][i][f] = sum_e L<mn|ef> t1_ie     G[n][e] = sum_mf t1_mf L<mn|fe>
+//   C[m][e][b][i]  = sum_nf <mn|ef> t2_in^fb  D[n][e][i][a] = sum_mf <mn|ef> t2_mi^af
+//   X[n][e][i][b]  = sum_mf t2_mi^fb L<mn|fe> E[m][b][e][i] = sum_n t1_nb B[m][n][e][i]
+//   t1L[m][e] = sum_nf t1_nf L<mn|ef>        Woo[m][n][i][j] = sum_ef <mn|ef> t2_ij^ef
+//   Q[m][n][i][j] = sum_f Bp[m][n][i][f] t1_jf
+// Hoooo and Hvvvv keep the reference's doubled single-t1 term (hbar.cc:80,
+// :93).
+void hbar(ugx_ctx *c) {
+  Engine &E = *c->eng;
+  const int o = c->no, v = c->nv;
+  const hipStream_t st = c->st;
+  const int P = PREC_FP64;
+  const long oo = (long)o * o, ov = (long)o * v, vv = (long)v * v;
+  for (auto q : {std::make_pair(&c->hb_ov, ov), std::make_pair(&c->hb_oo, oo), std::make_pair(&c->hb_vv, vv),
+                 std::make_pair(&c->hb_oooo, oo * oo), std::make_pair(&c->hb_vvvv, vv * vv),
+                 std::make_pair(&c->hb_vovv, ov * vv), std::make_pair(&c->hb_ooov, oo * ov),
+                 std::make_pair(&c->hb_ovvo, ov * ov), std::make_pair(&c->hb_ovov, ov * ov),
+                 std::make_pair(&c->hb_vvvo, vv * ov), std::make_pair(&c->hb_ovoo, ov * oo),
+                 std::make_pair(&c->hb_B, oo * ov), std::make_pair(&c->hb_Bp, oo * ov),
+                 std::make_pair(&c->hb_Lt, oo * ov), std::make_pair(&c->hb_C, ov * ov),
+                 std::make_pair(&c->hb_D, ov * ov), std::make_pair(&c->hb_X, ov * ov),
+                 std::make_pair(&c->hb_E, ov * ov), std::make_pair(&c->hb_G, ov), std::make_pair(&c->hb_t1L, ov),
+                 std::make_pair(&c->hb_Woo, oo * oo), std::make_pair(&c->hb_Q, oo * oo)})
+    q.first->alloc((size_t)q.second * 8);
+  launch_tau(o, v, c->t1.d(), c->t2.d(), c->tau.d(), c->ttau.d(), st);
+  auto V = [](const DBuf &b, std::vector<int> d) { return vref(b.d(), std::move(d)); };
+  auto Cn = [](const DBuf &b, std::vector<int> d) { return cref(b.d(), std::move(d)); };
+  const auto T1 = V(c->t1, {o, v}), T2 = V(c->t2, {o, o, v, v}), TAU = V(c->tau, {o, o, v, v});
+  const auto FOV = Cn(c->fov, {o, v}), FOO = Cn(c->foo, {o, o}), FVV = Cn(c->fvv, {v, v});
+  const auto OOOO = Cn(c->oooo, {o, o, o, o}), OOOV = Cn(c->ooov, {o, o, o, v}), OOVV = Cn(c->oovv, {o, o, v, v});
+  const auto OVOV = Cn(c->ovov, {o, v, o, v}), OVVV = Cn(c->ovvv, {o, v, v, v}), VVVV = Cn(c->vvvv, {v, v, v, v});
+  const auto LOOVV = Cn(c->Loovv, {o, o, v, v}), LOOOV = Cn(c->Looov, {o, o, o, v});
+  const auto LOVVV = Cn(c->Lovvv, {o, v, v, v});
+  auto K = [&](const char *spec, double a, const TRef &A, const TRef &B, double b, const TRef &C) {
+    E.contract(spec, a, A, B, b, C, P, "hbar");
+  };
+  // hbar.cc:38-45, Hov[m][e]
+  const auto HOV = V(c->hb_ov, {o, v});
+  E.permute("me->me", 1.0, FOV, 0.0, HOV);
+  K("nf,mnef->me", 1.0, T1, LOOVV, 1.0, HOV);
+  // :47-59, Hoo[m][i]; L<mn|ie> = Looov[m][n][i][e]
+  const auto HOO = V(c->hb_oo, {o, o});
+  E.permute("mi->mi", 1.0, FOO, 0.0, HOO);
+  K("ie,me->mi", 1.0, T1, FOV, 1.0, HOO);
+  K("ne,mnie->mi", 1.0, T1, LOOOV, 1.0, HOO);
+  K("inef,mnef->mi", 1.0, TAU, LOOVV, 1.0, HOO);
+  // :61-73, Hvv[a][e]; L<am|ef> = Lovvv[m][a][f][e]
+  const auto HVV = V(c->hb_vv, {v, v});
+  E.permute("ae->ae", 1.0, FVV, 0.0, HVV);
+  K("ma,me->ae", -1.0, T1, FOV, 1.0, HVV);
+  K("mf,mafe->ae", 1.0, T1, LOVVV, 1.0, HVV);
+  K("mnfa,mnfe->ae", -1.0, TAU, LOOVV, 1.0, HVV);
+  // :75-86, Hoooo[m][n][i][j]
+  const auto HOOOO = V(c->hb_oooo, {o, o, o, o});
+  E.permute("mnij->mnij", 1.0, OOOO, 0.0, HOOOO);
+  K("je,mnie->mnij", 2.0, T1, OOOV, 1.0, HOOOO);
+  K("ijef,mnef->mnij", 1.0, TAU, OOVV, 1.0, HOOOO);
+  // :88-99, Hvvvv[a][b][e][f]; <am|ef> = ovvv[m][a][f][e]
+  const auto HVVVV = V(c->hb_vvvv, {v, v, v, v});
+  E.permute("abef->abef", 1.0, VVVV, 0.0, HVVVV);
+  K("mb,mafe->abef", -2.0, T1, OVVV, 1.0, HVVVV);
+  K("mnab,mnef->abef", 1.0, TAU, OOVV, 1.0, HVVVV);
+  // :101-108, Hvovv[a][m][e][f]
+  const auto HVOVV = V(c->hb_vovv, {v, o, v, v});
+  E.permute("mafe->amef", 1.0, OVVV, 0.0, HVOVV);
+  K("na,nmef->amef", -1.0, T1, OOVV, 1.0, HVOVV);
+  // :110-117, Hooov[m][n][i][e]
+  const auto HOOOV = V(c->hb_ooov, {o, o, o, v});
+  E.permute("mnie->mnie", 1.0, OOOV, 0.0, HOOOV);
+  K("if,nmef->mnie", 1.0, T1, OOVV, 1.0, HOOOV);
+  // :119-134, Hovvo[m][b][e][j]; <mb|ej> = oovv[m][j][e][b], <mn|ej> = ooov[n][m][j][e]
+  const auto HOVVO = V(c->hb_ovvo, {o, v, v, o});
+  E.permute("mjeb->mbej", 1.0, OOVV, 0.0, HOVVO);
+  K("jf,mbef->mbej", 1.0, T1, OVVV, 1.0, HOVVO);
+  K("nb,nmje->mbej", -1.0, T1, OOOV, 1.0, HOVVO);
+  K("mnef,njbf->mbej", -1.0, OOVV, TAU, 1.0, HOVVO);
+  K("njfb,mnef->mbej", 1.0, T2, LOOVV, 1.0, HOVVO);
+  // :136-149, Hovov[m][b][j][e]; <bm|ef> = ovvv[m][b][f][e]
+  const auto HOVOV = V(c->hb_ovov, {o, v, o, v});
+  E.permute("mbje->mbje", 1.0, OVOV, 0.0, HOVOV);
+  K("jf,mbfe->mbje", 1.0, T1, OVVV, 1.0, HOVOV);
+  K("nb,mnje->mbje", -1.0, T1, OOOV, 1.0, HOVOV);
+  K("nmef,jnfb->mbje", -1.0, OOVV, TAU, 1.0, HOVOV);
+  // intermediates of the t1 t2 and t1 t1 t1 terms
+  const auto B = V(c->hb_B, {o, o, v, o}), Bp = V(c->hb_Bp, {o, o, o, v}), Lt = V(c->hb_Lt, {o, o, o, v});
+  const auto CC = V(c->hb_C, {o, v, v, o}), DD = V(c->hb_D, {o, v, o, v}), XX = V(c->hb_X, {o, v, o, v});
+  const auto EE = V(c->hb_E, {o, v, v, o}), G = V(c->hb_G, {o, v}), T1L = V(c->hb_t1L, {o, v});
+  const auto WOO = V(c->hb_Woo, {o, o, o, o}), QQ = V(c->hb_Q, {o, o, o, o});
+  K("mnef,if->mnei", 1.0, OOVV, T1, 0.0, B);
+  K("mnef,ie->mnif", 1.0, OOVV, T1, 0.0, Bp);
+  K("mnef,ie->mnif", 1.0, LOOVV, T1, 0.0, Lt);
+  K("mnef,infb->mebi", 1.0, OOVV, T2, 0.0, CC);
+  K("mnef,miaf->neia", 1.0, OOVV, T2, 0.0, DD);
+  K("mf,mnfe->ne", 1.0, T1, LOOVV, 0.0, G);
+  K("mifb,mnfe->neib", 1.0, T2, LOOVV, 0.0, XX);
+  K("nb,mnei->mbei", 1.0, T1, B, 0.0, EE);
+  K("nf,mnef->me", 1.0, T1, LOOVV, 0.0, T1L);
+  K("mnef,ijef->mnij", 1.0, OOVV, T2, 0.0, WOO);
+  K("mnif,jf->mnij", 1.0, Bp, T1, 0.0, QQ);
+  // :151-187, Hvvvo[a][b][e][i]; <ab|ei> = ovvv[i][e][b][a], <am|ei> = ovov[m][a][i][e],
+  // <bm|ie> = oovv[m][i][e][b], <mn|ei> = ooov[n][m][i][e], L<am|ef> = Lovvv[m][a][f][e]
+  const auto HVVVO = V(c->hb_vvvo, {v, v, v, o});
+  E.permute("ieba->abei", 1.0, OVVV, 0.0, HVVVO);
+  K("if,abef->abei", 1.0, T1, VVVV, 1.0, HVVVO);
+  K("mb,maie->abei", -1.0, T1, OVOV, 1.0, HVVVO);
+  K("ma,mieb->abei", -1.0, T1, OOVV, 1.0, HVVVO);
+  K("imfa,mbef->abei", -1.0, TAU, OVVV, 1.0, HVVVO);
+  K("imfb,mafe->abei", -1.0, TAU, OVVV, 1.0, HVVVO);
+  K("mnab,nmie->abei", 1.0, TAU, OOOV, 1.0, HVVVO);
+  K("me,miab->abei", -1.0, FOV, T2, 1.0, HVVVO);
+  K("mifb,mafe->abei", 1.0, T2, LOVVV, 1.0, HVVVO);
+  K("mnab,mnei->abei", 1.0, T2, B, 1.0, HVVVO);    // <mn|ef> t1_if t2_mn^ab
+  K("ma,mebi->abei", 1.0, T1, CC, 1.0, HVVVO);     // <mn|ef> t1_ma t2_in^fb
+  K("nb,neia->abei", 1.0, T1, DD, 1.0, HVVVO);     // <mn|ef> t1_nb t2_mi^af
+  K("niab,ne->abei", -1.0, T2, G, 1.0, HVVVO);     // t1_mf t2_ni^ab L<mn|fe>
+  K("na,neib->abei", -1.0, T1, XX, 1.0, HVVVO);    // t1_na t2_mi^fb L<mn|fe>
+  K("ma,mbei->abei", 1.0, T1, EE, 1.0, HVVVO);     // t1_if t1_ma t1_nb <mn|ef>
+  // :189-223, Hovoo[m][b][i][j]; <mb|ij> = ooov[i][j][m][b], <mb|ie> = ovov[m][b][i][e],
+  // <bm|je> = oovv[m][j][e][b], <nm|je> = ooov[n][m][j][e], L<mn|ie> = Looov[m][n][i][e]
+  const auto HOVOO = V(c->hb_ovoo, {o, v, o, o});
+  E.permute("ijmb->mbij", 1.0, OOOV, 0.0, HOVOO);
+  K("je,mbie->mbij", 1.0, T1, OVOV, 1.0, HOVOO);
+  K("ie,mjeb->mbij", 1.0, T1, OOVV, 1.0, HOVOO);
+  K("nb,mnij->mbij", -1.0, T1, OOOO, 1.0, HOVOO);
+  K("ineb,nmje->mbij", -1.0, TAU, OOOV, 1.0, HOVOO);
+  K("jneb,mnie->mbij", -1.0, TAU, OOOV, 1.0, HOVOO);
+  K("mbef,ijef->mbij", 1.0, OVVV, TAU, 1.0, HOVOO);
+  K("me,ijeb->mbij", 1.0, FOV, T2, 1.0, HOVOO);
+  K("mnie,njeb->mbij", 1.0, LOOOV, T2, 1.0, HOVOO);
+  K("ineb,mnej->mbij", -1.0, T2, B, 1.0, HOVOO);   // <mn|ef> t1_jf t2_in^eb
+  K("jnfb,mnif->mbij", -1.0, T2, Bp, 1.0, HOVOO);  // <mn|ef> t1_ie t2_jn^fb
+  K("nb,mnij->mbij", -1.0, T1, WOO, 1.0, HOVOO);   // <mn|ef> t1_nb t2_ij^ef
+  K("njfb,mnif->mbij", 1.0, T2, Lt, 1.0, HOVOO);   // t1_ie t2_nj^fb L<mn|ef>
+  K("me,ijeb->mbij", 1.0, T1L, T2, 1.0, HOVOO);    // t1_nf t2_ij^eb L<mn|ef>
+  K("nb,mnij->mbij", -1.0, T1, QQ, 1.0, HOVOO);    // t1_jf t1_ie t1_nb <mn|ef>
 }
 
 // ------------------------------------------------------------------ (T)
@@ -1886,6 +2033,31 @@ int ugx_intermediates(ugx_ctx *c, double *Fvv, double *Foo, double *Fov, double 
       c->eng->permute("mejb->mbej", 1.0, vref(c->WB.d(), {o, v, o, v}), 0.0, vref(tmp.d(), {o, v, v, o}));
       get(Wovvo, tmp.p, ovov);
     }
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+int ugx_hbar(ugx_ctx *c, double *Hov, double *Hoo, double *Hvv, double *Hoooo, double *Hvvvv, double *Hvovv,
+             double *Hooov, double *Hovvo, double *Hovov, double *Hvvvo, double *Hovoo) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    hbar(c);
+    const long o = c->no, v = c->nv;
+    auto get = [&](double *dst, const DBuf &src, long n) {
+      if (dst) check(hipMemcpyAsync(dst, src.p, n * 8, hipMemcpyDeviceToHost, c->st), "HBAR copy");
+    };
+    get(Hov, c->hb_ov, o * v);
+    get(Hoo, c->hb_oo, o * o);
+    get(Hvv, c->hb_vv, v * v);
+    get(Hoooo, c->hb_oooo, o * o * o * o);
+    get(Hvvvv, c->hb_vvvv, v * v * v * v);
+    get(Hvovv, c->hb_vovv, v * o * v * v);
+    get(Hooov, c->hb_ooov, o * o * o * v);
+    get(Hovvo, c->hb_ovvo, o * v * v * o);
+    get(Hovov, c->hb_ovov, o * v * o * v);
+    get(Hvvvo, c->hb_vvvo, v * v * v * o);
+    get(Hovoo, c->hb_ovoo, o * v * o * o);
     check(hipStreamSynchronize(c->st), "stream sync");
     return UGX_OK;
   });

@@ -25,7 +25,7 @@ PREC_NAMES = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 EXPORTS = [
     "ugx_default_options", "ugx_create", "ugx_destroy", "ugx_last_error", "ugx_load_hamiltonian",
     "ugx_set_amplitudes", "ugx_prepare", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
-    "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_set_profiling",
+    "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_hbar", "ugx_set_profiling",
     "ugx_kernel_stats", "ugx_gemm", "ugx_gemm_bf16x3_static", "ugx_synchronize", "ugx_batch_solve", "ugx_set_autotune", "ugx_tile_table",
     "ugx_save_tile_table", "ugx_load_tile_table", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
     "ugx_mlp_last_error", "ugx_mlp_set_params", "ugx_mlp_get_params", "ugx_mlp_set_scaling", "ugx_mlp_forward",
@@ -97,6 +97,7 @@ def load_library(path=LIBPATH):
     L.ugx_energy.argtypes = [vp, ip, dp]
     L.ugx_residual.argtypes = [vp, dp, dp, ip, dp, dp]
     L.ugx_intermediates.argtypes = [vp] + [dp] * 7
+    L.ugx_hbar.argtypes = [vp] + [dp] * 11
     L.ugx_set_profiling.argtypes = [vp, ip]
     L.ugx_set_profiling.restype = None
     L.ugx_kernel_stats.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
@@ -274,6 +275,20 @@ class Context:
                    Wovvo=np.zeros((o, v, v, o)), Wovoo=np.zeros((o, v, o, o)))
         self._check(self.L.ugx_intermediates(self.h, *[_p(out[k]) for k in
                                                        ("Fvv", "Foo", "Fov", "Woooo", "Wovov", "Wovvo", "Wovoo")]))
+        return out
+
+    HBAR_BLOCKS = ("Hov", "Hoo", "Hvv", "Hoooo", "Hvvvv", "Hvovv", "Hooov", "Hovvo", "Hovov", "Hvvvo", "Hovoo")
+
+    def hbar(self, which=None):
+        """ugx_hbar: the HBAR blocks (hbar.cc:8-228) of the current amplitudes,
+        as a dict of the blocks named in `which` (default all)."""
+        o, v = self.no, self.nv
+        shapes = dict(Hov=(o, v), Hoo=(o, o), Hvv=(v, v), Hoooo=(o, o, o, o), Hvvvv=(v, v, v, v),
+                      Hvovv=(v, o, v, v), Hooov=(o, o, o, v), Hovvo=(o, v, v, o), Hovov=(o, v, o, v),
+                      Hvvvo=(v, v, v, o), Hovoo=(o, v, o, o))
+        which = self.HBAR_BLOCKS if which is None else which
+        out = {k: np.zeros(shapes[k]) for k in which}
+        self._check(self.L.ugx_hbar(self.h, *[_p(out[k]) if k in out else None for k in self.HBAR_BLOCKS]))
         return out
 
     def set_profiling(self, on=True):
