@@ -219,6 +219,37 @@ int ugx_intermediates(ugx_ctx *ctx, double *Fvv, double *Foo, double *Fov, doubl
 int ugx_hbar(ugx_ctx *ctx, double *Hov, double *Hoo, double *Hvv, double *Hoooo, double *Hvvvv, double *Hvovv,
              double *Hooov, double *Hovvo, double *Hovov, double *Hvvvo, double *Hovoo);
 
+/* CCLambda::CCLambda + CCLambda::compute_lambda (cclambda.cc:13-46,
+ * :93-135): the CCSD Lambda equations over the HBAR of the context's current
+ * amplitudes (after ugx_ccsd), in FP64 on the device.  Guess l1 = 2 t1,
+ * l2 = 2 (2 t2 - t2^T); each iteration builds G (cclambda.cc:137-156), the
+ * raw l1 residual (:158-210) and l2 residual Z + Z^T (:212-268), updates
+ * l = l_old + R/D with the CCSD denominators and stops when the 2-norm of
+ * the raw residual is below r_convergence (before DIIS, as the reference);
+ * DIIS as in ugx_ccsd.  Options read: maxiter, diis, r_convergence,
+ * diis_nvec, diis_min_vecs, diis_policy, print (NULL: the defaults).
+ * Outputs (each may be NULL): the pseudoenergy 1/2 <ij|ab> l2_ij^ab of the
+ * final l2 (cclambda.cc:270-288), the iteration count, the final residual
+ * norm, and trace (maxiter + 1 values): the pseudoenergy of the guess, after
+ * each iteration (the values the reference prints) and of the final l2 at
+ * index niter.  UGX_NOT_CONVERGED with "Computation has not converged."
+ * as the reference's exception (cclambda.cc:133-134).  The (T) terms that
+ * CCSD_T with DERTYPE=FIRST adds (s1/s2, cclambda.cc:184, :235) are not
+ * included. */
+int ugx_lambda(ugx_ctx *ctx, const ugx_options *opt, double *pseudoenergy, int *niter, double *rms,
+               double *trace);
+
+/* The Lambda amplitudes of the last ugx_lambda, l1[i][a] and l2[i][j][a][b]
+ * (either pointer may be NULL). */
+int ugx_get_lambda(ugx_ctx *ctx, double *l1, double *l2);
+
+/* One evaluation of build_G, build_l1 and build_l2 (cclambda.cc:137-268) at
+ * the given l1/l2 over the HBAR of the current amplitudes: the raw residuals
+ * R1[i][a] and R2[i][j][a][b] (before the denominators) and Goo[m][i],
+ * Gvv[a][e].  Outputs may be NULL. */
+int ugx_lambda_residual(ugx_ctx *ctx, const double *l1, const double *l2, double *R1, double *R2, double *Goo,
+                        double *Gvv);
+
 /* Device time per contraction label of the last ugx_ccsd / ugx_triples
  * call when profiling is on, as "label launches ms gflop\n" lines.  Returns
  * the number of bytes needed (including the terminating zero). */

@@ -63,6 +63,14 @@ def _lib(omp=False):
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_residual_rows.argtypes = [vp, ctypes.c_int, dp, dp]
+        L.orc_lambda_init.argtypes = [vp]
+        L.orc_lambda_residual.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+        L.orc_lambda.restype = ctypes.c_int
+        L.orc_lambda.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.orc_get_lambda.argtypes = [vp, dp, dp]
+        L.orc_set_lambda.argtypes = [vp, dp, dp]
+        L.orc_lambda_pseudoenergy.restype = ctypes.c_double
+        L.orc_lambda_pseudoenergy.argtypes = [vp]
         _libs[omp] = L
     return _libs[omp]
 
@@ -222,6 +230,46 @@ class Oracle:
         out = {k: np.zeros(sh[k]) for k in self.HBAR_BLOCKS}
         self.L.orc_hbar(self.h, *[_p(out[k]) for k in self.HBAR_BLOCKS])
         return out
+
+    def lambda_init(self):
+        """CCLambda::CCLambda (cclambda.cc:13-46): HBAR of the current
+        amplitudes and the guess l1 = 2 t1, l2 = 2 (2 t2 - t2^T)."""
+        self.L.orc_lambda_init(self.h)
+
+    def lambda_residual(self, l1, l2):
+        """build_G, build_l1, build_l2 (cclambda.cc:137-268) at (l1, l2) over
+        the HBAR of lambda_init: (R1, R2, Goo, Gvv), R raw (before /D)."""
+        no, nv = self.no, self.nv
+        l1 = np.ascontiguousarray(l1, dtype=np.float64)
+        l2 = np.ascontiguousarray(l2, dtype=np.float64)
+        R1, R2 = np.zeros((no, nv)), np.zeros((no, no, nv, nv))
+        Goo, Gvv = np.zeros((no, no)), np.zeros((nv, nv))
+        self.L.orc_lambda_residual(self.h, _p(l1), _p(l2), _p(R1), _p(R2), _p(Goo), _p(Gvv))
+        return R1, R2, Goo, Gvv
+
+    def lambda_solve(self, maxiter=100, conv=1e-7, diis=True, diis_policy=0, min_vecs=3):
+        """CCLambda::compute_lambda (cclambda.cc:93-135) from the current
+        guess (lambda_init first)."""
+        out = np.zeros(3)
+        trace = np.zeros(maxiter + 1)
+        rc = self.L.orc_lambda(self.h, maxiter, conv, int(diis), diis_policy, min_vecs, _p(out), _p(trace))
+        n = int(out[2])
+        return dict(converged=rc == 0, pseudoenergy=out[0], rms=out[1], niter=n, trace=trace[:n + 1])
+
+    def lambdas(self):
+        l1 = np.zeros((self.no, self.nv))
+        l2 = np.zeros((self.no, self.no, self.nv, self.nv))
+        self.L.orc_get_lambda(self.h, _p(l1), _p(l2))
+        return l1, l2
+
+    def set_lambdas(self, l1, l2):
+        l1 = np.ascontiguousarray(l1, dtype=np.float64)
+        l2 = np.ascontiguousarray(l2, dtype=np.float64)
+        self.L.orc_set_lambda(self.h, _p(l1), _p(l2))
+
+    def pseudoenergy(self):
+        """cclambda.cc:270-288 of the current l2."""
+        return self.L.orc_lambda_pseudoenergy(self.h)
 
     def residual_rows(self, rows):
         """(R1, R2[:rows]) of the current amplitudes."""

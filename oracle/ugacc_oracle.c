@@ -46,6 +46,10 @@ typedef struct {
    * build_W and build_t1: no / nv except inside orc_iteration_sample */
   int lim_o, lim_v;
   int diis_count, diis_next;
+  /* Lambda (cclambda.cc): HBAR of the amplitudes Lambda was started from,
+   * l1/l2 and their previous values, Goo/Gvv, the half residual Z */
+  double *hb[11];
+  double *l1, *l1old, *l2, *l2old, *Goo, *Gvv, *lZ;
 } orc_state;
 
 #define NA (s->nact)
@@ -131,6 +135,8 @@ void orc_destroy(void *p) {
   free(s->Fvv); free(s->Foo); free(s->Fov);
   free(s->Woooo); free(s->Wovov); free(s->Wovvo); free(s->Wovoo);
   free(s->t1s); free(s->t2s);
+  for (int k = 0; k < 11; k++) free(s->hb[k]);
+  free(s->l1); free(s->l1old); free(s->l2); free(s->l2old); free(s->Goo); free(s->Gvv); free(s->lZ);
   diis_free(s);
   free(s);
 }
@@ -481,7 +487,11 @@ static int lin_solve(double *A, double *b, int n) {
 /* Adds (t - t_old) and t to the subspace; if the subspace holds at least
  * min_vecs vectors, overwrites t1/t2 with the extrapolated amplitudes.
  * Returns the subspace size used (0 when no extrapolation happened). */
-static int diis_step(orc_state *s, int nvec, int policy, int min_vecs) {
+/* One Pulay step over the vector pair (x1 | x2) with their previous values
+ * (x1o | x2o): the CCSD amplitudes (t) or the Lambda amplitudes (l), which
+ * share the subspace storage (never both at once). */
+static int diis_step_x(orc_state *s, double *x1, const double *x1o, double *x2, const double *x2o, int nvec,
+                       int policy, int min_vecs) {
   size_t ov = (size_t)s->no * s->nv, len = ov + ov * ov;
   if (!s->diis_err) {
     s->diis_nvec = nvec;
@@ -500,9 +510,9 @@ static int diis_step(orc_state *s, int nvec, int policy, int min_vecs) {
   }
   double *e = s->diis_err + slot * len, *x = s->diis_amp + slot * len;
   double nrm = 0;
-  for (size_t k = 0; k < ov; k++) { e[k] = s->t1[k] - s->t1old[k]; x[k] = s->t1[k]; nrm += e[k] * e[k]; }
+  for (size_t k = 0; k < ov; k++) { e[k] = x1[k] - x1o[k]; x[k] = x1[k]; nrm += e[k] * e[k]; }
   for (size_t k = 0; k < ov * ov; k++) {
-    e[ov + k] = s->t2[k] - s->t2old[k]; x[ov + k] = s->t2[k]; nrm += e[ov + k] * e[ov + k];
+    e[ov + k] = x2[k] - x2o[k]; x[ov + k] = x2[k]; nrm += e[ov + k] * e[ov + k];
   }
   s->diis_errnorm[slot] = nrm;
   if (s->diis_count < nvec) s->diis_count++;
@@ -530,15 +540,19 @@ static int diis_step(orc_state *s, int nvec, int policy, int min_vecs) {
   for (size_t k = 0; k < ov; k++) {
     double v = 0;
     for (int p = 0; p < n; p++) v += C[p] * s->diis_amp[p * len + k];
-    s->t1[k] = v;
+    x1[k] = v;
   }
   for (size_t k = 0; k < ov * ov; k++) {
     double v = 0;
     for (int p = 0; p < n; p++) v += C[p] * s->diis_amp[p * len + ov + k];
-    s->t2[k] = v;
+    x2[k] = v;
   }
   free(B); free(C);
   return n;
+}
+
+static int diis_step(orc_state *s, int nvec, int policy, int min_vecs) {
+  return diis_step_x(s, s->t1, s->t1old, s->t2, s->t2old, nvec, policy, min_vecs);
 }
 
 /* The CCSD driver.
@@ -1131,3 +1145,230 @@ void orc_hbar(void *p, double *Hov, double *Hoo, double *Hvv, double *Hoooo, dou
 #undef TAU
 }
 #undef O4
+
+/* ---------------------------------------------------------------- Lambda
+ * CCLambda (cclambda.cc:13-289): the Lagrange multipliers of the CCSD energy,
+ * iterated over the HBAR of fixed amplitudes.  The HBAR blocks are built once
+ * (orc_lambda_init, from the amplitudes current at that call: the converged
+ * t1/t2 after orc_ccsd) in the layouts of orc_hbar.  The CCSD_T/DERTYPE=FIRST
+ * terms s1/s2 (cclambda.cc:184, :235) come from the (T) gradient
+ * (ccwfn.cc:854-1316), which is not restated: this is the CCSD Lambda. */
+enum { HOV, HOO, HVV, HOOOO, HVVVV, HVOVV, HOOOV, HOVVO, HOVOV, HVVVO, HOVOO };
+#define Q4(a, b, c, d, n2, n3, n4) ((((size_t)(a) * (n2) + (b)) * (n3) + (c)) * (n4) + (d))
+
+/* cclambda.cc:28-42: l1 = 2 t1, l2 = 2 (2 t2 - t2^T); HBAR of the current
+ * amplitudes into s->hb. */
+void orc_lambda_init(void *p) {
+  orc_state *s = (orc_state *)p;
+  const size_t o = s->no, v = s->nv, ov = o * v, oovv = ov * ov;
+  const size_t sz[11] = {o * v, o * o, v * v, o * o * o * o, v * v * v * v, v * o * v * v, o * o * o * v,
+                         o * v * v * o, o * v * o * v, v * v * v * o, o * v * o * o};
+  for (int k = 0; k < 11; k++)
+    if (!s->hb[k]) s->hb[k] = zalloc(sz[k]);
+  if (!s->l1) {
+    s->l1 = zalloc(ov); s->l1old = zalloc(ov);
+    s->l2 = zalloc(oovv); s->l2old = zalloc(oovv); s->lZ = zalloc(oovv);
+    s->Goo = zalloc(o * o); s->Gvv = zalloc(v * v);
+  }
+  orc_hbar(s, s->hb[HOV], s->hb[HOO], s->hb[HVV], s->hb[HOOOO], s->hb[HVVVV], s->hb[HVOVV], s->hb[HOOOV],
+           s->hb[HOVVO], s->hb[HOVOV], s->hb[HVVVO], s->hb[HOVOO]);
+  const int no = s->no, nv = s->nv;
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      s->l1[O2(i, a)] = 2.0 * s->t1[O2(i, a)];
+      for (int j = 0; j < no; j++)
+        for (int b = 0; b < nv; b++)
+          s->l2[T4(i, j, a, b)] = 2.0 * (2.0 * s->t2[T4(i, j, a, b)] - s->t2[T4(i, j, b, a)]);
+    }
+}
+
+/* build_G, cclambda.cc:137-156, from l2 (the saved l2old in the reference):
+ *   Goo[m][i] = sum_jab t2_mj^ab l2_ij^ab,  Gvv[a][e] = -sum_ijb t2_ij^eb l2_ij^ab */
+static void lambda_G(orc_state *s, const double *l2) {
+  const int no = s->no, nv = s->nv;
+  const double *t2 = s->t2;
+  for (int m = 0; m < no; m++)
+    for (int i = 0; i < no; i++) {
+      double value = 0.0;
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++)
+          for (int j = 0; j < no; j++) value += t2[T4(m, j, a, b)] * l2[T4(i, j, a, b)];
+      s->Goo[(size_t)m * no + i] = value;
+    }
+  for (int a = 0; a < nv; a++)
+    for (int e = 0; e < nv; e++) {
+      double value = 0.0;
+      for (int i = 0; i < no; i++)
+        for (int j = 0; j < no; j++)
+          for (int b = 0; b < nv; b++) value -= t2[T4(i, j, e, b)] * l2[T4(i, j, a, b)];
+      s->Gvv[(size_t)a * nv + e] = value;
+    }
+}
+
+#define HB(k) (s->hb[k])
+#define Hov_(m, e) HB(HOV)[O2(m, e)]
+#define Hoo_(m, i) HB(HOO)[(size_t)(m) * no + (i)]
+#define Hvv_(a, e) HB(HVV)[(size_t)(a) * nv + (e)]
+#define Hoooo_(m, n, i, j) HB(HOOOO)[Q4(m, n, i, j, no, no, no)]
+#define Hvvvv_(a, b, e, f) HB(HVVVV)[Q4(a, b, e, f, nv, nv, nv)]
+#define Hvovv_(a, m, e, f) HB(HVOVV)[Q4(a, m, e, f, no, nv, nv)]
+#define Hooov_(m, n, i, e) HB(HOOOV)[Q4(m, n, i, e, no, no, nv)]
+#define Hovvo_(m, b, e, j) HB(HOVVO)[Q4(m, b, e, j, nv, nv, no)]
+#define Hovov_(m, b, j, e) HB(HOVOV)[Q4(m, b, j, e, nv, no, nv)]
+#define Hvvvo_(a, b, e, i) HB(HVVVO)[Q4(a, b, e, i, nv, nv, no)]
+#define Hovoo_(m, b, i, j) HB(HOVOO)[Q4(m, b, i, j, nv, no, no)]
+#define Goo_(m, i) s->Goo[(size_t)(m) * no + (i)]
+#define Gvv_(a, e) s->Gvv[(size_t)(a) * nv + (e)]
+
+/* build_l1, cclambda.cc:158-210: the raw l1 residual R1 from (l1, l2) and G */
+static void lambda_l1(orc_state *s, const double *l1, const double *l2, double *R1) {
+  const int no = s->no, nv = s->nv;
+#define L1(i, a) l1[O2(i, a)]
+#define L2(i, j, a, b) l2[T4(i, j, a, b)]
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      double value = 2 * Hov_(i, a);
+      for (int e = 0; e < nv; e++) value += L1(i, e) * Hvv_(e, a);
+      for (int m = 0; m < no; m++) value -= L1(m, a) * Hoo_(i, m);
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++) value += L1(m, e) * (2 * Hovvo_(i, e, a, m) - Hovov_(i, e, m, a));
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++)
+          for (int f = 0; f < nv; f++) value += L2(i, m, e, f) * Hvvvo_(e, f, a, m);
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++)
+          for (int e = 0; e < nv; e++) value -= L2(m, n, a, e) * Hovoo_(i, e, m, n);
+      for (int e = 0; e < nv; e++)
+        for (int f = 0; f < nv; f++) value -= Gvv_(e, f) * (2 * Hvovv_(e, i, f, a) - Hvovv_(e, i, a, f));
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++) value -= Goo_(m, n) * (2 * Hooov_(m, i, n, a) - Hooov_(i, m, n, a));
+      R1[O2(i, a)] = value;
+    }
+}
+
+/* build_l2, cclambda.cc:212-268: Z from (l1, l2) and G, then the raw l2
+ * residual R2_ijab = Z_ijab + Z_jiba */
+static void lambda_l2(orc_state *s, const double *l1, const double *l2, double *R2) {
+  const int no = s->no, nv = s->nv;
+  double *Z = s->lZ;
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) {
+          double value = LL(i, j, a + no, b + no);
+          value += 2.0 * L1(i, a) * Hov_(j, b) - L1(j, a) * Hov_(i, b);
+          for (int e = 0; e < nv; e++) value += L2(i, j, e, b) * Hvv_(e, a);
+          for (int m = 0; m < no; m++) value -= L2(m, j, a, b) * Hoo_(i, m);
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++) value += 0.5 * L2(m, n, a, b) * Hoooo_(i, j, m, n);
+          for (int e = 0; e < nv; e++)
+            for (int f = 0; f < nv; f++) value += 0.5 * L2(i, j, e, f) * Hvvvv_(e, f, a, b);
+          for (int e = 0; e < nv; e++) value += L1(i, e) * (2 * Hvovv_(e, j, a, b) - Hvovv_(e, j, b, a));
+          for (int m = 0; m < no; m++) value -= L1(m, b) * (2 * Hooov_(j, i, m, a) - Hooov_(i, j, m, a));
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++) {
+              value += (2 * Hovvo_(i, e, a, m) - Hovov_(i, e, m, a)) * L2(m, j, e, b);
+              value -= Hovov_(j, e, m, a) * L2(m, i, b, e);
+              value -= Hovvo_(j, e, a, m) * L2(m, i, e, b);
+            }
+          for (int e = 0; e < nv; e++) value += Gvv_(a, e) * LL(i, j, e + no, b + no);
+          for (int m = 0; m < no; m++) value -= Goo_(m, i) * LL(m, j, a + no, b + no);
+          Z[T4(i, j, a, b)] = value;
+        }
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) R2[T4(i, j, a, b)] = Z[T4(i, j, a, b)] + Z[T4(j, i, b, a)];
+}
+#undef L1
+#undef L2
+
+/* pseudoenergy, cclambda.cc:270-288: 1/2 sum <ij|ab> l2_ij^ab */
+static double lambda_pseudo(const orc_state *s, const double *l2) {
+  const int no = s->no, nv = s->nv;
+  double energy = 0.0;
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) energy += 0.5 * INT(i, j, a + no, b + no) * l2[T4(i, j, a, b)];
+  return energy;
+}
+
+double orc_lambda_pseudoenergy(void *p) {
+  orc_state *s = (orc_state *)p;
+  return s->l2 ? lambda_pseudo(s, s->l2) : 0.0;
+}
+
+/* One evaluation of build_G, build_l1, build_l2 at (l1, l2) over the HBAR
+ * built by orc_lambda_init: R1, R2 (raw, before the denominators) and the
+ * G intermediates.  The context's own l1/l2 are not touched. */
+void orc_lambda_residual(void *p, const double *l1, const double *l2, double *R1, double *R2, double *Goo,
+                         double *Gvv) {
+  orc_state *s = (orc_state *)p;
+  lambda_G(s, l2);
+  lambda_l1(s, l1, l2, R1);
+  lambda_l2(s, l1, l2, R2);
+  if (Goo) memcpy(Goo, s->Goo, (size_t)s->no * s->no * sizeof(double));
+  if (Gvv) memcpy(Gvv, s->Gvv, (size_t)s->nv * s->nv * sizeof(double));
+}
+
+/* CCLambda::compute_lambda, cclambda.cc:93-135: from the guess of
+ * orc_lambda_init, amp_save (:59-68), build_G/l1/l2, increment_amps
+ * (:70-91: l = l_old + R/D with the CCSD denominators, rms the 2-norm of the
+ * raw R), break on rms < conv before DIIS, else DIIS (the Pulay step of
+ * orc_ccsd on (l1 | l2), extrapolating from min_vecs stored vectors) and the
+ * pseudoenergy of the extrapolated l2.  trace[0..niter]: the pseudoenergy of
+ * the guess, after each iteration's DIIS, and of the final l2 last.
+ * out: pseudoenergy of the final l2, rms, niter.  Returns 0 on convergence,
+ * 1 if not converged (cclambda.cc:133-134). */
+int orc_lambda(void *p, int maxiter, double conv, int do_diis, int diis_policy, int min_vecs, double *out,
+               double *trace) {
+  orc_state *s = (orc_state *)p;
+  const int no = s->no, nv = s->nv;
+  const size_t ov = (size_t)no * nv;
+  diis_free(s);
+  if (trace) trace[0] = lambda_pseudo(s, s->l2);
+  double rms = 0.0;
+  int niter = 0;
+  for (int iter = 1; iter <= maxiter; iter++) {
+    double *x = s->l1; s->l1 = s->l1old; s->l1old = x;
+    x = s->l2; s->l2 = s->l2old; s->l2old = x;
+    orc_lambda_residual(s, s->l1old, s->l2old, s->l1, s->l2, NULL, NULL);
+    double r1 = 0.0, r2 = 0.0;
+    for (size_t k = 0; k < ov; k++) {
+      r1 += s->l1[k] * s->l1[k];
+      s->l1[k] = s->l1old[k] + s->l1[k] / s->D1[k];
+    }
+    for (size_t k = 0; k < ov * ov; k++) {
+      r2 += s->l2[k] * s->l2[k];
+      s->l2[k] = s->l2old[k] + s->l2[k] / s->D2[k];
+    }
+    rms = sqrt(r1 + r2);
+    niter = iter;
+    if (rms < conv) break;
+    if (do_diis) diis_step_x(s, s->l1, s->l1old, s->l2, s->l2old, 8, diis_policy, min_vecs);
+    if (trace) trace[iter] = lambda_pseudo(s, s->l2);
+  }
+  const double pe = lambda_pseudo(s, s->l2);
+  if (trace) trace[niter] = pe;
+  if (out) { out[0] = pe; out[1] = rms; out[2] = niter; }
+  diis_free(s);
+  return rms < conv ? 0 : 1;
+}
+
+void orc_get_lambda(void *p, double *l1, double *l2) {
+  orc_state *s = (orc_state *)p;
+  size_t ov = (size_t)s->no * s->nv;
+  if (l1 && s->l1) memcpy(l1, s->l1, ov * sizeof(double));
+  if (l2 && s->l2) memcpy(l2, s->l2, ov * ov * sizeof(double));
+}
+
+void orc_set_lambda(void *p, const double *l1, const double *l2) {
+  orc_state *s = (orc_state *)p;
+  size_t ov = (size_t)s->no * s->nv;
+  if (!s->l1) orc_lambda_init(s);
+  memcpy(s->l1, l1, ov * sizeof(double));
+  memcpy(s->l2, l2, ov * ov * sizeof(double));
+}
+#undef HB
+#undef Q4

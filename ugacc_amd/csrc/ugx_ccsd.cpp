@@ -129,6 +129,10 @@ struct ugx_ctx {
   // HBAR blocks (hbar.cc:8-228), built by ugx_hbar, and its intermediates
   DBuf hb_ov, hb_oo, hb_vv, hb_oooo, hb_vvvv, hb_vovv, hb_ooov, hb_ovvo, hb_ovov, hb_vvvo, hb_ovoo;
   DBuf hb_B, hb_Bp, hb_Lt, hb_C, hb_D, hb_X, hb_E, hb_G, hb_t1L, hb_Woo, hb_Q;
+  // Lambda (cclambda.cc): l1/l2 and their previous values, Goo/Gvv, the raw
+  // l1 residual and the half residual Z of l2, the update's partial sums
+  DBuf lam_l1, lam_l1o, lam_l2, lam_l2o, lam_Goo, lam_Gvv, lam_R1, lam_Z;
+  bool have_lambda = false;
   // residual of the last ugx_residual, for ugx_intermediates
   bool have_intermediates = false;
   double last_flops = 0.0;
@@ -634,7 +638,11 @@ void diis_alloc(ugx_ctx *c, int nvec) {
   diis_reset(c);
 }
 
-int diis_step(ugx_ctx *c, int nvec, int policy, int min_vecs) {
+// One Pulay step over (x1 | x2) with their previous values (x1o | x2o): the
+// CCSD amplitudes or the Lambda amplitudes, which share the subspace storage
+// (the Lambda iterations run after the CCSD ones and reset it).
+int diis_step_x(ugx_ctx *c, double *x1, const double *x1o, double *x2, const double *x2o, int nvec, int policy,
+                int min_vecs) {
   const long n1 = c->ov(), n2 = c->oovv_n(), len = n1 + n2;
   if (c->diis_nvec != nvec || (int)c->diis_err.size() != nvec) diis_alloc(c, nvec);
   int slot;
@@ -647,7 +655,7 @@ int diis_step(ugx_ctx *c, int nvec, int policy, int min_vecs) {
   }
   DBuf &e = *c->diis_err[slot];
   DBuf &x = *c->diis_amp[slot];
-  launch_diis_store(n1, n2, c->t1.d(), c->t1o.d(), c->t2.d(), c->t2o.d(), e.d(), x.d(), c->st);
+  launch_diis_store(n1, n2, x1, x1o, x2, x2o, e.d(), x.d(), c->st);
   if (c->diis_count < nvec) c->diis_count++;
   c->diis_next = (slot + 1) % nvec;
   const int n = c->diis_count;
@@ -682,8 +690,12 @@ int diis_step(ugx_ctx *c, int nvec, int policy, int min_vecs) {
   if (lin_solve(B, C, m) != 0) return 0;
   std::vector<const double *> amps(n);
   for (int p = 0; p < n; p++) amps[p] = c->diis_amp[p]->d();
-  launch_diis_combine(n1, n2, amps.data(), C.data(), n, c->t1.d(), c->t2.d(), c->st);
+  launch_diis_combine(n1, n2, amps.data(), C.data(), n, x1, x2, c->st);
   return n;
+}
+
+int diis_step(ugx_ctx *c, int nvec, int policy, int min_vecs) {
+  return diis_step_x(c, c->t1.d(), c->t1o.d(), c->t2.d(), c->t2o.d(), nvec, policy, min_vecs);
 }
 
 // -------------------------------------------------------------- setup
@@ -931,6 +943,131 @@ void hbar(ugx_ctx *c) {
   K("njfb,mnif->mbij", 1.0, T2, Lt, 1.0, HOVOO);   // t1_ie t2_nj^fb L<mn|ef>
   K("me,ijeb->mbij", 1.0, T1L, T2, 1.0, HOVOO);    // t1_nf t2_ij^eb L<mn|ef>
   K("nb,mnij->mbij", -1.0, T1, QQ, 1.0, HOVOO);    // t1_jf t1_ie t1_nb <mn|ef>
+}
+
+// -------------------------------------------------------------- Lambda
+// CCLambda (cclambda.cc:13-289), the CCSD Lagrange multipliers over the HBAR
+// of the context's amplitudes, in FP64.  build_G, build_l1 and build_l2 are
+// engine contractions on the hb_* blocks; the update l = l_old + R/D, with
+// R2 = Z + Z^T and the 2-norm of the raw residual, is the CCSD update kernel;
+// DIIS is the CCSD Pulay step on (l1 | l2).  The CCSD_T / DERTYPE=FIRST
+// terms s1/s2 (cclambda.cc:184, :235) belong to the (T) gradient, which
+// this library does not build.
+void lambda_alloc(ugx_ctx *c) {
+  const long ov = c->ov(), oovv = c->oovv_n();
+  for (DBuf *b : {&c->lam_l1, &c->lam_l1o, &c->lam_R1}) b->alloc(ov * 8);
+  for (DBuf *b : {&c->lam_l2, &c->lam_l2o, &c->lam_Z}) b->alloc(oovv * 8);
+  c->lam_Goo.alloc((size_t)c->no * c->no * 8);
+  c->lam_Gvv.alloc((size_t)c->nv * c->nv * 8);
+}
+
+// build_G (cclambda.cc:137-156), build_l1 (:158-210) and build_l2 (:212-268)
+// at (l1, l2): the raw l1 residual into R1 and the half residual Z of l2
+// (R2_ijab = Z_ijab + Z_jiba), Goo/Gvv into lam_Goo/lam_Gvv.
+void lambda_residual(ugx_ctx *c, const double *l1p, const double *l2p, double *R1p, double *Zp) {
+  Engine &E = *c->eng;
+  const int o = c->no, v = c->nv;
+  const int P = PREC_FP64;
+  auto V = [](const double *p, std::vector<int> d) { return vref(const_cast<double *>(p), std::move(d)); };
+  auto K = [&](const char *spec, double a, const TRef &A, const TRef &B, double b, const TRef &C) {
+    E.contract(spec, a, A, B, b, C, P, "lambda");
+  };
+  const auto L1 = V(l1p, {o, v}), L2 = V(l2p, {o, o, v, v}), T2 = V(c->t2.d(), {o, o, v, v});
+  const auto R1 = V(R1p, {o, v}), Z = V(Zp, {o, o, v, v});
+  const auto GOO = V(c->lam_Goo.d(), {o, o}), GVV = V(c->lam_Gvv.d(), {v, v});
+  const auto HOV = V(c->hb_ov.d(), {o, v}), HOO = V(c->hb_oo.d(), {o, o}), HVV = V(c->hb_vv.d(), {v, v});
+  const auto HOOOO = V(c->hb_oooo.d(), {o, o, o, o}), HVVVV = V(c->hb_vvvv.d(), {v, v, v, v});
+  const auto HVOVV = V(c->hb_vovv.d(), {v, o, v, v}), HOOOV = V(c->hb_ooov.d(), {o, o, o, v});
+  const auto HOVVO = V(c->hb_ovvo.d(), {o, v, v, o}), HOVOV = V(c->hb_ovov.d(), {o, v, o, v});
+  const auto HVVVO = V(c->hb_vvvo.d(), {v, v, v, o}), HOVOO = V(c->hb_ovoo.d(), {o, v, o, o});
+  const auto LOOVV = cref(c->Loovv.d(), {o, o, v, v});
+  // build_G
+  K("mjab,ijab->mi", 1.0, T2, L2, 0.0, GOO);
+  K("ijeb,ijab->ae", -1.0, T2, L2, 0.0, GVV);
+  // build_l1
+  E.permute("ia->ia", 2.0, HOV, 0.0, R1);
+  K("ie,ea->ia", 1.0, L1, HVV, 1.0, R1);
+  K("ma,im->ia", -1.0, L1, HOO, 1.0, R1);
+  K("me,ieam->ia", 2.0, L1, HOVVO, 1.0, R1);
+  K("me,iema->ia", -1.0, L1, HOVOV, 1.0, R1);
+  K("imef,efam->ia", 1.0, L2, HVVVO, 1.0, R1);
+  K("mnae,iemn->ia", -1.0, L2, HOVOO, 1.0, R1);
+  K("ef,eifa->ia", -2.0, GVV, HVOVV, 1.0, R1);
+  K("ef,eiaf->ia", 1.0, GVV, HVOVV, 1.0, R1);
+  K("mn,mina->ia", -2.0, GOO, HOOOV, 1.0, R1);
+  K("mn,imna->ia", 1.0, GOO, HOOOV, 1.0, R1);
+  // build_l2, Z before the pair symmetrization
+  E.permute("ijab->ijab", 1.0, LOOVV, 0.0, Z);
+  if (!E.sizing()) launch_lambda_outer(o, v, l1p, c->hb_ov.d(), Zp, c->st);
+  K("ijeb,ea->ijab", 1.0, L2, HVV, 1.0, Z);
+  K("mjab,im->ijab", -1.0, L2, HOO, 1.0, Z);
+  K("mnab,ijmn->ijab", 0.5, L2, HOOOO, 1.0, Z);
+  K("ijef,efab->ijab", 0.5, L2, HVVVV, 1.0, Z);
+  K("ie,ejab->ijab", 2.0, L1, HVOVV, 1.0, Z);
+  K("ie,ejba->ijab", -1.0, L1, HVOVV, 1.0, Z);
+  K("mb,jima->ijab", -2.0, L1, HOOOV, 1.0, Z);
+  K("mb,ijma->ijab", 1.0, L1, HOOOV, 1.0, Z);
+  K("ieam,mjeb->ijab", 2.0, HOVVO, L2, 1.0, Z);
+  K("iema,mjeb->ijab", -1.0, HOVOV, L2, 1.0, Z);
+  K("jema,mibe->ijab", -1.0, HOVOV, L2, 1.0, Z);
+  K("jeam,mieb->ijab", -1.0, HOVVO, L2, 1.0, Z);
+  K("ae,ijeb->ijab", 1.0, GVV, LOOVV, 1.0, Z);
+  K("mi,mjab->ijab", -1.0, GOO, LOOVV, 1.0, Z);
+}
+
+// pseudoenergy (cclambda.cc:270-288): 1/2 sum <ij|ab> l2_ij^ab
+double lambda_pseudo(ugx_ctx *c, const double *l2) {
+  const int np = 64;
+  launch_dot(c->oovv_n(), c->oovv.d(), l2, c->part.d(), np, c->st);
+  return 0.5 * host_sum(c, np);
+}
+
+// CCLambda::CCLambda + compute_lambda (cclambda.cc:28-42, :93-135): HBAR of
+// the current amplitudes, guess l1 = 2 t1, l2 = 2 (2 t2 - t2^T), Jacobi
+// updates, break on rms < r_convergence before DIIS, DIIS extrapolating from
+// diis_min_vecs stored vectors.  trace (maxiter + 1 values, or null): the
+// pseudoenergy of the guess, after each iteration's DIIS, and of the final l2
+// last.  Returns whether it converged.
+bool lambda_solve(ugx_ctx *c, const ugx_options &opt, double *pseudo, int *niter_out, double *rms_out,
+                  double *trace) {
+  hbar(c);
+  lambda_alloc(c);
+  const long ov = c->ov(), oovv = c->oovv_n();
+  const int o = c->no, v = c->nv;
+  // guess (cclambda.cc:28-42): l1 = 2 t1, l2 = 4 t2 - 2 t2^T
+  launch_axpby(ov, 2.0, c->t1.d(), 0.0, c->lam_l1.d(), c->st);
+  c->eng->permute("ijba->ijab", -2.0, vref(c->t2.d(), {o, o, v, v}), 0.0, vref(c->lam_l2.d(), {o, o, v, v}));
+  launch_axpby(oovv, 4.0, c->t2.d(), 1.0, c->lam_l2.d(), c->st);
+  if (trace) trace[0] = lambda_pseudo(c, c->lam_l2.d());
+  if (opt.diis) diis_alloc(c, opt.diis_nvec);
+  double rms = 0.0;
+  int niter = 0;
+  for (int iter = 1; iter <= opt.maxiter; iter++) {
+    std::swap(c->lam_l1.p, c->lam_l1o.p);
+    std::swap(c->lam_l1.bytes, c->lam_l1o.bytes);
+    std::swap(c->lam_l2.p, c->lam_l2o.p);
+    std::swap(c->lam_l2.bytes, c->lam_l2o.bytes);
+    lambda_residual(c, c->lam_l1o.d(), c->lam_l2o.d(), c->lam_R1.d(), c->lam_Z.d());
+    const int np = update_nparts(o, v);
+    launch_update(o, v, c->eps.d(), c->lam_R1.d(), c->lam_Z.d(), c->lam_l1o.d(), c->lam_l2o.d(), c->lam_l1.d(),
+                  c->lam_l2.d(), nullptr, c->part.d(), np, c->st);
+    rms = std::sqrt(host_sum(c, np));
+    niter = iter;
+    if (opt.print) std::fprintf(stderr, "ugx: Lambda iter %3d  rms %.3e\n", iter, rms);
+    if (rms < opt.r_convergence) break;
+    if (opt.diis)
+      diis_step_x(c, c->lam_l1.d(), c->lam_l1o.d(), c->lam_l2.d(), c->lam_l2o.d(), opt.diis_nvec, opt.diis_policy,
+                  opt.diis_min_vecs);
+    if (trace) trace[iter] = lambda_pseudo(c, c->lam_l2.d());
+  }
+  const double pe = lambda_pseudo(c, c->lam_l2.d());
+  if (trace) trace[niter] = pe;
+  if (opt.diis) diis_reset(c);
+  c->have_lambda = true;
+  *pseudo = pe;
+  *niter_out = niter;
+  *rms_out = rms;
+  return rms < opt.r_convergence;
 }
 
 // ------------------------------------------------------------------ (T)
@@ -1782,6 +1919,7 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
     alloc_work(c);
     check(hipStreamSynchronize(c->st), "stream sync");
     c->loaded = true;
+    c->have_lambda = false;
     return UGX_OK;
   });
 }
@@ -2058,6 +2196,68 @@ int ugx_hbar(ugx_ctx *c, double *Hov, double *Hoo, double *Hvv, double *Hoooo, d
     get(Hovov, c->hb_ovov, o * v * o * v);
     get(Hvvvo, c->hb_vvvo, v * v * v * o);
     get(Hovoo, c->hb_ovoo, o * v * o * o);
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+int ugx_lambda(ugx_ctx *c, const ugx_options *opt_in, double *pseudoenergy, int *niter, double *rms,
+               double *trace) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!c->have_amps) throw UgxError(UGX_NOT_READY, "ugx_lambda: no amplitudes (run ugx_ccsd first)");
+    ugx_options opt;
+    if (opt_in) opt = *opt_in;
+    else ugx_default_options(&opt);
+    validate_options(c, opt, "ugx_lambda");
+    double pe = 0.0, r = 0.0;
+    int n = 0;
+    const bool conv = lambda_solve(c, opt, &pe, &n, &r, trace);
+    if (pseudoenergy) *pseudoenergy = pe;
+    if (niter) *niter = n;
+    if (rms) *rms = r;
+    if (!conv) {
+      c->err = "Computation has not converged.";
+      return UGX_NOT_CONVERGED;
+    }
+    return UGX_OK;
+  });
+}
+
+int ugx_get_lambda(ugx_ctx *c, double *l1, double *l2) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!c->have_lambda) throw UgxError(UGX_NOT_READY, "ugx_get_lambda: no Lambda amplitudes (run ugx_lambda)");
+    if (l1) check(hipMemcpyAsync(l1, c->lam_l1.p, c->ov() * 8, hipMemcpyDeviceToHost, c->st), "l1 copy");
+    if (l2) check(hipMemcpyAsync(l2, c->lam_l2.p, c->oovv_n() * 8, hipMemcpyDeviceToHost, c->st), "l2 copy");
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+int ugx_lambda_residual(ugx_ctx *c, const double *l1, const double *l2, double *R1, double *R2, double *Goo,
+                        double *Gvv) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!c->have_amps) throw UgxError(UGX_NOT_READY, "ugx_lambda_residual: no amplitudes");
+    if (!l1 || !l2) throw UgxError(UGX_BAD_INPUT, "ugx_lambda_residual: null l1/l2");
+    hbar(c);
+    lambda_alloc(c);
+    const long ov = c->ov(), oovv = c->oovv_n();
+    // the arguments go into the "old" buffers: l1/l2 of a finished solve stay
+    check(hipMemcpyAsync(c->lam_l1o.p, l1, ov * 8, hipMemcpyHostToDevice, c->st), "l1 copy");
+    check(hipMemcpyAsync(c->lam_l2o.p, l2, oovv * 8, hipMemcpyHostToDevice, c->st), "l2 copy");
+    lambda_residual(c, c->lam_l1o.d(), c->lam_l2o.d(), c->lam_R1.d(), c->lam_Z.d());
+    if (R1) check(hipMemcpyAsync(R1, c->lam_R1.p, ov * 8, hipMemcpyDeviceToHost, c->st), "R1 copy");
+    if (R2) {
+      // R2 = Z + Z^T(ij,ab), formed in lam_l2o (its input copy is consumed)
+      const int o = c->no, v = c->nv;
+      c->eng->permute("jiba->ijab", 1.0, vref(c->lam_Z.d(), {o, o, v, v}), 0.0, vref(c->lam_l2o.d(), {o, o, v, v}));
+      launch_axpby(oovv, 1.0, c->lam_Z.d(), 1.0, c->lam_l2o.d(), c->st);
+      check(hipMemcpyAsync(R2, c->lam_l2o.p, oovv * 8, hipMemcpyDeviceToHost, c->st), "R2 copy");
+    }
+    if (Goo) check(hipMemcpyAsync(Goo, c->lam_Goo.p, (size_t)c->no * c->no * 8, hipMemcpyDeviceToHost, c->st), "Goo");
+    if (Gvv) check(hipMemcpyAsync(Gvv, c->lam_Gvv.p, (size_t)c->nv * c->nv * 8, hipMemcpyDeviceToHost, c->st), "Gvv");
     check(hipStreamSynchronize(c->st), "stream sync");
     return UGX_OK;
   });

@@ -152,6 +152,8 @@ void launch_pack_panel2(const double *in, long sp, long sr1, long sr2, long R2, 
 
 void launch_fill(long n, double v, double *y, hipStream_t st);
 // t1_ia = R1_ia / (eps_i - eps_a) (the update's T1 denominator, ccwfn.cc:62-65)
+// Z_ijab += 2 l1_ia Hov_jb - l1_ja Hov_ib (the l1 x Hov terms of the l2 residual)
+void launch_lambda_outer(int no, int nv, const double *l1, const double *hov, double *Z, hipStream_t st);
 void launch_t1_divide(int no, int nv, const double *eps, const double *R1, double *t1, hipStream_t st);
 // flag[0] = 1 when any of x[0..n) is NaN or infinite (flag untouched otherwise)
 void launch_nonfinite(long n, const double *x, double *flag, hipStream_t st);

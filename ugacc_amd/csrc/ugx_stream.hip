@@ -220,6 +220,20 @@ __global__ void k_t1_divide(int no, int nv, const double *__restrict__ eps, cons
   }
 }
 
+// Z_ijab += 2 l1_ia Hov_jb - l1_ja Hov_ib (cclambda.cc:238), b fastest
+__global__ void k_lambda_outer(int no, int nv, const double *__restrict__ l1, const double *__restrict__ hov,
+                               double *__restrict__ Z) {
+  const long n = (long)no * no * nv * nv;
+  for (long q = blockIdx.x * (long)NT + threadIdx.x; q < n; q += (long)gridDim.x * NT) {
+    const int b = (int)(q % nv);
+    const long r = q / nv;
+    const int a = (int)(r % nv);
+    const long ij = r / nv;
+    const int j = (int)(ij % no), i = (int)(ij / no);
+    Z[q] += 2.0 * l1[(long)i * nv + a] * hov[(long)j * nv + b] - l1[(long)j * nv + a] * hov[(long)i * nv + b];
+  }
+}
+
 // flag[0] |= 1 when any of the n values is NaN or infinite
 __global__ void k_nonfinite(long n, const double *__restrict__ x, double *flag) {
   bool bad = false;
@@ -672,6 +686,12 @@ void launch_t1_divide(int no, int nv, const double *eps, const double *R1, doubl
   const long n = (long)no * nv;
   if (n <= 0) return;
   hipLaunchKernelGGL(k_t1_divide, dim3(nblocks(n)), dim3(NT), 0, st, no, nv, eps, R1, t1);
+}
+
+void launch_lambda_outer(int no, int nv, const double *l1, const double *hov, double *Z, hipStream_t st) {
+  const long n = (long)no * no * nv * nv;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_lambda_outer, dim3(nblocks(n)), dim3(NT), 0, st, no, nv, l1, hov, Z);
 }
 
 void launch_nonfinite(long n, const double *x, double *flag, hipStream_t st) {

@@ -11,10 +11,11 @@
  *                    compute_energy steps (ugacc.cc:104-115) on the GPU:
  *                    the six Dirac integral blocks and the active Fock
  *                    matrix are built on the host and handed to ugx_ccsd.
- * Energies are printed in the layout of ccwfn.cc:234-247.  The post-CC
- * steps of DERTYPE FIRST / SECOND (HBAR, Lambda, densities, response,
- * ugacc.cc:117-) are not part of this library: such a request fails with
- * a PSIEXCEPTION instead of returning wrong numbers.
+ * Energies are printed in the layout of ccwfn.cc:234-247.  Of the post-CC
+ * steps of DERTYPE FIRST / SECOND (ugacc.cc:117-) the library builds HBAR
+ * (ugx_hbar) and Lambda (ugx_lambda) but not the densities or the linear
+ * response that those DERTYPEs print: such a request fails with a
+ * PSIEXCEPTION instead of returning wrong numbers.
  *
  * Integrals: MintsHelper::mo_eri over the active occupied / virtual AO
  * coefficients gives chemist (pq|rs); the blocks are <pq|rs> = (pr|qs).
@@ -104,7 +105,7 @@ extern "C" PSI_API SharedWavefunction ugacc(SharedWavefunction ref, Options &opt
     if (ref->soccpi()[h]) throw PSIEXCEPTION("UGACC is for closed-shell systems only.");
   if (options.get_str("DERTYPE") != "NONE")
     throw PSIEXCEPTION("ugacc (MI355X): DERTYPE " + options.get_str("DERTYPE") +
-                       " needs HBAR, Lambda and densities, which this library does not provide");
+                       " needs the CC densities (ccdensity.cc), which this library does not build");
 
   // active orbitals (frozen core and frozen virtuals removed), AO basis
   SharedMatrix Co = ref->Ca_subset("AO", "ACTIVE_OCC");

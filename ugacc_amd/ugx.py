@@ -25,7 +25,8 @@ PREC_NAMES = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
 EXPORTS = [
     "ugx_default_options", "ugx_create", "ugx_destroy", "ugx_last_error", "ugx_load_hamiltonian",
     "ugx_set_amplitudes", "ugx_prepare", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
-    "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_hbar", "ugx_set_profiling",
+    "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_hbar", "ugx_lambda",
+    "ugx_get_lambda", "ugx_lambda_residual", "ugx_set_profiling",
     "ugx_kernel_stats", "ugx_gemm", "ugx_gemm_bf16x3_static", "ugx_synchronize", "ugx_batch_solve", "ugx_set_autotune", "ugx_tile_table",
     "ugx_save_tile_table", "ugx_load_tile_table", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
     "ugx_mlp_last_error", "ugx_mlp_set_params", "ugx_mlp_get_params", "ugx_mlp_set_scaling", "ugx_mlp_forward",
@@ -98,6 +99,9 @@ def load_library(path=LIBPATH):
     L.ugx_residual.argtypes = [vp, dp, dp, ip, dp, dp]
     L.ugx_intermediates.argtypes = [vp] + [dp] * 7
     L.ugx_hbar.argtypes = [vp] + [dp] * 11
+    L.ugx_lambda.argtypes = [vp, ctypes.POINTER(Options), dp, ctypes.POINTER(ctypes.c_int), dp, dp]
+    L.ugx_get_lambda.argtypes = [vp, dp, dp]
+    L.ugx_lambda_residual.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.ugx_set_profiling.argtypes = [vp, ip]
     L.ugx_set_profiling.restype = None
     L.ugx_kernel_stats.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
@@ -290,6 +294,37 @@ class Context:
         out = {k: np.zeros(shapes[k]) for k in which}
         self._check(self.L.ugx_hbar(self.h, *[_p(out[k]) if k in out else None for k in self.HBAR_BLOCKS]))
         return out
+
+    def lambda_solve(self, options=None, **kw):
+        """ugx_lambda: the CCSD Lambda equations (cclambda.cc:93-135) over the
+        HBAR of the current amplitudes.  Returns converged, pseudoenergy,
+        niter, rms and the pseudoenergy trace."""
+        o = options if options is not None else default_options()
+        for k, v in kw.items():
+            setattr(o, k, v)
+        pe, rms, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        trace = np.zeros(o.maxiter + 1)
+        rc = self._check(self.L.ugx_lambda(self.h, ctypes.byref(o), ctypes.byref(pe), ctypes.byref(n),
+                                           ctypes.byref(rms), _p(trace)), allow=(OK, NOT_CONVERGED))
+        return dict(converged=rc == OK, pseudoenergy=pe.value, niter=n.value, rms=rms.value,
+                    trace=trace[:n.value + 1])
+
+    def lambdas(self):
+        """ugx_get_lambda: (l1, l2) of the last lambda_solve."""
+        l1 = np.zeros((self.no, self.nv))
+        l2 = np.zeros((self.no, self.no, self.nv, self.nv))
+        self._check(self.L.ugx_get_lambda(self.h, _p(l1), _p(l2)))
+        return l1, l2
+
+    def lambda_residual(self, l1, l2):
+        """ugx_lambda_residual: (R1, R2, Goo, Gvv) at (l1, l2) over the HBAR
+        of the current amplitudes (cclambda.cc:137-268)."""
+        o, v = self.no, self.nv
+        l1, l2 = _c64(l1), _c64(l2)
+        R1, R2 = np.zeros((o, v)), np.zeros((o, o, v, v))
+        Goo, Gvv = np.zeros((o, o)), np.zeros((v, v))
+        self._check(self.L.ugx_lambda_residual(self.h, _p(l1), _p(l2), _p(R1), _p(R2), _p(Goo), _p(Gvv)))
+        return R1, R2, Goo, Gvv
 
     def set_profiling(self, on=True):
         self.L.ugx_set_profiling(self.h, 1 if on else 0)
