@@ -250,6 +250,21 @@ int ugx_get_lambda(ugx_ctx *ctx, double *l1, double *l2);
 int ugx_lambda_residual(ugx_ctx *ctx, const double *l1, const double *l2, double *R1, double *R2, double *Goo,
                         double *Gvv);
 
+/* CCDensity::onepdm + twopdm (ccdensity.cc:89-339): the CCSD one- and
+ * two-particle density blocks, built on the device in FP64 from the
+ * context's amplitudes and the Lambda amplitudes of the last ugx_lambda, and
+ * their energies.  energies (8 values, or NULL): E_one (f_oo.Doo + f_vv.Dvv,
+ * ccdensity.cc:138-146), E_two, then the OOOO, VVVV, OOOV, VVVO, OVOV and
+ * OOVV parts of E_two (the values the reference prints).  Blocks asked for
+ * (non-NULL) are copied out in the reference's layouts: Doo[i][j], Dvv[a][b],
+ * Dov[i][a], Dvo[a][i], Goooo[i][j][k][l], Gvvvv[a][b][c][d],
+ * Gooov[i][j][k][a], Gvvvo[a][b][c][i], Govov[i][a][j][b], Goovv[i][j][a][b].
+ * At converged amplitudes and multipliers E_one + E_two is the CCSD
+ * correlation energy when f_ov = 0 (ugacc.cc:129-137).  The (T) parts of
+ * CCSD_T (ccdensity.cc:43-74) are not included. */
+int ugx_density(ugx_ctx *ctx, double *energies, double *Doo, double *Dvv, double *Dov, double *Dvo, double *Goooo,
+                double *Gvvvv, double *Gooov, double *Gvvvo, double *Govov, double *Goovv);
+
 /* Device time per contraction label of the last ugx_ccsd / ugx_triples
  * call when profiling is on, as "label launches ms gflop\n" lines.  Returns
  * the number of bytes needed (including the terminating zero). */

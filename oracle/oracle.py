@@ -70,6 +70,7 @@ def _lib(omp=False):
         L.orc_get_lambda.argtypes = [vp, dp, dp]
         L.orc_set_lambda.argtypes = [vp, dp, dp]
         L.orc_lambda_pseudoenergy.restype = ctypes.c_double
+        L.orc_density.argtypes = [vp] + [dp] * 11
         L.orc_lambda_pseudoenergy.argtypes = [vp]
         _libs[omp] = L
     return _libs[omp]
@@ -270,6 +271,21 @@ class Oracle:
     def pseudoenergy(self):
         """cclambda.cc:270-288 of the current l2."""
         return self.L.orc_lambda_pseudoenergy(self.h)
+
+    DENSITY_BLOCKS = ("Doo", "Dvv", "Dov", "Dvo", "Goooo", "Gvvvv", "Gooov", "Gvvvo", "Govov", "Goovv")
+    DENSITY_PARTS = ("oooo", "vvvv", "ooov", "vvvo", "ovov", "oovv")
+
+    def density(self):
+        """CCDensity::onepdm + twopdm (ccdensity.cc:89-339) from the current
+        amplitudes and the Lambda amplitudes: e_one, e_two, the six parts of
+        e_two by name and the ten density blocks."""
+        o, v = self.no, self.nv
+        sh = dict(Doo=(o, o), Dvv=(v, v), Dov=(o, v), Dvo=(v, o), Goooo=(o, o, o, o), Gvvvv=(v, v, v, v),
+                  Gooov=(o, o, o, v), Gvvvo=(v, v, v, o), Govov=(o, v, o, v), Goovv=(o, o, v, v))
+        blocks = {k: np.zeros(sh[k]) for k in self.DENSITY_BLOCKS}
+        E = np.zeros(8)
+        self.L.orc_density(self.h, _p(E), *[_p(blocks[k]) for k in self.DENSITY_BLOCKS])
+        return dict(e_one=E[0], e_two=E[1], parts=dict(zip(self.DENSITY_PARTS, E[2:])), blocks=blocks)
 
     def residual_rows(self, rows):
         """(R1, R2[:rows]) of the current amplitudes."""

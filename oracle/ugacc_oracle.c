@@ -1371,4 +1371,196 @@ void orc_set_lambda(void *p, const double *l1, const double *l2) {
   memcpy(s->l2, l2, ov * ov * sizeof(double));
 }
 #undef HB
+
+/* ------------------------------------------------------------- densities
+ * CCDensity::onepdm / twopdm (ccdensity.cc:89-339): the CCSD one- and
+ * two-particle density blocks from the amplitudes (t1, t2, tau as current)
+ * and the Lambda amplitudes (l1, l2 of orc_lambda), and the energies they
+ * give with the Fock matrix and the integrals.  The (T) parts that CCSD_T
+ * copies in first (ccdensity.cc:43-74, from ccwfn.cc:854-1316) are not
+ * restated: these are the CCSD densities.
+ *   E[0] = E_one (ccdensity.cc:138-146), E[1] = E_two, E[2..7] = the
+ *   OOOO, VVVV, OOOV, VVVO, OVOV, OOVV parts of E_two (the values printed).
+ * Blocks (each may be NULL): Doo[i][j] Dvv[a][b] Dov[i][a] Dvo[a][i]
+ * Goooo[i][j][k][l] Gvvvv[a][b][c][d] Gooov[i][j][k][a] Gvvvo[a][b][c][i]
+ * Govov[i][a][j][b] Goovv[i][j][a][b]. */
+void orc_density(void *p, double *E, double *Doo_o, double *Dvv_o, double *Dov_o, double *Dvo_o, double *Goooo_o,
+                 double *Gvvvv_o, double *Gooov_o, double *Gvvvo_o, double *Govov_o, double *Goovv_o) {
+  orc_state *s = (orc_state *)p;
+  const int no = s->no, nv = s->nv;
+  const size_t o = no, v = nv;
+  orc_build_tau(s);
+  const double *t1 = s->t1, *t2 = s->t2, *tau = s->tau, *l1 = s->l1, *l2 = s->l2;
+  double *Doo = zalloc(o * o), *Dvv = zalloc(v * v), *Dov = zalloc(o * v), *Dvo = zalloc(v * o);
+  double *Goooo = zalloc(o * o * o * o), *Gvvvv = zalloc(v * v * v * v), *Gooov = zalloc(o * o * o * v);
+  double *Gvvvo = zalloc(v * v * v * o), *Govov = zalloc(o * v * o * v), *Goovv = zalloc(o * o * v * v);
+#define T1(i, a) t1[O2(i, a)]
+#define L1(i, a) l1[O2(i, a)]
+#define T2(i, j, a, b) t2[T4(i, j, a, b)]
+#define TAU(i, j, a, b) tau[T4(i, j, a, b)]
+#define L2(i, j, a, b) l2[T4(i, j, a, b)]
+  /* onepdm, ccdensity.cc:89-137 */
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++) {
+      double d = 0.0;
+      for (int e = 0; e < nv; e++) d -= T1(i, e) * L1(j, e);
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++)
+          for (int f = 0; f < nv; f++) d -= T2(i, m, e, f) * L2(j, m, e, f);
+      Doo[(size_t)i * no + j] = d;
+    }
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++) {
+      double d = 0.0;
+      for (int m = 0; m < no; m++) d += T1(m, b) * L1(m, a);
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++)
+          for (int e = 0; e < nv; e++) d += T2(m, n, b, e) * L2(m, n, a, e);
+      Dvv[(size_t)a * nv + b] = d;
+    }
+  for (int a = 0; a < nv; a++)
+    for (int i = 0; i < no; i++) Dvo[(size_t)a * no + i] = L1(i, a);
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      double d = 2.0 * T1(i, a);
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++) d += L1(m, e) * (2.0 * T2(i, m, a, e) - TAU(m, i, a, e));
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++)
+          for (int e = 0; e < nv; e++)
+            for (int f = 0; f < nv; f++)
+              d -= L2(m, n, e, f) * (T1(m, a) * T2(i, n, e, f) + T1(i, e) * T2(m, n, a, f));
+      Dov[O2(i, a)] = d;
+    }
+  double eone = 0.0;
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++) eone += F(i, j) * Doo[(size_t)i * no + j];
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++) eone += F(a + no, b + no) * Dvv[(size_t)a * nv + b];
+
+  /* twopdm, ccdensity.cc:149-339 */
+  double Eoooo = 0.0, Evvvv = 0.0, Eooov = 0.0, Evvvo = 0.0, Eovov = 0.0, Eoovv = 0.0;
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int k = 0; k < no; k++)
+        for (int l = 0; l < no; l++) {
+          double g = 0.0;
+          for (int e = 0; e < nv; e++)
+            for (int f = 0; f < nv; f++) g += TAU(i, j, e, f) * L2(k, l, e, f);
+          Goooo[Q4(i, j, k, l, no, no, no)] = g;
+          Eoooo += 0.5 * INT(i, j, k, l) * g;
+        }
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++)
+      for (int c = 0; c < nv; c++)
+        for (int d = 0; d < nv; d++) {
+          double g = 0.0;
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++) g += TAU(m, n, a, b) * L2(m, n, c, d);
+          Gvvvv[Q4(a, b, c, d, nv, nv, nv)] = g;
+          Evvvv += 0.5 * INT(a + no, b + no, c + no, d + no) * g;
+        }
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int k = 0; k < no; k++)
+        for (int a = 0; a < nv; a++) {
+          double g = 0.0;
+          for (int e = 0; e < nv; e++) g -= L1(k, e) * (2.0 * TAU(i, j, e, a) - TAU(i, j, a, e));
+          for (int e = 0; e < nv; e++) g -= T1(i, e) * L2(j, k, a, e);
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++)
+              for (int f = 0; f < nv; f++)
+                g -= L2(k, m, e, f) * (2.0 * (T1(j, a) * T2(i, m, e, f) + T1(i, e) * T2(j, m, a, f)) -
+                                       (T1(i, a) * T2(j, m, e, f) + T1(j, e) * T2(i, m, a, f)) -
+                                       (T1(m, a) * T2(i, j, e, f) + T1(i, e) * T2(m, j, a, f) +
+                                        T1(j, f) * T2(i, m, e, a)));
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++)
+              for (int f = 0; f < nv; f++) g += L2(k, m, e, f) * T1(m, a) * T1(i, e) * T1(j, f);
+          Gooov[Q4(i, j, k, a, no, no, nv)] = g;
+          Eooov += INT(i, j, k, a + no) * g;
+        }
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++)
+      for (int c = 0; c < nv; c++)
+        for (int i = 0; i < no; i++) {
+          double g = 0.0;
+          for (int m = 0; m < no; m++) g += L1(m, c) * (2.0 * TAU(m, i, a, b) - TAU(i, m, a, b));
+          for (int m = 0; m < no; m++) g += T1(m, a) * L2(i, m, b, c);
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++)
+              for (int e = 0; e < nv; e++)
+                g += L2(n, m, c, e) * (2.0 * (T1(n, a) * T2(i, m, b, e) + T1(i, b) * T2(n, m, a, e)) -
+                                       (T1(n, b) * T2(i, m, a, e) + T1(i, a) * T2(n, m, b, e)) -
+                                       (T1(m, b) * T2(n, i, a, e) + T1(n, a) * T2(m, i, b, e) +
+                                        T1(i, e) * T2(n, m, a, b)));
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++)
+              for (int e = 0; e < nv; e++) g -= L2(n, m, c, e) * T1(m, b) * T1(n, a) * T1(i, e);
+          Gvvvo[Q4(a, b, c, i, nv, nv, no)] = g;
+          Evvvo += INT(a + no, b + no, c + no, i) * g;
+        }
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++)
+      for (int j = 0; j < no; j++)
+        for (int b = 0; b < nv; b++) {
+          double g = -T1(i, a) * L1(j, b);
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++) g -= TAU(m, i, b, e) * L2(j, m, e, a) + T2(i, m, b, e) * L2(m, j, e, a);
+          Govov[Q4(i, a, j, b, nv, no, nv)] = g;
+          Eovov += INT(i, a + no, j, b + no) * g;
+        }
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) {
+          double g = 4.0 * T1(i, a) * L1(j, b) + 4.0 * TAU(i, j, a, b) - 2.0 * TAU(i, j, b, a) + L2(i, j, a, b);
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++)
+              g += 2.0 * L1(m, e) *
+                   (2.0 * T1(i, a) * (2.0 * T2(j, m, b, e) - T2(j, m, e, b)) -
+                    T1(i, b) * (2.0 * T2(j, m, a, e) - T2(j, m, e, a)) -
+                    T1(i, e) * (2.0 * TAU(j, m, b, a) - TAU(j, m, a, b)) -
+                    T1(m, a) * (2.0 * T2(i, j, e, b) - T2(i, j, b, e)));
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++)
+              g += 4.0 * T2(i, m, a, e) * L2(m, j, e, b) - 2.0 * TAU(m, j, b, e) * L2(i, m, a, e);
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++)
+              for (int e = 0; e < nv; e++)
+                for (int f = 0; f < nv; f++) {
+                  const double l = L2(m, n, e, f);
+                  g += 2.0 * l *
+                       (0.5 * (T2(m, n, a, b) * T2(i, j, e, f) + T2(m, i, a, e) * T2(n, j, b, f) +
+                               T2(n, j, a, e) * T2(i, m, f, b)) +
+                        2.0 * (-TAU(i, j, a, e) * T2(m, n, b, f) - TAU(i, m, a, b) * T2(j, n, e, f) -
+                               TAU(m, j, b, e) * T2(i, n, a, f) + T2(i, m, a, e) * T2(j, n, b, f)) -
+                        (-TAU(i, j, b, e) * T2(m, n, a, f) - TAU(i, m, b, a) * T2(j, n, e, f) -
+                         TAU(m, j, a, e) * T2(i, n, b, f) + T2(i, m, b, e) * T2(j, n, a, f)));
+                  g += l * (T1(m, a) * T1(n, b) * T2(i, j, e, f) + T1(i, e) * T1(j, f) * T2(m, n, a, b) +
+                            T1(m, a) * T1(i, e) * T2(n, j, b, f) + T1(n, b) * T1(j, f) * T2(m, i, a, e) +
+                            T1(n, a) * T1(j, e) * T2(i, m, f, b) + T1(i, f) * T1(m, b) * T2(n, j, a, e));
+                  g += l * T1(m, a) * T1(n, b) * T1(i, e) * T1(j, f);
+                }
+          Goovv[T4(i, j, a, b)] = g;
+          Eoovv += 0.5 * INT(i, j, a + no, b + no) * g;
+        }
+#undef T1
+#undef L1
+#undef T2
+#undef TAU
+#undef L2
+  if (E) {
+    E[0] = eone;
+    E[1] = Eoooo + Evvvv + Eooov + Evvvo + Eovov + Eoovv;
+    E[2] = Eoooo; E[3] = Evvvv; E[4] = Eooov; E[5] = Evvvo; E[6] = Eovov; E[7] = Eoovv;
+  }
+#define OUT(dst, src, n) if (dst) memcpy(dst, src, (n) * sizeof(double))
+  OUT(Doo_o, Doo, o * o); OUT(Dvv_o, Dvv, v * v); OUT(Dov_o, Dov, o * v); OUT(Dvo_o, Dvo, v * o);
+  OUT(Goooo_o, Goooo, o * o * o * o); OUT(Gvvvv_o, Gvvvv, v * v * v * v); OUT(Gooov_o, Gooov, o * o * o * v);
+  OUT(Gvvvo_o, Gvvvo, v * v * v * o); OUT(Govov_o, Govov, o * v * o * v); OUT(Goovv_o, Goovv, o * o * v * v);
+#undef OUT
+  free(Doo); free(Dvv); free(Dov); free(Dvo); free(Goooo); free(Gvvvv); free(Gooov); free(Gvvvo); free(Govov);
+  free(Goovv);
+}
 #undef Q4

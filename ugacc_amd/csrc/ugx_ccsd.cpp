@@ -133,6 +133,11 @@ struct ugx_ctx {
   // l1 residual and the half residual Z of l2, the update's partial sums
   DBuf lam_l1, lam_l1o, lam_l2, lam_l2o, lam_Goo, lam_Gvv, lam_R1, lam_Z;
   bool have_lambda = false;
+  // CCSD densities (ccdensity.cc): the ten blocks, and the intermediates of
+  // their terms with three or more amplitudes
+  DBuf dn_oo, dn_vv, dn_ov, dn_vo, dn_oooo, dn_vvvv, dn_ooov, dn_vvvo, dn_ovov, dn_oovv;
+  DBuf dx_P, dx_Q, dx_h, dx_k, dx_k2, dx_S, dx_B, dx_C, dx_Y, dx_S1, dx_C1;
+  DBuf dx_R, dx_R2, dx_R3, dx_U, dx_V, dx_W, dx_Y2, dx_Ut2, dx_TU, dx_A1, dx_A2, dx_A3, dx_B1, dx_B2, dx_perm;
   // residual of the last ugx_residual, for ugx_intermediates
   bool have_intermediates = false;
   double last_flops = 0.0;
@@ -1068,6 +1073,193 @@ bool lambda_solve(ugx_ctx *c, const ugx_options &opt, double *pseudo, int *niter
   *niter_out = niter;
   *rms_out = rms;
   return rms < opt.r_convergence;
+}
+
+// ----------------------------------------------------------- densities
+// CCDensity::onepdm and twopdm (ccdensity.cc:89-339) from the amplitudes and
+// the Lambda amplitudes of the last ugx_lambda, in FP64, into the dn_*
+// blocks (the reference's layouts), and the energies they give.  Products of
+// three or more amplitudes go through pair intermediates (at most o^2 v^2
+// elements each):
+//   P[i][k] = sum_mef t2_im^ef l2_km^ef      Q[c][a] = sum_mne l2_nm^ce t2_nm^ae
+//   S[i][j][k][m] = sum_ef t2_ij^ef l2_km^ef B[k][m][i][f] = sum_e l2_km^ef t1_ie
+//   C[k][m][i][j] = sum_f B[k][m][i][f] t1_jf
+//   R[k][e][j][a]  = sum_mf l2_km^ef t2_jm^af   R2[k][e][j][a] = sum_mf l2_km^ef t2_mj^af
+//   R3[k][f][i][a] = sum_me l2_km^ef t2_im^ea
+//   U[n][c][i][b] = sum_me l2_nm^ce t2_im^be  V[m][c][i][a] = sum_ne l2_nm^ce t2_ni^ae
+//   W[n][c][i][b] = sum_me l2_nm^ce t2_mi^be  Y[n][m][c][i] = sum_e l2_nm^ce t1_ie
+//   Y2[n][b][c][i] = sum_m t1_mb Y[n][m][c][i]
+//   h[j][b] = sum_me (2 t2_jm^be - t2_jm^eb) l1_me   k[i][m] = sum_e t1_ie l1_me
+//   k2[e][a] = sum_m l1_me t1_ma
+//   A1[i][a][n][f] = sum_me t2_mi^ae l2_mn^ef  A2[j][a][m][f] = sum_ne t2_nj^ae l2_mn^ef
+//   A3[i][a][m][e] = sum_nf t2_in^af l2_mn^ef
+//   S1[i][j][a][n] = sum_m S[i][j][m][n] t1_ma  B1[a][n][i][f] = sum_m t1_ma B[m][n][i][f]
+//   B2[m][a][j][f] = sum_n t1_na B[m][n][j][f]  C1[a][n][i][j] = sum_m t1_ma C[m][n][i][j]
+// The (T) parts that CCSD_T adds first (ccdensity.cc:43-74) are not built.
+void density(ugx_ctx *c) {
+  Engine &E = *c->eng;
+  const int o = c->no, v = c->nv;
+  const int P = PREC_FP64;
+  const long oo = (long)o * o, ov = (long)o * v, vv = (long)v * v;
+  for (auto q : {std::make_pair(&c->dn_oo, oo), std::make_pair(&c->dn_vv, vv), std::make_pair(&c->dn_ov, ov),
+                 std::make_pair(&c->dn_vo, ov), std::make_pair(&c->dn_oooo, oo * oo),
+                 std::make_pair(&c->dn_vvvv, vv * vv), std::make_pair(&c->dn_ooov, oo * ov),
+                 std::make_pair(&c->dn_vvvo, vv * ov), std::make_pair(&c->dn_ovov, ov * ov),
+                 std::make_pair(&c->dn_oovv, ov * ov), std::make_pair(&c->dx_P, oo), std::make_pair(&c->dx_Q, vv),
+                 std::make_pair(&c->dx_h, ov), std::make_pair(&c->dx_k, oo), std::make_pair(&c->dx_k2, vv),
+                 std::make_pair(&c->dx_S, oo * oo), std::make_pair(&c->dx_B, oo * ov), std::make_pair(&c->dx_C, oo * oo),
+                 std::make_pair(&c->dx_Y, oo * ov), std::make_pair(&c->dx_S1, oo * ov),
+                 std::make_pair(&c->dx_C1, oo * ov), std::make_pair(&c->dx_R, ov * ov), std::make_pair(&c->dx_R2, ov * ov),
+                 std::make_pair(&c->dx_R3, ov * ov), std::make_pair(&c->dx_U, ov * ov), std::make_pair(&c->dx_V, ov * ov),
+                 std::make_pair(&c->dx_W, ov * ov), std::make_pair(&c->dx_Y2, ov * ov),
+                 std::make_pair(&c->dx_Ut2, ov * ov), std::make_pair(&c->dx_TU, ov * ov),
+                 std::make_pair(&c->dx_A1, ov * ov), std::make_pair(&c->dx_A2, ov * ov),
+                 std::make_pair(&c->dx_A3, ov * ov), std::make_pair(&c->dx_B1, ov * ov),
+                 std::make_pair(&c->dx_B2, ov * ov), std::make_pair(&c->dx_perm, vv * ov)})
+    q.first->alloc((size_t)q.second * 8);
+  launch_tau(o, v, c->t1.d(), c->t2.d(), c->tau.d(), c->ttau.d(), c->st);
+  auto V = [](const DBuf &b, std::vector<int> d) { return vref(b.d(), std::move(d)); };
+  auto K = [&](const char *spec, double a, const TRef &A, const TRef &B, double b, const TRef &C) {
+    E.contract(spec, a, A, B, b, C, P, "density");
+  };
+  const auto T1 = V(c->t1, {o, v}), T2 = V(c->t2, {o, o, v, v}), TAU = V(c->tau, {o, o, v, v});
+  const auto L1 = V(c->lam_l1, {o, v}), L2 = V(c->lam_l2, {o, o, v, v});
+  const auto DOO = V(c->dn_oo, {o, o}), DVV = V(c->dn_vv, {v, v}), DOV = V(c->dn_ov, {o, v});
+  const auto DVO = V(c->dn_vo, {v, o}), GOOOO = V(c->dn_oooo, {o, o, o, o}), GVVVV = V(c->dn_vvvv, {v, v, v, v});
+  const auto GOOOV = V(c->dn_ooov, {o, o, o, v}), GVVVO = V(c->dn_vvvo, {v, v, v, o});
+  const auto GOVOV = V(c->dn_ovov, {o, v, o, v}), GOOVV = V(c->dn_oovv, {o, o, v, v});
+  const auto XP = V(c->dx_P, {o, o}), XQ = V(c->dx_Q, {v, v}), XH = V(c->dx_h, {o, v});
+  const auto XK = V(c->dx_k, {o, o}), XK2 = V(c->dx_k2, {v, v}), XS = V(c->dx_S, {o, o, o, o});
+  const auto XB = V(c->dx_B, {o, o, o, v}), XC = V(c->dx_C, {o, o, o, o}), XY = V(c->dx_Y, {o, o, v, o});
+  const auto XS1 = V(c->dx_S1, {o, o, v, o}), XC1 = V(c->dx_C1, {v, o, o, o});
+  const auto XR = V(c->dx_R, {o, v, o, v}), XR2 = V(c->dx_R2, {o, v, o, v}), XR3 = V(c->dx_R3, {o, v, o, v});
+  const auto XU = V(c->dx_U, {o, v, o, v}), XV = V(c->dx_V, {o, v, o, v}), XW = V(c->dx_W, {o, v, o, v});
+  const auto XY2 = V(c->dx_Y2, {o, v, v, o}), UT2 = V(c->dx_Ut2, {o, o, v, v}), TU = V(c->dx_TU, {o, o, v, v});
+  const auto XA1 = V(c->dx_A1, {o, v, o, v}), XA2 = V(c->dx_A2, {o, v, o, v}), XA3 = V(c->dx_A3, {o, v, o, v});
+  const auto XB1 = V(c->dx_B1, {v, o, o, v}), XB2 = V(c->dx_B2, {o, v, o, v});
+  // intermediates
+  K("imef,kmef->ik", 1.0, T2, L2, 0.0, XP);
+  K("nmce,nmae->ca", 1.0, L2, T2, 0.0, XQ);
+  K("ijef,kmef->ijkm", 1.0, T2, L2, 0.0, XS);
+  K("kmef,ie->kmif", 1.0, L2, T1, 0.0, XB);
+  K("kmif,jf->kmij", 1.0, XB, T1, 0.0, XC);
+  K("kmef,jmaf->keja", 1.0, L2, T2, 0.0, XR);
+  K("kmef,mjaf->keja", 1.0, L2, T2, 0.0, XR2);
+  K("kmef,imea->kfia", 1.0, L2, T2, 0.0, XR3);
+  K("nmce,imbe->ncib", 1.0, L2, T2, 0.0, XU);
+  K("nmce,niae->mcia", 1.0, L2, T2, 0.0, XV);
+  K("nmce,mibe->ncib", 1.0, L2, T2, 0.0, XW);
+  K("nmce,ie->nmci", 1.0, L2, T1, 0.0, XY);
+  K("mb,nmci->nbci", 1.0, T1, XY, 0.0, XY2);
+  E.permute("jmbe->jmbe", 2.0, T2, 0.0, UT2);
+  E.permute("jmeb->jmbe", -1.0, T2, 1.0, UT2);
+  E.permute("jmba->jmba", 2.0, TAU, 0.0, TU);
+  E.permute("jmab->jmba", -1.0, TAU, 1.0, TU);
+  K("jmbe,me->jb", 1.0, UT2, L1, 0.0, XH);
+  K("ie,me->im", 1.0, T1, L1, 0.0, XK);
+  K("me,ma->ea", 1.0, L1, T1, 0.0, XK2);
+  K("miae,mnef->ianf", 1.0, T2, L2, 0.0, XA1);
+  K("njae,mnef->jamf", 1.0, T2, L2, 0.0, XA2);
+  K("inaf,mnef->iame", 1.0, T2, L2, 0.0, XA3);
+  K("ijmn,ma->ijan", 1.0, XS, T1, 0.0, XS1);
+  K("ma,mnif->anif", 1.0, T1, XB, 0.0, XB1);
+  K("na,mnjf->majf", 1.0, T1, XB, 0.0, XB2);
+  K("ma,mnij->anij", 1.0, T1, XC, 0.0, XC1);
+  // onepdm (ccdensity.cc:89-137)
+  K("ie,je->ij", -1.0, T1, L1, 0.0, DOO);
+  K("imef,jmef->ij", -1.0, T2, L2, 1.0, DOO);
+  K("mb,ma->ab", 1.0, T1, L1, 0.0, DVV);
+  K("mnbe,mnae->ab", 1.0, T2, L2, 1.0, DVV);
+  E.permute("ia->ai", 1.0, L1, 0.0, DVO);
+  E.permute("ia->ia", 2.0, T1, 0.0, DOV);
+  K("me,imae->ia", 2.0, L1, T2, 1.0, DOV);
+  K("me,miae->ia", -1.0, L1, TAU, 1.0, DOV);
+  K("ma,im->ia", -1.0, T1, XP, 1.0, DOV);
+  K("ie,ea->ia", -1.0, T1, XQ, 1.0, DOV);
+  // twopdm (ccdensity.cc:149-339)
+  K("ijef,klef->ijkl", 1.0, TAU, L2, 0.0, GOOOO);
+  K("mnab,mncd->abcd", 1.0, TAU, L2, 0.0, GVVVV);
+  // Gooov[i][j][k][a] (:181-200)
+  K("ke,ijea->ijka", -2.0, L1, TAU, 0.0, GOOOV);
+  K("ke,ijae->ijka", 1.0, L1, TAU, 1.0, GOOOV);
+  K("ie,jkae->ijka", -1.0, T1, L2, 1.0, GOOOV);
+  K("ja,ik->ijka", -2.0, T1, XP, 1.0, GOOOV);
+  K("ie,keja->ijka", -2.0, T1, XR, 1.0, GOOOV);
+  K("ia,jk->ijka", 1.0, T1, XP, 1.0, GOOOV);
+  K("je,keia->ijka", 1.0, T1, XR, 1.0, GOOOV);
+  K("ma,ijkm->ijka", 1.0, T1, XS, 1.0, GOOOV);
+  K("ie,keja->ijka", 1.0, T1, XR2, 1.0, GOOOV);
+  K("jf,kfia->ijka", 1.0, T1, XR3, 1.0, GOOOV);
+  K("ma,kmij->ijka", 1.0, T1, XC, 1.0, GOOOV);
+  // Gvvvo[a][b][c][i] (:210-229)
+  K("mc,miab->abci", 2.0, L1, TAU, 0.0, GVVVO);
+  K("mc,imab->abci", -1.0, L1, TAU, 1.0, GVVVO);
+  K("ma,imbc->abci", 1.0, T1, L2, 1.0, GVVVO);
+  K("na,ncib->abci", 2.0, T1, XU, 1.0, GVVVO);
+  K("ib,ca->abci", 2.0, T1, XQ, 1.0, GVVVO);
+  K("nb,ncia->abci", -1.0, T1, XU, 1.0, GVVVO);
+  K("ia,cb->abci", -1.0, T1, XQ, 1.0, GVVVO);
+  K("mb,mcia->abci", -1.0, T1, XV, 1.0, GVVVO);
+  K("na,ncib->abci", -1.0, T1, XW, 1.0, GVVVO);
+  K("nmab,nmci->abci", -1.0, T2, XY, 1.0, GVVVO);
+  K("na,nbci->abci", -1.0, T1, XY2, 1.0, GVVVO);
+  // Govov[i][a][j][b] (:239-246)
+  K("ia,jb->iajb", -1.0, T1, L1, 0.0, GOVOV);
+  K("mibe,jmea->iajb", -1.0, TAU, L2, 1.0, GOVOV);
+  K("imbe,mjea->iajb", -1.0, T2, L2, 1.0, GOVOV);
+  // Goovv[i][j][a][b] (:256-299)
+  K("ia,jb->ijab", 4.0, T1, L1, 0.0, GOOVV);
+  E.permute("ijab->ijab", 4.0, TAU, 1.0, GOOVV);
+  E.permute("ijba->ijab", -2.0, TAU, 1.0, GOOVV);
+  E.permute("ijab->ijab", 1.0, L2, 1.0, GOOVV);
+  K("ia,jb->ijab", 4.0, T1, XH, 1.0, GOOVV);
+  K("ib,ja->ijab", -2.0, T1, XH, 1.0, GOOVV);
+  K("im,jmba->ijab", -2.0, XK, TU, 1.0, GOOVV);
+  K("ea,ijeb->ijab", -2.0, XK2, UT2, 1.0, GOOVV);
+  K("imae,mjeb->ijab", 4.0, T2, L2, 1.0, GOOVV);
+  K("mjbe,imae->ijab", -2.0, TAU, L2, 1.0, GOOVV);
+  K("mnab,ijmn->ijab", 1.0, T2, XS, 1.0, GOOVV);
+  K("ianf,njbf->ijab", 1.0, XA1, T2, 1.0, GOOVV);
+  K("jamf,imfb->ijab", 1.0, XA2, T2, 1.0, GOOVV);
+  K("ijae,eb->ijab", -4.0, TAU, XQ, 1.0, GOOVV);
+  K("imab,jm->ijab", -4.0, TAU, XP, 1.0, GOOVV);
+  K("mjbe,iame->ijab", -4.0, TAU, XA3, 1.0, GOOVV);
+  K("imae,jbme->ijab", 4.0, T2, XA3, 1.0, GOOVV);
+  K("ijbe,ea->ijab", 2.0, TAU, XQ, 1.0, GOOVV);
+  K("imba,jm->ijab", 2.0, TAU, XP, 1.0, GOOVV);
+  K("mjae,ibme->ijab", 2.0, TAU, XA3, 1.0, GOOVV);
+  K("imbe,jame->ijab", -2.0, T2, XA3, 1.0, GOOVV);
+  K("ijan,nb->ijab", 1.0, XS1, T1, 1.0, GOOVV);
+  K("mnab,mnij->ijab", 1.0, T2, XC, 1.0, GOOVV);
+  K("anif,njbf->ijab", 1.0, XB1, T2, 1.0, GOOVV);
+  K("miae,bmje->ijab", 1.0, T2, XB1, 1.0, GOOVV);
+  K("imfb,majf->ijab", 1.0, T2, XB2, 1.0, GOOVV);
+  K("njae,nbie->ijab", 1.0, T2, XB2, 1.0, GOOVV);
+  K("nb,anij->ijab", 1.0, T1, XC1, 1.0, GOOVV);
+}
+
+// sum x.y on the device, summed on the host
+double device_dot(ugx_ctx *c, long n, const double *x, const double *y) {
+  const int np = 64;
+  launch_dot(n, x, y, c->part.d(), np, c->st);
+  return host_sum(c, np);
+}
+
+// The energies of the density blocks (ccdensity.cc:138-146 and the six
+// parts of twopdm): E[0] = E_one, E[1] = E_two, E[2..7] = the OOOO, VVVV,
+// OOOV, VVVO, OVOV, OOVV parts.  <ab|ci> = <ia|bc> = ovvv[i][a][b][c].
+void density_energies(ugx_ctx *c, double *E) {
+  const int o = c->no, v = c->nv;
+  const long oo = (long)o * o, ov = (long)o * v, vv = (long)v * v;
+  E[0] = device_dot(c, oo, c->foo.d(), c->dn_oo.d()) + device_dot(c, vv, c->fvv.d(), c->dn_vv.d());
+  E[2] = 0.5 * device_dot(c, oo * oo, c->oooo.d(), c->dn_oooo.d());
+  E[3] = 0.5 * device_dot(c, vv * vv, c->vvvv.d(), c->dn_vvvv.d());
+  E[4] = device_dot(c, oo * ov, c->ooov.d(), c->dn_ooov.d());
+  c->eng->permute("abci->iabc", 1.0, vref(c->dn_vvvo.d(), {v, v, v, o}), 0.0, vref(c->dx_perm.d(), {o, v, v, v}));
+  E[5] = device_dot(c, vv * ov, c->ovvv.d(), c->dx_perm.d());
+  E[6] = device_dot(c, ov * ov, c->ovov.d(), c->dn_ovov.d());
+  E[7] = 0.5 * device_dot(c, ov * ov, c->oovv.d(), c->dn_oovv.d());
+  E[1] = E[2] + E[3] + E[4] + E[5] + E[6] + E[7];
 }
 
 // ------------------------------------------------------------------ (T)
@@ -2258,6 +2450,35 @@ int ugx_lambda_residual(ugx_ctx *c, const double *l1, const double *l2, double *
     }
     if (Goo) check(hipMemcpyAsync(Goo, c->lam_Goo.p, (size_t)c->no * c->no * 8, hipMemcpyDeviceToHost, c->st), "Goo");
     if (Gvv) check(hipMemcpyAsync(Gvv, c->lam_Gvv.p, (size_t)c->nv * c->nv * 8, hipMemcpyDeviceToHost, c->st), "Gvv");
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+int ugx_density(ugx_ctx *c, double *energies, double *Doo, double *Dvv, double *Dov, double *Dvo, double *Goooo,
+                double *Gvvvv, double *Gooov, double *Gvvvo, double *Govov, double *Goovv) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!c->have_lambda) throw UgxError(UGX_NOT_READY, "ugx_density: no Lambda amplitudes (run ugx_lambda)");
+    density(c);
+    double E[8];
+    density_energies(c, E);
+    if (energies)
+      for (int k = 0; k < 8; k++) energies[k] = E[k];
+    const long o = c->no, v = c->nv;
+    auto get = [&](double *dst, const DBuf &src, long n) {
+      if (dst) check(hipMemcpyAsync(dst, src.p, n * 8, hipMemcpyDeviceToHost, c->st), "density copy");
+    };
+    get(Doo, c->dn_oo, o * o);
+    get(Dvv, c->dn_vv, v * v);
+    get(Dov, c->dn_ov, o * v);
+    get(Dvo, c->dn_vo, v * o);
+    get(Goooo, c->dn_oooo, o * o * o * o);
+    get(Gvvvv, c->dn_vvvv, v * v * v * v);
+    get(Gooov, c->dn_ooov, o * o * o * v);
+    get(Gvvvo, c->dn_vvvo, v * v * v * o);
+    get(Govov, c->dn_ovov, o * v * o * v);
+    get(Goovv, c->dn_oovv, o * o * v * v);
     check(hipStreamSynchronize(c->st), "stream sync");
     return UGX_OK;
   });

@@ -26,7 +26,7 @@ EXPORTS = [
     "ugx_default_options", "ugx_create", "ugx_destroy", "ugx_last_error", "ugx_load_hamiltonian",
     "ugx_set_amplitudes", "ugx_prepare", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
     "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_hbar", "ugx_lambda",
-    "ugx_get_lambda", "ugx_lambda_residual", "ugx_set_profiling",
+    "ugx_get_lambda", "ugx_lambda_residual", "ugx_density", "ugx_set_profiling",
     "ugx_kernel_stats", "ugx_gemm", "ugx_gemm_bf16x3_static", "ugx_synchronize", "ugx_batch_solve", "ugx_set_autotune", "ugx_tile_table",
     "ugx_save_tile_table", "ugx_load_tile_table", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
     "ugx_mlp_last_error", "ugx_mlp_set_params", "ugx_mlp_get_params", "ugx_mlp_set_scaling", "ugx_mlp_forward",
@@ -102,6 +102,7 @@ def load_library(path=LIBPATH):
     L.ugx_lambda.argtypes = [vp, ctypes.POINTER(Options), dp, ctypes.POINTER(ctypes.c_int), dp, dp]
     L.ugx_get_lambda.argtypes = [vp, dp, dp]
     L.ugx_lambda_residual.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.ugx_density.argtypes = [vp] + [dp] * 11
     L.ugx_set_profiling.argtypes = [vp, ip]
     L.ugx_set_profiling.restype = None
     L.ugx_kernel_stats.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
@@ -325,6 +326,23 @@ class Context:
         Goo, Gvv = np.zeros((o, o)), np.zeros((v, v))
         self._check(self.L.ugx_lambda_residual(self.h, _p(l1), _p(l2), _p(R1), _p(R2), _p(Goo), _p(Gvv)))
         return R1, R2, Goo, Gvv
+
+    DENSITY_BLOCKS = ("Doo", "Dvv", "Dov", "Dvo", "Goooo", "Gvvvv", "Gooov", "Gvvvo", "Govov", "Goovv")
+    DENSITY_PARTS = ("oooo", "vvvv", "ooov", "vvvo", "ovov", "oovv")
+
+    def density(self, which=None):
+        """ugx_density: CCSD densities (ccdensity.cc:89-339) of the current
+        amplitudes and the last Lambda solve: e_one, e_two, the six parts of
+        e_two and the blocks named in `which` (default all)."""
+        o, v = self.no, self.nv
+        shapes = dict(Doo=(o, o), Dvv=(v, v), Dov=(o, v), Dvo=(v, o), Goooo=(o, o, o, o), Gvvvv=(v, v, v, v),
+                      Gooov=(o, o, o, v), Gvvvo=(v, v, v, o), Govov=(o, v, o, v), Goovv=(o, o, v, v))
+        which = self.DENSITY_BLOCKS if which is None else which
+        blocks = {k: np.zeros(shapes[k]) for k in which}
+        E = np.zeros(8)
+        self._check(self.L.ugx_density(self.h, _p(E), *[_p(blocks[k]) if k in blocks else None
+                                                        for k in self.DENSITY_BLOCKS]))
+        return dict(e_one=E[0], e_two=E[1], parts=dict(zip(self.DENSITY_PARTS, E[2:])), blocks=blocks)
 
     def set_profiling(self, on=True):
         self.L.ugx_set_profiling(self.h, 1 if on else 0)
