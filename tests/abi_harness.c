@@ -7,8 +7,9 @@
  *   abi_harness solve DIR     CCSD(T) through the C ABI of the problem in
  *                             DIR (sizes.txt: "no nv"; fock.bin and the six
  *                             block files <name>.bin, little-endian FP64 in
- *                             the layouts of ugx.h), R_CONVERGENCE 1e-11;
- *                             one JSON line of the results
+ *                             the layouts of ugx.h), R_CONVERGENCE 1e-11,
+ *                             then Lambda and the density energies; one
+ *                             JSON line per step
  *
  * The tests compare the layout with the ctypes mirror in ugacc_amd/ugx.py
  * and the solve with a golden fixture.
@@ -103,6 +104,16 @@ static int solve(const char *dir) {
   printf("{\"rc\": %d, \"e_mp2\": %.17g, \"e_ccsd\": %.17g, \"e_t\": %.17g, \"niter\": %d, \"rms\": %.17g, "
          "\"device_allocs\": %d}\n",
          rc, res.e_mp2, res.e_ccsd, res.e_t, res.niter, res.rms, res.device_allocs);
+  /* the post-CC steps of DERTYPE=FIRST for WFN=CCSD: Lambda, then the
+   * density energies (ugacc.cc:118-137) */
+  double pseudo = 0.0, lrms = 0.0, energies[8] = {0};
+  int lniter = 0;
+  int rcl = ugx_lambda(ctx, &opt, &pseudo, &lniter, &lrms, NULL);
+  int rcd = rcl == UGX_OK ? ugx_density(ctx, energies, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL)
+                          : rcl;
+  printf("{\"rc_lambda\": %d, \"rc_density\": %d, \"pseudoenergy\": %.17g, \"niter_lambda\": %d, "
+         "\"e_one\": %.17g, \"e_two\": %.17g}\n",
+         rcl, rcd, pseudo, lniter, energies[0], energies[1]);
   /* a failure path through the ABI: MAXITER 1 is reported, not thrown */
   opt.maxiter = 1;
   int rc1 = ugx_ccsd(ctx, &opt, &res);

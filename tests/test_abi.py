@@ -60,9 +60,12 @@ def test_solve_through_c_client_matches_fixture(tmp_path):
     r = subprocess.run([abi_util.harness(), "solve", str(tmp_path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr + r.stdout
     lines = [json.loads(l) for l in r.stdout.splitlines()]
-    res, fail = lines[0], lines[1]
+    res, post, fail = lines[0], lines[1], lines[2]
     assert res["rc"] == ugx.OK and res["device_allocs"] == 0
     assert abs(res["e_mp2"] - d.scalars["e_mp2"]) < 1e-12
     assert abs(res["e_ccsd"] - d.scalars["eccsd"]) < 1e-10
     assert abs(res["e_t"] - d.scalars["e_t_tcorr"]) < 1e-10
     assert fail["rc_maxiter1"] == ugx.NOT_CONVERGED and fail["error"] == "Computation has not converged."
+    # Lambda and the density energy through the C ABI: E_one + E_two = E_CCSD (f_ov = 0)
+    assert post["rc_lambda"] == ugx.OK and post["rc_density"] == ugx.OK and post["niter_lambda"] > 0
+    assert abs(post["e_one"] + post["e_two"] - res["e_ccsd"]) < 1e-9

@@ -11,11 +11,13 @@
  *                    compute_energy steps (ugacc.cc:104-115) on the GPU:
  *                    the six Dirac integral blocks and the active Fock
  *                    matrix are built on the host and handed to ugx_ccsd.
- * Energies are printed in the layout of ccwfn.cc:234-247.  Of the post-CC
- * steps of DERTYPE FIRST / SECOND (ugacc.cc:117-) the library builds HBAR
- * (ugx_hbar) and Lambda (ugx_lambda) but not the densities or the linear
- * response that those DERTYPEs print: such a request fails with a
- * PSIEXCEPTION instead of returning wrong numbers.
+ * Energies are printed in the layout of ccwfn.cc:234-247.  DERTYPE=FIRST
+ * with WFN=CCSD runs the post-CC steps of ugacc.cc:117-141 on the GPU:
+ * Lambda over HBAR (ugx_lambda) and the density energies (ugx_density),
+ * printed as the reference prints them, with CURRENT ENERGY set.  The (T)
+ * densities of CCSD_T (ccwfn.cc:854-1316) and the linear response of
+ * DERTYPE=SECOND (ccpert.cc, ccresp.cc) are not part of this library: such
+ * a request fails with a PSIEXCEPTION instead of returning wrong numbers.
  *
  * Integrals: MintsHelper::mo_eri over the active occupied / virtual AO
  * coefficients gives chemist (pq|rs); the blocks are <pq|rs> = (pr|qs).
@@ -32,6 +34,7 @@
 #include "psi4/liboptions/liboptions.h"
 #include "psi4/libpsi4util/PsiOutStream.h"
 #include "psi4/libpsi4util/exception.h"
+#include "psi4/libpsi4util/process.h"
 #include "psi4/psi4-dec.h"
 
 #include "ugx.h"
@@ -103,9 +106,13 @@ extern "C" PSI_API SharedWavefunction ugacc(SharedWavefunction ref, Options &opt
   if (options.get_str("REFERENCE") != "RHF") throw PSIEXCEPTION("Only for use with RHF references.");
   for (int h = 0; h < ref->nirrep(); h++)
     if (ref->soccpi()[h]) throw PSIEXCEPTION("UGACC is for closed-shell systems only.");
-  if (options.get_str("DERTYPE") != "NONE")
-    throw PSIEXCEPTION("ugacc (MI355X): DERTYPE " + options.get_str("DERTYPE") +
-                       " needs the CC densities (ccdensity.cc), which this library does not build");
+  const std::string dertype = options.get_str("DERTYPE");
+  if (dertype == "SECOND")
+    throw PSIEXCEPTION("ugacc (MI355X): DERTYPE SECOND needs the linear response (ccpert.cc, ccresp.cc), "
+                       "which this library does not build");
+  if (dertype == "FIRST" && options.get_str("WFN") == "CCSD_T")
+    throw PSIEXCEPTION("ugacc (MI355X): DERTYPE FIRST with CCSD_T needs the (T) densities (ccwfn.cc:854-1316), "
+                       "which this library does not build");
 
   // active orbitals (frozen core and frozen virtuals removed), AO basis
   SharedMatrix Co = ref->Ca_subset("AO", "ACTIVE_OCC");
@@ -165,6 +172,35 @@ extern "C" PSI_API SharedWavefunction ugacc(SharedWavefunction ref, Options &opt
   }
   outfile->Printf("\t%d iterations (%d before the FP64 polish), CCSD %.3f s, (T) %.3f s on the GPU\n", res.niter,
                   res.niter_low, res.seconds_ccsd, res.seconds_triples);
+  if (dertype == "NONE") return ref;
+
+  // HBAR, Lambda and the densities (ugacc.cc:118-137), in FP64
+  double pseudo = 0.0, lrms = 0.0;
+  int lniter = 0;
+  std::vector<double> trace((size_t)opt.maxiter + 1, 0.0);
+  ugx_options lopt = opt;
+  lopt.precision = UGX_PREC_FP64;
+  rc = ugx_lambda(ctx.c, &lopt, &pseudo, &lniter, &lrms, trace.data());
+  outfile->Printf("\n\tThe Coupled-Cluster Lambda Iteration:\n");
+  outfile->Printf("\t-------------------------------------\n");
+  outfile->Printf("\t Iter   PseudoEnergy\n");
+  for (int k = 0; k < lniter; k++) outfile->Printf("\t  %3d  %20.15f\n", k, trace[k]);
+  if (rc == UGX_NOT_CONVERGED) throw PSIEXCEPTION("Computation has not converged.");
+  if (rc != UGX_OK) throw PSIEXCEPTION(std::string("ugacc (MI355X): ") + ugx_last_error(ctx.c));
+  double E[8];
+  rc = ugx_density(ctx.c, E, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                   nullptr);
+  if (rc != UGX_OK) throw PSIEXCEPTION(std::string("ugacc (MI355X): ") + ugx_last_error(ctx.c));
+  const char *part[6] = {"OOOO", "VVVV", "OOOV", "VVVO", "OVOV", "OOVV"};
+  for (int k = 0; k < 6; k++) outfile->Printf("\t%s Energy = %20.14f\n", part[k], E[2 + k]);
+  const double eone = E[0], etwo = E[1], ecc = res.e_ccsd;
+  outfile->Printf("\tOne-Electron Energy        = %20.14f\n", eone);
+  outfile->Printf("\tTwo-Electron Energy        = %20.14f\n", etwo);
+  outfile->Printf("\tCCSD Correlation Energy    = %20.14f (from density)\n", eone + etwo);
+  outfile->Printf("\tCCSD Correlation Energy    = %20.14f (from ccwfn)\n", ecc);
+  outfile->Printf("\tCCSD Total Energy          = %20.14f (from density)\n", eone + etwo + eref);
+  outfile->Printf("\tCCSD Total Energy          = %20.14f (from ccwfn)\n", ecc + eref);
+  Process::environment.globals["CURRENT ENERGY"] = ecc + eref;
   return ref;
 }
 
