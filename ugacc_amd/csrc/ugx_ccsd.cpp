@@ -72,7 +72,7 @@ struct ugx_ctx {
   std::vector<double> diis_norm;
   int diis_count = 0, diis_next = 0, diis_nvec = 0;
   // (T)
-  DBuf tP, tQ, tWbuf, ttrip, tptrs, tE, tPart;
+  DBuf tP, tQ, tWbuf, ttrip, tptrs, tptrs2, tE, tPart;
   DBuf tOff;  // blocked-W offset tables of the three (T) launches
   DBuf tOT2, tT1F;  // f_ov energy kernel: (<ij|ab>, t2) and (t1, f_ov) pair arrays
   DBuf tProb, tCounter;  // energy kernel: problem table (one entry), work-queue counter
@@ -80,12 +80,18 @@ struct ugx_ctx {
   // per launch g: the column table of the old orientation, used as the row
   // table of the transposed BF16x3 launch, holds aligned runs of 4
   bool tOff_row4[3] = {false, false, false};
+  bool tOff_merge4 = false;  // the same for the merged launch's tables (rows c, columns b, offsets a)
   int tW_f32 = 0;  // the (T) W buffer holds FP32 values
   DBuf f_tP, f_tQ, f_t2, f_ooov;  // FP32 operand copies for the mixed forms
   // BF16x3 (T): BF16 hi/lo pair operands, all k contiguous
   DBuf b_P, b_Q, b_OVt, b_t2, b_T2t, b_T2tq;
   // the same operands as MFMA panels, for the LDS-copy W3 kernel
   DBuf p_OV, p_t2, p_T2t, p_T2tq;
+  // the big panels again with rows (x, y) at x * VP + y, VP = v rounded up
+  // to 16, so every x-slab starts on a panel block (the merged launch)
+  DBuf q_P, q_Q, q_T2t, q_T2tq;
+  int t_merge = 1;   // BF16x3 W3: the z = c and z = b groups as one launch per (triplet, a)
+  int t_zgroup_m = 2;  // consecutive triplets per group in the merged launch
   // packed symmetric / antisymmetric ladder (launch_ladder_*): V+, V- once
   // per Hamiltonian; S, A and the two GEMM outputs per residual
   bool lad_packed = false;
@@ -1327,6 +1333,23 @@ void triples_prepare(ugx_ctx *c, int prec) {
       launch_pack_panel(c->t2.d(), (long)o * v2, 1, v2, o, v2, o, (unsigned short *)c->p_T2t.p, c->st);
       c->p_T2tq.alloc((size_t)panel_size(o, v2, o) * 4);
       launch_pack_panel2(c->t2.d(), (long)o * v2, 1, v, v, v2, o, v2, o, (unsigned short *)c->p_T2tq.p, c->st);
+      if (c->t_merge) {
+        // padded rows (x, y) -> x VP + y; rows y >= v are zeros
+        const long VP = (v + 15) / 16 * 16, R = (long)v * VP;
+        if (!c->q_P.p) {
+          c->q_P.alloc((size_t)panel_size(o, R, v) * 4);
+          launch_pack_panel_rows(c->tP.d(), v3, v2, v, v, VP, 1, o, R, v, (unsigned short *)c->q_P.p, c->st);
+          c->q_Q.alloc((size_t)panel_size(o, R, v) * 4);
+          launch_pack_panel_rows(c->tQ.d(), v3, v2, v, v, VP, 1, o, R, v, (unsigned short *)c->q_Q.p, c->st);
+        }
+        // T2t[p][x][y][m] = t2[p][m][x][y];  T2tq[p][y][x][m] = t2[p][m][x][y]
+        c->q_T2t.alloc((size_t)panel_size(o, R, o) * 4);
+        launch_pack_panel_rows(c->t2.d(), (long)o * v2, v, 1, v, VP, v2, o, R, o, (unsigned short *)c->q_T2t.p,
+                               c->st);
+        c->q_T2tq.alloc((size_t)panel_size(o, R, o) * 4);
+        launch_pack_panel_rows(c->t2.d(), (long)o * v2, 1, v, v, VP, v2, o, R, o, (unsigned short *)c->q_T2tq.p,
+                               c->st);
+      }
       return;
     }
     // amplitude-dependent pairs, rebuilt every call
@@ -1388,6 +1411,9 @@ void ensure_w_tables(ugx_ctx *c, int T) {
   c->tOff_row4[0] = runs4(g0n, v, g0m, vv);
   c->tOff_row4[1] = runs4(g1n, vv, g1m, v);
   c->tOff_row4[2] = runs4(g2n, vv, g2m, v);
+  c->tOff_merge4 = runs4(g0n, v, g1m, v);
+  for (long x = 0; x < v; x++)
+    if (g2m[x] % 4) c->tOff_merge4 = false;
   check(hipStreamSynchronize(c->st), "stream sync");  // the old tables may still be in use
   c->tOff.alloc(off.size() * 8);
   check(hipMemcpy(c->tOff.p, off.data(), off.size() * 8, hipMemcpyHostToDevice), "W offset tables");
@@ -1591,6 +1617,92 @@ void triples_build_w_pair(ugx_ctx *c, const TEnt *ents, int nb) {
   }
 }
 
+// The z = c and z = b permutation groups as one launch.  For a fixed a both
+// cover the W slab [a][b][c]: group c as rows c (t2 / <qr|mz> slices) times
+// columns (a,b) (rows of the big panels P_i, T2t_i, Q_j, T2tq_j), group b as
+// rows (a,c) (P_i, T2t_i, Q_k, T2tq_k) times columns b (slices).  Taking
+// the rows (a, .) of the big panels as one operand block (the padded q_*
+// panels start every a-slab on a panel block), both become a GEMM with rows
+// c and columns b, eight K segments (2 (v + o) + 2 (v + o) = 768 at
+// (H2O)8) summed in one accumulator, and one write of the slab.  Each
+// (triplet, a) is one sub-entry (GemmDesc::zsub = v) of the triplet's table
+// entry; W's blocked offsets split into c (rows), b (columns) and a (offz).
+// Entries are in the order of the z = c launch (its pair of big panels),
+// groups of zgroup triplets running a by a on one XCD.
+void triples_w_merged(ugx_ctx *c, const TEnt *ents, int nb, const std::vector<int> &pos) {
+  const int o = c->no, v = c->nv;
+  const long v2 = (long)v * v;
+  const long VP = (v + 15) / 16 * 16, R = (long)v * VP;
+  const long pT = panel_size(1, v, v), nT = panel_size(o * o, v, v);     // t2 slices
+  const long pO = panel_size(1, v, o), nO = panel_size(o * o, v, o);     // OV slices
+  const long qP = panel_size(1, R, v), nP = panel_size(o, R, v);         // padded P, Q
+  const long qT = panel_size(1, R, o), nTT = panel_size(o, R, o);        // padded T2t, T2tq
+  const long blk_v = 16L * ((v + 31) / 32 * 32), blk_o = 16L * ((o + 31) / 32 * 32);
+  std::vector<GemmPtrs> tab((size_t)nb);
+  for (int b = 0; b < nb; b++) {
+    const ugx_ctx *src = ents[b].src;
+    const unsigned short *P = (const unsigned short *)src->q_P.p, *Q = (const unsigned short *)src->q_Q.p;
+    const unsigned short *T2 = (const unsigned short *)src->p_t2.p, *OV = (const unsigned short *)src->p_OV.p;
+    const unsigned short *T2t = (const unsigned short *)src->q_T2t.p, *T2q = (const unsigned short *)src->q_T2tq.p;
+    const long i = ents[b].i, j = ents[b].j, k = ents[b].k;
+    double *W = (double *)((char *)c->tWbuf.p + (size_t)b * tw_stride(v, tw_blk(c->tW_f32)) * (c->tW_f32 ? 4 : 8));
+    GemmPtrs &A = tab[pos[b]];
+    // group c: rows c from the slices, columns b from the a-slab of the big panels
+    A.A[0] = T2 + (k * o + j) * pT;   A.B[0] = P + i * qP;
+    A.A[1] = OV + (j * o + k) * pO;   A.B[1] = T2t + i * qT;
+    A.A[2] = T2 + (k * o + i) * pT;   A.B[2] = Q + j * qP;
+    A.A[3] = OV + (i * o + k) * pO;   A.B[3] = T2q + j * qT;
+    // group b: rows c from the a-slab of the big panels, columns b from the slices
+    A.A[4] = P + i * qP;              A.B[4] = T2 + (j * o + k) * pT;
+    A.A[5] = T2t + i * qT;            A.B[5] = OV + (k * o + j) * pO;
+    A.A[6] = Q + k * qP;              A.B[6] = T2 + (j * o + i) * pT;
+    A.A[7] = T2q + k * qT;            A.B[7] = OV + (i * o + j) * pO;
+    A.C = W;
+  }
+  c->tptrs2.alloc(tab.size() * sizeof(GemmPtrs));
+  check(hipMemcpyAsync(c->tptrs2.p, tab.data(), tab.size() * sizeof(GemmPtrs), hipMemcpyHostToDevice, c->st),
+        "pointer table copy");
+  ensure_w_tables(c, tw_blk(c->tW_f32));
+  const long *t = (const long *)c->tOff.p;
+  // the v-entry tables: tw_off_c (rows c), tw_off_b (columns b), tw_off_a (a)
+  const long *offc = t + v2, *offb = offc + v, *offa = offb + v + v2;
+  GemmDesc d{};
+  d.nseg = 8;
+  d.batch = nb;
+  d.vec_ok = 1;
+  for (int s = 0; s < 8; s++) {
+    const bool part = (s % 2 == 0), big_b = (s < 4);
+    GemmSeg &q = d.seg[s];
+    q.K = part ? v : o;
+    q.sign = part ? 1.0 : -1.0;
+    q.sak = 1; q.sbk = 1;
+    q.sam = BK_PANEL; q.sbn = BK_PANEL;
+    const long lo_small = part ? nT : nO, lo_big = part ? nP : nTT, blk = part ? blk_v : blk_o;
+    q.pblkA = blk; q.pblkB = blk;
+    const long slab = VP / 16 * blk;  // one a-slab of a big panel, elements
+    if (big_b) { q.loA = lo_small; q.loB = lo_big; q.subB = slab; }
+    else { q.loA = lo_big; q.loB = lo_small; q.subA = slab; }
+  }
+  d.M = v;
+  d.N = v;
+  d.alpha = 1.0;
+  d.beta = 0.0;
+  d.c_f32 = c->tW_f32;
+  d.tag = 1;
+  d.offm = offc;
+  d.offn = offb;
+  d.offz = offa;
+  d.zsub = v;
+  d.row4 = (c->tW_f32 && c->tOff_merge4) ? 1 : 0;
+  d.zgroup = c->t_zgroup_m;
+  d.dma = c->t_dma;
+  const int tile = v <= 32 ? 2 : v <= 64 ? 1 : v <= 128 ? 0 : 11;  // one tile covers the slab
+  const double fl = 8.0 * (double)nb * (double)v2 * v * (double)(v + o);
+  c->eng->timed("triples_w:bf16x3", fl, [&]() {
+    launch_gemm(d, (const GemmPtrs *)c->tptrs2.p, PREC_BF16X3_PRESPLIT, tile, c->st);
+  });
+}
+
 // triples_build_w_pair with every operand an MFMA panel, for the kernel
 // that copies panel blocks straight into LDS (GemmDesc::dma): the same
 // three transposed grouped GEMMs, pointer choices and K segments.
@@ -1654,7 +1766,13 @@ void triples_build_w_dma(ugx_ctx *c, const TEnt *ents, int nb) {
   auto tile_rows_v = [&](int n) { return n <= 32 ? 2 : n <= 64 ? 3 : n <= 80 ? 8 : n <= 128 ? 0 : n <= 160 ? 5 : 0; };
   auto tile_cols_v = [&](int n) { return n <= 32 ? 2 : n <= 64 ? 4 : n <= 80 ? 10 : n <= 128 ? 0 : n <= 160 ? 9 : 0; };
   const double fl = 4.0 * (double)nb * (double)v3 * (double)(v + o);
-  for (int g = 0; g < 3; g++) {
+  // the z = c and z = b groups as one launch (then only z = a accumulates)
+  int g_first = 0;
+  if (c->t_merge && v <= 160) {
+    triples_w_merged(c, ents, nb, pos[0]);
+    g_first = 2;
+  }
+  for (int g = g_first; g < 3; g++) {
     GemmDesc d{};
     d.nseg = 4;
     d.batch = nb;
@@ -1967,6 +2085,8 @@ int ugx_create(ugx_ctx **out, int device) {
   if (const char *e = std::getenv("UGX_T_SORT")) c->t_sort = std::atoi(e);
   if (const char *e = std::getenv("UGX_T_ALIGN")) c->t_align = std::atoi(e);
   if (const char *e = std::getenv("UGX_T_WGIB")) c->t_wbytes = std::max(1L, std::atol(e)) << 30;
+  if (const char *e = std::getenv("UGX_T_MERGE")) c->t_merge = std::atoi(e);
+  if (const char *e = std::getenv("UGX_T_ZGROUP_M")) c->t_zgroup_m = std::atoi(e);
   if (const char *e = std::getenv("UGX_LADDER")) c->lad_full = std::string(e) == "full";
   if (c->t_dma != 0 && c->t_dma != 2 && c->t_dma != 3) c->t_dma = 0;
   // a tuned tile table to start from (once per process)
@@ -2006,6 +2126,7 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
     c->f_tP.release(); c->f_tQ.release(); c->f_ooov.release(); c->f_t2.release();
   c->b_P.release(); c->b_Q.release(); c->b_OVt.release();
   c->p_OV.release(); c->p_t2.release(); c->p_T2t.release(); c->p_T2tq.release();
+  c->q_P.release(); c->q_Q.release(); c->q_T2t.release(); c->q_T2tq.release();
     c->diis_err.clear(); c->diis_amp.clear(); c->diis_nvec = 0;
     c->prepared_for.clear();
     c->no = no; c->nv = nv;

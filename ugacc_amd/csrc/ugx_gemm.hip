@@ -204,6 +204,28 @@ __device__ inline void zgroup_of(int G, int &bx, long &z) {
   z = g * G + r % gz;
 }
 
+// Launches with d.zsub > 0 (grid (tiles * zsub, batch)): block -> (tile,
+// table entry z, sub-entry u).  As in zgroup_of, XCD x takes a contiguous
+// range of work items, here in the order (group of G table entries,
+// sub-entry, tile, entry): the G entries of a group run sub-entry by
+// sub-entry on one XCD, so the operand blocks that depend on the sub-entry
+// (shared by entries with equal table pointers) and the entries' own blocks
+// (used by every sub-entry) are reused through its L2.
+__device__ inline void zsub_of(int G, int zsub, int tiles, int &bx, long &z, int &u) {
+  constexpr long NXCD = 8;
+  const long nb = gridDim.y, total = (long)gridDim.x * nb;
+  const long L = (long)blockIdx.y * gridDim.x + blockIdx.x;
+  const long q = total / NXCD, rem = total % NXCD, x = L % NXCD;
+  const long w = x * q + (x < rem ? x : rem) + L / NXCD;  // work item
+  const long per = (long)G * zsub * tiles;
+  const long g = w / per, r = w % per;
+  const long gz = (nb - g * G < G) ? nb - g * G : G;      // entries in group g
+  const long ut = r / gz;                                 // (sub-entry, tile)
+  u = (int)(ut / tiles);
+  bx = (int)(ut % tiles);
+  z = g * G + r % gz;
+}
+
 // Address parts of C(m, n): C[c_row_off(m) + c_col_off(n)] (an offset table,
 // a two-level split, or a plain stride per dimension).
 __device__ inline long c_row_off(const GemmDesc &d, int m) {
@@ -696,7 +718,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(STAGES == 2 
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   int tm, tn, bx = blockIdx.x;
   long z = blockIdx.y;
-  if (d.zgroup > 1) {
+  int u = 0;  // sub-entry (d.zsub > 0)
+  if (d.zsub > 0) {
+    zsub_of(d.zgroup > 1 ? d.zgroup : 1, d.zsub, tiles_m * tiles_n, bx, z, u);
+    tm = bx % tiles_m;
+    tn = bx / tiles_m;
+  } else if (d.zgroup > 1) {
     zgroup_of(d.zgroup, bx, z);
     tm = bx % tiles_m;
     tn = bx / tiles_m;
@@ -724,7 +751,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(STAGES == 2 
     int k0;
     const int s = S.locate(t, k0);
     if (s != cur_seg) {
-      bA = S.a(d, ptrs, z, s); bB = S.b(d, ptrs, z, s);
+      bA = S.a(d, ptrs, z, s) + (long)u * d.seg[s].subA;
+      bB = S.b(d, ptrs, z, s) + (long)u * d.seg[s].subB;
       loA = d.seg[s].loA; loB = d.seg[s].loB; pbA = d.seg[s].pblkA; pbB = d.seg[s].pblkB;
       cur_seg = s;
     }
@@ -811,6 +839,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(STAGES == 2 
 
   void *Cv = ptrs ? (void *)ptrs[z].C
                   : (d.c_f32 ? (void *)((float *)d.C + z * d.sCz) : (void *)(d.C + z * d.sCz));
+  if (d.zsub > 0) Cv = d.c_f32 ? (void *)((float *)Cv + d.offz[u]) : (void *)((double *)Cv + d.offz[u]);
   long ro[TM][4], co[TN];
 #pragma unroll
   for (int i = 0; i < TM; i++)
@@ -863,8 +892,11 @@ template <typename T, int WGM, int WGN, int TM, int TN>
 void launch_t(const GemmDesc &d, const GemmPtrs *ptrs, int prec, hipStream_t st) {
   constexpr int BM = 16 * WGM * TM, BN = 16 * WGN * TN;
   long tiles = (long)((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
+  if (d.zsub > 0) tiles *= d.zsub;
   if (tiles > 0x7fffffffL || d.batch > 65535) throw std::runtime_error("ugx: GEMM grid too large");
   dim3 grid((unsigned)tiles, (unsigned)d.batch, 1);
+  if (d.zsub > 0 && !(prec == PREC_BF16X3_PRESPLIT && d.dma > 1 && ptrs && d.offz && (2 * (BM + BN) / 16) % 4 == 0))
+    throw std::runtime_error("ugx: sub-entries need the LDS-copy BF16x3 kernel, a pointer table and offz");
   if (prec == PREC_BF16X3_PRESPLIT) {
     // tiles whose block copies split evenly over the 4 waves can take the
     // LDS-copy kernel; the others fall back to the register-staged one
@@ -949,7 +981,7 @@ const char *gemm_tile_name(int tile) { return (tile >= 0 && tile < kNumTiles) ? 
 
 void launch_gemm(const GemmDesc &d, const GemmPtrs *ptrs, int prec, int tile, hipStream_t st) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
-  if (d.nseg < 1 || d.nseg > MAXSEG) throw std::runtime_error("ugx: GEMM needs one to four K segments");
+  if (d.nseg < 1 || d.nseg > MAXSEG) throw std::runtime_error("ugx: GEMM needs one to eight K segments");
   if (tile < 0) tile = pick_tile_impl(d);
   if (prec == PREC_FP64) launch_tile<double>(tile, d, ptrs, prec, st);
   else if (prec == PREC_FP32 || prec == PREC_BF16X3_PRESPLIT) launch_tile<float>(tile, d, ptrs, prec, st);

@@ -21,7 +21,7 @@ enum Prec : int {
   PREC_BF16X3_PRESPLIT = 3,
 };
 
-constexpr int GEMM_MAXSEG = 4;
+constexpr int GEMM_MAXSEG = 8;
 constexpr int BK_PANEL = 32;  // k depth of one MFMA panel slice (GemmSeg::pblkA/B)
 
 // One K segment of a contraction C(m,n) += sign * sum_k A(m,k) B(k,n).
@@ -43,6 +43,8 @@ struct GemmSeg {
   // up to 32; its sam / sbn are then ignored.  0 = row-major with stride
   // sam / sbn.
   long pblkA, pblkB;
+  // GemmDesc::zsub > 0: element offsets of A and B per sub-entry index
+  long subA, subB;
 };
 
 // The 8-k chunk c of panel row rr (0..15) is stored at chunk position
@@ -94,6 +96,14 @@ struct GemmDesc {
   // selects the kernel that copies panel blocks straight into LDS (global
   // loads to LDS, no register staging) with that many LDS stages (2 or 3)
   int dma;
+  // LDS-copy kernel with a pointer table: > 0 expands every table entry z
+  // into zsub sub-entries u = 0..zsub-1, sub-entry u reading segment s's A
+  // at ptrs[z].A[s] + u*subA and B at ptrs[z].B[s] + u*subB and writing C
+  // at ptrs[z].C + offz[u]; the grid is (tiles * zsub, batch).  Groups of
+  // zgroup table entries run sub-entry by sub-entry on one XCD, so the
+  // sub-entry-dependent operand blocks they share come from L2.
+  int zsub;
+  const long *offz;
 };
 
 // Per-batch pointers for grouped launches (used by the (T) build where each
@@ -147,6 +157,10 @@ void launch_pack_panel(const double *in, long sp, long sr, long sk, int np, long
                        hipStream_t st);
 // The same with a two-level row index: row r reads in[p*sp + (r / R2)*sr1
 // + (r % R2)*sr2 + k*sk].
+// launch_pack_panel2 with the rows' inner index r % R2 valid only below
+// R2v (rows at or past it pack as zeros): a row set padded per outer index
+void launch_pack_panel_rows(const double *in, long sp, long sr1, long sr2, long R2v, long R2, long sk, int np,
+                            long R, int K, unsigned short *out, hipStream_t st);
 void launch_pack_panel2(const double *in, long sp, long sr1, long sr2, long R2, long sk, int np, long R, int K,
                         unsigned short *out, hipStream_t st);
 

@@ -438,8 +438,8 @@ __global__ void k_diis_combine(long n1, long n2, VecList vl, int nvec, double *t
 // One thread per panel element: i -> (p, row block, k slice, row, stored
 // 8-k chunk, k in chunk); the stored chunk of row rr holds logical chunk
 // chunk ^ ((rr >> 1) & 3) (the swizzle of panel_chunk).
-__global__ void k_pack_panel(const double *in, long sp, long sr1, long sr2, long R2, long sk, long R, int K, int nrb,
-                             int nks, long n, unsigned short *out) {
+__global__ void k_pack_panel(const double *in, long sp, long sr1, long sr2, long R2, long R2v, long sk, long R, int K,
+                             int nrb, int nks, long n, unsigned short *out) {
   for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
     const int kk = (int)(i & 31), rr = (int)((i >> 5) & 15);
     const long rest = i >> 9;
@@ -448,7 +448,7 @@ __global__ void k_pack_panel(const double *in, long sp, long sr1, long sr2, long
     const long r = rb * 16 + rr;
     const int k = ks * 32 + panel_chunk(rr, kk >> 3) * 8 + (kk & 7);
     float x = 0.0f;
-    if (r < R && k < K) x = (float)in[p * sp + (r / R2) * sr1 + (r % R2) * sr2 + k * sk];
+    if (r < R && k < K && r % R2 < R2v) x = (float)in[p * sp + (r / R2) * sr1 + (r % R2) * sr2 + k * sk];
     const unsigned short h = bf16_rne(x);
     out[i] = h;
     out[n + i] = bf16_rne(x - __uint_as_float(((unsigned int)h) << 16));
@@ -634,13 +634,18 @@ void launch_pack_panel(const double *in, long sp, long sr, long sk, int np, long
   launch_pack_panel2(in, sp, 0, sr, R, sk, np, R, K, out, st);
 }
 
-void launch_pack_panel2(const double *in, long sp, long sr1, long sr2, long R2, long sk, int np, long R, int K,
-                        unsigned short *out, hipStream_t st) {
+void launch_pack_panel_rows(const double *in, long sp, long sr1, long sr2, long R2v, long R2, long sk, int np,
+                            long R, int K, unsigned short *out, hipStream_t st) {
   const long n = panel_size(np, R, K);
   if (n <= 0) return;
-  if (R2 <= 0) throw std::runtime_error("ugx: panel row split must be positive");
-  hipLaunchKernelGGL(k_pack_panel, dim3(nblocks(n)), dim3(NT), 0, st, in, sp, sr1, sr2, R2, sk, R, K,
+  if (R2 <= 0 || R2v <= 0 || R2v > R2) throw std::runtime_error("ugx: bad panel row split");
+  hipLaunchKernelGGL(k_pack_panel, dim3(nblocks(n)), dim3(NT), 0, st, in, sp, sr1, sr2, R2, R2v, sk, R, K,
                      (int)((R + 15) / 16), (K + 31) / 32, n, out);
+}
+
+void launch_pack_panel2(const double *in, long sp, long sr1, long sr2, long R2, long sk, int np, long R, int K,
+                        unsigned short *out, hipStream_t st) {
+  launch_pack_panel_rows(in, sp, sr1, sr2, R2, R2, sk, np, R, K, out, st);
 }
 
 void launch_permute(const PermDesc &p, hipStream_t st) {

@@ -55,6 +55,23 @@ __device__ inline double block_sum(double v) {
   return s;
 }
 
+// A load through the global address space.  The per-problem arrays are
+// reached through pointers read from the problem table, which the compiler
+// cannot place in an address space: plain dereferences become FLAT loads,
+// which count against lgkmcnt as well as vmcnt, so the compute loop's first
+// LDS wait would also wait for the next step's prefetch from HBM.
+// (Moved as a native integer of the same size: a struct copy through the
+// cast pointer would still go through a generic reference.)
+template <typename T>
+__device__ inline T gload(const T *p) {
+  typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "4-, 8- or 16-byte elements");
+  if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) unsigned *)p);
+  else if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) unsigned long long *)p);
+  else return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) u64x2 *)p);
+}
+
 // Block pair index p -> (A, B) with B <= A.
 __device__ inline void pair_of(int p, int &A, int &B) {
   A = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
@@ -139,7 +156,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
   }
   auto qsl = [&](int sl) { return sl == 0 ? Q0 : (sl == 1 ? Q1 : Q2); };
   auto fsl = [&](int sl) { return sl == 0 ? F0 : (sl == 1 ? F1 : F2); };
-  const double eo = eps[i] + eps[j] + eps[k];
+  const double eo = gload(eps + i) + gload(eps + j) + gload(eps + k);
   const double fac = 2.0 - ((i == j) + (i == k) + (j == k));
   const int A0 = Ab * E, B0 = Bb * E;
 
@@ -164,17 +181,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     const int gx = gx0 + lx, gy = gy0 + ly;
     const bool in = gx < nv && gy < nv;
     if constexpr (FOV) {
-      return in ? qsl(s)[gx * v + gy] : OE{};
+      return in ? gload(qsl(s) + gx * v + gy) : OE{};
     } else {
-      return (OE)(in ? osl(s)[gx * v + gy] : 0.0);
+      return (OE)(in ? gload(osl(s) + gx * v + gy) : 0.0);
     }
   };
   // t1 (and f_ov) of slice s at global virtual index gx (< nv)
   auto tget = [&](int s, int gx) {
     if constexpr (FOV) {
-      return fsl(s)[gx];
+      return gload(fsl(s) + gx);
     } else {
-      return t1sl(s)[gx];
+      return gload(t1sl(s) + gx);
     }
   };
   auto tzero = []() { return TE{}; };
@@ -214,7 +231,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     double re = 0.0;
     if (tid < 2 * E) {  // T1 of slice 2 and eps for blocks A, B
       const int bx = tid / E, lx = tid % E, gx = (bx ? B0 : A0) + lx;
-      if (gx < nv) { rt2 = tget(2, gx); re = eps[no + gx]; }
+      if (gx < nv) { rt2 = tget(2, gx); re = gload(eps + no + gx); }
     }
 #pragma unroll
     for (int q = 0; q < PF; q++) {
@@ -278,7 +295,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     rtc = tzero();
     if (tid < 4 * E) {  // T1 of slices 0..2 at block C, then eps
       const int sl = tid / E, lx = tid % E, gx = C0 + lx;
-      if (gx < nv) rtc = sl < 3 ? tget(sl, gx) : tval(eps[no + gx]);
+      if (gx < nv) rtc = sl < 3 ? tget(sl, gx) : tval(gload(eps + no + gx));
     }
   };
   auto commit = [&]() {
