@@ -72,6 +72,8 @@ def _lib(omp=False):
         L.orc_lambda_pseudoenergy.restype = ctypes.c_double
         L.orc_density.argtypes = [vp] + [dp] * 11
         L.orc_lambda_pseudoenergy.argtypes = [vp]
+        L.orc_tgrad.argtypes = [vp] + [dp] * 8
+        L.orc_tgrad_enable.argtypes = [vp, ctypes.c_int]
         _libs[omp] = L
     return _libs[omp]
 
@@ -286,6 +288,24 @@ class Oracle:
         E = np.zeros(8)
         self.L.orc_density(self.h, _p(E), *[_p(blocks[k]) for k in self.DENSITY_BLOCKS])
         return dict(e_one=E[0], e_two=E[1], parts=dict(zip(self.DENSITY_PARTS, E[2:])), blocks=blocks)
+
+    TGRAD_BLOCKS = ("s1", "s2", "Doo", "Dvv", "Dov", "Gooov", "Gvvvo", "Goovv")
+
+    def tgrad(self, enable=True):
+        """CCWfn::tgrad (ccwfn.cc:854-1098) from the current amplitudes: the
+        Lambda sources s1, s2 and the (T) density parts by name.  enable: the
+        following lambda_solve / lambda_residual / density include them
+        (cclambda.cc:181, :243; ccdensity.cc:43-74)."""
+        o, v = self.no, self.nv
+        sh = dict(s1=(o, v), s2=(o, o, v, v), Doo=(o, o), Dvv=(v, v), Dov=(o, v), Gooov=(o, o, o, v),
+                  Gvvvo=(v, v, v, o), Goovv=(o, o, v, v))
+        out = {k: np.zeros(sh[k]) for k in self.TGRAD_BLOCKS}
+        self.L.orc_tgrad(self.h, *[_p(out[k]) for k in self.TGRAD_BLOCKS])
+        self.L.orc_tgrad_enable(self.h, int(enable))
+        return out
+
+    def tgrad_enable(self, on):
+        self.L.orc_tgrad_enable(self.h, int(on))
 
     def residual_rows(self, rows):
         """(R1, R2[:rows]) of the current amplitudes."""

@@ -50,6 +50,11 @@ typedef struct {
    * l1/l2 and their previous values, Goo/Gvv, the half residual Z */
   double *hb[11];
   double *l1, *l1old, *l2, *l2old, *Goo, *Gvv, *lZ;
+  /* (T) gradient terms of CCSD(T) with DERTYPE=FIRST (ccwfn.cc:854-1098):
+   * the Lambda sources s1/s2 and the (T) density parts; tg_on: orc_lambda
+   * and orc_density include them */
+  int tg_on;
+  double *tg_s1, *tg_s2, *tg_Doo, *tg_Dvv, *tg_Dov, *tg_Gooov, *tg_Gvvvo, *tg_Goovv;
 } orc_state;
 
 #define NA (s->nact)
@@ -137,6 +142,8 @@ void orc_destroy(void *p) {
   free(s->t1s); free(s->t2s);
   for (int k = 0; k < 11; k++) free(s->hb[k]);
   free(s->l1); free(s->l1old); free(s->l2); free(s->l2old); free(s->Goo); free(s->Gvv); free(s->lZ);
+  free(s->tg_s1); free(s->tg_s2); free(s->tg_Doo); free(s->tg_Dvv); free(s->tg_Dov);
+  free(s->tg_Gooov); free(s->tg_Gvvvo); free(s->tg_Goovv);
   diis_free(s);
   free(s);
 }
@@ -1151,8 +1158,8 @@ void orc_hbar(void *p, double *Hov, double *Hoo, double *Hvv, double *Hoooo, dou
  * iterated over the HBAR of fixed amplitudes.  The HBAR blocks are built once
  * (orc_lambda_init, from the amplitudes current at that call: the converged
  * t1/t2 after orc_ccsd) in the layouts of orc_hbar.  The CCSD_T/DERTYPE=FIRST
- * terms s1/s2 (cclambda.cc:184, :235) come from the (T) gradient
- * (ccwfn.cc:854-1316), which is not restated: this is the CCSD Lambda. */
+ * terms s1 and s2/2 (cclambda.cc:181, :243) come from orc_tgrad (ccwfn.cc:854-1098)
+ * and are added when orc_tgrad_enable(1) is in effect. */
 enum { HOV, HOO, HVV, HOOOO, HVVVV, HVOVV, HOOOV, HOVVO, HOVOV, HVVVO, HOVOO };
 #define Q4(a, b, c, d, n2, n3, n4) ((((size_t)(a) * (n2) + (b)) * (n3) + (c)) * (n4) + (d))
 
@@ -1228,6 +1235,7 @@ static void lambda_l1(orc_state *s, const double *l1, const double *l2, double *
   for (int i = 0; i < no; i++)
     for (int a = 0; a < nv; a++) {
       double value = 2 * Hov_(i, a);
+      if (s->tg_on) value += s->tg_s1[O2(i, a)];  /* cclambda.cc:181 */
       for (int e = 0; e < nv; e++) value += L1(i, e) * Hvv_(e, a);
       for (int m = 0; m < no; m++) value -= L1(m, a) * Hoo_(i, m);
       for (int m = 0; m < no; m++)
@@ -1256,6 +1264,7 @@ static void lambda_l2(orc_state *s, const double *l1, const double *l2, double *
       for (int a = 0; a < nv; a++)
         for (int b = 0; b < nv; b++) {
           double value = LL(i, j, a + no, b + no);
+          if (s->tg_on) value += 0.5 * s->tg_s2[T4(i, j, a, b)];  /* cclambda.cc:243 */
           value += 2.0 * L1(i, a) * Hov_(j, b) - L1(j, a) * Hov_(i, b);
           for (int e = 0; e < nv; e++) value += L2(i, j, e, b) * Hvv_(e, a);
           for (int m = 0; m < no; m++) value -= L2(m, j, a, b) * Hoo_(i, m);
@@ -1376,9 +1385,9 @@ void orc_set_lambda(void *p, const double *l1, const double *l2) {
  * CCDensity::onepdm / twopdm (ccdensity.cc:89-339): the CCSD one- and
  * two-particle density blocks from the amplitudes (t1, t2, tau as current)
  * and the Lambda amplitudes (l1, l2 of orc_lambda), and the energies they
- * give with the Fock matrix and the integrals.  The (T) parts that CCSD_T
- * copies in first (ccdensity.cc:43-74, from ccwfn.cc:854-1316) are not
- * restated: these are the CCSD densities.
+ * give with the Fock matrix and the integrals.  With orc_tgrad_enable(1) the
+ * blocks start from the (T) parts of orc_tgrad, as CCSD_T copies them in
+ * first (ccdensity.cc:43-74); otherwise these are the CCSD densities.
  *   E[0] = E_one (ccdensity.cc:138-146), E[1] = E_two, E[2..7] = the
  *   OOOO, VVVV, OOOV, VVVO, OVOV, OOVV parts of E_two (the values printed).
  * Blocks (each may be NULL): Doo[i][j] Dvv[a][b] Dov[i][a] Dvo[a][i]
@@ -1402,7 +1411,7 @@ void orc_density(void *p, double *E, double *Doo_o, double *Dvv_o, double *Dov_o
   /* onepdm, ccdensity.cc:89-137 */
   for (int i = 0; i < no; i++)
     for (int j = 0; j < no; j++) {
-      double d = 0.0;
+      double d = s->tg_on ? s->tg_Doo[(size_t)i * no + j] : 0.0;
       for (int e = 0; e < nv; e++) d -= T1(i, e) * L1(j, e);
       for (int m = 0; m < no; m++)
         for (int e = 0; e < nv; e++)
@@ -1411,7 +1420,7 @@ void orc_density(void *p, double *E, double *Doo_o, double *Dvv_o, double *Dov_o
     }
   for (int a = 0; a < nv; a++)
     for (int b = 0; b < nv; b++) {
-      double d = 0.0;
+      double d = s->tg_on ? s->tg_Dvv[(size_t)a * nv + b] : 0.0;
       for (int m = 0; m < no; m++) d += T1(m, b) * L1(m, a);
       for (int m = 0; m < no; m++)
         for (int n = 0; n < no; n++)
@@ -1422,7 +1431,7 @@ void orc_density(void *p, double *E, double *Doo_o, double *Dvv_o, double *Dov_o
     for (int i = 0; i < no; i++) Dvo[(size_t)a * no + i] = L1(i, a);
   for (int i = 0; i < no; i++)
     for (int a = 0; a < nv; a++) {
-      double d = 2.0 * T1(i, a);
+      double d = (s->tg_on ? s->tg_Dov[O2(i, a)] : 0.0) + 2.0 * T1(i, a);
       for (int m = 0; m < no; m++)
         for (int e = 0; e < nv; e++) d += L1(m, e) * (2.0 * T2(i, m, a, e) - TAU(m, i, a, e));
       for (int m = 0; m < no; m++)
@@ -1464,7 +1473,7 @@ void orc_density(void *p, double *E, double *Doo_o, double *Dvv_o, double *Dov_o
     for (int j = 0; j < no; j++)
       for (int k = 0; k < no; k++)
         for (int a = 0; a < nv; a++) {
-          double g = 0.0;
+          double g = s->tg_on ? s->tg_Gooov[Q4(i, j, k, a, no, no, nv)] : 0.0;
           for (int e = 0; e < nv; e++) g -= L1(k, e) * (2.0 * TAU(i, j, e, a) - TAU(i, j, a, e));
           for (int e = 0; e < nv; e++) g -= T1(i, e) * L2(j, k, a, e);
           for (int m = 0; m < no; m++)
@@ -1484,7 +1493,7 @@ void orc_density(void *p, double *E, double *Doo_o, double *Dvv_o, double *Dov_o
     for (int b = 0; b < nv; b++)
       for (int c = 0; c < nv; c++)
         for (int i = 0; i < no; i++) {
-          double g = 0.0;
+          double g = s->tg_on ? s->tg_Gvvvo[Q4(a, b, c, i, nv, nv, no)] : 0.0;
           for (int m = 0; m < no; m++) g += L1(m, c) * (2.0 * TAU(m, i, a, b) - TAU(i, m, a, b));
           for (int m = 0; m < no; m++) g += T1(m, a) * L2(i, m, b, c);
           for (int m = 0; m < no; m++)
@@ -1514,7 +1523,8 @@ void orc_density(void *p, double *E, double *Doo_o, double *Dvv_o, double *Dov_o
     for (int j = 0; j < no; j++)
       for (int a = 0; a < nv; a++)
         for (int b = 0; b < nv; b++) {
-          double g = 4.0 * T1(i, a) * L1(j, b) + 4.0 * TAU(i, j, a, b) - 2.0 * TAU(i, j, b, a) + L2(i, j, a, b);
+          double g = (s->tg_on ? s->tg_Goovv[T4(i, j, a, b)] : 0.0) + 4.0 * T1(i, a) * L1(j, b) +
+                     4.0 * TAU(i, j, a, b) - 2.0 * TAU(i, j, b, a) + L2(i, j, a, b);
           for (int m = 0; m < no; m++)
             for (int e = 0; e < nv; e++)
               g += 2.0 * L1(m, e) *
@@ -1562,5 +1572,234 @@ void orc_density(void *p, double *E, double *Doo_o, double *Dvv_o, double *Dov_o
 #undef OUT
   free(Doo); free(Dvv); free(Dov); free(Dvo); free(Goooo); free(Gvvvv); free(Gooov); free(Gvvvo); free(Govov);
   free(Goovv);
+}
+
+/* ------------------------------------------------ (T) gradient terms
+ * CCWfn::tgrad, ccwfn.cc:854-1098 (the in-core form, OOC=false, which
+ * compute_energy runs for WFN=CCSD_T with DERTYPE=FIRST, :252-255), from the
+ * current t1/t2 with t1s/t2s of build_tstar (:628-645):
+ *   t3 = W3/D (the connected t3 of tcorr, :660-690);
+ *   s1 (T3 --> Lambda1, :874-881);
+ *   s2 = 4 X2 - 2 X2(ab swapped) with X2 = Y2 + Y2^T (T3 --> Lambda2,
+ *        :884-915), then + Y2' + Y2'^T (Lambda3 --> Lambda2, :958-983);
+ *   l3 = the six-permutation sum of l3_tmp (:917-956);
+ *   the (T) density parts Doo, Dvv, Dov, Gooov, Gvvvo, Goovv (:985-1083).
+ * Outputs (each may be NULL) in the layouts of orc_density; the terms are
+ * kept in the state, and orc_tgrad_enable(1) makes orc_lambda add s1 and
+ * s2/2 to the l1/l2 residuals (cclambda.cc:181, :243) and orc_density start
+ * its blocks from the (T) parts (ccdensity.cc:43-74). */
+void orc_tgrad(void *p, double *s1_o, double *s2_o, double *Doo_o, double *Dvv_o, double *Dov_o, double *Gooov_o,
+               double *Gvvvo_o, double *Goovv_o) {
+  orc_state *s = (orc_state *)p;
+  const int no = s->no, nv = s->nv;
+  const size_t o = no, v = nv, vvv = v * v * v, ooo = o * o * o;
+  orc_build_tstar(s);
+  const double *t1 = s->t1, *t2 = s->t2, *t1s = s->t1s, *t2s = s->t2s;
+  double *t3 = zalloc(ooo * vvv), *l3t = zalloc(ooo * vvv), *l3 = zalloc(ooo * vvv);
+#define X6(i, j, k, a, b, c) ((((size_t)(i) * no + (j)) * no + (k)) * vvv + V3(a, b, c))
+#define T3(i, j, k, a, b, c) t3[X6(i, j, k, a, b, c)]
+#define L3(i, j, k, a, b, c) l3[X6(i, j, k, a, b, c)]
+#define T2(i, j, a, b) t2[T4(i, j, a, b)]
+#define T2S(i, j, a, b) t2s[T4(i, j, a, b)]
+#define DEN3(i, j, k, a, b, c) \
+  (F(i, i) + F(j, j) + F(k, k) - F((a) + no, (a) + no) - F((b) + no, (b) + no) - F((c) + no, (c) + no))
+#pragma omp parallel for collapse(3) schedule(static)
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int k = 0; k < no; k++)
+        for (int a = 0; a < nv; a++)
+          for (int b = 0; b < nv; b++)
+            for (int c = 0; c < nv; c++) T3(i, j, k, a, b, c) = w3_elem(s, t2, i, j, k, a, b, c) / DEN3(i, j, k, a, b, c);
+  free(s->tg_s1); free(s->tg_s2); free(s->tg_Doo); free(s->tg_Dvv); free(s->tg_Dov);
+  free(s->tg_Gooov); free(s->tg_Gvvvo); free(s->tg_Goovv);
+  double *s1 = s->tg_s1 = zalloc(o * v), *s2 = s->tg_s2 = zalloc(o * o * v * v);
+  double *Doo = s->tg_Doo = zalloc(o * o), *Dvv = s->tg_Dvv = zalloc(v * v), *Dov = s->tg_Dov = zalloc(o * v);
+  double *Gooov = s->tg_Gooov = zalloc(o * o * o * v), *Gvvvo = s->tg_Gvvvo = zalloc(v * v * v * o);
+  double *Goovv = s->tg_Goovv = zalloc(o * o * v * v);
+  double *Y2 = zalloc(o * o * v * v), *X2 = zalloc(o * o * v * v);
+
+  /* T3 --> Lambda1, :874-881 */
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      double x = 0.0;
+      for (int k = 0; k < no; k++)
+        for (int l = 0; l < no; l++)
+          for (int c = 0; c < nv; c++)
+            for (int d = 0; d < nv; d++)
+              x += 2.0 * (T3(i, k, l, a, c, d) - T3(k, i, l, a, c, d)) * LL(k, l, c + no, d + no);
+      s1[O2(i, a)] += x;
+    }
+  /* T3 --> Lambda2, :884-915 */
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) {
+          double x = 0.0;
+          for (int k = 0; k < no; k++)
+            for (int c = 0; c < nv; c++) x += (T3(i, j, k, a, b, c) - T3(k, j, i, a, b, c)) * F(k, c + no);
+          for (int k = 0; k < no; k++)
+            for (int c = 0; c < nv; c++)
+              for (int l = 0; l < no; l++)
+                x -= (2.0 * T3(j, k, l, b, a, c) - T3(j, l, k, b, a, c) - T3(l, k, j, b, a, c)) * INT(k, l, i, c + no);
+          for (int k = 0; k < no; k++)
+            for (int c = 0; c < nv; c++)
+              for (int d = 0; d < nv; d++)
+                x += (2.0 * T3(j, i, k, b, c, d) - T3(j, k, i, b, c, d) - T3(k, i, j, b, c, d)) *
+                     INT(a + no, k, c + no, d + no);
+          Y2[T4(i, j, a, b)] = x;
+        }
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) X2[T4(i, j, a, b)] += Y2[T4(i, j, a, b)] + Y2[T4(j, i, b, a)];
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) s2[T4(i, j, a, b)] += 4.0 * X2[T4(i, j, a, b)] - 2.0 * X2[T4(i, j, b, a)];
+
+  /* Lambda3 amplitudes, :917-956 */
+#pragma omp parallel for collapse(3) schedule(static)
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int k = 0; k < no; k++)
+        for (int a = 0; a < nv; a++)
+          for (int b = 0; b < nv; b++)
+            for (int c = 0; c < nv; c++) {
+              double x = F(i, a + no) * T2S(j, k, b, c) - F(j, a + no) * T2S(i, k, b, c);
+              x += LL(i, j, a + no, b + no) * t1s[O2(k, c)] - LL(i, j, a + no, c + no) * t1s[O2(k, b)];
+              for (int f = 0; f < nv; f++) {
+                x += LL(f + no, j, a + no, b + no) * T2S(i, k, f, c);
+                x -= INT(k, f + no, a + no, b + no) * T2S(i, j, c, f);
+              }
+              for (int n = 0; n < no; n++) {
+                x -= LL(i, j, a + no, n) * T2S(n, k, b, c);
+                x += INT(k, j, a + no, n) * T2S(n, i, b, c);
+              }
+              l3t[X6(i, j, k, a, b, c)] = x / DEN3(i, j, k, a, b, c);
+            }
+#pragma omp parallel for collapse(3) schedule(static)
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int k = 0; k < no; k++)
+        for (int a = 0; a < nv; a++)
+          for (int b = 0; b < nv; b++)
+            for (int c = 0; c < nv; c++)
+              L3(i, j, k, a, b, c) = l3t[X6(i, j, k, a, b, c)] + l3t[X6(i, k, j, a, c, b)] + l3t[X6(j, i, k, b, a, c)] +
+                                     l3t[X6(j, k, i, b, c, a)] + l3t[X6(k, i, j, c, a, b)] + l3t[X6(k, j, i, c, b, a)];
+
+  /* Lambda3 --> Lambda2, :958-983 */
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) {
+          double y = 0.0;
+          for (int k = 0; k < no; k++)
+            for (int c = 0; c < nv; c++)
+              for (int l = 0; l < no; l++) y -= L3(i, k, l, a, c, b) * INT(c + no, j, k, l);
+          for (int k = 0; k < no; k++)
+            for (int c = 0; c < nv; c++)
+              for (int d = 0; d < nv; d++) y += L3(i, k, j, a, c, d) * INT(c + no, d + no, k, b + no);
+          Y2[T4(i, j, a, b)] = y;
+        }
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) s2[T4(i, j, a, b)] += Y2[T4(i, j, a, b)] + Y2[T4(j, i, b, a)];
+
+  /* (T) density contributions, :985-1083 */
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++) {
+      double d = 0.0;
+      for (int l = 0; l < no; l++)
+        for (int m = 0; m < no; m++)
+          for (int dd = 0; dd < nv; dd++)
+            for (int e = 0; e < nv; e++)
+              for (int f = 0; f < nv; f++) d -= 0.5 * T3(i, l, m, dd, e, f) * L3(j, l, m, dd, e, f);
+      Doo[(size_t)i * no + j] = d;
+    }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++) {
+      double d = 0.0;
+      for (int l = 0; l < no; l++)
+        for (int m = 0; m < no; m++)
+          for (int n = 0; n < no; n++)
+            for (int dd = 0; dd < nv; dd++)
+              for (int e = 0; e < nv; e++) d += 0.5 * T3(l, m, n, b, dd, e) * L3(l, m, n, a, dd, e);
+      Dvv[(size_t)a * nv + b] = d;
+    }
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      double d = 0.0;
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++)
+          for (int e = 0; e < nv; e++)
+            for (int f = 0; f < nv; f++) d += (T3(m, n, i, e, f, a) - T3(m, i, n, e, f, a)) * T2S(m, n, e, f);
+      Dov[O2(i, a)] = d;
+    }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int k = 0; k < no; k++)
+        for (int a = 0; a < nv; a++) {
+          double g = 0.0;
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++)
+              for (int f = 0; f < nv; f++)
+                g -= (4.0 * T2(k, m, e, f) - 2.0 * T2(k, m, f, e)) *
+                     (2.0 * T3(j, i, m, a, e, f) - T3(i, j, m, a, e, f) - T3(m, i, j, a, e, f));
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++)
+              for (int f = 0; f < nv; f++) g -= T2(k, m, e, f) * L3(m, j, i, f, a, e);
+          Gooov[Q4(i, j, k, a, no, no, nv)] = g;
+        }
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++)
+      for (int c = 0; c < nv; c++)
+        for (int i = 0; i < no; i++) {
+          double g = 0.0;
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++)
+              for (int e = 0; e < nv; e++)
+                g += (4.0 * T2(m, n, e, c) - 2.0 * T2(m, n, c, e)) *
+                     (2.0 * T3(n, i, m, a, b, e) - T3(n, i, m, b, a, e) - T3(n, i, m, a, e, b));
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++)
+              for (int e = 0; e < nv; e++) g += T2(m, n, e, c) * L3(n, i, m, a, b, e);
+          Gvvvo[Q4(a, b, c, i, nv, nv, no)] = g;
+        }
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) {
+          double g = 0.0;
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++)
+              g += 4.0 * t1[O2(m, e)] *
+                   (2.0 * (T3(i, j, m, a, b, e) - T3(i, j, m, a, e, b)) - (T3(i, j, m, b, a, e) - T3(i, j, m, b, e, a)));
+          Goovv[T4(i, j, a, b)] = g;
+        }
+#undef X6
+#undef T3
+#undef L3
+#undef T2
+#undef T2S
+#undef DEN3
+  free(t3); free(l3t); free(l3); free(Y2); free(X2);
+#define OUT(dst, src, n) if (dst) memcpy(dst, src, (n) * sizeof(double))
+  OUT(s1_o, s1, o * v); OUT(s2_o, s2, o * o * v * v); OUT(Doo_o, Doo, o * o); OUT(Dvv_o, Dvv, v * v);
+  OUT(Dov_o, Dov, o * v); OUT(Gooov_o, Gooov, o * o * o * v); OUT(Gvvvo_o, Gvvvo, v * v * v * o);
+  OUT(Goovv_o, Goovv, o * o * v * v);
+#undef OUT
+}
+
+/* 1: orc_lambda and orc_density include the terms of the last orc_tgrad */
+void orc_tgrad_enable(void *p, int on) {
+  orc_state *s = (orc_state *)p;
+  s->tg_on = on && s->tg_s1;
 }
 #undef Q4
