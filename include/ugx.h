@@ -26,6 +26,10 @@
  *   ugx_intermediates     <- the members Fvv_, Foo_, Fov_, Woooo_, Wovov_,
  *                            Wovvo_, Wovoo_ (ccwfn.h) after build_F/build_W
  *   ugx_w3                <- CCWfn::W3_ijk, ccwfn.cc:1388-1418
+ *   ugx_hbar              <- HBAR::HBAR, hbar.cc:8-228
+ *   ugx_lambda            <- CCLambda::compute_lambda, cclambda.cc:93-135
+ *   ugx_tgrad             <- CCWfn::tgrad, ccwfn.cc:854-1098
+ *   ugx_density           <- CCDensity::onepdm / twopdm, ccdensity.cc:89-339
  *   UGX_NOT_CONVERGED     <- PSIEXCEPTION("Computation has not converged."),
  *                            ccwfn.cc:230-231
  *
@@ -233,9 +237,10 @@ int ugx_hbar(ugx_ctx *ctx, double *Hov, double *Hoo, double *Hvv, double *Hoooo,
  * norm, and trace (maxiter + 1 values): the pseudoenergy of the guess, after
  * each iteration (the values the reference prints) and of the final l2 at
  * index niter.  UGX_NOT_CONVERGED with "Computation has not converged."
- * as the reference's exception (cclambda.cc:133-134).  The (T) terms that
- * CCSD_T with DERTYPE=FIRST adds (s1/s2, cclambda.cc:184, :235) are not
- * included. */
+ * as the reference's exception (cclambda.cc:133-134).  With
+ * opt->wfn_ccsd_t = 1 and ugx_tgrad run on the current amplitudes, the
+ * residuals include the (T) sources s1 and s2/2 that CCSD_T with
+ * DERTYPE=FIRST adds (cclambda.cc:181, :243): the CCSD(T) Lambda. */
 int ugx_lambda(ugx_ctx *ctx, const ugx_options *opt, double *pseudoenergy, int *niter, double *rms,
                double *trace);
 
@@ -250,6 +255,18 @@ int ugx_get_lambda(ugx_ctx *ctx, double *l1, double *l2);
 int ugx_lambda_residual(ugx_ctx *ctx, const double *l1, const double *l2, double *R1, double *R2, double *Goo,
                         double *Gvv);
 
+/* CCWfn::tgrad (ccwfn.cc:854-1098; compute_energy runs it for WFN=CCSD_T
+ * with DERTYPE=FIRST, :252-255): the (T) gradient terms of CCSD(T) from the
+ * context's current amplitudes (after ugx_ccsd), in FP64 on the device, with
+ * t3 and l3 (:917-956) formed one occupied slice at a time (the peak extra
+ * device memory is about 3 no^2 nv^3 doubles).  They stay on the device for
+ * the next ugx_lambda (wfn_ccsd_t = 1) and ugx_density, until the amplitudes
+ * change.  Outputs asked for (non-NULL): the Lambda sources s1[i][a] and
+ * s2[i][j][a][b], and the (T) density parts Doo[i][j], Dvv[a][b], Dov[i][a],
+ * Gooov[i][j][k][a], Gvvvo[a][b][c][i], Goovv[i][j][a][b]. */
+int ugx_tgrad(ugx_ctx *ctx, double *s1, double *s2, double *Doo, double *Dvv, double *Dov, double *Gooov,
+              double *Gvvvo, double *Goovv);
+
 /* CCDensity::onepdm + twopdm (ccdensity.cc:89-339): the CCSD one- and
  * two-particle density blocks, built on the device in FP64 from the
  * context's amplitudes and the Lambda amplitudes of the last ugx_lambda, and
@@ -259,9 +276,11 @@ int ugx_lambda_residual(ugx_ctx *ctx, const double *l1, const double *l2, double
  * (non-NULL) are copied out in the reference's layouts: Doo[i][j], Dvv[a][b],
  * Dov[i][a], Dvo[a][i], Goooo[i][j][k][l], Gvvvv[a][b][c][d],
  * Gooov[i][j][k][a], Gvvvo[a][b][c][i], Govov[i][a][j][b], Goovv[i][j][a][b].
- * At converged amplitudes and multipliers E_one + E_two is the CCSD
- * correlation energy when f_ov = 0 (ugacc.cc:129-137).  The (T) parts of
- * CCSD_T (ccdensity.cc:43-74) are not included. */
+ * When the last ugx_lambda was the CCSD(T) Lambda, the blocks include the
+ * (T) parts of ugx_tgrad, as CCSD_T copies them in first
+ * (ccdensity.cc:43-74).  At converged amplitudes and multipliers E_one +
+ * E_two is the CCSD (or CCSD(T)) correlation energy when f_ov = 0
+ * (ugacc.cc:129-137). */
 int ugx_density(ugx_ctx *ctx, double *energies, double *Doo, double *Dvv, double *Dov, double *Dvo, double *Goooo,
                 double *Gvvvv, double *Gooov, double *Gvvvo, double *Govov, double *Goovv);
 

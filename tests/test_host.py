@@ -57,6 +57,8 @@ def test_plugin_rejects_what_the_reference_rejects():
     r.socc = 2
     with pytest.raises(plugin.PsiException, match="closed-shell systems only"):
         plugin.ugacc(r, opts)
+    with pytest.raises(plugin.PsiException, match="linear response"):
+        plugin.ugacc(_Ref(), dict(opts, DERTYPE="SECOND"))
 
 
 def test_context_without_gpu_reports_no_device():

@@ -144,6 +144,16 @@ struct ugx_ctx {
   DBuf dn_oo, dn_vv, dn_ov, dn_vo, dn_oooo, dn_vvvv, dn_ooov, dn_vvvo, dn_ovov, dn_oovv;
   DBuf dx_P, dx_Q, dx_h, dx_k, dx_k2, dx_S, dx_B, dx_C, dx_Y, dx_S1, dx_C1;
   DBuf dx_R, dx_R2, dx_R3, dx_U, dx_V, dx_W, dx_Y2, dx_Ut2, dx_TU, dx_A1, dx_A2, dx_A3, dx_B1, dx_B2, dx_perm;
+  // (T) gradient terms of CCSD(T) (ccwfn.cc:854-1098): the Lambda sources
+  // s1, s2 and the (T) density parts; tg_valid while they belong to the
+  // current amplitudes; lam_with_t: the last Lambda solve included them (and
+  // the densities then do)
+  DBuf tg_s1, tg_s2, tg_oo, tg_vv, tg_ov, tg_ooov, tg_vvvo, tg_oovv;
+  // one occupied slice of t3 and l3, [j][k][a][b][c]; the accumulators of the
+  // l2 sources by slice, the [i][a][b][c] accumulator of Gvvvo; t1s, t2s;
+  // <zy|an> and L_xyan with the slice index first
+  DBuf tg_T, tg_L, tg_Y2a, tg_Y2b, tg_Y2l, tg_X, tg_Gt, tg_t1s, tg_t2s, tg_ooovT, tg_LooovT;
+  bool tg_valid = false, lam_with_t = false;
   // residual of the last ugx_residual, for ugx_intermediates
   bool have_intermediates = false;
   double last_flops = 0.0;
@@ -975,7 +985,8 @@ void lambda_alloc(ugx_ctx *c) {
 // build_G (cclambda.cc:137-156), build_l1 (:158-210) and build_l2 (:212-268)
 // at (l1, l2): the raw l1 residual into R1 and the half residual Z of l2
 // (R2_ijab = Z_ijab + Z_jiba), Goo/Gvv into lam_Goo/lam_Gvv.
-void lambda_residual(ugx_ctx *c, const double *l1p, const double *l2p, double *R1p, double *Zp) {
+// with_t: the CCSD(T) sources s1, s2/2 of tgrad() enter the residuals
+void lambda_residual(ugx_ctx *c, const double *l1p, const double *l2p, double *R1p, double *Zp, bool with_t) {
   Engine &E = *c->eng;
   const int o = c->no, v = c->nv;
   const int P = PREC_FP64;
@@ -997,6 +1008,7 @@ void lambda_residual(ugx_ctx *c, const double *l1p, const double *l2p, double *R
   K("ijeb,ijab->ae", -1.0, T2, L2, 0.0, GVV);
   // build_l1
   E.permute("ia->ia", 2.0, HOV, 0.0, R1);
+  if (with_t) E.permute("ia->ia", 1.0, V(c->tg_s1.d(), {o, v}), 1.0, R1);  // cclambda.cc:181
   K("ie,ea->ia", 1.0, L1, HVV, 1.0, R1);
   K("ma,im->ia", -1.0, L1, HOO, 1.0, R1);
   K("me,ieam->ia", 2.0, L1, HOVVO, 1.0, R1);
@@ -1009,6 +1021,7 @@ void lambda_residual(ugx_ctx *c, const double *l1p, const double *l2p, double *R
   K("mn,imna->ia", 1.0, GOO, HOOOV, 1.0, R1);
   // build_l2, Z before the pair symmetrization
   E.permute("ijab->ijab", 1.0, LOOVV, 0.0, Z);
+  if (with_t) E.permute("ijab->ijab", 0.5, V(c->tg_s2.d(), {o, o, v, v}), 1.0, Z);  // cclambda.cc:243
   if (!E.sizing()) launch_lambda_outer(o, v, l1p, c->hb_ov.d(), Zp, c->st);
   K("ijeb,ea->ijab", 1.0, L2, HVV, 1.0, Z);
   K("mjab,im->ijab", -1.0, L2, HOO, 1.0, Z);
@@ -1058,7 +1071,7 @@ bool lambda_solve(ugx_ctx *c, const ugx_options &opt, double *pseudo, int *niter
     std::swap(c->lam_l1.bytes, c->lam_l1o.bytes);
     std::swap(c->lam_l2.p, c->lam_l2o.p);
     std::swap(c->lam_l2.bytes, c->lam_l2o.bytes);
-    lambda_residual(c, c->lam_l1o.d(), c->lam_l2o.d(), c->lam_R1.d(), c->lam_Z.d());
+    lambda_residual(c, c->lam_l1o.d(), c->lam_l2o.d(), c->lam_R1.d(), c->lam_Z.d(), c->lam_with_t);
     const int np = update_nparts(o, v);
     launch_update(o, v, c->eps.d(), c->lam_R1.d(), c->lam_Z.d(), c->lam_l1o.d(), c->lam_l2o.d(), c->lam_l1.d(),
                   c->lam_l2.d(), nullptr, c->part.d(), np, c->st);
@@ -1242,6 +1255,205 @@ void density(ugx_ctx *c) {
   K("imfb,majf->ijab", 1.0, T2, XB2, 1.0, GOOVV);
   K("njae,nbie->ijab", 1.0, T2, XB2, 1.0, GOOVV);
   K("nb,anij->ijab", 1.0, T1, XC1, 1.0, GOOVV);
+  if (c->lam_with_t) {
+    // the (T) parts CCSD_T copies in first (ccdensity.cc:43-74)
+    E.permute("ij->ij", 1.0, V(c->tg_oo, {o, o}), 1.0, DOO);
+    E.permute("ab->ab", 1.0, V(c->tg_vv, {v, v}), 1.0, DVV);
+    E.permute("ia->ia", 1.0, V(c->tg_ov, {o, v}), 1.0, DOV);
+    E.permute("ijka->ijka", 1.0, V(c->tg_ooov, {o, o, o, v}), 1.0, GOOOV);
+    E.permute("abci->abci", 1.0, V(c->tg_vvvo, {v, v, v, o}), 1.0, GVVVO);
+    E.permute("ijab->ijab", 1.0, V(c->tg_oovv, {o, o, v, v}), 1.0, GOOVV);
+  }
+}
+
+// CCWfn::tgrad (ccwfn.cc:854-1098): the (T) gradient terms of CCSD(T) from
+// the context's amplitudes, in FP64 on the device.  The reference holds t3
+// and l3 whole (o^3 v^3 each); here they exist one occupied slice at a time,
+// T[j][k][a][b][c] = t3[p][j][k][a][b][c] and the same for l3, and every
+// term is rewritten with the slice index first through the symmetry of t3
+// and l3 under simultaneous permutations of their (occupied, virtual) pairs,
+// t3[x][y][z][a][b][c] = t3[y][x][z][b][a][c] = ...  Each slice is
+//   T = W3 / D with W3 = the twelve terms of ccwfn.cc:664-689 (i = p),
+//   L = (sum over the six pair permutations of the l3_tmp numerator,
+//        ccwfn.cc:923-941) / D  (D is invariant under them),
+// then adds its share of s1 (:874-881), of the T3 and L3 sources of s2
+// (:884-915, :958-983) and of the (T) density parts (:985-1083).  Every
+// term is one FP64 engine contraction; the peak extra memory is three
+// o^2 v^3 arrays (T, L and the engine's permute workspace).
+void tgrad(ugx_ctx *c) {
+  Engine &E = *c->eng;
+  const int o = c->no, v = c->nv;
+  const int PR = PREC_FP64;
+  const long ov = (long)o * v, oovv = ov * ov, v3 = (long)v * v * v, oov = (long)o * o * v;
+  const long s5 = (long)o * o * v3;
+  c->tg_s1.alloc(ov * 8); c->tg_s2.alloc(oovv * 8); c->tg_oo.alloc((size_t)o * o * 8);
+  c->tg_vv.alloc((size_t)v * v * 8); c->tg_ov.alloc(ov * 8); c->tg_ooov.alloc((size_t)o * oov * 8);
+  c->tg_vvvo.alloc((size_t)o * v3 * 8); c->tg_oovv.alloc(oovv * 8);
+  c->tg_T.alloc(s5 * 8); c->tg_L.alloc(s5 * 8);
+  for (DBuf *b : {&c->tg_Y2a, &c->tg_Y2b, &c->tg_Y2l, &c->tg_X, &c->tg_t2s}) b->alloc(oovv * 8);
+  c->tg_Gt.alloc((size_t)o * v3 * 8); c->tg_t1s.alloc(ov * 8);
+  c->tg_ooovT.alloc((size_t)o * oov * 8); c->tg_LooovT.alloc((size_t)o * oov * 8);
+  auto V = [](const DBuf &b, std::vector<int> d) { return vref(b.d(), std::move(d)); };
+  auto VP = [](const double *p, std::vector<int> d) { return vref(const_cast<double *>(p), std::move(d)); };
+  auto K = [&](const std::string &spec, double a, const TRef &A, const TRef &B, double b, const TRef &C) {
+    E.contract(spec, a, A, B, b, C, PR, "tgrad");
+  };
+  // t1s = 2 t1, t2s = 4 t2 - 2 t2^T (build_tstar, ccwfn.cc:628-645)
+  const auto T1 = V(c->t1, {o, v}), T2 = V(c->t2, {o, o, v, v});
+  const auto T1S = V(c->tg_t1s, {o, v}), T2S = V(c->tg_t2s, {o, o, v, v});
+  E.permute("ia->ia", 2.0, T1, 0.0, T1S);
+  E.permute("ijab->ijab", 4.0, T2, 0.0, T2S);
+  E.permute("ijba->ijab", -2.0, T2, 1.0, T2S);
+  // ooovT[z][y][n][a] = ooov[y][z][n][a] = <zy|an>; LooovT[x][y][n][a] =
+  // Looov[y][x][n][a] = L_xyan: the slice index of these terms first
+  E.permute("yzna->zyna", 1.0, cref(c->ooov.d(), {o, o, o, v}), 0.0, V(c->tg_ooovT, {o, o, o, v}));
+  E.permute("yxna->xyna", 1.0, cref(c->Looov.d(), {o, o, o, v}), 0.0, V(c->tg_LooovT, {o, o, o, v}));
+  for (DBuf *b : {&c->tg_s1, &c->tg_oo, &c->tg_vv, &c->tg_ov, &c->tg_ooov, &c->tg_oovv, &c->tg_Y2a, &c->tg_Y2b,
+                  &c->tg_Y2l, &c->tg_Gt})
+    if (!E.sizing()) launch_fill(b->bytes / 8, 0.0, b->d(), c->st);
+  const auto OVVV = cref(c->ovvv.d(), {o, v, v, v}), OOOV = cref(c->ooov.d(), {o, o, o, v});
+  const auto LOOVV = cref(c->Loovv.d(), {o, o, v, v}), LOVVV = cref(c->Lovvv.d(), {o, v, v, v});
+  const auto LOOOV = cref(c->Looov.d(), {o, o, o, v}), FOV = cref(c->fov.d(), {o, v});
+  const auto OOOVT = cref(c->tg_ooovT.d(), {o, o, o, v}), LOOOVT = cref(c->tg_LooovT.d(), {o, o, o, v});
+  const auto TS = V(c->tg_T, {o, o, v, v, v}), LS = V(c->tg_L, {o, o, v, v, v});
+  const auto S1 = V(c->tg_s1, {o, v}), Y2A = V(c->tg_Y2a, {o, o, v, v}), DOO = V(c->tg_oo, {o, o});
+  const auto DVV = V(c->tg_vv, {v, v}), DOV = V(c->tg_ov, {o, v});
+  struct Opd {
+    TRef r;
+    std::string l;
+  };
+  for (int p = 0; p < o; p++) {
+    // slices of the operands at occupied index p
+    const auto ovvv_p = VP(c->ovvv.d() + p * v3, {v, v, v}), lovvv_p = VP(c->Lovvv.d() + p * v3, {v, v, v});
+    const auto ooov_p = VP(c->ooov.d() + p * oov, {o, o, v}), ooovT_p = VP(c->tg_ooovT.d() + p * oov, {o, o, v});
+    const auto looov_p = VP(c->Looov.d() + p * oov, {o, o, v});
+    const auto looovT_p = VP(c->tg_LooovT.d() + p * oov, {o, o, v});
+    const auto t2_p = VP(c->t2.d() + p * ov * v, {o, v, v}), t2s_p = VP(c->tg_t2s.d() + p * ov * v, {o, v, v});
+    const auto loovv_p = VP(c->Loovv.d() + p * ov * v, {o, v, v});
+    const auto fov_p = VP(c->fov.d() + (long)p * v, {v}), t1s_p = VP(c->tg_t1s.d() + (long)p * v, {v});
+    // ---- T = W3[p][j][k][a][b][c] / D (ccwfn.cc:664-689 with i = p)
+    K("eab,kjce->jkabc", 1.0, ovvv_p, T2, 0.0, TS);
+    K("eac,jkbe->jkabc", 1.0, ovvv_p, T2, 1.0, TS);
+    K("keca,jeb->jkabc", 1.0, OVVV, t2_p, 1.0, TS);
+    K("kecb,jae->jkabc", 1.0, OVVV, t2_p, 1.0, TS);
+    K("jebc,kae->jkabc", 1.0, OVVV, t2_p, 1.0, TS);
+    K("jeba,kec->jkabc", 1.0, OVVV, t2_p, 1.0, TS);
+    K("jkmc,mab->jkabc", -1.0, OOOV, t2_p, 1.0, TS);
+    K("kjmb,mac->jkabc", -1.0, OOOV, t2_p, 1.0, TS);
+    K("jmb,kmca->jkabc", -1.0, ooov_p, T2, 1.0, TS);
+    K("jma,kmcb->jkabc", -1.0, ooovT_p, T2, 1.0, TS);
+    K("kma,jmbc->jkabc", -1.0, ooovT_p, T2, 1.0, TS);
+    K("kmc,jmba->jkabc", -1.0, ooov_p, T2, 1.0, TS);
+    if (!E.sizing()) launch_div_d3(o, v, p, c->eps.d(), c->tg_T.d(), c->st);
+    // ---- L = l3[p][j][k][a][b][c]: the l3_tmp numerator N[x][y][z][al][be][ga]
+    // (ccwfn.cc:923-941) summed over the six pair permutations (:947-952),
+    // each operand taken with the slice index P first
+    auto str = [](std::initializer_list<char> cs) { return std::string(cs); };
+    auto pairsym = [&](const TRef &full, const TRef &slice, char y, char z, char b1, char b2) -> Opd {
+      if (y == 'P') return {slice, str({z, b1, b2})};
+      if (z == 'P') return {slice, str({y, b2, b1})};
+      return {full, str({y, z, b1, b2})};
+    };
+    auto row = [&](const TRef &full, const TRef &slice, char x, char a1) -> Opd {
+      if (x == 'P') return {slice, str({a1})};
+      return {full, str({x, a1})};
+    };
+    auto t2s_ = [&](char y, char z, char b1, char b2) { return pairsym(T2S, t2s_p, y, z, b1, b2); };
+    auto loovv_ = [&](char x, char y, char a1, char b1) { return pairsym(LOOVV, loovv_p, x, y, a1, b1); };
+    auto fov_ = [&](char x, char a1) { return row(FOV, fov_p, x, a1); };
+    auto t1s_ = [&](char z, char g) { return row(T1S, t1s_p, z, g); };
+    // L_fyab = Lovvv[y][f][b][a];  <zf|ab> = ovvv[z][f][a][b]
+    auto lvovv_ = [&](char f, char y, char a1, char b1) -> Opd {
+      if (y == 'P') return {lovvv_p, str({f, b1, a1})};
+      return {LOVVV, str({y, f, b1, a1})};
+    };
+    auto ovvv_ = [&](char z, char f, char a1, char b1) -> Opd {
+      if (z == 'P') return {ovvv_p, str({f, a1, b1})};
+      return {OVVV, str({z, f, a1, b1})};
+    };
+    // L_xyan = Looov[y][x][n][a];  <zy|an> = ooov[y][z][n][a]
+    auto loovo_ = [&](char x, char y, char a1, char n) -> Opd {
+      if (y == 'P') return {looov_p, str({x, n, a1})};
+      if (x == 'P') return {looovT_p, str({y, n, a1})};
+      return {LOOOV, str({y, x, n, a1})};
+    };
+    auto oovo_ = [&](char z, char y, char a1, char n) -> Opd {
+      if (y == 'P') return {ooov_p, str({z, n, a1})};
+      if (z == 'P') return {ooovT_p, str({y, n, a1})};
+      return {OOOV, str({y, z, n, a1})};
+    };
+    bool first = true;
+    auto term = [&](double coef, const Opd &A, const Opd &B) {
+      K(A.l + "," + B.l + "->jkabc", coef, A.r, B.r, first ? 0.0 : 1.0, LS);
+      first = false;
+    };
+    const char sig[6][6] = {{'P', 'j', 'k', 'a', 'b', 'c'}, {'P', 'k', 'j', 'a', 'c', 'b'},
+                            {'j', 'P', 'k', 'b', 'a', 'c'}, {'j', 'k', 'P', 'b', 'c', 'a'},
+                            {'k', 'P', 'j', 'c', 'a', 'b'}, {'k', 'j', 'P', 'c', 'b', 'a'}};
+    for (const auto &g : sig) {
+      const char x = g[0], y = g[1], z = g[2], al = g[3], be = g[4], ga = g[5];
+      term(1.0, fov_(x, al), t2s_(y, z, be, ga));
+      term(-1.0, fov_(y, al), t2s_(x, z, be, ga));
+      term(1.0, loovv_(x, y, al, be), t1s_(z, ga));
+      term(-1.0, loovv_(x, y, al, ga), t1s_(z, be));
+      term(1.0, lvovv_('f', y, al, be), t2s_(x, z, 'f', ga));
+      term(-1.0, ovvv_(z, 'f', al, be), t2s_(x, y, ga, 'f'));
+      term(-1.0, loovo_(x, y, al, 'n'), t2s_('n', z, be, ga));
+      term(1.0, oovo_(z, y, al, 'n'), t2s_('n', x, be, ga));
+    }
+    if (!E.sizing()) launch_div_d3(o, v, p, c->eps.d(), c->tg_L.d(), c->st);
+    // ---- s1 (ccwfn.cc:874-881): i = p, then k = p
+    K("klacd,klcd->a", 2.0, TS, LOOVV, 1.0, VP(c->tg_s1.d() + (long)p * v, {v}));
+    K("ilacd,lcd->ia", -2.0, TS, loovv_p, 1.0, S1);
+    // ---- Y2 of the T3 sources (ccwfn.cc:884-904): terms with i = p into
+    // Y2a[i][j][a][b], with k = p into Y2a, with j = p into Y2b[j][i][a][b]
+    const auto y2a_p = VP(c->tg_Y2a.d() + p * ov * v, {o, v, v}), y2b_p = VP(c->tg_Y2b.d() + p * ov * v, {o, v, v});
+    K("jkabc,kc->jab", 1.0, TS, FOV, 1.0, y2a_p);
+    K("jiabc,c->ijab", -1.0, TS, fov_p, 1.0, Y2A);
+    K("klbac,klic->iab", -2.0, TS, OOOV, 1.0, y2b_p);
+    K("lkbac,klic->iab", 1.0, TS, OOOV, 1.0, y2b_p);
+    K("lkcba,klic->iab", 1.0, TS, OOOV, 1.0, y2b_p);
+    K("ikbcd,kadc->iab", 2.0, TS, OVVV, 1.0, y2b_p);
+    K("kibcd,kadc->iab", -1.0, TS, OVVV, 1.0, y2b_p);
+    K("kidbc,kadc->iab", -1.0, TS, OVVV, 1.0, y2b_p);
+    // ---- Y2 of the L3 sources (ccwfn.cc:958-977), i = p
+    const auto y2l_p = VP(c->tg_Y2l.d() + p * ov * v, {o, v, v});
+    K("klacb,lkjc->jab", -1.0, LS, OOOV, 1.0, y2l_p);
+    K("kjacd,kbcd->jab", 1.0, LS, OVVV, 1.0, y2l_p);
+    // ---- (T) density parts (ccwfn.cc:985-1083)
+    K("imedf,jmedf->ij", -0.5, TS, LS, 1.0, DOO);
+    K("mnbde,mnade->ab", 0.5, TS, LS, 1.0, DVV);
+    K("niefa,nef->ia", 1.0, TS, t2s_p, 1.0, DOV);
+    K("inefa,nef->ia", -1.0, TS, t2s_p, 1.0, DOV);
+    const auto g3_p = VP(c->tg_ooov.d() + p * oov, {o, o, v});
+    K("jmeaf,kmef->jka", -2.0, TS, T2S, 1.0, g3_p);
+    K("jmaef,kmef->jka", 1.0, TS, T2S, 1.0, g3_p);
+    K("mjeaf,kmef->jka", 1.0, TS, T2S, 1.0, g3_p);
+    K("mjefa,kmef->jka", -1.0, LS, T2, 1.0, g3_p);
+    const auto gt_p = VP(c->tg_Gt.d() + p * v3, {v, v, v});
+    K("nmbae,mnec->abc", 2.0, TS, T2S, 1.0, gt_p);
+    K("nmabe,mnec->abc", -1.0, TS, T2S, 1.0, gt_p);
+    K("nmeab,mnec->abc", -1.0, TS, T2S, 1.0, gt_p);
+    K("nmbae,mnec->abc", 1.0, LS, T2, 1.0, gt_p);
+    const auto goovv_p = VP(c->tg_oovv.d() + p * ov * v, {o, v, v});
+    K("jmabe,me->jab", 8.0, TS, T1, 1.0, goovv_p);
+    K("jmaeb,me->jab", -8.0, TS, T1, 1.0, goovv_p);
+    K("jmbae,me->jab", -4.0, TS, T1, 1.0, goovv_p);
+    K("jmbea,me->jab", 4.0, TS, T1, 1.0, goovv_p);
+  }
+  // Y2 = Y2a + Y2b^T(ij); X2 = Y2 + Y2^T(pair); s2 = 4 X2 - 2 X2(ab swapped)
+  // (ccwfn.cc:906-915), then + Y2l + Y2l^T(pair) (:979-983)
+  const auto Y2B = V(c->tg_Y2b, {o, o, v, v}), Y2L = V(c->tg_Y2l, {o, o, v, v}), X2 = V(c->tg_X, {o, o, v, v});
+  const auto S2 = V(c->tg_s2, {o, o, v, v});
+  E.permute("jiab->ijab", 1.0, Y2B, 1.0, Y2A);
+  E.permute("ijab->ijab", 1.0, Y2A, 0.0, X2);
+  E.permute("jiba->ijab", 1.0, Y2A, 1.0, X2);
+  E.permute("ijab->ijab", 4.0, X2, 0.0, S2);
+  E.permute("ijba->ijab", -2.0, X2, 1.0, S2);
+  E.permute("ijab->ijab", 1.0, Y2L, 1.0, S2);
+  E.permute("jiba->ijab", 1.0, Y2L, 1.0, S2);
+  // Gvvvo[a][b][c][i] from the [i][a][b][c] accumulator
+  E.permute("iabc->abci", 1.0, V(c->tg_Gt, {o, v, v, v}), 0.0, V(c->tg_vvvo, {v, v, v, o}));
 }
 
 // sum x.y on the device, summed on the host
@@ -2122,6 +2334,7 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
     c->loaded = false;
     c->tables_ready = false;
     c->have_amps = false;
+    c->tg_valid = false;
     c->have_intermediates = false;
     c->f_tP.release(); c->f_tQ.release(); c->f_ooov.release(); c->f_t2.release();
   c->b_P.release(); c->b_Q.release(); c->b_OVt.release();
@@ -2245,6 +2458,7 @@ int ugx_set_amplitudes(ugx_ctx *c, const double *t1, const double *t2) {
     check(hipMemcpyAsync(c->t2.p, t2, c->oovv_n() * 8, hipMemcpyHostToDevice, c->st), "t2 copy");
     check(hipStreamSynchronize(c->st), "stream sync");
     c->have_amps = true;
+    c->tg_valid = false;
     return UGX_OK;
   });
 }
@@ -2372,6 +2586,7 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
     }
     if (res) *res = r;
     c->have_amps = true;
+    c->tg_valid = false;
     if (!conv) {
       c->err = "Computation has not converged.";
       return UGX_NOT_CONVERGED;
@@ -2523,6 +2738,9 @@ int ugx_lambda(ugx_ctx *c, const ugx_options *opt_in, double *pseudoenergy, int 
     if (opt_in) opt = *opt_in;
     else ugx_default_options(&opt);
     validate_options(c, opt, "ugx_lambda");
+    // WFN=CCSD_T with DERTYPE=FIRST adds the (T) sources (cclambda.cc:181,
+    // :243): here, when the caller has run ugx_tgrad on these amplitudes
+    c->lam_with_t = opt.wfn_ccsd_t && c->tg_valid;
     double pe = 0.0, r = 0.0;
     int n = 0;
     const bool conv = lambda_solve(c, opt, &pe, &n, &r, trace);
@@ -2560,7 +2778,7 @@ int ugx_lambda_residual(ugx_ctx *c, const double *l1, const double *l2, double *
     // the arguments go into the "old" buffers: l1/l2 of a finished solve stay
     check(hipMemcpyAsync(c->lam_l1o.p, l1, ov * 8, hipMemcpyHostToDevice, c->st), "l1 copy");
     check(hipMemcpyAsync(c->lam_l2o.p, l2, oovv * 8, hipMemcpyHostToDevice, c->st), "l2 copy");
-    lambda_residual(c, c->lam_l1o.d(), c->lam_l2o.d(), c->lam_R1.d(), c->lam_Z.d());
+    lambda_residual(c, c->lam_l1o.d(), c->lam_l2o.d(), c->lam_R1.d(), c->lam_Z.d(), c->tg_valid);
     if (R1) check(hipMemcpyAsync(R1, c->lam_R1.p, ov * 8, hipMemcpyDeviceToHost, c->st), "R1 copy");
     if (R2) {
       // R2 = Z + Z^T(ij,ab), formed in lam_l2o (its input copy is consumed)
@@ -2571,6 +2789,31 @@ int ugx_lambda_residual(ugx_ctx *c, const double *l1, const double *l2, double *
     }
     if (Goo) check(hipMemcpyAsync(Goo, c->lam_Goo.p, (size_t)c->no * c->no * 8, hipMemcpyDeviceToHost, c->st), "Goo");
     if (Gvv) check(hipMemcpyAsync(Gvv, c->lam_Gvv.p, (size_t)c->nv * c->nv * 8, hipMemcpyDeviceToHost, c->st), "Gvv");
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+int ugx_tgrad(ugx_ctx *c, double *s1, double *s2, double *Doo, double *Dvv, double *Dov, double *Gooov,
+              double *Gvvvo, double *Goovv) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!c->have_amps) throw UgxError(UGX_NOT_READY, "ugx_tgrad: no amplitudes (run ugx_ccsd first)");
+    c->tg_valid = false;
+    tgrad(c);
+    c->tg_valid = true;
+    const long o = c->no, v = c->nv;
+    auto get = [&](double *dst, const DBuf &src, long n) {
+      if (dst) check(hipMemcpyAsync(dst, src.p, n * 8, hipMemcpyDeviceToHost, c->st), "tgrad copy");
+    };
+    get(s1, c->tg_s1, o * v);
+    get(s2, c->tg_s2, o * o * v * v);
+    get(Doo, c->tg_oo, o * o);
+    get(Dvv, c->tg_vv, v * v);
+    get(Dov, c->tg_ov, o * v);
+    get(Gooov, c->tg_ooov, o * o * o * v);
+    get(Gvvvo, c->tg_vvvo, v * v * v * o);
+    get(Goovv, c->tg_oovv, o * o * v * v);
     check(hipStreamSynchronize(c->st), "stream sync");
     return UGX_OK;
   });

@@ -192,6 +192,8 @@ void launch_pairs_unpack(long nrows, int no, const double *ZP, const double *ZM,
 void launch_ladder_unpack(int no, int nv, const double *LP, const double *LM, double scale, double *X,
                           hipStream_t st);
 void launch_axpby(long n, double a, const double *x, double b, double *y, hipStream_t st);  // y = a x + b y
+// x[j][k][a][b][c] /= eps[p] + eps[j] + eps[k] - eps[no+a] - eps[no+b] - eps[no+c]
+void launch_div_d3(int no, int nv, int p, const double *eps, double *x, hipStream_t st);
 void launch_diff(long n, const double *x, const double *y, double *z, hipStream_t st);       // z = x - y
 // tau = t2 + t1 (x) t1, ttau = t2 + 0.5 t1 (x) t1 (ttau may be null)
 void launch_tau(int no, int nv, const double *t1, const double *t2, double *tau, double *ttau, hipStream_t st);

@@ -192,6 +192,23 @@ __global__ void k_axpby(long n, double a, const double *x, double b, double *y) 
     y[i] = a * x[i] + (b != 0.0 ? b * y[i] : 0.0);
 }
 
+// x[j][k][a][b][c] /= eps_p + eps_j + eps_k - eps_a - eps_b - eps_c (virtual
+// energies at no + a, ...): one occupied slice p of a six-index triples
+// amplitude over its denominators
+__global__ void k_div_d3(int no, int nv, int p, const double *__restrict__ eps, double *x) {
+  const long n = (long)no * no * nv * nv * nv;
+  const double ep = eps[p];
+  for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    long r = i;
+    const int c = (int)(r % nv); r /= nv;
+    const int b = (int)(r % nv); r /= nv;
+    const int a = (int)(r % nv); r /= nv;
+    const int k = (int)(r % no);
+    const int j = (int)(r / no);
+    x[i] /= ep + eps[j] + eps[k] - eps[no + a] - eps[no + b] - eps[no + c];
+  }
+}
+
 __global__ void k_diff(long n, const double *__restrict__ x, const double *__restrict__ y, double *__restrict__ z) {
   for (long i = blockIdx.x * (long)NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) z[i] = x[i] - y[i];
 }
@@ -707,6 +724,12 @@ void launch_nonfinite(long n, const double *x, double *flag, hipStream_t st) {
 void launch_axpby(long n, double a, const double *x, double b, double *y, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(NT), 0, st, n, a, x, b, y);
+}
+
+void launch_div_d3(int no, int nv, int p, const double *eps, double *x, hipStream_t st) {
+  const long n = (long)no * no * nv * nv * nv;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_div_d3, dim3(nblocks(n)), dim3(NT), 0, st, no, nv, p, eps, x);
 }
 
 void launch_diff(long n, const double *x, const double *y, double *z, hipStream_t st) {

@@ -6,8 +6,11 @@ read_options() returns the reference's option names and defaults
 same messages (ugacc.cc:95-99), builds the CC wave function and runs
 compute_energy (ugacc.cc:109-117), returning the reference wave function as
 the reference does for DERTYPE=NONE.  CCWfn.compute_energy mirrors
-ccwfn.cc:188-258: iteration table, summary, (T) for WFN=CCSD_T and
-"Computation has not converged." on failure.
+ccwfn.cc:188-258: iteration table, summary, (T) for WFN=CCSD_T (and the
+(T) gradient terms with DERTYPE=FIRST, :252-255) and "Computation has not
+converged." on failure.  DERTYPE=FIRST then runs HBAR, Lambda and the
+densities and prints the density energies (ugacc.cc:118-141), setting
+ref.current_energy; DERTYPE=SECOND (the linear response) is refused.
 
 ``ref`` is any object with the attributes of the front end's
 chem.scf.Reference: no, nv, fock (active Fock), blocks (six Dirac blocks),
@@ -80,6 +83,8 @@ class CCWfn:
             w("\tCCSD(T) Correlation Energy = %20.14f\n" % (r["e_ccsd"] + r["e_t"]))
             w("\tCCSD(T) Total Energy       = %20.14f\n" % (eref + r["e_ccsd"] + r["e_t"]))
             etotal += r["e_t"]
+        if self.wfn == "CCSD_T" and self.options.get("DERTYPE", "NONE") == "FIRST":
+            self.ctx.tgrad(which=())  # ccwfn.cc:252-255
         return etotal
 
     def close(self):
@@ -94,14 +99,36 @@ def ugacc(ref, options):
         raise PsiException("Only for use with RHF references.")
     if getattr(ref, "socc", 0):
         raise PsiException("UGACC is for closed-shell systems only.")
-    if options.get("DERTYPE", "NONE") != "NONE":
-        raise PsiException("DERTYPE=%s needs the post-CC modules, which this build does not provide"
-                           % options["DERTYPE"])
+    dertype = options.get("DERTYPE", "NONE")
+    if dertype not in ("NONE", "FIRST"):
+        raise PsiException("DERTYPE=%s needs the linear response (ccpert.cc, ccresp.cc), which this build "
+                           "does not provide" % dertype)
     cc = CCWfn(ref, options)
     try:
         ecc = cc.compute_energy()
+        ref.cc_energy = ecc
+        ref.cc_result = cc.result
+        if dertype == "FIRST":
+            # HBAR, Lambda (with the (T) sources for CCSD_T) and the densities,
+            # ugacc.cc:118-141
+            lo = cc._opts()
+            lo.precision = ugx.PREC_FP64
+            lr = cc.ctx.lambda_solve(lo)
+            if not lr["converged"]:
+                raise PsiException("Computation has not converged.")
+            d = cc.ctx.density(which=())
+            eone, etwo = d["e_one"], d["e_two"]
+            eref = ref.escf
+            wfn = "CCSD(T)" if cc.wfn == "CCSD_T" else cc.wfn
+            w = cc.out.write
+            w("\tOne-Electron Energy        = %20.14f\n" % eone)
+            w("\tTwo-Electron Energy        = %20.14f\n" % etwo)
+            w("\t%s Correlation Energy    = %20.14f (from density)\n" % (wfn, eone + etwo))
+            w("\t%s Correlation Energy    = %20.14f (from ccwfn)\n" % (wfn, ecc))
+            w("\t%s Total Energy          = %20.14f (from density)\n" % (wfn, eone + etwo + eref))
+            w("\t%s Total Energy          = %20.14f (from ccwfn)\n" % (wfn, ecc + eref))
+            ref.density_energy = eone + etwo
+            ref.current_energy = ecc + eref
     finally:
         cc.close()
-    ref.cc_energy = ecc
-    ref.cc_result = cc.result
     return ref

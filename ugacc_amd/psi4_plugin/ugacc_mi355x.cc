@@ -12,12 +12,14 @@
  *                    the six Dirac integral blocks and the active Fock
  *                    matrix are built on the host and handed to ugx_ccsd.
  * Energies are printed in the layout of ccwfn.cc:234-247.  DERTYPE=FIRST
- * with WFN=CCSD runs the post-CC steps of ugacc.cc:117-141 on the GPU:
- * Lambda over HBAR (ugx_lambda) and the density energies (ugx_density),
- * printed as the reference prints them, with CURRENT ENERGY set.  The (T)
- * densities of CCSD_T (ccwfn.cc:854-1316) and the linear response of
- * DERTYPE=SECOND (ccpert.cc, ccresp.cc) are not part of this library: such
- * a request fails with a PSIEXCEPTION instead of returning wrong numbers.
+ * runs the post-CC steps of ugacc.cc:117-141 on the GPU: for CCSD_T first
+ * the (T) gradient terms (ugx_tgrad, as compute_energy runs tgrad,
+ * ccwfn.cc:252-255), then Lambda over HBAR (ugx_lambda, with the (T) sources
+ * for CCSD_T) and the density energies (ugx_density, with the (T) parts),
+ * printed as the reference prints them, with CURRENT ENERGY set.  The linear
+ * response of DERTYPE=SECOND (ccpert.cc, ccresp.cc) is not part of this
+ * library: such a request fails with a PSIEXCEPTION instead of returning
+ * wrong numbers.
  *
  * Integrals: MintsHelper::mo_eri over the active occupied / virtual AO
  * coefficients gives chemist (pq|rs); the blocks are <pq|rs> = (pr|qs).
@@ -110,9 +112,6 @@ extern "C" PSI_API SharedWavefunction ugacc(SharedWavefunction ref, Options &opt
   if (dertype == "SECOND")
     throw PSIEXCEPTION("ugacc (MI355X): DERTYPE SECOND needs the linear response (ccpert.cc, ccresp.cc), "
                        "which this library does not build");
-  if (dertype == "FIRST" && options.get_str("WFN") == "CCSD_T")
-    throw PSIEXCEPTION("ugacc (MI355X): DERTYPE FIRST with CCSD_T needs the (T) densities (ccwfn.cc:854-1316), "
-                       "which this library does not build");
 
   // active orbitals (frozen core and frozen virtuals removed), AO basis
   SharedMatrix Co = ref->Ca_subset("AO", "ACTIVE_OCC");
@@ -174,7 +173,12 @@ extern "C" PSI_API SharedWavefunction ugacc(SharedWavefunction ref, Options &opt
                   res.niter_low, res.seconds_ccsd, res.seconds_triples);
   if (dertype == "NONE") return ref;
 
-  // HBAR, Lambda and the densities (ugacc.cc:118-137), in FP64
+  // the (T) gradient terms (ccwfn.cc:252-255), then HBAR, Lambda and the
+  // densities (ugacc.cc:118-137), in FP64
+  if (opt.wfn_ccsd_t) {
+    rc = ugx_tgrad(ctx.c, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (rc != UGX_OK) throw PSIEXCEPTION(std::string("ugacc (MI355X): ") + ugx_last_error(ctx.c));
+  }
   double pseudo = 0.0, lrms = 0.0;
   int lniter = 0;
   std::vector<double> trace((size_t)opt.maxiter + 1, 0.0);
@@ -193,13 +197,14 @@ extern "C" PSI_API SharedWavefunction ugacc(SharedWavefunction ref, Options &opt
   if (rc != UGX_OK) throw PSIEXCEPTION(std::string("ugacc (MI355X): ") + ugx_last_error(ctx.c));
   const char *part[6] = {"OOOO", "VVVV", "OOOV", "VVVO", "OVOV", "OOVV"};
   for (int k = 0; k < 6; k++) outfile->Printf("\t%s Energy = %20.14f\n", part[k], E[2 + k]);
-  const double eone = E[0], etwo = E[1], ecc = res.e_ccsd;
+  const double eone = E[0], etwo = E[1], ecc = res.e_ccsd + (opt.wfn_ccsd_t ? res.e_t : 0.0);
+  const char *wfn = opt.wfn_ccsd_t ? "CCSD(T)" : "CCSD";
   outfile->Printf("\tOne-Electron Energy        = %20.14f\n", eone);
   outfile->Printf("\tTwo-Electron Energy        = %20.14f\n", etwo);
-  outfile->Printf("\tCCSD Correlation Energy    = %20.14f (from density)\n", eone + etwo);
-  outfile->Printf("\tCCSD Correlation Energy    = %20.14f (from ccwfn)\n", ecc);
-  outfile->Printf("\tCCSD Total Energy          = %20.14f (from density)\n", eone + etwo + eref);
-  outfile->Printf("\tCCSD Total Energy          = %20.14f (from ccwfn)\n", ecc + eref);
+  outfile->Printf("\t%s Correlation Energy    = %20.14f (from density)\n", wfn, eone + etwo);
+  outfile->Printf("\t%s Correlation Energy    = %20.14f (from ccwfn)\n", wfn, ecc);
+  outfile->Printf("\t%s Total Energy          = %20.14f (from density)\n", wfn, eone + etwo + eref);
+  outfile->Printf("\t%s Total Energy          = %20.14f (from ccwfn)\n", wfn, ecc + eref);
   Process::environment.globals["CURRENT ENERGY"] = ecc + eref;
   return ref;
 }

@@ -26,7 +26,7 @@ EXPORTS = [
     "ugx_default_options", "ugx_create", "ugx_destroy", "ugx_last_error", "ugx_load_hamiltonian",
     "ugx_set_amplitudes", "ugx_prepare", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
     "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_hbar", "ugx_lambda",
-    "ugx_get_lambda", "ugx_lambda_residual", "ugx_density", "ugx_set_profiling",
+    "ugx_get_lambda", "ugx_lambda_residual", "ugx_tgrad", "ugx_density", "ugx_set_profiling",
     "ugx_kernel_stats", "ugx_gemm", "ugx_gemm_bf16x3_static", "ugx_synchronize", "ugx_batch_solve", "ugx_set_autotune", "ugx_tile_table",
     "ugx_save_tile_table", "ugx_load_tile_table", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
     "ugx_mlp_last_error", "ugx_mlp_set_params", "ugx_mlp_get_params", "ugx_mlp_set_scaling", "ugx_mlp_forward",
@@ -102,6 +102,7 @@ def load_library(path=LIBPATH):
     L.ugx_lambda.argtypes = [vp, ctypes.POINTER(Options), dp, ctypes.POINTER(ctypes.c_int), dp, dp]
     L.ugx_get_lambda.argtypes = [vp, dp, dp]
     L.ugx_lambda_residual.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.ugx_tgrad.argtypes = [vp] + [dp] * 8
     L.ugx_density.argtypes = [vp] + [dp] * 11
     L.ugx_set_profiling.argtypes = [vp, ip]
     L.ugx_set_profiling.restype = None
@@ -327,13 +328,30 @@ class Context:
         self._check(self.L.ugx_lambda_residual(self.h, _p(l1), _p(l2), _p(R1), _p(R2), _p(Goo), _p(Gvv)))
         return R1, R2, Goo, Gvv
 
+    TGRAD_BLOCKS = ("s1", "s2", "Doo", "Dvv", "Dov", "Gooov", "Gvvvo", "Goovv")
+
+    def tgrad(self, which=None):
+        """ugx_tgrad: the (T) gradient terms of CCSD(T) (ccwfn.cc:854-1098) of
+        the current amplitudes: the Lambda sources s1, s2 and the (T) density
+        parts named in `which` (default all).  A later lambda_solve with
+        wfn_ccsd_t=1 adds the sources, and the density of that Lambda adds
+        the (T) parts."""
+        o, v = self.no, self.nv
+        shapes = dict(s1=(o, v), s2=(o, o, v, v), Doo=(o, o), Dvv=(v, v), Dov=(o, v), Gooov=(o, o, o, v),
+                      Gvvvo=(v, v, v, o), Goovv=(o, o, v, v))
+        which = self.TGRAD_BLOCKS if which is None else which
+        out = {k: np.zeros(shapes[k]) for k in which}
+        self._check(self.L.ugx_tgrad(self.h, *[_p(out[k]) if k in out else None for k in self.TGRAD_BLOCKS]))
+        return out
+
     DENSITY_BLOCKS = ("Doo", "Dvv", "Dov", "Dvo", "Goooo", "Gvvvv", "Gooov", "Gvvvo", "Govov", "Goovv")
     DENSITY_PARTS = ("oooo", "vvvv", "ooov", "vvvo", "ovov", "oovv")
 
     def density(self, which=None):
         """ugx_density: CCSD densities (ccdensity.cc:89-339) of the current
-        amplitudes and the last Lambda solve: e_one, e_two, the six parts of
-        e_two and the blocks named in `which` (default all)."""
+        amplitudes and the last Lambda solve (with the (T) parts when that
+        solve included the (T) sources): e_one, e_two, the six parts of e_two
+        and the blocks named in `which` (default all)."""
         o, v = self.no, self.nv
         shapes = dict(Doo=(o, o), Dvv=(v, v), Dov=(o, v), Dvo=(v, o), Goooo=(o, o, o, o), Gvvvv=(v, v, v, v),
                       Gooov=(o, o, o, v), Gvvvo=(v, v, v, o), Govov=(o, v, o, v), Goovv=(o, o, v, v))
