@@ -92,6 +92,7 @@ struct ugx_ctx {
   DBuf q_P, q_Q, q_T2t, q_T2tq;
   int t_merge = 1;   // BF16x3 W3: the z = c and z = b groups as one launch per (triplet, a)
   int t_zgroup_m = 2;  // consecutive triplets per group in the merged launch
+  int t_za_first = 0;  // merged W3: the z = a launch writes W first, the merged launch accumulates
   // packed symmetric / antisymmetric ladder (launch_ladder_*): V+, V- once
   // per Hamiltonian; S, A and the two GEMM outputs per residual
   bool lad_packed = false;
@@ -1841,7 +1842,7 @@ void triples_build_w_pair(ugx_ctx *c, const TEnt *ents, int nb) {
 // entry; W's blocked offsets split into c (rows), b (columns) and a (offz).
 // Entries are in the order of the z = c launch (its pair of big panels),
 // groups of zgroup triplets running a by a on one XCD.
-void triples_w_merged(ugx_ctx *c, const TEnt *ents, int nb, const std::vector<int> &pos) {
+void triples_w_merged(ugx_ctx *c, const TEnt *ents, int nb, const std::vector<int> &pos, double beta) {
   const int o = c->no, v = c->nv;
   const long v2 = (long)v * v;
   const long VP = (v + 15) / 16 * 16, R = (long)v * VP;
@@ -1898,7 +1899,7 @@ void triples_w_merged(ugx_ctx *c, const TEnt *ents, int nb, const std::vector<in
   d.M = v;
   d.N = v;
   d.alpha = 1.0;
-  d.beta = 0.0;
+  d.beta = beta;
   d.c_f32 = c->tW_f32;
   d.tag = 1;
   d.offm = offc;
@@ -1978,13 +1979,11 @@ void triples_build_w_dma(ugx_ctx *c, const TEnt *ents, int nb) {
   auto tile_rows_v = [&](int n) { return n <= 32 ? 2 : n <= 64 ? 3 : n <= 80 ? 8 : n <= 128 ? 0 : n <= 160 ? 5 : 0; };
   auto tile_cols_v = [&](int n) { return n <= 32 ? 2 : n <= 64 ? 4 : n <= 80 ? 10 : n <= 128 ? 0 : n <= 160 ? 9 : 0; };
   const double fl = 4.0 * (double)nb * (double)v3 * (double)(v + o);
-  // the z = c and z = b groups as one launch (then only z = a accumulates)
-  int g_first = 0;
-  if (c->t_merge && v <= 160) {
-    triples_w_merged(c, ents, nb, pos[0]);
-    g_first = 2;
-  }
-  for (int g = g_first; g < 3; g++) {
+  // the z = c and z = b groups as one launch, with the z = a launch before
+  // (writing W) or after it (accumulating)
+  const bool merged = c->t_merge && v <= 160, za_first = merged && c->t_za_first;
+  if (merged && !za_first) triples_w_merged(c, ents, nb, pos[0], 0.0);
+  for (int g = merged ? 2 : 0; g < 3; g++) {
     GemmDesc d{};
     d.nseg = 4;
     d.batch = nb;
@@ -2005,7 +2004,7 @@ void triples_build_w_dma(ugx_ctx *c, const TEnt *ents, int nb) {
     if (g == 0) { d.M = v; d.N = (int)v2; tile = tile_cols_v(v); }
     else { d.M = (int)v2; d.N = v; tile = tile_rows_v(v); }
     d.alpha = 1.0;
-    d.beta = (g == 0) ? 0.0 : 1.0;
+    d.beta = (g == 0 || za_first) ? 0.0 : 1.0;
     d.c_f32 = c->tW_f32;
     d.tag = 1;
     set_w_tables(c, d, g, true);
@@ -2017,6 +2016,7 @@ void triples_build_w_dma(ugx_ctx *c, const TEnt *ents, int nb) {
       launch_gemm(d, (const GemmPtrs *)c->tptrs.p + (size_t)g * nb, PREC_BF16X3_PRESPLIT, tile, c->st);
     });
   }
+  if (za_first) triples_w_merged(c, ents, nb, pos[0], 1.0);
 }
 
 // The (T) operands of a context at a precision: the integral tables and
@@ -2298,6 +2298,7 @@ int ugx_create(ugx_ctx **out, int device) {
   if (const char *e = std::getenv("UGX_T_ALIGN")) c->t_align = std::atoi(e);
   if (const char *e = std::getenv("UGX_T_WGIB")) c->t_wbytes = std::max(1L, std::atol(e)) << 30;
   if (const char *e = std::getenv("UGX_T_MERGE")) c->t_merge = std::atoi(e);
+  if (const char *e = std::getenv("UGX_T_ZAFIRST")) c->t_za_first = std::atoi(e);
   if (const char *e = std::getenv("UGX_T_ZGROUP_M")) c->t_zgroup_m = std::atoi(e);
   if (const char *e = std::getenv("UGX_LADDER")) c->lad_full = std::string(e) == "full";
   if (c->t_dma != 0 && c->t_dma != 2 && c->t_dma != 3) c->t_dma = 0;
