@@ -184,17 +184,20 @@ def test_bf16x3_panel_operands_match_pair_arrays(no, nv, monkeypatch):
 @pytest.mark.gpu
 @pytest.mark.parametrize("no,nv,env,val", [(8, 40, "UGX_T_DMA", "3"), (5, 19, "UGX_T_DMA", "0"),
                                            (3, 70, "UGX_T_DMA", "0"), (6, 38, "UGX_T_MERGE", "0"),
-                                           (4, 130, "UGX_T_MERGE", "0")])
+                                           (4, 130, "UGX_T_MERGE", "0"), (5, 38, "UGX_T_ZAFIRST", "0"),
+                                           (4, 152, "UGX_T_ZAFIRST", "0")])
 def test_triples_bf16x3_lds_copy_matches_register_path(no, nv, env, val, monkeypatch):
     """The default BF16x3 W3 build (panel operands copied straight into LDS,
-    two stages, the z = c and z = b permutation groups merged into one
-    launch per (triplet, a)) against the other W3 kernels, selected when a
+    two stages, the z = a group's launch writing W, then the z = c and z = b
+    permutation groups merged into one accumulating launch per (triplet, a))
+    against the other W3 kernels, selected when a
     context is created: UGX_T_DMA=3 (three LDS stages) runs the same MFMA
     products in the same order, so per-triplet energies and W3 agree
     bitwise; UGX_T_DMA=0 (register-staged operands) and UGX_T_MERGE=0 (three
     launches) round the z = c group's FP32 partial sums into W before the
-    z = b group adds to them, so they agree to FP32 rounding (1e-6 of the
-    largest value).  v = 19, 38, 70 and 130 pad the panels' rows and k (and
+    z = b group adds to them, and UGX_T_ZAFIRST=0 (the merged launch writes W,
+    the z = a launch accumulates) rounds in the other order, so they agree to
+    FP32 rounding (1e-6 of the largest value).  v = 19, 38, 70 and 130 pad the panels' rows and k (and
     the merged launch's a-slabs); o = 3 and 5 leave the hole terms' k mostly
     padding."""
     announce("triples bf16x3 default vs %s=%s o=%d v=%d" % (env, val, no, nv))

@@ -91,8 +91,11 @@ struct ugx_ctx {
   // to 16, so every x-slab starts on a panel block (the merged launch)
   DBuf q_P, q_Q, q_T2t, q_T2tq;
   int t_merge = 1;   // BF16x3 W3: the z = c and z = b groups as one launch per (triplet, a)
-  int t_zgroup_m = 2;  // consecutive triplets per group in the merged launch
-  int t_za_first = 0;  // merged W3: the z = a launch writes W first, the merged launch accumulates
+  int t_zgroup_m = 4;  // consecutive triplets per group in the merged launch
+  // merged W3: the z = a launch writes W first and the merged launch (K four
+  // times as long) accumulates, so the read of the old W hides behind more
+  // MFMA work
+  int t_za_first = 1;
   // packed symmetric / antisymmetric ladder (launch_ladder_*): V+, V- once
   // per Hamiltonian; S, A and the two GEMM outputs per residual
   bool lad_packed = false;
