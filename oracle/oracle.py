@@ -73,6 +73,18 @@ def _lib(omp=False):
         L.orc_density.argtypes = [vp] + [dp] * 11
         L.orc_lambda_pseudoenergy.argtypes = [vp]
         L.orc_tgrad.argtypes = [vp] + [dp] * 8
+        L.orc_pert_create.restype = vp
+        L.orc_pert_create.argtypes = [vp, dp, ctypes.c_double]
+        L.orc_pert_destroy.argtypes = [vp]
+        L.orc_pert_solve.restype = ctypes.c_int
+        L.orc_pert_solve.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, dp]
+        L.orc_pert_pseudoresponse.restype = ctypes.c_double
+        L.orc_pert_pseudoresponse.argtypes = [vp, ctypes.c_int]
+        L.orc_pert_residual.argtypes = [vp, ctypes.c_int, dp, dp, dp, dp]
+        L.orc_pert_get.argtypes = [vp, dp, dp, dp, dp]
+        L.orc_pert_set.argtypes = [vp, dp, dp, dp, dp]
+        L.orc_linresp.restype = ctypes.c_double
+        L.orc_linresp.argtypes = [vp, vp]
         L.orc_tgrad_enable.argtypes = [vp, ctypes.c_int]
         _libs[omp] = L
     return _libs[omp]
@@ -307,6 +319,12 @@ class Oracle:
     def tgrad_enable(self, on):
         self.L.orc_tgrad_enable(self.h, int(on))
 
+    def pert(self, A, omega=0.0):
+        """CCPert::CCPert (ccpert.cc:11-75) of the one-electron perturbation A
+        (nact x nact) at omega, over the HBAR of lambda_init and the current
+        Lambda amplitudes."""
+        return Pert(self, A, omega)
+
     def residual_rows(self, rows):
         """(R1, R2[:rows]) of the current amplitudes."""
         R1 = np.zeros((self.no, self.nv))
@@ -317,3 +335,60 @@ class Oracle:
     def iteration_sample(self, rows):
         """Runs rows/no of one CCSD iteration (for the bounded CPU baseline)."""
         self.L.orc_iteration_sample(self.h, rows)
+
+
+class Pert:
+    """One perturbed wave function (ccpert.cc): pertbar, the right-hand (X)
+    and left-hand (Y) perturbed amplitudes, and CCResp::linresp."""
+
+    def __init__(self, orc, A, omega):
+        self.o = orc
+        self.L = orc.L
+        self.A = np.ascontiguousarray(A, dtype=np.float64)
+        self.omega = omega
+        self.h = self.L.orc_pert_create(orc.h, _p(self.A), omega)
+
+    def close(self):
+        if self.h:
+            self.L.orc_pert_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, hand=0, maxiter=200, conv=1e-10, diis=True, min_vecs=3):
+        """CCPert::solve (ccpert.cc:178-225): hand 0 right (X), 1 left (Y)."""
+        out = np.zeros(3)
+        rc = self.L.orc_pert_solve(self.h, hand, maxiter, conv, int(diis), min_vecs, _p(out))
+        return dict(converged=rc == 0, pseudoresponse=out[0], rms=out[1], niter=int(out[2]))
+
+    def pseudoresponse(self, hand=0):
+        return self.L.orc_pert_pseudoresponse(self.h, hand)
+
+    def residual(self, hand, A1, A2):
+        no, nv = self.o.no, self.o.nv
+        A1 = np.ascontiguousarray(A1, dtype=np.float64)
+        A2 = np.ascontiguousarray(A2, dtype=np.float64)
+        R1, R2 = np.zeros((no, nv)), np.zeros((no, no, nv, nv))
+        self.L.orc_pert_residual(self.h, hand, _p(A1), _p(A2), _p(R1), _p(R2))
+        return R1, R2
+
+    def amplitudes(self):
+        no, nv = self.o.no, self.o.nv
+        X1, Y1 = np.zeros((no, nv)), np.zeros((no, nv))
+        X2, Y2 = np.zeros((no, no, nv, nv)), np.zeros((no, no, nv, nv))
+        self.L.orc_pert_get(self.h, _p(X1), _p(X2), _p(Y1), _p(Y2))
+        return X1, X2, Y1, Y2
+
+    def set_amplitudes(self, X1=None, X2=None, Y1=None, Y2=None):
+        c = lambda a: None if a is None else _p(np.ascontiguousarray(a, dtype=np.float64))
+        keep = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None for a in (X1, X2, Y1, Y2)]
+        self.L.orc_pert_set(self.h, *[None if a is None else _p(a) for a in keep])
+
+    def linresp(self, other):
+        """CCResp::linresp (ccresp.cc:27-124): <<A;B>> with A this
+        perturbation and B other's."""
+        return self.L.orc_linresp(self.h, other.h)

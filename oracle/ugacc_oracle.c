@@ -1802,4 +1802,632 @@ void orc_tgrad_enable(void *p, int on) {
   orc_state *s = (orc_state *)p;
   s->tg_on = on && s->tg_s1;
 }
+
+/* -------------------------------------------------- linear response
+ * CCPert (ccpert.cc) and CCResp::linresp (ccresp.cc:27-124): the perturbed
+ * amplitudes of a one-electron perturbation A (an nact x nact MO matrix,
+ * Perturbation::prop_p, perturbation.cc) at frequency omega over the HBAR of
+ * orc_lambda_init and the Lambda amplitudes of orc_lambda, and the linear
+ * response function of two of them. */
+#define HB(k) (s->hb[k])
+typedef struct {
+  orc_state *s;
+  double omega;
+  double *pert;                                        /* nact x nact */
+  double *Aov, *Aoo, *Avv, *Avo, *Aovoo, *Avvvo, *Avvoo; /* pertbar, ccpert.cc:112-176 */
+  double *D1, *D2;
+  double *X1, *X1o, *X2, *X2o, *Y1, *Y1o, *Y2, *Y2o;
+  double *Goo, *Gvv;
+} orc_pert;
+
+#define PA(p, q) (P->pert[(size_t)(p) * NA + (q)])
+#define AOV(m, e) P->Aov[O2(m, e)]
+#define AOO(m, i) P->Aoo[(size_t)(m) * no + (i)]
+#define AVV(a, e) P->Avv[(size_t)(a) * nv + (e)]
+#define AVO(a, i) P->Avo[(size_t)(a) * no + (i)]
+#define AOVOO(m, b, i, j) P->Aovoo[Q4(m, b, i, j, nv, no, no)]
+#define AVVVO(a, b, e, i) P->Avvvo[Q4(a, b, e, i, nv, nv, no)]
+#define AVVOO(a, b, i, j) P->Avvoo[Q4(a, b, i, j, nv, no, no)]
+
+/* CCPert::pertbar, ccpert.cc:112-176: the similarity-transformed perturbation */
+static void pert_bar(orc_pert *P) {
+  orc_state *s = P->s;
+  const int no = s->no, nv = s->nv;
+  const double *t1 = s->t1, *t2 = s->t2;
+  for (int m = 0; m < no; m++)
+    for (int e = 0; e < nv; e++) AOV(m, e) = PA(m, e + no);
+  for (int m = 0; m < no; m++)
+    for (int i = 0; i < no; i++) {
+      double x = PA(m, i);
+      for (int e = 0; e < nv; e++) x += t1[O2(i, e)] * PA(m, e + no);
+      AOO(m, i) = x;
+    }
+  for (int a = 0; a < nv; a++)
+    for (int e = 0; e < nv; e++) {
+      double x = PA(a + no, e + no);
+      for (int m = 0; m < no; m++) x -= t1[O2(m, a)] * PA(m, e + no);
+      AVV(a, e) = x;
+    }
+  for (int a = 0; a < nv; a++)
+    for (int i = 0; i < no; i++) {
+      double x = PA(a + no, i);
+      for (int e = 0; e < nv; e++) x += t1[O2(i, e)] * PA(a + no, e + no);
+      for (int m = 0; m < no; m++) x -= t1[O2(m, a)] * PA(m, i);
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++)
+          x += (2.0 * t2[T4(m, i, e, a)] - t2[T4(i, m, e, a)] - t1[O2(i, e)] * t1[O2(m, a)]) * PA(m, e + no);
+      AVO(a, i) = x;
+    }
+  for (int m = 0; m < no; m++)
+    for (int b = 0; b < nv; b++)
+      for (int i = 0; i < no; i++)
+        for (int j = 0; j < no; j++) {
+          double x = 0.0;
+          for (int e = 0; e < nv; e++) x += t2[T4(i, j, e, b)] * PA(m, e + no);
+          AOVOO(m, b, i, j) = x;
+        }
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++)
+      for (int e = 0; e < nv; e++)
+        for (int i = 0; i < no; i++) {
+          double x = 0.0;
+          for (int m = 0; m < no; m++) x -= t2[T4(m, i, a, b)] * PA(m, e + no);
+          AVVVO(a, b, e, i) = x;
+        }
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++)
+      for (int i = 0; i < no; i++)
+        for (int j = 0; j < no; j++) {
+          double x = 0.0;
+          for (int e = 0; e < nv; e++) x += t2[T4(i, j, e, b)] * AVV(a, e);
+          for (int m = 0; m < no; m++) x -= t2[T4(m, j, a, b)] * AOO(m, i);
+          AVVOO(a, b, i, j) = x;
+        }
+}
+
+/* CCPert::CCPert, ccpert.cc:11-75: pertbar, the HBAR-diagonal denominators
+ * and the guesses X = Y = Abar / (D + omega).  Needs orc_lambda_init (HBAR)
+ * and the Lambda amplitudes (orc_lambda or orc_set_lambda). */
+void *orc_pert_create(void *p, const double *pert, double omega) {
+  orc_state *s = (orc_state *)p;
+  const int no = s->no, nv = s->nv;
+  const size_t o = no, v = nv, n = s->nact, ov = o * v;
+  orc_pert *P = (orc_pert *)calloc(1, sizeof(orc_pert));
+  P->s = s; P->omega = omega;
+  P->pert = zalloc(n * n); memcpy(P->pert, pert, n * n * sizeof(double));
+  P->Aov = zalloc(ov); P->Aoo = zalloc(o * o); P->Avv = zalloc(v * v); P->Avo = zalloc(ov);
+  P->Aovoo = zalloc(o * v * o * o); P->Avvvo = zalloc(v * v * v * o); P->Avvoo = zalloc(v * v * o * o);
+  P->D1 = zalloc(ov); P->D2 = zalloc(ov * ov);
+  P->X1 = zalloc(ov); P->X1o = zalloc(ov); P->Y1 = zalloc(ov); P->Y1o = zalloc(ov);
+  P->X2 = zalloc(ov * ov); P->X2o = zalloc(ov * ov); P->Y2 = zalloc(ov * ov); P->Y2o = zalloc(ov * ov);
+  P->Goo = zalloc(o * o); P->Gvv = zalloc(v * v);
+  pert_bar(P);
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      P->D1[O2(i, a)] = Hoo_(i, i) - Hvv_(a, a);
+      P->X1[O2(i, a)] = P->Y1[O2(i, a)] = AVO(a, i) / (P->D1[O2(i, a)] + omega);
+    }
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) {
+          const double d = Hoo_(i, i) + Hoo_(j, j) - Hvv_(a, a) - Hvv_(b, b);
+          P->D2[T4(i, j, a, b)] = d;
+          P->X2[T4(i, j, a, b)] = P->Y2[T4(i, j, a, b)] = (AVVOO(a, b, i, j) + AVVOO(b, a, j, i)) / (d + omega);
+        }
+  return P;
+}
+
+void orc_pert_destroy(void *q) {
+  orc_pert *P = (orc_pert *)q;
+  if (!P) return;
+  free(P->pert); free(P->Aov); free(P->Aoo); free(P->Avv); free(P->Avo); free(P->Aovoo); free(P->Avvvo);
+  free(P->Avvoo); free(P->D1); free(P->D2); free(P->X1); free(P->X1o); free(P->X2); free(P->X2o);
+  free(P->Y1); free(P->Y1o); free(P->Y2); free(P->Y2o); free(P->Goo); free(P->Gvv);
+  free(P);
+}
+
+/* CCPert::build_X1, ccpert.cc:382-407: raw residual from (X1o, X2o) into R1 */
+static void pert_X1(orc_pert *P, const double *X1o, const double *X2o, double *R1) {
+  orc_state *s = P->s;
+  const int no = s->no, nv = s->nv;
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      double x = AVO(a, i) - P->omega * X1o[O2(i, a)];
+      for (int e = 0; e < nv; e++) x += X1o[O2(i, e)] * Hvv_(a, e);
+      for (int m = 0; m < no; m++) x -= X1o[O2(m, a)] * Hoo_(m, i);
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++) x += X1o[O2(m, e)] * (2.0 * Hovvo_(m, a, e, i) - Hovov_(m, a, i, e));
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++) x += Hov_(m, e) * (2.0 * X2o[T4(m, i, e, a)] - X2o[T4(i, m, e, a)]);
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++)
+          for (int f = 0; f < nv; f++) x += X2o[T4(i, m, e, f)] * (2.0 * Hvovv_(a, m, e, f) - Hvovv_(a, m, f, e));
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++)
+          for (int e = 0; e < nv; e++) x -= X2o[T4(m, n, a, e)] * (2.0 * Hooov_(m, n, i, e) - Hooov_(n, m, i, e));
+      R1[O2(i, a)] = x;
+    }
+}
+
+/* CCPert::build_X2, ccpert.cc:409-470: R2 = Z + Z^T from (X1o, X2o) */
+static void pert_X2(orc_pert *P, const double *X1o, const double *X2o, double *R2) {
+  orc_state *s = P->s;
+  const int no = s->no, nv = s->nv;
+  const double *t2 = s->t2;
+  double *Zvv = zalloc((size_t)nv * nv), *Zoo = zalloc((size_t)no * no), *Z = zalloc((size_t)no * no * nv * nv);
+  for (int a = 0; a < nv; a++)
+    for (int e = 0; e < nv; e++) {
+      double x = 0.0;
+      for (int m = 0; m < no; m++)
+        for (int f = 0; f < nv; f++) x += (2.0 * Hvovv_(a, m, e, f) - Hvovv_(a, m, f, e)) * X1o[O2(m, f)];
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++)
+          for (int f = 0; f < nv; f++) x -= LL(m, n, e + no, f + no) * X2o[T4(m, n, a, f)];
+      Zvv[(size_t)a * nv + e] = x;
+    }
+  for (int m = 0; m < no; m++)
+    for (int i = 0; i < no; i++) {
+      double x = 0.0;
+      for (int n = 0; n < no; n++)
+        for (int e = 0; e < nv; e++) x -= (2.0 * Hooov_(m, n, i, e) - Hooov_(n, m, i, e)) * X1o[O2(n, e)];
+      for (int n = 0; n < no; n++)
+        for (int e = 0; e < nv; e++)
+          for (int f = 0; f < nv; f++) x -= LL(m, n, e + no, f + no) * X2o[T4(i, n, e, f)];
+      Zoo[(size_t)m * no + i] = x;
+    }
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) {
+          double x = AVVOO(a, b, i, j) - 0.5 * P->omega * X2o[T4(i, j, a, b)];
+          for (int e = 0; e < nv; e++) x += X1o[O2(i, e)] * Hvvvo_(a, b, e, j);
+          for (int m = 0; m < no; m++) x -= X1o[O2(m, a)] * Hovoo_(m, b, i, j);
+          for (int m = 0; m < no; m++) x += Zoo[(size_t)m * no + i] * t2[T4(m, j, a, b)];
+          for (int e = 0; e < nv; e++) x += Zvv[(size_t)a * nv + e] * t2[T4(i, j, e, b)];
+          for (int e = 0; e < nv; e++) x += X2o[T4(i, j, e, b)] * Hvv_(a, e);
+          for (int m = 0; m < no; m++) x -= X2o[T4(m, j, a, b)] * Hoo_(m, i);
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++) x += 0.5 * X2o[T4(m, n, a, b)] * Hoooo_(m, n, i, j);
+          for (int e = 0; e < nv; e++)
+            for (int f = 0; f < nv; f++) x += 0.5 * X2o[T4(i, j, e, f)] * Hvvvv_(a, b, e, f);
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++)
+              x -= X2o[T4(i, m, e, b)] * Hovov_(m, a, j, e) + X2o[T4(i, m, e, a)] * Hovvo_(m, b, e, j);
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++)
+              x += X2o[T4(m, i, e, a)] * (2.0 * Hovvo_(m, b, e, j) - Hovov_(m, b, j, e));
+          Z[T4(i, j, a, b)] = x;
+        }
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) R2[T4(i, j, a, b)] = Z[T4(i, j, a, b)] + Z[T4(j, i, b, a)];
+  free(Zvv); free(Zoo); free(Z);
+}
+
+/* CCPert::build_G, ccpert.cc:489-513, from Y2o */
+static void pert_G(orc_pert *P, const double *Y2o) {
+  orc_state *s = P->s;
+  const int no = s->no, nv = s->nv;
+  const double *t2 = s->t2;
+  for (int m = 0; m < no; m++)
+    for (int i = 0; i < no; i++) {
+      double x = 0.0;
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++)
+          for (int j = 0; j < no; j++) x += t2[T4(m, j, a, b)] * Y2o[T4(i, j, a, b)];
+      P->Goo[(size_t)m * no + i] = x;
+    }
+  for (int a = 0; a < nv; a++)
+    for (int e = 0; e < nv; e++) {
+      double x = 0.0;
+      for (int i = 0; i < no; i++)
+        for (int j = 0; j < no; j++)
+          for (int b = 0; b < nv; b++) x -= t2[T4(i, j, e, b)] * Y2o[T4(i, j, a, b)];
+      P->Gvv[(size_t)a * nv + e] = x;
+    }
+}
+
+/* CCPert::build_Y1, ccpert.cc:516-703: the left-hand residual from (Y1o,
+ * Y2o) and G, with the terms of the right-hand amplitudes X1/X2, the Lambda
+ * amplitudes and the perturbation (the reference's (i) .. (t)) */
+static void pert_Y1(orc_pert *P, const double *Y1o, const double *Y2o, double *R1) {
+  orc_state *s = P->s;
+  const int no = s->no, nv = s->nv;
+  const double *t2 = s->t2, *l1 = s->l1, *l2 = s->l2, *X1 = P->X1, *X2 = P->X2;
+  const double w = P->omega;
+#define Y1_(i, a) Y1o[O2(i, a)]
+#define Y2_(i, j, a, b) Y2o[T4(i, j, a, b)]
+#define X1_(i, a) X1[O2(i, a)]
+#define X2_(i, j, a, b) X2[T4(i, j, a, b)]
+#define l1_(i, a) l1[O2(i, a)]
+#define l2_(i, j, a, b) l2[T4(i, j, a, b)]
+#define T2_(i, j, a, b) t2[T4(i, j, a, b)]
+#define GVV(a, e) P->Gvv[(size_t)(a) * nv + (e)]
+#define GOO(m, i) P->Goo[(size_t)(m) * no + (i)]
+#define LV(p, q, r, t) LL(p, q, (r) + no, (t) + no)
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      double x = 2 * PA(i, a + no) + w * Y1_(i, a);
+      for (int e = 0; e < nv; e++) x += Y1_(i, e) * Hvv_(e, a);
+      for (int m = 0; m < no; m++) x -= Y1_(m, a) * Hoo_(i, m);
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++) x += Y1_(m, e) * (2 * Hovvo_(i, e, a, m) - Hovov_(i, e, m, a));
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++)
+          for (int f = 0; f < nv; f++) x += Y2_(i, m, e, f) * Hvvvo_(e, f, a, m);
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++)
+          for (int e = 0; e < nv; e++) x -= Y2_(m, n, a, e) * Hovoo_(i, e, m, n);
+      for (int e = 0; e < nv; e++)
+        for (int f = 0; f < nv; f++) x -= GVV(e, f) * (2 * Hvovv_(e, i, f, a) - Hvovv_(e, i, a, f));
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++) x -= GOO(m, n) * (2 * Hooov_(m, i, n, a) - Hooov_(i, m, n, a));
+      /* (i) */
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++) x += 2.0 * X1_(m, e) * LV(i, m, a, e);
+      /* (m) */
+      for (int m = 0; m < no; m++) x -= l1_(m, a) * AOO(i, m);
+      for (int e = 0; e < nv; e++) x += l1_(i, e) * AVV(e, a);
+      /* (n) */
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++)
+          for (int f = 0; f < nv; f++) x += l2_(i, m, f, e) * AVVVO(f, e, a, m);
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++)
+          for (int e = 0; e < nv; e++) {
+            x -= 0.5 * l2_(m, n, e, a) * AOVOO(i, e, n, m);
+            x -= 0.5 * l2_(m, n, a, e) * AOVOO(i, e, m, n);
+          }
+      /* (q) */
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++) {
+          const double xm = X1_(m, e);
+          x -= xm * Hov_(m, a) * l1_(i, e);
+          x -= xm * Hov_(i, e) * l1_(m, a);
+          for (int n = 0; n < no; n++) {
+            x -= xm * (2 * Hooov_(m, i, n, a) - Hooov_(i, m, n, a)) * l1_(n, e);
+            x -= xm * (2 * Hooov_(i, m, n, e) - Hooov_(m, i, n, e)) * l1_(n, a);
+          }
+          for (int f = 0; f < nv; f++) {
+            x += xm * (2 * Hvovv_(f, m, a, e) - Hvovv_(f, m, e, a)) * l1_(i, f);
+            x += xm * (2 * Hvovv_(f, i, e, a) - Hvovv_(f, i, a, e)) * l1_(m, f);
+          }
+        }
+      /* (r) */
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++)
+          for (int e = 0; e < nv; e++)
+            for (int f = 0; f < nv; f++) {
+              const double xm = X2_(m, n, e, f);
+              x += 2.0 * xm * LV(i, m, a, e) * l1_(n, f);
+              x -= xm * LV(i, m, a, f) * l1_(n, e);
+              x -= xm * LV(m, i, e, f) * l1_(n, a);
+              x -= xm * LV(n, m, f, a) * l1_(i, e);
+            }
+      /* (s) */
+      for (int m = 0; m < no; m++)
+        for (int e = 0; e < nv; e++) {
+          const double xm = X1_(m, e);
+          for (int n = 0; n < no; n++)
+            for (int f = 0; f < nv; f++) {
+              x -= xm * Hovov_(m, f, n, a) * l2_(n, i, e, f);
+              x -= xm * Hovov_(i, f, n, e) * l2_(n, m, a, f);
+              x -= xm * Hovvo_(m, f, a, n) * l2_(n, i, f, e);
+              x -= xm * Hovvo_(i, f, e, n) * l2_(n, m, f, a);
+            }
+          for (int f = 0; f < nv; f++)
+            for (int g = 0; g < nv; g++) {
+              x += 0.5 * xm * Hvvvv_(f, g, a, e) * l2_(i, m, f, g);
+              x += 0.5 * xm * Hvvvv_(f, g, e, a) * l2_(m, i, f, g);
+            }
+          for (int n = 0; n < no; n++)
+            for (int q = 0; q < no; q++) {
+              x += 0.5 * xm * Hoooo_(i, m, n, q) * l2_(n, q, a, e);
+              x += 0.5 * xm * Hoooo_(m, i, n, q) * l2_(n, q, e, a);
+            }
+        }
+      /* the four-factor terms, ccpert.cc:629-640 */
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++)
+          for (int e = 0; e < nv; e++)
+            for (int f = 0; f < nv; f++)
+              for (int j = 0; j < no; j++)
+                for (int b = 0; b < nv; b++) {
+                  x -= LV(i, n, a, f) * X1_(n, b) * T2_(j, m, f, e) * l2_(j, m, b, e);
+                  x -= LV(m, i, e, f) * X1_(m, e) * T2_(j, n, f, b) * l2_(j, n, a, b);
+                  x -= LV(m, n, e, a) * X1_(m, e) * T2_(j, n, f, b) * l2_(j, i, f, b);
+                  x -= LV(i, n, a, f) * X1_(j, f) * T2_(n, m, e, b) * l2_(j, m, e, b);
+                }
+      /* (t) */
+      for (int m = 0; m < no; m++)
+        for (int n = 0; n < no; n++)
+          for (int e = 0; e < nv; e++)
+            for (int f = 0; f < nv; f++) {
+              const double xm = X2_(m, n, e, f);
+              x -= xm * Hov_(m, a) * l2_(n, i, f, e);
+              x -= xm * Hov_(i, e) * l2_(n, m, f, a);
+              for (int g = 0; g < nv; g++) {
+                x -= xm * Hvovv_(g, n, e, a) * l2_(i, m, f, g);
+                x -= xm * Hvovv_(g, n, a, e) * l2_(m, i, f, g);
+                x -= xm * Hvovv_(g, i, e, f) * l2_(n, m, a, g);
+                x += xm * (2 * Hvovv_(g, m, a, e) - Hvovv_(g, m, e, a)) * l2_(n, i, f, g);
+                x += xm * (2 * Hvovv_(g, i, e, a) - Hvovv_(g, i, a, e)) * l2_(n, m, f, g);
+              }
+              for (int q = 0; q < no; q++) {
+                x += xm * Hooov_(m, n, q, a) * l2_(q, i, e, f);
+                x += xm * Hooov_(i, n, q, e) * l2_(q, m, a, f);
+                x += xm * Hooov_(m, i, q, f) * l2_(q, n, e, a);
+                x -= xm * (2 * Hooov_(m, i, q, a) - Hooov_(i, m, q, a)) * l2_(n, q, f, e);
+                x -= xm * (2 * Hooov_(i, m, q, e) - Hooov_(m, i, q, e)) * l2_(n, q, f, a);
+              }
+            }
+      R1[O2(i, a)] = x;
+    }
+}
+
+/* CCPert::build_Y2, ccpert.cc:705-882: R2 = Z + Z^T from (Y1o, Y2o) and G,
+ * with the terms (o) .. (x) */
+static void pert_Y2(orc_pert *P, const double *Y1o, const double *Y2o, double *R2) {
+  orc_state *s = P->s;
+  const int no = s->no, nv = s->nv;
+  const double *l1 = s->l1, *l2 = s->l2, *X1 = P->X1, *X2 = P->X2;
+  const double w = P->omega;
+  double *Z = zalloc((size_t)no * no * nv * nv);
+#define IV(p, q, r, t) INT(p, q, (r) + no, (t) + no)
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) {
+          double x = 0.5 * w * Y2_(i, j, a, b);
+          x += 2.0 * Y1_(i, a) * Hov_(j, b) - Y1_(j, a) * Hov_(i, b);
+          for (int e = 0; e < nv; e++) x += Y2_(i, j, e, b) * Hvv_(e, a);
+          for (int m = 0; m < no; m++) x -= Y2_(m, j, a, b) * Hoo_(i, m);
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++) x += 0.5 * Y2_(m, n, a, b) * Hoooo_(i, j, m, n);
+          for (int e = 0; e < nv; e++)
+            for (int f = 0; f < nv; f++) x += 0.5 * Y2_(i, j, e, f) * Hvvvv_(e, f, a, b);
+          for (int e = 0; e < nv; e++) x += Y1_(i, e) * (2 * Hvovv_(e, j, a, b) - Hvovv_(e, j, b, a));
+          for (int m = 0; m < no; m++) x -= Y1_(m, b) * (2 * Hooov_(j, i, m, a) - Hooov_(i, j, m, a));
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++) {
+              x += Y2_(m, j, e, b) * (2 * Hovvo_(i, e, a, m) - Hovov_(i, e, m, a));
+              x -= Y2_(m, i, b, e) * Hovov_(j, e, m, a);
+              x -= Y2_(m, i, e, b) * Hovvo_(j, e, a, m);
+            }
+          for (int e = 0; e < nv; e++) x += GVV(a, e) * LV(i, j, e, b);
+          for (int m = 0; m < no; m++) x -= GOO(m, i) * LV(m, j, a, b);
+          /* (o), (p) */
+          x += 2 * l1_(j, b) * PA(i, a + no) - l1_(i, b) * PA(j, a + no);
+          for (int e = 0; e < nv; e++) x += l2_(i, j, e, b) * AVV(e, a);
+          for (int m = 0; m < no; m++) x -= l2_(m, j, a, b) * AOO(i, m);
+          /* (u) */
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++) {
+              const double xm = X1_(m, e);
+              x -= xm * l1_(j, a) * LV(m, i, e, b);
+              x -= xm * l1_(m, b) * LV(i, j, a, e);
+              x -= xm * l1_(i, e) * LV(j, m, b, a);
+              x += 2.0 * xm * l1_(j, b) * LV(i, m, a, e);
+            }
+          /* (w) */
+          for (int m = 0; m < no; m++)
+            for (int e = 0; e < nv; e++) {
+              const double xm = X1_(m, e);
+              x -= Hov_(m, a) * l2_(j, i, b, e) * xm;
+              x -= Hov_(i, e) * l2_(j, m, b, a) * xm;
+              for (int f = 0; f < nv; f++) {
+                x -= Hvovv_(f, m, b, a) * l2_(j, i, f, e) * xm;
+                x -= Hvovv_(f, j, e, a) * l2_(m, i, f, b) * xm;
+                x -= Hvovv_(f, i, b, e) * l2_(j, m, f, a) * xm;
+                x += (2 * Hvovv_(f, m, a, e) - Hvovv_(f, m, e, a)) * l2_(j, i, b, f) * xm;
+                x += (2 * Hvovv_(f, i, e, a) - Hvovv_(f, i, a, e)) * l2_(j, m, b, f) * xm;
+              }
+              for (int n = 0; n < no; n++) {
+                x += Hooov_(j, m, n, a) * l2_(n, i, b, e) * xm;
+                x += Hooov_(m, j, n, a) * l2_(n, i, e, b) * xm;
+                x += Hooov_(j, i, n, e) * l2_(n, m, b, a) * xm;
+                x -= (2 * Hooov_(m, i, n, a) - Hooov_(i, m, n, a)) * l2_(j, n, b, e) * xm;
+                x -= (2 * Hooov_(i, m, n, e) - Hooov_(m, i, n, e)) * l2_(j, n, b, a) * xm;
+              }
+            }
+          /* (x) */
+          for (int m = 0; m < no; m++)
+            for (int n = 0; n < no; n++)
+              for (int e = 0; e < nv; e++)
+                for (int f = 0; f < nv; f++) {
+                  const double xm = X2_(m, n, e, f);
+                  x += 0.25 * IV(n, m, a, b) * l2_(j, i, e, f) * xm;
+                  x += 0.25 * IV(m, n, a, b) * l2_(j, i, f, e) * xm;
+                  x += 0.25 * IV(j, n, a, e) * l2_(m, i, f, b) * xm;
+                  x += 0.25 * IV(j, m, a, f) * l2_(n, i, e, b) * xm;
+                  x += 0.25 * IV(n, j, a, e) * l2_(m, i, b, f) * xm;
+                  x += 0.25 * IV(m, j, a, f) * l2_(n, i, b, e) * xm;
+                  x += 0.25 * IV(j, n, e, a) * l2_(i, m, f, b) * xm;
+                  x += 0.25 * IV(j, m, f, a) * l2_(i, n, e, b) * xm;
+                  x += 0.25 * IV(n, j, e, a) * l2_(i, m, b, f) * xm;
+                  x += 0.25 * IV(m, j, f, a) * l2_(i, n, b, e) * xm;
+                  x += 0.25 * IV(j, i, e, f) * l2_(n, m, a, b) * xm;
+                  x += 0.25 * IV(j, i, f, e) * l2_(m, n, a, b) * xm;
+                  x -= 0.5 * LV(i, m, a, f) * l2_(j, n, b, e) * xm;
+                  x -= 0.5 * LV(i, n, a, e) * l2_(j, m, b, f) * xm;
+                  x -= 0.5 * LV(m, i, e, f) * l2_(j, n, b, a) * xm;
+                  x -= 0.5 * LV(m, n, e, a) * l2_(j, i, b, f) * xm;
+                  x -= 0.5 * LV(n, m, f, a) * l2_(j, i, b, e) * xm;
+                  x -= 0.5 * LV(n, i, f, e) * l2_(j, m, b, a) * xm;
+                  x -= LV(i, j, a, e) * l2_(n, m, f, b) * xm;
+                  x -= LV(i, m, a, b) * l2_(n, j, f, e) * xm;
+                  x -= LV(m, j, e, a) * l2_(n, i, f, b) * xm;
+                  x += 2.0 * LV(i, m, a, e) * l2_(n, j, f, b) * xm;
+                }
+          Z[T4(i, j, a, b)] = x;
+        }
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) R2[T4(i, j, a, b)] = Z[T4(i, j, a, b)] + Z[T4(j, i, b, a)];
+  free(Z);
+#undef IV
+}
+
+/* CCPert::pseudoresponse, ccpert.cc:229-249 */
+static double pert_pseudo(orc_pert *P, const double *X1, const double *X2) {
+  orc_state *s = P->s;
+  const int no = s->no, nv = s->nv;
+  double p1 = 0.0, p2 = 0.0;
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      p1 += 2.0 * X1[O2(i, a)] * AVO(a, i);
+      for (int j = 0; j < no; j++)
+        for (int b = 0; b < nv; b++)
+          p2 += 2.0 * (2.0 * X2[T4(i, j, a, b)] - X2[T4(i, j, b, a)]) * AVVOO(a, b, i, j);
+    }
+  return -2.0 * (p1 + p2);
+}
+
+double orc_pert_pseudoresponse(void *q, int hand) {
+  orc_pert *P = (orc_pert *)q;
+  return hand == 0 ? pert_pseudo(P, P->X1, P->X2) : pert_pseudo(P, P->Y1, P->Y2);
+}
+
+/* One residual evaluation at the given amplitudes (the object's own are not
+ * touched, except that hand 1 uses its X1/X2): hand 0 build_X1/build_X2,
+ * hand 1 build_G/build_Y1/build_Y2; raw R1, R2 (before the denominators). */
+void orc_pert_residual(void *q, int hand, const double *A1, const double *A2, double *R1, double *R2) {
+  orc_pert *P = (orc_pert *)q;
+  if (hand == 0) {
+    pert_X1(P, A1, A2, R1);
+    pert_X2(P, A1, A2, R2);
+  } else {
+    pert_G(P, A2);
+    pert_Y1(P, A1, A2, R1);
+    pert_Y2(P, A1, A2, R2);
+  }
+}
+
+/* CCPert::solve, ccpert.cc:178-225: amp_save, build, increment_amps
+ * (x = x_old + R/(D + omega), rms the 2-norm of the raw R), break on rms <
+ * conv before DIIS, else DIIS (the Pulay step of orc_ccsd, extrapolating from
+ * min_vecs stored vectors).  hand 0: X (right), 1: Y (left; needs the
+ * converged X).  out: pseudoresponse, rms, niter.  Returns 0 on convergence. */
+int orc_pert_solve(void *q, int hand, int maxiter, double conv, int do_diis, int min_vecs, double *out) {
+  orc_pert *P = (orc_pert *)q;
+  orc_state *s = P->s;
+  const size_t ov = (size_t)s->no * s->nv;
+  double **A1 = hand == 0 ? &P->X1 : &P->Y1, **A1o = hand == 0 ? &P->X1o : &P->Y1o;
+  double **A2 = hand == 0 ? &P->X2 : &P->Y2, **A2o = hand == 0 ? &P->X2o : &P->Y2o;
+  diis_free(s);
+  double rms = 0.0;
+  int niter = 0;
+  for (int iter = 1; iter <= maxiter; iter++) {
+    double *x = *A1; *A1 = *A1o; *A1o = x;
+    x = *A2; *A2 = *A2o; *A2o = x;
+    orc_pert_residual(P, hand, *A1o, *A2o, *A1, *A2);
+    double r1 = 0.0, r2 = 0.0;
+    for (size_t k = 0; k < ov; k++) {
+      r1 += (*A1)[k] * (*A1)[k];
+      (*A1)[k] = (*A1o)[k] + (*A1)[k] / (P->D1[k] + P->omega);
+    }
+    for (size_t k = 0; k < ov * ov; k++) {
+      r2 += (*A2)[k] * (*A2)[k];
+      (*A2)[k] = (*A2o)[k] + (*A2)[k] / (P->D2[k] + P->omega);
+    }
+    rms = sqrt(r1 + r2);
+    niter = iter;
+    if (rms < conv) break;
+    if (do_diis) diis_step_x(s, *A1, *A1o, *A2, *A2o, 8, 0, min_vecs);
+  }
+  diis_free(s);
+  if (out) {
+    out[0] = orc_pert_pseudoresponse(P, hand);
+    out[1] = rms;
+    out[2] = niter;
+  }
+  return rms < conv ? 0 : 1;
+}
+
+void orc_pert_get(void *q, double *X1, double *X2, double *Y1, double *Y2) {
+  orc_pert *P = (orc_pert *)q;
+  const size_t ov = (size_t)P->s->no * P->s->nv;
+  if (X1) memcpy(X1, P->X1, ov * sizeof(double));
+  if (X2) memcpy(X2, P->X2, ov * ov * sizeof(double));
+  if (Y1) memcpy(Y1, P->Y1, ov * sizeof(double));
+  if (Y2) memcpy(Y2, P->Y2, ov * ov * sizeof(double));
+}
+
+void orc_pert_set(void *q, const double *X1, const double *X2, const double *Y1, const double *Y2) {
+  orc_pert *P = (orc_pert *)q;
+  const size_t ov = (size_t)P->s->no * P->s->nv;
+  if (X1) memcpy(P->X1, X1, ov * sizeof(double));
+  if (X2) memcpy(P->X2, X2, ov * ov * sizeof(double));
+  if (Y1) memcpy(P->Y1, Y1, ov * sizeof(double));
+  if (Y2) memcpy(P->Y2, Y2, ov * ov * sizeof(double));
+}
+
+/* CCResp::linresp, ccresp.cc:27-124: <<A;B>> from the perturbation of qx
+ * (A, its pertbar blocks) and the amplitudes of qy (X of B and the Y of B in
+ * its current state: the guess when only the right-hand equations were
+ * solved), with the Lambda amplitudes */
+double orc_linresp(void *qx, void *qy) {
+  orc_pert *P = (orc_pert *)qx, *Q = (orc_pert *)qy;
+  orc_state *s = P->s;
+  const int no = s->no, nv = s->nv;
+  const double *l1 = s->l1, *l2 = s->l2;
+  const double *X1y = Q->X1, *X2y = Q->X2, *Y1y = Q->Y1, *Y2y = Q->Y2;
+  double polar1 = 0.0, polar2 = 0.0;
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      polar1 += AVO(a, i) * Y1y[O2(i, a)];
+      for (int j = 0; j < no; j++)
+        for (int b = 0; b < nv; b++) polar1 += 0.5 * (AVVOO(a, b, i, j) + AVVOO(b, a, j, i)) * Y2y[T4(i, j, a, b)];
+    }
+  for (int i = 0; i < no; i++)
+    for (int a = 0; a < nv; a++) {
+      const double la = l1[O2(i, a)];
+      for (int j = 0; j < no; j++)
+        for (int b = 0; b < nv; b++)
+          polar2 += la * PA(j, b + no) * (2.0 * X2y[T4(i, j, a, b)] - X2y[T4(i, j, b, a)]);
+      for (int c = 0; c < nv; c++) polar2 += la * AVV(a, c) * X1y[O2(i, c)];
+      for (int k = 0; k < no; k++) polar2 -= la * AOO(k, i) * X1y[O2(k, a)];
+      for (int j = 0; j < no; j++)
+        for (int b = 0; b < nv; b++) {
+          const double l = l2[T4(i, j, a, b)];
+          for (int c = 0; c < nv; c++) polar2 += l2[T4(i, j, b, c)] * X1y[O2(i, a)] * AVVVO(b, c, a, j);
+          for (int k = 0; k < no; k++) {
+            polar2 -= 0.5 * l * X1y[O2(k, a)] * AOVOO(k, b, i, j);
+            polar2 -= 0.5 * l * X1y[O2(k, b)] * AOVOO(k, a, j, i);
+            polar2 -= 0.5 * l * X2y[T4(k, j, a, b)] * AOO(k, i);
+            polar2 -= 0.5 * l * X2y[T4(k, i, b, a)] * AOO(k, j);
+          }
+          for (int c = 0; c < nv; c++) {
+            polar2 += 0.5 * l * X2y[T4(i, j, a, c)] * AVV(b, c);
+            polar2 += 0.5 * l * X2y[T4(j, i, b, c)] * AVV(a, c);
+          }
+        }
+      polar2 += 2.0 * PA(i, a + no) * X1y[O2(i, a)];
+    }
+  return -1.0 * (polar1 + polar2);
+}
+#undef Y1_
+#undef Y2_
+#undef X1_
+#undef X2_
+#undef l1_
+#undef l2_
+#undef T2_
+#undef GVV
+#undef GOO
+#undef LV
+#undef PA
+#undef AOV
+#undef AOO
+#undef AVV
+#undef AVO
+#undef AOVOO
+#undef AVVVO
+#undef AVVOO
+#undef HB
 #undef Q4

@@ -82,6 +82,12 @@ def psi4(path, name):
             out["e_t"] = floats(l)[-1]
         elif "Number of ijk index combinations" in l:
             out["n_triplets"] = int(l.split()[-1])
+    # the SCF electronic dipole moment (a.u., about the origin): the line
+    # after the first "Electronic Dipole Moment: (a.u.)" header
+    for k, l in enumerate(txt):
+        if "Electronic Dipole Moment: (a.u.)" in l:
+            out["scf_electronic_dipole"] = floats(txt[k + 1])[:3]
+            break
     return out
 
 

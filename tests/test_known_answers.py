@@ -80,6 +80,60 @@ def test_hof_sto3g(key, nfzc):
     assert abs(o.tcorr_tjl() - ka["e_t"]) < 1e-12
 
 
+def test_hof_scf_electronic_dipole():
+    # Psi4's SCF electronic dipole moment about the origin (psi4/hof/output.c1_nofzc:263-264),
+    # printed to 4 decimals: sum_pq D_pq <p|-r|q> with the dipole integrals the "Mu"
+    # perturbation of the response uses
+    ka = KA["hof_sto3g_c1_allelec"]
+    ref = Reference(molecules.HOF, "sto-3g")
+    s = ref.scf
+    P = 2 * s.C[:, :s.nocc] @ s.C[:, :s.nocc].T
+    mu = ref.perturbation("Mu")
+    for k in range(3):
+        # in the MO basis: twice the trace of -r over the occupied orbitals
+        e_mo = 2 * np.trace(mu[k][:ref.no, :ref.no])
+        assert abs(e_mo - ka["scf_electronic_dipole"][k]) < 6e-5, (k, e_mo)
+    from ugacc_amd.chem import integrals
+    D, _ = integrals.properties(s.shells)
+    for k in range(3):
+        assert abs(np.sum(P * -D[k]) - ka["scf_electronic_dipole"][k]) < 6e-5
+
+
+def test_angular_momentum_integrals_within_a_p_shell():
+    # one atom at the origin: (x d/dy - y d/dx) p_x = -p_y, so
+    # <p_y(i)| M_z |p_x(j)> = -<p_y(i)|p_y(j)> for any two contracted p shells,
+    # and M is antisymmetric; the "L" operator is -M/2 (perturbation.cc)
+    from ugacc_amd.chem import integrals
+    from ugacc_amd.chem.basis import shells_for
+    shells = shells_for([("O", (0.0, 0.0, 0.0))], "cc-pvdz")
+    S, _, _ = integrals.one_electron(shells, [8.0], np.zeros((1, 3)))
+    D, M = integrals.properties(shells)
+    for k in range(3):
+        assert np.max(np.abs(M[k] + M[k].T)) < 1e-13
+        assert np.max(np.abs(D[k] - D[k].T)) < 1e-13
+    # spherical p components are ordered (z, x, y) in the pure-function order of
+    # the front end when l = 1 (cart2sph), located from the overlap pattern
+    npx = [i for i, sh in enumerate(shells) if sh[1] == 1]
+    assert npx
+    off = np.cumsum([0] + [2 * sh[1] + 1 for sh in shells])
+    cx, cy, cz = [[off[i] + c for i in npx] for c in _p_components()]
+    Mz, Mx, My = M[2], M[0], M[1]
+    for i in range(len(npx)):
+        for j in range(len(npx)):
+            assert abs(Mz[cy[i], cx[j]] + S[cy[i], cy[j]]) < 1e-12
+            assert abs(Mx[cz[i], cy[j]] + S[cz[i], cz[j]]) < 1e-12
+            assert abs(My[cx[i], cz[j]] + S[cx[i], cx[j]]) < 1e-12
+
+
+def _p_components():
+    # positions of x, y, z among the spherical p functions of the front end
+    from ugacc_amd.chem import integrals
+    sh = [((0.0, 0.0, 0.0), 1, [(1.0, 1.0)])]
+    D, _ = integrals.properties([((0.0, 0.0, 0.0), 0, [(1.0, 1.0)])] + sh)
+    # <s|x|p_c> is non-zero only for the p function along x, and so on
+    return [int(np.argmax(np.abs(D[k][0, 1:]))) for k in range(3)]
+
+
 def test_spherical_d_shell_is_orthonormal():
     # a lone d shell: the five pure functions are orthonormal
     from ugacc_amd.chem import integrals

@@ -420,6 +420,66 @@ void one_cart(const Shell &A, const Shell &B, int natom, const double *Z, const 
     }
 }
 
+// 1-D overlaps s[i][j] = <x_A^i e^(-a x_A^2) | x_B^j e^(-b x_B^2)> for i <= la,
+// j <= jmax (McMurchie-Davidson E^{ij}_0 times sqrt(pi / p))
+void overlap_1d(double a, double b, double Ax, double Bx, int la, int jmax, double (*s)[LMAX + 4]) {
+  const double p = a + b, mu = a * b / p, Px = (a * Ax + b * Bx) / p;
+  const double XPA = Px - Ax, XPB = Px - Bx, ip2 = 0.5 / p;
+  double e[LMAX + 2][LMAX + 4][2 * LMAX + 6];
+  std::memset(e, 0, sizeof(e));
+  e[0][0][0] = std::exp(-mu * (Ax - Bx) * (Ax - Bx));
+  for (int i = 0; i <= la; i++)
+    for (int j = 0; j <= jmax; j++) {
+      if (i == 0 && j == 0) continue;
+      for (int t = 0; t <= i + j; t++) {
+        const double *src = i > 0 ? e[i - 1][j] : e[i][j - 1];
+        const int top = i > 0 ? i - 1 + j : j - 1;
+        e[i][j][t] = (t > 0 ? ip2 * src[t - 1] : 0.0) + (i > 0 ? XPA : XPB) * src[t] +
+                     (t + 1 <= top ? (t + 1) * src[t + 1] : 0.0);
+      }
+    }
+  const double f = std::sqrt(PI / p);
+  for (int i = 0; i <= la; i++)
+    for (int j = 0; j <= jmax; j++) s[i][j] = e[i][j][0] * f;
+}
+
+// Property integrals of a cartesian shell pair about the point O:
+// D[x] = <A| (r - O)_x |B> and M[x] = <A| ((r - O) x grad)_x |B> (real,
+// antisymmetric: M[0] = <A| y_O d/dz - z_O d/dy |B>, ...).  Per axis, with
+// B's power j about B: (x - O_x) B_j = B_{j+1} + (B_x - O_x) B_j and
+// d/dx B_j = j B_{j-1} - 2 b B_{j+1}.
+void prop_cart(const Shell &A, const Shell &B, const double *O, double *D, double *M) {
+  int la[6][3], lb[6][3];
+  cart_list(A.l, la);
+  cart_list(B.l, lb);
+  const int nca = ncart_of(A.l), ncb = ncart_of(B.l);
+  for (int x = 0; x < 3; x++)
+    for (int k = 0; k < nca * ncb; k++) D[x * 36 + k] = M[x * 36 + k] = 0.0;
+  for (size_t i = 0; i < A.a.size(); i++)
+    for (size_t j = 0; j < B.a.size(); j++) {
+      const double a = A.a[i], b = B.a[j], cc = A.c[i] * B.c[j];
+      double s[3][LMAX + 2][LMAX + 4];
+      for (int x = 0; x < 3; x++) overlap_1d(a, b, A.A[x], B.A[x], A.l, B.l + 1, s[x]);
+      for (int ca = 0; ca < nca; ca++)
+        for (int cb = 0; cb < ncb; cb++) {
+          double ov[3], mo[3], de[3];
+          for (int x = 0; x < 3; x++) {
+            const int ii = la[ca][x], jj = lb[cb][x];
+            ov[x] = s[x][ii][jj];
+            mo[x] = s[x][ii][jj + 1] + (B.A[x] - O[x]) * s[x][ii][jj];
+            de[x] = (jj > 0 ? jj * s[x][ii][jj - 1] : 0.0) - 2.0 * b * s[x][ii][jj + 1];
+          }
+          const int k = ca * ncb + cb;
+          D[0 * 36 + k] += cc * mo[0] * ov[1] * ov[2];
+          D[1 * 36 + k] += cc * ov[0] * mo[1] * ov[2];
+          D[2 * 36 + k] += cc * ov[0] * ov[1] * mo[2];
+          M[0 * 36 + k] += cc * ov[0] * (mo[1] * de[2] - de[1] * mo[2]);
+          M[1 * 36 + k] += cc * ov[1] * (mo[2] * de[0] - de[2] * mo[0]);
+          M[2 * 36 + k] += cc * ov[2] * (mo[0] * de[1] - de[0] * mo[1]);
+        }
+    }
+}
+
 // cartesian block (na_c x nb_c) -> spherical (na_s x nb_s)
 void to_sph2(int la, int lb, const double *cart, double *sph) {
   double Ta[5][6], Tb[5][6];
@@ -472,6 +532,35 @@ int chem_one_electron(int nsh, const double *xyz, const int *l, const int *nprim
           T[k] = st[i * Bs.nsph + j];
           V[k] = sv[i * Bs.nsph + j];
         }
+    }
+  return 0;
+}
+
+// Dipole and angular-momentum integrals about origin[3]: D[3][nbf][nbf] =
+// <p| (r - O) |q>, M[3][nbf][nbf] = <p| (r - O) x grad |q> (real parts of
+// the operators, without charges or factors of -i)
+int chem_properties(int nsh, const double *xyz, const int *l, const int *nprim, const int *off, const double *exps,
+                    const double *coefs, const double *origin, double *D, double *M) {
+  for (int s = 0; s < nsh; s++)
+    if (l[s] < 0 || l[s] > LMAX) return -1;
+  Basis B = make_basis(nsh, xyz, l, nprim, off, exps, coefs);
+  const long n = B.nbf;
+#pragma omp parallel for schedule(dynamic)
+  for (int a = 0; a < nsh; a++)
+    for (int b = 0; b < nsh; b++) {
+      const Shell &A = B.sh[a], &Bs = B.sh[b];
+      double cd[3 * 36], cm[3 * 36], sd[25], sm[25];
+      prop_cart(A, Bs, origin, cd, cm);
+      for (int x = 0; x < 3; x++) {
+        to_sph2(A.l, Bs.l, cd + 36 * x, sd);
+        to_sph2(A.l, Bs.l, cm + 36 * x, sm);
+        for (int i = 0; i < A.nsph; i++)
+          for (int j = 0; j < Bs.nsph; j++) {
+            const size_t k = (size_t)x * n * n + (size_t)(A.off + i) * n + Bs.off + j;
+            D[k] = sd[i * Bs.nsph + j];
+            M[k] = sm[i * Bs.nsph + j];
+          }
+      }
     }
   return 0;
 }

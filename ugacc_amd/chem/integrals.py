@@ -29,6 +29,7 @@ def lib():
         L.chem_nbf.argtypes = [ctypes.c_int, ip]
         L.chem_one_electron.argtypes = [ctypes.c_int, dp, ip, ip, ip, dp, dp, ctypes.c_int, dp, dp, dp, dp, dp]
         L.chem_eri.argtypes = [ctypes.c_int, dp, ip, ip, ip, dp, dp, ctypes.c_double, dp]
+        L.chem_properties.argtypes = [ctypes.c_int, dp, ip, ip, ip, dp, dp, dp, dp, dp]
         L.chem_jk.argtypes = [ctypes.c_int, dp, dp, dp, dp]
         L.chem_jk.restype = None
         L.chem_num_threads.restype = ctypes.c_int
@@ -88,3 +89,16 @@ def jk(eri_ao, D):
     J, K = np.zeros((n, n)), np.zeros((n, n))
     lib().chem_jk(n, _dp(eri_ao), _dp(D), _dp(J), _dp(K))
     return J, K
+
+
+def properties(shells, origin=(0.0, 0.0, 0.0)):
+    """AO dipole integrals <p|(r - O)|q> and angular-momentum integrals
+    <p|(r - O) x grad|q> (real, antisymmetric), each (3, nbf, nbf)."""
+    sa = ShellArrays(shells)
+    n = sa.nbf
+    D, M = np.zeros((3, n, n)), np.zeros((3, n, n))
+    O = np.ascontiguousarray(origin, dtype=np.float64)
+    rc = lib().chem_properties(*sa.args(), _dp(O), _dp(D), _dp(M))
+    if rc != 0:
+        raise ValueError("angular momentum above d")
+    return D, M

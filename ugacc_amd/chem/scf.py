@@ -32,6 +32,7 @@ class RHF:
         self.atoms = atoms
         self.basis = basis
         shells = shells_for(atoms, basis)
+        self.shells = shells
         Zs = [ZTAB[a] for a, _ in atoms]
         nel = sum(Zs) - charge
         if nel % 2:
@@ -172,6 +173,23 @@ class Reference:
         self.escf = s.energy
         self.enuc = s.enuc
         self.blocks = transform_blocks(s.eri, C, nfzc, s.nocc)
+
+    def perturbation(self, op, origin=(0.0, 0.0, 0.0)):
+        """The three Cartesian components of a one-electron property operator
+        in the active MO basis (nact x nact each), as Perturbation::prop_p
+        holds them (perturbation.cc): "Mu" the dipole operator with the
+        electron charge, -(r - O) (Psi4's dipole integrals), "L" the magnetic
+        dipole -1/2 (r - O) x grad (the angular-momentum integrals without
+        their factor -i, scaled by -1/2 as perturbation.cc does)."""
+        D, M = integrals.properties(self.scf.shells, origin)
+        if op == "Mu":
+            X = -D
+        elif op == "L":
+            X = -0.5 * M
+        else:
+            raise ValueError("unknown property operator %r (Mu or L)" % op)
+        C = self.scf.C[:, self.nfzc:]
+        return [np.ascontiguousarray(C.T @ X[k] @ C) for k in range(3)]
 
     def full_ints(self):
         """nact^4 Dirac array (the reference's ints_ layout, for the oracle)."""
