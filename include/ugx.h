@@ -30,6 +30,9 @@
  *   ugx_lambda            <- CCLambda::compute_lambda, cclambda.cc:93-135
  *   ugx_tgrad             <- CCWfn::tgrad, ccwfn.cc:854-1098
  *   ugx_density           <- CCDensity::onepdm / twopdm, ccdensity.cc:89-339
+ *   ugx_pert_create       <- CCPert::CCPert + pertbar, ccpert.cc:11-176
+ *   ugx_pert_solve        <- CCPert::solve, ccpert.cc:178-225
+ *   ugx_linresp           <- CCResp::linresp, ccresp.cc:27-124
  *   UGX_NOT_CONVERGED     <- PSIEXCEPTION("Computation has not converged."),
  *                            ccwfn.cc:230-231
  *
@@ -283,6 +286,49 @@ int ugx_tgrad(ugx_ctx *ctx, double *s1, double *s2, double *Doo, double *Dvv, do
  * (ugacc.cc:129-137). */
 int ugx_density(ugx_ctx *ctx, double *energies, double *Doo, double *Dvv, double *Dov, double *Dvo, double *Goooo,
                 double *Gvvvv, double *Gooov, double *Gvvvo, double *Govov, double *Goovv);
+
+/* ---------------------------------------------------------------------
+ * Linear response (DERTYPE=SECOND, ugacc.cc:143-241).  A perturbation is a
+ * one-electron MO operator over the active orbitals (nact x nact, row-major,
+ * occupied first), such as Perturbation::prop_p (perturbation.cc): the
+ * dipole -(r - O) ("Mu") or the magnetic dipole -1/2 (r - O) x grad ("L").
+ * Each perturbation lives in a slot of the context, named by a handle,
+ * until ugx_pert_release or the next ugx_load_hamiltonian.
+ * --------------------------------------------------------------------- */
+
+/* CCPert::CCPert (ccpert.cc:11-75): the perturbation at frequency omega over
+ * the HBAR and Lambda amplitudes of the last ugx_lambda: its similarity
+ * transform (pertbar, :112-176), the HBAR-diagonal denominators and the
+ * guesses X = Y = Abar / (D + omega).  *handle receives the slot. */
+int ugx_pert_create(ugx_ctx *ctx, const double *pert, double omega, int *handle);
+
+/* CCPert::solve (ccpert.cc:178-225): hand 0 solves the right-hand perturbed
+ * amplitudes X (build_X1/X2, :382-470), hand 1 the left-hand Y (build_G,
+ * build_Y1/Y2, :489-882; needs the slot's X); x = x_old + R/(D + omega),
+ * break on the 2-norm of the raw residual below r_convergence before DIIS,
+ * DIIS as in ugx_ccsd.  Options read: maxiter, diis, r_convergence,
+ * diis_nvec, diis_min_vecs, diis_policy, print (NULL: the defaults).
+ * Outputs (may be NULL): the pseudoresponse (:229-249), iterations, final
+ * residual norm.  UGX_NOT_CONVERGED with "Computation has not converged.". */
+int ugx_pert_solve(ugx_ctx *ctx, int handle, int hand, const ugx_options *opt, double *pseudoresponse, int *niter,
+                   double *rms);
+
+/* The slot's perturbed amplitudes X1[i][a], X2[i][j][a][b], Y1, Y2 (any
+ * pointer may be NULL); Y holds its guess until a left-hand solve. */
+int ugx_pert_amplitudes(ugx_ctx *ctx, int handle, double *X1, double *X2, double *Y1, double *Y2);
+
+/* One residual evaluation of the slot's equations at amplitudes (A1, A2):
+ * hand 0 build_X1/X2, hand 1 build_G/Y1/Y2 (with the slot's X); raw R1 and
+ * R2 (before the denominators).  Outputs may be NULL. */
+int ugx_pert_residual(ugx_ctx *ctx, int handle, int hand, const double *A1, const double *A2, double *R1, double *R2);
+
+/* CCResp::linresp (ccresp.cc:27-124): <<A;B>> with A the perturbation of
+ * slot hx and B the amplitudes of slot hy (its X, and its Y as they are:
+ * the guess after a right-hand solve only, as with MYHAND=RIGHT). */
+int ugx_linresp(ugx_ctx *ctx, int hx, int hy, double *value);
+
+/* Frees a perturbation slot. */
+int ugx_pert_release(ugx_ctx *ctx, int handle);
 
 /* Device time per contraction label of the last ugx_ccsd / ugx_triples
  * call when profiling is on, as "label launches ms gflop\n" lines.  Returns

@@ -39,6 +39,29 @@
 using namespace ugx;
 
 
+// One perturbed wave function of the linear response (ccpert.cc): the MO
+// perturbation's blocks P*, its similarity transform A*, and the right- and
+// left-hand perturbed amplitudes with their previous values, by name;
+// aliases point names at caller arrays for one residual evaluation.
+struct PertSlot {
+  double omega = 0.0;
+  bool x_solved = false;
+  std::map<std::string, std::unique_ptr<ugx::DBuf>> bufs;
+  std::map<std::string, const double *> aliases;
+  double *buf(const char *n) {
+    auto it = bufs.find(n);
+    if (it == bufs.end()) throw std::runtime_error(std::string("ugx: response slot has no ") + n);
+    return it->second->d();
+  }
+  void make(const char *n, size_t elems) {
+    auto &b = bufs[n];
+    if (!b) b.reset(new ugx::DBuf());
+    b->alloc(elems * 8);
+  }
+  void swap(const char *a, const char *b) { std::swap(bufs[a], bufs[b]); }
+  void alias(const char *n, const double *p) { aliases[n] = p; }
+};
+
 struct ugx_ctx {
   int device = 0;
   hipStream_t st = nullptr;
@@ -158,6 +181,13 @@ struct ugx_ctx {
   // <zy|an> and L_xyan with the slice index first
   DBuf tg_T, tg_L, tg_Y2a, tg_Y2b, tg_Y2l, tg_X, tg_Gt, tg_t1s, tg_t2s, tg_ooovT, tg_LooovT;
   bool tg_valid = false, lam_with_t = false;
+  // linear response: perturbation slots (ugx_pert_*), the scratch of their
+  // residuals (R1, half Z, the Y source S1/S2, W), the HBAR diagonals as
+  // denominators, and named intermediates of the term tables
+  std::vector<std::unique_ptr<PertSlot>> perts;
+  DBuf rp_R1, rp_S1, rp_Z, rp_S2, rp_W, rp_eps, rp_A1, rp_A2;
+  std::map<std::string, std::unique_ptr<DBuf>> rp_tmp;
+  std::map<std::string, long> rp_tmp_size;
   // residual of the last ugx_residual, for ugx_intermediates
   bool have_intermediates = false;
   double last_flops = 0.0;
@@ -989,8 +1019,11 @@ void lambda_alloc(ugx_ctx *c) {
 // build_G (cclambda.cc:137-156), build_l1 (:158-210) and build_l2 (:212-268)
 // at (l1, l2): the raw l1 residual into R1 and the half residual Z of l2
 // (R2_ijab = Z_ijab + Z_jiba), Goo/Gvv into lam_Goo/lam_Gvv.
-// with_t: the CCSD(T) sources s1, s2/2 of tgrad() enter the residuals
-void lambda_residual(ugx_ctx *c, const double *l1p, const double *l2p, double *R1p, double *Zp, bool with_t) {
+// with_t: the CCSD(T) sources s1, s2/2 of tgrad() enter the residuals;
+// inhom = false drops the inhomogeneous terms 2 Hov and L(oovv) (the left-hand
+// perturbed equations apply the same operator to their amplitudes)
+void lambda_residual(ugx_ctx *c, const double *l1p, const double *l2p, double *R1p, double *Zp, bool with_t,
+                     bool inhom = true) {
   Engine &E = *c->eng;
   const int o = c->no, v = c->nv;
   const int P = PREC_FP64;
@@ -1011,8 +1044,8 @@ void lambda_residual(ugx_ctx *c, const double *l1p, const double *l2p, double *R
   K("mjab,ijab->mi", 1.0, T2, L2, 0.0, GOO);
   K("ijeb,ijab->ae", -1.0, T2, L2, 0.0, GVV);
   // build_l1
-  E.permute("ia->ia", 2.0, HOV, 0.0, R1);
-  if (with_t) E.permute("ia->ia", 1.0, V(c->tg_s1.d(), {o, v}), 1.0, R1);  // cclambda.cc:181
+  E.permute("ia->ia", inhom ? 2.0 : 0.0, HOV, 0.0, R1);
+  if (with_t && inhom) E.permute("ia->ia", 1.0, V(c->tg_s1.d(), {o, v}), 1.0, R1);  // cclambda.cc:181
   K("ie,ea->ia", 1.0, L1, HVV, 1.0, R1);
   K("ma,im->ia", -1.0, L1, HOO, 1.0, R1);
   K("me,ieam->ia", 2.0, L1, HOVVO, 1.0, R1);
@@ -1024,8 +1057,8 @@ void lambda_residual(ugx_ctx *c, const double *l1p, const double *l2p, double *R
   K("mn,mina->ia", -2.0, GOO, HOOOV, 1.0, R1);
   K("mn,imna->ia", 1.0, GOO, HOOOV, 1.0, R1);
   // build_l2, Z before the pair symmetrization
-  E.permute("ijab->ijab", 1.0, LOOVV, 0.0, Z);
-  if (with_t) E.permute("ijab->ijab", 0.5, V(c->tg_s2.d(), {o, o, v, v}), 1.0, Z);  // cclambda.cc:243
+  E.permute("ijab->ijab", inhom ? 1.0 : 0.0, LOOVV, 0.0, Z);
+  if (with_t && inhom) E.permute("ijab->ijab", 0.5, V(c->tg_s2.d(), {o, o, v, v}), 1.0, Z);  // cclambda.cc:243
   if (!E.sizing()) launch_lambda_outer(o, v, l1p, c->hb_ov.d(), Zp, c->st);
   K("ijeb,ea->ijab", 1.0, L2, HVV, 1.0, Z);
   K("mjab,im->ijab", -1.0, L2, HOO, 1.0, Z);
@@ -1482,6 +1515,486 @@ void density_energies(ugx_ctx *c, double *E) {
   E[6] = device_dot(c, ov * ov, c->ovov.d(), c->dn_ovov.d());
   E[7] = 0.5 * device_dot(c, ov * ov, c->oovv.d(), c->dn_oovv.d());
   E[1] = E[2] + E[3] + E[4] + E[5] + E[6] + E[7];
+}
+
+// ------------------------------------------------------ linear response
+// CCPert (ccpert.cc) and CCResp::linresp (ccresp.cc:27-124) in FP64 over the
+// HBAR and Lambda amplitudes of the last ugx_lambda.  Each perturbation slot
+// holds the MO perturbation's blocks, its similarity transform (pertbar,
+// ccpert.cc:112-176), the right-hand (X) and left-hand (Y) perturbed
+// amplitudes and their previous values.  The terms of pertbar, of the X
+// residual (build_X1/X2, :382-470) and of the part of the Y residual that
+// does not depend on Y (the terms of build_Y1/Y2 in the right-hand
+// amplitudes, the Lambda amplitudes and the perturbation, the reference's (i)
+// .. (x), :516-882) are tables of einsum contractions; the Y-dependent part
+// is the Lambda operator (build_G/l1/l2 without their inhomogeneous terms),
+// so each Y iteration costs one Lambda residual.  Index letters fix the
+// extents: i j k l m n o occupied, a b c d e f g virtual.
+struct PTerm {
+  const char *spec;
+  double alpha;
+  const char *a, *b;  // b null: a permute of a into c
+  double beta;
+  const char *c;
+};
+
+// pertbar, ccpert.cc:112-176 (Aov is the perturbation's ov block itself)
+const PTerm kPertBar[] = {
+    {"mi->mi", 1.0, "Poo", nullptr, 0.0, "Aoo"},
+    {"ie,me->mi", 1.0, "t1", "Pov", 1.0, "Aoo"},
+    {"ae->ae", 1.0, "Pvv", nullptr, 0.0, "Avv"},
+    {"ma,me->ae", -1.0, "t1", "Pov", 1.0, "Avv"},
+    {"ai->ai", 1.0, "Pvo", nullptr, 0.0, "Avo"},
+    {"ie,ae->ai", 1.0, "t1", "Pvv", 1.0, "Avo"},
+    {"ma,mi->ai", -1.0, "t1", "Poo", 1.0, "Avo"},
+    {"miea,me->ai", 2.0, "t2", "Pov", 1.0, "Avo"},
+    {"imea,me->ai", -1.0, "t2", "Pov", 1.0, "Avo"},
+    {"ma,me->ae", 1.0, "t1", "Pov", 0.0, "Qae"},
+    {"ie,ae->ai", -1.0, "t1", "Qae", 1.0, "Avo"},
+    {"ijeb,me->mbij", 1.0, "t2", "Pov", 0.0, "Aovoo"},
+    {"miab,me->abei", -1.0, "t2", "Pov", 0.0, "Avvvo"},
+    {"ijeb,ae->abij", 1.0, "t2", "Avv", 0.0, "Avvoo"},
+    {"mjab,mi->abij", -1.0, "t2", "Aoo", 1.0, "Avvoo"},
+};
+
+// build_X1 / build_X2 (ccpert.cc:382-470) at (X1o, X2o) without the omega
+// terms: R1 and the half residual Z (R2 = Z + Z^T)
+const PTerm kPertX[] = {
+    {"ai->ia", 1.0, "Avo", nullptr, 0.0, "R1"},
+    {"ie,ae->ia", 1.0, "X1o", "Hvv", 1.0, "R1"},
+    {"ma,mi->ia", -1.0, "X1o", "Hoo", 1.0, "R1"},
+    {"me,maei->ia", 2.0, "X1o", "Hovvo", 1.0, "R1"},
+    {"me,maie->ia", -1.0, "X1o", "Hovov", 1.0, "R1"},
+    {"me,miea->ia", 2.0, "Hov", "X2o", 1.0, "R1"},
+    {"me,imea->ia", -1.0, "Hov", "X2o", 1.0, "R1"},
+    {"imef,amef->ia", 2.0, "X2o", "Hvovv", 1.0, "R1"},
+    {"imef,amfe->ia", -1.0, "X2o", "Hvovv", 1.0, "R1"},
+    {"mnae,mnie->ia", -2.0, "X2o", "Hooov", 1.0, "R1"},
+    {"mnae,nmie->ia", 1.0, "X2o", "Hooov", 1.0, "R1"},
+    {"amef,mf->ae", 2.0, "Hvovv", "X1o", 0.0, "Zvv"},
+    {"amfe,mf->ae", -1.0, "Hvovv", "X1o", 1.0, "Zvv"},
+    {"mnef,mnaf->ae", -1.0, "Loovv", "X2o", 1.0, "Zvv"},
+    {"mnie,ne->mi", -2.0, "Hooov", "X1o", 0.0, "Zoo"},
+    {"nmie,ne->mi", 1.0, "Hooov", "X1o", 1.0, "Zoo"},
+    {"mnef,inef->mi", -1.0, "Loovv", "X2o", 1.0, "Zoo"},
+    {"abij->ijab", 1.0, "Avvoo", nullptr, 0.0, "Z"},
+    {"ie,abej->ijab", 1.0, "X1o", "Hvvvo", 1.0, "Z"},
+    {"ma,mbij->ijab", -1.0, "X1o", "Hovoo", 1.0, "Z"},
+    {"mi,mjab->ijab", 1.0, "Zoo", "t2", 1.0, "Z"},
+    {"ae,ijeb->ijab", 1.0, "Zvv", "t2", 1.0, "Z"},
+    {"ijeb,ae->ijab", 1.0, "X2o", "Hvv", 1.0, "Z"},
+    {"mjab,mi->ijab", -1.0, "X2o", "Hoo", 1.0, "Z"},
+    {"mnab,mnij->ijab", 0.5, "X2o", "Hoooo", 1.0, "Z"},
+    {"ijef,abef->ijab", 0.5, "X2o", "Hvvvv", 1.0, "Z"},
+    {"imeb,maje->ijab", -1.0, "X2o", "Hovov", 1.0, "Z"},
+    {"imea,mbej->ijab", -1.0, "X2o", "Hovvo", 1.0, "Z"},
+    {"miea,mbej->ijab", 2.0, "X2o", "Hovvo", 1.0, "Z"},
+    {"miea,mbje->ijab", -1.0, "X2o", "Hovov", 1.0, "Z"},
+};
+
+// The Y-independent terms of build_Y1 (into S1) and build_Y2 (into the half
+// S2), ccpert.cc:516-882, from the converged X1/X2, l1/l2, the perturbation
+// and HBAR; products of three or four factors go through the intermediates
+// Q*, P*, T*, U*, X* (at most o v^3 elements)
+const PTerm kPertYSource[] = {
+    {"ia->ia", 2.0, "Pov", nullptr, 0.0, "S1"},
+    // (i)
+    {"me,imae->ia", 2.0, "X1", "Loovv", 1.0, "S1"},
+    // (m), (n)
+    {"ma,im->ia", -1.0, "l1", "Aoo", 1.0, "S1"},
+    {"ie,ea->ia", 1.0, "l1", "Avv", 1.0, "S1"},
+    {"imfe,feam->ia", 1.0, "l2", "Avvvo", 1.0, "S1"},
+    {"mnea,ienm->ia", -0.5, "l2", "Aovoo", 1.0, "S1"},
+    {"mnae,iemn->ia", -0.5, "l2", "Aovoo", 1.0, "S1"},
+    // (q)
+    {"me,ma->ea", 1.0, "X1", "Hov", 0.0, "Q1"},
+    {"ie,ea->ia", -1.0, "l1", "Q1", 1.0, "S1"},
+    {"me,ie->mi", 1.0, "X1", "Hov", 0.0, "Q2"},
+    {"ma,mi->ia", -1.0, "l1", "Q2", 1.0, "S1"},
+    {"me,ne->mn", 1.0, "X1", "l1", 0.0, "Q3"},
+    {"mn,mina->ia", -2.0, "Q3", "Hooov", 1.0, "S1"},
+    {"mn,imna->ia", 1.0, "Q3", "Hooov", 1.0, "S1"},
+    {"me,imne->in", 2.0, "X1", "Hooov", 0.0, "Q4"},
+    {"me,mine->in", -1.0, "X1", "Hooov", 1.0, "Q4"},
+    {"in,na->ia", -1.0, "Q4", "l1", 1.0, "S1"},
+    {"me,fmae->fa", 2.0, "X1", "Hvovv", 0.0, "Q5"},
+    {"me,fmea->fa", -1.0, "X1", "Hvovv", 1.0, "Q5"},
+    {"if,fa->ia", 1.0, "l1", "Q5", 1.0, "S1"},
+    {"me,mf->ef", 1.0, "X1", "l1", 0.0, "Q6"},
+    {"ef,fiea->ia", 2.0, "Q6", "Hvovv", 1.0, "S1"},
+    {"ef,fiae->ia", -1.0, "Q6", "Hvovv", 1.0, "S1"},
+    // (r)
+    {"mnef,nf->me", 1.0, "X2", "l1", 0.0, "Q7"},
+    {"me,imae->ia", 2.0, "Q7", "Loovv", 1.0, "S1"},
+    {"mnef,ne->mf", 1.0, "X2", "l1", 0.0, "Q8"},
+    {"mf,imaf->ia", -1.0, "Q8", "Loovv", 1.0, "S1"},
+    {"mnef,mief->ni", 1.0, "X2", "Loovv", 0.0, "Q9"},
+    {"ni,na->ia", -1.0, "Q9", "l1", 1.0, "S1"},
+    {"mnef,nmfa->ea", 1.0, "X2", "Loovv", 0.0, "Q10"},
+    {"ie,ea->ia", -1.0, "l1", "Q10", 1.0, "S1"},
+    // (s)
+    {"me,mfna->efna", 1.0, "X1", "Hovov", 0.0, "Tvvov"},
+    {"efna,nief->ia", -1.0, "Tvvov", "l2", 1.0, "S1"},
+    {"me,ifne->mifn", 1.0, "X1", "Hovov", 0.0, "Tooo1"},
+    {"mifn,nmaf->ia", -1.0, "Tooo1", "l2", 1.0, "S1"},
+    {"me,mfan->efan", 1.0, "X1", "Hovvo", 0.0, "Tvvvo"},
+    {"efan,nife->ia", -1.0, "Tvvvo", "l2", 1.0, "S1"},
+    {"me,ifen->mifn", 1.0, "X1", "Hovvo", 0.0, "Tooo1"},
+    {"mifn,nmfa->ia", -1.0, "Tooo1", "l2", 1.0, "S1"},
+    {"imfg,fgae->imae", 1.0, "l2", "Hvvvv", 0.0, "T2"},
+    {"imae,me->ia", 0.5, "T2", "X1", 1.0, "S1"},
+    {"mifg,fgea->miea", 1.0, "l2", "Hvvvv", 0.0, "T2"},
+    {"miea,me->ia", 0.5, "T2", "X1", 1.0, "S1"},
+    {"imno,noae->imae", 1.0, "Hoooo", "l2", 0.0, "T2"},
+    {"imae,me->ia", 0.5, "T2", "X1", 1.0, "S1"},
+    {"mino,noea->miea", 1.0, "Hoooo", "l2", 0.0, "T2"},
+    {"miea,me->ia", 0.5, "T2", "X1", 1.0, "S1"},
+    // the four-factor terms, ccpert.cc:629-640
+    {"jmfe,jmbe->fb", 1.0, "t2", "l2", 0.0, "P1"},
+    {"nb,fb->nf", 1.0, "X1", "P1", 0.0, "P2"},
+    {"nf,inaf->ia", -1.0, "P2", "Loovv", 1.0, "S1"},
+    {"jnfb,jnab->fa", 1.0, "t2", "l2", 0.0, "P3"},
+    {"me,mief->if", 1.0, "X1", "Loovv", 0.0, "P4"},
+    {"if,fa->ia", -1.0, "P4", "P3", 1.0, "S1"},
+    {"jnfb,jifb->ni", 1.0, "t2", "l2", 0.0, "P5"},
+    {"me,mnea->na", 1.0, "X1", "Loovv", 0.0, "P6"},
+    {"ni,na->ia", -1.0, "P5", "P6", 1.0, "S1"},
+    {"nmeb,jmeb->nj", 1.0, "t2", "l2", 0.0, "P7"},
+    {"nj,jf->nf", 1.0, "P7", "X1", 0.0, "P8"},
+    {"nf,inaf->ia", -1.0, "P8", "Loovv", 1.0, "S1"},
+    // (t)
+    {"mnef,nife->mi", 1.0, "X2", "l2", 0.0, "Q14"},
+    {"mi,ma->ia", -1.0, "Q14", "Hov", 1.0, "S1"},
+    {"mnef,nmfa->ea", 1.0, "X2", "l2", 0.0, "Q15"},
+    {"ie,ea->ia", -1.0, "Hov", "Q15", 1.0, "S1"},
+    {"mnef,imfg->ineg", 1.0, "X2", "l2", 0.0, "T2"},
+    {"ineg,gnea->ia", -1.0, "T2", "Hvovv", 1.0, "S1"},
+    {"mnef,mifg->ineg", 1.0, "X2", "l2", 0.0, "T2"},
+    {"ineg,gnae->ia", -1.0, "T2", "Hvovv", 1.0, "S1"},
+    {"mnef,gief->mngi", 1.0, "X2", "Hvovv", 0.0, "Tooo1"},
+    {"mngi,nmag->ia", -1.0, "Tooo1", "l2", 1.0, "S1"},
+    {"mnef,nifg->meig", 1.0, "X2", "l2", 0.0, "T2"},
+    {"meig,gmae->ia", 2.0, "T2", "Hvovv", 1.0, "S1"},
+    {"meig,gmea->ia", -1.0, "T2", "Hvovv", 1.0, "S1"},
+    {"mnef,nmfg->eg", 1.0, "X2", "l2", 0.0, "Q20"},
+    {"eg,giea->ia", 2.0, "Q20", "Hvovv", 1.0, "S1"},
+    {"eg,giae->ia", -1.0, "Q20", "Hvovv", 1.0, "S1"},
+    {"mnef,oief->mnoi", 1.0, "X2", "l2", 0.0, "Toooo"},
+    {"mnoi,mnoa->ia", 1.0, "Toooo", "Hooov", 1.0, "S1"},
+    {"mnef,omaf->neoa", 1.0, "X2", "l2", 0.0, "T2"},
+    {"neoa,inoe->ia", 1.0, "T2", "Hooov", 1.0, "S1"},
+    {"mnef,onea->mfoa", 1.0, "X2", "l2", 0.0, "T2"},
+    {"mfoa,miof->ia", 1.0, "T2", "Hooov", 1.0, "S1"},
+    {"mnef,nofe->mo", 1.0, "X2", "l2", 0.0, "Q24"},
+    {"mo,mioa->ia", -2.0, "Q24", "Hooov", 1.0, "S1"},
+    {"mo,imoa->ia", 1.0, "Q24", "Hooov", 1.0, "S1"},
+    {"mnef,nofa->meoa", 1.0, "X2", "l2", 0.0, "T2"},
+    {"meoa,imoe->ia", -2.0, "T2", "Hooov", 1.0, "S1"},
+    {"meoa,mioe->ia", 1.0, "T2", "Hooov", 1.0, "S1"},
+    // build_Y2: (o), (p)
+    {"jb,ia->ijab", 2.0, "l1", "Pov", 0.0, "S2"},
+    {"ib,ja->ijab", -1.0, "l1", "Pov", 1.0, "S2"},
+    {"ijeb,ea->ijab", 1.0, "l2", "Avv", 1.0, "S2"},
+    {"mjab,im->ijab", -1.0, "l2", "Aoo", 1.0, "S2"},
+    // (u)
+    {"me,mieb->ib", 1.0, "X1", "Loovv", 0.0, "U1"},
+    {"ja,ib->ijab", -1.0, "l1", "U1", 1.0, "S2"},
+    {"ijae,eb->ijab", -1.0, "Loovv", "Q6", 1.0, "S2"},
+    {"me,ie->im", 1.0, "X1", "l1", 0.0, "U3"},
+    {"im,jmba->ijab", -1.0, "U3", "Loovv", 1.0, "S2"},
+    {"me,imae->ia", 1.0, "X1", "Loovv", 0.0, "U4"},
+    {"jb,ia->ijab", 2.0, "l1", "U4", 1.0, "S2"},
+    // (w)
+    {"jibe,ea->ijab", -1.0, "l2", "Q1", 1.0, "S2"},
+    {"mi,jmba->ijab", -1.0, "Q2", "l2", 1.0, "S2"},
+    {"jife,me->jifm", 1.0, "l2", "X1", 0.0, "Tooo1"},
+    {"jifm,fmba->ijab", -1.0, "Tooo1", "Hvovv", 1.0, "S2"},
+    {"me,mifb->eifb", 1.0, "X1", "l2", 0.0, "Tvovv"},
+    {"eifb,fjea->ijab", -1.0, "Tvovv", "Hvovv", 1.0, "S2"},
+    {"me,jmfa->jefa", 1.0, "X1", "l2", 0.0, "Tvovv"},
+    {"jefa,fibe->ijab", -1.0, "Tvovv", "Hvovv", 1.0, "S2"},
+    {"jibf,fa->ijab", 1.0, "l2", "Q5", 1.0, "S2"},
+    {"me,jmbf->jebf", 1.0, "X1", "l2", 0.0, "Tvovv"},
+    {"jebf,fiea->ijab", 2.0, "Tvovv", "Hvovv", 1.0, "S2"},
+    {"jebf,fiae->ijab", -1.0, "Tvovv", "Hvovv", 1.0, "S2"},
+    {"nibe,me->nibm", 1.0, "l2", "X1", 0.0, "Tooo1"},
+    {"nibm,jmna->ijab", 1.0, "Tooo1", "Hooov", 1.0, "S2"},
+    {"nieb,me->nibm", 1.0, "l2", "X1", 0.0, "Tooo1"},
+    {"nibm,mjna->ijab", 1.0, "Tooo1", "Hooov", 1.0, "S2"},
+    {"jine,me->jinm", 1.0, "Hooov", "X1", 0.0, "Toooo"},
+    {"jinm,nmba->ijab", 1.0, "Toooo", "l2", 1.0, "S2"},
+    {"jnbe,me->jnbm", 1.0, "l2", "X1", 0.0, "Tooo1"},
+    {"jnbm,mina->ijab", -2.0, "Tooo1", "Hooov", 1.0, "S2"},
+    {"jnbm,imna->ijab", 1.0, "Tooo1", "Hooov", 1.0, "S2"},
+    {"in,jnba->ijab", -1.0, "Q4", "l2", 1.0, "S2"},
+    // (x)
+    {"mnef,jief->mnji", 1.0, "X2", "l2", 0.0, "Toooo"},
+    {"mnji,nmab->ijab", 0.25, "Toooo", "oovv", 1.0, "S2"},
+    {"mnef,jife->mnji", 1.0, "X2", "l2", 0.0, "Toooo"},
+    {"mnji,mnab->ijab", 0.25, "Toooo", "oovv", 1.0, "S2"},
+    {"mnef,mifb->neib", 1.0, "X2", "l2", 0.0, "T2"},
+    {"neib,jnae->ijab", 0.25, "T2", "oovv", 1.0, "S2"},
+    {"mnef,nieb->mfib", 1.0, "X2", "l2", 0.0, "T2"},
+    {"mfib,jmaf->ijab", 0.25, "T2", "oovv", 1.0, "S2"},
+    {"mnef,mibf->neib", 1.0, "X2", "l2", 0.0, "T2"},
+    {"neib,njae->ijab", 0.25, "T2", "oovv", 1.0, "S2"},
+    {"mnef,nibe->mfib", 1.0, "X2", "l2", 0.0, "T2"},
+    {"mfib,mjaf->ijab", 0.25, "T2", "oovv", 1.0, "S2"},
+    {"mnef,imfb->neib", 1.0, "X2", "l2", 0.0, "T2"},
+    {"neib,jnea->ijab", 0.25, "T2", "oovv", 1.0, "S2"},
+    {"mnef,ineb->mfib", 1.0, "X2", "l2", 0.0, "T2"},
+    {"mfib,jmfa->ijab", 0.25, "T2", "oovv", 1.0, "S2"},
+    {"mnef,imbf->neib", 1.0, "X2", "l2", 0.0, "T2"},
+    {"neib,njea->ijab", 0.25, "T2", "oovv", 1.0, "S2"},
+    {"mnef,inbe->mfib", 1.0, "X2", "l2", 0.0, "T2"},
+    {"mfib,mjfa->ijab", 0.25, "T2", "oovv", 1.0, "S2"},
+    {"jief,mnef->jimn", 1.0, "oovv", "X2", 0.0, "Toooo"},
+    {"jimn,nmab->ijab", 0.25, "Toooo", "l2", 1.0, "S2"},
+    {"jife,mnef->jimn", 1.0, "oovv", "X2", 0.0, "Toooo"},
+    {"jimn,mnab->ijab", 0.25, "Toooo", "l2", 1.0, "S2"},
+    {"mnef,jnbe->mfjb", 1.0, "X2", "l2", 0.0, "T2"},
+    {"mfjb,imaf->ijab", -0.5, "T2", "Loovv", 1.0, "S2"},
+    {"mnef,jmbf->nejb", 1.0, "X2", "l2", 0.0, "T2"},
+    {"nejb,inae->ijab", -0.5, "T2", "Loovv", 1.0, "S2"},
+    {"mief,mnef->in", 1.0, "Loovv", "X2", 0.0, "Xo"},
+    {"in,jnba->ijab", -0.5, "Xo", "l2", 1.0, "S2"},
+    {"mnea,mnef->fa", 1.0, "Loovv", "X2", 0.0, "Xp"},
+    {"jibf,fa->ijab", -0.5, "l2", "Xp", 1.0, "S2"},
+    {"jibe,ea->ijab", -0.5, "l2", "Q10", 1.0, "S2"},
+    {"nife,mnef->im", 1.0, "Loovv", "X2", 0.0, "Xr"},
+    {"im,jmba->ijab", -0.5, "Xr", "l2", 1.0, "S2"},
+    {"ijae,eb->ijab", -1.0, "Loovv", "Q15", 1.0, "S2"},
+    {"imab,mj->ijab", -1.0, "Loovv", "Q14", 1.0, "S2"},
+    {"mnef,nifb->meib", 1.0, "X2", "l2", 0.0, "T2"},
+    {"meib,mjea->ijab", -1.0, "T2", "Loovv", 1.0, "S2"},
+    {"mejb,imae->ijab", 2.0, "T2", "Loovv", 1.0, "S2"},
+};
+
+// extent of an index letter
+inline int letter_extent(const ugx_ctx *c, char ch) {
+  if (std::strchr("ijklmno", ch)) return c->no;
+  if (std::strchr("abcdefgh", ch)) return c->nv;
+  throw std::runtime_error(std::string("ugx: index letter with no extent: ") + ch);
+}
+
+std::vector<int> letter_dims(const ugx_ctx *c, const std::string &l) {
+  std::vector<int> d;
+  for (char ch : l) d.push_back(letter_extent(c, ch));
+  return d;
+}
+
+// The operands of a slot's tables that are not scratch: fixed names, then
+// the slot's aliases and buffers; null for a scratch intermediate
+double *pert_operand(ugx_ctx *c, PertSlot &P, const std::string &n) {
+  static const char *const fixed[] = {"t1",    "t2",    "l1",    "l2",    "Loovv", "oovv",  "Hov",
+                                      "Hoo",   "Hvv",   "Hoooo", "Hvvvv", "Hvovv", "Hooov", "Hovvo",
+                                      "Hovov", "Hvvvo", "Hovoo", "R1",    "Z",     "S1",    "S2"};
+  DBuf *const bufs[] = {&c->t1,      &c->t2,      &c->lam_l1,  &c->lam_l2,  &c->Loovv,   &c->oovv,    &c->hb_ov,
+                        &c->hb_oo,   &c->hb_vv,   &c->hb_oooo, &c->hb_vvvv, &c->hb_vovv, &c->hb_ooov, &c->hb_ovvo,
+                        &c->hb_ovov, &c->hb_vvvo, &c->hb_ovoo, &c->rp_R1,   &c->rp_Z,    &c->rp_S1,   &c->rp_S2};
+  static_assert(sizeof(fixed) / sizeof(fixed[0]) == sizeof(bufs) / sizeof(bufs[0]), "one buffer per name");
+  for (size_t k = 0; k < sizeof(fixed) / sizeof(fixed[0]); k++)
+    if (n == fixed[k]) return bufs[k]->d();
+  auto at = P.aliases.find(n);
+  if (at != P.aliases.end()) return const_cast<double *>(at->second);
+  auto it = P.bufs.find(n);
+  if (it != P.bufs.end()) return it->second->d();
+  return nullptr;
+}
+
+// Every operand of a slot's tables: pert_operand, then the context's scratch
+// intermediates
+double *pert_name(ugx_ctx *c, PertSlot &P, const std::string &n) {
+  if (double *p = pert_operand(c, P, n)) return p;
+  auto jt = c->rp_tmp.find(n);
+  if (jt != c->rp_tmp.end()) return jt->second->d();
+  throw std::runtime_error("ugx: response operand without a buffer: " + n);
+}
+
+// Parses "A,B->C" / "A->C" into the letters of each operand.
+void split_spec(const std::string &s, std::string &a, std::string &b, std::string &cc) {
+  const size_t arrow = s.find("->"), comma = s.find(',');
+  cc = s.substr(arrow + 2);
+  if (comma == std::string::npos || comma > arrow) {
+    a = s.substr(0, arrow);
+    b.clear();
+  } else {
+    a = s.substr(0, comma);
+    b = s.substr(comma + 1, arrow - comma - 1);
+  }
+}
+
+// Allocates every scratch intermediate a table names (its largest use), so a
+// table's run allocates nothing.
+void size_terms(ugx_ctx *c, PertSlot &P, const PTerm *t, int n) {
+  auto need = [&](const char *name, const std::string &letters) {
+    // a scratch buffer from an earlier, smaller problem is resized here too
+    if (!name || pert_operand(c, P, name)) return;
+    long sz = 1;
+    for (int d : letter_dims(c, letters)) sz *= d;
+    auto &slot = c->rp_tmp_size[name];
+    slot = std::max(slot, sz);
+  };
+  for (int k = 0; k < n; k++) {
+    std::string a, b, cc;
+    split_spec(t[k].spec, a, b, cc);
+    need(t[k].c, cc);
+  }
+  for (auto &kv : c->rp_tmp_size) {
+    auto &buf = c->rp_tmp[kv.first];
+    if (!buf) buf.reset(new DBuf());
+    buf->alloc((size_t)kv.second * 8);
+  }
+}
+
+void run_terms(ugx_ctx *c, PertSlot &P, const PTerm *t, int n) {
+  Engine &E = *c->eng;
+  size_terms(c, P, t, n);
+  for (int k = 0; k < n; k++) {
+    std::string a, b, cc;
+    split_spec(t[k].spec, a, b, cc);
+    const TRef A = vref(pert_name(c, P, t[k].a), letter_dims(c, a));
+    const TRef C = vref(pert_name(c, P, t[k].c), letter_dims(c, cc));
+    if (!t[k].b) {
+      E.permute(a + "->" + cc, t[k].alpha, A, t[k].beta, C);
+    } else {
+      const TRef B = vref(pert_name(c, P, t[k].b), letter_dims(c, b));
+      E.contract(t[k].spec, t[k].alpha, A, B, t[k].beta, C, PREC_FP64, "response");
+    }
+  }
+}
+
+void pert_alloc_scratch(ugx_ctx *c) {
+  const long ov = c->ov(), oovv = c->oovv_n();
+  for (DBuf *b : {&c->rp_R1, &c->rp_S1}) b->alloc(ov * 8);
+  for (DBuf *b : {&c->rp_Z, &c->rp_S2, &c->rp_W}) b->alloc(oovv * 8);
+  c->rp_eps.alloc((size_t)(c->no + c->nv) * 8);
+}
+
+// The HBAR diagonals as "orbital energies": the perturbed denominators are
+// Hoo_ii - Hvv_aa (+ ...) (ccpert.cc:44-56)
+void pert_denominators(ugx_ctx *c) {
+  const int o = c->no, v = c->nv;
+  check(hipMemcpy2DAsync(c->rp_eps.p, 8, c->hb_oo.p, (size_t)(o + 1) * 8, 8, o, hipMemcpyDeviceToDevice, c->st),
+        "Hoo diagonal");
+  check(hipMemcpy2DAsync(c->rp_eps.d() + o, 8, c->hb_vv.p, (size_t)(v + 1) * 8, 8, v, hipMemcpyDeviceToDevice,
+                         c->st),
+        "Hvv diagonal");
+}
+
+// build_X1/X2 (hand 0) or build_G/Y1/Y2 (hand 1) at (A1, A2) into rp_R1 and
+// the half residual rp_Z; the Y source (rp_S1, rp_S2) must be current
+void pert_residual(ugx_ctx *c, PertSlot &P, int hand, const double *A1, const double *A2) {
+  const long ov = c->ov(), oovv = c->oovv_n();
+  if (hand == 0) {
+    P.alias("X1o", A1);
+    P.alias("X2o", A2);
+    run_terms(c, P, kPertX, (int)(sizeof(kPertX) / sizeof(kPertX[0])));
+    launch_axpby(ov, -P.omega, A1, 1.0, c->rp_R1.d(), c->st);
+    launch_axpby(oovv, -0.5 * P.omega, A2, 1.0, c->rp_Z.d(), c->st);
+  } else {
+    lambda_residual(c, A1, A2, c->rp_R1.d(), c->rp_Z.d(), false, false);
+    launch_axpby(ov, P.omega, A1, 1.0, c->rp_R1.d(), c->st);
+    launch_axpby(ov, 1.0, c->rp_S1.d(), 1.0, c->rp_R1.d(), c->st);
+    launch_axpby(oovv, 0.5 * P.omega, A2, 1.0, c->rp_Z.d(), c->st);
+    launch_axpby(oovv, 1.0, c->rp_S2.d(), 1.0, c->rp_Z.d(), c->st);
+  }
+}
+
+void pert_y_source(ugx_ctx *c, PertSlot &P) {
+  run_terms(c, P, kPertYSource, (int)(sizeof(kPertYSource) / sizeof(kPertYSource[0])));
+}
+
+// CCPert::pseudoresponse, ccpert.cc:229-249
+double pert_pseudo(ugx_ctx *c, PertSlot &P, int hand) {
+  const int o = c->no, v = c->nv;
+  Engine &E = *c->eng;
+  const double *X1 = hand == 0 ? P.buf("X1") : P.buf("Y1"), *X2 = hand == 0 ? P.buf("X2") : P.buf("Y2");
+  const auto W = vref(c->rp_W.d(), {o, o, v, v});
+  E.permute("abij->ijab", 1.0, vref(P.buf("Avvoo"), {v, v, o, o}), 0.0, W);
+  E.permute("ai->ia", 1.0, vref(P.buf("Avo"), {v, o}), 0.0, vref(c->rp_R1.d(), {o, v}));
+  // 2 (2 X2 - X2^T(ab)) . Avvoo = 4 X2.W - 2 X2.W^T(ab)
+  E.permute("ijba->ijab", 1.0, vref(const_cast<double *>(X2), {o, o, v, v}), 0.0, vref(c->rp_Z.d(), {o, o, v, v}));
+  const double p1 = 2.0 * device_dot(c, c->ov(), X1, c->rp_R1.d());
+  const double p2 = 4.0 * device_dot(c, c->oovv_n(), X2, c->rp_W.d()) - 2.0 * device_dot(c, c->oovv_n(), c->rp_Z.d(), c->rp_W.d());
+  return -2.0 * (p1 + p2);
+}
+
+// CCPert::solve, ccpert.cc:178-225, with the library's DIIS on (x1 | x2)
+bool pert_solve(ugx_ctx *c, PertSlot &P, int hand, const ugx_options &opt, double *pseudo, int *niter_out,
+                double *rms_out) {
+  const int o = c->no, v = c->nv;
+  pert_alloc_scratch(c);
+  pert_denominators(c);
+  if (hand == 1) pert_y_source(c, P);
+  const char *n1 = hand == 0 ? "X1" : "Y1", *n2 = hand == 0 ? "X2" : "Y2";
+  const char *n1o = hand == 0 ? "X1old" : "Y1old", *n2o = hand == 0 ? "X2old" : "Y2old";
+  if (opt.diis) diis_alloc(c, opt.diis_nvec);
+  double rms = 0.0;
+  int niter = 0;
+  for (int iter = 1; iter <= opt.maxiter; iter++) {
+    P.swap(n1, n1o);
+    P.swap(n2, n2o);
+    pert_residual(c, P, hand, P.buf(n1o), P.buf(n2o));
+    const int np = update_nparts(o, v);
+    launch_update_shift(o, v, c->rp_eps.d(), P.omega, c->rp_R1.d(), c->rp_Z.d(), P.buf(n1o), P.buf(n2o), P.buf(n1),
+                        P.buf(n2), c->part.d(), np, c->st);
+    rms = std::sqrt(host_sum(c, np));
+    niter = iter;
+    if (opt.print) std::fprintf(stderr, "ugx: perturbed (%s) iter %3d  rms %.3e\n", hand ? "left" : "right", iter, rms);
+    if (rms < opt.r_convergence) break;
+    if (opt.diis)
+      diis_step_x(c, P.buf(n1), P.buf(n1o), P.buf(n2), P.buf(n2o), opt.diis_nvec, opt.diis_policy, opt.diis_min_vecs);
+  }
+  if (opt.diis) diis_reset(c);
+  *pseudo = pert_pseudo(c, P, hand);
+  *niter_out = niter;
+  *rms_out = rms;
+  return rms < opt.r_convergence;
+}
+
+// CCResp::linresp, ccresp.cc:27-124: A from slot Px, amplitudes from Py
+double pert_linresp(ugx_ctx *c, PertSlot &Px, PertSlot &Py) {
+  const int o = c->no, v = c->nv;
+  const long ov = c->ov(), oovv = c->oovv_n();
+  Engine &E = *c->eng;
+  auto V = [](const double *p, std::vector<int> d) { return vref(const_cast<double *>(p), std::move(d)); };
+  auto K = [&](const char *spec, double a, const TRef &A, const TRef &B, double b, const TRef &C) {
+    E.contract(spec, a, A, B, b, C, PREC_FP64, "response");
+  };
+  pert_alloc_scratch(c);
+  const auto X1 = V(Py.buf("X1"), {o, v}), X2 = V(Py.buf("X2"), {o, o, v, v});
+  const auto L2 = V(c->lam_l2.d(), {o, o, v, v});
+  const auto R1 = V(c->rp_R1.d(), {o, v}), W = V(c->rp_W.d(), {o, o, v, v}), Z = V(c->rp_Z.d(), {o, o, v, v});
+  const auto AVO = V(Px.buf("Avo"), {v, o}), AVVOO = V(Px.buf("Avvoo"), {v, v, o, o});
+  const auto AOO = V(Px.buf("Aoo"), {o, o}), AVV = V(Px.buf("Avv"), {v, v}), POV = V(Px.buf("Pov"), {o, v});
+  const auto AVVVO = V(Px.buf("Avvvo"), {v, v, v, o}), AOVOO = V(Px.buf("Aovoo"), {o, v, o, o});
+  // polar1 = Avo . Y1 + 1/2 (Avvoo + Avvoo^T) . Y2
+  E.permute("ai->ia", 1.0, AVO, 0.0, R1);
+  double polar1 = device_dot(c, ov, c->rp_R1.d(), Py.buf("Y1"));
+  E.permute("abij->ijab", 0.5, AVVOO, 0.0, W);
+  E.permute("baji->ijab", 0.5, AVVOO, 1.0, W);
+  polar1 += device_dot(c, oovv, c->rp_W.d(), Py.buf("Y2"));
+  // polar2, the l1 terms: l1 . [ (2 X2 - X2^T) . Pov + Avv.X1 - Aoo.X1 ]
+  E.permute("ijab->ijab", 2.0, X2, 0.0, W);
+  E.permute("ijba->ijab", -1.0, X2, 1.0, W);
+  K("ijab,jb->ia", 1.0, W, POV, 0.0, R1);
+  K("ac,ic->ia", 1.0, AVV, X1, 1.0, R1);
+  K("ki,ka->ia", -1.0, AOO, X1, 1.0, R1);
+  double polar2 = device_dot(c, ov, c->rp_R1.d(), c->lam_l1.d());
+  // X1 . (l2 . Avvvo)
+  K("ijbc,bcaj->ia", 1.0, L2, AVVVO, 0.0, R1);
+  polar2 += device_dot(c, ov, c->rp_R1.d(), Py.buf("X1"));
+  // the l2 terms: l2 . Z
+  K("ka,kbij->ijab", -0.5, X1, AOVOO, 0.0, Z);
+  K("kb,kaji->ijab", -0.5, X1, AOVOO, 1.0, Z);
+  K("kjab,ki->ijab", -0.5, X2, AOO, 1.0, Z);
+  K("kiba,kj->ijab", -0.5, X2, AOO, 1.0, Z);
+  K("ijac,bc->ijab", 0.5, X2, AVV, 1.0, Z);
+  K("jibc,ac->ijab", 0.5, X2, AVV, 1.0, Z);
+  polar2 += device_dot(c, oovv, c->rp_Z.d(), c->lam_l2.d());
+  polar2 += 2.0 * device_dot(c, ov, Px.buf("Pov"), Py.buf("X1"));
+  return -1.0 * (polar1 + polar2);
 }
 
 // ------------------------------------------------------------------ (T)
@@ -2339,6 +2852,7 @@ int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const d
     c->tables_ready = false;
     c->have_amps = false;
     c->tg_valid = false;
+    c->perts.clear();
     c->have_intermediates = false;
     c->f_tP.release(); c->f_tQ.release(); c->f_ooov.release(); c->f_t2.release();
   c->b_P.release(); c->b_Q.release(); c->b_OVt.release();
@@ -2819,6 +3333,149 @@ int ugx_tgrad(ugx_ctx *c, double *s1, double *s2, double *Doo, double *Dvv, doub
     get(Gvvvo, c->tg_vvvo, v * v * v * o);
     get(Goovv, c->tg_oovv, o * o * v * v);
     check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+namespace {
+PertSlot &pert_slot(ugx_ctx *c, int h) {
+  if (h < 0 || h >= (int)c->perts.size() || !c->perts[h])
+    throw UgxError(UGX_BAD_INPUT, "no perturbation with handle " + std::to_string(h));
+  return *c->perts[h];
+}
+}  // namespace
+
+int ugx_pert_create(ugx_ctx *c, const double *pert, double omega, int *handle) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!c->have_lambda) throw UgxError(UGX_NOT_READY, "ugx_pert_create: no Lambda amplitudes (run ugx_lambda)");
+    if (!pert || !handle) throw UgxError(UGX_BAD_INPUT, "ugx_pert_create: null argument");
+    const int o = c->no, v = c->nv, n = o + v;
+    const long ov = c->ov(), oovv = c->oovv_n();
+    std::unique_ptr<PertSlot> S(new PertSlot());
+    PertSlot &P = *S;
+    P.omega = omega;
+    P.make("P", (size_t)n * n);
+    P.make("Poo", (size_t)o * o); P.make("Pov", ov); P.make("Pvo", ov); P.make("Pvv", (size_t)v * v);
+    P.make("Aoo", (size_t)o * o); P.make("Avv", (size_t)v * v); P.make("Avo", ov);
+    P.make("Aovoo", (size_t)o * ov * o); P.make("Avvvo", (size_t)v * v * ov); P.make("Avvoo", oovv);
+    for (const char *x : {"X1", "X1old", "Y1", "Y1old"}) P.make(x, ov);
+    for (const char *x : {"X2", "X2old", "Y2", "Y2old"}) P.make(x, oovv);
+    check(hipMemcpyAsync(P.buf("P"), pert, (size_t)n * n * 8, hipMemcpyHostToDevice, c->st), "perturbation copy");
+    auto block = [&](const char *dst, int r0, int c0, int nr, int nc) {
+      check(hipMemcpy2DAsync(P.buf(dst), (size_t)nc * 8, P.buf("P") + (size_t)r0 * n + c0, (size_t)n * 8,
+                             (size_t)nc * 8, nr, hipMemcpyDeviceToDevice, c->st),
+            "perturbation block");
+    };
+    block("Poo", 0, 0, o, o);
+    block("Pov", 0, o, o, v);
+    block("Pvo", o, 0, v, o);
+    block("Pvv", o, o, v, v);
+    run_terms(c, P, kPertBar, (int)(sizeof(kPertBar) / sizeof(kPertBar[0])));
+    // guesses X = Y = Abar / (D + omega) (ccpert.cc:58-70): one update from zero
+    pert_alloc_scratch(c);
+    pert_denominators(c);
+    c->eng->permute("ai->ia", 1.0, vref(P.buf("Avo"), {v, o}), 0.0, vref(c->rp_R1.d(), {o, v}));
+    c->eng->permute("abij->ijab", 1.0, vref(P.buf("Avvoo"), {v, v, o, o}), 0.0, vref(c->rp_Z.d(), {o, o, v, v}));
+    launch_fill(ov, 0.0, P.buf("X1old"), c->st);
+    launch_fill(oovv, 0.0, P.buf("X2old"), c->st);
+    const int np = update_nparts(o, v);
+    launch_update_shift(o, v, c->rp_eps.d(), omega, c->rp_R1.d(), c->rp_Z.d(), P.buf("X1old"), P.buf("X2old"),
+                        P.buf("X1"), P.buf("X2"), c->part.d(), np, c->st);
+    check(hipMemcpyAsync(P.buf("Y1"), P.buf("X1"), ov * 8, hipMemcpyDeviceToDevice, c->st), "Y1 guess");
+    check(hipMemcpyAsync(P.buf("Y2"), P.buf("X2"), oovv * 8, hipMemcpyDeviceToDevice, c->st), "Y2 guess");
+    check(hipStreamSynchronize(c->st), "stream sync");
+    int h = -1;
+    for (int k = 0; k < (int)c->perts.size(); k++)
+      if (!c->perts[k]) { h = k; break; }
+    if (h < 0) { h = (int)c->perts.size(); c->perts.emplace_back(); }
+    c->perts[h] = std::move(S);
+    *handle = h;
+    return UGX_OK;
+  });
+}
+
+int ugx_pert_solve(ugx_ctx *c, int handle, int hand, const ugx_options *opt_in, double *pseudoresponse, int *niter,
+                   double *rms) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    PertSlot &P = pert_slot(c, handle);
+    if (hand != 0 && hand != 1) throw UgxError(UGX_BAD_INPUT, "ugx_pert_solve: hand is 0 (right) or 1 (left)");
+    if (hand == 1 && !P.x_solved)
+      throw UgxError(UGX_NOT_READY, "ugx_pert_solve: the left-hand equations need the right-hand amplitudes");
+    ugx_options opt;
+    if (opt_in) opt = *opt_in;
+    else ugx_default_options(&opt);
+    validate_options(c, opt, "ugx_pert_solve");
+    double pr = 0.0, r = 0.0;
+    int n = 0;
+    const bool conv = pert_solve(c, P, hand, opt, &pr, &n, &r);
+    if (hand == 0) P.x_solved = conv;
+    if (pseudoresponse) *pseudoresponse = pr;
+    if (niter) *niter = n;
+    if (rms) *rms = r;
+    if (!conv) {
+      c->err = "Computation has not converged.";
+      return UGX_NOT_CONVERGED;
+    }
+    return UGX_OK;
+  });
+}
+
+int ugx_pert_amplitudes(ugx_ctx *c, int handle, double *X1, double *X2, double *Y1, double *Y2) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    PertSlot &P = pert_slot(c, handle);
+    const long ov = c->ov(), oovv = c->oovv_n();
+    auto get = [&](double *dst, const char *n, long cnt) {
+      if (dst) check(hipMemcpyAsync(dst, P.buf(n), cnt * 8, hipMemcpyDeviceToHost, c->st), "perturbed amplitudes");
+    };
+    get(X1, "X1", ov); get(X2, "X2", oovv); get(Y1, "Y1", ov); get(Y2, "Y2", oovv);
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+int ugx_pert_residual(ugx_ctx *c, int handle, int hand, const double *A1, const double *A2, double *R1, double *R2) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    PertSlot &P = pert_slot(c, handle);
+    if (!A1 || !A2) throw UgxError(UGX_BAD_INPUT, "ugx_pert_residual: null amplitudes");
+    if (hand != 0 && hand != 1) throw UgxError(UGX_BAD_INPUT, "ugx_pert_residual: hand is 0 (right) or 1 (left)");
+    const int o = c->no, v = c->nv;
+    const long ov = c->ov(), oovv = c->oovv_n();
+    pert_alloc_scratch(c);
+    c->rp_A1.alloc(ov * 8);
+    c->rp_A2.alloc(oovv * 8);
+    check(hipMemcpyAsync(c->rp_A1.p, A1, ov * 8, hipMemcpyHostToDevice, c->st), "amplitude copy");
+    check(hipMemcpyAsync(c->rp_A2.p, A2, oovv * 8, hipMemcpyHostToDevice, c->st), "amplitude copy");
+    if (hand == 1) pert_y_source(c, P);
+    pert_residual(c, P, hand, c->rp_A1.d(), c->rp_A2.d());
+    if (R1) check(hipMemcpyAsync(R1, c->rp_R1.p, ov * 8, hipMemcpyDeviceToHost, c->st), "R1 copy");
+    if (R2) {
+      c->eng->permute("jiba->ijab", 1.0, vref(c->rp_Z.d(), {o, o, v, v}), 0.0, vref(c->rp_W.d(), {o, o, v, v}));
+      launch_axpby(oovv, 1.0, c->rp_Z.d(), 1.0, c->rp_W.d(), c->st);
+      check(hipMemcpyAsync(R2, c->rp_W.p, oovv * 8, hipMemcpyDeviceToHost, c->st), "R2 copy");
+    }
+    check(hipStreamSynchronize(c->st), "stream sync");
+    return UGX_OK;
+  });
+}
+
+int ugx_linresp(ugx_ctx *c, int hx, int hy, double *value) {
+  return guarded(c, [&]() {
+    require_loaded(c);
+    if (!value) throw UgxError(UGX_BAD_INPUT, "ugx_linresp: null value");
+    PertSlot &Px = pert_slot(c, hx), &Py = pert_slot(c, hy);
+    *value = pert_linresp(c, Px, Py);
+    return UGX_OK;
+  });
+}
+
+int ugx_pert_release(ugx_ctx *c, int handle) {
+  return guarded(c, [&]() {
+    (void)pert_slot(c, handle);
+    c->perts[handle].reset();
     return UGX_OK;
   });
 }

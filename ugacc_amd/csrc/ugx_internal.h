@@ -208,6 +208,11 @@ int update_nparts(int no, int nv);
 void launch_update(int no, int nv, const double *eps, const double *R1, const double *X, const double *t1old,
                    const double *t2old, double *t1, double *t2, double *R2out, double *partials, int npart,
                    hipStream_t st);
+// the same update with every denominator shifted by `shift` (x = x_old +
+// R / (D + shift), the perturbed-amplitude equations at a frequency)
+void launch_update_shift(int no, int nv, const double *eps, double shift, const double *R1, const double *X,
+                         const double *t1old, const double *t2old, double *t1, double *t2, double *partials, int npart,
+                         hipStream_t st);
 // MP2 guess t1 = f_ia/D1, t2 = <ij|ab>/D2 from the full Fock matrix
 void launch_guess(int no, int nv, const double *fock, const double *oovv, double *t1, double *t2, hipStream_t st);
 // partial sums of x.y

@@ -321,7 +321,7 @@ __global__ void k_build_Y(int no, int nv, const double *t1, const double *t2, do
 // half X (R2_ijab = X_ijab + X_jiba): ||R||^2 partials, t = t_old + R/D,
 // D_ijab = (f_ii + f_jj) - (f_aa + f_bb) so that D_ijab == D_jiba exactly.
 __global__ void k_update(int no, int nv, const double *eps, const double *R1, const double *X, const double *t1old,
-                         const double *t2old, double *t1, double *t2, double *R2out, double *partials) {
+                         const double *t2old, double *t1, double *t2, double *R2out, double *partials, double shift) {
   long n1 = (long)no * nv, n2 = n1 * n1;
   double s = 0.0;
   for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n2 + n1; idx += (long)gridDim.x * NT) {
@@ -332,7 +332,7 @@ __global__ void k_update(int no, int nv, const double *eps, const double *R1, co
       int j = r % no; r /= no;
       int i = (int)r;
       double R = X[idx] + X[(((long)j * no + i) * nv + b) * nv + a];
-      double D = (eps[i] + eps[j]) - (eps[no + a] + eps[no + b]);
+      double D = (eps[i] + eps[j]) - (eps[no + a] + eps[no + b]) + shift;
       s += R * R;
       if (R2out) R2out[idx] = R;
       t2[idx] = t2old[idx] + R / D;
@@ -341,7 +341,7 @@ __global__ void k_update(int no, int nv, const double *eps, const double *R1, co
       int i = q / nv, a = q % nv;
       double R = R1[q];
       s += R * R;
-      t1[q] = t1old[q] + R / (eps[i] - eps[no + a]);
+      t1[q] = t1old[q] + R / (eps[i] - eps[no + a] + shift);
     }
   }
   double tot = block_sum(s);
@@ -762,7 +762,14 @@ void launch_update(int no, int nv, const double *eps, const double *R1, const do
                    const double *t2old, double *t1, double *t2, double *R2out, double *partials, int npart,
                    hipStream_t st) {
   hipLaunchKernelGGL(k_update, dim3(npart), dim3(NT), 0, st, no, nv, eps, R1, X, t1old, t2old, t1, t2, R2out,
-                     partials);
+                     partials, 0.0);
+}
+
+void launch_update_shift(int no, int nv, const double *eps, double shift, const double *R1, const double *X,
+                         const double *t1old, const double *t2old, double *t1, double *t2, double *partials, int npart,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_update, dim3(npart), dim3(NT), 0, st, no, nv, eps, R1, X, t1old, t2old, t1, t2, nullptr,
+                     partials, shift);
 }
 
 int energy_nparts(int no, int nv) {

@@ -26,7 +26,8 @@ EXPORTS = [
     "ugx_default_options", "ugx_create", "ugx_destroy", "ugx_last_error", "ugx_load_hamiltonian",
     "ugx_set_amplitudes", "ugx_prepare", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
     "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_hbar", "ugx_lambda",
-    "ugx_get_lambda", "ugx_lambda_residual", "ugx_tgrad", "ugx_density", "ugx_set_profiling",
+    "ugx_get_lambda", "ugx_lambda_residual", "ugx_tgrad", "ugx_density", "ugx_pert_create", "ugx_pert_solve",
+    "ugx_pert_amplitudes", "ugx_pert_residual", "ugx_linresp", "ugx_pert_release", "ugx_set_profiling",
     "ugx_kernel_stats", "ugx_gemm", "ugx_gemm_bf16x3_static", "ugx_synchronize", "ugx_batch_solve", "ugx_set_autotune", "ugx_tile_table",
     "ugx_save_tile_table", "ugx_load_tile_table", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
     "ugx_mlp_last_error", "ugx_mlp_set_params", "ugx_mlp_get_params", "ugx_mlp_set_scaling", "ugx_mlp_forward",
@@ -103,6 +104,13 @@ def load_library(path=LIBPATH):
     L.ugx_get_lambda.argtypes = [vp, dp, dp]
     L.ugx_lambda_residual.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.ugx_tgrad.argtypes = [vp] + [dp] * 8
+    pint = ctypes.POINTER(ctypes.c_int)
+    L.ugx_pert_create.argtypes = [vp, dp, cd, pint]
+    L.ugx_pert_solve.argtypes = [vp, ip, ip, ctypes.POINTER(Options), dp, pint, dp]
+    L.ugx_pert_amplitudes.argtypes = [vp, ip, dp, dp, dp, dp]
+    L.ugx_pert_residual.argtypes = [vp, ip, ip, dp, dp, dp, dp]
+    L.ugx_linresp.argtypes = [vp, ip, ip, dp]
+    L.ugx_pert_release.argtypes = [vp, ip]
     L.ugx_density.argtypes = [vp] + [dp] * 11
     L.ugx_set_profiling.argtypes = [vp, ip]
     L.ugx_set_profiling.restype = None
@@ -344,6 +352,12 @@ class Context:
         self._check(self.L.ugx_tgrad(self.h, *[_p(out[k]) if k in out else None for k in self.TGRAD_BLOCKS]))
         return out
 
+    def pert(self, A, omega=0.0):
+        """ugx_pert_create: a perturbation slot for the one-electron MO
+        operator A (nact x nact) at omega (ccpert.cc:11-176), over the HBAR
+        and Lambda amplitudes of the last lambda_solve."""
+        return Perturbation(self, A, omega)
+
     DENSITY_BLOCKS = ("Doo", "Dvv", "Dov", "Dvo", "Goooo", "Gvvvv", "Gooov", "Gvvvo", "Govov", "Goovv")
     DENSITY_PARTS = ("oooo", "vvvv", "ooov", "vvvo", "ovov", "oovv")
 
@@ -445,6 +459,58 @@ class Context:
         """Network used by GUESS_MLP (None clears it); the context keeps a reference."""
         self._net = net
         self._check(self.L.ugx_set_guess_network(self.h, net.h if net is not None else None))
+
+
+class Perturbation:
+    """A perturbation slot of a context (ugx_pert_*): the perturbed wave
+    function of ccpert.cc and its linear response (ccresp.cc)."""
+
+    def __init__(self, ctx, A, omega):
+        self.ctx = ctx
+        self.L = ctx.L
+        n = ctx.no + ctx.nv
+        A = _c64(A)
+        if A.shape != (n, n):
+            raise ValueError("perturbation must be nact x nact")
+        h = ctypes.c_int()
+        ctx._check(self.L.ugx_pert_create(ctx.h, _p(A), float(omega), ctypes.byref(h)))
+        self.h = h.value
+        self.omega = omega
+
+    def solve(self, hand=0, options=None, **kw):
+        """ugx_pert_solve: hand 0 right (X), 1 left (Y)."""
+        o = options if options is not None else default_options()
+        for k, v in kw.items():
+            setattr(o, k, v)
+        pr, rms, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        rc = self.ctx._check(self.L.ugx_pert_solve(self.ctx.h, self.h, hand, ctypes.byref(o), ctypes.byref(pr),
+                                                   ctypes.byref(n), ctypes.byref(rms)), allow=(OK, NOT_CONVERGED))
+        return dict(converged=rc == OK, pseudoresponse=pr.value, niter=n.value, rms=rms.value)
+
+    def amplitudes(self):
+        no, nv = self.ctx.no, self.ctx.nv
+        X1, Y1 = np.zeros((no, nv)), np.zeros((no, nv))
+        X2, Y2 = np.zeros((no, no, nv, nv)), np.zeros((no, no, nv, nv))
+        self.ctx._check(self.L.ugx_pert_amplitudes(self.ctx.h, self.h, _p(X1), _p(X2), _p(Y1), _p(Y2)))
+        return X1, X2, Y1, Y2
+
+    def residual(self, hand, A1, A2):
+        no, nv = self.ctx.no, self.ctx.nv
+        A1, A2 = _c64(A1), _c64(A2)
+        R1, R2 = np.zeros((no, nv)), np.zeros((no, no, nv, nv))
+        self.ctx._check(self.L.ugx_pert_residual(self.ctx.h, self.h, hand, _p(A1), _p(A2), _p(R1), _p(R2)))
+        return R1, R2
+
+    def linresp(self, other):
+        """ugx_linresp: <<A;B>> with A this perturbation, B other's."""
+        v = ctypes.c_double()
+        self.ctx._check(self.L.ugx_linresp(self.ctx.h, self.h, other.h, ctypes.byref(v)))
+        return v.value
+
+    def release(self):
+        """ugx_pert_release.  The handle is kept: later calls with it are
+        rejected by the library (UGX_BAD_INPUT)."""
+        self.ctx._check(self.L.ugx_pert_release(self.ctx.h, self.h))
 
 
 class Network:
