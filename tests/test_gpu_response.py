@@ -2,8 +2,9 @@
 contractions from the term tables of ugx_ccsd.cpp) against the oracle's
 restatement of ccpert.cc / ccresp.cc.
 
-- synthetic o=2..4 at omega = 0 and omega != 0: at the library's converged
-  CCSD and Lambda amplitudes (given to the oracle), the right- and left-hand
+- synthetic o=2..4 at omega = 0 and omega != 0, one perturbation symmetric
+  and one general: at the library's converged CCSD and Lambda amplitudes
+  (given to the oracle), the right- and left-hand
   residuals at random amplitudes to 1e-12 of their scale; the solved X and Y
   to 1e-10; the pseudoresponses and linresp of pairs of perturbations, with
   the right-hand amplitudes only (the reference's MYHAND=RIGHT, Y at its
@@ -12,7 +13,10 @@ restatement of ccpert.cc / ccresp.cc.
   residuals vanish at the library's converged X and Y, and the static
   polarizability <<mu_z;mu_z>> equals -d2E/dF2 from CCSD solves of the
   library at F = +-h, +-2h (the identity the oracle tests pin), also for
-  (H2O)2 where no oracle runs.
+  (H2O)2 where no oracle runs;
+- the plugin mirror's DERTYPE=SECOND (ugacc.cc:143-241, both MYHAND): the
+  polarizability and optical rotation tensors of H2O rotate with the
+  molecule.
 """
 import numpy as np
 import pytest
@@ -53,7 +57,8 @@ def test_response_matches_oracle(no, nv, seed, omega):
     c, fock, ints = loaded(no, nv, seed, scale=0.3)
     _ground(c)
     o = _oracle_like(c, fock, ints, no)
-    A, B = _sym(n, seed + 1), _sym(n, seed + 2)
+    # B general: an antisymmetric part as the magnetic dipole ("L") has
+    A, B = _sym(n, seed + 1), np.random.default_rng(seed + 2).standard_normal((n, n))
     ga, gb = c.pert(A, omega), c.pert(B, omega)
     oa, ob = o.pert(A, omega), o.pert(B, omega)
     # the guesses and the right-hand residual at random amplitudes
@@ -156,3 +161,38 @@ def test_h2o2_static_polarizability():
     ref = Reference(molecules.random_dimer(200), "cc-pvdz")
     alpha, fd = _field_check(ref, 0, 1e-10, None)
     assert abs(alpha - fd) < 2e-6 * abs(fd)
+
+
+def _plugin_second(atoms, hand, omega):
+    from ugacc_amd import plugin
+    from ugacc_amd.chem import molecules
+    from ugacc_amd.chem.scf import Reference
+    ref = Reference(molecules.to_bohr(atoms), "cc-pvdz")
+    opts = dict(plugin.read_options(), DERTYPE="SECOND", MYHAND=hand, MY_OMEGA=omega, R_CONVERGENCE=1e-10)
+    plugin.ugacc(ref, opts)
+    ax = plugin.CART
+    P = np.array([[ref.polarizabilities["<<Mu_%s;Mu_%s>>" % (p, q)] for q in ax] for p in ax])
+    G = np.array([[ref.rotations["<<Mu_%s;L_%s>>" % (p, q)] for q in ax] for p in ax])
+    return P, G
+
+
+@pytest.mark.parametrize("hand", ["LEFT", "RIGHT"])
+def test_plugin_second_derivative_tensors_rotate_with_the_molecule(hand):
+    """The plugin mirror's DERTYPE=SECOND (ugacc.cc:143-241) at 589 nm: the
+    polarizability and the optical rotation tensor (gauge origin at O, the
+    rotation centre) of a rotated H2O are R P R^T and R G R^T; in its own
+    frame (C2v, molecule in xz) the polarizability is diagonal."""
+    announce("plugin DERTYPE=SECOND h2o, MYHAND=%s" % hand)
+    from ugacc_amd.chem import molecules
+    omega = 0.0773571
+    R = molecules.random_rotation(np.random.default_rng(11))
+    w0 = molecules.water()
+    w1 = [(s, R @ np.asarray(x)) for s, x in w0]
+    P0, G0 = _plugin_second(w0, hand, omega)
+    P1, G1 = _plugin_second(w1, hand, omega)
+    print("P0", np.diag(P0), "G0 max", np.max(np.abs(G0)))
+    assert np.max(np.abs(P0 - P0.T)) < 1e-12
+    assert np.max(np.abs(P0 - np.diag(np.diag(P0)))) < 1e-8 * np.max(np.abs(P0))
+    assert np.max(np.abs(P1 - R @ P0 @ R.T)) < 1e-7 * np.max(np.abs(P0))
+    assert np.max(np.abs(G0)) > 1e-3
+    assert np.max(np.abs(G1 - R @ G0 @ R.T)) < 1e-7 * np.max(np.abs(G0))

@@ -57,8 +57,17 @@ def test_plugin_rejects_what_the_reference_rejects():
     r.socc = 2
     with pytest.raises(plugin.PsiException, match="closed-shell systems only"):
         plugin.ugacc(r, opts)
-    with pytest.raises(plugin.PsiException, match="linear response"):
+    # the linear response needs the property integrals of the front end
+    with pytest.raises(plugin.PsiException, match="property integrals"):
         plugin.ugacc(_Ref(), dict(opts, DERTYPE="SECOND"))
+
+    class _RefP(_Ref):
+        def perturbation(self, op):
+            return [np.zeros((5, 5))] * 3
+    # the reference crashes here (CCSD_T with DERTYPE=SECOND reads (T)
+    # density parts it never allocated); the mirror refuses it
+    with pytest.raises(plugin.PsiException, match="NONE and FIRST only"):
+        plugin.ugacc(_RefP(), dict(opts, WFN="CCSD_T", DERTYPE="SECOND"))
 
 
 def test_context_without_gpu_reports_no_device():

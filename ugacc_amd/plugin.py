@@ -10,11 +10,19 @@ ccwfn.cc:188-258: iteration table, summary, (T) for WFN=CCSD_T (and the
 (T) gradient terms with DERTYPE=FIRST, :252-255) and "Computation has not
 converged." on failure.  DERTYPE=FIRST then runs HBAR, Lambda and the
 densities and prints the density energies (ugacc.cc:118-141), setting
-ref.current_energy; DERTYPE=SECOND (the linear response) is refused.
+ref.current_energy.  DERTYPE=SECOND adds the linear response of
+ugacc.cc:143-241: the dipole ("Mu") and magnetic-dipole ("L") perturbed wave
+functions per Cartesian component at MY_OMEGA (right-hand; with MYHAND=LEFT
+also left-hand; with MYHAND=RIGHT and MY_OMEGA != 0 also Mu at -MY_OMEGA),
+the dipole polarizabilities <<Mu_p;Mu_q>> and the length-gauge optical
+rotation tensor <<Mu_p;L_q>>, printed and stored as ref.polarizabilities and
+ref.rotations (label -> value, the reference's labels).
 
 ``ref`` is any object with the attributes of the front end's
 chem.scf.Reference: no, nv, fock (active Fock), blocks (six Dirac blocks),
-escf, and optionally enuc, efzc, reference ("RHF"), socc (open-shell count).
+escf, and optionally enuc, efzc, reference ("RHF"), socc (open-shell count);
+DERTYPE=SECOND also needs perturbation(op) -> three nact x nact MO matrices
+(Perturbation::prop_p, perturbation.cc).
 """
 import sys
 
@@ -91,8 +99,97 @@ class CCWfn:
         self.ctx.close()
 
 
+CART = ("X", "Y", "Z")
+
+
+def _std_to_string(x):
+    """std::to_string of a double (the "%f" format), used in the reference's
+    perturbation keys."""
+    return "%f" % x
+
+
+def _solve_pert(cc, g, hand, opts):
+    """CCPert::solve (ccpert.cc:177-228): the banner, the converged line, and
+    "Computation has not converged." when the residual stays above
+    R_CONVERGENCE."""
+    w = cc.out.write
+    w("\n\tCoupled-Cluster %s Perturbed Wfn Iteration:\n" % ("RH" if hand == 0 else "LH"))
+    w("\t-------------------------------------\n")
+    w("\t Iter   Pseudoresponse      RMS   \n")
+    w("\t-------------------------------------\n")
+    r = g.solve(hand, opts)
+    w("\t  %3d  %20.15f  %5.3e\n" % (r["niter"], r["pseudoresponse"], r["rms"]))
+    if not r["converged"]:
+        raise PsiException("Computation has not converged.")
+    return r
+
+
+def _linear_response(cc, ref, options):
+    """ugacc.cc:143-241: perturbed wave functions, polarizabilities and the
+    length-gauge optical rotation tensor."""
+    if not hasattr(ref, "perturbation"):
+        raise PsiException("DERTYPE=SECOND needs the property integrals (ref.perturbation)")
+    mu, am = ref.perturbation("Mu"), ref.perturbation("L")
+    omega = float(options["MY_OMEGA"])
+    right = str(options["MYHAND"]) == "RIGHT"
+    opts = cc._opts()
+    opts.precision = ugx.PREC_FP64
+    w = cc.out.write
+    perts = {}
+    try:
+        w("\n\tSolving right hand perturbed CC amplitudes\n")
+        for k, x in enumerate(CART):
+            entry = "Mu" + x + _std_to_string(omega)
+            entry_1 = "L" + x + _std_to_string(omega)
+            w("\n\tCC RH Perturbed Wavefunction: %s\n" % entry)
+            perts[entry] = cc.ctx.pert(mu[k], omega)
+            _solve_pert(cc, perts[entry], 0, opts)
+            w("\n\tCC RH Perturbed Wavefunction: %s\n" % entry_1)
+            perts[entry_1] = cc.ctx.pert(am[k], omega)
+            _solve_pert(cc, perts[entry_1], 0, opts)
+            if omega != 0.0 and right:
+                entry = "Mu" + x + _std_to_string(-omega)
+                w("\n\tCC RH Perturbed Wavefunction: %s\n" % entry)
+                perts[entry] = cc.ctx.pert(mu[k], -omega)
+                _solve_pert(cc, perts[entry], 0, opts)
+            if not right:
+                w("\n\tSolving left hand perturbed CC amplitudes\n")
+                w("\n\tCC LH Perturbed Wavefunction: %s\n" % entry)
+                _solve_pert(cc, perts[entry], 1, opts)
+                w("\n\tCC LH Perturbed Wavefunction: %s\n" % entry_1)
+                _solve_pert(cc, perts[entry_1], 1, opts)
+        polars, rots = {}, {}
+        for p in range(3):
+            for q in range(p + 1):
+                gp = perts["Mu" + CART[p] + _std_to_string(omega)]
+                gq = perts["Mu" + CART[q] + _std_to_string(omega)]
+                if p == q:
+                    polar = gp.linresp(gq)
+                else:
+                    polar = 0.5 * gp.linresp(gq) + 0.5 * gq.linresp(gp)
+                polars["<<Mu_%s;Mu_%s>>" % (CART[p], CART[q])] = polar
+                polars["<<Mu_%s;Mu_%s>>" % (CART[q], CART[p])] = polar
+        for p in range(3):
+            for q in range(3):
+                gp = perts["Mu" + CART[p] + _std_to_string(omega)]
+                gq = perts["L" + CART[q] + _std_to_string(omega)]
+                rotation = 0.5 * gp.linresp(gq) - 0.5 * gq.linresp(gp)
+                rots["<<Mu_%s;L_%s>>" % (CART[p], CART[q])] = -rotation
+    finally:
+        for g in perts.values():
+            g.release()
+    w("\n\tDipole Polarizabilities (au)\n\n")
+    for label in sorted(polars):
+        w("\t%s : %20.14f \n\n" % (label, polars[label]))
+    w("\n\tOptical rotation (length gauge) (a.u)\n\n")
+    for label in sorted(rots):
+        w("\t%s : %20.14f \n\n" % (label, rots[label]))
+    ref.polarizabilities = polars
+    ref.rotations = rots
+
+
 def ugacc(ref, options):
-    """The plugin entry (ugacc.cc:75-117) for DERTYPE = NONE."""
+    """The plugin entry (ugacc.cc:75-241)."""
     if ref is None:
         raise PsiException("SCF has not been run yet!")
     if options.get("REFERENCE", "RHF") != "RHF" or getattr(ref, "reference", "RHF") != "RHF":
@@ -100,17 +197,24 @@ def ugacc(ref, options):
     if getattr(ref, "socc", 0):
         raise PsiException("UGACC is for closed-shell systems only.")
     dertype = options.get("DERTYPE", "NONE")
-    if dertype not in ("NONE", "FIRST"):
-        raise PsiException("DERTYPE=%s needs the linear response (ccpert.cc, ccresp.cc), which this build "
-                           "does not provide" % dertype)
+    # as in ugacc.cc:119-143, every DERTYPE other than NONE and FIRST runs
+    # the linear response
+    second = dertype not in ("NONE", "FIRST")
+    if second and not hasattr(ref, "perturbation"):
+        raise PsiException("DERTYPE=SECOND needs the property integrals (ref.perturbation)")
+    if second and options.get("WFN") == "CCSD_T":
+        # the reference's CCDensity copies the (T) density parts for every
+        # CCSD_T run (ccdensity.cc:44-69), which CCWfn allocates only for
+        # DERTYPE=FIRST (ccwfn.cc:108-118)
+        raise PsiException("WFN=CCSD_T supports DERTYPE=NONE and FIRST only")
     cc = CCWfn(ref, options)
     try:
         ecc = cc.compute_energy()
         ref.cc_energy = ecc
         ref.cc_result = cc.result
-        if dertype == "FIRST":
-            # HBAR, Lambda (with the (T) sources for CCSD_T) and the densities,
-            # ugacc.cc:118-141
+        if dertype != "NONE":
+            # HBAR, Lambda (with the (T) sources for CCSD_T at DERTYPE=FIRST:
+            # cclambda.cc:181, :243) and the densities, ugacc.cc:118-141
             lo = cc._opts()
             lo.precision = ugx.PREC_FP64
             lr = cc.ctx.lambda_solve(lo)
@@ -129,6 +233,8 @@ def ugacc(ref, options):
             w("\t%s Total Energy          = %20.14f (from ccwfn)\n" % (wfn, ecc + eref))
             ref.density_energy = eone + etwo
             ref.current_energy = ecc + eref
+        if second:
+            _linear_response(cc, ref, options)
     finally:
         cc.close()
     return ref
