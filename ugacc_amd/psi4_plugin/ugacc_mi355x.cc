@@ -16,10 +16,17 @@
  * the (T) gradient terms (ugx_tgrad, as compute_energy runs tgrad,
  * ccwfn.cc:252-255), then Lambda over HBAR (ugx_lambda, with the (T) sources
  * for CCSD_T) and the density energies (ugx_density, with the (T) parts),
- * printed as the reference prints them, with CURRENT ENERGY set.  The linear
- * response of DERTYPE=SECOND (ccpert.cc, ccresp.cc) is not part of this
- * library: such a request fails with a PSIEXCEPTION instead of returning
- * wrong numbers.
+ * printed as the reference prints them, with CURRENT ENERGY set.  Every
+ * other DERTYPE (SECOND) adds the linear response of ugacc.cc:143-241 on the
+ * GPU (ugx_pert_create / ugx_pert_solve / ugx_linresp): the dipole ("Mu",
+ * MintsHelper::ao_dipole) and magnetic-dipole ("L", ao_angular_momentum
+ * scaled by -1/2, perturbation.cc:83-86) perturbed wave functions per
+ * Cartesian component at MY_OMEGA, left-hand ones too with MYHAND=LEFT and
+ * Mu at -MY_OMEGA with MYHAND=RIGHT and MY_OMEGA != 0, then the dipole
+ * polarizabilities and the length-gauge optical rotation tensor, printed in
+ * the reference's layout.  WFN=CCSD_T with SECOND is refused: the
+ * reference's CCDensity reads (T) density parts that CCWfn allocates for
+ * DERTYPE=FIRST only (ccdensity.cc:44-69, ccwfn.cc:108-118).
  *
  * Integrals: MintsHelper::mo_eri over the active occupied / virtual AO
  * coefficients gives chemist (pq|rs); the blocks are <pq|rs> = (pr|qs).
@@ -27,6 +34,7 @@
  * (the reference uses QT order; every energy is invariant to the order).
  */
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -80,6 +88,17 @@ std::vector<double> dirac_block(const SharedMatrix &G, int np, int nq, int nr, i
   return X;
 }
 
+// An AO one-electron operator in the active MO basis (occupied first),
+// row-major nact x nact, times scale.
+std::vector<double> mo_property(const SharedMatrix &X, const SharedMatrix &Ca, double scale) {
+  SharedMatrix M = linalg::triplet(Ca, X, Ca, true, false, false);
+  const int n = M->rowspi()[0];
+  std::vector<double> out((size_t)n * n);
+  for (int p = 0; p < n; p++)
+    for (int q = 0; q < n; q++) out[(size_t)p * n + q] = scale * M->get(p, q);
+  return out;
+}
+
 struct Context {
   ugx_ctx *c = nullptr;
   ~Context() {
@@ -101,6 +120,8 @@ extern "C" PSI_API SharedWavefunction ugacc(SharedWavefunction ref, Options &opt
   outfile->Printf("\tDIIS           = %s\n", options.get_bool("DIIS") ? "Yes" : "No");
   outfile->Printf("\tOut-of-core    = %s\n", options.get_bool("OOC") ? "Yes" : "No");
   outfile->Printf("\tDertype        = %s\n", options.get_str("DERTYPE").c_str());
+  outfile->Printf("\tOMEGA          = %3.1e\n", options.get_double("MY_OMEGA"));
+  outfile->Printf("\tHAND           = %s\n", options.get_str("MYHAND").c_str());
   outfile->Printf("\tPrecision      = %s\n", options.get_str("UGX_PRECISION").c_str());
 
   // the reference's error trapping (closed-shell RHF in place)
@@ -109,9 +130,9 @@ extern "C" PSI_API SharedWavefunction ugacc(SharedWavefunction ref, Options &opt
   for (int h = 0; h < ref->nirrep(); h++)
     if (ref->soccpi()[h]) throw PSIEXCEPTION("UGACC is for closed-shell systems only.");
   const std::string dertype = options.get_str("DERTYPE");
-  if (dertype == "SECOND")
-    throw PSIEXCEPTION("ugacc (MI355X): DERTYPE SECOND needs the linear response (ccpert.cc, ccresp.cc), "
-                       "which this library does not build");
+  const bool second = dertype != "NONE" && dertype != "FIRST";  // ugacc.cc:119-143
+  if (second && options.get_str("WFN") == "CCSD_T")
+    throw PSIEXCEPTION("ugacc (MI355X): WFN=CCSD_T supports DERTYPE=NONE and FIRST only");
 
   // active orbitals (frozen core and frozen virtuals removed), AO basis
   SharedMatrix Co = ref->Ca_subset("AO", "ACTIVE_OCC");
@@ -206,6 +227,88 @@ extern "C" PSI_API SharedWavefunction ugacc(SharedWavefunction ref, Options &opt
   outfile->Printf("\t%s Total Energy          = %20.14f (from density)\n", wfn, eone + etwo + eref);
   outfile->Printf("\t%s Total Energy          = %20.14f (from ccwfn)\n", wfn, ecc + eref);
   Process::environment.globals["CURRENT ENERGY"] = ecc + eref;
+  if (!second) return ref;
+
+  // the linear response, ugacc.cc:143-241
+  std::vector<SharedMatrix> dip = mints->ao_dipole(), ang = mints->ao_angular_momentum();
+  std::vector<double> mu[3], am[3];
+  for (int k = 0; k < 3; k++) {
+    mu[k] = mo_property(dip[k], Ca, 1.0);
+    am[k] = mo_property(ang[k], Ca, -0.5);
+  }
+  const double omega = options.get_double("MY_OMEGA");
+  const bool right = options.get_str("MYHAND") == "RIGHT";
+  ugx_options popt = lopt;
+  std::map<std::string, int> perts;
+  auto create = [&](const std::string &key, const std::vector<double> &A, double w) {
+    int h = -1;
+    if (ugx_pert_create(ctx.c, A.data(), w, &h) != UGX_OK)
+      throw PSIEXCEPTION(std::string("ugacc (MI355X): ") + ugx_last_error(ctx.c));
+    perts[key] = h;
+  };
+  auto solve = [&](const std::string &key, int hand) {
+    outfile->Printf("\n\tCoupled-Cluster %s Perturbed Wfn Iteration:\n", hand ? "LH" : "RH");
+    outfile->Printf("\t-------------------------------------\n");
+    outfile->Printf("\t Iter   Pseudoresponse      RMS   \n");
+    outfile->Printf("\t-------------------------------------\n");
+    double pr = 0.0, prms = 0.0;
+    int pn = 0;
+    const int prc = ugx_pert_solve(ctx.c, perts.at(key), hand, &popt, &pr, &pn, &prms);
+    if (prc == UGX_OK || prc == UGX_NOT_CONVERGED) outfile->Printf("\t  %3d  %20.15f  %5.3e\n", pn, pr, prms);
+    if (prc == UGX_NOT_CONVERGED) throw PSIEXCEPTION("Computation has not converged.");
+    if (prc != UGX_OK) throw PSIEXCEPTION(std::string("ugacc (MI355X): ") + ugx_last_error(ctx.c));
+  };
+  auto linresp = [&](const std::string &x, const std::string &y) {
+    double v = 0.0;
+    if (ugx_linresp(ctx.c, perts.at(x), perts.at(y), &v) != UGX_OK)
+      throw PSIEXCEPTION(std::string("ugacc (MI355X): ") + ugx_last_error(ctx.c));
+    return v;
+  };
+  const char *cart[3] = {"X", "Y", "Z"};
+  outfile->Printf("\n\tSolving right hand perturbed CC amplitudes\n");
+  for (int k = 0; k < 3; k++) {
+    std::string entry = std::string("Mu") + cart[k] + std::to_string(omega);
+    const std::string entry_1 = std::string("L") + cart[k] + std::to_string(omega);
+    outfile->Printf("\n\tCC RH Perturbed Wavefunction: %s\n", entry.c_str());
+    create(entry, mu[k], omega);
+    solve(entry, 0);
+    outfile->Printf("\n\tCC RH Perturbed Wavefunction: %s\n", entry_1.c_str());
+    create(entry_1, am[k], omega);
+    solve(entry_1, 0);
+    if (omega != 0.0 && right) {
+      entry = std::string("Mu") + cart[k] + std::to_string(-omega);
+      outfile->Printf("\n\tCC RH Perturbed Wavefunction: %s\n", entry.c_str());
+      create(entry, mu[k], -omega);
+      solve(entry, 0);
+    }
+    if (!right) {
+      outfile->Printf("\n\tSolving left hand perturbed CC amplitudes\n");
+      outfile->Printf("\n\tCC LH Perturbed Wavefunction: %s\n", entry.c_str());
+      solve(entry, 1);
+      outfile->Printf("\n\tCC LH Perturbed Wavefunction: %s\n", entry_1.c_str());
+      solve(entry_1, 1);
+    }
+  }
+  std::map<std::string, double> polars, rots;
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q <= p; q++) {
+      const std::string mp = std::string("Mu") + cart[p] + std::to_string(omega);
+      const std::string mq = std::string("Mu") + cart[q] + std::to_string(omega);
+      const double polar = p == q ? linresp(mp, mq) : 0.5 * linresp(mp, mq) + 0.5 * linresp(mq, mp);
+      polars[std::string("<<Mu_") + cart[p] + ";Mu_" + cart[q] + ">>"] = polar;
+      polars[std::string("<<Mu_") + cart[q] + ";Mu_" + cart[p] + ">>"] = polar;
+    }
+  for (int p = 0; p < 3; p++)
+    for (int q = 0; q < 3; q++) {
+      const std::string mp = std::string("Mu") + cart[p] + std::to_string(omega);
+      const std::string lq = std::string("L") + cart[q] + std::to_string(omega);
+      const double rotation = 0.5 * linresp(mp, lq) - 0.5 * linresp(lq, mp);
+      rots[std::string("<<Mu_") + cart[p] + ";L_" + cart[q] + ">>"] = -rotation;
+    }
+  outfile->Printf("\n\tDipole Polarizabilities (au)\n\n");
+  for (const auto &e : polars) outfile->Printf("\t%s : %20.14lf \n\n", e.first.c_str(), e.second);
+  outfile->Printf("\n\tOptical rotation (length gauge) (a.u)\n\n");
+  for (const auto &e : rots) outfile->Printf("\t%s : %20.14lf \n\n", e.first.c_str(), e.second);
   return ref;
 }
 
