@@ -122,3 +122,36 @@ def test_h2o4_fixture_converged_to_1e10():
     print("(H2O)4: E_CCSD %.12f (oracle %.12f), E_(T) %.12f (oracle %.12f), %d iterations (oracle %d)"
           % (r["e_ccsd"], d.scalars["eccsd"], r["e_t"], d.scalars["e_t_fov_tjl"], r["niter"],
              d.scalars["niter"]))
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLDEN, "h2o4_seed1.ugxd")), reason="fixture not generated")
+def test_h2o4_fixture_with_library_fp64_gemms():
+    """With autotuning on, the engine's large single FP64 GEMMs (at least
+    2^30 multiply-adds, unit-stride operands) run on rocBLAS dgemm where it
+    times faster than every tile of the library's kernel (at (H2O)4 the ring
+    and ladder GEMMs qualify): the fixture's energies and amplitudes to the
+    same bars; rocBLAS runs with atomics off, so a second eager solve and a
+    solve with graph-replayed residuals (rocBLAS captured with its preset
+    workspace) give bitwise-equal amplitudes."""
+    announce("golden h2o4_seed1 with rocBLAS FP64 GEMMs, eager and graph")
+    d = _read("h2o4_seed1")
+    ref = _rebuild(d)
+    c = ugx.Context(0)
+    c.set_autotune(True)
+    c.load_hamiltonian(ref.no, ref.nv, ref.fock, ref.blocks)
+    r = c.ccsd(maxiter=100, r_convergence=1e-10, wfn_ccsd_t=1)
+    assert r["converged"]
+    assert abs(r["e_ccsd"] - d.scalars["eccsd"]) < 1e-10
+    assert abs(r["e_t"] - d.scalars["e_t_fov_tjl"]) < 1e-10
+    t1, t2 = c.amplitudes()
+    assert np.max(np.abs(t1 - d["t1"])) < 1e-9
+    lib = [l for l in c.tile_table().splitlines() if "rocblas" in l]
+    print("\n".join(lib))
+    assert lib, "no FP64 shape tuned to rocBLAS"
+    for graph in (0, 1):
+        r2 = c.ccsd(maxiter=100, r_convergence=1e-10, wfn_ccsd_t=1, graph=graph)
+        u1, u2 = c.amplitudes()
+        assert r2["niter"] == r["niter"] and r2["e_ccsd"] == r["e_ccsd"], graph
+        np.testing.assert_array_equal(u1, t1)
+        np.testing.assert_array_equal(u2, t2)
+    c.close()

@@ -1,6 +1,8 @@
 // Contraction engine (see ugx_engine.h).
 #include "ugx_engine.h"
 
+#include <rocblas/rocblas.h>
+
 #include <cstdlib>
 
 #include <algorithm>
@@ -79,6 +81,8 @@ size_t esize(int dtype) { return dtype == 0 ? 8 : 4; }
 }  // namespace
 
 Engine::~Engine() {
+  if (blas_) (void)rocblas_destroy_handle(blas_);
+  if (blas_ws_) (void)hipFree(blas_ws_);
   for (auto &s : slots_)
     if (s.first) (void)hipFree(s.first);
   for (void *p : retired_) (void)hipFree(p);
@@ -440,7 +444,7 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
           if (!direct) (void)slot(2, (size_t)Re * Ce * 8);
           return;
         }
-        launch_gemm(d2, nullptr, kprec, t2, st_);
+        run_gemm(d2, kprec, t2);
         if (direct) {
           launch_splitk_reduce((int)S, (int)Re, (int)Ce, W, alpha, beta, (double *)C.p, Ce, 1, st_);
         } else {
@@ -463,22 +467,27 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
   if (direct) {
     d.C = (double *)C.p; d.scm = Ce; d.scn = 1; d.scm1 = 0; d.alpha = alpha; d.beta = beta;
     const int tile = tile_for(d, nullptr, kprec, -1);
+    if (tile == TILE_LIBRARY) blas_ready();
     if (sizing_) return;
-    launch_gemm(d, nullptr, kprec, tile, st_);
-  } else if (two_level(rows, cols)) {
+    run_gemm(d, kprec, tile);
+  } else if (two_level(rows, cols) &&
+             !(fp64 && autotune_ && (double)Re * (double)Ce * (double)Ke >= 1073741824.0)) {
+    // (a large FP64 GEMM takes the plain output below instead, where the
+    // tuner can also time rocBLAS; the permute costs a few percent of it)
     const OutMap rm = outmap(rows), cm = outmap(cols);
     d.C = (double *)C.p; d.alpha = alpha; d.beta = beta;
     d.scm = rm.s0; d.cdiv = (int)rm.div; d.scm1 = rm.s1;
     d.scn = cm.s0; d.ndiv = (int)cm.div; d.scn1 = cm.s1;
     const int tile = tile_for(d, nullptr, kprec, -1);
     if (sizing_) return;
-    launch_gemm(d, nullptr, kprec, tile, st_);
+    run_gemm(d, kprec, tile);
   } else {
     double *tmp = (double *)slot(2, (size_t)Re * Ce * 8);
     d.C = tmp; d.scm = Ce; d.scn = 1; d.scm1 = 0; d.alpha = 1.0; d.beta = 0.0;
     const int tile = tile_for(d, nullptr, kprec, -1);
+    if (tile == TILE_LIBRARY) blas_ready();
     if (sizing_) return;
-    launch_gemm(d, nullptr, kprec, tile, st_);
+    run_gemm(d, kprec, tile);
     TRef T = vref(tmp, {});
     for (char ch : rows + cols) T.dims.push_back(ext.at(ch));
     permute(rows + cols + "->" + sp.c, alpha, T, beta, C);
@@ -533,6 +542,66 @@ long out_extent(const GemmDesc &d, long table_extent) {
 }
 
 }  // namespace
+
+const char *tile_name(int tile) { return tile == TILE_LIBRARY ? "rocblas" : gemm_tile_name(tile); }
+
+// A single FP64 GEMM with plain row-major C (unit column stride), unit
+// stride along k or along the row / column index of each operand, and at
+// least 2^30 multiply-adds (smaller ones stay on the library's kernel,
+// where launch cost dominates).
+bool Engine::blas_fits(const GemmDesc &d, int kprec) const {
+  if (kprec != PREC_FP64 || d.nseg != 1 || d.batch != 1 || d.offm || d.offn || d.offz || d.cdiv || d.ndiv ||
+      d.c_f32 || d.tag || d.ktotal || d.zsub || d.scn != 1 || d.scm < d.N)
+    return false;
+  const GemmSeg &g = d.seg[0];
+  if (g.sign != 1.0 || g.pblkA || g.pblkB || !g.A || !g.B) return false;
+  const bool a_ok = (g.sak == 1 && g.sam >= g.K) || (g.sam == 1 && g.sak >= d.M);
+  const bool b_ok = (g.sbn == 1 && g.sbk >= d.N) || (g.sbk == 1 && g.sbn >= g.K);
+  return a_ok && b_ok && (double)d.M * (double)d.N * (double)g.K >= 1073741824.0;
+}
+
+void Engine::blas_ready() {
+  if (blas_) return;
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) throw std::runtime_error("ugx: rocblas_create_handle failed");
+  blas_ = h;
+  const size_t ws = 32u << 20;
+  hip_check(hipMalloc(&blas_ws_, ws), "rocBLAS workspace");
+  if (rocblas_set_stream(h, st_) != rocblas_status_success || rocblas_set_workspace(h, blas_ws_, ws) != rocblas_status_success ||
+      rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed) != rocblas_status_success ||
+      rocblas_set_pointer_mode(h, rocblas_pointer_mode_host) != rocblas_status_success)
+    throw std::runtime_error("ugx: rocBLAS handle set-up failed");
+}
+
+// Row-major C (M x N, row stride scm) = alpha A (M x K) B (K x N) + beta C
+// as the column-major product C^T = B^T A^T.
+void Engine::blas_gemm(const GemmDesc &d) {
+  const GemmSeg &g = d.seg[0];
+  const bool bn = (g.sbn == 1 && g.sbk >= d.N);
+  const bool an = (g.sak == 1 && g.sam >= g.K);
+  const rocblas_operation opB = bn ? rocblas_operation_none : rocblas_operation_transpose;
+  const rocblas_operation opA = an ? rocblas_operation_none : rocblas_operation_transpose;
+  const rocblas_int ldb = (rocblas_int)(bn ? g.sbk : g.sbn), lda = (rocblas_int)(an ? g.sam : g.sak);
+  const double alpha = d.alpha, beta = d.beta;
+  const rocblas_status r = rocblas_dgemm(blas_, opB, opA, d.N, d.M, g.K, &alpha, (const double *)g.B, ldb,
+                                         (const double *)g.A, lda, &beta, (double *)d.C, (rocblas_int)d.scm);
+  if (r != rocblas_status_success)
+    throw std::runtime_error(std::string("ugx: rocblas_dgemm failed: ") + rocblas_status_to_string(r));
+}
+
+// A tuned tile runs on the library's kernel or, for TILE_LIBRARY, on
+// rocBLAS (a table entry that no longer fits falls back to the heuristic tile)
+void Engine::run_gemm(const GemmDesc &d, int kprec, int tile) {
+  if (tile == TILE_LIBRARY) {
+    if (blas_fits(d, kprec)) {
+      blas_ready();
+      blas_gemm(d);
+      return;
+    }
+    tile = -1;
+  }
+  launch_gemm(d, nullptr, kprec, tile, st_);
+}
 
 int Engine::tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kprec, int fallback) {
   if (!autotune_) return fallback;
@@ -593,15 +662,22 @@ int Engine::tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kpre
   int best = fallback >= 0 ? fallback : gemm_pick_tile(d);
   double best_ms = 1e300;
   const int reps = 3;
-  for (int tile = 0; tile < gemm_num_tiles(); tile++) {
-    launch_gemm(t, tab_dev, kprec, tile, st_);  // warm
+  const bool lib = !ptrs_host && blas_fits(d, kprec);
+  if (lib) blas_ready();
+  for (int tile = 0; tile < gemm_num_tiles() + (lib ? 1 : 0); tile++) {
+    const int id = tile < gemm_num_tiles() ? tile : TILE_LIBRARY;
+    if (id == TILE_LIBRARY) blas_gemm(t);  // warm
+    else launch_gemm(t, tab_dev, kprec, id, st_);
     hip_check(hipEventRecord(e0, st_), "event record");
-    for (int r = 0; r < reps; r++) launch_gemm(t, tab_dev, kprec, tile, st_);
+    for (int r = 0; r < reps; r++) {
+      if (id == TILE_LIBRARY) blas_gemm(t);
+      else launch_gemm(t, tab_dev, kprec, id, st_);
+    }
     hip_check(hipEventRecord(e1, st_), "event record");
     hip_check(hipEventSynchronize(e1), "event sync");
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
-    if (ms / reps < best_ms) { best_ms = ms / reps; best = tile; }
+    if (ms / reps < best_ms) { best_ms = ms / reps; best = id; }
   }
   free_events_.push_back(e0);
   free_events_.push_back(e1);
@@ -616,7 +692,7 @@ std::string Engine::tile_table() const {
   std::string s;
   std::lock_guard<std::mutex> lk(g_tiles_mu);
   for (auto &kv : g_tiles)
-    s += kv.first + " -> " + gemm_tile_name(kv.second.tile) + " " + std::to_string(kv.second.ms) + " ms\n";
+    s += kv.first + " -> " + tile_name(kv.second.tile) + " " + std::to_string(kv.second.ms) + " ms\n";
   return s;
 }
 
@@ -627,7 +703,7 @@ bool tiles_save(const std::string &path) {
   {
     std::lock_guard<std::mutex> lk(g_tiles_mu);
     for (auto &kv : g_tiles)
-      s += kv.first + "\t" + gemm_tile_name(kv.second.tile) + "\t" + std::to_string(kv.second.ms) + "\n";
+      s += kv.first + "\t" + tile_name(kv.second.tile) + "\t" + std::to_string(kv.second.ms) + "\n";
   }
   const std::string tmp = path + ".tmp";
   FILE *f = std::fopen(tmp.c_str(), "w");
@@ -642,6 +718,7 @@ long tiles_load(const std::string &path) {
   if (!f) return -1;
   std::map<std::string, int> by_name;
   for (int t = 0; t < gemm_num_tiles(); t++) by_name[gemm_tile_name(t)] = t;
+  by_name[tile_name(TILE_LIBRARY)] = TILE_LIBRARY;
   long n = 0;
   std::string line;
   char buf[4096];

@@ -22,6 +22,8 @@
 
 #include "ugx_internal.h"
 
+struct _rocblas_handle;  // rocBLAS handle (rocblas_handle)
+
 namespace ugx {
 
 struct TRef {
@@ -39,6 +41,10 @@ struct KernelStat {
   double ms = 0.0;
   double flops = 0.0;
 };
+
+// Tile id of a GEMM the autotuner hands to rocBLAS (plain FP64 GEMMs only,
+// when its timing beats every tile of the library's kernel)
+constexpr int TILE_LIBRARY = 1000;
 
 class Engine {
  public:
@@ -143,6 +149,16 @@ class Engine {
   bool autotune_ = false;
   long out_extent_hint_ = 0;
   int tile_for_impl(const GemmDesc &d, const GemmPtrs *ptrs_host, int kprec, int fallback);
+  // rocBLAS for large single FP64 GEMMs with unit-stride operands: one
+  // handle on this engine's stream (created at the first tuning or sizing
+  // use, with its workspace, so no allocation happens in a captured or
+  // timed region), atomics off (bitwise repeatable)
+  ::_rocblas_handle *blas_ = nullptr;
+  void *blas_ws_ = nullptr;
+  bool blas_fits(const GemmDesc &d, int kprec) const;
+  void blas_ready();
+  void blas_gemm(const GemmDesc &d);
+  void run_gemm(const GemmDesc &d, int kprec, int tile);
   std::map<std::tuple<const void *, std::string, int>, std::pair<void *, size_t>> cache_;
   std::map<std::tuple<const void *, std::string, int>, PermDesc> cache_pd_;  // how each entry was filled
   bool profiling_ = false;
