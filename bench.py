@@ -169,6 +169,114 @@ def cpu_full_h2o():
     return out
 
 
+def _post_gpu(ugx, ref, reps=2):
+    """The steps after a converged CCSD(T) on the GPU (FP64, R_CONVERGENCE
+    1e-10): Lambda (HBAR first), the densities with their energies, the (T)
+    gradient terms, and one dipole perturbation (mu_z, omega = 0) solved
+    right- and left-hand with its linresp; seconds of the last of `reps`
+    passes (the first one tunes tiles), each step bracketed by
+    synchronisations."""
+    c = ugx.Context(0)
+    c.set_autotune(True)
+    c.load_hamiltonian(ref.no, ref.nv, ref.fock, ref.blocks)
+    r = c.ccsd(maxiter=100, r_convergence=FP64_CONV, wfn_ccsd_t=1)
+    mu = ref.perturbation("Mu")[2]
+
+    def timed(fn):
+        c.synchronize()
+        t0 = time.perf_counter()
+        x = fn()
+        c.synchronize()
+        return x, time.perf_counter() - t0
+
+    out = {}
+    for _ in range(reps):
+        lam, t_lam = timed(lambda: c.lambda_solve(maxiter=100, r_convergence=FP64_CONV))
+        den, t_den = timed(lambda: c.density(which=()))
+        _, t_tg = timed(lambda: c.tgrad(which=()))
+        g, t_pc = timed(lambda: c.pert(mu, 0.0))
+        xr, t_x = timed(lambda: g.solve(0, maxiter=100, r_convergence=FP64_CONV))
+        yr, t_y = timed(lambda: g.solve(1, maxiter=100, r_convergence=FP64_CONV))
+        alpha, t_lr = timed(lambda: g.linresp(g))
+        g.release()
+        out = dict(lambda_s=t_lam, lambda_iterations=lam["niter"], density_s=t_den, tgrad_s=t_tg,
+                   pert_create_s=t_pc, pert_right_s=t_x, pert_right_iterations=xr["niter"], pert_left_s=t_y,
+                   pert_left_iterations=yr["niter"], linresp_s=t_lr, alpha_zz=alpha,
+                   e_density=den["e_one"] + den["e_two"], e_ccsd=r["e_ccsd"],
+                   converged=bool(lam["converged"] and xr["converged"] and yr["converged"]))
+    c.close()
+    return out
+
+
+def _post_oracle(ref, left_iterations):
+    """The same steps with the oracle on one thread (the reference's
+    serial loops, restated), at the same convergence; the left-hand solve
+    (its Y2 terms are o^4 v^4 loops, ~2 s per iteration at H2O) is a
+    three-iteration sample scaled to `left_iterations`."""
+    from oracle import Oracle
+    o = Oracle(ref.fock, ref.full_ints(), ref.no)
+    assert o.ccsd(maxiter=100, conv=FP64_CONV)["converged"]
+    o.build_tstar()
+    mu = ref.perturbation("Mu")[2]
+    out = {}
+    t0 = time.perf_counter()
+    o.lambda_init()
+    lam = o.lambda_solve(maxiter=100, conv=FP64_CONV)
+    out["lambda_s"] = time.perf_counter() - t0
+    out["lambda_iterations"] = lam["niter"]
+    t0 = time.perf_counter()
+    o.density()
+    out["density_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.tgrad(enable=False)
+    out["tgrad_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    P = o.pert(mu, 0.0)
+    out["pert_create_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    xr = P.solve(0, maxiter=100, conv=FP64_CONV)
+    out["pert_right_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    yr = P.solve(1, maxiter=3, conv=FP64_CONV)
+    out["pert_left_s"] = (time.perf_counter() - t0) * left_iterations / max(1, yr["niter"])
+    out["pert_left_sample"] = "%d iterations timed, scaled to %d" % (yr["niter"], left_iterations)
+    t0 = time.perf_counter()
+    out["alpha_zz"] = P.linresp(P)
+    out["linresp_s"] = time.perf_counter() - t0
+    out.update(pert_right_iterations=xr["niter"], cores=1, kind="port")
+    out.pop("alpha_zz")  # Y is not converged in the sample
+    P.close()
+    o.close()
+    return out
+
+
+def post_ccsd_measure(cpu=True):
+    """Lambda, densities, (T) gradient terms and the linear response after a
+    converged CCSD(T): the GPU at (H2O)4/cc-pVDZ (o=20, v=76) and at
+    H2O/cc-pVDZ (o=5, v=19), where the serial oracle runs the same steps
+    beside it (its (H2O)4 steps take minutes to hours: not sampled here)."""
+    from ugacc_amd import ugx
+    from ugacc_amd.chem import molecules
+    from ugacc_amd.chem.scf import Reference
+    out = {}
+    h2o = Reference(molecules.to_bohr(molecules.water()), "cc-pvdz", log=None)
+    for label, ref in (("h2o4", Reference(molecules.water_cluster(4, seed=1), "cc-pvdz", log=None)), ("h2o", h2o)):
+        t0 = time.perf_counter()
+        out[label] = _post_gpu(ugx, ref)
+        log("post-CCSD steps %s on the GPU: %s (%.1f s with the CCSD and the tuning pass)"
+            % (label, ", ".join("%s %.3f" % (k, v) for k, v in out[label].items() if k.endswith("_s")),
+               time.perf_counter() - t0))
+    if cpu:
+        out["h2o_oracle"] = _post_oracle(h2o, out["h2o"]["pert_left_iterations"])
+        log("post-CCSD steps h2o, oracle (1 thread): %s"
+            % ", ".join("%s %.3f" % (k, v) for k, v in out["h2o_oracle"].items() if k.endswith("_s")))
+    out["note"] = ("seconds per step after a converged FP64 CCSD(T) (R_CONVERGENCE 1e-10): Lambda with HBAR, "
+                   "the densities and their energies, the (T) gradient terms, one dipole perturbation (mu_z, "
+                   "omega 0) right- and left-hand and its linresp; GPU = the library (FP64), oracle = the "
+                   "restated reference loops on one thread")
+    return out
+
+
 def cold_solve(ugx, no, nv, fock, blocks, opts, net, label):
     """The first ugx_ccsd after ugx_load_hamiltonian on a fresh context:
     every per-Hamiltonian conversion (integral copies in each precision,
@@ -379,6 +487,8 @@ def main():
     ap.add_argument("--ref-warmup", type=int, default=1)
     ap.add_argument("--no-batch", action="store_true", help="skip the batch64 object of the default line")
     ap.add_argument("--no-cold", action="store_true", help="skip the cold (first-solve) measurements")
+    ap.add_argument("--no-post", action="store_true",
+                    help="skip the post-CCSD object (Lambda, densities, (T) gradient, response timings)")
     ap.add_argument("--batch-polish-mode", default="plain", choices=["refine", "plain"])
     ap.add_argument("--tiles", default=None, help="tuned tile table to load first (ugx_load_tile_table)")
     ap.add_argument("--save-tiles", default=None, help="write the tuned tile table here after the run")
@@ -534,6 +644,8 @@ def main():
     }
     line.update(out)
     line.update({"kernels": stats, "roofline": roofline(stats, args.workload), "cpu_baseline": cpu})
+    if args.form == "mixed" and args.workload == "h2o8" and not args.no_post:
+        line["post_ccsd"] = post_ccsd_measure(cpu=not args.no_cpu)
     if args.form == "mixed" and args.workload == "h2o8" and not args.no_batch:
         ctx.close()
         args.batch_polish_mode = args.batch_polish_mode or "plain"

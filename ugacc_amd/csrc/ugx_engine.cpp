@@ -470,10 +470,7 @@ void Engine::contract(const std::string &s, double alpha, const TRef &A, const T
     if (tile == TILE_LIBRARY) blas_ready();
     if (sizing_) return;
     run_gemm(d, kprec, tile);
-  } else if (two_level(rows, cols) &&
-             !(fp64 && autotune_ && (double)Re * (double)Ce * (double)Ke >= 1073741824.0)) {
-    // (a large FP64 GEMM takes the plain output below instead, where the
-    // tuner can also time rocBLAS; the permute costs a few percent of it)
+  } else if (two_level(rows, cols) && !plain_to_library(d, kprec, Re, Ce)) {
     const OutMap rm = outmap(rows), cm = outmap(cols);
     d.C = (double *)C.p; d.alpha = alpha; d.beta = beta;
     d.scm = rm.s0; d.cdiv = (int)rm.div; d.scm1 = rm.s1;
@@ -587,6 +584,17 @@ void Engine::blas_gemm(const GemmDesc &d) {
                                          (const double *)g.A, lda, &beta, (double *)d.C, (rocblas_int)d.scm);
   if (r != rocblas_status_success)
     throw std::runtime_error(std::string("ugx: rocblas_dgemm failed: ") + rocblas_status_to_string(r));
+}
+
+// A GEMM whose output needs the two-level map takes the plain temporary and
+// a permute instead when the tuner picks rocBLAS for that plain form (large
+// FP64 GEMMs only; the permute costs a few percent of such a GEMM)
+bool Engine::plain_to_library(const GemmDesc &d, int kprec, long Re, long Ce) {
+  if (kprec != PREC_FP64 || !autotune_ || (double)Re * (double)Ce * (double)d.seg[0].K < 1073741824.0) return false;
+  GemmDesc p = d;
+  p.C = (double *)slot(2, (size_t)Re * Ce * 8);
+  p.scm = Ce; p.scn = 1; p.scm1 = 0; p.cdiv = 0; p.ndiv = 0; p.scn1 = 0; p.alpha = 1.0; p.beta = 0.0;
+  return blas_fits(p, kprec) && tile_for(p, nullptr, kprec, -1) == TILE_LIBRARY;
 }
 
 // A tuned tile runs on the library's kernel or, for TILE_LIBRARY, on

@@ -159,6 +159,7 @@ class Engine {
   void blas_ready();
   void blas_gemm(const GemmDesc &d);
   void run_gemm(const GemmDesc &d, int kprec, int tile);
+  bool plain_to_library(const GemmDesc &d, int kprec, long Re, long Ce);
   std::map<std::tuple<const void *, std::string, int>, std::pair<void *, size_t>> cache_;
   std::map<std::tuple<const void *, std::string, int>, PermDesc> cache_pd_;  // how each entry was filled
   bool profiling_ = false;
