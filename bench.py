@@ -277,6 +277,43 @@ def post_ccsd_measure(cpu=True):
     return out
 
 
+def library_shapes(wl, reps=10):
+    """The vendor libraries through torch on the workload's GEMM shapes (one
+    plain product each, random operands): BF16 torch.bmm (hipBLASLt) on the
+    (T) W3 launch shapes with their 160-wide tile padding, batch 64 of
+    v^2 x 160 x 2(v + o) and batch 4096 of 160 x 160 x 4(v + o); FP64 matmul
+    (rocBLAS) on the ring (ov x ov x ov) and the packed ladder
+    (o(o+1)/2 x v(v+1)/2 x v(v+1)/2)."""
+    import torch
+    o, v = wl["no"], wl["nv"]
+
+    def run(shape_a, shape_b, dtype, batched):
+        a = torch.randn(*shape_a, device="cuda", dtype=dtype)
+        b = torch.randn(*shape_b, device="cuda", dtype=dtype)
+        f = (lambda: torch.bmm(a, b)) if batched else (lambda: a @ b)
+        f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        m, k = shape_a[-2], shape_a[-1]
+        n = shape_b[-1]
+        bat = shape_a[0] if batched else 1
+        return dict(shape=[bat, m, n, k], ms=dt * 1e3, tflops=2.0 * bat * m * n * k / dt * 1e-12)
+
+    v2, ov, pv, po = v * v, o * v, v * (v + 1) // 2, o * (o + 1) // 2
+    return {
+        "bf16_bmm_w3_za": run((64, v2, 2 * (v + o)), (64, 2 * (v + o), 160), torch.bfloat16, True),
+        "bf16_bmm_w3_merged": run((4096, 160, 4 * (v + o)), (4096, 4 * (v + o), 160), torch.bfloat16, True),
+        "fp64_ring": run((ov, ov), (ov, ov), torch.float64, False),
+        "fp64_ladder": run((po, pv), (pv, pv), torch.float64, False),
+        "note": "one BF16 product per call (the library kernel's BF16x3 W3 runs three); torch.bmm is hipBLASLt, "
+                "the FP64 matmul rocBLAS",
+    }
+
+
 def cold_solve(ugx, no, nv, fock, blocks, opts, net, label):
     """The first ugx_ccsd after ugx_load_hamiltonian on a fresh context:
     every per-Hamiltonian conversion (integral copies in each precision,
@@ -501,7 +538,13 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (torch) five times: the reference kernel whose "
                          "FETCH_SIZE / WRITE_SIZE counters calibrate the HBM traffic of the library's kernels")
+    ap.add_argument("--library-shapes", action="store_true",
+                    help="only time the vendor libraries on this workload's GEMM shapes: hipBLASLt BF16 batched "
+                         "products of the two (T) W3 launch shapes, rocBLAS FP64 on the ring and ladder shapes")
     args = ap.parse_args()
+    if args.library_shapes:
+        print(json.dumps(library_shapes(WORKLOADS[args.workload])), flush=True)
+        return
     if args.copy_calibration:
         import torch
         x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
