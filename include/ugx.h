@@ -29,6 +29,7 @@
  *   ugx_hbar              <- HBAR::HBAR, hbar.cc:8-228
  *   ugx_lambda            <- CCLambda::compute_lambda, cclambda.cc:93-135
  *   ugx_tgrad             <- CCWfn::tgrad, ccwfn.cc:854-1098
+ *   ugx_tgrad_ooc         <- CCWfn::tgrad_ooc, ccwfn.cc:1150-1316
  *   ugx_density           <- CCDensity::onepdm / twopdm, ccdensity.cc:89-339
  *   ugx_pert_create       <- CCPert::CCPert + pertbar, ccpert.cc:11-176
  *   ugx_pert_solve        <- CCPert::solve, ccpert.cc:178-225
@@ -269,6 +270,17 @@ int ugx_lambda_residual(ugx_ctx *ctx, const double *l1, const double *l2, double
  * Gooov[i][j][k][a], Gvvvo[a][b][c][i], Goovv[i][j][a][b]. */
 int ugx_tgrad(ugx_ctx *ctx, double *s1, double *s2, double *Doo, double *Dvv, double *Dov, double *Gooov,
               double *Gvvvo, double *Goovv);
+
+/* CCWfn::tgrad_ooc (ccwfn.cc:1150-1316; compute_energy runs it instead of
+ * tgrad when OOC=true, :252-255): the same terms and outputs as ugx_tgrad,
+ * except that the (T) parts of Doo and Dvv keep only their diagonals (the
+ * reference accumulates Doo[i][i] and Dvv[a][a] only, :1197, :1242).  For
+ * canonical Fock matrices that is tgrad_ooc's result.  tgrad_ooc's s2
+ * takes a disconnected f_ov term in another form than tgrad's, a difference
+ * linear in f_ov: up to |f_ov| = 1e-8 (a converged SCF's residual) it stays
+ * below ~1e-8 of s2's scale; beyond that the call returns UGX_BAD_INPUT. */
+int ugx_tgrad_ooc(ugx_ctx *ctx, double *s1, double *s2, double *Doo, double *Dvv, double *Dov, double *Gooov,
+                  double *Gvvvo, double *Goovv);
 
 /* CCDensity::onepdm + twopdm (ccdensity.cc:89-339): the CCSD one- and
  * two-particle density blocks, built on the device in FP64 from the

@@ -73,6 +73,7 @@ def _lib(omp=False):
         L.orc_density.argtypes = [vp] + [dp] * 11
         L.orc_lambda_pseudoenergy.argtypes = [vp]
         L.orc_tgrad.argtypes = [vp] + [dp] * 8
+        L.orc_tgrad_ooc.argtypes = [vp] + [dp] * 8
         L.orc_pert_create.restype = vp
         L.orc_pert_create.argtypes = [vp, dp, ctypes.c_double]
         L.orc_pert_destroy.argtypes = [vp]
@@ -313,6 +314,17 @@ class Oracle:
                   Gvvvo=(v, v, v, o), Goovv=(o, o, v, v))
         out = {k: np.zeros(sh[k]) for k in self.TGRAD_BLOCKS}
         self.L.orc_tgrad(self.h, *[_p(out[k]) for k in self.TGRAD_BLOCKS])
+        self.L.orc_tgrad_enable(self.h, int(enable))
+        return out
+
+    def tgrad_ooc(self, enable=True):
+        """CCWfn::tgrad_ooc (ccwfn.cc:1150-1316), the OOC=true form: the same
+        blocks as tgrad(), replacing its terms in the state."""
+        o, v = self.no, self.nv
+        sh = dict(s1=(o, v), s2=(o, o, v, v), Doo=(o, o), Dvv=(v, v), Dov=(o, v), Gooov=(o, o, o, v),
+                  Gvvvo=(v, v, v, o), Goovv=(o, o, v, v))
+        out = {k: np.zeros(sh[k]) for k in self.TGRAD_BLOCKS}
+        self.L.orc_tgrad_ooc(self.h, *[_p(out[k]) for k in self.TGRAD_BLOCKS])
         self.L.orc_tgrad_enable(self.h, int(enable))
         return out
 

@@ -1797,6 +1797,116 @@ void orc_tgrad(void *p, double *s1_o, double *s2_o, double *Doo_o, double *Dvv_o
 #undef OUT
 }
 
+/* CCWfn::tgrad_ooc (ccwfn.cc:1150-1316, the OOC=true form): the same
+ * outputs as orc_tgrad from W and V blocks formed one (i,j,k) at a time
+ * (M3_ijk, N3_ijk, :1420-1515) and, for Doo, one (a,b,c) at a time
+ * (M3_abc, N3_abc, :1456-1540), with no t3 or l3 array; Doo and Dvv get
+ * their diagonals only.  The outputs replace those of orc_tgrad in the
+ * state (orc_tgrad_enable applies to them alike). */
+void orc_tgrad_ooc(void *p, double *s1_o, double *s2_o, double *Doo_o, double *Dvv_o, double *Dov_o,
+                   double *Gooov_o, double *Gvvvo_o, double *Goovv_o) {
+  orc_state *s = (orc_state *)p;
+  const int no = s->no, nv = s->nv;
+  const size_t o = no, v = nv, vvv = v * v * v, ooo = o * o * o;
+  const double *t1 = s->t1, *t2 = s->t2;
+  free(s->tg_s1); free(s->tg_s2); free(s->tg_Doo); free(s->tg_Dvv); free(s->tg_Dov);
+  free(s->tg_Gooov); free(s->tg_Gvvvo); free(s->tg_Goovv);
+  double *s1 = s->tg_s1 = zalloc(o * v), *s2 = s->tg_s2 = zalloc(o * o * v * v);
+  double *Doo = s->tg_Doo = zalloc(o * o), *Dvv = s->tg_Dvv = zalloc(v * v), *Dov = s->tg_Dov = zalloc(o * v);
+  double *Gooov = s->tg_Gooov = zalloc(o * o * o * v), *Gvvvo = s->tg_Gvvvo = zalloc(v * v * v * o);
+  double *Goovv = s->tg_Goovv = zalloc(o * o * v * v);
+  double *X2 = zalloc(o * o * v * v);
+#define DEN3(i, j, k, a, b, c) \
+  (F(i, i) + F(j, j) + F(k, k) - F((a) + no, (a) + no) - F((b) + no, (b) + no) - F((c) + no, (c) + no))
+#define NUMN(i, j, k, a, b, c)                                                                              \
+  (INT(i, j, (a) + no, (b) + no) * t1[O2(k, c)] + INT(i, k, (a) + no, (c) + no) * t1[O2(j, b)] +          \
+   INT(j, k, (b) + no, (c) + no) * t1[O2(i, a)] + t2[T4(i, j, a, b)] * F(k, (c) + no) +                   \
+   t2[T4(i, k, a, c)] * F(j, (b) + no) + t2[T4(j, k, b, c)] * F(i, (a) + no))
+  double *M3 = zalloc(vvv), *N3 = zalloc(vvv), *X3 = zalloc(vvv), *Y3 = zalloc(vvv), *Z3 = zalloc(vvv);
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int k = 0; k < no; k++) {
+        for (int a = 0; a < nv; a++)
+          for (int b = 0; b < nv; b++)
+            for (int c = 0; c < nv; c++) {
+              const double den = DEN3(i, j, k, a, b, c);
+              M3[V3(a, b, c)] = w3_elem(s, t2, i, j, k, a, b, c) / den;
+              N3[V3(a, b, c)] = NUMN(i, j, k, a, b, c) / den;
+            }
+#define P6(X, a, b, c)                                                                                       \
+  (8.0 * X[V3(a, b, c)] - 4.0 * X[V3(b, a, c)] - 4.0 * X[V3(a, c, b)] - 4.0 * X[V3(c, b, a)] +             \
+   2.0 * X[V3(c, a, b)] + 2.0 * X[V3(b, c, a)])
+        for (int a = 0; a < nv; a++)
+          for (int b = 0; b < nv; b++)
+            for (int c = 0; c < nv; c++) {
+              X3[V3(a, b, c)] = P6(M3, a, b, c);
+              Y3[V3(a, b, c)] = P6(N3, a, b, c);
+              Z3[V3(a, b, c)] = 2.0 * (M3[V3(a, b, c)] - M3[V3(a, c, b)]) - (M3[V3(b, a, c)] - M3[V3(b, c, a)]);
+            }
+        for (int a = 0; a < nv; a++)
+          for (int b = 0; b < nv; b++)
+            for (int c = 0; c < nv; c++) {
+              const double m = M3[V3(a, b, c)], xy = 2.0 * X3[V3(a, b, c)] + Y3[V3(a, b, c)];
+              Dvv[(size_t)a * v + a] += 0.5 * m * (X3[V3(a, b, c)] + Y3[V3(a, b, c)]);
+              Dov[O2(i, a)] += (m - M3[V3(c, b, a)]) * (4.0 * t2[T4(j, k, b, c)] - 2.0 * t2[T4(j, k, c, b)]);
+              s1[O2(i, a)] += (4.0 * m - 4.0 * M3[V3(b, a, c)] - 2.0 * M3[V3(a, c, b)] + 2.0 * M3[V3(c, a, b)]) *
+                              INT(j, k, b + no, c + no);
+              Goovv[T4(i, j, a, b)] += 4.0 * t1[O2(k, c)] * Z3[V3(a, b, c)];
+              for (int l = 0; l < no; l++) {
+                X2[T4(i, l, a, b)] -= xy * INT(j, k, l, c + no);
+                Gooov[(((size_t)j * o + i) * o + l) * v + a] -= xy * t2[T4(l, k, b, c)];
+              }
+              for (int d = 0; d < nv; d++) {
+                X2[T4(i, j, a, d)] += xy * INT(d + no, k, b + no, c + no);
+                Gvvvo[(((size_t)a * v + b) * v + d) * o + j] += xy * t2[T4(k, i, c, d)];
+              }
+            }
+      }
+  free(M3); free(N3); free(X3); free(Y3); free(Z3);
+  M3 = zalloc(ooo); N3 = zalloc(ooo); X3 = zalloc(ooo); Y3 = zalloc(ooo);
+#define O3(i, j, k) ((((size_t)(i)) * no + (j)) * no + (k))
+#define Q6(X, i, j, k)                                                                                       \
+  (8.0 * X[O3(i, j, k)] - 4.0 * X[O3(j, i, k)] - 4.0 * X[O3(i, k, j)] - 4.0 * X[O3(k, j, i)] +             \
+   2.0 * X[O3(k, i, j)] + 2.0 * X[O3(j, k, i)])
+  for (int a = 0; a < nv; a++)
+    for (int b = 0; b < nv; b++)
+      for (int c = 0; c < nv; c++) {
+        for (int i = 0; i < no; i++)
+          for (int j = 0; j < no; j++)
+            for (int k = 0; k < no; k++) {
+              const double den = DEN3(i, j, k, a, b, c);
+              M3[O3(i, j, k)] = w3_elem(s, t2, i, j, k, a, b, c) / den;
+              N3[O3(i, j, k)] = NUMN(i, j, k, a, b, c) / den;
+            }
+        for (int i = 0; i < no; i++)
+          for (int j = 0; j < no; j++)
+            for (int k = 0; k < no; k++) {
+              X3[O3(i, j, k)] = Q6(M3, i, j, k);
+              Y3[O3(i, j, k)] = Q6(N3, i, j, k);
+            }
+        for (int i = 0; i < no; i++)
+          for (int j = 0; j < no; j++)
+            for (int k = 0; k < no; k++)
+              Doo[(size_t)i * o + i] -= 0.5 * M3[O3(i, j, k)] * (X3[O3(i, j, k)] + Y3[O3(i, j, k)]);
+      }
+  free(M3); free(N3); free(X3); free(Y3);
+  for (int i = 0; i < no; i++)
+    for (int j = 0; j < no; j++)
+      for (int a = 0; a < nv; a++)
+        for (int b = 0; b < nv; b++) s2[T4(i, j, a, b)] = X2[T4(i, j, a, b)] + X2[T4(j, i, b, a)];
+  free(X2);
+#undef DEN3
+#undef NUMN
+#undef P6
+#undef Q6
+#undef O3
+#define OUT(dst, src, n) if (dst) memcpy(dst, src, (n) * sizeof(double))
+  OUT(s1_o, s1, o * v); OUT(s2_o, s2, o * o * v * v); OUT(Doo_o, Doo, o * o); OUT(Dvv_o, Dvv, v * v);
+  OUT(Dov_o, Dov, o * v); OUT(Gooov_o, Gooov, o * o * o * v); OUT(Gvvvo_o, Gvvvo, v * v * v * o);
+  OUT(Goovv_o, Goovv, o * o * v * v);
+#undef OUT
+}
+
 /* 1: orc_lambda and orc_density include the terms of the last orc_tgrad */
 void orc_tgrad_enable(void *p, int on) {
   orc_state *s = (orc_state *)p;

@@ -144,3 +144,48 @@ def test_plugin_first_derivative_ccsd_t_h2o():
     r = plugin.ugacc(ref, opts)
     assert abs(r.density_energy - r.cc_energy) < 1e-10
     assert abs(r.current_energy - r.cc_energy - ref.escf) < 1e-12
+
+
+def test_tgrad_ooc_matches_oracle_and_refuses_fov():
+    """ugx_tgrad_ooc (the OOC=true form, ccwfn.cc:1150-1316) against the
+    oracle's restatement of tgrad_ooc on a canonical synthetic Hamiltonian
+    (every block to 1e-12 of its scale; Doo and Dvv diagonal), the CCSD(T)
+    Lambda and density built on it, and UGX_BAD_INPUT for |f_ov| beyond a
+    converged SCF's residual (here 0.02)."""
+    announce("tgrad_ooc o=3 v=6")
+    no, nv = 3, 6
+    c, fock, ints = loaded(no, nv, 76, scale=0.3)
+    o = Oracle(fock, ints, no)
+    assert o.ccsd(maxiter=400, conv=1e-13)["converged"]
+    o.build_tstar()
+    c.set_amplitudes(*o.amps())
+    ref = o.tgrad_ooc(enable=True)
+    got = c.tgrad(ooc=True)
+    for k in ugx.Context.TGRAD_BLOCKS:
+        assert _rel(got[k], ref[k]) < 1e-12, (k, _rel(got[k], ref[k]))
+    assert np.max(np.abs(got["Doo"] - np.diag(np.diag(got["Doo"])))) == 0.0
+    assert c.lambda_solve(maxiter=300, r_convergence=1e-12, wfn_ccsd_t=1)["converged"]
+    o.lambda_init()
+    assert o.lambda_solve(maxiter=300, conv=1e-12)["converged"]
+    assert np.max(np.abs(c.lambdas()[1] - o.lambdas()[1])) < 1e-10
+    o.set_lambdas(*c.lambdas())
+    d, do = c.density(), o.density()
+    for k, r in do["blocks"].items():
+        assert _rel(d["blocks"][k], r) < 1e-11, k
+    o.close()
+    c2, _, _ = loaded(no, nv, 77, scale=0.3, fov=0.02)
+    assert c2.ccsd(maxiter=300, r_convergence=1e-12)["converged"]
+    with pytest.raises(ugx.UgxError):
+        c2.tgrad(ooc=True)
+
+
+def test_plugin_first_derivative_ccsd_t_ooc_h2o():
+    announce("plugin CCSD_T DERTYPE=FIRST OOC=true h2o")
+    from ugacc_amd import plugin
+    from ugacc_amd.chem import molecules
+    from ugacc_amd.chem.scf import Reference
+    ref = Reference(molecules.to_bohr(molecules.water()), "cc-pvdz")
+    opts = dict(plugin.read_options(), WFN="CCSD_T", DERTYPE="FIRST", OOC=True, R_CONVERGENCE=1e-11)
+    r = plugin.ugacc(ref, opts)
+    # canonical orbitals: the diagonal-only (T) Doo / Dvv give the same density energy
+    assert abs(r.density_energy - r.cc_energy) < 1e-10

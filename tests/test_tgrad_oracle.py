@@ -160,3 +160,35 @@ def test_converged_density_energy_is_the_ccsd_t_energy(kw):
         assert abs(d["e_one"] + d["e_two"] - r["eccsd"] - et) < 1e-12  # the reference's printed check
     # s2 has the pair symmetry of the l2 residual it enters
     assert np.max(np.abs(tg["s2"] - tg["s2"].transpose(1, 0, 3, 2))) < 1e-15
+
+
+@pytest.mark.parametrize("no,nv,seed", [(3, 5, 4), (2, 6, 9)])
+def test_out_of_core_form_is_the_in_core_form_with_diagonal_doo_dvv(no, nv, seed):
+    """tgrad_ooc (ccwfn.cc:1150-1316, which compute_energy runs with
+    OOC=true) forms W and V one (i,j,k) and one (a,b,c) at a time and
+    accumulates only the diagonals of Doo and Dvv: for a canonical Fock
+    matrix every other block equals tgrad's to rounding, and Doo, Dvv equal
+    the diagonals of tgrad's (whose off-diagonals are not small).  With some
+    f_ov != 0 the two forms differ in s2 (tgrad_ooc's V carries t2 f_ov
+    terms that tgrad's l3 carries in another form), linearly in f_ov: the
+    case ugx_tgrad_ooc refuses beyond |f_ov| = 1e-8."""
+    fock, ints = random_hamiltonian(no, nv, seed=seed, scale=0.3)
+    o = Oracle(fock, ints, no)
+    assert o.ccsd(maxiter=500, conv=1e-13)["converged"]
+    o.build_tstar()
+    a = o.tgrad(enable=False)
+    b = o.tgrad_ooc(enable=False)
+    for k in a:
+        if k in ("Doo", "Dvv"):
+            assert np.max(np.abs(b[k] - np.diag(np.diag(a[k])))) < 1e-15
+            assert np.max(np.abs(a[k] - np.diag(np.diag(a[k])))) > 1e-7
+        else:
+            assert np.max(np.abs(a[k] - b[k])) < 1e-14 * max(1.0, np.max(np.abs(a[k]))), k
+    o.close()
+    fock, ints = random_hamiltonian(no, nv, seed=seed, scale=0.3, fov=0.03)
+    o = Oracle(fock, ints, no)
+    assert o.ccsd(maxiter=500, conv=1e-13)["converged"]
+    o.build_tstar()
+    a, b = o.tgrad(enable=False), o.tgrad_ooc(enable=False)
+    assert np.max(np.abs(a["s2"] - b["s2"])) > 1e-6
+    o.close()

@@ -3312,15 +3312,41 @@ int ugx_lambda_residual(ugx_ctx *c, const double *l1, const double *l2, double *
   });
 }
 
-int ugx_tgrad(ugx_ctx *c, double *s1, double *s2, double *Doo, double *Dvv, double *Dov, double *Gooov,
-              double *Gvvvo, double *Goovv) {
+namespace {
+// ugx_tgrad (ooc = 0) and ugx_tgrad_ooc (ooc = 1): tgrad_ooc
+// (ccwfn.cc:1150-1316) gives tgrad's terms except that the (T) Doo and Dvv
+// keep their diagonals only, for canonical Fock matrices; with f_ov != 0 its
+// s2 takes another disconnected term than tgrad's l3, a difference linear in
+// f_ov: f_ov beyond a converged SCF's residual (1e-8) is refused.
+int tgrad_entry(ugx_ctx *c, int ooc, double *s1, double *s2, double *Doo, double *Dvv, double *Dov, double *Gooov,
+                double *Gvvvo, double *Goovv) {
   return guarded(c, [&]() {
     require_loaded(c);
     if (!c->have_amps) throw UgxError(UGX_NOT_READY, "ugx_tgrad: no amplitudes (run ugx_ccsd first)");
+    if (ooc) {
+      double m = 0.0;
+      for (double x : c->h_fov) m = std::max(m, std::fabs(x));
+      if (m > 1e-8)
+        throw UgxError(UGX_BAD_INPUT, "ugx_tgrad_ooc: the OOC=true (T) gradient terms are reproduced for canonical "
+                                      "orbitals (|f_ov| <= 1e-8); largest |f_ov| is " + std::to_string(m));
+    }
     c->tg_valid = false;
     tgrad(c);
-    c->tg_valid = true;
     const long o = c->no, v = c->nv;
+    if (ooc) {
+      // ccwfn.cc:1197 and :1242 accumulate Dvv[a][a] and Doo[i][i] only
+      for (auto ov : {std::make_pair(&c->tg_oo, o), std::make_pair(&c->tg_vv, v)}) {
+        std::vector<double> h((size_t)ov.second * ov.second);
+        check(hipMemcpyAsync(h.data(), ov.first->p, h.size() * 8, hipMemcpyDeviceToHost, c->st), "tgrad copy");
+        check(hipStreamSynchronize(c->st), "stream sync");
+        for (long p = 0; p < ov.second; p++)
+          for (long q = 0; q < ov.second; q++)
+            if (p != q) h[(size_t)p * ov.second + q] = 0.0;
+        check(hipMemcpyAsync(ov.first->p, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->st), "tgrad copy");
+        check(hipStreamSynchronize(c->st), "stream sync");
+      }
+    }
+    c->tg_valid = true;
     auto get = [&](double *dst, const DBuf &src, long n) {
       if (dst) check(hipMemcpyAsync(dst, src.p, n * 8, hipMemcpyDeviceToHost, c->st), "tgrad copy");
     };
@@ -3335,6 +3361,17 @@ int ugx_tgrad(ugx_ctx *c, double *s1, double *s2, double *Doo, double *Dvv, doub
     check(hipStreamSynchronize(c->st), "stream sync");
     return UGX_OK;
   });
+}
+}  // namespace
+
+int ugx_tgrad(ugx_ctx *c, double *s1, double *s2, double *Doo, double *Dvv, double *Dov, double *Gooov,
+              double *Gvvvo, double *Goovv) {
+  return tgrad_entry(c, 0, s1, s2, Doo, Dvv, Dov, Gooov, Gvvvo, Goovv);
+}
+
+int ugx_tgrad_ooc(ugx_ctx *c, double *s1, double *s2, double *Doo, double *Dvv, double *Dov, double *Gooov,
+                  double *Gvvvo, double *Goovv) {
+  return tgrad_entry(c, 1, s1, s2, Doo, Dvv, Dov, Gooov, Gvvvo, Goovv);
 }
 
 namespace {

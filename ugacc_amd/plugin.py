@@ -92,7 +92,8 @@ class CCWfn:
             w("\tCCSD(T) Total Energy       = %20.14f\n" % (eref + r["e_ccsd"] + r["e_t"]))
             etotal += r["e_t"]
         if self.wfn == "CCSD_T" and self.options.get("DERTYPE", "NONE") == "FIRST":
-            self.ctx.tgrad(which=())  # ccwfn.cc:252-255
+            # tgrad, or tgrad_ooc with OOC=true (ccwfn.cc:252-255)
+            self.ctx.tgrad(which=(), ooc=bool(self.options["OOC"]))
         return etotal
 
     def close(self):

@@ -13,9 +13,10 @@
  *                    matrix are built on the host and handed to ugx_ccsd.
  * Energies are printed in the layout of ccwfn.cc:234-247.  DERTYPE=FIRST
  * runs the post-CC steps of ugacc.cc:117-141 on the GPU: for CCSD_T first
- * the (T) gradient terms (ugx_tgrad, as compute_energy runs tgrad,
- * ccwfn.cc:252-255), then Lambda over HBAR (ugx_lambda, with the (T) sources
- * for CCSD_T) and the density energies (ugx_density, with the (T) parts),
+ * the (T) gradient terms (ugx_tgrad, or ugx_tgrad_ooc with OOC=true, as
+ * compute_energy runs tgrad or tgrad_ooc, ccwfn.cc:252-255), then Lambda
+ * over HBAR (ugx_lambda, with the (T) sources for CCSD_T) and the density
+ * energies (ugx_density, with the (T) parts),
  * printed as the reference prints them, with CURRENT ENERGY set.  Every
  * other DERTYPE (SECOND) adds the linear response of ugacc.cc:143-241 on the
  * GPU (ugx_pert_create / ugx_pert_solve / ugx_linresp): the dipole ("Mu",
@@ -197,7 +198,9 @@ extern "C" PSI_API SharedWavefunction ugacc(SharedWavefunction ref, Options &opt
   // the (T) gradient terms (ccwfn.cc:252-255), then HBAR, Lambda and the
   // densities (ugacc.cc:118-137), in FP64
   if (opt.wfn_ccsd_t) {
-    rc = ugx_tgrad(ctx.c, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    // tgrad, or tgrad_ooc with OOC=true (ccwfn.cc:252-255)
+    rc = (opt.ooc ? ugx_tgrad_ooc : ugx_tgrad)(ctx.c, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                               nullptr);
     if (rc != UGX_OK) throw PSIEXCEPTION(std::string("ugacc (MI355X): ") + ugx_last_error(ctx.c));
   }
   double pseudo = 0.0, lrms = 0.0;

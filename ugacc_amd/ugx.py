@@ -26,7 +26,7 @@ EXPORTS = [
     "ugx_default_options", "ugx_create", "ugx_destroy", "ugx_last_error", "ugx_load_hamiltonian",
     "ugx_set_amplitudes", "ugx_prepare", "ugx_ccsd", "ugx_triples", "ugx_triples_per_triplet", "ugx_w3",
     "ugx_get_amplitudes", "ugx_energy", "ugx_residual", "ugx_intermediates", "ugx_hbar", "ugx_lambda",
-    "ugx_get_lambda", "ugx_lambda_residual", "ugx_tgrad", "ugx_density", "ugx_pert_create", "ugx_pert_solve",
+    "ugx_get_lambda", "ugx_lambda_residual", "ugx_tgrad", "ugx_tgrad_ooc", "ugx_density", "ugx_pert_create", "ugx_pert_solve",
     "ugx_pert_amplitudes", "ugx_pert_residual", "ugx_linresp", "ugx_pert_release", "ugx_set_profiling",
     "ugx_kernel_stats", "ugx_gemm", "ugx_gemm_bf16x3_static", "ugx_synchronize", "ugx_batch_solve", "ugx_set_autotune", "ugx_tile_table",
     "ugx_save_tile_table", "ugx_load_tile_table", "ugx_mlp_num_params", "ugx_mlp_create", "ugx_mlp_destroy",
@@ -104,6 +104,7 @@ def load_library(path=LIBPATH):
     L.ugx_get_lambda.argtypes = [vp, dp, dp]
     L.ugx_lambda_residual.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.ugx_tgrad.argtypes = [vp] + [dp] * 8
+    L.ugx_tgrad_ooc.argtypes = [vp] + [dp] * 8
     pint = ctypes.POINTER(ctypes.c_int)
     L.ugx_pert_create.argtypes = [vp, dp, cd, pint]
     L.ugx_pert_solve.argtypes = [vp, ip, ip, ctypes.POINTER(Options), dp, pint, dp]
@@ -338,18 +339,20 @@ class Context:
 
     TGRAD_BLOCKS = ("s1", "s2", "Doo", "Dvv", "Dov", "Gooov", "Gvvvo", "Goovv")
 
-    def tgrad(self, which=None):
+    def tgrad(self, which=None, ooc=False):
         """ugx_tgrad: the (T) gradient terms of CCSD(T) (ccwfn.cc:854-1098) of
         the current amplitudes: the Lambda sources s1, s2 and the (T) density
         parts named in `which` (default all).  A later lambda_solve with
         wfn_ccsd_t=1 adds the sources, and the density of that Lambda adds
-        the (T) parts."""
+        the (T) parts.  ooc=True: ugx_tgrad_ooc (ccwfn.cc:1150-1316, the
+        OOC=true form: the (T) Doo and Dvv keep their diagonals only)."""
         o, v = self.no, self.nv
         shapes = dict(s1=(o, v), s2=(o, o, v, v), Doo=(o, o), Dvv=(v, v), Dov=(o, v), Gooov=(o, o, o, v),
                       Gvvvo=(v, v, v, o), Goovv=(o, o, v, v))
         which = self.TGRAD_BLOCKS if which is None else which
         out = {k: np.zeros(shapes[k]) for k in which}
-        self._check(self.L.ugx_tgrad(self.h, *[_p(out[k]) if k in out else None for k in self.TGRAD_BLOCKS]))
+        fn = self.L.ugx_tgrad_ooc if ooc else self.L.ugx_tgrad
+        self._check(fn(self.h, *[_p(out[k]) if k in out else None for k in self.TGRAD_BLOCKS]))
         return out
 
     def pert(self, A, omega=0.0):
