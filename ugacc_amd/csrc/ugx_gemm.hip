@@ -795,34 +795,41 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(STAGES == 2 
     int k0;
     const int s = S.locate(t, k0);
     const bool neg = S.sg(d, s) < 0;
-    bf16x8 ah[TM], al[TM], bh[TN], bl[TN];
+    bf16x8 ah[TM], al[TM];
 #pragma unroll
     for (int i = 0; i < TM; i++) {
       const int blk = wm * TM + i;
       ah[i] = *reinterpret_cast<const bf16x8 *>(&Lr[blk][foff]);
       al[i] = *reinterpret_cast<const bf16x8 *>(&Lr[RA + blk][foff]);
     }
+    if (t + PRE < ntiles) issue(t + PRE, Li);
+    // B columns software-pipelined: column j + 1's fragments are read
+    // before column j's MFMAs, so each read overlaps MFMAs
+    auto rdB = [&](int j, uint4 &uh, uint4 &ul) {
+      const int blk = wn * TN + j;
+      uh = *reinterpret_cast<const uint4 *>(&Lr[2 * RA + blk][foff]);
+      ul = *reinterpret_cast<const uint4 *>(&Lr[2 * RA + RB + blk][foff]);
+    };
+    uint4 ch, cl;
+    rdB(0, ch, cl);
 #pragma unroll
     for (int j = 0; j < TN; j++) {
-      const int blk = wn * TN + j;
-      uint4 uh = *reinterpret_cast<const uint4 *>(&Lr[2 * RA + blk][foff]);
-      uint4 ul = *reinterpret_cast<const uint4 *>(&Lr[2 * RA + RB + blk][foff]);
+      uint4 nh = ch, nl = cl;
+      if (j + 1 < TN) rdB(j + 1, nh, nl);
       if (neg) {
-        uh.x ^= 0x80008000u; uh.y ^= 0x80008000u; uh.z ^= 0x80008000u; uh.w ^= 0x80008000u;
-        ul.x ^= 0x80008000u; ul.y ^= 0x80008000u; ul.z ^= 0x80008000u; ul.w ^= 0x80008000u;
+        ch.x ^= 0x80008000u; ch.y ^= 0x80008000u; ch.z ^= 0x80008000u; ch.w ^= 0x80008000u;
+        cl.x ^= 0x80008000u; cl.y ^= 0x80008000u; cl.z ^= 0x80008000u; cl.w ^= 0x80008000u;
       }
-      bh[j] = *reinterpret_cast<const bf16x8 *>(&uh);
-      bl[j] = *reinterpret_cast<const bf16x8 *>(&ul);
-    }
-    if (t + PRE < ntiles) issue(t + PRE, Li);
-#pragma unroll
-    for (int j = 0; j < TN; j++)
+      const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&ch), bl = *reinterpret_cast<const bf16x8 *>(&cl);
 #pragma unroll
       for (int i = 0; i < TM; i++) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
       }
+      ch = nh;
+      cl = nl;
+    }
   };
   issue(0, L0);
   if (STAGES == 3 && ntiles > 1) issue(1, L1);
