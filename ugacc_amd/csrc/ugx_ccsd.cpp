@@ -14,6 +14,8 @@
 // X with a factor 1/2.
 #include <hip/hip_runtime.h>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -197,6 +199,16 @@ struct ugx_ctx {
 };
 
 namespace {
+
+// A named host-side range of an entry point or phase (roctx): rocprofv3
+// --marker-trace records it beside the kernel trace; without a profiler
+// attached the push and pop are no-ops
+struct Phase {
+  explicit Phase(const char *name) { roctxRangePushA(name); }
+  ~Phase() { roctxRangePop(); }
+  Phase(const Phase &) = delete;
+  Phase &operator=(const Phase &) = delete;
+};
 
 void require_loaded(ugx_ctx *c) {
   if (!c->loaded) throw UgxError(UGX_NOT_READY, "no Hamiltonian loaded (call ugx_load_hamiltonian)");
@@ -2843,6 +2855,7 @@ const char *ugx_last_error(const ugx_ctx *c) { return c ? c->err.c_str() : "null
 int ugx_load_hamiltonian(ugx_ctx *c, int no, int nv, const double *fock, const double *oooo, const double *ooov,
                          const double *oovv, const double *ovov, const double *ovvv, const double *vvvv) {
   return guarded(c, [&]() {
+    Phase ph("ugx_load_hamiltonian");
     if (no <= 0 || nv <= 0 || !fock || !oooo || !ooov || !oovv || !ovov || !ovvv || !vvvv)
       throw UgxError(UGX_BAD_INPUT, "ugx_load_hamiltonian: sizes must be positive and all blocks given");
     check(hipStreamSynchronize(c->st), "stream sync");
@@ -3014,6 +3027,7 @@ int ugx_prepare(ugx_ctx *c, const ugx_options *opt_in) {
 
 int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
   return guarded(c, [&]() {
+    Phase ph("ugx_ccsd");
     require_loaded(c);
     ugx_options opt;
     if (opt_in) opt = *opt_in; else ugx_default_options(&opt);
@@ -3098,7 +3112,10 @@ int ugx_ccsd(ugx_ctx *c, const ugx_options *opt_in, ugx_result *res) {
     const bool conv = (rms < opt.r_convergence);
     if (conv && opt.wfn_ccsd_t) {
       auto t2c = std::chrono::steady_clock::now();
-      r.e_t = triples(c, opt.precision, nullptr);
+      {
+        Phase pt("ugx (T)");
+        r.e_t = triples(c, opt.precision, nullptr);
+      }
       r.seconds_triples = std::chrono::duration<double>(std::chrono::steady_clock::now() - t2c).count();
       E.collect_stats();
     }
@@ -3225,6 +3242,7 @@ int ugx_intermediates(ugx_ctx *c, double *Fvv, double *Foo, double *Fov, double 
 int ugx_hbar(ugx_ctx *c, double *Hov, double *Hoo, double *Hvv, double *Hoooo, double *Hvvvv, double *Hvovv,
              double *Hooov, double *Hovvo, double *Hovov, double *Hvvvo, double *Hovoo) {
   return guarded(c, [&]() {
+    Phase ph("ugx_hbar");
     require_loaded(c);
     hbar(c);
     const long o = c->no, v = c->nv;
@@ -3250,6 +3268,7 @@ int ugx_hbar(ugx_ctx *c, double *Hov, double *Hoo, double *Hvv, double *Hoooo, d
 int ugx_lambda(ugx_ctx *c, const ugx_options *opt_in, double *pseudoenergy, int *niter, double *rms,
                double *trace) {
   return guarded(c, [&]() {
+    Phase ph("ugx_lambda");
     require_loaded(c);
     if (!c->have_amps) throw UgxError(UGX_NOT_READY, "ugx_lambda: no amplitudes (run ugx_ccsd first)");
     ugx_options opt;
@@ -3321,6 +3340,7 @@ namespace {
 int tgrad_entry(ugx_ctx *c, int ooc, double *s1, double *s2, double *Doo, double *Dvv, double *Dov, double *Gooov,
                 double *Gvvvo, double *Goovv) {
   return guarded(c, [&]() {
+    Phase ph("ugx_tgrad");
     require_loaded(c);
     if (!c->have_amps) throw UgxError(UGX_NOT_READY, "ugx_tgrad: no amplitudes (run ugx_ccsd first)");
     if (ooc) {
@@ -3435,6 +3455,7 @@ int ugx_pert_create(ugx_ctx *c, const double *pert, double omega, int *handle) {
 int ugx_pert_solve(ugx_ctx *c, int handle, int hand, const ugx_options *opt_in, double *pseudoresponse, int *niter,
                    double *rms) {
   return guarded(c, [&]() {
+    Phase ph("ugx_pert_solve");
     require_loaded(c);
     PertSlot &P = pert_slot(c, handle);
     if (hand != 0 && hand != 1) throw UgxError(UGX_BAD_INPUT, "ugx_pert_solve: hand is 0 (right) or 1 (left)");
@@ -3520,6 +3541,7 @@ int ugx_pert_release(ugx_ctx *c, int handle) {
 int ugx_density(ugx_ctx *c, double *energies, double *Doo, double *Dvv, double *Dov, double *Dvo, double *Goooo,
                 double *Gvvvv, double *Gooov, double *Gvvvo, double *Govov, double *Goovv) {
   return guarded(c, [&]() {
+    Phase ph("ugx_density");
     require_loaded(c);
     if (!c->have_lambda) throw UgxError(UGX_NOT_READY, "ugx_density: no Lambda amplitudes (run ugx_lambda)");
     density(c);
@@ -3664,6 +3686,7 @@ int ugx_synchronize(ugx_ctx *c) {
 int ugx_batch_solve(ugx_ctx *c, int nprob, const ugx_problem *probs, const ugx_options *opt_in, int nworkers,
                     ugx_result *results, double *wall_seconds) {
   return guarded(c, [&]() {
+    Phase ph("ugx_batch_solve");
     if (nprob <= 0 || !probs || !results || nworkers < 1 || nworkers > 64)
       throw UgxError(UGX_BAD_INPUT, "ugx_batch_solve: bad arguments");
     for (int p = 0; p < nprob; p++) {
