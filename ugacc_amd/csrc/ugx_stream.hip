@@ -49,7 +49,31 @@ struct PermArgs {
   double alpha, beta;
   long n;  // total elements (offset of the lo half of a BF16 pair output)
   long pk, pblk, plo;  // MFMA panel output: k extent, row-block stride, lo offset
+  // n < 2^31: output indices are split into coordinates in 32 bits, by
+  // multiply-high with the magic numbers of od[] and pk (div31)
+  int i32;
+  unsigned dm[6], pkm;
+  int ds[6], pks;
 };
+
+// floor(x / d) for 0 <= x < 2^31 without a division (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", PLDI
+// 1994): with l = ceil(log2 d) >= 1 and m = floor(2^(31+l) / d) + 1, which
+// is below 2^32, x m / 2^(31+l) = x / d + e with 0 < e < 2^-l <= 1 / d, so
+// floor(x / d) = mulhi32(x, m) >> (l - 1).  m = 0 stands for d = 1.
+__device__ inline unsigned div31(unsigned x, unsigned m, int s) { return m ? (__umulhi(x, m) >> s) : x; }
+
+void magic31(long d, unsigned &m, int &s) {
+  if (d <= 1) {
+    m = 0;
+    s = 0;
+    return;
+  }
+  int l = 0;
+  while ((1L << l) < d) l++;
+  m = (unsigned)((1ULL << (31 + l)) / (unsigned long long)d + 1);
+  s = l - 1;
+}
 
 // Output element policies: FP64, FP32, and the split BF16 pair (hi at
 // out[idx], lo at out[n + idx], hi = bf16(x), lo = bf16(x - hi), both round
@@ -87,7 +111,15 @@ struct OutP {
   typedef float tile_t;
   __device__ static double get(const PermArgs &, long) { return 0.0; }
   __device__ static void put(const PermArgs &p, long i, double v) {
-    const long r = i / p.pk, k = i - r * p.pk;
+    long r, k;
+    if (p.i32) {
+      const unsigned q = div31((unsigned)i, p.pkm, p.pks);
+      r = q;
+      k = (unsigned)i - q * (unsigned)p.pk;
+    } else {
+      r = i / p.pk;
+      k = i - r * p.pk;
+    }
     const int rr = (int)(r & 15), kk = (int)(k & 31);
     const long a = (r >> 4) * p.pblk + (k >> 5) * 512 + rr * 32 + 8 * panel_chunk(rr, kk >> 3) + (kk & 7);
     float x = (float)v;
@@ -103,11 +135,24 @@ struct OutP {
 template <typename TI, class OW>
 __global__ void k_perm_linear(PermArgs p, long n) {
   for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n; idx += (long)gridDim.x * NT) {
-    long r = idx, off = 0;
-    for (int d = p.rank - 1; d >= 0; d--) {
-      int c = (int)(r % p.od[d]);
-      r /= p.od[d];
-      off += c * p.istr[d];
+    long off = 0;
+    if (p.i32) {
+      // all six dims, unrolled: the ones past the rank have extent 1 and
+      // stride 0
+      unsigned r = (unsigned)idx;
+#pragma unroll
+      for (int d = 5; d >= 0; d--) {
+        const unsigned q = div31(r, p.dm[d], p.ds[d]);
+        off += (long)(r - q * (unsigned)p.od[d]) * p.istr[d];
+        r = q;
+      }
+    } else {
+      long r = idx;
+      for (int d = p.rank - 1; d >= 0; d--) {
+        int c = (int)(r % p.od[d]);
+        r /= p.od[d];
+        off += c * p.istr[d];
+      }
     }
     double v = p.alpha * (double)((const TI *)p.in)[off];
     if (p.beta != 0.0) v += p.beta * OW::get(p, idx);
@@ -320,29 +365,53 @@ __global__ void k_build_Y(int no, int nv, const double *t1, const double *t2, do
 // Jacobi update, ccwfn.cc:555-582, with the t2 residual assembled from its
 // half X (R2_ijab = X_ijab + X_jiba): ||R||^2 partials, t = t_old + R/D,
 // D_ijab = (f_ii + f_jj) - (f_aa + f_bb) so that D_ijab == D_jiba exactly.
-__global__ void k_update(int no, int nv, const double *eps, const double *R1, const double *X, const double *t1old,
-                         const double *t2old, double *t1, double *t2, double *R2out, double *partials, double shift) {
-  long n1 = (long)no * nv, n2 = n1 * n1;
+__global__ __launch_bounds__(NT) void k_update(int no, int nv, const double *eps, const double *R1, const double *X,
+                                               const double *t1old, const double *t2old, double *t1, double *t2,
+                                               double *R2out, double *partials, double shift) {
+  // R_ijab = X_ijab + X_jiba, one 32 x 32 (a, b) tile of one (i, j) pair at
+  // a time: X_jiba is read along a into LDS and taken back transposed, so
+  // both reads of X are coalesced (a per-element read of X_jiba strides by
+  // nv along b and spreads each cache line over the XCDs' caches)
+  __shared__ double sT[32][33];
+  const int tb = (nv + 31) / 32;
+  const long ntiles = (long)no * no * tb * tb;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  constexpr int TY = NT / 32;
+  const long v2 = (long)nv * nv;
   double s = 0.0;
-  for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n2 + n1; idx += (long)gridDim.x * NT) {
-    if (idx < n2) {
-      long r = idx;
-      int b = r % nv; r /= nv;
-      int a = r % nv; r /= nv;
-      int j = r % no; r /= no;
-      int i = (int)r;
-      double R = X[idx] + X[(((long)j * no + i) * nv + b) * nv + a];
-      double D = (eps[i] + eps[j]) - (eps[no + a] + eps[no + b]) + shift;
-      s += R * R;
-      if (R2out) R2out[idx] = R;
-      t2[idx] = t2old[idx] + R / D;
-    } else {
-      long q = idx - n2;
-      int i = q / nv, a = q % nv;
-      double R = R1[q];
-      s += R * R;
-      t1[q] = t1old[q] + R / (eps[i] - eps[no + a] + shift);
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int tbb = (int)(tile % tb);
+    long r = tile / tb;
+    const int tba = (int)(r % tb);
+    r /= tb;
+    const int j = (int)(r % no), i = (int)(r / no);
+    const int a0 = tba * 32, b0 = tbb * 32;
+    const double *Xji = X + ((long)j * no + i) * v2;
+    __syncthreads();  // the previous tile's reads of sT are done
+    for (int y = ty; y < 32; y += TY) {  // sT[b - b0][a - a0] = X_jiba
+      const int b = b0 + y, a = a0 + tx;
+      sT[y][tx] = (b < nv && a < nv) ? Xji[(long)b * nv + a] : 0.0;
     }
+    __syncthreads();
+    const long base = ((long)i * no + j) * v2;
+    for (int y = ty; y < 32; y += TY) {
+      const int a = a0 + y, b = b0 + tx;
+      if (a < nv && b < nv) {
+        const long idx = base + (long)a * nv + b;
+        const double R = X[idx] + sT[tx][y];
+        const double D = (eps[i] + eps[j]) - (eps[no + a] + eps[no + b]) + shift;
+        s += R * R;
+        if (R2out) R2out[idx] = R;
+        t2[idx] = t2old[idx] + R / D;
+      }
+    }
+  }
+  const long n1 = (long)no * nv;
+  for (long q = blockIdx.x * (long)NT + threadIdx.x; q < n1; q += (long)gridDim.x * NT) {
+    const int i = (int)(q / nv), a = (int)(q % nv);
+    const double R = R1[q];
+    s += R * R;
+    t1[q] = t1old[q] + R / (eps[i] - eps[no + a] + shift);
   }
   double tot = block_sum(s);
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
@@ -677,6 +746,9 @@ void launch_permute(const PermDesc &p, hipStream_t st) {
   }
   a.in = p.in; a.out = p.out; a.alpha = p.alpha; a.beta = p.beta;
   a.pk = p.pk; a.pblk = p.pblk; a.plo = p.plo;
+  a.i32 = a.n < (1L << 31);
+  for (int d = 0; d < 6; d++) magic31(a.od[d], a.dm[d], a.ds[d]);
+  magic31(p.out_t == 3 ? p.pk : 1, a.pkm, a.pks);
   if (p.out_t >= 2 && p.beta != 0.0) throw std::runtime_error("ugx: BF16 pair output cannot accumulate");
   if (p.out_t == 3 && (p.pk <= 0 || a.n % p.pk != 0 || p.pblk != 16 * ((p.pk + 31) / 32 * 32)))
     throw std::runtime_error("ugx: bad MFMA panel output shape");
