@@ -653,22 +653,57 @@ __global__ void k_pairs_unpack(long nrows, int no, const double *__restrict__ ZP
 
 // X[i][j][a][b] += scale (L+[p(ij)][p(ab)] + s_ij s_ab L-[s(ij)][s(ab)]),
 // indices sorted into pairs, s = +1 for i > j (a > b), -1 below, and no
-// L- term on a diagonal
-__global__ void k_ladder_unpack(int no, int nv, const double *__restrict__ LP, const double *__restrict__ LM,
-                                double scale, double *__restrict__ X) {
+// L- term on a diagonal.  One 32 x 32 (a, b) tile of one (i, j) pair at a
+// time, with the packed rows read along their pair index: along b in a tile
+// on or below the diagonal (p(ab) = a(a+1)/2 + b for a >= b), along a in a
+// tile above it (p(ba) = b(b+1)/2 + a), staged in LDS and written to X
+// along b.
+__global__ __launch_bounds__(NT) void k_ladder_unpack(int no, int nv, const double *__restrict__ LP,
+                                                      const double *__restrict__ LM, double scale,
+                                                      double *__restrict__ X) {
+  __shared__ double sL[32][33];
   const long v = nv, P = (long)nv * (nv + 1) / 2, Ps = (long)nv * (nv - 1) / 2;
-  const long n = (long)no * no * v * v;
-  for (long idx = blockIdx.x * (long)NT + threadIdx.x; idx < n; idx += (long)gridDim.x * NT) {
-    const int b = (int)(idx % v), a = (int)((idx / v) % v);
-    const long ij = idx / (v * v);
-    const int j = (int)(ij % no), i = (int)(ij / no);
-    const int hi = i > j ? i : j, lo = i > j ? j : i, ha = a > b ? a : b, la = a > b ? b : a;
-    double x = LP[((long)hi * (hi + 1) / 2 + lo) * P + (long)ha * (ha + 1) / 2 + la];
-    if (i != j && a != b) {
-      const double m = LM[((long)hi * (hi - 1) / 2 + lo) * Ps + (long)ha * (ha - 1) / 2 + la];
-      x += ((i > j) == (a > b)) ? m : -m;
+  const int tb = (nv + 31) / 32;
+  const long ntiles = (long)no * no * tb * tb;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  constexpr int TY = NT / 32;
+  for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int tbb = (int)(tile % tb);
+    long r = tile / tb;
+    const int tba = (int)(r % tb);
+    r /= tb;
+    const int j = (int)(r % no), i = (int)(r / no);
+    const int a0 = tba * 32, b0 = tbb * 32;
+    const int hi = i > j ? i : j, lo = i > j ? j : i;
+    const double *lp = LP + ((long)hi * (hi + 1) / 2 + lo) * P;
+    const double *lm = i != j ? LM + ((long)hi * (hi - 1) / 2 + lo) * Ps : LM;  // read only for i != j
+    auto val = [&](int a, int b) {
+      const int ha = a > b ? a : b, la = a > b ? b : a;
+      double x = lp[(long)ha * (ha + 1) / 2 + la];
+      if (i != j && a != b) {
+        const double m = lm[(long)ha * (ha - 1) / 2 + la];
+        x += ((i > j) == (a > b)) ? m : -m;
+      }
+      return x;
+    };
+    double *Xij = X + ((long)i * no + j) * v * v;
+    if (tba >= tbb) {
+      for (int y = ty; y < 32; y += TY) {
+        const int a = a0 + y, b = b0 + tx;
+        if (a < nv && b < nv) Xij[(long)a * v + b] += scale * val(a, b);
+      }
+    } else {
+      __syncthreads();  // the previous tile's reads of sL are done
+      for (int y = ty; y < 32; y += TY) {  // sL[b - b0][a - a0]
+        const int b = b0 + y, a = a0 + tx;
+        sL[y][tx] = (a < nv && b < nv) ? val(a, b) : 0.0;
+      }
+      __syncthreads();
+      for (int y = ty; y < 32; y += TY) {
+        const int a = a0 + y, b = b0 + tx;
+        if (a < nv && b < nv) Xij[(long)a * v + b] += scale * sL[tx][y];
+      }
     }
-    X[idx] += scale * x;
   }
 }
 
@@ -711,8 +746,9 @@ void launch_pairs_unpack(long nrows, int no, const double *ZP, const double *ZM,
 
 void launch_ladder_unpack(int no, int nv, const double *LP, const double *LM, double scale, double *X,
                           hipStream_t st) {
-  const long n = (long)no * no * nv * nv;
-  hipLaunchKernelGGL(k_ladder_unpack, dim3(nblocks(n)), dim3(NT), 0, st, no, nv, LP, LM, scale, X);
+  const long tb = (nv + 31) / 32, ntiles = (long)no * no * tb * tb;
+  hipLaunchKernelGGL(k_ladder_unpack, dim3((unsigned)std::min<long>(std::max<long>(ntiles, 1), 1 << 16)), dim3(NT), 0,
+                     st, no, nv, LP, LM, scale, X);
 }
 
 void launch_pack_panel(const double *in, long sp, long sr, long sk, int np, long R, int K, unsigned short *out,
