@@ -15,8 +15,9 @@
 // (Hendrycks & Gimpel), written here as z / (1 + exp(-2u)).
 //
 // Inference (k_mlp_infer) is one fused kernel on BF16 MFMA
-// (v_mfma_f32_16x16x32_bf16, FP32 accumulate).  Each wave takes 16
-// amplitudes at a time and computes the transposed layers
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulate).  Each wave forms the
+// features of 64 amplitudes (one per lane), then takes them 16 at a time
+// and computes the transposed layers
 //   Z^T[f][n] = sum_k W[f][k] H^T[k][n]
 // so the weights are the A operand (from LDS) and the activations the B
 // operand (in registers).  The accumulator of lane l holds rows
@@ -26,7 +27,10 @@
 //   pi_s(q, j) = 32s + 4q + j          (j < 4)
 //              = 32s + 16 + 4q + j - 4  (j >= 4),
 // and the host packs W[f][pi_s(q, j)] into the A fragments in the same order.
-// Activations therefore never leave the registers between layers.
+// Activations therefore never leave the registers between layers.  The
+// first layer (K = 8) is one MFMA per 16 rows with the features and W1
+// split into BF16 hi + lo (three of the four k-groups: W1hi xhi, W1hi xlo,
+// W1lo xhi), accurate to about 2^-16 before its BF16 output rounding.
 //
 // Training runs in FP32 on the library's MFMA GEMM (forward Z = H W^T,
 // backward dH = dZ W and dW = dZ^T H) with the element-wise kernels below
@@ -50,12 +54,13 @@ __device__ inline float gelu_f(float z) {
   return z / (1.0f + __expf(-2.0f * u));
 }
 
-// Inference form: the hardware reciprocal (1 ulp) instead of an IEEE
-// division; its output is rounded to BF16 (or summed in FP32 by the output
-// layer), far coarser than the difference.
+// Inference form: exp(-2u) as 2^(z (c1 + c3 z^2)) with c1 = -2 sqrt(2/pi)
+// log2(e) and c3 = 0.044715 c1 folded, and the hardware reciprocal (1 ulp)
+// instead of an IEEE division; its output is rounded to BF16 (or summed in
+// FP32 by the output layer), far coarser than the difference.
 __device__ inline float gelu_infer_f(float z) {
-  const float u = 0.7978845608028654f * (z + 0.044715f * z * z * z);
-  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+  const float e = __builtin_amdgcn_exp2f(z * (-2.302208198144325f + -0.1029432395800235f * (z * z)));
+  return z * __builtin_amdgcn_rcpf(1.0f + e);
 }
 
 __device__ inline float gelu_grad_f(float z) {
@@ -114,22 +119,41 @@ __global__ void k_features(int no, int nv, const double *__restrict__ eps, const
   for (int k = 0; k < NF; k++) X[r * NF + k] = x[k];
 }
 
-// One wave per 16-amplitude tile, four waves per workgroup, grid-stride over
-// tiles (the trip count is uniform across a wave, so every lane reaches
-// every MFMA).
+// One wave per 64 amplitudes at a time (four 16-amplitude MFMA tiles),
+// eight waves per workgroup, grid-stride over the 64-amplitude groups (the
+// trip count is uniform across a wave, so every lane reaches every MFMA).
+// 512 threads: the 64 KB of weight fragments in LDS are shared by eight
+// waves (two workgroups per CU, four waves per SIMD).
+constexpr int NTI = 512;
+
 template <bool HAM>
-__global__ __launch_bounds__(NT) void k_mlp_infer(long N, const float *__restrict__ Xraw, int no, int nv,
+__global__ __launch_bounds__(NTI) __attribute__((amdgpu_waves_per_eu(4))) void k_mlp_infer(long N, const float *__restrict__ Xraw, int no, int nv,
                                                   const double *__restrict__ eps, const double *__restrict__ oovv,
                                                   MlpDevice P, float *__restrict__ out) {
-  __shared__ u32x4 sW2[8 * 4 * 64], sW3[8 * 4 * 64];
-  __shared__ float sW1[WID * NF], sb1[WID], sb2[WID], sb3[WID], sW4[WID];
+  __shared__ u32x4 sW1[8 * 64], sW2[8 * 4 * 64], sW3[8 * 4 * 64];
+  __shared__ float sb1[WID], sb2[WID], sb3[WID], sW4[WID];
   const u32x4 *gW2 = (const u32x4 *)P.w2p, *gW3 = (const u32x4 *)P.w3p;
-  for (int e = threadIdx.x; e < 8 * 4 * 64; e += NT) {
+  for (int e = threadIdx.x; e < 8 * 4 * 64; e += NTI) {
     sW2[e] = gW2[e];
     sW3[e] = gW3[e];
   }
-  for (int e = threadIdx.x; e < WID * NF; e += NT) sW1[e] = P.w1f[e];
-  for (int e = threadIdx.x; e < WID; e += NT) {
+  // layer 1 (K = 8) as one MFMA per 16 rows in split BF16: the fragment of
+  // lane (r, g) for rows 16F + r holds W1 hi in k-groups g = 0, 1, W1 lo in
+  // g = 2 and zeros in g = 3; the feature fragments hold x hi, x lo, x hi,
+  // zeros, so the products are W1hi xhi + W1hi xlo + W1lo xhi
+  static_assert(8 * 64 <= NTI && WID == 128 && NF == 8, "one layer-1 fragment per thread");
+  if (threadIdx.x < 8 * 64) {
+    const int l = threadIdx.x & 63, gq = l >> 4, fr = 16 * (threadIdx.x >> 6) + (l & 15);
+    unsigned int h[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const float w = P.w1f[fr * NF + k];
+      const unsigned int hi = bf16_rne(w), lo = bf16_rne(w - __uint_as_float(hi << 16));
+      h[k] = gq < 2 ? hi : (gq == 2 ? lo : 0u);
+    }
+    sW1[threadIdx.x] = u32x4{h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
+  }
+  for (int e = threadIdx.x; e < WID; e += NTI) {
     sb1[e] = P.b1f[e];
     sb2[e] = P.b2[e];
     sb3[e] = P.b3[e];
@@ -139,52 +163,51 @@ __global__ __launch_bounds__(NT) void k_mlp_infer(long N, const float *__restric
   const float b4 = P.b4;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n = lane & 15, q = lane >> 4;
-  const long ntile = (N + 15) / 16;
-  for (long t = (long)blockIdx.x * (NT / 64) + wave; t < ntile; t += (long)gridDim.x * (NT / 64)) {
-    const long g = t * 16 + n;
-    const bool valid = g < N;
-    float x[NF];
-    if (valid) {
-      if (HAM) amp_features<true>(no, nv, eps, oovv, g, x);
-      else {
+  // a wave takes 64 amplitudes at a time: each lane forms the features of
+  // one, and the four 16-amplitude tiles take theirs by lane shuffles
+  const long ngroup = (N + 63) / 64;
+  for (long t = (long)blockIdx.x * (NTI / 64) + wave; t < ngroup; t += (long)gridDim.x * (NTI / 64)) {
+    // features of amplitude t * 64 + lane, split into BF16 hi and lo pairs
+    unsigned int xh[NF / 2], xl[NF / 2];
+    {
+      float xo[NF];
+      const long g = t * 64 + lane;
+      if (g < N) {
+        if (HAM) amp_features<true>(no, nv, eps, oovv, g, xo);
+        else {
 #pragma unroll
-        for (int k = 0; k < NF; k++) x[k] = Xraw[g * NF + k];
+          for (int k = 0; k < NF; k++) xo[k] = Xraw[g * NF + k];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NF; k++) xo[k] = 0.0f;
       }
-    } else {
 #pragma unroll
-      for (int k = 0; k < NF; k++) x[k] = 0.0f;
+      for (int m = 0; m < NF / 2; m++) {
+        const unsigned int h0 = bf16_rne(xo[2 * m]), h1 = bf16_rne(xo[2 * m + 1]);
+        xh[m] = h0 | (h1 << 16);
+        xl[m] = bf16_rne(xo[2 * m] - __uint_as_float(h0 << 16)) |
+                (bf16_rne(xo[2 * m + 1] - __uint_as_float(h1 << 16)) << 16);
+      }
     }
-    // layer 1 (K = 8) on the VALU, straight into the B fragments
-    u32x4 bop[4];
+#pragma unroll 1
+    for (int u = 0; u < 4; u++) {
+      // the weight fragments are re-read from LDS every tile: held across
+      // tiles they would take 256 registers and leave one wave per SIMD
+      asm volatile("" ::: "memory");
+      const long g = t * 64 + 16 * u + n;
+      const bool valid = g < N;
+      // the tile's feature fragments: x hi for k-groups 0 and 2, x lo for 1
+      unsigned int bf[NF / 2];
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-      unsigned int hb[8];
-#pragma unroll
-      for (int jj = 0; jj < 8; jj++) {
-        const int f = 32 * s + (jj < 4 ? 4 * q + jj : 16 + 4 * q + jj - 4);
-        float z = sb1[f];
-#pragma unroll
-        for (int k = 0; k < NF; k++) z += sW1[f * NF + k] * x[k];
-        hb[jj] = bf16_rne(gelu_infer_f(z));
+      for (int m = 0; m < NF / 2; m++) {
+        const unsigned int hv = __shfl(xh[m], 16 * u + n, 64), lv = __shfl(xl[m], 16 * u + n, 64);
+        bf[m] = q == 1 ? lv : (q == 3 ? 0u : hv);
       }
-      bop[s] = u32x4{hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16), hb[6] | (hb[7] << 16)};
-    }
-    // layers 2 and 3 on MFMA
-    float part = 0.0f;
-#pragma unroll
-    for (int layer = 0; layer < 2; layer++) {
-      const u32x4 *sW = layer == 0 ? sW2 : sW3;
-      const float *sb = layer == 0 ? sb2 : sb3;
-      flt4 acc[8];
-#pragma unroll
-      for (int F = 0; F < 8; F++) {
-        acc[F] = flt4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-          acc[F] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, sW[(F * 4 + s) * 64 + lane]),
-                                                           __builtin_bit_cast(bf16x8, bop[s]), acc[F], 0, 0, 0);
-      }
-      if (layer == 0) {
+      const u32x4 bx = u32x4{bf[0], bf[1], bf[2], bf[3]};
+      // GELU of rows 32 s + (the lane's 4 + 4 rows) of an accumulator pair,
+      // packed as the next layer's B fragment s
+      auto pack = [&](const flt4 *acc, const float *sb, u32x4 *bo) {
 #pragma unroll
         for (int s = 0; s < 4; s++) {
           unsigned int hb[8];
@@ -193,24 +216,52 @@ __global__ __launch_bounds__(NT) void k_mlp_infer(long N, const float *__restric
             hb[r] = bf16_rne(gelu_infer_f(acc[2 * s][r] + sb[32 * s + 4 * q + r]));
             hb[4 + r] = bf16_rne(gelu_infer_f(acc[2 * s + 1][r] + sb[32 * s + 16 + 4 * q + r]));
           }
-          bop[s] = u32x4{hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16),
-                         hb[6] | (hb[7] << 16)};
+          bo[s] = u32x4{hb[0] | (hb[1] << 16), hb[2] | (hb[3] << 16), hb[4] | (hb[5] << 16), hb[6] | (hb[7] << 16)};
         }
-      } else {
-        // output layer (128 -> 1): this lane's 32 features, then the four
-        // lane groups of column n
+      };
+      u32x4 bop[4];
+      {
+        flt4 acc[8];
 #pragma unroll
         for (int F = 0; F < 8; F++)
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            const int f = 16 * F + 4 * q + r;
-            part += sW4[f] * gelu_infer_f(acc[F][r] + sb[f]);
-          }
+          acc[F] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, sW1[F * 64 + lane]),
+                                                           __builtin_bit_cast(bf16x8, bx), flt4{0.f, 0.f, 0.f, 0.f},
+                                                           0, 0, 0);
+        pack(acc, sb1, bop);
       }
+      // layers 2 and 3 on MFMA
+      float part = 0.0f;
+#pragma unroll
+      for (int layer = 0; layer < 2; layer++) {
+        const u32x4 *sW = layer == 0 ? sW2 : sW3;
+        const float *sb = layer == 0 ? sb2 : sb3;
+        flt4 acc[8];
+#pragma unroll
+        for (int F = 0; F < 8; F++) {
+          acc[F] = flt4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int s = 0; s < 4; s++)
+            acc[F] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, sW[(F * 4 + s) * 64 + lane]),
+                                                             __builtin_bit_cast(bf16x8, bop[s]), acc[F], 0, 0, 0);
+        }
+        if (layer == 0) {
+          pack(acc, sb, bop);
+        } else {
+          // output layer (128 -> 1): this lane's 32 features, then the four
+          // lane groups of column n
+#pragma unroll
+          for (int F = 0; F < 8; F++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              const int f = 16 * F + 4 * q + r;
+              part += sW4[f] * gelu_infer_f(acc[F][r] + sb[f]);
+            }
+        }
+      }
+      part += __shfl_xor(part, 16, 64);
+      part += __shfl_xor(part, 32, 64);
+      if (q == 0 && valid) out[g] = part + b4;
     }
-    part += __shfl_xor(part, 16, 64);
-    part += __shfl_xor(part, 32, 64);
-    if (q == 0 && valid) out[g] = part + b4;
   }
 }
 
@@ -353,14 +404,14 @@ void launch_features(int no, int nv, const double *eps, const double *oovv, long
 }
 
 static unsigned infer_grid(long N) {
-  const long waves = (N + 15) / 16;
-  const long b = (waves + 3) / 4;
+  const long waves = (N + 63) / 64;
+  const long b = (waves + NTI / 64 - 1) / (NTI / 64);
   return (unsigned)(b < 2048 ? (b > 0 ? b : 1) : 2048);
 }
 
 void launch_mlp_infer_x(long N, const float *X, const MlpDevice &P, float *out, hipStream_t st) {
   if (N <= 0) return;
-  hipLaunchKernelGGL(k_mlp_infer<false>, dim3(infer_grid(N)), dim3(NT), 0, st, N, X, 0, 0, (const double *)nullptr,
+  hipLaunchKernelGGL(k_mlp_infer<false>, dim3(infer_grid(N)), dim3(NTI), 0, st, N, X, 0, 0, (const double *)nullptr,
                      (const double *)nullptr, P, out);
 }
 
@@ -368,7 +419,7 @@ void launch_mlp_infer_ham(int no, int nv, const double *eps, const double *oovv,
                           hipStream_t st) {
   const long N = (long)no * no * nv * nv;
   if (N <= 0) return;
-  hipLaunchKernelGGL(k_mlp_infer<true>, dim3(infer_grid(N)), dim3(NT), 0, st, N, (const float *)nullptr, no, nv, eps,
+  hipLaunchKernelGGL(k_mlp_infer<true>, dim3(infer_grid(N)), dim3(NTI), 0, st, N, (const float *)nullptr, no, nv, eps,
                      oovv, P, out);
 }
 
