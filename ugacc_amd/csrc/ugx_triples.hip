@@ -41,17 +41,25 @@ namespace ugx {
 namespace {
 
 constexpr int NT = 256;
+// Threads per workgroup of the energy kernel: with an FP64 W its LDS
+// footprint admits one workgroup per CU, so eight waves (two per SIMD)
+// instead of four; with an FP32 W two workgroups of four waves fit, and
+// eight-wave ones would not fit their registers in the budget of four
+// waves per SIMD.
+template <typename WT>
+constexpr int energy_nt() { return sizeof(WT) == 4 ? 256 : 512; }
 
+template <int NW>
 __device__ inline double block_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  __shared__ double red[NT / 64];
+  __shared__ double red[NW];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) red[w] = v;
   __syncthreads();
   double s = 0.0;
   if (threadIdx.x == 0)
-    for (int k = 0; k < NT / 64; k++) s += red[k];
+    for (int k = 0; k < NW; k++) s += red[k];
   return s;
 }
 
@@ -86,12 +94,12 @@ template <> struct Vec2<double> { typedef double2 type; };
 template <> struct Vec2<float> { typedef float2 type; };
 
 template <typename WT, int E, typename OT, bool FOV>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) == 4 ? 2 : 1))) void k_triples_energy(int no, int nv, int ntrip, const int *__restrict__ trip,
+__global__ __launch_bounds__(energy_nt<WT>()) __attribute__((amdgpu_waves_per_eu(2))) void k_triples_energy(int no, int nv, int ntrip, const int *__restrict__ trip,
                                                        const TripProb *__restrict__ probs,
                                                        const WT *__restrict__ Wall, long wstride,
                                                        double *__restrict__ part, int npairs,
                                                        int *__restrict__ counter) {
-  constexpr int E3 = E * E * E, E2 = E * E;
+  constexpr int E3 = E * E * E, E2 = E * E, NTE = energy_nt<WT>();
   // padded cube layout idx(a,b,c) = a*SA + b*SB + c: the strides are odd, so
   // the stores along a or b (and the reads along c) do not collide in banks
   constexpr int SB = E + 1, SA = E * (E + 1) + 1;
@@ -104,8 +112,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
   constexpr int EPC = 16 / (int)sizeof(WT), NCH = E3 / EPC;
   static_assert(E3 % EPC == 0, "sub-cubes are whole 16-byte chunks");
   constexpr int NPC = 3 * 5 * E2, NPF = 3 * 4 * E2;
-  constexpr int PW = (NCH + NT - 1) / NT, PC = (NPC + NT - 1) / NT, PF = (NPF + NT - 1) / NT;
-  static_assert(4 * E <= NT, "t1 / eps staging takes one element per thread");
+  constexpr int PW = (NCH + NTE - 1) / NTE, PC = (NPC + NTE - 1) / NTE, PF = (NPF + NTE - 1) / NTE;
+  static_assert(4 * E <= NTE, "t1 / eps staging takes one element per thread");
   __shared__ WT S[6][E * SA];
   // disconnected-term operands of the three blocks X, Y in {A, B, C}:
   // O[s][X][Y][x][y] = <pq|xy> for the slices s = (ij), (ik), (jk) (OT),
@@ -214,7 +222,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     OE rf[PF];
 #pragma unroll
     for (int q = 0; q < PF; q++) {
-      const int e = tid + q * NT;
+      const int e = tid + q * NTE;
       rf[q] = OE{};
       if (e < NPF) {
         const int sl = e / (4 * E2), r = e % (4 * E2), pp = r / E2, xy = r % E2;
@@ -235,7 +243,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     }
 #pragma unroll
     for (int q = 0; q < PF; q++) {
-      const int e = tid + q * NT;
+      const int e = tid + q * NTE;
       if (e < NPF) {
         const int sl = e / (4 * E2), r = e % (4 * E2), pp = r / E2, xy = r % E2;
         O[sl][pp >> 1][pp & 1][xy] = rf[q];
@@ -273,7 +281,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     const WT *c3 = cube(Bb, Cb, Ab), *c4 = cube(Cb, Ab, Bb), *c5 = cube(Cb, Bb, Ab);
 #pragma unroll
     for (int q = 0; q < PW; q++) {
-      const int ch = tid + q * NT;
+      const int ch = tid + q * NTE;
       if (ch < NCH) {
         rw[0][q] = reinterpret_cast<const W16 *>(c0)[ch]; rw[1][q] = reinterpret_cast<const W16 *>(c1)[ch];
         rw[2][q] = reinterpret_cast<const W16 *>(c2)[ch]; rw[3][q] = reinterpret_cast<const W16 *>(c3)[ch];
@@ -282,7 +290,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     }
 #pragma unroll
     for (int q = 0; q < PC; q++) {
-      const int e = tid + q * NT;
+      const int e = tid + q * NTE;
       rc[q] = OE{};
       if (e < NPC) {
         const int sl = e / (5 * E2), r = e % (5 * E2), p = r / E2, xy = r % E2;
@@ -308,7 +316,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     // that would need them are skipped.
 #pragma unroll
     for (int q = 0; q < PW; q++) {
-      const int ch = tid + q * NT;
+      const int ch = tid + q * NTE;
       if (ch < NCH) {
         const WT *w0 = reinterpret_cast<const WT *>(&rw[0][q]), *w1 = reinterpret_cast<const WT *>(&rw[1][q]);
         const WT *w2 = reinterpret_cast<const WT *>(&rw[2][q]), *w3 = reinterpret_cast<const WT *>(&rw[3][q]);
@@ -328,7 +336,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     }
 #pragma unroll
     for (int q = 0; q < PC; q++) {
-      const int e = tid + q * NT;
+      const int e = tid + q * NTE;
       if (e < NPC) {
         const int sl = e / (5 * E2), r = e % (5 * E2), p = r / E2, xy = r % E2;
         int bx, by;
@@ -351,7 +359,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     __syncthreads();
     if (Cb < Bb) fetch(Cb + 1);  // in flight while this step computes
 #pragma unroll 1
-    for (int e = tid; e < E3; e += NT) {
+    for (int e = tid; e < E3; e += NTE) {
       int cc = e % E, bb = (e / E) % E, aa = e / (E * E);
       int a = A0 + aa, b = B0 + bb, c = C0 + cc;
       if (a >= nv || b > a || c > b) continue;
@@ -371,7 +379,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(sizeof(WT) =
     }
     __syncthreads();
   }
-  double tot = block_sum(s);
+  double tot = block_sum<NTE / 64>(s);
   if (tid == 0) part[(long)t * npairs + pidx] = tot;
   }
 }
@@ -397,11 +405,12 @@ void launch_e(int no, int nv, int ntrip, const int *trip4, const TripProb *probs
   int dev = 0, ncu = 0, per = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_triples_energy<WT, E, OT, FOV>, NT, 0);
+  constexpr int NTE = energy_nt<WT>();
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_triples_energy<WT, E, OT, FOV>, NTE, 0);
   long grid = (long)std::max(ncu, 1) * std::max(per, 1);
   grid = std::min<long>(grid, (long)ntrip * npairs);
   hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, counter);
-  hipLaunchKernelGGL((k_triples_energy<WT, E, OT, FOV>), dim3((unsigned)grid), dim3(NT), 0, st, no, nv, ntrip, trip4,
+  hipLaunchKernelGGL((k_triples_energy<WT, E, OT, FOV>), dim3((unsigned)grid), dim3(NTE), 0, st, no, nv, ntrip, trip4,
                      probs, (const WT *)W, wstride, part, npairs, counter);
   hipLaunchKernelGGL(k_triples_reduce, dim3((ntrip + NT - 1) / NT), dim3(NT), 0, st, npairs, ntrip, part, etrip);
 }
