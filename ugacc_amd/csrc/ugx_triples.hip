@@ -89,6 +89,28 @@ __device__ inline void pair_of(int p, int &A, int &B) {
 }
 
 // (<pq|xy>, t2_pq^xy) and (t1, f_ov) pairs of the FOV variant
+// Local points (x, y, z) of an E^3 step on a block diagonal, in order:
+// pair[p] = x << 4 | y for y <= x, tri[q] = x << 8 | y << 4 | z for
+// z <= y <= x (E <= 15)
+template <int E>
+struct DiagPoints {
+  unsigned short pair[E * (E + 1) / 2];
+  unsigned short tri[E * (E + 1) * (E + 2) / 6];
+};
+template <int E>
+constexpr DiagPoints<E> make_diag_points() {
+  static_assert(E <= 15, "four bits per local index");
+  DiagPoints<E> t{};
+  int p = 0, q = 0;
+  for (int x = 0; x < E; x++)
+    for (int y = 0; y <= x; y++) {
+      t.pair[p++] = (unsigned short)(x << 4 | y);
+      for (int z = 0; z <= y; z++) t.tri[q++] = (unsigned short)(x << 8 | y << 4 | z);
+    }
+  return t;
+}
+__constant__ DiagPoints<12> kDiag12 = make_diag_points<12>();
+
 template <typename T> struct Vec2;
 template <> struct Vec2<double> { typedef double2 type; };
 template <> struct Vec2<float> { typedef float2 type; };
@@ -358,9 +380,29 @@ __global__ __launch_bounds__(energy_nt<WT>()) __attribute__((amdgpu_waves_per_eu
     commit();
     __syncthreads();
     if (Cb < Bb) fetch(Cb + 1);  // in flight while this step computes
+    // the step's points: the whole E^3 cube when A > B > C; on a block
+    // diagonal only the local points in the order it demands (cc <= bb when
+    // B = C, bb <= aa when A = B), enumerated compactly so that no lane idles
+    // on a point outside a >= b >= c
+    static_assert(E == 12, "diagonal point tables for E = 12");
+    constexpr int P2 = E * (E + 1) / 2;
+    const int type = (Cb == Bb ? 1 : 0) | (Bb == Ab ? 2 : 0);
+    const int npts = type == 0 ? E3 : (type == 3 ? E * (E + 1) * (E + 2) / 6 : E * P2);
 #pragma unroll 1
-    for (int e = tid; e < E3; e += NTE) {
-      int cc = e % E, bb = (e / E) % E, aa = e / (E * E);
+    for (int e = tid; e < npts; e += NTE) {
+      int aa, bb, cc;
+      if (type == 0) {
+        cc = e % E; bb = (e / E) % E; aa = e / (E * E);
+      } else if (type == 1) {  // B = C: any aa, cc <= bb
+        const int pr = kDiag12.pair[e % P2];
+        aa = e / P2; bb = pr >> 4; cc = pr & 15;
+      } else if (type == 2) {  // A = B: bb <= aa, any cc
+        const int pr = kDiag12.pair[e / E];
+        aa = pr >> 4; bb = pr & 15; cc = e % E;
+      } else {  // A = B = C: cc <= bb <= aa
+        const int t3 = kDiag12.tri[e];
+        aa = t3 >> 8; bb = (t3 >> 4) & 15; cc = t3 & 15;
+      }
       int a = A0 + aa, b = B0 + bb, c = C0 + cc;
       if (a >= nv || b > a || c > b) continue;
       // 1 / (1 + d_ab + d_ac + d_bc): 1, 1/2 or 1/4, exact
