@@ -9,6 +9,7 @@
 #include "ugx_internal.h"
 
 #include <algorithm>
+#include <cstdint>
 #include <stdexcept>
 
 namespace ugx {
@@ -54,6 +55,8 @@ struct PermArgs {
   int i32;
   unsigned dm[6], pkm;
   int ds[6], pks;
+  // eight consecutive outputs per thread (k_perm_linear8)
+  int vec8;
 };
 
 // floor(x / d) for 0 <= x < 2^31 without a division (Granlund and
@@ -82,15 +85,33 @@ struct OutD {
   typedef double tile_t;
   __device__ static double get(const PermArgs &p, long i) { return ((const double *)p.out)[i]; }
   __device__ static void put(const PermArgs &p, long i, double v) { ((double *)p.out)[i] = v; }
-};
-struct OutF {
-  typedef float tile_t;
-  __device__ static double get(const PermArgs &p, long i) { return ((const float *)p.out)[i]; }
-  __device__ static void put(const PermArgs &p, long i, double v) { ((float *)p.out)[i] = (float)v; }
+  __device__ static void put8(const PermArgs &p, long i, const double *v) {
+    for (int u = 0; u < 8; u++) put(p, i + u, v[u]);
+  }
 };
 __device__ inline unsigned short bf16_rne(float x) {
   unsigned int u = __float_as_uint(x);
   return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+struct OutF {
+  typedef float tile_t;
+  __device__ static double get(const PermArgs &p, long i) { return ((const float *)p.out)[i]; }
+  __device__ static void put(const PermArgs &p, long i, double v) { ((float *)p.out)[i] = (float)v; }
+  __device__ static void put8(const PermArgs &p, long i, const double *v) {
+    for (int u = 0; u < 8; u++) put(p, i + u, v[u]);
+  }
+};
+// eight values as BF16 hi and lo halves, packed two per word
+__device__ inline void split8(const double *v, uint4 &hi, uint4 &lo) {
+  unsigned int h[8], l[8];
+#pragma unroll
+  for (int u = 0; u < 8; u++) {
+    const float x = (float)v[u];
+    h[u] = bf16_rne(x);
+    l[u] = bf16_rne(x - __uint_as_float(h[u] << 16));
+  }
+  hi = uint4{h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
+  lo = uint4{l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16), l[6] | (l[7] << 16)};
 }
 struct OutB3 {
   typedef float tile_t;
@@ -101,6 +122,13 @@ struct OutB3 {
     float hf = __uint_as_float(((unsigned int)h) << 16);
     ((unsigned short *)p.out)[i] = h;
     ((unsigned short *)p.out)[p.n + i] = bf16_rne(x - hf);
+  }
+  // elements i .. i + 7, i a multiple of 8 (16-byte aligned halves)
+  __device__ static void put8(const PermArgs &p, long i, const double *v) {
+    uint4 hi, lo;
+    split8(v, hi, lo);
+    *reinterpret_cast<uint4 *>((unsigned short *)p.out + i) = hi;
+    *reinterpret_cast<uint4 *>((unsigned short *)p.out + p.n + i) = lo;
   }
 };
 
@@ -127,6 +155,18 @@ struct OutP {
     float hf = __uint_as_float(((unsigned int)h) << 16);
     ((unsigned short *)p.out)[a] = h;
     ((unsigned short *)p.out)[p.plo + a] = bf16_rne(x - hf);
+  }
+  // elements i .. i + 7 of one row, i a multiple of 8 and pk too: one stored
+  // 8-k chunk (n < 2^31)
+  __device__ static void put8(const PermArgs &p, long i, const double *v) {
+    const unsigned q = div31((unsigned)i, p.pkm, p.pks);
+    const long r = q, k = (unsigned)i - q * (unsigned)p.pk;
+    const int rr = (int)(r & 15), kk = (int)(k & 31);
+    const long a = (r >> 4) * p.pblk + (k >> 5) * 512 + rr * 32 + 8 * panel_chunk(rr, kk >> 3);
+    uint4 hi, lo;
+    split8(v, hi, lo);
+    *reinterpret_cast<uint4 *>((unsigned short *)p.out + a) = hi;
+    *reinterpret_cast<uint4 *>((unsigned short *)p.out + p.plo + a) = lo;
   }
 };
 
@@ -157,6 +197,29 @@ __global__ void k_perm_linear(PermArgs p, long n) {
     double v = p.alpha * (double)((const TI *)p.in)[off];
     if (p.beta != 0.0) v += p.beta * OW::get(p, idx);
     OW::put(p, idx, v);
+  }
+}
+
+// Eight consecutive outputs per thread (vec8: BF16 pair outputs, n < 2^31,
+// no accumulation, the innermost output dim of unit input stride and a
+// multiple of 8 extent, so outputs idx .. idx + 7 read inputs off .. off + 7):
+// one index split, eight loads, two 16-byte stores.
+template <typename TI, class OW>
+__global__ void k_perm_linear8(PermArgs p, long n8) {
+  for (long t = blockIdx.x * (long)NT + threadIdx.x; t < n8; t += (long)gridDim.x * NT) {
+    const unsigned idx = (unsigned)(t * 8);
+    long off = 0;
+    unsigned r = idx;
+#pragma unroll
+    for (int d = 5; d >= 0; d--) {
+      const unsigned q = div31(r, p.dm[d], p.ds[d]);
+      off += (long)(r - q * (unsigned)p.od[d]) * p.istr[d];
+      r = q;
+    }
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = p.alpha * (double)((const TI *)p.in)[off + u];
+    OW::put8(p, idx, v);
   }
 }
 
@@ -220,7 +283,9 @@ void perm_launch(const PermArgs &a, hipStream_t st) {
   if (a.istr[L] != 1)
     for (int d = 0; d < L; d++)
       if (a.istr[d] == 1 && a.od[d] > 1) q = d;
-  if (q < 0 || n < 4096) {
+  if (a.vec8) {
+    hipLaunchKernelGGL((k_perm_linear8<TI, OW>), dim3(nblocks(n / 8)), dim3(NT), 0, st, a, n / 8);
+  } else if (q < 0 || n < 4096) {
     hipLaunchKernelGGL((k_perm_linear<TI, OW>), dim3(nblocks(n)), dim3(NT), 0, st, a, n);
   } else {
     int tq = (a.od[q] + 63) / 64, tl = (a.od[L] + 63) / 64;
@@ -785,6 +850,9 @@ void launch_permute(const PermDesc &p, hipStream_t st) {
   a.i32 = a.n < (1L << 31);
   for (int d = 0; d < 6; d++) magic31(a.od[d], a.dm[d], a.ds[d]);
   magic31(p.out_t == 3 ? p.pk : 1, a.pkm, a.pks);
+  const int L = p.rank - 1;
+  a.vec8 = (p.out_t == 2 || p.out_t == 3) && a.i32 && p.beta == 0.0 && a.od[L] % 8 == 0 && a.istr[L] == 1 &&
+           a.n % 8 == 0 && ((uintptr_t)p.out & 15) == 0 && (p.out_t != 3 || (p.pk % 8 == 0 && p.plo % 8 == 0));
   if (p.out_t >= 2 && p.beta != 0.0) throw std::runtime_error("ugx: BF16 pair output cannot accumulate");
   if (p.out_t == 3 && (p.pk <= 0 || a.n % p.pk != 0 || p.pblk != 16 * ((p.pk + 31) / 32 * 32)))
     throw std::runtime_error("ugx: bad MFMA panel output shape");
