@@ -55,8 +55,9 @@ struct PermArgs {
   int i32;
   unsigned dm[6], pkm;
   int ds[6], pks;
-  // eight consecutive outputs per thread (k_perm_linear8)
-  int vec8;
+  // eight consecutive outputs per thread: k_perm_linear8 (vec8), the write
+  // side of k_perm_tiled (vec8t)
+  int vec8, vec8t;
 };
 
 // floor(x / d) for 0 <= x < 2^31 without a division (Granlund and
@@ -260,14 +261,29 @@ __global__ void k_perm_tiled(PermArgs p, int q, long nrest, int tq, int tl) {
       tile[lq][ll] = v;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < 64 * 64; e += NT) {
-      int ll = e & 63, lq = e >> 6;
-      int gq = q0 + lq, gl = l0 + ll;
-      if (gq < Dq && gl < Dl) {
-        long o = obase + gq * ostr[q] + gl;
-        double v = (double)tile[lq][ll];
-        if (p.beta != 0.0) v += p.beta * OW::get(p, o);
-        OW::put(p, o, v);
+    if (p.vec8t) {
+      // eight consecutive l per thread (Dl and every output stride a
+      // multiple of 8: one stored chunk each)
+      for (int e = threadIdx.x; e < 64 * 8; e += NT) {
+        const int ll = (e & 7) * 8, lq = e >> 3;
+        const int gq = q0 + lq, gl = l0 + ll;
+        if (gq < Dq && gl < Dl) {
+          double v[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) v[u] = (double)tile[lq][ll + u];
+          OW::put8(p, obase + gq * ostr[q] + gl, v);
+        }
+      }
+    } else {
+      for (int e = threadIdx.x; e < 64 * 64; e += NT) {
+        int ll = e & 63, lq = e >> 6;
+        int gq = q0 + lq, gl = l0 + ll;
+        if (gq < Dq && gl < Dl) {
+          long o = obase + gq * ostr[q] + gl;
+          double v = (double)tile[lq][ll];
+          if (p.beta != 0.0) v += p.beta * OW::get(p, o);
+          OW::put(p, o, v);
+        }
       }
     }
     __syncthreads();
@@ -851,8 +867,9 @@ void launch_permute(const PermDesc &p, hipStream_t st) {
   for (int d = 0; d < 6; d++) magic31(a.od[d], a.dm[d], a.ds[d]);
   magic31(p.out_t == 3 ? p.pk : 1, a.pkm, a.pks);
   const int L = p.rank - 1;
-  a.vec8 = (p.out_t == 2 || p.out_t == 3) && a.i32 && p.beta == 0.0 && a.od[L] % 8 == 0 && a.istr[L] == 1 &&
-           a.n % 8 == 0 && ((uintptr_t)p.out & 15) == 0 && (p.out_t != 3 || (p.pk % 8 == 0 && p.plo % 8 == 0));
+  a.vec8t = (p.out_t == 2 || p.out_t == 3) && a.i32 && p.beta == 0.0 && a.od[L] % 8 == 0 && a.n % 8 == 0 &&
+            ((uintptr_t)p.out & 15) == 0 && (p.out_t != 3 || (p.pk % 8 == 0 && p.plo % 8 == 0));
+  a.vec8 = a.vec8t && a.istr[L] == 1;
   if (p.out_t >= 2 && p.beta != 0.0) throw std::runtime_error("ugx: BF16 pair output cannot accumulate");
   if (p.out_t == 3 && (p.pk <= 0 || a.n % p.pk != 0 || p.pblk != 16 * ((p.pk + 31) / 32 * 32)))
     throw std::runtime_error("ugx: bad MFMA panel output shape");
