@@ -10,8 +10,10 @@
   first t2 rows (and all of R1) at the GPU's converged amplitudes, and the
   GPU per-triplet (T) energies with the oracle's on a spread of triplets.
 """
+import ctypes
 import json
 import os
+import re
 
 import numpy as np
 import pytest
@@ -50,6 +52,42 @@ def test_h2o_single_water_configuration():
     t1, t2 = c.amplitudes()
     assert np.max(np.abs(t1 - t1o)) < 1e-9 and np.max(np.abs(t2 - t2o)) < 1e-9
     assert abs(r["e_t"] - o.tcorr_tjl()) < 1e-10
+
+
+def test_printed_iteration_table_matches_oracle(capfd):
+    """The iteration table ugx_ccsd prints with opt.print (ccwfn.cc:194-196,
+    224): row 0 holds E_MP2, row k the energy, the T1 norm (t1norm,
+    ccwfn.cc:541-553) and the residual norm after iteration k's DIIS, and
+    the converged iteration prints no row.  Every row against the oracle
+    stopped after exactly k iterations (printed precision: 15 decimals,
+    6 decimals, 7 significant digits)."""
+    announce("h2o printed iteration table vs oracle")
+    ref = Reference(molecules.to_bohr(molecules.water()), "cc-pvdz")
+    capfd.readouterr()
+    c, r = _solve(ref, print=1, wfn_ccsd_t=0)
+    ctypes.CDLL(None).fflush(None)
+    out = capfd.readouterr().out
+    rows = [l.split() for l in out.splitlines() if re.match(r"^\t +\d+ +-?\d", l)]
+    assert len(rows) == r["niter"], (len(rows), r["niter"], out[-2000:])
+    assert rows[0][0] == "0" and len(rows[0]) == 2
+    o = Oracle(ref.fock, ref.full_ints(), ref.no)
+    ro = o.ccsd(maxiter=100, conv=1e-10, mode=0)
+    assert abs(float(rows[0][1]) - ro["emp2"]) < 1e-12
+    assert ro["niter"] == r["niter"]
+    norms = []
+    for k in range(1, r["niter"]):
+        it, e, t1n, rms = rows[k]
+        assert int(it) == k
+        assert abs(float(e) - ro["trace"][k]) < 1e-11, (k, e, ro["trace"][k])
+        ok = Oracle(ref.fock, ref.full_ints(), ref.no)
+        rk = ok.ccsd(maxiter=k, conv=1e-10, mode=0)
+        assert abs(float(t1n) - ok.t1norm()) <= 6e-7, (k, t1n, ok.t1norm())
+        assert abs(float(rms) - rk["rms"]) <= 1e-6 * rk["rms"], (k, rms, rk["rms"])
+        norms.append(ok.t1norm())
+        ok.close()
+    # the column is not constant: a dropped or misplaced term would show
+    assert max(norms) - min(norms) > 1e-4
+    o.close()
 
 
 def test_psi3_h2o_known_answers_on_gpu():
