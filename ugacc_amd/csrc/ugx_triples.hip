@@ -387,7 +387,13 @@ __global__ __launch_bounds__(energy_nt<WT>()) __attribute__((amdgpu_waves_per_eu
     static_assert(E == 12, "diagonal point tables for E = 12");
     constexpr int P2 = E * (E + 1) / 2;
     const int type = (Cb == Bb ? 1 : 0) | (Bb == Ab ? 2 : 0);
-    const int npts = type == 0 ? E3 : (type == 3 ? E * (E + 1) * (E + 2) / 6 : E * P2);
+    // only block A can end past nv (A >= B >= C), and every enumeration
+    // runs aa slowest: stopping at the first aa past nv skips those points
+    const int nA = min(E, nv - A0);
+    const int npts = type == 0   ? nA * E * E
+                     : type == 1 ? nA * P2
+                     : type == 2 ? nA * (nA + 1) / 2 * E
+                                 : nA * (nA + 1) * (nA + 2) / 6;
 #pragma unroll 1
     for (int e = tid; e < npts; e += NTE) {
       int aa, bb, cc;
