@@ -148,8 +148,14 @@ __global__ __launch_bounds__(energy_nt<WT>()) __attribute__((amdgpu_waves_per_eu
   __shared__ TE T1[3][3][E];
   __shared__ double EPS[3][E];
   __shared__ int item_sh;
+  // the diagonal point tables in LDS (per-lane reads of the constant copy
+  // would be vector memory loads)
+  __shared__ unsigned short sPair[E * (E + 1) / 2], sTri[E * (E + 1) * (E + 2) / 6];
   const int tid = threadIdx.x;
   const long total = (long)ntrip * npairs;
+  static_assert(E == 12, "diagonal point tables for E = 12");
+  for (int e = tid; e < E * (E + 1) / 2; e += NTE) sPair[e] = kDiag12.pair[e];
+  for (int e = tid; e < E * (E + 1) * (E + 2) / 6; e += NTE) sTri[e] = kDiag12.tri[e];
   // persistent: each workgroup takes (entry, block pair) items from the
   // queue until it is empty; every workgroup reaches the exit test
   for (;;) {
@@ -400,13 +406,13 @@ __global__ __launch_bounds__(energy_nt<WT>()) __attribute__((amdgpu_waves_per_eu
       if (type == 0) {
         cc = e % E; bb = (e / E) % E; aa = e / (E * E);
       } else if (type == 1) {  // B = C: any aa, cc <= bb
-        const int pr = kDiag12.pair[e % P2];
+        const int pr = sPair[e % P2];
         aa = e / P2; bb = pr >> 4; cc = pr & 15;
       } else if (type == 2) {  // A = B: bb <= aa, any cc
-        const int pr = kDiag12.pair[e / E];
+        const int pr = sPair[e / E];
         aa = pr >> 4; bb = pr & 15; cc = e % E;
       } else {  // A = B = C: cc <= bb <= aa
-        const int t3 = kDiag12.tri[e];
+        const int t3 = sTri[e];
         aa = t3 >> 8; bb = (t3 >> 4) & 15; cc = t3 & 15;
       }
       int a = A0 + aa, b = B0 + bb, c = C0 + cc;
