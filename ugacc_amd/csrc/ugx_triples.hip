@@ -80,6 +80,17 @@ __device__ inline T gload(const T *p) {
   else return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) u64x2 *)p);
 }
 
+// 1 / d from the hardware reciprocal estimate refined by two Newton-Raphson
+// steps r <- r + r (1 - d r) (each squares the relative error): within an
+// ulp or two of the quotient, without the division's scaling sequence.
+__device__ inline double recip(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
 // Block pair index p -> (A, B) with B <= A.
 __device__ inline void pair_of(int p, int &A, int &B) {
   A = (int)((sqrt(8.0 * p + 1.0) - 1.0) * 0.5);
@@ -429,7 +440,7 @@ __global__ __launch_bounds__(energy_nt<WT>()) __attribute__((amdgpu_waves_per_eu
       double Y = vabc + vbca + vcab;
       double Z = vacb + vbac + vcba;
       double denom = eo - (EPS[0][aa] + EPS[1][bb] + EPS[2][cc]);
-      s += ((Y - 2.0 * Z) * (wabc + wbca + wcab) + (Z - 2.0 * Y) * (wacb + wbac + wcba) + 3.0 * X) * fac / denom;
+      s += ((Y - 2.0 * Z) * (wabc + wbca + wcab) + (Z - 2.0 * Y) * (wacb + wbac + wcba) + 3.0 * X) * fac * recip(denom);
     }
     __syncthreads();
   }
