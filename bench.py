@@ -283,7 +283,8 @@ def library_shapes(wl, reps=10):
     (T) W3 launch shapes with their 160-wide tile padding, batch 64 of
     v^2 x 160 x 2(v + o) and batch 4096 of 160 x 160 x 4(v + o); FP64 matmul
     (rocBLAS) on the ring (ov x ov x ov) and the packed ladder
-    (o(o+1)/2 x v(v+1)/2 x v(v+1)/2)."""
+    (o(o+1)/2 x v(v+1)/2 x v(v+1)/2); BF16 matmul (hipBLASLt) on the ring
+    shape, whose BF16x3 form the CCSD's low-precision iterations run."""
     import torch
     o, v = wl["no"], wl["nv"]
 
@@ -309,6 +310,7 @@ def library_shapes(wl, reps=10):
         "bf16_bmm_w3_merged": run((4096, 160, 4 * (v + o)), (4096, 4 * (v + o), 160), torch.bfloat16, True),
         "fp64_ring": run((ov, ov), (ov, ov), torch.float64, False),
         "fp64_ladder": run((po, pv), (pv, pv), torch.float64, False),
+        "bf16_ring": run((ov, ov), (ov, ov), torch.bfloat16, False),
         "note": "one BF16 product per call (the library kernel's BF16x3 W3 runs three); torch.bmm is hipBLASLt, "
                 "the FP64 matmul rocBLAS",
     }
